@@ -1,0 +1,816 @@
+// Host side of the C ABI (include/sieve_gpu.h): index creation and the batched search
+// pipeline. The index is immutable after creation; each sieve_stream owns a HIP stream and
+// grow-only scratch buffers, so host threads share an index and each drive their own stream
+// (the reference's shared `Index` + per-thread `Scratch`, src/ivf.rs:498-519).
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace sieve {
+
+static thread_local std::string g_error;
+void set_error(const std::string& msg) { g_error = msg; }
+
+static bool on_device(const void* p) {
+    if (!p) return false;
+    hipPointerAttribute_t at;
+    hipError_t e = hipPointerGetAttributes(&at, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+template <class T>
+static T* grow(DevBuf& b, size_t n) {
+    size_t bytes = std::max<size_t>(n * sizeof(T), 16);
+    if (bytes > b.cap) {
+        b.release();
+        hipError_t e = hipMalloc(&b.p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            b.p = nullptr;
+            throw Error{SIEVE_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+        }
+        b.cap = bytes;
+    }
+    return static_cast<T*>(b.p);
+}
+
+template <class T>
+static T* dmalloc(size_t n, size_t* acct) {
+    void* p = nullptr;
+    size_t bytes = std::max<size_t>(n * sizeof(T), 16);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error{SIEVE_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+    }
+    if (acct) *acct += bytes;
+    return static_cast<T*>(p);
+}
+
+}  // namespace sieve
+
+using namespace sieve;
+
+struct sieve_index {
+    int device = 0;
+    uint32_t k = 0;
+    uint64_t n = 0, nblocks = 0, nslots = 0;
+    uint32_t max_cluster_blocks = 0;
+    float* cents = nullptr;
+    uint16_t* cents_bf = nullptr;
+    float cent_norm_max = 0.0f;
+    uint32_t* block_off = nullptr;
+    uint32_t* cl_slots = nullptr;
+    uint4* codes = nullptr;
+    float* mul = nullptr;
+    float* err = nullptr;
+    uint32_t* ids = nullptr;
+    float* rotation = nullptr;
+    int8_t* rerank = nullptr;
+    bool rerank_owned = true;
+    float* quant = nullptr;
+    float* norm256 = nullptr;
+    float* t2_sigma = nullptr;
+    uint8_t* t2_codes = nullptr;
+    float* t2_norm = nullptr;
+    size_t device_bytes = 0;
+
+    ~sieve_index() {
+        void* ps[] = {cents, cents_bf, block_off, cl_slots, codes, mul, err, ids, rotation, quant, norm256, t2_sigma,
+                      t2_codes, t2_norm};
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+        if (rerank && rerank_owned) (void)hipFree(rerank);
+    }
+};
+
+enum Slot {
+    B_Q256, B_Q768, B_TABLES, B_QAB, B_WQ768, B_WQ256, B_INV, B_QBF, B_QNORM, B_APPROX, B_PS, B_PC, B_PN, B_CE, B_CR,
+    B_CX, B_CN, B_OS, B_OR, B_ON, B_DEL, B_ALLOW, B_AFLAGS, B_CNT, B_OVF, B_TMP1, B_TMP2, B_TMP3, B_NSLOTS
+};
+
+struct sieve_stream {
+    hipStream_t st = nullptr;
+    int device = 0;
+    DevBuf b[B_NSLOTS];
+    hipEvent_t ev[8] = {};
+    ~sieve_stream() {
+        for (auto& x : b) x.release();
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+namespace {
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        set_error("");
+        return SIEVE_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return SIEVE_ENOMEM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return SIEVE_EINVAL;
+    } catch (...) {
+        set_error("unknown error");
+        return SIEVE_EINVAL;
+    }
+}
+
+// A device view of a caller array: the array itself if it is device memory, else a copy in
+// stream scratch.
+template <class T>
+const T* dev_in(sieve_stream* s, int slot, const T* p, size_t n) {
+    if (!p || n == 0) return p;
+    if (on_device(p)) return p;
+    T* d = grow<T>(s->b[slot], n);
+    SIEVE_HIP(hipMemcpyAsync(d, p, n * sizeof(T), hipMemcpyHostToDevice, s->st));
+    return d;
+}
+
+// Where a kernel writes an output: the caller's array if it is device memory, else scratch
+// that is copied back by out_done().
+template <class T>
+struct DevOut {
+    T* user;
+    T* dev;
+    size_t n;
+    bool host;
+};
+
+template <class T>
+DevOut<T> dev_out(sieve_stream* s, int slot, T* p, size_t n) {
+    if (on_device(p)) return {p, p, n, false};
+    return {p, grow<T>(s->b[slot], n), n, true};
+}
+
+template <class T>
+void out_done(sieve_stream* s, const DevOut<T>& o) {
+    if (o.host && o.n) SIEVE_HIP(hipMemcpyAsync(o.user, o.dev, o.n * sizeof(T), hipMemcpyDeviceToHost, s->st));
+}
+
+void check_params(const sieve_params* p, bool need_probe) {
+    SIEVE_REQUIRE(p, SIEVE_EINVAL, "params is null");
+    SIEVE_REQUIRE(p->cand >= 1 && p->cand <= (uint32_t)MAX_CAND, SIEVE_ELIMIT,
+                  "cand must be in [1, " + std::to_string(MAX_CAND) + "]");
+    SIEVE_REQUIRE(p->k >= 1 && p->k <= (uint32_t)MAX_CAND, SIEVE_ELIMIT,
+                  "k must be in [1, " + std::to_string(MAX_CAND) + "]");
+    SIEVE_REQUIRE(p->rerank <= SIEVE_RERANK_SSD, SIEVE_EINVAL, "unknown rerank tag " + std::to_string(p->rerank));
+    if (need_probe)
+        SIEVE_REQUIRE(p->nprobe >= 1 && p->nprobe <= (uint32_t)MAX_PROBE, SIEVE_ELIMIT,
+                      "nprobe must be in [1, " + std::to_string(MAX_PROBE) + "]");
+    SIEVE_REQUIRE(p->prune_m <= (uint32_t)MAX_CAND, SIEVE_ELIMIT, "prune rank beyond cand limit");
+}
+
+ViewArgs make_view(sieve_stream* s, const sieve_view* v, size_t nq) {
+    ViewArgs a{};
+    if (!v) return a;
+    a.deleted = dev_in(s, B_DEL, v->deleted, v->deleted ? v->deleted_words : 0);
+    a.deleted_words = v->deleted ? v->deleted_words : 0;
+    if (v->allow) {
+        size_t words = v->allow_stride ? v->allow_stride * nq : v->allow_words;
+        a.allow = dev_in(s, B_ALLOW, v->allow, words);
+        a.allow_words = v->allow_words;
+        a.allow_stride = v->allow_stride;
+        SIEVE_REQUIRE(!v->allow_stride || v->allow_stride >= v->allow_words, SIEVE_EINVAL,
+                      "allow_stride smaller than allow_words");
+        a.allow_flags = v->allow_flags ? dev_in(s, B_AFLAGS, v->allow_flags, nq) : nullptr;
+    }
+    return a;
+}
+
+float elapsed_ns(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms * 1e6f;
+}
+
+// Per-batch device state of the pipeline.
+struct Batch {
+    QueryPrep prep;
+    const float* q256 = nullptr;
+    const float* q768 = nullptr;
+};
+
+void prep_queries(const sieve_index* ix, sieve_stream* s, const float* q256d, const float* q768d, size_t nq,
+                  Batch& bt) {
+    bt.q256 = q256d;
+    bt.q768 = q768d;
+    bt.prep.tables = grow<uint8_t>(s->b[B_TABLES], nq * 1024);
+    bt.prep.qab = grow<float4>(s->b[B_QAB], nq);
+    bt.prep.wq768 = grow<int8_t>(s->b[B_WQ768], nq * DIM);
+    bt.prep.wq256 = grow<int8_t>(s->b[B_WQ256], nq * PDIM);
+    bt.prep.inv = grow<float2>(s->b[B_INV], nq);
+    bt.prep.qbf = grow<uint16_t>(s->b[B_QBF], nq * PDIM);
+    bt.prep.qnorm = grow<float>(s->b[B_QNORM], nq);
+    // 256^-1.5 = 2^-12 exactly.
+    launch_query_prep(q256d, q768d, nq, ix->rotation, 1.0f / 4096.0f, ix->quant, bt.prep, s->st);
+}
+
+// Route nq prepared queries into (ps, pc, pn) with row stride `stride`.
+void route_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, size_t nq, const sieve_params* p, float* ps,
+                 uint32_t* pc, uint32_t* pn, int stride) {
+    uint32_t* ovf = grow<uint32_t>(s->b[B_OVF], 1);
+    SIEVE_HIP(hipMemsetAsync(ovf, 0, 4, s->st));
+    const size_t per = std::max<size_t>(1, std::min<size_t>(nq, (size_t(1) << 30) / (4 * (size_t)ix->k)));
+    float* approx = grow<float>(s->b[B_APPROX], per * ix->k);
+    for (size_t q0 = 0; q0 < nq; q0 += per) {
+        size_t m = std::min(per, nq - q0);
+        launch_gemm_bf16_route(bt.prep.qbf + q0 * PDIM, ix->cents_bf, approx, (int)m, (int)ix->k, ix->k, s->st);
+        RouteArgs ra{};
+        ra.approx = approx;
+        ra.lds = ix->k;
+        ra.k = (int)ix->k;
+        ra.cents = ix->cents;
+        ra.q256 = bt.q256 + q0 * PDIM;
+        ra.qnorm = bt.prep.qnorm + q0;
+        ra.cl_slots = ix->cl_slots;
+        ra.cent_norm_max = ix->cent_norm_max;
+        ra.nprobe = (int)p->nprobe;
+        ra.margin = p->margin;
+        ra.budget = p->budget;
+        ra.out_s = ps + q0 * stride;
+        ra.out_c = pc + q0 * stride;
+        ra.out_n = pn + q0;
+        ra.out_stride = stride;
+        ra.overflow = ovf;
+        launch_route_select(ra, m, s->st);
+    }
+    SIEVE_HIP(hipGetLastError());
+}
+
+void check_overflow(sieve_stream* s) {
+    uint32_t h = 0;
+    SIEVE_HIP(hipMemcpyAsync(&h, s->b[B_OVF].p, 4, hipMemcpyDeviceToHost, s->st));
+    SIEVE_HIP(hipStreamSynchronize(s->st));
+    SIEVE_REQUIRE(h == 0, SIEVE_ELIMIT,
+                  "routing: more than " + std::to_string(MAX_RESCORE) +
+                      " centroids within the BF16 error band of the nprobe-th score");
+}
+
+void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, size_t nq, const float* ps,
+                const uint32_t* pc, const uint32_t* pn, int pstride, const sieve_params* p, const ViewArgs& view,
+                float* ce, uint32_t* cr, float* cx, uint32_t* cn, int cstride, unsigned long long* scanned) {
+    ScanArgs a{};
+    a.codes = ix->codes;
+    a.mul = ix->mul;
+    a.err = ix->err;
+    a.ids = ix->ids;
+    a.block_off = ix->block_off;
+    a.tables = bt.prep.tables;
+    a.qab = bt.prep.qab;
+    a.probe_s = ps;
+    a.probe_c = pc;
+    a.probe_n = pn;
+    a.probe_stride = pstride;
+    a.cand = (int)p->cand;
+    a.view = view;
+    a.out_est = ce;
+    a.out_row = cr;
+    a.out_err = cx;
+    a.out_n = cn;
+    a.out_stride = cstride;
+    a.scanned = scanned;
+    const size_t lds = scan_lds_bytes(a.cand, pstride);
+    SIEVE_REQUIRE(lds <= 160 * 1024, SIEVE_ELIMIT, "scan workspace exceeds LDS");
+    launch_scan(a, nq, s->st);
+    SIEVE_HIP(hipGetLastError());
+}
+
+void rerank_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, size_t nq, const float* ce,
+                  const uint32_t* cr, const float* cx, const uint32_t* cn, int cstride, const sieve_params* p,
+                  float* os, uint32_t* orr, uint32_t* on, unsigned long long* reranked) {
+    RerankArgs a{};
+    a.est = ce;
+    a.rows = cr;
+    a.errs = cx;
+    a.cand_n = cn;
+    a.cand_stride = cstride;
+    a.rerank = ix->rerank;
+    a.norm256 = ix->norm256;
+    a.t2_codes = ix->t2_codes;
+    a.t2_norm = ix->t2_norm;
+    a.t2_sigma = ix->t2_sigma;
+    a.q256 = bt.q256;
+    a.wq768 = bt.prep.wq768;
+    a.wq256 = bt.prep.wq256;
+    a.inv = bt.prep.inv;
+    a.mode = (int)p->rerank;
+    a.r = (int)p->rerank_r;
+    a.k = (int)p->k;
+    a.prune_m = (int)p->prune_m;
+    a.prune_w = p->prune_w;
+    a.out_s = os;
+    a.out_r = orr;
+    a.out_n = on;
+    a.reranked = reranked;
+    const bool t2 = a.mode == SIEVE_RERANK_TIERED && a.t2_codes;
+    SIEVE_REQUIRE(rerank_lds_bytes(cstride, t2) <= 160 * 1024, SIEVE_ELIMIT, "rerank workspace exceeds LDS");
+    launch_rerank(a, nq, s->st);
+    SIEVE_HIP(hipGetLastError());
+}
+
+void ensure_events(sieve_stream* s) {
+    for (auto& e : s->ev)
+        if (!e) SIEVE_HIP(hipEventCreate(&e));
+}
+
+// ------------------------------------------------------------------ index building ---
+struct HostCopy {
+    std::vector<uint8_t> data;
+};
+
+template <class T>
+const T* host_view(const T* p, size_t n, HostCopy& hc) {
+    if (!p || !on_device(p)) return p;
+    hc.data.resize(n * sizeof(T));
+    SIEVE_HIP(hipMemcpy(hc.data.data(), p, n * sizeof(T), hipMemcpyDeviceToHost));
+    return reinterpret_cast<const T*>(hc.data.data());
+}
+
+template <class T>
+T* upload(const T* src, size_t n, size_t* acct) {
+    T* d = dmalloc<T>(n, acct);
+    if (n) SIEVE_HIP(hipMemcpy(d, src, n * sizeof(T), on_device(src) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+    return d;
+}
+
+sieve_index* create_index(const sieve_index_desc* d) {
+    SIEVE_REQUIRE(d, SIEVE_EINVAL, "desc is null");
+    SIEVE_REQUIRE(d->k >= 1, SIEVE_EINVAL, "k must be >= 1");
+    SIEVE_REQUIRE(d->centroids && d->block_off && d->blocks && d->ids && d->rerank && d->quant && d->norm256,
+                  SIEVE_EINVAL, "desc is missing a required array");
+    SIEVE_REQUIRE(d->factors && d->rotation, SIEVE_EINVAL,
+                  "the GPU path needs a RaBitQ index (factors.f32 and rotation.f32)");
+    SIEVE_REQUIRE(d->nblocks * BLOCK < (1ull << 32) - 1, SIEVE_ELIMIT, "more than 2^32 - 1 slots");
+    std::unique_ptr<sieve_index> ix(new sieve_index());
+    SIEVE_HIP(hipGetDevice(&ix->device));
+    ix->k = d->k;
+    ix->n = d->n;
+    ix->nblocks = d->nblocks;
+    ix->nslots = d->nblocks * BLOCK;
+    size_t* acct = &ix->device_bytes;
+
+    HostCopy hb, hc;
+    const uint64_t* bo = host_view(d->block_off, d->k + 1, hb);
+    SIEVE_REQUIRE(bo[0] == 0 && bo[d->k] == d->nblocks, SIEVE_EINVAL, "block_off does not match nblocks");
+    std::vector<uint32_t> bo32(d->k + 1);
+    for (uint32_t c = 0; c <= d->k; c++) {
+        if (c) SIEVE_REQUIRE(bo[c] >= bo[c - 1], SIEVE_EINVAL, "block_off is not monotone");
+        bo32[c] = (uint32_t)bo[c];
+        if (c) ix->max_cluster_blocks = std::max<uint32_t>(ix->max_cluster_blocks, (uint32_t)(bo[c] - bo[c - 1]));
+    }
+    const float* cents = host_view(d->centroids, (size_t)d->k * PDIM, hc);
+    double mx = 0.0;
+    for (uint32_t c = 0; c < d->k; c++) {
+        double s = 0.0;
+        for (int i = 0; i < PDIM; i++) s += (double)cents[(size_t)c * PDIM + i] * cents[(size_t)c * PDIM + i];
+        mx = std::max(mx, std::sqrt(s));
+    }
+    ix->cent_norm_max = (float)(mx * (1.0 + 1e-6));
+
+    hipStream_t st = nullptr;
+    ix->cents = upload(cents, (size_t)d->k * PDIM, acct);
+    ix->cents_bf = dmalloc<uint16_t>((size_t)d->k * PDIM, acct);
+    launch_to_bf16(ix->cents, (size_t)d->k * PDIM, ix->cents_bf, st);
+    ix->block_off = upload(bo32.data(), d->k + 1, acct);
+    ix->cl_slots = dmalloc<uint32_t>(d->k, acct);
+    launch_cluster_slots(ix->block_off, d->k, ix->cl_slots, st);
+
+    // codes: blocks.bin -> slot-major, in chunks through a staging buffer
+    ix->codes = dmalloc<uint4>(ix->nslots * 2, acct);
+    const bool blocks_dev = on_device(d->blocks);
+    const size_t chunk = 1 << 18;  // blocks per staging pass (256 MiB)
+    uint8_t* stage = nullptr;
+    if (!blocks_dev && d->nblocks) stage = dmalloc<uint8_t>(std::min<size_t>(chunk, d->nblocks) * BLOCK_BYTES, nullptr);
+    for (size_t b0 = 0; b0 < d->nblocks; b0 += chunk) {
+        size_t nb = std::min(chunk, (size_t)d->nblocks - b0);
+        const uint8_t* src = d->blocks + b0 * BLOCK_BYTES;
+        if (!blocks_dev) {
+            SIEVE_HIP(hipMemcpy(stage, src, nb * BLOCK_BYTES, hipMemcpyHostToDevice));
+            src = stage;
+        }
+        launch_transpose_blocks(src, nb, ix->codes + b0 * BLOCK * 2, st);
+        SIEVE_HIP(hipDeviceSynchronize());
+    }
+    if (stage) (void)hipFree(stage);
+
+    // factors -> mul[], err[]
+    ix->mul = dmalloc<float>(ix->nslots, acct);
+    ix->err = dmalloc<float>(ix->nslots, acct);
+    {
+        const float* f = d->factors;
+        float* tmp = nullptr;
+        if (!on_device(f)) {
+            tmp = dmalloc<float>(ix->nslots * 2, nullptr);
+            SIEVE_HIP(hipMemcpy(tmp, f, ix->nslots * 8, hipMemcpyHostToDevice));
+            f = tmp;
+        }
+        launch_split_factors(f, ix->nslots, ix->mul, ix->err, st);
+        SIEVE_HIP(hipDeviceSynchronize());
+        if (tmp) (void)hipFree(tmp);
+    }
+    ix->ids = upload(d->ids, ix->nslots, acct);
+    ix->rotation = upload(d->rotation, 3 * PDIM, acct);
+    if (d->borrow_rerank && on_device(d->rerank)) {
+        ix->rerank = const_cast<int8_t*>(d->rerank);
+        ix->rerank_owned = false;
+    } else {
+        ix->rerank = upload(d->rerank, (size_t)d->n * DIM, acct);
+    }
+    ix->quant = upload(d->quant, DIM, acct);
+    ix->norm256 = upload(d->norm256, d->n, acct);
+    if (d->tier2_sigma && d->tier2_codes && d->tier2_norm) {
+        ix->t2_sigma = upload(d->tier2_sigma, PDIM, acct);
+        ix->t2_codes = upload(d->tier2_codes, (size_t)d->n * 64, acct);
+        ix->t2_norm = upload(d->tier2_norm, d->n, acct);
+    }
+    SIEVE_HIP(hipDeviceSynchronize());
+    return ix.release();
+}
+
+struct Mapped {
+    void* p = nullptr;
+    size_t len = 0;
+    ~Mapped() {
+        if (p && len) munmap(p, len);
+    }
+};
+
+bool map_file(const std::string& path, Mapped& m, bool required) {
+    int fd = ::open(path.c_str(), O_RDONLY);
+    if (fd < 0) {
+        SIEVE_REQUIRE(!required, SIEVE_EIO, "cannot open " + path);
+        return false;
+    }
+    struct stat stt;
+    if (fstat(fd, &stt) != 0) {
+        ::close(fd);
+        throw Error{SIEVE_EIO, "cannot stat " + path};
+    }
+    m.len = (size_t)stt.st_size;
+    if (m.len) {
+        m.p = mmap(nullptr, m.len, PROT_READ, MAP_PRIVATE, fd, 0);
+        if (m.p == MAP_FAILED) {
+            m.p = nullptr;
+            ::close(fd);
+            throw Error{SIEVE_EIO, "cannot map " + path};
+        }
+    }
+    ::close(fd);
+    return true;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI ==========
+extern "C" {
+
+const char* sieve_last_error(void) { return g_error.c_str(); }
+
+int sieve_stream_create(sieve_stream** out) {
+    return guarded([&] {
+        SIEVE_REQUIRE(out, SIEVE_EINVAL, "out is null");
+        std::unique_ptr<sieve_stream> s(new sieve_stream());
+        SIEVE_HIP(hipGetDevice(&s->device));
+        SIEVE_HIP(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+        ensure_events(s.get());
+        *out = s.release();
+    });
+}
+
+void sieve_stream_destroy(sieve_stream* s) { delete s; }
+
+int sieve_stream_sync(sieve_stream* s) {
+    return guarded([&] {
+        SIEVE_REQUIRE(s, SIEVE_EINVAL, "stream is null");
+        SIEVE_HIP(hipStreamSynchronize(s->st));
+    });
+}
+
+void* sieve_stream_handle(sieve_stream* s) { return s ? (void*)s->st : nullptr; }
+
+int sieve_gemm_abt(const float* a, const float* b, size_t m, size_t n, size_t k, float* c) {
+    return guarded([&] {
+        SIEVE_REQUIRE(a && b && c, SIEVE_EINVAL, "null matrix");
+        SIEVE_REQUIRE(m < (1u << 31) && n < (1u << 31) && k < (1u << 31), SIEVE_ELIMIT, "matrix too large");
+        if (m == 0 || n == 0) return;
+        sieve_stream s;
+        SIEVE_HIP(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
+        const float* da = dev_in(&s, B_TMP1, a, m * k);
+        const float* db = dev_in(&s, B_TMP2, b, n * k);
+        DevOut<float> dc = dev_out(&s, B_TMP3, c, m * n);
+        launch_gemm_f32_abt(da, db, dc.dev, (int)m, (int)n, (int)k, s.st);
+        SIEVE_HIP(hipGetLastError());
+        out_done(&s, dc);
+        SIEVE_HIP(hipStreamSynchronize(s.st));
+    });
+}
+
+int sieve_lut_scan(const uint8_t* tables, uint32_t groups, const uint8_t* blocks, size_t nblocks, uint16_t* out) {
+    return guarded([&] {
+        SIEVE_REQUIRE(tables && blocks && out, SIEVE_EINVAL, "null argument");
+        SIEVE_REQUIRE(groups == 64 || groups == 32, SIEVE_EINVAL, "groups must be 64 or 32");
+        if (nblocks == 0) return;
+        sieve_stream s;
+        SIEVE_HIP(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
+        const uint8_t* dt = dev_in(&s, B_TMP1, tables, (size_t)groups * 16);
+        const uint8_t* dbk = dev_in(&s, B_TMP2, blocks, nblocks * groups * 16);
+        DevOut<uint16_t> o = dev_out(&s, B_TMP3, out, nblocks * BLOCK);
+        launch_lut_scan(dt, (int)groups, dbk, nblocks, o.dev, s.st);
+        SIEVE_HIP(hipGetLastError());
+        out_done(&s, o);
+        SIEVE_HIP(hipStreamSynchronize(s.st));
+    });
+}
+
+int sieve_index_create(const sieve_index_desc* desc, sieve_index** out) {
+    return guarded([&] {
+        SIEVE_REQUIRE(out, SIEVE_EINVAL, "out is null");
+        *out = create_index(desc);
+    });
+}
+
+int sieve_index_open(const char* index_dir, const char* ingest_dir, sieve_index** out) {
+    return guarded([&] {
+        SIEVE_REQUIRE(index_dir && ingest_dir && out, SIEVE_EINVAL, "null argument");
+        std::string I(index_dir), C(ingest_dir);
+        Mapped cents, bo, blocks, ids, factors, rot, rr, quant, norm, t2s, t2c, t2n;
+        map_file(I + "/centroids.f32", cents, true);
+        map_file(I + "/block_off.u64", bo, true);
+        map_file(I + "/blocks.bin", blocks, true);
+        map_file(I + "/ids.u32", ids, true);
+        map_file(I + "/factors.f32", factors, true);
+        map_file(I + "/rotation.f32", rot, true);
+        map_file(C + "/rerank.i8", rr, true);
+        map_file(C + "/quant.f32", quant, true);
+        map_file(C + "/norm256.f32", norm, true);
+        bool t2 = map_file(I + "/tier2.bin", t2c, false);
+        if (t2) {
+            map_file(I + "/tier2.sigma", t2s, true);
+            map_file(I + "/tier2.norm", t2n, true);
+        }
+        sieve_index_desc d{};
+        d.k = (uint32_t)(cents.len / (4 * PDIM));
+        d.n = norm.len / 4;
+        d.nblocks = blocks.len / BLOCK_BYTES;
+        SIEVE_REQUIRE(cents.len == (size_t)d.k * PDIM * 4, SIEVE_EIO, "centroids.f32 size");
+        SIEVE_REQUIRE(bo.len == (size_t)(d.k + 1) * 8, SIEVE_EIO, "block_off.u64 does not match centroids");
+        SIEVE_REQUIRE(blocks.len == d.nblocks * BLOCK_BYTES, SIEVE_EIO, "blocks.bin size");
+        SIEVE_REQUIRE(ids.len == d.nblocks * BLOCK * 4, SIEVE_EIO, "ids.u32 size");
+        SIEVE_REQUIRE(factors.len == d.nblocks * BLOCK * 8, SIEVE_EIO, "factors.f32 size");
+        SIEVE_REQUIRE(rot.len == 3 * PDIM * 4, SIEVE_EIO, "rotation.f32 size");
+        SIEVE_REQUIRE(rr.len == d.n * DIM, SIEVE_EIO, "rerank.i8 size");
+        SIEVE_REQUIRE(quant.len == DIM * 4, SIEVE_EIO, "quant.f32 size");
+        d.centroids = (const float*)cents.p;
+        d.block_off = (const uint64_t*)bo.p;
+        d.blocks = (const uint8_t*)blocks.p;
+        d.ids = (const uint32_t*)ids.p;
+        d.factors = (const float*)factors.p;
+        d.rotation = (const float*)rot.p;
+        d.rerank = (const int8_t*)rr.p;
+        d.quant = (const float*)quant.p;
+        d.norm256 = (const float*)norm.p;
+        if (t2) {
+            SIEVE_REQUIRE(t2c.len == d.n * 64 && t2n.len == d.n * 4 && t2s.len == PDIM * 4, SIEVE_EIO, "tier2 size");
+            d.tier2_codes = (const uint8_t*)t2c.p;
+            d.tier2_norm = (const float*)t2n.p;
+            d.tier2_sigma = (const float*)t2s.p;
+        }
+        *out = create_index(&d);
+    });
+}
+
+void sieve_index_free(sieve_index* ix) { delete ix; }
+
+int sieve_index_get_info(const sieve_index* ix, sieve_index_info* out) {
+    return guarded([&] {
+        SIEVE_REQUIRE(ix && out, SIEVE_EINVAL, "null argument");
+        out->k = ix->k;
+        out->n = ix->n;
+        out->nblocks = ix->nblocks;
+        out->rabitq = 1;
+        out->tier2 = ix->t2_codes ? 1 : 0;
+        out->max_cluster_blocks = ix->max_cluster_blocks;
+        out->device_bytes = ix->device_bytes;
+    });
+}
+
+int sieve_route(const sieve_index* ix, sieve_stream* s, const float* q256, size_t nq, const sieve_params* p,
+                float* scores, uint32_t* clusters, uint32_t* counts, sieve_timing* t) {
+    return guarded([&] {
+        SIEVE_REQUIRE(ix && s && q256 && scores && clusters && counts, SIEVE_EINVAL, "null argument");
+        check_params(p, true);
+        if (nq == 0) return;
+        const int stride = (int)p->nprobe;
+        const float* q2 = dev_in(s, B_Q256, q256, nq * PDIM);
+        // query prep needs a q768; routing ignores the int8 parts, so q256 stands in for it
+        // only where prep reads 768 values: give it a zero buffer of the right size.
+        float* q7 = grow<float>(s->b[B_Q768], nq * DIM);
+        SIEVE_HIP(hipMemsetAsync(q7, 0, nq * DIM * 4, s->st));
+        SIEVE_HIP(hipEventRecord(s->ev[0], s->st));
+        Batch bt;
+        prep_queries(ix, s, q2, q7, nq, bt);
+        DevOut<float> os = dev_out(s, B_PS, scores, nq * stride);
+        DevOut<uint32_t> oc = dev_out(s, B_PC, clusters, nq * stride);
+        DevOut<uint32_t> on = dev_out(s, B_PN, counts, nq);
+        route_batch(ix, s, bt, nq, p, os.dev, oc.dev, on.dev, stride);
+        SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
+        out_done(s, os);
+        out_done(s, oc);
+        out_done(s, on);
+        check_overflow(s);
+        if (t) t->route += (uint64_t)elapsed_ns(s->ev[0], s->ev[1]);
+    });
+}
+
+int sieve_candidates(const sieve_index* ix, sieve_stream* s, const float* q256, size_t nq, const float* probe_scores,
+                     const uint32_t* probe_clusters, const uint32_t* probe_counts, size_t nprobe_stride,
+                     const sieve_params* p, const sieve_view* v, float* est, uint32_t* rows, float* errs,
+                     uint32_t* counts, sieve_timing* t) {
+    return guarded([&] {
+        SIEVE_REQUIRE(ix && s && q256 && probe_scores && probe_clusters && probe_counts && est && rows && errs && counts,
+                      SIEVE_EINVAL, "null argument");
+        check_params(p, false);
+        SIEVE_REQUIRE(nprobe_stride >= 1 && nprobe_stride <= (size_t)MAX_PROBE, SIEVE_ELIMIT, "probe stride limit");
+        if (nq == 0) return;
+        // Probe lists come from the caller: check them on the host before any kernel reads them.
+        std::vector<uint32_t> hn(nq), hc(nq * nprobe_stride);
+        if (on_device(probe_counts)) SIEVE_HIP(hipMemcpy(hn.data(), probe_counts, nq * 4, hipMemcpyDeviceToHost));
+        else std::memcpy(hn.data(), probe_counts, nq * 4);
+        if (on_device(probe_clusters)) SIEVE_HIP(hipMemcpy(hc.data(), probe_clusters, hc.size() * 4, hipMemcpyDeviceToHost));
+        else std::memcpy(hc.data(), probe_clusters, hc.size() * 4);
+        for (size_t i = 0; i < nq; i++) {
+            SIEVE_REQUIRE(hn[i] <= nprobe_stride, SIEVE_EINVAL, "probe count exceeds stride");
+            for (uint32_t j = 0; j < hn[i]; j++)
+                SIEVE_REQUIRE(hc[i * nprobe_stride + j] < ix->k, SIEVE_EINVAL, "probe cluster out of range");
+        }
+        const float* q2 = dev_in(s, B_Q256, q256, nq * PDIM);
+        float* q7 = grow<float>(s->b[B_Q768], nq * DIM);
+        SIEVE_HIP(hipMemsetAsync(q7, 0, nq * DIM * 4, s->st));
+        const float* ps = dev_in(s, B_PS, probe_scores, nq * nprobe_stride);
+        const uint32_t* pc = dev_in(s, B_PC, probe_clusters, nq * nprobe_stride);
+        const uint32_t* pn = dev_in(s, B_PN, probe_counts, nq);
+        ViewArgs view = make_view(s, v, nq);
+        unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 4);
+        SIEVE_HIP(hipMemsetAsync(cnt, 0, 32, s->st));
+        Batch bt;
+        prep_queries(ix, s, q2, q7, nq, bt);
+        DevOut<float> oe = dev_out(s, B_CE, est, nq * p->cand);
+        DevOut<uint32_t> orow = dev_out(s, B_CR, rows, nq * p->cand);
+        DevOut<float> oerr = dev_out(s, B_CX, errs, nq * p->cand);
+        DevOut<uint32_t> on = dev_out(s, B_CN, counts, nq);
+        SIEVE_HIP(hipEventRecord(s->ev[0], s->st));
+        scan_batch(ix, s, bt, nq, ps, pc, pn, (int)nprobe_stride, p, view, oe.dev, orow.dev, oerr.dev, on.dev,
+                   (int)p->cand, cnt);
+        SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
+        out_done(s, oe);
+        out_done(s, orow);
+        out_done(s, oerr);
+        out_done(s, on);
+        unsigned long long hcnt[4] = {0, 0, 0, 0};
+        SIEVE_HIP(hipMemcpyAsync(hcnt, cnt, 32, hipMemcpyDeviceToHost, s->st));
+        SIEVE_HIP(hipStreamSynchronize(s->st));
+        if (t) {
+            t->scan += (uint64_t)elapsed_ns(s->ev[0], s->ev[1]);
+            t->scanned += hcnt[0];
+            for (size_t i = 0; i < nq; i++) t->probed += hn[i];
+        }
+    });
+}
+
+int sieve_rerank(const sieve_index* ix, sieve_stream* s, const float* q768, const float* q256, size_t nq,
+                 const float* est, const uint32_t* rows, const float* errs, const uint32_t* cand_counts,
+                 size_t cand_stride, const sieve_params* p, const sieve_view* v, float* scores, uint32_t* out_rows,
+                 uint32_t* counts, sieve_timing* t) {
+    (void)v;
+    return guarded([&] {
+        SIEVE_REQUIRE(ix && s && q768 && q256 && est && rows && errs && cand_counts && scores && out_rows && counts,
+                      SIEVE_EINVAL, "null argument");
+        check_params(p, false);
+        SIEVE_REQUIRE(cand_stride >= 1 && cand_stride <= (size_t)MAX_CAND, SIEVE_ELIMIT, "candidate stride limit");
+        if (nq == 0) return;
+        std::vector<uint32_t> hn(nq), hr(nq * cand_stride);
+        if (on_device(cand_counts)) SIEVE_HIP(hipMemcpy(hn.data(), cand_counts, nq * 4, hipMemcpyDeviceToHost));
+        else std::memcpy(hn.data(), cand_counts, nq * 4);
+        if (on_device(rows)) SIEVE_HIP(hipMemcpy(hr.data(), rows, hr.size() * 4, hipMemcpyDeviceToHost));
+        else std::memcpy(hr.data(), rows, hr.size() * 4);
+        for (size_t i = 0; i < nq; i++) {
+            SIEVE_REQUIRE(hn[i] <= cand_stride, SIEVE_EINVAL, "candidate count exceeds stride");
+            for (uint32_t j = 0; j < hn[i]; j++)
+                SIEVE_REQUIRE(hr[i * cand_stride + j] < ix->n, SIEVE_EINVAL, "candidate row out of range");
+        }
+        const float* q2 = dev_in(s, B_Q256, q256, nq * PDIM);
+        const float* q7 = dev_in(s, B_Q768, q768, nq * DIM);
+        const float* ce = dev_in(s, B_CE, est, nq * cand_stride);
+        const uint32_t* cr = dev_in(s, B_CR, rows, nq * cand_stride);
+        const float* cx = dev_in(s, B_CX, errs, nq * cand_stride);
+        const uint32_t* cn = dev_in(s, B_CN, cand_counts, nq);
+        unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 4);
+        SIEVE_HIP(hipMemsetAsync(cnt, 0, 32, s->st));
+        Batch bt;
+        prep_queries(ix, s, q2, q7, nq, bt);
+        DevOut<float> os = dev_out(s, B_OS, scores, nq * p->k);
+        DevOut<uint32_t> orr = dev_out(s, B_OR, out_rows, nq * p->k);
+        DevOut<uint32_t> on = dev_out(s, B_ON, counts, nq);
+        SIEVE_HIP(hipEventRecord(s->ev[0], s->st));
+        rerank_batch(ix, s, bt, nq, ce, cr, cx, cn, (int)cand_stride, p, os.dev, orr.dev, on.dev, cnt + 1);
+        SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
+        out_done(s, os);
+        out_done(s, orr);
+        out_done(s, on);
+        unsigned long long hcnt[4] = {0, 0, 0, 0};
+        SIEVE_HIP(hipMemcpyAsync(hcnt, cnt, 32, hipMemcpyDeviceToHost, s->st));
+        SIEVE_HIP(hipStreamSynchronize(s->st));
+        if (t) {
+            t->rerank += (uint64_t)elapsed_ns(s->ev[0], s->ev[1]);
+            t->reranked += hcnt[1];
+        }
+    });
+}
+
+int sieve_search_batch(const sieve_index* ix, sieve_stream* s, const float* q768, const float* q256, size_t nq,
+                       const sieve_params* p, const sieve_view* v, float* scores, uint32_t* rows, uint32_t* counts,
+                       sieve_timing* t) {
+    return guarded([&] {
+        SIEVE_REQUIRE(ix && s && q768 && q256 && scores && rows && counts, SIEVE_EINVAL, "null argument");
+        check_params(p, true);
+        if (nq == 0) return;
+        const int pstride = (int)std::min<uint32_t>(p->nprobe, ix->k);
+        const float* q2 = dev_in(s, B_Q256, q256, nq * PDIM);
+        const float* q7 = dev_in(s, B_Q768, q768, nq * DIM);
+        ViewArgs view = make_view(s, v, nq);
+        unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 4);
+        float* ps = grow<float>(s->b[B_PS], nq * pstride);
+        uint32_t* pc = grow<uint32_t>(s->b[B_PC], nq * pstride);
+        uint32_t* pn = grow<uint32_t>(s->b[B_PN], nq);
+        float* ce = grow<float>(s->b[B_CE], nq * p->cand);
+        uint32_t* cr = grow<uint32_t>(s->b[B_CR], nq * p->cand);
+        float* cx = grow<float>(s->b[B_CX], nq * p->cand);
+        uint32_t* cn = grow<uint32_t>(s->b[B_CN], nq);
+        DevOut<float> os = dev_out(s, B_OS, scores, nq * p->k);
+        DevOut<uint32_t> orr = dev_out(s, B_OR, rows, nq * p->k);
+        DevOut<uint32_t> on = dev_out(s, B_ON, counts, nq);
+        SIEVE_HIP(hipMemsetAsync(cnt, 0, 32, s->st));
+        SIEVE_HIP(hipEventRecord(s->ev[0], s->st));
+        Batch bt;
+        prep_queries(ix, s, q2, q7, nq, bt);
+        route_batch(ix, s, bt, nq, p, ps, pc, pn, pstride);
+        SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
+        scan_batch(ix, s, bt, nq, ps, pc, pn, pstride, p, view, ce, cr, cx, cn, (int)p->cand, cnt);
+        SIEVE_HIP(hipEventRecord(s->ev[2], s->st));
+        rerank_batch(ix, s, bt, nq, ce, cr, cx, cn, (int)p->cand, p, os.dev, orr.dev, on.dev, cnt + 1);
+        SIEVE_HIP(hipEventRecord(s->ev[3], s->st));
+        out_done(s, os);
+        out_done(s, orr);
+        out_done(s, on);
+        unsigned long long hcnt[4] = {0, 0, 0, 0};
+        SIEVE_HIP(hipMemcpyAsync(hcnt, cnt, 32, hipMemcpyDeviceToHost, s->st));
+        std::vector<uint32_t> hpn(t ? nq : 0);
+        if (t) SIEVE_HIP(hipMemcpyAsync(hpn.data(), pn, nq * 4, hipMemcpyDeviceToHost, s->st));
+        check_overflow(s);
+        if (t) {
+            t->route += (uint64_t)elapsed_ns(s->ev[0], s->ev[1]);
+            t->scan += (uint64_t)elapsed_ns(s->ev[1], s->ev[2]);
+            t->rerank += (uint64_t)elapsed_ns(s->ev[2], s->ev[3]);
+            t->scanned += hcnt[0];
+            t->reranked += hcnt[1];
+            for (uint32_t x : hpn) t->probed += x;
+        }
+    });
+}
+
+}  // extern "C"

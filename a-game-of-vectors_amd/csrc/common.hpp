@@ -1,0 +1,99 @@
+// Shared definitions of the sieve MI355X library (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/sieve_gpu.h"
+
+namespace sieve {
+
+constexpr int DIM = 768;
+constexpr int PDIM = 256;
+constexpr int GROUPS = 64;           // 4-dim groups of the 256-dim prefix
+constexpr int BLOCK = 32;            // slots per code block
+constexpr int BLOCK_BYTES = 1024;    // GROUPS * 16
+constexpr uint32_t PAD = 0xFFFFFFFFu;
+constexpr int WAVE = 64;
+
+// Largest candidate list a query may keep (p->cand) and largest probe count (p->nprobe):
+// both bound LDS arrays of the per-query kernels.
+constexpr int MAX_CAND = 2048;
+constexpr int MAX_PROBE = 1024;
+// Centroids within the BF16 error band of the nprobe-th routing score that one query may
+// rescore in f32.
+constexpr int MAX_RESCORE = 4096;
+
+// ------------------------------------------------------------------ errors (host) ---
+void set_error(const std::string& msg);
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define SIEVE_HIP(expr)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw ::sieve::Error{SIEVE_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+#define SIEVE_REQUIRE(cond, code, msg)                                                     \
+    do {                                                                                   \
+        if (!(cond)) throw ::sieve::Error{(code), (msg)};                                  \
+    } while (0)
+
+// ------------------------------------------------------------------ device helpers ---
+// Order-preserving map of an f32 to u32: a > b  <=>  key(a) > key(b) (no NaNs reach here;
+// -0.0 maps below +0.0, which only orders values that compare equal).
+__device__ __forceinline__ uint32_t f2key(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key2f(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    return __uint_as_float(u);
+}
+
+__device__ __forceinline__ bool bit_set(const uint64_t* set, uint64_t words, uint32_t id) {
+    uint64_t w = id >> 6;
+    return w < words && ((set[w] >> (id & 63)) & 1ull);
+}
+
+// Rust `f32 as i8`: saturating.
+__device__ __forceinline__ int8_t sat_i8(float v) {
+    if (v != v) return 0;
+    if (v >= 127.0f) return 127;
+    if (v <= -128.0f) return -128;
+    return (int8_t)(int)v;
+}
+
+__device__ __forceinline__ int wave_lane() { return __lane_id(); }
+
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+}  // namespace sieve

@@ -1,0 +1,121 @@
+// Kernel launchers of the sieve MI355X library. Every launcher enqueues on `st` and
+// returns immediately; shapes are checked by the host code in api.hip before a launch.
+#pragma once
+
+#include "common.hpp"
+
+namespace sieve {
+
+// Per-query state written by the query-prep kernel (rabitq::Query::new, src/rabitq.rs:119-125,
+// and the int8 rerank queries, src/ivf.rs:753-755 / :786-788).
+struct QueryPrep {
+    uint8_t* tables;    // nq x 1024: 64 groups x 16 u8 entries
+    float4* qab;        // nq: (a, b, scale, bias)
+    int8_t* wq768;      // nq x 768
+    int8_t* wq256;      // nq x 256
+    float2* inv;        // nq: (1/s of the 768-d query, 1/s of the 256-d query)
+    uint16_t* qbf;      // nq x 256 BF16 copy of q256 (routing GEMM operand)
+    float* qnorm;       // nq: |q256|
+};
+
+void launch_query_prep(const float* q256, const float* q768, size_t nq, const float* signs, float rot_norm,
+                       const float* quant, const QueryPrep& out, hipStream_t st);
+
+// Index re-layout at creation.
+void launch_transpose_blocks(const uint8_t* blocks, size_t nblocks, uint4* codes, hipStream_t st);
+void launch_split_factors(const float* factors, size_t nslots, float* mul, float* err, hipStream_t st);
+void launch_to_bf16(const float* x, size_t n, uint16_t* y, hipStream_t st);
+void launch_cluster_slots(const uint32_t* block_off, uint32_t k, uint32_t* slots, hipStream_t st);
+
+// C = A * B^T in f32 (m x k, n x k -> m x n, row-major), f32 MFMA.
+void launch_gemm_f32_abt(const float* a, const float* b, float* c, int m, int n, int k, hipStream_t st);
+// S = Qbf * Cbf^T (m x 256, n x 256 BF16 -> m x n f32, row stride lds), BF16 MFMA.
+void launch_gemm_bf16_route(const uint16_t* qbf, const uint16_t* cbf, float* s, int m, int n, size_t lds,
+                            hipStream_t st);
+
+struct RouteArgs {
+    const float* approx;       // nq x k approximate scores (row stride lds)
+    size_t lds;
+    int k;
+    const float* cents;        // k x 256 f32
+    const float* q256;         // nq x 256
+    const float* qnorm;        // nq
+    const uint32_t* cl_slots;  // k: stored slots per cluster
+    float cent_norm_max;
+    int nprobe;
+    float margin;              // NaN = none
+    uint64_t budget;           // 0 = none
+    float* out_s;
+    uint32_t* out_c;
+    uint32_t* out_n;
+    int out_stride;
+    uint32_t* overflow;        // set to 1 if a query had more than MAX_RESCORE near-margin centroids
+};
+void launch_route_select(const RouteArgs& a, size_t nq, hipStream_t st);
+
+struct ViewArgs {
+    const uint64_t* deleted;
+    uint64_t deleted_words;
+    const uint64_t* allow;
+    uint64_t allow_words;
+    uint64_t allow_stride;
+    const uint8_t* allow_flags;
+};
+
+struct ScanArgs {
+    const uint4* codes;          // slot-major 32-byte codes
+    const float* mul;
+    const float* err;
+    const uint32_t* ids;
+    const uint32_t* block_off;   // k + 1
+    const uint8_t* tables;       // nq x 1024
+    const float4* qab;           // nq
+    const float* probe_s;        // nq x probe_stride
+    const uint32_t* probe_c;
+    const uint32_t* probe_n;     // nq
+    int probe_stride;
+    int cand;
+    ViewArgs view;
+    float* out_est;              // nq x out_stride
+    uint32_t* out_row;
+    float* out_err;
+    uint32_t* out_n;
+    int out_stride;
+    unsigned long long* scanned; // += stored slots scanned
+};
+size_t scan_lds_bytes(int cand, int probe_stride);
+void launch_scan(const ScanArgs& a, size_t nq, hipStream_t st);
+
+// The standalone lut::scan seam over the reference's block layout.
+void launch_lut_scan(const uint8_t* tables, int groups, const uint8_t* blocks, size_t nblocks, uint16_t* out,
+                     hipStream_t st);
+
+struct RerankArgs {
+    const float* est;            // nq x cand_stride
+    const uint32_t* rows;
+    const float* errs;
+    const uint32_t* cand_n;      // nq
+    int cand_stride;
+    const int8_t* rerank;        // n x 768
+    const float* norm256;
+    const uint8_t* t2_codes;     // n x 64 or null
+    const float* t2_norm;
+    const float* t2_sigma;
+    const float* q256;           // nq x 256 (tier-2 tables)
+    const int8_t* wq768;
+    const int8_t* wq256;
+    const float2* inv;
+    int mode;                    // SIEVE_RERANK_*
+    int r;
+    int k;
+    int prune_m;
+    float prune_w;
+    float* out_s;                // nq x k
+    uint32_t* out_r;
+    uint32_t* out_n;
+    unsigned long long* reranked;
+};
+size_t rerank_lds_bytes(int cand_stride, bool tier2_tables);
+void launch_rerank(const RerankArgs& a, size_t nq, hipStream_t st);
+
+}  // namespace sieve

@@ -1,0 +1,207 @@
+// Per-query preparation and index re-layout kernels.
+//
+// Query prep restates, per query, rabitq::Query::new (src/rabitq.rs:119-125): the rotation
+// (src/rabitq.rs:59-84), the u8 LUT (lut::Lut::new, src/lut.rs:58-81) and the affine map
+// (a, b); and the int8 rerank queries (quant::query_i8, src/quant.rs:102-110, over
+// w_i = q_i / scale_i, src/ivf.rs:753-755 and :786-788). Every f32 operation is the
+// reference's, in the reference's order, and the library is built with -ffp-contract=off, so
+// tables, a, b and the int8 queries are bit-identical to the reference's.
+#include <float.h>
+
+#include "kernels.hpp"
+
+namespace sieve {
+
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// One wave per query.
+__global__ __launch_bounds__(64) void k_query_prep(const float* __restrict__ q256, const float* __restrict__ q768,
+                                                   const float* __restrict__ signs, float rot_norm,
+                                                   const float* __restrict__ quant, QueryPrep out) {
+    __shared__ float v[PDIM];
+    __shared__ float mins[GROUPS];
+    const size_t qi = blockIdx.x;
+    const int lane = threadIdx.x;
+    const float* q = q256 + qi * PDIM;
+
+    float nn = 0.0f;
+    for (int i = lane; i < PDIM; i += WAVE) {
+        float x = q[i];
+        v[i] = x;
+        out.qbf[qi * PDIM + i] = f32_to_bf16_rne(x);
+        nn += x * x;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) nn += __shfl_xor(nn, d, 64);
+    if (lane == 0) out.qnorm[qi] = sqrtf(nn);
+    __syncthreads();
+
+    // Rotation::apply: three rounds of (sign flip, unnormalised FWHT), then * d^-1.5.
+    for (int r = 0; r < 3; r++) {
+        for (int i = lane; i < PDIM; i += WAVE) v[i] = v[i] * signs[r * PDIM + i];
+        __syncthreads();
+        for (int h = 1; h < PDIM; h <<= 1) {
+            for (int j = lane; j < PDIM / 2; j += WAVE) {
+                int blk = j / h, i = j - blk * h;
+                int a = blk * 2 * h + i, b = a + h;
+                float x = v[a], y = v[b];
+                v[a] = x + y;
+                v[b] = x - y;
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = lane; i < PDIM; i += WAVE) v[i] = v[i] * rot_norm;
+    __syncthreads();
+
+    // Lut::new: lane g builds group g's 16 sums (each a left-to-right sum from -0.0).
+    const int g = lane;
+    float f[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+        float s = -0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if ((c >> j) & 1) s = s + v[4 * g + j];
+        f[c] = s;
+    }
+    float mn = FLT_MAX, mx = -FLT_MAX;
+#pragma unroll
+    for (int c = 0; c < 16; c++) mn = fminf(mn, f[c]);
+#pragma unroll
+    for (int c = 0; c < 16; c++) mx = fmaxf(mx, f[c]);
+    // max over groups is exact and order-free: fold(0, max) == max(0, wave max).
+    float max_range = fmaxf(0.0f, wave_max(mx - mn));
+    float scale = max_range > 0.0f ? 255.0f / max_range : 1.0f;
+    uint32_t packed[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+        float t = roundf((f[c] - mn) * scale);
+        t = fminf(fmaxf(t, 0.0f), 255.0f);
+        packed[c >> 2] |= ((uint32_t)t) << ((c & 3) * 8);
+    }
+    reinterpret_cast<uint4*>(out.tables + qi * 1024)[g] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+    mins[g] = mn;
+    __syncthreads();
+    if (lane == 0) {
+        float bias = -0.0f;
+        for (int i = 0; i < GROUPS; i++) bias = bias + mins[i];
+        float sum = -0.0f;
+        for (int i = 0; i < PDIM; i++) sum = sum + v[i];
+        float a = 2.0f / scale;
+        float b = 2.0f * bias - sum;
+        out.qab[qi] = make_float4(a, b, scale, bias);
+    }
+
+    // int8 rerank queries: w = q / scale (division), s = 127 / max(1e-12, max |w|).
+    const float* q7 = q768 + qi * DIM;
+    float w[DIM / WAVE];
+    float m = 1e-12f;
+#pragma unroll
+    for (int j = 0; j < DIM / WAVE; j++) {
+        int i = lane + j * WAVE;
+        w[j] = q7[i] / quant[i];
+        m = fmaxf(m, fabsf(w[j]));
+    }
+    m = wave_max(m);
+    float s = 127.0f / m;
+#pragma unroll
+    for (int j = 0; j < DIM / WAVE; j++) out.wq768[qi * DIM + lane + j * WAVE] = sat_i8(roundf(w[j] * s));
+    float p[PDIM / WAVE];
+    float mp = 1e-12f;
+#pragma unroll
+    for (int j = 0; j < PDIM / WAVE; j++) {
+        int i = lane + j * WAVE;
+        p[j] = q[i] / quant[i];
+        mp = fmaxf(mp, fabsf(p[j]));
+    }
+    mp = wave_max(mp);
+    float sp = 127.0f / mp;
+#pragma unroll
+    for (int j = 0; j < PDIM / WAVE; j++) out.wq256[qi * PDIM + lane + j * WAVE] = sat_i8(roundf(p[j] * sp));
+    if (lane == 0) out.inv[qi] = make_float2(1.0f / s, 1.0f / sp);
+}
+
+void launch_query_prep(const float* q256, const float* q768, size_t nq, const float* signs, float rot_norm,
+                       const float* quant, const QueryPrep& out, hipStream_t st) {
+    if (nq == 0) return;
+    hipLaunchKernelGGL(k_query_prep, dim3((unsigned)nq), dim3(WAVE), 0, st, q256, q768, signs, rot_norm, quant, out);
+}
+
+// ------------------------------------------------------------------ re-layout --------
+// blocks.bin (byte g*16+k = doc k nibble g | doc k+16 nibble g << 4) -> 32 raw code bytes
+// per slot (bit i of dim i at byte i/8, src/quant.rs:22-32), so a lane reads its slot's code
+// with two 16-byte loads.
+__global__ __launch_bounds__(256) void k_transpose_blocks(const uint8_t* __restrict__ blocks, size_t nblocks,
+                                                          uint8_t* __restrict__ codes) {
+    __shared__ uint8_t blk[BLOCK_BYTES];
+    const size_t b = blockIdx.x;
+    const int t = threadIdx.x;
+    reinterpret_cast<uint32_t*>(blk)[t] = reinterpret_cast<const uint32_t*>(blocks + b * BLOCK_BYTES)[t];
+    __syncthreads();
+    // thread t -> (slot k = t / 8, bytes 4*(t%8) .. +3)
+    const int k = t >> 3, j0 = (t & 7) * 4;
+    uint32_t word = 0;
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+        int j = j0 + jj;  // code byte j holds groups 2j (low nibble) and 2j+1 (high nibble)
+        uint8_t lo, hi;
+        if (k < 16) {
+            lo = blk[(2 * j) * 16 + k] & 0x0F;
+            hi = blk[(2 * j + 1) * 16 + k] & 0x0F;
+        } else {
+            lo = blk[(2 * j) * 16 + k - 16] >> 4;
+            hi = blk[(2 * j + 1) * 16 + k - 16] >> 4;
+        }
+        word |= (uint32_t)(lo | (hi << 4)) << (8 * jj);
+    }
+    reinterpret_cast<uint32_t*>(codes + (b * BLOCK + k) * 32)[t & 7] = word;
+}
+
+void launch_transpose_blocks(const uint8_t* blocks, size_t nblocks, uint4* codes, hipStream_t st) {
+    if (nblocks == 0) return;
+    hipLaunchKernelGGL(k_transpose_blocks, dim3((unsigned)nblocks), dim3(256), 0, st, blocks, nblocks,
+                       reinterpret_cast<uint8_t*>(codes));
+}
+
+__global__ void k_split_factors(const float2* __restrict__ f, size_t n, float* __restrict__ mul,
+                                float* __restrict__ err) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        float2 x = f[i];
+        mul[i] = x.x;
+        err[i] = x.y;
+    }
+}
+
+void launch_split_factors(const float* factors, size_t nslots, float* mul, float* err, hipStream_t st) {
+    if (nslots == 0) return;
+    hipLaunchKernelGGL(k_split_factors, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, st,
+                       reinterpret_cast<const float2*>(factors), nslots, mul, err);
+}
+
+__global__ void k_to_bf16(const float* __restrict__ x, size_t n, uint16_t* __restrict__ y) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = f32_to_bf16_rne(x[i]);
+}
+
+void launch_to_bf16(const float* x, size_t n, uint16_t* y, hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, n, y);
+}
+
+__global__ void k_cluster_slots(const uint32_t* __restrict__ block_off, uint32_t k, uint32_t* __restrict__ slots) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < k) slots[c] = (block_off[c + 1] - block_off[c]) * BLOCK;
+}
+
+void launch_cluster_slots(const uint32_t* block_off, uint32_t k, uint32_t* slots, hipStream_t st) {
+    if (k == 0) return;
+    hipLaunchKernelGGL(k_cluster_slots, dim3((k + 255) / 256), dim3(256), 0, st, block_off, k, slots);
+}
+
+}  // namespace sieve

@@ -1,0 +1,279 @@
+// The scan: Index::candidates (src/ivf.rs:643-729) for a batch, one 256-thread workgroup per
+// query, plus the standalone lut::scan seam (src/lut.rs:121-152).
+//
+// Per query the workgroup walks the blocks of its probed clusters in probe order as one
+// sequence, 1024 slots (32 blocks) per round, one lane per slot:
+//   * score: the 64 nibbles of the slot's 32-byte code index the query's 64 16-entry u8
+//     tables, held in LDS (all lanes of a wave read the same 16-byte table row at a time, so
+//     the reads are broadcast and conflict-free); the u16 sum equals score_code
+//     (src/lut.rs:83-86) exactly;
+//   * estimate: cent + mul * (a * sc + b), f32, no FMA (rabitq::Query::estimate,
+//     src/rabitq.rs:127-130);
+//   * admission: rejected if est <= threshold, if the slot is padding, or if the view hides
+//     its row (src/ivf.rs:669); otherwise appended to an LDS buffer;
+//   * TopK (src/topk.rs:27-44): when the buffer holds 2*cand entries it is cut to the best
+//     cand by a radix select in LDS and the threshold rises to the cand-th estimate.
+// At the end the best cand slots are sorted by (row asc, est desc) and deduplicated by row
+// (src/ivf.rs:725-726). Ties at the cand-th estimate keep the lower slot index, so results
+// are deterministic; the reference keeps an arbitrary one (select_nth_unstable).
+//
+// Traffic: 32 code bytes + 4 mul bytes per stored slot, read once per (query, probed
+// cluster); ids and err only for admitted slots.
+#include "kernels.hpp"
+#include "select.hpp"
+
+namespace sieve {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SLOTS_PER_THREAD = 4;
+constexpr int ROUND_BLOCKS = SCAN_THREADS * SLOTS_PER_THREAD / BLOCK;  // 32
+constexpr int ROUND_SLOTS = ROUND_BLOCKS * BLOCK;                     // 1024
+
+struct ScanLds {
+    int cap;
+    size_t off_pref, off_start, off_cent, off_buf, off_tmp, off_hist, off_misc, total;
+};
+
+__host__ __device__ inline ScanLds scan_layout(int cand, int probe_stride) {
+    ScanLds l;
+    l.cap = 2 * cand + ROUND_SLOTS;
+    size_t o = 1024;  // tables
+    auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
+    l.off_pref = o;
+    o = al(o + 4 * (size_t)(probe_stride + 1));
+    l.off_start = o;
+    o = al(o + 4 * (size_t)probe_stride);
+    l.off_cent = o;
+    o = al(o + 4 * (size_t)probe_stride);
+    l.off_buf = o;
+    o = al(o + 8 * (size_t)l.cap);
+    l.off_tmp = o;
+    o = al(o + 8 * (size_t)cand);
+    l.off_hist = o;
+    o = al(o + 4 * 256);
+    l.off_misc = o;
+    o = al(o + 4 * 16);
+    l.total = o;
+    return l;
+}
+
+size_t scan_lds_bytes(int cand, int probe_stride) { return scan_layout(cand, probe_stride).total; }
+
+// Cuts buf[0, n) to its best k entries (under (est key, ~slot)) and raises the threshold.
+__device__ void scan_compact(uint2* buf, uint2* tmp, int n, int k, uint32_t* hist, uint32_t* misc) {
+    const int t = threadIdx.x;
+    uint2 kth = block_kth_largest(buf, n, k, hist, misc + 8);
+    if (t == 0) misc[3] = 0;
+    __syncthreads();
+    for (int i = t; i < n; i += SCAN_THREADS) {
+        uint2 e = buf[i];
+        if (ge2(e, kth)) tmp[atomicAdd(&misc[3], 1u)] = e;
+    }
+    __syncthreads();
+    for (int i = t; i < k; i += SCAN_THREADS) buf[i] = tmp[i];
+    if (t == 0) {
+        misc[0] = (uint32_t)k;  // count
+        misc[1] = 1;            // has threshold
+        misc[2] = kth.x;        // threshold key = the k-th estimate
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const ScanLds L = scan_layout(a.cand, a.probe_stride);
+    uint8_t* tab = lds;
+    uint32_t* pref = reinterpret_cast<uint32_t*>(lds + L.off_pref);
+    uint32_t* pstart = reinterpret_cast<uint32_t*>(lds + L.off_start);
+    float* pcent = reinterpret_cast<float*>(lds + L.off_cent);
+    uint2* buf = reinterpret_cast<uint2*>(lds + L.off_buf);
+    uint2* tmp = reinterpret_cast<uint2*>(lds + L.off_tmp);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(lds + L.off_hist);
+    uint32_t* misc = reinterpret_cast<uint32_t*>(lds + L.off_misc);  // 0 count, 1 has_thr, 2 thr, 3 tmp, 8.. select
+
+    const size_t qi = blockIdx.x;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    reinterpret_cast<uint32_t*>(tab)[t] = reinterpret_cast<const uint32_t*>(a.tables + qi * 1024)[t];
+    const float4 qab = a.qab[qi];
+    const int np = (int)a.probe_n[qi];
+    for (int j = t; j < np; j += SCAN_THREADS) {
+        uint32_t c = a.probe_c[qi * a.probe_stride + j];
+        uint32_t b0 = a.block_off[c];
+        pstart[j] = b0;
+        pref[j] = a.block_off[c + 1] - b0;
+        pcent[j] = a.probe_s[qi * a.probe_stride + j];
+    }
+    if (t < 4) misc[t] = 0;
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t run = 0;
+        for (int base = 0; base < np; base += 64) {
+            int j = base + lane;
+            uint32_t v = j < np ? pref[j] : 0u;
+            uint32_t incl = wave_incl_scan(v);
+            if (j < np) pref[j] = run + incl - v;
+            run += __shfl(incl, 63, 64);
+        }
+        if (lane == 0) pref[np] = run;
+    }
+    __syncthreads();
+    const uint32_t total = pref[np];
+    if (t == 0 && a.scanned) atomicAdd(a.scanned, (unsigned long long)total * BLOCK);
+
+    const uint64_t* allow = nullptr;
+    if (a.view.allow && (a.view.allow_stride == 0 || !a.view.allow_flags || a.view.allow_flags[qi]))
+        allow = a.view.allow + qi * a.view.allow_stride;
+    const int cand = a.cand;
+
+    for (uint32_t vb0 = 0; vb0 < total; vb0 += ROUND_BLOCKS) {
+        uint4 c0[SLOTS_PER_THREAD], c1[SLOTS_PER_THREAD];
+        float mul[SLOTS_PER_THREAD], cent[SLOTS_PER_THREAD];
+        uint32_t gs[SLOTS_PER_THREAD];
+        bool ok[SLOTS_PER_THREAD];
+#pragma unroll
+        for (int u = 0; u < SLOTS_PER_THREAD; u++) {
+            uint32_t vb = vb0 + (t >> 5) + 8 * u;
+            ok[u] = vb < total;
+            gs[u] = 0;
+            cent[u] = 0.0f;
+            if (ok[u]) {
+                int lo = 0, hi = np - 1;  // last j with pref[j] <= vb
+                while (lo < hi) {
+                    int mid = (lo + hi + 1) >> 1;
+                    if (pref[mid] <= vb) lo = mid;
+                    else hi = mid - 1;
+                }
+                gs[u] = (pstart[lo] + (vb - pref[lo])) * BLOCK + (t & 31);
+                cent[u] = pcent[lo];
+                const uint4* cp = a.codes + 2 * (size_t)gs[u];
+                c0[u] = cp[0];
+                c1[u] = cp[1];
+                mul[u] = a.mul[gs[u]];
+            }
+        }
+        const bool has_thr = misc[1] != 0;
+        const uint32_t thr = misc[2];
+#pragma unroll
+        for (int u = 0; u < SLOTS_PER_THREAD; u++) {
+            bool pass = false;
+            uint32_t key = 0;
+            if (ok[u]) {
+                const uint32_t w[8] = {c0[u].x, c0[u].y, c0[u].z, c0[u].w, c1[u].x, c1[u].y, c1[u].z, c1[u].w};
+                uint32_t sc = 0;
+#pragma unroll
+                for (int d = 0; d < 8; d++)
+#pragma unroll
+                    for (int j = 0; j < 8; j++) sc += tab[(8 * d + j) * 16 + ((w[d] >> (4 * j)) & 15u)];
+                float est = cent[u] + mul[u] * (qab.x * (float)sc + qab.y);
+                est = est + 0.0f;  // -0.0 -> +0.0 so keys order like the floats compare
+                key = f2key(est);
+                pass = !has_thr || key > thr;
+                if (pass) {
+                    uint32_t id = a.ids[gs[u]];
+                    pass = id != PAD;
+                    if (pass && a.view.deleted && bit_set(a.view.deleted, a.view.deleted_words, id)) pass = false;
+                    if (pass && allow && !bit_set(allow, a.view.allow_words, id)) pass = false;
+                }
+            }
+            uint64_t bal = __ballot(pass);
+            if (bal) {
+                uint32_t nw = (uint32_t)__popcll(bal);
+                uint32_t before = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&misc[0], nw);
+                base = __shfl(base, 0, 64);
+                if (pass) buf[base + before] = make_uint2(key, ~gs[u]);
+            }
+        }
+        __syncthreads();
+        const int count = (int)misc[0];
+        __syncthreads();  // every wave has read the count before the next round appends
+        if (count >= 2 * cand) scan_compact(buf, tmp, count, cand, hist, misc);
+    }
+    if ((int)misc[0] > cand) scan_compact(buf, tmp, (int)misc[0], cand, hist, misc);
+
+    // Sort the kept slots by (row asc, est desc) and keep the first of each row.
+    const int n = (int)misc[0];
+    const int n2 = pow2_at_least(max(n, 1));
+    uint32_t* pay = reinterpret_cast<uint32_t*>(tmp);
+    for (int i = t; i < n2; i += SCAN_THREADS) {
+        if (i < n) {
+            uint2 e = buf[i];
+            uint32_t slot = ~e.y;
+            buf[i] = make_uint2(a.ids[slot], ~e.x);
+            pay[i] = slot;
+        } else {
+            buf[i] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+            pay[i] = 0;
+        }
+    }
+    __syncthreads();
+    block_sort_pairs(buf, pay, n2, true);
+    // Each thread takes a contiguous run of the sorted entries.
+    const int per = (n2 + SCAN_THREADS - 1) / SCAN_THREADS;
+    const int i0 = t * per;
+    uint32_t mine = 0;
+    for (int i = i0; i < i0 + per && i < n; i++)
+        if (i == 0 || buf[i].x != buf[i - 1].x) mine++;
+    uint32_t tot;
+    uint32_t off = block_excl_scan(mine, misc + 4, &tot);
+    float* oe = a.out_est + qi * a.out_stride;
+    uint32_t* orow = a.out_row + qi * a.out_stride;
+    float* oerr = a.out_err + qi * a.out_stride;
+    for (int i = i0; i < i0 + per && i < n; i++) {
+        if (i == 0 || buf[i].x != buf[i - 1].x) {
+            oe[off] = key2f(~buf[i].y);
+            orow[off] = buf[i].x;
+            oerr[off] = a.err[pay[i]];
+            off++;
+        }
+    }
+    if (t == 0) a.out_n[qi] = tot;
+}
+
+void launch_scan(const ScanArgs& a, size_t nq, hipStream_t st) {
+    if (nq == 0) return;
+    const size_t lds = scan_lds_bytes(a.cand, a.probe_stride);
+    if (lds > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_scan, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_scan, dim3((unsigned)nq), dim3(SCAN_THREADS), lds, st, a);
+}
+
+// ------------------------------------------------------------------ lut::scan seam -----
+// Over the reference's nibble-transposed blocks: one 256-thread workgroup scores 8 blocks;
+// the blocks are staged in LDS with a 16-byte pad per block so the two half-waves of
+// different blocks fall in different banks.
+__global__ __launch_bounds__(256) void k_lut_scan(const uint8_t* __restrict__ tables, int G,
+                                                  const uint8_t* __restrict__ blocks, size_t nblocks,
+                                                  uint16_t* __restrict__ out) {
+    __shared__ uint8_t tab[64 * 16];
+    __shared__ __align__(16) uint8_t blk[8][1024 + 16];
+    const int t = threadIdx.x;
+    const size_t b0 = (size_t)blockIdx.x * 8;
+    const int bb = G * 16;
+    for (int i = t; i < bb; i += 256) tab[i] = tables[i];
+    const int nb = (int)min((size_t)8, nblocks - b0);
+    for (int i = t; i < nb * bb / 16; i += 256) {
+        int b = i / (bb / 16), o = i % (bb / 16);
+        *reinterpret_cast<uint4*>(&blk[b][o * 16]) = reinterpret_cast<const uint4*>(blocks + (b0 + b) * bb)[o];
+    }
+    __syncthreads();
+    const int b = t >> 5, k = t & 31;
+    if (b < nb) {
+        uint32_t sc = 0;
+        for (int g = 0; g < G; g++) {
+            uint8_t x = blk[b][g * 16 + (k & 15)];
+            uint8_t nib = k < 16 ? (x & 15) : (x >> 4);
+            sc += tab[g * 16 + nib];
+        }
+        out[(b0 + b) * BLOCK + k] = (uint16_t)sc;
+    }
+}
+
+void launch_lut_scan(const uint8_t* tables, int groups, const uint8_t* blocks, size_t nblocks, uint16_t* out,
+                     hipStream_t st) {
+    if (nblocks == 0) return;
+    hipLaunchKernelGGL(k_lut_scan, dim3((unsigned)((nblocks + 7) / 8)), dim3(256), 0, st, tables, groups, blocks,
+                       nblocks, out);
+}
+
+}  // namespace sieve

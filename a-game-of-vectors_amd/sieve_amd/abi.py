@@ -1,0 +1,127 @@
+"""ctypes binding of libsieve_gpu.so (include/sieve_gpu.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback anywhere in the
+package. Array arguments may be numpy arrays (host memory; the library copies them) or torch
+tensors on the GPU (device memory, used in place).
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsieve_gpu.so")
+HEADER = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "sieve_gpu.h")
+
+PAD = 0xFFFFFFFF
+RERANK_TAGS = {"none": 0, "prefix": 1, "full": 2, "tiered": 3, "ssd": 4}
+STATUS = {0: "ok", -1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EIO", -5: "ELIMIT"}
+
+
+class SieveError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"sieve {STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("cand", ctypes.c_uint32), ("rerank", ctypes.c_uint32), ("rerank_r", ctypes.c_uint32),
+                ("k", ctypes.c_uint32), ("prune_m", ctypes.c_uint32), ("prune_w", ctypes.c_float),
+                ("nprobe", ctypes.c_uint32), ("margin", ctypes.c_float), ("budget", ctypes.c_uint64)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in
+                ("route", "scan", "rerank", "tier2", "scanned", "reranked", "probed", "fetch", "wait")]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class IndexDesc(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_uint32), ("n", ctypes.c_uint64), ("nblocks", ctypes.c_uint64),
+                ("centroids", ctypes.c_void_p), ("block_off", ctypes.c_void_p), ("blocks", ctypes.c_void_p),
+                ("ids", ctypes.c_void_p), ("factors", ctypes.c_void_p), ("rotation", ctypes.c_void_p),
+                ("rerank", ctypes.c_void_p), ("quant", ctypes.c_void_p), ("norm256", ctypes.c_void_p),
+                ("tier2_sigma", ctypes.c_void_p), ("tier2_codes", ctypes.c_void_p),
+                ("tier2_norm", ctypes.c_void_p), ("borrow_rerank", ctypes.c_uint32)]
+
+
+class View(ctypes.Structure):
+    _fields_ = [("deleted", ctypes.c_void_p), ("deleted_words", ctypes.c_uint64), ("allow", ctypes.c_void_p),
+                ("allow_words", ctypes.c_uint64), ("allow_stride", ctypes.c_uint64),
+                ("allow_flags", ctypes.c_void_p)]
+
+
+class IndexInfo(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_uint32), ("n", ctypes.c_uint64), ("nblocks", ctypes.c_uint64),
+                ("rabitq", ctypes.c_uint32), ("tier2", ctypes.c_uint32), ("max_cluster_blocks", ctypes.c_uint32),
+                ("device_bytes", ctypes.c_uint64)]
+
+
+_lib = None
+
+
+def header_symbols():
+    """Every function the public header declares."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\**(sieve_\w+)\s*\(", text, re.M)))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libsieve_gpu.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, sz, i32, u32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint32
+    P, T, V = ctypes.POINTER(Params), ctypes.POINTER(Timing), ctypes.POINTER(View)
+    sig = {
+        "sieve_last_error": (ctypes.c_char_p, []),
+        "sieve_stream_create": (i32, [ctypes.POINTER(vp)]),
+        "sieve_stream_destroy": (None, [vp]),
+        "sieve_stream_sync": (i32, [vp]),
+        "sieve_stream_handle": (vp, [vp]),
+        "sieve_gemm_abt": (i32, [vp, vp, sz, sz, sz, vp]),
+        "sieve_lut_scan": (i32, [vp, u32, vp, sz, vp]),
+        "sieve_index_open": (i32, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]),
+        "sieve_index_create": (i32, [ctypes.POINTER(IndexDesc), ctypes.POINTER(vp)]),
+        "sieve_index_free": (None, [vp]),
+        "sieve_index_get_info": (i32, [vp, ctypes.POINTER(IndexInfo)]),
+        "sieve_route": (i32, [vp, vp, vp, sz, P, vp, vp, vp, T]),
+        "sieve_candidates": (i32, [vp, vp, vp, sz, vp, vp, vp, sz, P, V, vp, vp, vp, vp, T]),
+        "sieve_rerank": (i32, [vp, vp, vp, vp, sz, vp, vp, vp, vp, sz, P, V, vp, vp, vp, T]),
+        "sieve_search_batch": (i32, [vp, vp, vp, vp, sz, P, V, vp, vp, vp, T]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise SieveError(rc, lib().sieve_last_error().decode())
+
+
+def make_params(p):
+    """A sieve_params from a sieve_amd.Params."""
+    tag, r = p.rerank
+    prune = p.prune if p.prune is not None else (0, 0.0)
+    return Params(cand=p.cand, rerank=RERANK_TAGS[tag], rerank_r=r, k=p.k, prune_m=prune[0], prune_w=prune[1],
+                  nprobe=p.nprobe, margin=float("nan") if p.margin is None else p.margin, budget=p.budget or 0)
+
+
+def ptr(a):
+    """Address of a numpy array or a torch tensor (contiguous)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"], "array must be contiguous"
+        return ctypes.c_void_p(a.ctypes.data)
+    assert a.is_contiguous(), "tensor must be contiguous"
+    return ctypes.c_void_p(a.data_ptr())

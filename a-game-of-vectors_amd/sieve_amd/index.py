@@ -1,0 +1,216 @@
+"""The device index and its query path, mirroring the reference's Index API (src/ivf.rs:552-815):
+open / route / candidates / rerank / search, here over batches of queries.
+
+Inputs may be numpy arrays (results come back as numpy) or torch tensors on the GPU (results
+are torch tensors on the same device, and nothing crosses PCIe).
+"""
+import ctypes
+
+import numpy as np
+
+from . import abi
+from .params import Params
+
+_HOST_DTYPES = dict(centroids=np.float32, block_off=np.uint64, blocks=np.uint8, ids=np.uint32, factors=np.float32,
+                    rotation=np.float32, rerank=np.int8, quant=np.float32, norm256=np.float32,
+                    tier2_sigma=np.float32, tier2_codes=np.uint8, tier2_norm=np.float32)
+
+
+def _is_torch(x):
+    return hasattr(x, "data_ptr") and hasattr(x, "is_cuda")
+
+
+class Stream:
+    """A sieve_stream: one HIP stream plus scratch, for one host thread."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        abi.check(abi.lib().sieve_stream_create(ctypes.byref(h)))
+        self._h = h
+
+    def sync(self):
+        abi.check(abi.lib().sieve_stream_sync(self._h))
+
+    @property
+    def handle(self):
+        return abi.lib().sieve_stream_handle(self._h)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                abi.lib().sieve_stream_destroy(self._h)
+        except Exception:
+            pass
+        self._h = None
+
+
+class Index:
+    """A RaBitQ/SOAR IVF index resident in HBM (sieve_index)."""
+
+    def __init__(self, handle, keep=None):
+        self._h = handle
+        self._keep = keep  # arrays the index borrows
+        info = abi.IndexInfo()
+        abi.check(abi.lib().sieve_index_get_info(self._h, ctypes.byref(info)))
+        self.k, self.n, self.nblocks = info.k, info.n, info.nblocks
+        self.tier2 = bool(info.tier2)
+        self.device_bytes = info.device_bytes
+        self.max_cluster_blocks = info.max_cluster_blocks
+        self._stream = None
+
+    @classmethod
+    def open(cls, index_dir, ingest_dir):
+        """Index::open + Corpus::open over the reference's files."""
+        h = ctypes.c_void_p()
+        abi.check(abi.lib().sieve_index_open(index_dir.encode(), ingest_dir.encode(), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def create(cls, arrays, borrow_rerank=False):
+        """From a dict of arrays in the on-disk layout (numpy, or torch tensors on the GPU)."""
+        keep = {}
+        d = abi.IndexDesc()
+        for name, dt in _HOST_DTYPES.items():
+            a = arrays.get(name)
+            if a is None:
+                continue
+            if not _is_torch(a):
+                a = np.ascontiguousarray(a, dtype=dt)
+            else:
+                a = a.contiguous()
+            keep[name] = a
+            setattr(d, name, abi.ptr(a).value)
+        d.k = _numel(keep["centroids"]) // 256
+        d.n = _numel(keep["rerank"]) // 768
+        d.nblocks = _numel(keep["ids"]) // 32
+        d.borrow_rerank = 1 if borrow_rerank else 0
+        h = ctypes.c_void_p()
+        abi.check(abi.lib().sieve_index_create(ctypes.byref(d), ctypes.byref(h)))
+        return cls(h, keep if borrow_rerank else None)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                abi.lib().sieve_index_free(self._h)
+        except Exception:
+            pass
+        self._h = None
+
+    def stream(self):
+        if self._stream is None:
+            self._stream = Stream()
+        return self._stream
+
+    # ------------------------------------------------------------------ helpers -----
+    @staticmethod
+    def _out(like, shape, dtype):
+        if _is_torch(like):
+            import torch
+            tdt = {np.float32: torch.float32, np.uint32: torch.int32, np.int8: torch.int8}[dtype]
+            return torch.empty(shape, dtype=tdt, device=like.device)
+        return np.empty(shape, dtype)
+
+    @staticmethod
+    def _in(x, dtype):
+        if x is None or _is_torch(x):
+            return x
+        return np.ascontiguousarray(x, dtype=dtype)
+
+    # ------------------------------------------------------------------ path --------
+    def route(self, q256, p: Params, timing=None, stream=None):
+        """Index::route for a batch: (scores nq x nprobe, clusters, counts)."""
+        q256 = self._in(q256, np.float32)
+        nq = q256.shape[0]
+        s = self._out(q256, (nq, p.nprobe), np.float32)
+        c = self._out(q256, (nq, p.nprobe), np.uint32)
+        n = self._out(q256, (nq,), np.uint32)
+        t = timing if timing is not None else abi.Timing()
+        st = stream or self.stream()
+        abi.check(abi.lib().sieve_route(self._h, st._h, abi.ptr(q256), nq, ctypes.byref(abi.make_params(p)),
+                                        abi.ptr(s), abi.ptr(c), abi.ptr(n), ctypes.byref(t)))
+        return s, c, n
+
+    def candidates(self, q256, probe_s, probe_c, probe_n, p: Params, view=None, timing=None, stream=None):
+        """Index::candidates for a batch: (est, rows, errs) nq x cand and counts."""
+        q256 = self._in(q256, np.float32)
+        probe_s, probe_c, probe_n = self._in(probe_s, np.float32), self._in(probe_c, np.uint32), \
+            self._in(probe_n, np.uint32)
+        nq = q256.shape[0]
+        est = self._out(q256, (nq, p.cand), np.float32)
+        rows = self._out(q256, (nq, p.cand), np.uint32)
+        errs = self._out(q256, (nq, p.cand), np.float32)
+        n = self._out(q256, (nq,), np.uint32)
+        t = timing if timing is not None else abi.Timing()
+        st = stream or self.stream()
+        v = view.c if view is not None else None
+        abi.check(abi.lib().sieve_candidates(self._h, st._h, abi.ptr(q256), nq, abi.ptr(probe_s), abi.ptr(probe_c),
+                                             abi.ptr(probe_n), probe_s.shape[1], ctypes.byref(abi.make_params(p)),
+                                             ctypes.byref(v) if v is not None else None, abi.ptr(est),
+                                             abi.ptr(rows), abi.ptr(errs), abi.ptr(n), ctypes.byref(t)))
+        return est, rows, errs, n
+
+    def rerank(self, q768, q256, est, rows, errs, counts, p: Params, timing=None, stream=None):
+        """Index::rerank for a batch: (scores nq x k, rows, counts)."""
+        q768, q256 = self._in(q768, np.float32), self._in(q256, np.float32)
+        est, rows, errs, counts = (self._in(est, np.float32), self._in(rows, np.uint32), self._in(errs, np.float32),
+                                   self._in(counts, np.uint32))
+        nq = q256.shape[0]
+        s = self._out(q256, (nq, p.k), np.float32)
+        r = self._out(q256, (nq, p.k), np.uint32)
+        n = self._out(q256, (nq,), np.uint32)
+        t = timing if timing is not None else abi.Timing()
+        st = stream or self.stream()
+        abi.check(abi.lib().sieve_rerank(self._h, st._h, abi.ptr(q768), abi.ptr(q256), nq, abi.ptr(est),
+                                         abi.ptr(rows), abi.ptr(errs), abi.ptr(counts), est.shape[1],
+                                         ctypes.byref(abi.make_params(p)), None, abi.ptr(s), abi.ptr(r), abi.ptr(n),
+                                         ctypes.byref(t)))
+        return s, r, n
+
+    def search(self, q768, q256, p: Params, view=None, timing=None, stream=None):
+        """Index::search / search_view for a batch: (scores nq x k, rows, counts)."""
+        q768, q256 = self._in(q768, np.float32), self._in(q256, np.float32)
+        nq = q256.shape[0]
+        s = self._out(q256, (nq, p.k), np.float32)
+        r = self._out(q256, (nq, p.k), np.uint32)
+        n = self._out(q256, (nq,), np.uint32)
+        t = timing if timing is not None else abi.Timing()
+        st = stream or self.stream()
+        v = view.c if view is not None else None
+        abi.check(abi.lib().sieve_search_batch(self._h, st._h, abi.ptr(q768), abi.ptr(q256), nq,
+                                               ctypes.byref(abi.make_params(p)),
+                                               ctypes.byref(v) if v is not None else None, abi.ptr(s), abi.ptr(r),
+                                               abi.ptr(n), ctypes.byref(t)))
+        return s, r, n
+
+
+class ViewSpec:
+    """ivf::View (src/ivf.rs:462-496): tombstones and allowlists as u64 bitsets over rows."""
+
+    def __init__(self, deleted=None, allow=None, allow_flags=None):
+        self._keep = [deleted, allow, allow_flags]
+        v = abi.View()
+        if deleted is not None:
+            v.deleted = abi.ptr(deleted).value
+            v.deleted_words = deleted.shape[0]
+        if allow is not None:
+            v.allow = abi.ptr(allow).value
+            if allow.ndim == 2:
+                v.allow_words = allow.shape[1]
+                v.allow_stride = allow.shape[1]
+            else:
+                v.allow_words = allow.shape[0]
+            if allow_flags is not None:
+                v.allow_flags = abi.ptr(allow_flags).value
+        self.c = v
+
+
+def bitset(rows, n):
+    """A u64 bitset over n rows with the given rows set (src/live.rs:123-129)."""
+    words = np.zeros((n + 63) // 64, np.uint64)
+    rows = np.asarray(rows, np.uint64)
+    np.bitwise_or.at(words, (rows >> np.uint64(6)).astype(np.int64), np.uint64(1) << (rows & np.uint64(63)))
+    return words
+
+
+def _numel(a):
+    return int(a.numel()) if _is_torch(a) else int(a.size)

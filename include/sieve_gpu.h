@@ -1,0 +1,230 @@
+/*
+ * sieve_gpu.h — C ABI of the MI355X search path for sieve (IVF + SOAR + RaBitQ 1-bit LUT
+ * scan + int8 rerank).
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, returns 0 on success
+ * and a negative status on failure (the message is in sieve_last_error()), and never lets
+ * a C++ exception cross the boundary. Array arguments may be host memory or device memory
+ * (HIP allocations, e.g. a torch tensor on cuda:0); the library detects which with
+ * hipPointerGetAttributes and copies host arrays in and out itself. The caller owns every
+ * array it passes and the library never keeps a pointer to one after the call returns,
+ * except where sieve_index_desc says "borrowed".
+ *
+ * Threading mirrors the reference's `Index: &self` / per-thread `Scratch` split
+ * (src/ivf.rs:498-519, src/bin/search.rs:279-295): a sieve_index is immutable after
+ * creation and may be shared by any number of host threads; each host thread owns one
+ * sieve_stream (a HIP stream plus its scratch buffers), and calls on one stream are
+ * serialised by the caller.
+ *
+ * Reference citations are into the sieve crate (18Abhinav07/A-Game-Of-Vectors).
+ */
+#ifndef SIEVE_GPU_H
+#define SIEVE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Compile-time widths (src/lib.rs:21-26, src/lut.rs:42-44, src/ivf.rs:18). */
+#define SIEVE_DIM 768
+#define SIEVE_PDIM 256
+#define SIEVE_CODE_BYTES 32
+#define SIEVE_BLOCK 32
+#define SIEVE_PAD 0xFFFFFFFFu
+
+/* Status codes. */
+#define SIEVE_OK 0
+#define SIEVE_EINVAL (-1)   /* bad argument, shape or file contents            */
+#define SIEVE_EHIP (-2)     /* a HIP runtime call failed                      */
+#define SIEVE_ENOMEM (-3)   /* device allocation failed / over the HBM budget */
+#define SIEVE_EIO (-4)      /* index file missing or unreadable               */
+#define SIEVE_ELIMIT (-5)   /* a size beyond a documented kernel limit        */
+
+/* Rerank modes: the tags of shard::Header (src/shard.rs:51-52) for Rerank
+ * (src/ivf.rs:426-439). */
+#define SIEVE_RERANK_NONE 0
+#define SIEVE_RERANK_PREFIX 1
+#define SIEVE_RERANK_FULL 2
+#define SIEVE_RERANK_TIERED 3
+#define SIEVE_RERANK_SSD 4
+
+/* Query parameters: ivf::Params (src/ivf.rs:442-460), laid out like shard::Header
+ * (src/shard.rs:48-62) plus the two routing options the header leaves out.
+ *   prune_m == 0        -> prune = None, else Some((prune_m, prune_w))
+ *   margin is NaN       -> margin = None
+ *   budget == 0         -> budget = None                                     */
+typedef struct sieve_params {
+    uint32_t cand;
+    uint32_t rerank;
+    uint32_t rerank_r;
+    uint32_t k;
+    uint32_t prune_m;
+    float prune_w;
+    uint32_t nprobe;
+    float margin;
+    uint64_t budget;
+} sieve_params;
+
+/* Stage counters: ivf::Timing (src/ivf.rs:522-538), same field order. Times are
+ * nanoseconds of GPU time measured with HIP events on the call's stream; counts are
+ * summed over the queries of a call. */
+typedef struct sieve_timing {
+    uint64_t route;
+    uint64_t scan;
+    uint64_t rerank;
+    uint64_t tier2;
+    uint64_t scanned;
+    uint64_t reranked;
+    uint64_t probed;
+    uint64_t fetch;
+    uint64_t wait;
+} sieve_timing;
+
+/* An index in the reference's on-disk layout, as arrays. The IVF build output
+ * (src/ivf.rs:354-391) and the ingest output (src/bin/ingest.rs:99-166):
+ *   centroids  k x 256 f32           centroids.f32
+ *   block_off  k + 1 u64             block_off.u64
+ *   blocks     nblocks x 1024 B      blocks.bin   (nibble-transposed, src/lut.rs:98-108)
+ *   ids        nblocks x 32 u32      ids.u32      (PAD = 0xFFFFFFFF)
+ *   factors    nblocks x 32 x 2 f32  factors.f32  ([mul, err] per slot; NULL = global
+ *                                                  sign codes, no RaBitQ)
+ *   rotation   3 x 256 f32           rotation.f32 (NULL without RaBitQ)
+ *   rerank     n x 768 i8            rerank.i8
+ *   quant      768 f32               quant.f32    (per-dimension int8 scales)
+ *   norm256    n f32                 norm256.f32
+ *   tier2_*    optional 2-bit tier   tier2.sigma (256 f32), tier2.bin (n x 64 B),
+ *                                    tier2.norm (n f32)
+ * All arrays are copied into the library's device layout at creation except `rerank`,
+ * which is borrowed (used in place, no copy) when it is device memory and
+ * borrow_rerank is nonzero: the caller then keeps it alive until sieve_index_free. */
+typedef struct sieve_index_desc {
+    uint32_t k;
+    uint64_t n;
+    uint64_t nblocks;
+    const float* centroids;
+    const uint64_t* block_off;
+    const uint8_t* blocks;
+    const uint32_t* ids;
+    const float* factors;
+    const float* rotation;
+    const int8_t* rerank;
+    const float* quant;
+    const float* norm256;
+    const float* tier2_sigma;
+    const uint8_t* tier2_codes;
+    const float* tier2_norm;
+    uint32_t borrow_rerank;
+} sieve_index_desc;
+
+/* What a query sees beyond the immutable base: ivf::View (src/ivf.rs:462-496).
+ * deleted: bitset over row ids, a set bit hides the row (tombstones, src/live.rs:157-158).
+ * allow:   bitset over row ids; when present only rows with a set bit are returned
+ *          (filters, src/filter.rs:18-39). allow_stride == 0 means one bitset for every
+ *          query of the call; otherwise query i uses allow + i * allow_stride words and
+ *          allow_flags[i] (nonzero = filtered) says whether query i carries a filter.
+ * Word counts bound the bitsets: a row past the end reads as clear
+ * (src/ivf.rs:474-477). */
+typedef struct sieve_view {
+    const uint64_t* deleted;
+    uint64_t deleted_words;
+    const uint64_t* allow;
+    uint64_t allow_words;
+    uint64_t allow_stride;
+    const uint8_t* allow_flags;
+} sieve_view;
+
+typedef struct sieve_index sieve_index;
+typedef struct sieve_stream sieve_stream;
+
+typedef struct sieve_index_info {
+    uint32_t k;
+    uint64_t n;
+    uint64_t nblocks;
+    uint32_t rabitq;
+    uint32_t tier2;
+    uint32_t max_cluster_blocks;
+    uint64_t device_bytes;
+} sieve_index_info;
+
+/* ---------------------------------------------------------------- runtime --- */
+
+/* Message of the last failing call on this host thread ("" if none). */
+const char* sieve_last_error(void);
+
+/* A HIP stream plus per-stream scratch on the current device (one per host thread). */
+int sieve_stream_create(sieve_stream** out);
+void sieve_stream_destroy(sieve_stream* s);
+int sieve_stream_sync(sieve_stream* s);
+/* The underlying hipStream_t, for callers that order their own work with it. */
+void* sieve_stream_handle(sieve_stream* s);
+
+/* ---------------------------------------------------------- kernel seams --- */
+
+/* C = A * B^T, row-major f32; a is m x k, b is n x k, c is m x n.
+ * Replaces blas::gemm_abt (src/blas.rs:5-10), the crate's only FFI on the path. Computed
+ * with f32 MFMA (v_mfma_f32_16x16x4_f32) and f32 accumulation. */
+int sieve_gemm_abt(const float* a, const float* b, size_t m, size_t n, size_t k, float* c);
+
+/* Scores every 32-slot block of nibble-transposed codes against G 16-entry u8 tables:
+ * out[32*b + j] = sum_g tables[g][nibble_g(slot j of block b)].
+ * Replaces lut::scan (src/lut.rs:121-128); bit-exact with score_code (src/lut.rs:83-86).
+ * groups is 64 (256-bit codes) or 32 (128-bit codes, src/bigann.rs:53-56). */
+int sieve_lut_scan(const uint8_t* tables, uint32_t groups, const uint8_t* blocks,
+                   size_t nblocks, uint16_t* out);
+
+/* ------------------------------------------------------------ the index --- */
+
+/* Index::open (src/ivf.rs:553-578) + Corpus::open (src/ivf.rs:75-90): reads the build
+ * directory and the ingest directory and uploads them to the current device. */
+int sieve_index_open(const char* index_dir, const char* ingest_dir, sieve_index** out);
+/* The same from arrays already in memory (host or device). */
+int sieve_index_create(const sieve_index_desc* desc, sieve_index** out);
+void sieve_index_free(sieve_index* ix);
+int sieve_index_get_info(const sieve_index* ix, sieve_index_info* out);
+
+/* ------------------------------------------------------------- the path --- */
+
+/* Index::route (src/ivf.rs:587-613) for nq queries. q256 is nq x 256 (unit rows).
+ * Writes probe lists best first: scores[i*nprobe + j], clusters[i*nprobe + j] for
+ * j < counts[i] (counts[i] <= min(nprobe, k)). Routing runs as a BF16 MFMA GEMM with f32
+ * accumulation; every centroid within the BF16 error bound of the nprobe-th score is
+ * rescored in f32 exactly as the reference's gemm_abt fallback sums (src/blas.rs:47-53),
+ * so scores and the probe set are the reference's. */
+int sieve_route(const sieve_index* ix, sieve_stream* s, const float* q256, size_t nq,
+                const sieve_params* p, float* scores, uint32_t* clusters, uint32_t* counts,
+                sieve_timing* t);
+
+/* Index::candidates (src/ivf.rs:643-729) for nq queries over given probe lists (as
+ * sieve_route writes them, stride nprobe_stride). Writes, per query, up to p->cand
+ * (estimate, row, error half-width) triples deduplicated by row, in row order:
+ * est/rows/errs[i*p->cand + j] for j < counts[i]. */
+int sieve_candidates(const sieve_index* ix, sieve_stream* s, const float* q256, size_t nq,
+                     const float* probe_scores, const uint32_t* probe_clusters,
+                     const uint32_t* probe_counts, size_t nprobe_stride,
+                     const sieve_params* p, const sieve_view* v, float* est, uint32_t* rows,
+                     float* errs, uint32_t* counts, sieve_timing* t);
+
+/* Index::rerank (src/ivf.rs:731-815) for nq queries over candidate lists (as
+ * sieve_candidates writes them, stride cand_stride). Writes the top p->k (score, row)
+ * best first, ties by row ascending: scores/rows[i*p->k + j] for j < counts[i]. */
+int sieve_rerank(const sieve_index* ix, sieve_stream* s, const float* q768, const float* q256,
+                 size_t nq, const float* est, const uint32_t* rows, const float* errs,
+                 const uint32_t* cand_counts, size_t cand_stride, const sieve_params* p,
+                 const sieve_view* v, float* scores, uint32_t* out_rows, uint32_t* counts,
+                 sieve_timing* t);
+
+/* Index::search / search_view (src/ivf.rs:620-633) for a batch: route, scan and rerank
+ * of nq queries in one call, all stages on the device. Output as sieve_rerank. */
+int sieve_search_batch(const sieve_index* ix, sieve_stream* s, const float* q768,
+                       const float* q256, size_t nq, const sieve_params* p,
+                       const sieve_view* v, float* scores, uint32_t* rows, uint32_t* counts,
+                       sieve_timing* t);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIEVE_GPU_H */

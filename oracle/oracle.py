@@ -1,0 +1,508 @@
+"""CPU oracle for the sieve search path — TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module, and
+only as the checker (or, in bench.py, as the timed CPU baseline). It wraps
+oracle/libsieve_oracle.so, a C++ restatement of the reference crate's x86 (non-NEON) path
+(sieve_oracle.cpp; every function there cites the Rust it follows).
+
+The reference is a Rust crate and no Rust toolchain exists in this image, so the reference
+itself cannot be built or run (there is no oracle/_ref). The reference ships no golden vector
+files; its own unit tests hold the known answers and invariants the oracle is pinned to.
+
+Parity status of each oracle function and what pins it (tests/test_oracle_*.py):
+
+  gemm_abt        (blas.rs:43-54)    pinned: blas.rs:58-65 known answer; float64 bound;
+                                      bit-exact against a sequential float32 numpy sum
+  Rng             (ivf.rs:120-135)   pinned: published splitmix64 output for seed 0/1234567
+  lut_new         (lut.rs:58-81)     pinned: lut.rs:223-234 float bound; table invariants
+                                      (min entry 0 per group, 255 in the widest group)
+  score_code/scan (lut.rs:84-152)    pinned: lut.rs:204-221, :237-266 (kernel == score_code,
+                                      G=32 fixture, partial block); brute-force bit sum
+  pack/unpack     (lut.rs:98-119)    pinned: round trip (lut.rs:246-252); bit-level brute force
+  hamming         (lut.rs:182-191)   pinned: lut.rs:268-273 known answer; numpy popcount
+  fwht/rotation   (rabitq.rs:34-84)  pinned: explicit Hadamard matrix product (float64),
+                                      orthogonality rabitq.rs:147-157
+  rabitq_encode   (rabitq.rs:86-108) pinned: closed form of mul/err in float64;
+                                      estimate bound rabitq.rs:159-187
+  query_new/estimate (rabitq.rs:112-131) pinned: estimator identity
+                                      a*lut+b == sum(sign_i * (Pq)_i) within LUT rounding
+  normalize/sign_code/int8 (quant.rs) pinned: quant.rs:147-171 known answers
+  query_i8/dot_i8i8 (quant.rs:102-125) pinned: integer identity vs numpy int64
+  TopK            (topk.rs:4-62)     pinned: topk.rs:257-270 (== full sort)
+  tier2           (tier2.rs)         pinned: tier2.rs:326-341 (table score == reconstruction)
+  prune_threshold (ivf.rs:540-550)   pinned: ivf.rs:856-868 known answer
+  route           (ivf.rs:587-613)   pinned: float64 brute-force probe set (up to f32 ties)
+  candidates      (ivf.rs:643-729)   pinned: brute-force estimate over all probed slots,
+                                      top-cand by full sort, dedup by row
+  rerank          (ivf.rs:731-815)   pinned: brute-force int64 dots + full sort per mode
+  l2 search       (bigann.rs:652-747) pinned: brute-force L2 estimate/rerank by full sort
+  assign_top/soar (ivf.rs:137-186)   pinned: float64 brute force (up to near-ties)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsieve_oracle.so")
+
+PDIM = 256
+DIM = 768
+BLOCK = 32
+PAD = 0xFFFFFFFF
+
+RERANK_TAGS = {"none": 0, "prefix": 1, "full": 2, "tiered": 3, "ssd": 4}
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("cand", ctypes.c_uint32), ("rerank", ctypes.c_uint32), ("rerank_r", ctypes.c_uint32),
+                ("k", ctypes.c_uint32), ("prune_m", ctypes.c_uint32), ("prune_w", ctypes.c_float),
+                ("nprobe", ctypes.c_uint32), ("margin", ctypes.c_float), ("budget", ctypes.c_uint64)]
+
+
+class _Timing(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in
+                ("route", "scan", "rerank", "tier2", "scanned", "reranked", "probed", "fetch", "wait")]
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_uint32), ("n", ctypes.c_uint64), ("nblocks", ctypes.c_uint64),
+                ("centroids", ctypes.c_void_p), ("block_off", ctypes.c_void_p), ("blocks", ctypes.c_void_p),
+                ("ids", ctypes.c_void_p), ("factors", ctypes.c_void_p), ("rotation", ctypes.c_void_p),
+                ("rerank", ctypes.c_void_p), ("quant", ctypes.c_void_p), ("norm256", ctypes.c_void_p),
+                ("tier2_sigma", ctypes.c_void_p), ("tier2_codes", ctypes.c_void_p),
+                ("tier2_norm", ctypes.c_void_p), ("borrow_rerank", ctypes.c_uint32)]
+
+
+class _View(ctypes.Structure):
+    _fields_ = [("deleted", ctypes.c_void_p), ("deleted_words", ctypes.c_uint64), ("allow", ctypes.c_void_p),
+                ("allow_words", ctypes.c_uint64), ("allow_stride", ctypes.c_uint64),
+                ("allow_flags", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, sz, i32, u32, u64, f32 = (ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint32,
+                                      ctypes.c_uint64, ctypes.c_float)
+        sig = {
+            "or_rng_next": (u64, [vp]),
+            "or_gemm_abt": (None, [vp, vp, sz, sz, sz, vp, i32]),
+            "or_lut_new": (None, [vp, i32, vp, vp, vp]),
+            "or_score_code": (ctypes.c_uint16, [vp, i32, vp]),
+            "or_float_score": (f32, [vp, i32, vp]),
+            "or_pack_block": (None, [vp, i32, i32, vp]),
+            "or_unpack_slot": (None, [vp, i32, i32, vp]),
+            "or_scan": (None, [vp, i32, vp, sz, vp]),
+            "or_hamming": (u32, [vp, vp, i32]),
+            "or_rotation_signs": (None, [u64, i32, vp]),
+            "or_fwht": (None, [vp, i32]),
+            "or_rotation_apply": (None, [vp, i32, vp]),
+            "or_rabitq_encode": (None, [vp, vp, i32, vp, vp]),
+            "or_query_new": (None, [vp, i32, vp, vp, vp]),
+            "or_estimate": (f32, [f32, f32, ctypes.c_uint16, f32, f32]),
+            "or_normalize": (f32, [vp, i32]),
+            "or_sign_code": (None, [vp, i32, vp]),
+            "or_int8_fit": (None, [vp, sz, vp]),
+            "or_int8_encode": (None, [vp, vp, i32, vp]),
+            "or_dot_f32": (f32, [vp, vp, i32]),
+            "or_dot_i8": (f32, [vp, vp, i32]),
+            "or_query_i8": (f32, [vp, i32, vp]),
+            "or_dot_i8i8": (ctypes.c_int32, [vp, vp, i32]),
+            "or_topk_f32": (sz, [vp, vp, sz, sz, vp, vp]),
+            "or_topk_u16": (sz, [vp, vp, sz, sz, vp, vp]),
+            "or_tier2_fit": (None, [vp, sz, vp]),
+            "or_tier2_encode": (f32, [vp, vp, vp]),
+            "or_tier2_tables": (None, [vp, vp, vp]),
+            "or_tier2_score": (f32, [vp, vp, f32]),
+            "or_prune_threshold": (f32, [vp, vp, sz, sz, f32]),
+            "or_index_create": (vp, [ctypes.POINTER(_Desc)]),
+            "or_index_free": (None, [vp]),
+            "or_route": (sz, [vp, vp, ctypes.POINTER(_Params), vp, vp]),
+            "or_candidates": (sz, [vp, vp, vp, vp, sz, ctypes.POINTER(_Params), ctypes.POINTER(_View), sz,
+                                   vp, vp, vp, vp]),
+            "or_rerank": (sz, [vp, vp, vp, vp, vp, vp, sz, ctypes.POINTER(_Params), vp, vp, vp]),
+            "or_search_batch": (None, [vp, vp, vp, sz, ctypes.POINTER(_Params), ctypes.POINTER(_View), i32,
+                                       vp, vp, vp, ctypes.POINTER(_Timing)]),
+            "or_search_latency": (None, [vp, vp, vp, sz, ctypes.POINTER(_Params), vp]),
+            "or_assign_top": (None, [vp, sz, vp, sz, sz, vp, i32]),
+            "or_soar_secondary": (u32, [vp, vp, vp, sz, f32]),
+            "or_l2_leaf_norms": (None, [vp, sz, vp]),
+            "or_l2_create": (vp, [u32, vp, vp, vp, vp, vp, vp, vp, vp]),
+            "or_l2_free": (None, [vp]),
+            "or_l2_search": (sz, [vp, vp, u32, u32, u32, f32, vp, vp, vp, vp, vp, vp, vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# ------------------------------------------------------------------ small kernels ----
+def rng_stream(seed, n):
+    st = np.array([seed], dtype=np.uint64)
+    return np.array([lib().or_rng_next(_p(st)) for _ in range(n)], dtype=np.uint64)
+
+
+def gemm_abt(a, b, threads=1):
+    a, b = _c(a, np.float32), _c(b, np.float32)
+    m, k = a.shape
+    n = b.shape[0]
+    c = np.empty((m, n), np.float32)
+    lib().or_gemm_abt(_p(a), _p(b), m, n, k, _p(c), threads)
+    return c
+
+
+def lut_new(q):
+    q = _c(q, np.float32)
+    G = q.shape[0] // 4
+    t = np.empty((G, 16), np.uint8)
+    s, b = ctypes.c_float(), ctypes.c_float()
+    lib().or_lut_new(_p(q), G, _p(t), ctypes.byref(s), ctypes.byref(b))
+    return t, s.value, b.value
+
+
+def score_code(tables, code):
+    tables, code = _c(tables, np.uint8), _c(code, np.uint8)
+    return lib().or_score_code(_p(tables), tables.shape[0], _p(code))
+
+
+def float_score(q, code):
+    q, code = _c(q, np.float32), _c(code, np.uint8)
+    return lib().or_float_score(_p(q), q.shape[0], _p(code))
+
+
+def pack_block(codes, G):
+    codes = _c(codes, np.uint8).reshape(-1, G // 2)
+    out = np.empty(G * 16, np.uint8)
+    lib().or_pack_block(_p(codes), codes.shape[0], G, _p(out))
+    return out
+
+
+def unpack_slot(block, k):
+    block = _c(block, np.uint8)
+    G = block.shape[0] // 16
+    code = np.empty(G // 2, np.uint8)
+    lib().or_unpack_slot(_p(block), G, k, _p(code))
+    return code
+
+
+def scan(tables, blocks):
+    tables, blocks = _c(tables, np.uint8), _c(blocks, np.uint8)
+    G = tables.shape[0]
+    nb = blocks.size // (G * 16)
+    out = np.empty(nb * BLOCK, np.uint16)
+    lib().or_scan(_p(tables), G, _p(blocks), nb, _p(out))
+    return out
+
+
+def hamming(a, b):
+    a, b = _c(a, np.uint8), _c(b, np.uint8)
+    return lib().or_hamming(_p(a), _p(b), a.shape[0])
+
+
+def rotation_signs(seed, d=PDIM):
+    s = np.empty(3 * d, np.float32)
+    lib().or_rotation_signs(seed, d, _p(s))
+    return s
+
+
+def fwht(v):
+    v = _c(v, np.float32).copy()
+    lib().or_fwht(_p(v), v.shape[0])
+    return v
+
+
+def rotation_apply(signs, v):
+    v = _c(v, np.float32).copy()
+    signs = _c(signs, np.float32)
+    lib().or_rotation_apply(_p(signs), v.shape[0], _p(v))
+    return v
+
+
+def rabitq_encode(xr, cr):
+    xr, cr = _c(xr, np.float32), _c(cr, np.float32)
+    d = xr.shape[0]
+    code = np.empty(d // 8, np.uint8)
+    me = np.empty(2, np.float32)
+    lib().or_rabitq_encode(_p(xr), _p(cr), d, _p(code), _p(me))
+    return code, me
+
+
+def query_new(signs, q):
+    """Returns (tables G x 16 u8, scale, bias, a, b) of rabitq::Query::new."""
+    signs, q = _c(signs, np.float32), _c(q, np.float32)
+    d = q.shape[0]
+    t = np.empty((d // 4, 16), np.uint8)
+    o = np.empty(4, np.float32)
+    lib().or_query_new(_p(signs), d, _p(q), _p(t), _p(o))
+    return t, o[0], o[1], o[2], o[3]
+
+
+def estimate(cent, mul, sc, a, b):
+    return lib().or_estimate(cent, mul, int(sc), a, b)
+
+
+def normalize(v):
+    v = _c(v, np.float32).copy()
+    n = lib().or_normalize(_p(v), v.shape[0])
+    return v, n
+
+
+def sign_code(p):
+    p = _c(p, np.float32)
+    out = np.empty(p.shape[0] // 8, np.uint8)
+    lib().or_sign_code(_p(p), p.shape[0], _p(out))
+    return out
+
+
+def int8_fit(rows):
+    rows = _c(rows, np.float32).reshape(-1, DIM)
+    s = np.empty(DIM, np.float32)
+    lib().or_int8_fit(_p(rows), rows.shape[0], _p(s))
+    return s
+
+
+def int8_encode(v, scale):
+    v, scale = _c(v, np.float32), _c(scale, np.float32)
+    out = np.empty(v.shape[0], np.int8)
+    lib().or_int8_encode(_p(v), _p(scale), v.shape[0], _p(out))
+    return out
+
+
+def dot_f32(a, b):
+    a, b = _c(a, np.float32), _c(b, np.float32)
+    return lib().or_dot_f32(_p(a), _p(b), a.shape[0])
+
+
+def dot_i8(w, x):
+    w, x = _c(w, np.float32), _c(x, np.int8)
+    return lib().or_dot_i8(_p(w), _p(x), w.shape[0])
+
+
+def query_i8(w):
+    w = _c(w, np.float32)
+    out = np.empty(w.shape[0], np.int8)
+    s = lib().or_query_i8(_p(w), w.shape[0], _p(out))
+    return out, s
+
+
+def dot_i8i8(a, b):
+    a, b = _c(a, np.int8), _c(b, np.int8)
+    return lib().or_dot_i8i8(_p(a), _p(b), a.shape[0])
+
+
+def topk(scores, k, ids=None):
+    scores = np.ascontiguousarray(scores)
+    n = scores.shape[0]
+    ids = _c(np.arange(n) if ids is None else ids, np.uint32)
+    if scores.dtype == np.uint16:
+        os_ = np.empty(max(k, 1), np.uint16)
+        oi = np.empty(max(k, 1), np.uint32)
+        c = lib().or_topk_u16(_p(scores), _p(ids), n, k, _p(os_), _p(oi))
+    else:
+        scores = _c(scores, np.float32)
+        os_ = np.empty(max(k, 1), np.float32)
+        oi = np.empty(max(k, 1), np.uint32)
+        c = lib().or_topk_f32(_p(scores), _p(ids), n, k, _p(os_), _p(oi))
+    return os_[:c], oi[:c]
+
+
+def tier2_fit(rows):
+    rows = _c(rows, np.float32).reshape(-1, PDIM)
+    s = np.empty(PDIM, np.float32)
+    lib().or_tier2_fit(_p(rows), rows.shape[0], _p(s))
+    return s
+
+
+def tier2_encode(sigma, prefix):
+    sigma, prefix = _c(sigma, np.float32), _c(prefix, np.float32)
+    out = np.empty(PDIM // 4, np.uint8)
+    n = lib().or_tier2_encode(_p(sigma), _p(prefix), _p(out))
+    return out, n
+
+
+def tier2_tables(sigma, q):
+    sigma, q = _c(sigma, np.float32), _c(q, np.float32)
+    t = np.empty((PDIM // 4, 256), np.float32)
+    lib().or_tier2_tables(_p(sigma), _p(q), _p(t))
+    return t
+
+
+def tier2_score(tables, code, norm):
+    tables, code = _c(tables, np.float32), _c(code, np.uint8)
+    return lib().or_tier2_score(_p(tables), _p(code), norm)
+
+
+def prune_threshold(est, err, m, w):
+    est, err = _c(est, np.float32), _c(err, np.float32)
+    return lib().or_prune_threshold(_p(est), _p(err), est.shape[0], m, w)
+
+
+def assign_top(rows, cents, m, threads=8):
+    rows, cents = _c(rows, np.float32), _c(cents, np.float32)
+    out = np.empty((rows.shape[0], m), np.uint32)
+    lib().or_assign_top(_p(rows), rows.shape[0], _p(cents), cents.shape[0], m, _p(out), threads)
+    return out
+
+
+def soar_secondary(x, cents, cand, lam):
+    x, cents, cand = _c(x, np.float32), _c(cents, np.float32), _c(cand, np.uint32)
+    return lib().or_soar_secondary(_p(x), _p(cents), _p(cand), cand.shape[0], lam)
+
+
+# ------------------------------------------------------------------ params / view ----
+def make_params(p):
+    """p: any object with nprobe, cand, k, rerank (str tag or (tag, r)), prune, margin, budget."""
+    rr = p.rerank
+    tag, r = (rr, 0) if isinstance(rr, str) else rr
+    prune = p.prune if p.prune is not None else (0, 0.0)
+    margin = float("nan") if p.margin is None else p.margin
+    return _Params(cand=p.cand, rerank=RERANK_TAGS[tag], rerank_r=r, k=p.k, prune_m=prune[0], prune_w=prune[1],
+                   nprobe=p.nprobe, margin=margin, budget=p.budget or 0)
+
+
+def make_view(deleted=None, allow=None, allow_stride=0, allow_flags=None):
+    v = _View()
+    if deleted is not None:
+        v.deleted = deleted.ctypes.data
+        v.deleted_words = deleted.shape[0]
+    if allow is not None:
+        v.allow = allow.ctypes.data
+        v.allow_words = allow.shape[-1] if allow_stride else allow.shape[0]
+        v.allow_stride = allow_stride
+        if allow_flags is not None:
+            v.allow_flags = allow_flags.ctypes.data
+    return v
+
+
+# ------------------------------------------------------------------ the index --------
+class Index:
+    """The reference's ivf::Index + Corpus over arrays in the on-disk layout (host numpy)."""
+
+    def __init__(self, arrays):
+        keep = {}
+        for name, dt in (("centroids", np.float32), ("block_off", np.uint64), ("blocks", np.uint8),
+                         ("ids", np.uint32), ("factors", np.float32), ("rotation", np.float32),
+                         ("rerank", np.int8), ("quant", np.float32), ("norm256", np.float32),
+                         ("tier2_sigma", np.float32), ("tier2_codes", np.uint8), ("tier2_norm", np.float32)):
+            a = arrays.get(name)
+            keep[name] = None if a is None else np.ascontiguousarray(a, dtype=dt)
+        self._keep = keep
+        d = _Desc()
+        d.k = keep["centroids"].size // PDIM
+        d.n = keep["rerank"].size // DIM
+        d.nblocks = keep["ids"].size // BLOCK
+        for name, a in keep.items():
+            setattr(d, name, None if a is None else a.ctypes.data)
+        self.k, self.n, self.nblocks = d.k, d.n, d.nblocks
+        self._h = lib().or_index_create(ctypes.byref(d))
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().or_index_free(self._h)
+        except Exception:
+            pass
+        self._h = None
+
+    def route(self, q256, p):
+        q = _c(q256, np.float32)
+        s = np.empty(max(p.nprobe, 1), np.float32)
+        c = np.empty(max(p.nprobe, 1), np.uint32)
+        n = lib().or_route(self._h, _p(q), ctypes.byref(make_params(p)), _p(s), _p(c))
+        return s[:n], c[:n]
+
+    def candidates(self, q256, probe_scores, probe_clusters, p, view=None, qi=0):
+        q = _c(q256, np.float32)
+        ps, pc = _c(probe_scores, np.float32), _c(probe_clusters, np.uint32)
+        est = np.empty(max(p.cand, 1), np.float32)
+        rows = np.empty(max(p.cand, 1), np.uint32)
+        errs = np.empty(max(p.cand, 1), np.float32)
+        sc = np.zeros(1, np.uint64)
+        v = view if view is not None else _View()
+        n = lib().or_candidates(self._h, _p(q), _p(ps), _p(pc), ps.shape[0], ctypes.byref(make_params(p)),
+                                ctypes.byref(v), qi, _p(est), _p(rows), _p(errs), _p(sc))
+        return est[:n], rows[:n], errs[:n], int(sc[0])
+
+    def rerank(self, q768, q256, est, rows, errs, p):
+        q7, q2 = _c(q768, np.float32), _c(q256, np.float32)
+        est, rows, errs = _c(est, np.float32), _c(rows, np.uint32), _c(errs, np.float32)
+        os_ = np.empty(max(p.k, 1), np.float32)
+        orr = np.empty(max(p.k, 1), np.uint32)
+        rr = np.zeros(1, np.uint64)
+        n = lib().or_rerank(self._h, _p(q7), _p(q2), _p(est), _p(rows), _p(errs), est.shape[0],
+                            ctypes.byref(make_params(p)), _p(os_), _p(orr), _p(rr))
+        return os_[:n], orr[:n], int(rr[0])
+
+    def search_batch(self, q768, q256, p, view=None, threads=1):
+        q7, q2 = _c(q768, np.float32), _c(q256, np.float32)
+        nq = q7.shape[0]
+        s = np.zeros((nq, p.k), np.float32)
+        r = np.full((nq, p.k), PAD, np.uint32)
+        cnt = np.zeros(nq, np.uint32)
+        t = _Timing()
+        v = view if view is not None else _View()
+        lib().or_search_batch(self._h, _p(q7), _p(q2), nq, ctypes.byref(make_params(p)), ctypes.byref(v), threads,
+                              _p(s), _p(r), _p(cnt), ctypes.byref(t))
+        return s, r, cnt, {f: getattr(t, f) for f, _ in _Timing._fields_}
+
+    def search_latency(self, q768, q256, p):
+        q7, q2 = _c(q768, np.float32), _c(q256, np.float32)
+        lat = np.zeros(q7.shape[0], np.uint64)
+        lib().or_search_latency(self._h, _p(q7), _p(q2), q7.shape[0], ctypes.byref(make_params(p)), _p(lat))
+        return lat
+
+
+class L2Index:
+    """bigann::Index over arrays (leaves k x 100, blocks of 512 B, [mul, |x|^2] factors)."""
+
+    def __init__(self, leaves, block_off, blocks, factors, ids, rotation, base):
+        self._keep = [np.ascontiguousarray(leaves, np.float32), np.ascontiguousarray(block_off, np.uint64),
+                      np.ascontiguousarray(blocks, np.uint8), np.ascontiguousarray(factors, np.float32),
+                      np.ascontiguousarray(ids, np.uint32), np.ascontiguousarray(rotation, np.float32),
+                      np.ascontiguousarray(base, np.int8)]
+        lv = self._keep[0]
+        self.k = lv.shape[0]
+        self.leaf_norm = np.empty(self.k, np.float32)
+        lib().or_l2_leaf_norms(_p(lv), self.k, _p(self.leaf_norm))
+        lv, bo, bl, fa, ids, rot, base = self._keep
+        self._h = lib().or_l2_create(self.k, _p(lv), _p(self.leaf_norm), _p(bo), _p(bl), _p(fa), _p(ids), _p(rot),
+                                     _p(base))
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().or_l2_free(self._h)
+        except Exception:
+            pass
+        self._h = None
+
+    def search(self, q, nprobe, cand, k, margin=0.0):
+        q = _c(q, np.int8)
+        probe = np.empty(max(nprobe, 1), np.uint32)
+        npo = np.zeros(1, np.uint32)
+        ci = np.empty(max(cand, 1), np.uint32)
+        ce = np.empty(max(cand, 1), np.float32)
+        nc = np.zeros(1, np.uint32)
+        os_ = np.empty(max(k, 1), np.float32)
+        orr = np.empty(max(k, 1), np.uint32)
+        n = lib().or_l2_search(self._h, _p(q), nprobe, cand, k, margin, _p(probe), _p(npo), _p(ci), _p(ce), _p(nc),
+                               _p(os_), _p(orr))
+        return (probe[:npo[0]], ci[:nc[0]], ce[:nc[0]], os_[:n], orr[:n])

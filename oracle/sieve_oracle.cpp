@@ -1,0 +1,1026 @@
+// sieve_oracle.cpp — TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+//
+// A CPU restatement of the reference's x86 (non-NEON) search path, used only as the checker:
+// by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg. Nothing in the
+// product (a-game-of-vectors_amd/) links, loads or calls it.
+//
+// Every function follows the Rust it cites (paths are into the sieve crate,
+// 18Abhinav07/A-Game-Of-Vectors) with Rust's floating-point rules: IEEE f32, no FMA
+// contraction (built with -ffp-contract=off), left-to-right sums, `Sum for f32` seeded with
+// -0.0, round() half away from zero, `as u8/i8` saturating.
+//
+// Parity status of each part is recorded in the header of oracle/oracle.py.
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <thread>
+#include <vector>
+
+#include "../include/sieve_gpu.h"
+
+namespace {
+
+constexpr int DIM = 768;
+constexpr int PDIM = 256;
+constexpr int BLOCK = 32;
+constexpr uint32_t PAD = 0xFFFFFFFFu;
+constexpr float EPS0 = 1.9f;  // src/rabitq.rs:28
+
+// ---------------------------------------------------------------- Rng -----------------
+// splitmix64, src/ivf.rs:120-135.
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed) {}
+    uint64_t next() {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    size_t below(size_t n) { return (size_t)(next() % (uint64_t)n); }
+};
+
+// Rust `Iterator::sum::<f32>()`: fold from -0.0, left to right.
+inline float rsum_seed() { return -0.0f; }
+
+// ---------------------------------------------------------------- TopK ----------------
+// Amortised top-k, src/topk.rs:4-62. Ties at the k-th boundary are broken by nth_element
+// here and by select_nth_unstable in the reference: results agree up to ties.
+template <class S>
+struct TopK {
+    size_t k;
+    std::vector<std::pair<S, uint32_t>> items;
+    bool has_thr = false;
+    S thr{};
+    explicit TopK(size_t k_) : k(k_) { items.reserve(2 * std::max<size_t>(k_, 1)); }
+    bool rejects(S score) const { return has_thr && score <= thr; }
+    void push(S score, uint32_t id) {  // src/topk.rs:27-36
+        if (has_thr && score <= thr) return;
+        items.emplace_back(score, id);
+        if (items.size() >= 2 * k) compact();
+    }
+    void compact() {  // src/topk.rs:38-44
+        if (items.size() > k) {
+            std::nth_element(items.begin(), items.begin() + (k - 1), items.end(),
+                             [](const auto& a, const auto& b) { return b.first < a.first; });
+            items.resize(k);
+            thr = items[k - 1].first;
+            has_thr = true;
+        }
+    }
+    std::vector<std::pair<S, uint32_t>> into_sorted() {  // src/topk.rs:47-61
+        compact();
+        std::sort(items.begin(), items.end(), [](const auto& a, const auto& b) {
+            if (a.first != b.first) return b.first < a.first;
+            return a.second < b.second;
+        });
+        if (items.size() > k) items.resize(k);
+        return items;
+    }
+};
+
+// ---------------------------------------------------------------- blas ----------------
+// gemm_abt, non-macOS fallback (src/blas.rs:43-54): each c[i][j] a sequential f32 sum.
+void gemm_abt(const float* a, const float* b, size_t m, size_t n, size_t k, float* c, int threads) {
+    auto rows = [&](size_t r0, size_t r1) {
+        for (size_t i = r0; i < r1; i++) {
+            const float* ar = a + i * k;
+            for (size_t j = 0; j < n; j++) {
+                const float* br = b + j * k;
+                float acc = rsum_seed();
+                for (size_t t = 0; t < k; t++) acc = acc + ar[t] * br[t];
+                c[i * n + j] = acc;
+            }
+        }
+    };
+    if (threads <= 1 || m < 2) {
+        rows(0, m);
+        return;
+    }
+    std::vector<std::thread> th;
+    size_t per = (m + threads - 1) / threads;
+    for (int t = 0; t < threads; t++) {
+        size_t r0 = std::min(m, t * per), r1 = std::min(m, (t + 1) * per);
+        if (r0 < r1) th.emplace_back(rows, r0, r1);
+    }
+    for (auto& x : th) x.join();
+}
+
+// ---------------------------------------------------------------- lut -----------------
+inline uint8_t nibble(const uint8_t* code, int g) {  // src/lut.rs:53-56
+    return (code[g >> 1] >> ((g & 1) * 4)) & 0x0F;
+}
+
+// Lut::new, src/lut.rs:58-81. tables is G x 16.
+void lut_new(const float* q, int G, uint8_t* tables, float* scale_out, float* bias_out) {
+    std::vector<float> f((size_t)G * 16);
+    for (int g = 0; g < G; g++)
+        for (int c = 0; c < 16; c++) {
+            float v = rsum_seed();
+            for (int j = 0; j < 4; j++)
+                if ((c >> j) & 1) v = v + q[4 * g + j];
+            f[g * 16 + c] = v;
+        }
+    std::vector<float> mins(G);
+    for (int g = 0; g < G; g++) {
+        float m = std::numeric_limits<float>::max();
+        for (int c = 0; c < 16; c++) m = std::fmin(m, f[g * 16 + c]);
+        mins[g] = m;
+    }
+    float max_range = 0.0f;
+    for (int g = 0; g < G; g++) {
+        float mx = std::numeric_limits<float>::lowest();
+        for (int c = 0; c < 16; c++) mx = std::fmax(mx, f[g * 16 + c]);
+        max_range = std::fmax(max_range, mx - mins[g]);
+    }
+    float scale = max_range > 0.0f ? 255.0f / max_range : 1.0f;
+    for (int g = 0; g < G; g++)
+        for (int c = 0; c < 16; c++) {
+            float v = std::round((f[g * 16 + c] - mins[g]) * scale);
+            v = std::min(std::max(v, 0.0f), 255.0f);
+            tables[g * 16 + c] = (uint8_t)v;
+        }
+    float bias = rsum_seed();
+    for (int g = 0; g < G; g++) bias = bias + mins[g];
+    *scale_out = scale;
+    *bias_out = bias;
+}
+
+uint16_t score_code(const uint8_t* tables, int G, const uint8_t* code) {  // src/lut.rs:84-86
+    uint16_t s = 0;
+    for (int g = 0; g < G; g++) s = (uint16_t)(s + tables[g * 16 + nibble(code, g)]);
+    return s;
+}
+
+// pack_block, src/lut.rs:98-108: codes is n x (G/2) bytes, n <= 32, out G x 16 bytes.
+void pack_block(const uint8_t* codes, int n, int G, uint8_t* out) {
+    int cb = G / 2;
+    auto get = [&](int k, int g) -> uint8_t { return k < n ? nibble(codes + (size_t)k * cb, g) : 0; };
+    for (int g = 0; g < G; g++)
+        for (int k = 0; k < 16; k++) out[g * 16 + k] = (uint8_t)(get(k, g) | (get(k + 16, g) << 4));
+}
+
+void unpack_slot(const uint8_t* block, int G, int k, uint8_t* code) {  // src/lut.rs:110-119
+    std::memset(code, 0, G / 2);
+    for (int g = 0; g < G; g++) {
+        uint8_t b = block[g * 16 + (k & 15)];
+        uint8_t nib = k < 16 ? (b & 0x0F) : (b >> 4);
+        code[g >> 1] |= (uint8_t)(nib << ((g & 1) * 4));
+    }
+}
+
+// scan_block_scalar, src/lut.rs:142-152, over nblocks blocks (scan, src/lut.rs:121-128).
+void scan(const uint8_t* tables, int G, const uint8_t* blocks, size_t nblocks, uint16_t* out) {
+    for (size_t b = 0; b < nblocks; b++) {
+        const uint8_t* blk = blocks + b * (size_t)G * 16;
+        uint16_t* o = out + b * BLOCK;
+        for (int k = 0; k < BLOCK; k++) o[k] = 0;
+        for (int g = 0; g < G; g++) {
+            const uint8_t* t = tables + g * 16;
+            for (int k = 0; k < 16; k++) {
+                uint8_t x = blk[g * 16 + k];
+                o[k] = (uint16_t)(o[k] + t[x & 0x0F]);
+                o[k + 16] = (uint16_t)(o[k + 16] + t[x >> 4]);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- rabitq --------------
+// Rotation::with_dim signs, src/rabitq.rs:40-44.
+void rotation_signs(uint64_t seed, int d, float* signs) {
+    Rng r(seed);
+    for (int i = 0; i < 3 * d; i++) signs[i] = (r.next() & 1) == 1 ? 1.0f : -1.0f;
+}
+
+void fwht(float* v, int n) {  // src/rabitq.rs:72-84
+    for (int h = 1; h < n; h *= 2)
+        for (int blk = 0; blk < n; blk += 2 * h)
+            for (int i = 0; i < h; i++) {
+                float x = v[blk + i], y = v[blk + h + i];
+                v[blk + i] = x + y;
+                v[blk + h + i] = x - y;
+            }
+}
+
+void rotation_apply(const float* signs, int d, float* v) {  // src/rabitq.rs:59-69
+    for (int r = 0; r < 3; r++) {
+        for (int i = 0; i < d; i++) v[i] = v[i] * signs[r * d + i];
+        fwht(v, d);
+    }
+    float norm = std::pow((float)d, -1.5f);
+    for (int i = 0; i < d; i++) v[i] = v[i] * norm;
+}
+
+// encode, src/rabitq.rs:86-108. Writes d/8 code bytes and [mul, err].
+void rabitq_encode(const float* xr, const float* cr, int d, uint8_t* code, float* mul_err) {
+    std::memset(code, 0, d / 8);
+    float nr2 = 0.0f, l1 = 0.0f;
+    for (int i = 0; i < d; i++) {
+        float r = xr[i] - cr[i];
+        nr2 = nr2 + r * r;
+        l1 = l1 + std::fabs(r);
+        if (r > 0.0f) code[i >> 3] |= (uint8_t)(1u << (i & 7));
+    }
+    if (l1 < 1e-12f) {
+        mul_err[0] = 0.0f;
+        mul_err[1] = 0.0f;
+        return;
+    }
+    float df = (float)d;
+    float nr = std::sqrt(nr2);
+    float od = l1 / (nr * std::sqrt(df));
+    float inner = (1.0f - od * od) / (od * od * (df - 1.0f));
+    inner = std::fmax(inner, 0.0f);
+    float err = nr * EPS0 * std::sqrt(inner);
+    mul_err[0] = nr2 / l1;
+    mul_err[1] = err;
+}
+
+struct QueryState {  // rabitq::Query, src/rabitq.rs:112-131
+    std::vector<uint8_t> tables;
+    float scale, bias, a, b;
+};
+
+void query_new(const float* signs, int d, const float* q, QueryState& qs) {  // src/rabitq.rs:119-125
+    std::vector<float> qr(q, q + d);
+    rotation_apply(signs, d, qr.data());
+    int G = d / 4;
+    qs.tables.resize((size_t)G * 16);
+    lut_new(qr.data(), G, qs.tables.data(), &qs.scale, &qs.bias);
+    float sum = rsum_seed();
+    for (int i = 0; i < d; i++) sum = sum + qr[i];
+    qs.a = 2.0f / qs.scale;
+    qs.b = 2.0f * qs.bias - sum;
+}
+
+inline float estimate(const QueryState& qs, float cent, float mul, uint16_t sc) {  // src/rabitq.rs:127-130
+    return cent + mul * (qs.a * (float)sc + qs.b);
+}
+
+// ---------------------------------------------------------------- quant ---------------
+float normalize(float* v, int n) {  // src/quant.rs:5-13
+    float s = rsum_seed();
+    for (int i = 0; i < n; i++) s = s + v[i] * v[i];
+    float nn = std::sqrt(s);
+    if (nn > 0.0f) {
+        float inv = 1.0f / nn;
+        for (int i = 0; i < n; i++) v[i] = v[i] * inv;
+    }
+    return nn;
+}
+
+void sign_code(const float* p, int n, uint8_t* out) {  // src/quant.rs:22-32
+    std::memset(out, 0, n / 8);
+    for (int i = 0; i < n; i++)
+        if (p[i] > 0.0f) out[i >> 3] |= (uint8_t)(1u << (i & 7));
+}
+
+float dot_f32(const float* a, const float* b, int n) {  // src/quant.rs:86-100
+    float acc[16];
+    for (float& x : acc) x = 0.0f;
+    int full = n / 16 * 16;
+    for (int i = 0; i < full; i += 16)
+        for (int j = 0; j < 16; j++) acc[j] = acc[j] + a[i + j] * b[i + j];
+    float tail = rsum_seed();
+    for (int i = full; i < n; i++) tail = tail + a[i] * b[i];
+    float s = rsum_seed();
+    for (float x : acc) s = s + x;
+    return s + tail;
+}
+
+float dot_i8(const float* w, const int8_t* x, int n) {  // src/quant.rs:70-84
+    float acc[16];
+    for (float& v : acc) v = 0.0f;
+    int full = n / 16 * 16;
+    for (int i = 0; i < full; i += 16)
+        for (int j = 0; j < 16; j++) acc[j] = acc[j] + w[i + j] * (float)x[i + j];
+    float tail = rsum_seed();
+    for (int i = full; i < n; i++) tail = tail + w[i] * (float)x[i];
+    float s = rsum_seed();
+    for (float v : acc) s = s + v;
+    return s + tail;
+}
+
+inline int8_t sat_i8(float v) {  // Rust `f32 as i8`: saturating, NaN -> 0
+    if (v != v) return 0;
+    if (v >= 127.0f) return 127;
+    if (v <= -128.0f) return -128;
+    return (int8_t)v;
+}
+
+float query_i8(const float* w, int n, int8_t* out) {  // src/quant.rs:102-110
+    float m = 1e-12f;
+    for (int i = 0; i < n; i++) m = std::fmax(m, std::fabs(w[i]));
+    float s = 127.0f / m;
+    for (int i = 0; i < n; i++) out[i] = sat_i8(std::round(w[i] * s));
+    return s;
+}
+
+int32_t dot_i8i8(const int8_t* a, const int8_t* b, int n) {  // src/quant.rs:123-125
+    int32_t s = 0;
+    for (int i = 0; i < n; i++) s += (int32_t)a[i] * (int32_t)b[i];
+    return s;
+}
+
+// ---------------------------------------------------------------- tier2 ---------------
+constexpr float T2_T = 0.9816f;  // src/tier2.rs:10-11
+constexpr float T2_LEVELS[4] = {-1.510f, -0.4528f, 0.4528f, 1.510f};
+
+void tier2_fit(const float* rows, size_t nrows, float* sigma) {  // src/tier2.rs:23-31
+    float n = (float)std::max<size_t>(nrows, 1);
+    std::vector<double> ss(PDIM, 0.0);
+    for (size_t r = 0; r < nrows; r++)
+        for (int i = 0; i < PDIM; i++) {
+            float x = rows[r * PDIM + i];
+            ss[i] += (double)(x * x);
+        }
+    for (int i = 0; i < PDIM; i++) sigma[i] = std::fmax((float)std::sqrt(ss[i] / (double)n), 1e-6f);
+}
+
+inline uint8_t tier2_level(float x, float s) {  // src/tier2.rs:33-37
+    float t = T2_T * s;
+    if (x < -t) return 0;
+    if (x < 0.0f) return 1;
+    if (x < t) return 2;
+    return 3;
+}
+
+float tier2_encode(const float* sigma, const float* prefix, uint8_t* out) {  // src/tier2.rs:39-50
+    std::memset(out, 0, PDIM / 4);
+    float nn = 0.0f;
+    for (int i = 0; i < PDIM; i++) {
+        uint8_t l = tier2_level(prefix[i], sigma[i]);
+        out[i >> 2] |= (uint8_t)(l << ((i & 3) * 2));
+        float v = T2_LEVELS[l] * sigma[i];
+        nn = nn + v * v;
+    }
+    return std::sqrt(nn);
+}
+
+void tier2_tables(const float* sigma, const float* q, float* tables) {  // src/tier2.rs:52-66
+    for (int g = 0; g < PDIM / 4; g++) {
+        float half[2][16];
+        for (int h = 0; h < 2; h++)
+            for (int b = 0; b < 16; b++) {
+                float s = rsum_seed();
+                for (int j = 0; j < 2; j++) {
+                    int i = 4 * g + 2 * h + j;
+                    s = s + q[i] * T2_LEVELS[(b >> (2 * j)) & 3] * sigma[i];
+                }
+                half[h][b] = s;
+            }
+        for (int b = 0; b < 256; b++) tables[g * 256 + b] = half[0][b & 15] + half[1][b >> 4];
+    }
+}
+
+float tier2_score(const float* tables, const uint8_t* code, float norm) {  // src/tier2.rs:69-75
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int g = 0; g < PDIM / 4; g++) acc[g & 3] = acc[g & 3] + tables[g * 256 + code[g]];
+    return (acc[0] + acc[1] + acc[2] + acc[3]) / norm;
+}
+
+// ---------------------------------------------------------------- ivf -----------------
+struct View {  // src/ivf.rs:462-496 (tombstones and allowlist; delta lists not modelled)
+    const uint64_t* deleted = nullptr;
+    uint64_t deleted_words = 0;
+    const uint64_t* allow = nullptr;
+    uint64_t allow_words = 0;
+    static bool bit(const uint64_t* set, uint64_t words, uint32_t id) {  // src/ivf.rs:474-477
+        uint64_t w = id >> 6;
+        return w < words && ((set[w] >> (id & 63)) & 1) == 1;
+    }
+    bool hidden(uint32_t id) const {  // src/ivf.rs:480-483
+        return (deleted && bit(deleted, deleted_words, id)) || (allow && !bit(allow, allow_words, id));
+    }
+};
+
+struct Index {
+    uint32_t k = 0;
+    uint64_t n = 0, nblocks = 0;
+    const float* centroids = nullptr;
+    const uint64_t* block_off = nullptr;
+    const uint8_t* blocks = nullptr;
+    const uint32_t* ids = nullptr;
+    const float* factors = nullptr;
+    const float* rotation = nullptr;
+    const int8_t* rerank = nullptr;
+    const float* quant = nullptr;
+    const float* norm256 = nullptr;
+    const float* t2_sigma = nullptr;
+    const uint8_t* t2_codes = nullptr;
+    const float* t2_norm = nullptr;
+
+    size_t slots(uint32_t c) const { return (size_t)(block_off[c + 1] - block_off[c]) * BLOCK; }  // src/ivf.rs:615-618
+};
+
+struct Probe {
+    float score;
+    uint32_t cluster;
+};
+struct Cand {
+    float est;
+    uint32_t row;
+    float err;
+};
+
+inline bool has_margin(const sieve_params& p) { return !(p.margin != p.margin); }
+
+// Index::route, src/ivf.rs:587-613.
+std::vector<Probe> route(const Index& ix, const float* q256, const sieve_params& p, std::vector<float>& cent) {
+    cent.resize(ix.k);
+    gemm_abt(q256, ix.centroids, 1, ix.k, PDIM, cent.data(), 1);
+    TopK<float> top(std::min<size_t>(p.nprobe, ix.k));
+    for (uint32_t c = 0; c < ix.k; c++) top.push(cent[c], c);
+    auto sorted = top.into_sorted();
+    std::vector<Probe> probe;
+    for (auto& x : sorted) probe.push_back({x.first, x.second});
+    if (has_margin(p) && !probe.empty()) {
+        float lim = probe[0].score - p.margin;
+        std::vector<Probe> kept;
+        for (auto& x : probe)
+            if (x.score >= lim) kept.push_back(x);
+        probe.swap(kept);
+    }
+    if (p.budget > 0) {
+        uint64_t used = 0;
+        size_t keep = 0;
+        for (auto& x : probe) {
+            used += ix.slots(x.cluster);
+            if (used <= p.budget) keep++;
+            else break;
+        }
+        probe.resize(std::min(probe.size(), std::max<size_t>(keep, 1)));
+    }
+    return probe;
+}
+
+// Index::candidates, src/ivf.rs:643-729 (RaBitQ and global-sign-code branches).
+std::vector<Cand> candidates(const Index& ix, const float* q256, const std::vector<Probe>& probe,
+                             const sieve_params& p, const View& v, uint64_t* scanned) {
+    std::vector<Cand> cand;
+    std::vector<uint16_t> scores;
+    if (ix.rotation && ix.factors) {
+        QueryState rq;
+        query_new(ix.rotation, PDIM, q256, rq);
+        TopK<float> top(p.cand);
+        std::vector<std::pair<uint32_t, float>> meta;
+        for (const Probe& pr : probe) {
+            uint64_t b0 = ix.block_off[pr.cluster], b1 = ix.block_off[pr.cluster + 1];
+            size_t ns = (size_t)(b1 - b0) * BLOCK;
+            scores.resize(ns);
+            scan(rq.tables.data(), PDIM / 4, ix.blocks + b0 * 1024, b1 - b0, scores.data());
+            *scanned += ns;
+            const float* f = ix.factors + b0 * BLOCK * 2;
+            const uint32_t* ids = ix.ids + b0 * BLOCK;
+            for (size_t j = 0; j < ns; j++) {
+                float est = estimate(rq, pr.score, f[2 * j], scores[j]);
+                if (top.rejects(est) || ids[j] == PAD || v.hidden(ids[j])) continue;
+                top.push(est, (uint32_t)meta.size());
+                meta.emplace_back(ids[j], f[2 * j + 1]);
+            }
+        }
+        for (auto& x : top.into_sorted()) cand.push_back({x.first, meta[x.second].first, meta[x.second].second});
+    } else {
+        std::vector<uint8_t> tables(PDIM / 4 * 16);
+        float scale, bias;
+        lut_new(q256, PDIM / 4, tables.data(), &scale, &bias);
+        TopK<uint16_t> top(p.cand);
+        for (const Probe& pr : probe) {
+            uint64_t b0 = ix.block_off[pr.cluster], b1 = ix.block_off[pr.cluster + 1];
+            size_t ns = (size_t)(b1 - b0) * BLOCK;
+            scores.resize(ns);
+            scan(tables.data(), PDIM / 4, ix.blocks + b0 * 1024, b1 - b0, scores.data());
+            *scanned += ns;
+            const uint32_t* ids = ix.ids + b0 * BLOCK;
+            for (size_t j = 0; j < ns; j++) {
+                if (top.rejects(scores[j]) || ids[j] == PAD || v.hidden(ids[j])) continue;
+                top.push(scores[j], ids[j]);
+            }
+        }
+        for (auto& x : top.into_sorted())
+            cand.push_back({(float)x.first / scale + bias, x.second, std::numeric_limits<float>::infinity()});
+    }
+    // Sort by (row asc, est desc) and keep the first of each row (src/ivf.rs:725-726).
+    std::sort(cand.begin(), cand.end(), [](const Cand& a, const Cand& b) {
+        if (a.row != b.row) return a.row < b.row;
+        return b.est < a.est;
+    });
+    std::vector<Cand> out;
+    for (const Cand& c : cand)
+        if (out.empty() || out.back().row != c.row) out.push_back(c);
+    return out;
+}
+
+// prune_threshold, src/ivf.rs:540-550.
+float prune_threshold(const std::vector<Cand>& cand, size_t m, float w) {
+    if (m == 0 || cand.size() < m) return -std::numeric_limits<float>::infinity();
+    std::vector<float> lo;
+    for (const Cand& c : cand) lo.push_back(c.est - w * c.err);
+    std::nth_element(lo.begin(), lo.begin() + (m - 1), lo.end(), [](float a, float b) { return b < a; });
+    return lo[m - 1];
+}
+
+struct RerankQuery {  // the int8 query of a rerank pass (src/ivf.rs:753-755)
+    std::vector<int8_t> wq;
+    float inv;
+};
+
+RerankQuery rerank_query(const Index& ix, const float* q, int n) {
+    std::vector<float> w(n);
+    for (int i = 0; i < n; i++) w[i] = q[i] / ix.quant[i];
+    RerankQuery r;
+    r.wq.resize(n);
+    r.inv = 1.0f / query_i8(w.data(), n, r.wq.data());
+    return r;
+}
+
+// read_and_rerank, src/ivf.rs:818-832 (all rows are resident, so base and delta rows are
+// one store; the pushes keep its order: base rows first).
+void read_and_rerank(const Index& ix, const float* q768, const std::vector<uint32_t>& ids, TopK<float>& out) {
+    RerankQuery r = rerank_query(ix, q768, DIM);
+    for (uint32_t id : ids) out.push((float)dot_i8i8(r.wq.data(), ix.rerank + (size_t)id * DIM, DIM) * r.inv, id);
+}
+
+// Index::rerank, src/ivf.rs:731-815.
+std::vector<std::pair<float, uint32_t>> rerank(const Index& ix, const float* q768, const float* q256,
+                                               std::vector<Cand> cand, const sieve_params& p, uint64_t* reranked) {
+    if (p.prune_m > 0) {
+        float tau = prune_threshold(cand, p.prune_m, p.prune_w);
+        std::vector<Cand> kept;
+        for (const Cand& c : cand)
+            if (c.est + p.prune_w * c.err >= tau) kept.push_back(c);
+        cand.swap(kept);
+    }
+    *reranked += cand.size();
+    TopK<float> out(p.k);
+    switch (p.rerank) {
+        case SIEVE_RERANK_NONE:
+            for (const Cand& c : cand) out.push(c.est, c.row);
+            break;
+        case SIEVE_RERANK_FULL: {
+            RerankQuery r = rerank_query(ix, q768, DIM);
+            for (const Cand& c : cand)
+                out.push((float)dot_i8i8(r.wq.data(), ix.rerank + (size_t)c.row * DIM, DIM) * r.inv, c.row);
+            break;
+        }
+        case SIEVE_RERANK_SSD: {
+            TopK<float> best(p.rerank_r);
+            for (const Cand& c : cand) best.push(c.est, c.row);
+            std::vector<uint32_t> ids;
+            for (auto& x : best.into_sorted()) ids.push_back(x.second);
+            read_and_rerank(ix, q768, ids, out);
+            break;
+        }
+        default: {
+            if (p.rerank == SIEVE_RERANK_TIERED && ix.t2_sigma) {
+                std::vector<float> tables(64 * 256);
+                tier2_tables(ix.t2_sigma, q256, tables.data());
+                TopK<float> pre(p.rerank_r);
+                for (const Cand& c : cand)
+                    pre.push(tier2_score(tables.data(), ix.t2_codes + (size_t)c.row * 64, ix.t2_norm[c.row]), c.row);
+                std::vector<uint32_t> ids;
+                for (auto& x : pre.into_sorted()) ids.push_back(x.second);
+                read_and_rerank(ix, q768, ids, out);
+                break;
+            }
+            // Prefix, or Tiered without a 2-bit tier.
+            RerankQuery r = rerank_query(ix, q256, PDIM);
+            TopK<float> pre(p.rerank == SIEVE_RERANK_TIERED ? p.rerank_r : p.k);
+            for (const Cand& c : cand) {
+                float norm = ix.norm256[c.row];
+                pre.push((float)dot_i8i8(r.wq.data(), ix.rerank + (size_t)c.row * DIM, PDIM) * r.inv / norm, c.row);
+            }
+            if (p.rerank == SIEVE_RERANK_PREFIX) {
+                out = pre;
+            } else {
+                RerankQuery f = rerank_query(ix, q768, DIM);
+                for (auto& x : pre.into_sorted())
+                    out.push((float)dot_i8i8(f.wq.data(), ix.rerank + (size_t)x.second * DIM, DIM) * f.inv, x.second);
+            }
+        }
+    }
+    return out.into_sorted();
+}
+
+}  // namespace
+
+// ======================================================================================
+// extern "C" surface for ctypes (oracle/oracle.py)
+// ======================================================================================
+extern "C" {
+
+uint64_t or_rng_next(uint64_t* state) {
+    Rng r(*state);
+    uint64_t v = r.next();
+    *state = r.s;
+    return v;
+}
+
+void or_gemm_abt(const float* a, const float* b, size_t m, size_t n, size_t k, float* c, int threads) {
+    gemm_abt(a, b, m, n, k, c, threads);
+}
+
+void or_lut_new(const float* q, int G, uint8_t* tables, float* scale, float* bias) { lut_new(q, G, tables, scale, bias); }
+uint16_t or_score_code(const uint8_t* tables, int G, const uint8_t* code) { return score_code(tables, G, code); }
+float or_float_score(const float* q, int d, const uint8_t* code) {  // src/lut.rs:93-96
+    float s = rsum_seed();
+    for (int i = 0; i < d; i++)
+        if ((code[i >> 3] >> (i & 7)) & 1) s = s + q[i];
+    return s;
+}
+void or_pack_block(const uint8_t* codes, int n, int G, uint8_t* out) { pack_block(codes, n, G, out); }
+void or_unpack_slot(const uint8_t* block, int G, int k, uint8_t* code) { unpack_slot(block, G, k, code); }
+void or_scan(const uint8_t* tables, int G, const uint8_t* blocks, size_t nblocks, uint16_t* out) {
+    scan(tables, G, blocks, nblocks, out);
+}
+uint32_t or_hamming(const uint8_t* a, const uint8_t* b, int nbytes) {  // src/lut.rs:182-191
+    uint32_t s = 0;
+    for (int i = 0; i < nbytes; i++) s += (uint32_t)__builtin_popcount((unsigned)(a[i] ^ b[i]));
+    return s;
+}
+
+void or_rotation_signs(uint64_t seed, int d, float* signs) { rotation_signs(seed, d, signs); }
+void or_fwht(float* v, int n) { fwht(v, n); }
+void or_rotation_apply(const float* signs, int d, float* v) { rotation_apply(signs, d, v); }
+void or_rabitq_encode(const float* xr, const float* cr, int d, uint8_t* code, float* mul_err) {
+    rabitq_encode(xr, cr, d, code, mul_err);
+}
+// rabitq::Query::new: tables (d/4 x 16), then [scale, bias, a, b] in out4.
+void or_query_new(const float* signs, int d, const float* q, uint8_t* tables, float* out4) {
+    QueryState qs;
+    query_new(signs, d, q, qs);
+    std::memcpy(tables, qs.tables.data(), qs.tables.size());
+    out4[0] = qs.scale;
+    out4[1] = qs.bias;
+    out4[2] = qs.a;
+    out4[3] = qs.b;
+}
+float or_estimate(float cent, float mul, uint16_t sc, float a, float b) { return cent + mul * (a * (float)sc + b); }
+
+float or_normalize(float* v, int n) { return normalize(v, n); }
+void or_sign_code(const float* p, int n, uint8_t* out) { sign_code(p, n, out); }
+void or_int8_fit(const float* rows, size_t nrows, float* scale) {  // Int8Quantizer::fit, src/quant.rs:40-50
+    std::vector<float> maxabs(DIM, 1e-6f);
+    for (size_t r = 0; r < nrows; r++)
+        for (int i = 0; i < DIM; i++) maxabs[i] = std::fmax(maxabs[i], std::fabs(rows[r * DIM + i]));
+    for (int i = 0; i < DIM; i++) scale[i] = 127.0f / maxabs[i];
+}
+void or_int8_encode(const float* v, const float* scale, int d, int8_t* out) {  // src/quant.rs:52-56
+    for (int i = 0; i < d; i++) {
+        float x = std::round(v[i] * scale[i]);
+        x = std::min(std::max(x, -127.0f), 127.0f);
+        out[i] = (int8_t)x;
+    }
+}
+float or_dot_f32(const float* a, const float* b, int n) { return dot_f32(a, b, n); }
+float or_dot_i8(const float* w, const int8_t* x, int n) { return dot_i8(w, x, n); }
+float or_query_i8(const float* w, int n, int8_t* out) { return query_i8(w, n, out); }
+int32_t or_dot_i8i8(const int8_t* a, const int8_t* b, int n) { return dot_i8i8(a, b, n); }
+
+// TopK over (score, id) pairs pushed in order: best first, (score desc, id asc).
+size_t or_topk_f32(const float* scores, const uint32_t* ids, size_t n, size_t k, float* out_s, uint32_t* out_i) {
+    TopK<float> t(k);
+    for (size_t i = 0; i < n; i++) t.push(scores[i], ids[i]);
+    auto s = t.into_sorted();
+    for (size_t i = 0; i < s.size(); i++) {
+        out_s[i] = s[i].first;
+        out_i[i] = s[i].second;
+    }
+    return s.size();
+}
+size_t or_topk_u16(const uint16_t* scores, const uint32_t* ids, size_t n, size_t k, uint16_t* out_s, uint32_t* out_i) {
+    TopK<uint16_t> t(k);
+    for (size_t i = 0; i < n; i++) t.push(scores[i], ids[i]);
+    auto s = t.into_sorted();
+    for (size_t i = 0; i < s.size(); i++) {
+        out_s[i] = s[i].first;
+        out_i[i] = s[i].second;
+    }
+    return s.size();
+}
+
+void or_tier2_fit(const float* rows, size_t nrows, float* sigma) { tier2_fit(rows, nrows, sigma); }
+float or_tier2_encode(const float* sigma, const float* prefix, uint8_t* out) { return tier2_encode(sigma, prefix, out); }
+void or_tier2_tables(const float* sigma, const float* q, float* tables) { tier2_tables(sigma, q, tables); }
+float or_tier2_score(const float* tables, const uint8_t* code, float norm) { return tier2_score(tables, code, norm); }
+
+float or_prune_threshold(const float* est, const float* err, size_t n, size_t m, float w) {
+    std::vector<Cand> c(n);
+    for (size_t i = 0; i < n; i++) c[i] = {est[i], (uint32_t)i, err[i]};
+    return prune_threshold(c, m, w);
+}
+
+// ---- the index -------------------------------------------------------------------------
+void* or_index_create(const sieve_index_desc* d) {
+    Index* ix = new Index();
+    ix->k = d->k;
+    ix->n = d->n;
+    ix->nblocks = d->nblocks;
+    ix->centroids = d->centroids;
+    ix->block_off = d->block_off;
+    ix->blocks = d->blocks;
+    ix->ids = d->ids;
+    ix->factors = d->factors;
+    ix->rotation = d->rotation;
+    ix->rerank = d->rerank;
+    ix->quant = d->quant;
+    ix->norm256 = d->norm256;
+    ix->t2_sigma = d->tier2_sigma;
+    ix->t2_codes = d->tier2_codes;
+    ix->t2_norm = d->tier2_norm;
+    return ix;
+}
+void or_index_free(void* ix) { delete (Index*)ix; }
+
+static View make_view(const sieve_view* v, size_t qi) {
+    View out;
+    if (!v) return out;
+    out.deleted = v->deleted;
+    out.deleted_words = v->deleted_words;
+    if (v->allow && (v->allow_stride == 0 || !v->allow_flags || v->allow_flags[qi])) {
+        out.allow = v->allow + (v->allow_stride ? qi * v->allow_stride : 0);
+        out.allow_words = v->allow_words;
+    }
+    return out;
+}
+
+// route for one query; returns the number of probes written.
+size_t or_route(const void* h, const float* q256, const sieve_params* p, float* scores, uint32_t* clusters) {
+    const Index& ix = *(const Index*)h;
+    std::vector<float> cent;
+    auto probe = route(ix, q256, *p, cent);
+    for (size_t i = 0; i < probe.size(); i++) {
+        scores[i] = probe[i].score;
+        clusters[i] = probe[i].cluster;
+    }
+    return probe.size();
+}
+
+// candidates for one query over a given probe list; returns the number written.
+size_t or_candidates(const void* h, const float* q256, const float* pscores, const uint32_t* pclusters, size_t np,
+                     const sieve_params* p, const sieve_view* v, size_t qi, float* est, uint32_t* rows, float* errs,
+                     uint64_t* scanned) {
+    const Index& ix = *(const Index*)h;
+    std::vector<Probe> probe(np);
+    for (size_t i = 0; i < np; i++) probe[i] = {pscores[i], pclusters[i]};
+    uint64_t sc = 0;
+    auto c = candidates(ix, q256, probe, *p, make_view(v, qi), &sc);
+    if (scanned) *scanned += sc;
+    for (size_t i = 0; i < c.size(); i++) {
+        est[i] = c[i].est;
+        rows[i] = c[i].row;
+        errs[i] = c[i].err;
+    }
+    return c.size();
+}
+
+size_t or_rerank(const void* h, const float* q768, const float* q256, const float* est, const uint32_t* rows,
+                 const float* errs, size_t nc, const sieve_params* p, float* out_s, uint32_t* out_r, uint64_t* reranked) {
+    const Index& ix = *(const Index*)h;
+    std::vector<Cand> c(nc);
+    for (size_t i = 0; i < nc; i++) c[i] = {est[i], rows[i], errs[i]};
+    uint64_t rr = 0;
+    auto res = rerank(ix, q768, q256, c, *p, &rr);
+    if (reranked) *reranked += rr;
+    for (size_t i = 0; i < res.size(); i++) {
+        out_s[i] = res[i].first;
+        out_r[i] = res[i].second;
+    }
+    return res.size();
+}
+
+// Index::search_view for one query (src/ivf.rs:620-633).
+static size_t search_one(const Index& ix, const float* q768, const float* q256, const sieve_params& p, const View& v,
+                         float* out_s, uint32_t* out_r, sieve_timing* t) {
+    std::vector<float> cent;
+    auto probe = route(ix, q256, p, cent);
+    uint64_t scanned = 0, reranked = 0;
+    auto c = candidates(ix, q256, probe, p, v, &scanned);
+    auto res = rerank(ix, q768, q256, c, p, &reranked);
+    if (t) {
+        t->probed += probe.size();
+        t->scanned += scanned;
+        t->reranked += reranked;
+    }
+    for (size_t i = 0; i < res.size(); i++) {
+        out_s[i] = res[i].first;
+        out_r[i] = res[i].second;
+    }
+    return res.size();
+}
+
+// nq queries on `threads` workers, each taking whole queries from a shared counter, as the
+// reference's ivf_run (src/bin/search.rs:310-332). Outputs are nq x p->k, counts per query.
+void or_search_batch(const void* h, const float* q768, const float* q256, size_t nq, const sieve_params* p,
+                     const sieve_view* v, int threads, float* out_s, uint32_t* out_r, uint32_t* counts,
+                     sieve_timing* t) {
+    const Index& ix = *(const Index*)h;
+    std::atomic<size_t> next{0};
+    std::vector<sieve_timing> per(std::max(threads, 1));
+    for (auto& x : per) std::memset(&x, 0, sizeof x);
+    auto work = [&](int w) {
+        for (;;) {
+            size_t i = next.fetch_add(1);
+            if (i >= nq) break;
+            counts[i] = (uint32_t)search_one(ix, q768 + i * DIM, q256 + i * PDIM, *p, make_view(v, i),
+                                             out_s + i * p->k, out_r + i * p->k, &per[w]);
+        }
+    };
+    if (threads <= 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> th;
+        for (int w = 0; w < threads; w++) th.emplace_back(work, w);
+        for (auto& x : th) x.join();
+    }
+    if (t) {
+        for (auto& x : per) {
+            t->probed += x.probed;
+            t->scanned += x.scanned;
+            t->reranked += x.reranked;
+        }
+    }
+}
+
+// Single-thread latency of each query in ns (src/bin/search.rs:730-738).
+void or_search_latency(const void* h, const float* q768, const float* q256, size_t nq, const sieve_params* p,
+                       uint64_t* lat_ns) {
+    const Index& ix = *(const Index*)h;
+    std::vector<float> s(p->k);
+    std::vector<uint32_t> r(p->k);
+    View v;
+    for (size_t i = 0; i < nq; i++) {
+        auto t0 = std::chrono::steady_clock::now();
+        search_one(ix, q768 + i * DIM, q256 + i * PDIM, *p, v, s.data(), r.data(), nullptr);
+        lat_ns[i] = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count();
+    }
+}
+
+// ---- build-side restatements, for checking the product's index writer ----------------
+// assign_top, src/ivf.rs:137-158: the m best centroids of each row, best first.
+void or_assign_top(const float* rows, size_t n, const float* cents, size_t k, size_t m, uint32_t* out, int threads) {
+    const size_t B = 8192;
+    std::vector<float> s;
+    for (size_t r0 = 0; r0 < n; r0 += B) {
+        size_t rn = std::min(B, n - r0);
+        s.resize(rn * k);
+        gemm_abt(rows + r0 * PDIM, cents, rn, k, PDIM, s.data(), threads);
+        for (size_t i = 0; i < rn; i++) {
+            const float* row = s.data() + i * k;
+            if (m == 1) {
+                size_t best = 0;
+                float bv = std::numeric_limits<float>::lowest();
+                for (size_t j = 0; j < k; j++)
+                    if (row[j] > bv) {
+                        bv = row[j];
+                        best = j;
+                    }
+                out[(r0 + i)] = (uint32_t)best;
+                continue;
+            }
+            TopK<float> t(m);
+            for (size_t j = 0; j < k; j++) t.push(row[j], (uint32_t)j);
+            auto so = t.into_sorted();
+            for (size_t j = 0; j < m && j < so.size(); j++) out[(r0 + i) * m + j] = so[j].second;
+        }
+    }
+}
+
+// soar_secondary, src/ivf.rs:164-186.
+uint32_t or_soar_secondary(const float* x, const float* cents, const uint32_t* cand, size_t ncand, float lambda) {
+    const float* c1 = cents + (size_t)cand[0] * PDIM;
+    float x_c1 = dot_f32(x, c1, PDIM);
+    float r2 = std::fmax(2.0f - 2.0f * x_c1, 1e-12f);
+    float r_x = 1.0f - x_c1;
+    float best = std::numeric_limits<float>::max();
+    uint32_t bc = cand[1];
+    for (size_t i = 1; i < ncand; i++) {
+        const float* cv = cents + (size_t)cand[i] * PDIM;
+        float x_c = dot_f32(x, cv, PDIM);
+        float r_c = x_c - dot_f32(c1, cv, PDIM);
+        float along = r_x - r_c;
+        float loss = (2.0f - 2.0f * x_c) + lambda * along * along / r2;
+        if (loss < best) {
+            best = loss;
+            bc = cand[i];
+        }
+    }
+    return bc;
+}
+
+// ---- L2 path over 100-d int8 rows (src/bigann.rs) -----------------------------------
+// Index arrays: leaves k x 100 f32, leaf norms k, block_off k+1, blocks nblocks x 512 B,
+// factors nblocks x 32 x [mul, |x|^2], ids nblocks x 32, rotation 3 x 128 signs,
+// base n x 100 i8.
+struct L2Index {
+    uint32_t k;
+    const float* leaves;
+    const float* leaf_norm;
+    const uint64_t* block_off;
+    const uint8_t* blocks;
+    const float* factors;
+    const uint32_t* ids;
+    const float* rotation;
+    const int8_t* base;
+};
+
+void or_l2_leaf_norms(const float* leaves, size_t k, float* out) {  // norms, src/bigann.rs:135-137
+    for (size_t c = 0; c < k; c++) {
+        float s = rsum_seed();
+        for (int i = 0; i < 100; i++) s = s + leaves[c * 100 + i] * leaves[c * 100 + i];
+        out[c] = s;
+    }
+}
+
+void* or_l2_create(uint32_t k, const float* leaves, const float* leaf_norm, const uint64_t* block_off,
+                   const uint8_t* blocks, const float* factors, const uint32_t* ids, const float* rotation,
+                   const int8_t* base) {
+    return new L2Index{k, leaves, leaf_norm, block_off, blocks, factors, ids, rotation, base};
+}
+void or_l2_free(void* h) { delete (L2Index*)h; }
+
+// bigann::Index::search without refinement (src/bigann.rs:652-747): route, candidates, rerank.
+// Writes probes (count returned in *nprobe_out), candidates and the final top-k.
+size_t or_l2_search(const void* h, const int8_t* q, uint32_t nprobe, uint32_t cand, uint32_t k, float margin,
+                    uint32_t* probe_out, uint32_t* nprobe_out, uint32_t* cand_ids, float* cand_est,
+                    uint32_t* ncand_out, float* out_s, uint32_t* out_r) {
+    const L2Index& ix = *(const L2Index*)h;
+    const int D = 100, PD = 128, G = 32;
+    // route, src/bigann.rs:659-678
+    std::vector<float> qf(D);
+    for (int i = 0; i < D; i++) qf[i] = (float)q[i];
+    std::vector<float> dots(ix.k);
+    gemm_abt(qf.data(), ix.leaves, 1, ix.k, D, dots.data(), 1);
+    TopK<float> top(nprobe);
+    for (uint32_t c = 0; c < ix.k; c++) top.push(2.0f * dots[c] - ix.leaf_norm[c], c);
+    auto probe = top.into_sorted();
+    if (margin > 0.0f && !probe.empty()) {
+        float qq = rsum_seed();
+        for (int i = 0; i < D; i++) qq = qq + qf[i] * qf[i];
+        float lim = (qq - probe[0].first) * (1.0f + margin);
+        std::vector<std::pair<float, uint32_t>> kept;
+        for (auto& x : probe)
+            if (qq - x.first <= lim) kept.push_back(x);
+        probe.swap(kept);
+    }
+    *nprobe_out = (uint32_t)probe.size();
+    for (size_t i = 0; i < probe.size(); i++) probe_out[i] = probe[i].second;
+    // candidates, src/bigann.rs:683-734 (refine off)
+    std::vector<float> qp(PD, 0.0f);
+    for (int i = 0; i < D; i++) qp[i] = (float)q[i];
+    QueryState rq;
+    query_new(ix.rotation, PD, qp.data(), rq);
+    std::vector<std::pair<uint32_t, float>> info;  // (id, |x|^2); est lives in the TopK
+    std::vector<float> info_est;
+    TopK<float> ct(cand);
+    std::vector<uint16_t> scores;
+    for (auto& pr : probe) {
+        uint32_t c = pr.second;
+        float cent = dots[c];
+        uint64_t b0 = ix.block_off[c], b1 = ix.block_off[c + 1];
+        size_t ns = (size_t)(b1 - b0) * BLOCK;
+        scores.resize(ns);
+        scan(rq.tables.data(), G, ix.blocks + b0 * 512, b1 - b0, scores.data());
+        const float* f = ix.factors + b0 * BLOCK * 2;
+        const uint32_t* ids = ix.ids + b0 * BLOCK;
+        for (size_t j = 0; j < ns; j++) {
+            if (ids[j] == PAD) continue;
+            float mul = f[2 * j], nx = f[2 * j + 1];
+            float est = 2.0f * (cent + mul * (rq.a * (float)scores[j] + rq.b)) - nx;
+            if (!ct.has_thr || est > ct.thr) {
+                ct.push(est, (uint32_t)info.size());
+                info.emplace_back(ids[j], nx);
+                info_est.push_back(est);
+            }
+        }
+    }
+    auto cs = ct.into_sorted();
+    *ncand_out = (uint32_t)cs.size();
+    // rerank, src/bigann.rs:736-747
+    TopK<float> out(k);
+    for (size_t i = 0; i < cs.size(); i++) {
+        uint32_t id = info[cs[i].second].first;
+        float nx = info[cs[i].second].second;
+        cand_ids[i] = id;
+        cand_est[i] = info_est[cs[i].second];
+        const int8_t* x = ix.base + (size_t)id * D;
+        int32_t dot = 0;
+        for (int j = 0; j < D; j++) dot += (int32_t)q[j] * (int32_t)x[j];
+        out.push(2.0f * (float)dot - nx, id);
+    }
+    auto res = out.into_sorted();
+    for (size_t i = 0; i < res.size(); i++) {
+        out_s[i] = res[i].first;
+        out_r[i] = res[i].second;
+    }
+    return res.size();
+}
+
+}  // extern "C"
