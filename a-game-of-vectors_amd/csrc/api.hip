@@ -119,6 +119,7 @@ struct sieve_stream {
     int device = 0;
     DevBuf b[B_NSLOTS];
     hipEvent_t ev[8] = {};
+    sieve_stats last{};
     ~sieve_stream() {
         for (auto& x : b) x.release();
         for (auto& e : ev)
@@ -444,6 +445,15 @@ sieve_index* create_index(const sieve_index_desc* d) {
         if (tmp) (void)hipFree(tmp);
     }
     ix->ids = upload(d->ids, ix->nslots, acct);
+    {
+        unsigned long long* bad = dmalloc<unsigned long long>(1, nullptr);
+        SIEVE_HIP(hipMemset(bad, 0, 8));
+        launch_count_bad_ids(ix->ids, ix->nslots, d->n, bad, st);
+        unsigned long long hbad = 0;
+        SIEVE_HIP(hipMemcpy(&hbad, bad, 8, hipMemcpyDeviceToHost));
+        (void)hipFree(bad);
+        SIEVE_REQUIRE(hbad == 0, SIEVE_EINVAL, std::to_string(hbad) + " slot ids are beyond the corpus (ids.u32)");
+    }
     ix->rotation = upload(d->rotation, 3 * PDIM, acct);
     if (d->borrow_rerank && on_device(d->rerank)) {
         ix->rerank = const_cast<int8_t*>(d->rerank);
@@ -522,6 +532,13 @@ int sieve_stream_sync(sieve_stream* s) {
 }
 
 void* sieve_stream_handle(sieve_stream* s) { return s ? (void*)s->st : nullptr; }
+
+int sieve_stream_last_stats(const sieve_stream* s, sieve_stats* out) {
+    return guarded([&] {
+        SIEVE_REQUIRE(s && out, SIEVE_EINVAL, "null argument");
+        *out = s->last;
+    });
+}
 
 int sieve_gemm_abt(const float* a, const float* b, size_t m, size_t n, size_t k, float* c) {
     return guarded([&] {
@@ -653,7 +670,10 @@ int sieve_route(const sieve_index* ix, sieve_stream* s, const float* q256, size_
         out_done(s, oc);
         out_done(s, on);
         check_overflow(s);
-        if (t) t->route += (uint64_t)elapsed_ns(s->ev[0], s->ev[1]);
+        s->last = sieve_stats{};
+        s->last.queries = nq;
+        s->last.route_ns = elapsed_ns(s->ev[0], s->ev[1]);
+        if (t) t->route += (uint64_t)s->last.route_ns;
     });
 }
 
@@ -704,10 +724,16 @@ int sieve_candidates(const sieve_index* ix, sieve_stream* s, const float* q256, 
         unsigned long long hcnt[4] = {0, 0, 0, 0};
         SIEVE_HIP(hipMemcpyAsync(hcnt, cnt, 32, hipMemcpyDeviceToHost, s->st));
         SIEVE_HIP(hipStreamSynchronize(s->st));
+        s->last = sieve_stats{};
+        s->last.queries = nq;
+        s->last.scanned = hcnt[0];
+        s->last.id_reads = hcnt[2];
+        for (size_t i = 0; i < nq; i++) s->last.probed += hn[i];
+        s->last.scan_ns = elapsed_ns(s->ev[0], s->ev[1]);
         if (t) {
-            t->scan += (uint64_t)elapsed_ns(s->ev[0], s->ev[1]);
+            t->scan += (uint64_t)s->last.scan_ns;
             t->scanned += hcnt[0];
-            for (size_t i = 0; i < nq; i++) t->probed += hn[i];
+            t->probed += s->last.probed;
         }
     });
 }
@@ -755,8 +781,12 @@ int sieve_rerank(const sieve_index* ix, sieve_stream* s, const float* q768, cons
         unsigned long long hcnt[4] = {0, 0, 0, 0};
         SIEVE_HIP(hipMemcpyAsync(hcnt, cnt, 32, hipMemcpyDeviceToHost, s->st));
         SIEVE_HIP(hipStreamSynchronize(s->st));
+        s->last = sieve_stats{};
+        s->last.queries = nq;
+        s->last.reranked = hcnt[1];
+        s->last.rerank_ns = elapsed_ns(s->ev[0], s->ev[1]);
         if (t) {
-            t->rerank += (uint64_t)elapsed_ns(s->ev[0], s->ev[1]);
+            t->rerank += (uint64_t)s->last.rerank_ns;
             t->reranked += hcnt[1];
         }
     });
@@ -799,16 +829,25 @@ int sieve_search_batch(const sieve_index* ix, sieve_stream* s, const float* q768
         out_done(s, on);
         unsigned long long hcnt[4] = {0, 0, 0, 0};
         SIEVE_HIP(hipMemcpyAsync(hcnt, cnt, 32, hipMemcpyDeviceToHost, s->st));
-        std::vector<uint32_t> hpn(t ? nq : 0);
-        if (t) SIEVE_HIP(hipMemcpyAsync(hpn.data(), pn, nq * 4, hipMemcpyDeviceToHost, s->st));
+        std::vector<uint32_t> hpn(nq);
+        SIEVE_HIP(hipMemcpyAsync(hpn.data(), pn, nq * 4, hipMemcpyDeviceToHost, s->st));
         check_overflow(s);
+        s->last = sieve_stats{};
+        s->last.queries = nq;
+        s->last.scanned = hcnt[0];
+        s->last.id_reads = hcnt[2];
+        s->last.reranked = hcnt[1];
+        for (uint32_t x : hpn) s->last.probed += x;
+        s->last.route_ns = elapsed_ns(s->ev[0], s->ev[1]);
+        s->last.scan_ns = elapsed_ns(s->ev[1], s->ev[2]);
+        s->last.rerank_ns = elapsed_ns(s->ev[2], s->ev[3]);
         if (t) {
-            t->route += (uint64_t)elapsed_ns(s->ev[0], s->ev[1]);
-            t->scan += (uint64_t)elapsed_ns(s->ev[1], s->ev[2]);
-            t->rerank += (uint64_t)elapsed_ns(s->ev[2], s->ev[3]);
+            t->route += (uint64_t)s->last.route_ns;
+            t->scan += (uint64_t)s->last.scan_ns;
+            t->rerank += (uint64_t)s->last.rerank_ns;
             t->scanned += hcnt[0];
             t->reranked += hcnt[1];
-            for (uint32_t x : hpn) t->probed += x;
+            t->probed += s->last.probed;
         }
     });
 }

@@ -26,6 +26,8 @@ void launch_transpose_blocks(const uint8_t* blocks, size_t nblocks, uint4* codes
 void launch_split_factors(const float* factors, size_t nslots, float* mul, float* err, hipStream_t st);
 void launch_to_bf16(const float* x, size_t n, uint16_t* y, hipStream_t st);
 void launch_cluster_slots(const uint32_t* block_off, uint32_t k, uint32_t* slots, hipStream_t st);
+// Counts slot ids that are neither PAD nor a row below n (a corrupt ids.u32).
+void launch_count_bad_ids(const uint32_t* ids, size_t nslots, uint64_t n, unsigned long long* bad, hipStream_t st);
 
 // C = A * B^T in f32 (m x k, n x k -> m x n, row-major), f32 MFMA.
 void launch_gemm_f32_abt(const float* a, const float* b, float* c, int m, int n, int k, hipStream_t st);
@@ -81,7 +83,7 @@ struct ScanArgs {
     float* out_err;
     uint32_t* out_n;
     int out_stride;
-    unsigned long long* scanned; // += stored slots scanned
+    unsigned long long* scanned; // [0] += stored slots scanned, [2] += slots past the threshold
 };
 size_t scan_lds_bytes(int cand, int probe_stride);
 void launch_scan(const ScanArgs& a, size_t nq, hipStream_t st);

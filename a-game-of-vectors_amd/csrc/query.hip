@@ -204,4 +204,18 @@ void launch_cluster_slots(const uint32_t* block_off, uint32_t k, uint32_t* slots
     hipLaunchKernelGGL(k_cluster_slots, dim3((k + 255) / 256), dim3(256), 0, st, block_off, k, slots);
 }
 
+__global__ void k_count_bad_ids(const uint32_t* __restrict__ ids, size_t nslots, uint64_t n,
+                                unsigned long long* __restrict__ bad) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nslots) {
+        uint32_t id = ids[i];
+        if (id != PAD && (uint64_t)id >= n) atomicAdd(bad, 1ull);
+    }
+}
+
+void launch_count_bad_ids(const uint32_t* ids, size_t nslots, uint64_t n, unsigned long long* bad, hipStream_t st) {
+    if (nslots == 0) return;
+    hipLaunchKernelGGL(k_count_bad_ids, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, st, ids, nslots, n, bad);
+}
+
 }  // namespace sieve

@@ -266,10 +266,13 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
     }
     __syncthreads();
     block_sort_desc(sorted, n2);
-    if (t == 0) {
+    if (t == 0 && nc == 0) {
+        a.out_n[qi] = 0;
+        sh[2] = 0;
+    } else if (t == 0) {
         // The top n, then the margin cut (keep s >= best - margin) and the scan budget
         // (take clusters while the stored slots so far fit; at least one).
-        int keep = n;
+        int keep = min(n, nc);  // nc < n only for non-finite scores
         const float best = key2f(sorted[0].x);
         if (a.margin == a.margin) {
             const float lim = best - a.margin;

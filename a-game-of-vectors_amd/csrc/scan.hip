@@ -67,7 +67,10 @@ __device__ void scan_compact(uint2* buf, uint2* tmp, int n, int k, uint32_t* his
     __syncthreads();
     for (int i = t; i < n; i += SCAN_THREADS) {
         uint2 e = buf[i];
-        if (ge2(e, kth)) tmp[atomicAdd(&misc[3], 1u)] = e;
+        if (ge2(e, kth)) {
+            uint32_t pos = atomicAdd(&misc[3], 1u);
+            if (pos < (uint32_t)k) tmp[pos] = e;  // k exactly when entries are unique
+        }
     }
     __syncthreads();
     for (int i = t; i < k; i += SCAN_THREADS) buf[i] = tmp[i];
@@ -124,6 +127,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
     if (a.view.allow && (a.view.allow_stride == 0 || !a.view.allow_flags || a.view.allow_flags[qi]))
         allow = a.view.allow + qi * a.view.allow_stride;
     const int cand = a.cand;
+    uint32_t id_reads = 0;  // slots past the threshold: their id is read
 
     for (uint32_t vb0 = 0; vb0 < total; vb0 += ROUND_BLOCKS) {
         uint4 c0[SLOTS_PER_THREAD], c1[SLOTS_PER_THREAD];
@@ -169,6 +173,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
                 key = f2key(est);
                 pass = !has_thr || key > thr;
                 if (pass) {
+                    id_reads++;
                     uint32_t id = a.ids[gs[u]];
                     pass = id != PAD;
                     if (pass && a.view.deleted && bit_set(a.view.deleted, a.view.deleted_words, id)) pass = false;
@@ -190,6 +195,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
         __syncthreads();  // every wave has read the count before the next round appends
         if (count >= 2 * cand) scan_compact(buf, tmp, count, cand, hist, misc);
     }
+    id_reads = wave_sum(id_reads);
+    if (lane == 0 && a.scanned) atomicAdd(a.scanned + 2, (unsigned long long)id_reads);
     if ((int)misc[0] > cand) scan_compact(buf, tmp, (int)misc[0], cand, hist, misc);
 
     // Sort the kept slots by (row asc, est desc) and keep the first of each row.

@@ -54,6 +54,15 @@ class View(ctypes.Structure):
                 ("allow_flags", ctypes.c_void_p)]
 
 
+class Stats(ctypes.Structure):
+    _fields_ = [("queries", ctypes.c_uint64), ("probed", ctypes.c_uint64), ("scanned", ctypes.c_uint64),
+                ("id_reads", ctypes.c_uint64), ("reranked", ctypes.c_uint64), ("route_ns", ctypes.c_double),
+                ("scan_ns", ctypes.c_double), ("rerank_ns", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class IndexInfo(ctypes.Structure):
     _fields_ = [("k", ctypes.c_uint32), ("n", ctypes.c_uint64), ("nblocks", ctypes.c_uint64),
                 ("rabitq", ctypes.c_uint32), ("tier2", ctypes.c_uint32), ("max_cluster_blocks", ctypes.c_uint32),
@@ -84,6 +93,7 @@ def lib():
         "sieve_stream_destroy": (None, [vp]),
         "sieve_stream_sync": (i32, [vp]),
         "sieve_stream_handle": (vp, [vp]),
+        "sieve_stream_last_stats": (i32, [vp, ctypes.POINTER(Stats)]),
         "sieve_gemm_abt": (i32, [vp, vp, sz, sz, sz, vp]),
         "sieve_lut_scan": (i32, [vp, u32, vp, sz, vp]),
         "sieve_index_open": (i32, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]),

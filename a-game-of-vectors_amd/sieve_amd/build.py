@@ -46,10 +46,23 @@ def seq_sum_sq(x):
     return acc
 
 
+def f32_div(a, b):
+    """Correctly rounded f32 a / b: the f64 quotient rounded once to f32 (f64 carries more than
+    2 * 24 + 2 bits, so the double rounding is exact). torch's own f32 paths evaluate
+    `scalar / tensor` as reciprocal * scalar and may use approximate device division."""
+    a = a if torch.is_tensor(a) else torch.tensor(a, dtype=torch.float64, device=b.device)
+    return (a.double() / b.double()).float()
+
+
+def f32_sqrt(a):
+    """Correctly rounded f32 sqrt (torch's vectorised f32 sqrt is not always)."""
+    return a.double().sqrt().float()
+
+
 def normalize_rows(x):
     """quant::normalize (src/quant.rs:5-13) per row; returns (unit rows, norms)."""
-    n = torch.sqrt(seq_sum_sq(x))
-    inv = torch.where(n > 0, 1.0 / n, torch.ones_like(n))
+    n = f32_sqrt(seq_sum_sq(x))
+    inv = torch.where(n > 0, f32_div(1.0, n), torch.ones_like(n))
     return x * inv[:, None], n
 
 
@@ -127,12 +140,12 @@ def rabitq_encode(xr, cr):
         l1 = l1 + torch.abs(c)
     code = pack_bits(r > 0)
     df = float(d)
-    nr = torch.sqrt(nr2)
-    od = l1 / (nr * np.float32(np.sqrt(np.float32(df))))
-    inner = (1.0 - od * od) / (od * od * np.float32(df - 1.0))
+    nr = f32_sqrt(nr2)
+    od = f32_div(l1, nr * np.float32(np.sqrt(np.float32(df))))
+    inner = f32_div(1.0 - od * od, od * od * np.float32(df - 1.0))
     inner = torch.where(inner > 0, inner, torch.zeros_like(inner))
-    err = nr * np.float32(EPS0) * torch.sqrt(inner)
-    mul = nr2 / l1
+    err = nr * np.float32(EPS0) * f32_sqrt(inner)
+    mul = f32_div(nr2, l1)
     zero = l1 < 1e-12
     mul = torch.where(zero, torch.zeros_like(mul), mul)
     err = torch.where(zero, torch.zeros_like(err), err)
@@ -173,7 +186,7 @@ def ingest_batch(emb, quant):
     codes = pack_bits(p > 0)
     if quant is None:
         maxabs = torch.clamp(v.abs().amax(0), min=1e-6)                 # Int8Quantizer::fit
-        quant = 127.0 / maxabs
+        quant = f32_div(127.0, maxabs)
     q = round_half_away(v * quant).clamp(-127, 127).to(torch.int8)
     return q, quant, norm, codes
 
@@ -193,7 +206,7 @@ def ingest(mix: synth.Mixture, n, batch=65536):
 
 def prefix_f32(corpus: Corpus, rows):
     """Corpus::prefix_f32 (src/ivf.rs:110-116): the unit 256-d prefix rebuilt from int8 rows."""
-    x = corpus.rerank[rows, :PDIM].to(torch.float32) / corpus.quant[:PDIM]
+    x = f32_div(corpus.rerank[rows, :PDIM].to(torch.float32), corpus.quant[:PDIM])
     return normalize_rows(x)[0]
 
 
@@ -255,7 +268,7 @@ def soar_secondary(x, cents, cand, lam):
     x_c = dot_f32_rows(x[:, None, :].expand_as(cv), cv)
     r_c = x_c - dot_f32_rows(c1[:, None, :].expand_as(cv), cv)
     along = r_x[:, None] - r_c
-    loss = (2.0 - 2.0 * x_c) + np.float32(lam) * along * along / r2[:, None]
+    loss = (2.0 - 2.0 * x_c) + f32_div(np.float32(lam) * along * along, r2[:, None].expand_as(along))
     best = torch.argmin(loss, dim=1)
     return cand[:, 1:].gather(1, best[:, None])[:, 0]
 
@@ -265,7 +278,7 @@ def tier2_fit(sample):
     """Tier2::fit (src/tier2.rs:23-31)."""
     n = float(np.float32(max(sample.shape[0], 1)))
     ss = (sample * sample).double().sum(0)
-    return torch.clamp((ss / n).sqrt().float(), min=1e-6)
+    return torch.clamp((ss / torch.full_like(ss, n)).sqrt().float(), min=1e-6)
 
 
 def tier2_encode(prefix, sigma):
@@ -279,7 +292,7 @@ def tier2_encode(prefix, sigma):
         nn = nn + v[:, i] * v[:, i]
     sh = (2 * torch.arange(4, device=prefix.device, dtype=torch.int32))
     codes = (lv.view(-1, PDIM // 4, 4).to(torch.int32) << sh).sum(-1).to(torch.uint8)
-    return codes, torch.sqrt(nn)
+    return codes, f32_sqrt(nn)
 
 
 # ----------------------------------------------------------------------- build ---------

@@ -31,6 +31,12 @@ class Stream:
     def sync(self):
         abi.check(abi.lib().sieve_stream_sync(self._h))
 
+    def last_stats(self):
+        """Counters of the last call on this stream (sieve_stats) as a dict."""
+        st = abi.Stats()
+        abi.check(abi.lib().sieve_stream_last_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
     @property
     def handle(self):
         return abi.lib().sieve_stream_handle(self._h)

@@ -136,6 +136,20 @@ typedef struct sieve_view {
     const uint8_t* allow_flags;
 } sieve_view;
 
+/* Counters of the last call on a stream (no counterpart in the reference): what the kernels
+ * read, for roofline accounting. Times are GPU nanoseconds between HIP events recorded on the
+ * stream around each stage's kernels. */
+typedef struct sieve_stats {
+    uint64_t queries;
+    uint64_t probed;     /* clusters scanned, summed over queries                        */
+    uint64_t scanned;    /* stored slots scanned (32 code bytes + 4 mul bytes each)       */
+    uint64_t id_reads;   /* slots past the admission threshold (their id is read)        */
+    uint64_t reranked;   /* candidates reranked after pruning                            */
+    double route_ns;
+    double scan_ns;
+    double rerank_ns;
+} sieve_stats;
+
 typedef struct sieve_index sieve_index;
 typedef struct sieve_stream sieve_stream;
 
@@ -160,6 +174,8 @@ void sieve_stream_destroy(sieve_stream* s);
 int sieve_stream_sync(sieve_stream* s);
 /* The underlying hipStream_t, for callers that order their own work with it. */
 void* sieve_stream_handle(sieve_stream* s);
+/* Counters of the last route / candidates / rerank / search call on this stream. */
+int sieve_stream_last_stats(const sieve_stream* s, sieve_stats* out);
 
 /* ---------------------------------------------------------- kernel seams --- */
 
