@@ -109,9 +109,35 @@ struct sieve_index {
     }
 };
 
+struct sieve_l2_index {
+    int device = 0;
+    uint32_t k = 0;
+    uint64_t n = 0, nblocks = 0, nslots = 0;
+    float* leaves = nullptr;       // k x 100 f32
+    uint16_t* leaves_bf = nullptr; // k x 256 BF16 [hi | lo]
+    float* leaf_norm = nullptr;    // k: |c|^2 summed left to right
+    float cent_norm_max = 0.0f;
+    uint32_t* block_off = nullptr;
+    uint4* codes = nullptr;        // 16 bytes per slot
+    float* mul = nullptr;
+    float* nx = nullptr;
+    uint32_t* ids = nullptr;
+    float* rotation = nullptr;     // 3 x 128
+    int8_t* base = nullptr;
+    bool base_owned = true;
+    size_t device_bytes = 0;
+    ~sieve_l2_index() {
+        void* ps[] = {leaves, leaves_bf, leaf_norm, block_off, codes, mul, nx, ids, rotation};
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+        if (base && base_owned) (void)hipFree(base);
+    }
+};
+
 enum Slot {
     B_Q256, B_Q768, B_TABLES, B_QAB, B_WQ768, B_WQ256, B_INV, B_QBF, B_QNORM, B_APPROX, B_PS, B_PC, B_PN, B_CE, B_CR,
-    B_CX, B_CN, B_OS, B_OR, B_ON, B_DEL, B_ALLOW, B_AFLAGS, B_CNT, B_OVF, B_TMP1, B_TMP2, B_TMP3, B_NSLOTS
+    B_CX, B_CN, B_OS, B_OR, B_ON, B_DEL, B_ALLOW, B_AFLAGS, B_CNT, B_OVF, B_TMP1, B_TMP2, B_TMP3, B_QQ, B_QF,
+    B_NSLOTS
 };
 
 struct sieve_stream {
@@ -268,6 +294,11 @@ void route_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, size_t
         ra.out_n = pn + q0;
         ra.out_stride = stride;
         ra.overflow = ovf;
+        ra.l2 = 0;
+        ra.dims = PDIM;
+        ra.cstride = PDIM;
+        const float u = 1.0f / 256.0f;  // BF16 unit roundoff
+        ra.err_coef = (2.0f * u + u * u + 256.0f / 16777216.0f) * ix->cent_norm_max;
         launch_route_select(ra, m, s->st);
     }
     SIEVE_HIP(hipGetLastError());
@@ -424,7 +455,7 @@ sieve_index* create_index(const sieve_index_desc* d) {
             SIEVE_HIP(hipMemcpy(stage, src, nb * BLOCK_BYTES, hipMemcpyHostToDevice));
             src = stage;
         }
-        launch_transpose_blocks(src, nb, ix->codes + b0 * BLOCK * 2, st);
+        launch_transpose_blocks(src, nb, 64, ix->codes + b0 * BLOCK * 2, st);
         SIEVE_HIP(hipDeviceSynchronize());
     }
     if (stage) (void)hipFree(stage);
@@ -467,6 +498,98 @@ sieve_index* create_index(const sieve_index_desc* d) {
         ix->t2_sigma = upload(d->tier2_sigma, PDIM, acct);
         ix->t2_codes = upload(d->tier2_codes, (size_t)d->n * 64, acct);
         ix->t2_norm = upload(d->tier2_norm, d->n, acct);
+    }
+    SIEVE_HIP(hipDeviceSynchronize());
+    return ix.release();
+}
+
+sieve_l2_index* create_l2_index(const sieve_l2_desc* d) {
+    constexpr int D = 100, PD = 128, BB = 512;
+    SIEVE_REQUIRE(d, SIEVE_EINVAL, "desc is null");
+    SIEVE_REQUIRE(d->k >= 1, SIEVE_EINVAL, "k must be >= 1");
+    SIEVE_REQUIRE(d->leaves && d->block_off && d->blocks && d->ids && d->factors && d->rotation && d->base,
+                  SIEVE_EINVAL, "desc is missing a required array");
+    SIEVE_REQUIRE(d->nblocks * BLOCK < (1ull << 32) - 1, SIEVE_ELIMIT, "more than 2^32 - 1 slots");
+    std::unique_ptr<sieve_l2_index> ix(new sieve_l2_index());
+    SIEVE_HIP(hipGetDevice(&ix->device));
+    ix->k = d->k;
+    ix->n = d->n;
+    ix->nblocks = d->nblocks;
+    ix->nslots = d->nblocks * BLOCK;
+    size_t* acct = &ix->device_bytes;
+    HostCopy hb, hc;
+    const uint64_t* bo = host_view(d->block_off, d->k + 1, hb);
+    SIEVE_REQUIRE(bo[0] == 0 && bo[d->k] == d->nblocks, SIEVE_EINVAL, "block_off does not match nblocks");
+    std::vector<uint32_t> bo32(d->k + 1);
+    for (uint32_t c = 0; c <= d->k; c++) {
+        if (c) SIEVE_REQUIRE(bo[c] >= bo[c - 1], SIEVE_EINVAL, "block_off is not monotone");
+        bo32[c] = (uint32_t)bo[c];
+    }
+    const float* lv = host_view(d->leaves, (size_t)d->k * D, hc);
+    // norms (src/bigann.rs:135-137): sum x*x left to right from -0.0
+    std::vector<float> ln(d->k);
+    double mx = 0.0;
+    for (uint32_t c = 0; c < d->k; c++) {
+        float s = -0.0f;
+        double s64 = 0.0;
+        for (int i = 0; i < D; i++) {
+            float x = lv[(size_t)c * D + i];
+            s = s + x * x;
+            s64 += (double)x * x;
+        }
+        ln[c] = s;
+        mx = std::max(mx, std::sqrt(s64));
+    }
+    ix->cent_norm_max = (float)(mx * (1.0 + 1e-6));
+    hipStream_t st = nullptr;
+    ix->leaves = upload(lv, (size_t)d->k * D, acct);
+    ix->leaves_bf = dmalloc<uint16_t>((size_t)d->k * PDIM, acct);
+    launch_split_leaves(ix->leaves, d->k, ix->leaves_bf, st);
+    ix->leaf_norm = upload(ln.data(), d->k, acct);
+    ix->block_off = upload(bo32.data(), d->k + 1, acct);
+    ix->codes = dmalloc<uint4>(ix->nslots, acct);
+    {
+        const uint8_t* src = d->blocks;
+        uint8_t* tmp = nullptr;
+        if (!on_device(src)) {
+            tmp = dmalloc<uint8_t>(d->nblocks * BB, nullptr);
+            SIEVE_HIP(hipMemcpy(tmp, src, d->nblocks * BB, hipMemcpyHostToDevice));
+            src = tmp;
+        }
+        launch_transpose_blocks(src, d->nblocks, 32, ix->codes, st);
+        SIEVE_HIP(hipDeviceSynchronize());
+        if (tmp) (void)hipFree(tmp);
+    }
+    ix->mul = dmalloc<float>(ix->nslots, acct);
+    ix->nx = dmalloc<float>(ix->nslots, acct);
+    {
+        const float* f = d->factors;
+        float* tmp = nullptr;
+        if (!on_device(f)) {
+            tmp = dmalloc<float>(ix->nslots * 2, nullptr);
+            SIEVE_HIP(hipMemcpy(tmp, f, ix->nslots * 8, hipMemcpyHostToDevice));
+            f = tmp;
+        }
+        launch_split_factors(f, ix->nslots, ix->mul, ix->nx, st);
+        SIEVE_HIP(hipDeviceSynchronize());
+        if (tmp) (void)hipFree(tmp);
+    }
+    ix->ids = upload(d->ids, ix->nslots, acct);
+    {
+        unsigned long long* bad = dmalloc<unsigned long long>(1, nullptr);
+        SIEVE_HIP(hipMemset(bad, 0, 8));
+        launch_count_bad_ids(ix->ids, ix->nslots, d->n, bad, st);
+        unsigned long long hbad = 0;
+        SIEVE_HIP(hipMemcpy(&hbad, bad, 8, hipMemcpyDeviceToHost));
+        (void)hipFree(bad);
+        SIEVE_REQUIRE(hbad == 0, SIEVE_EINVAL, std::to_string(hbad) + " slot ids are beyond the base");
+    }
+    ix->rotation = upload(d->rotation, 3 * PD, acct);
+    if (d->borrow_base && on_device(d->base)) {
+        ix->base = const_cast<int8_t*>(d->base);
+        ix->base_owned = false;
+    } else {
+        ix->base = upload(d->base, (size_t)d->n * D, acct);
     }
     SIEVE_HIP(hipDeviceSynchronize());
     return ix.release();
@@ -847,6 +970,163 @@ int sieve_search_batch(const sieve_index* ix, sieve_stream* s, const float* q768
             t->rerank += (uint64_t)s->last.rerank_ns;
             t->scanned += hcnt[0];
             t->reranked += hcnt[1];
+            t->probed += s->last.probed;
+        }
+    });
+}
+
+int sieve_l2_index_create(const sieve_l2_desc* desc, sieve_l2_index** out) {
+    return guarded([&] {
+        SIEVE_REQUIRE(out, SIEVE_EINVAL, "out is null");
+        *out = create_l2_index(desc);
+    });
+}
+
+void sieve_l2_index_free(sieve_l2_index* ix) { delete ix; }
+
+int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_t* q, size_t nq,
+                          const sieve_l2_params* p, float* scores, uint32_t* rows, uint32_t* counts,
+                          uint32_t* probes, uint32_t* probe_counts, uint32_t* cand_rows, float* cand_est,
+                          uint32_t* cand_counts, sieve_timing* t) {
+    return guarded([&] {
+        constexpr int D = 100;
+        SIEVE_REQUIRE(ix && s && q && p && scores && rows && counts, SIEVE_EINVAL, "null argument");
+        SIEVE_REQUIRE(p->refine == 0, SIEVE_EINVAL, "multi-bit refinement is not supported (refine must be 0)");
+        SIEVE_REQUIRE(p->nprobe >= 1 && p->nprobe <= (uint32_t)MAX_PROBE, SIEVE_ELIMIT,
+                      "nprobe must be in [1, " + std::to_string(MAX_PROBE) + "]");
+        SIEVE_REQUIRE(p->cand >= 1 && p->cand <= (uint32_t)MAX_CAND, SIEVE_ELIMIT,
+                      "cand must be in [1, " + std::to_string(MAX_CAND) + "]");
+        SIEVE_REQUIRE(p->k >= 1 && p->k <= (uint32_t)MAX_CAND, SIEVE_ELIMIT, "k out of range");
+        if (nq == 0) return;
+        const int pstride = (int)std::min<uint32_t>(p->nprobe, ix->k);
+        const int8_t* dq = dev_in(s, B_Q768, q, nq * D);
+        L2Prep prep;
+        prep.tables = grow<uint8_t>(s->b[B_TABLES], nq * 512);
+        prep.qab = grow<float4>(s->b[B_QAB], nq);
+        prep.qbf = grow<uint16_t>(s->b[B_QBF], nq * PDIM);
+        prep.qf = grow<float>(s->b[B_QF], nq * D);
+        prep.qq = grow<float>(s->b[B_QQ], nq);
+        prep.qnorm = grow<float>(s->b[B_QNORM], nq);
+        unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 4);
+        uint32_t* ovf = grow<uint32_t>(s->b[B_OVF], 1);
+        float* ps = grow<float>(s->b[B_PS], nq * pstride);
+        uint32_t* pc = grow<uint32_t>(s->b[B_PC], nq * pstride);
+        uint32_t* pn = grow<uint32_t>(s->b[B_PN], nq);
+        float* ce = grow<float>(s->b[B_CE], nq * p->cand);
+        uint32_t* cr = grow<uint32_t>(s->b[B_CR], nq * p->cand);
+        float* cx = grow<float>(s->b[B_CX], nq * p->cand);
+        uint32_t* cn = grow<uint32_t>(s->b[B_CN], nq);
+        DevOut<float> os = dev_out(s, B_OS, scores, nq * p->k);
+        DevOut<uint32_t> orr = dev_out(s, B_OR, rows, nq * p->k);
+        DevOut<uint32_t> on = dev_out(s, B_ON, counts, nq);
+        SIEVE_HIP(hipMemsetAsync(cnt, 0, 32, s->st));
+        SIEVE_HIP(hipMemsetAsync(ovf, 0, 4, s->st));
+        SIEVE_HIP(hipEventRecord(s->ev[0], s->st));
+        // 128^-1.5 as (d as f32).powf(-1.5) (src/rabitq.rs:67)
+        launch_query_prep_l2(dq, nq, ix->rotation, std::pow(128.0f, -1.5f), prep, s->st);
+        const size_t per = std::max<size_t>(1, std::min<size_t>(nq, (size_t(1) << 30) / (4 * (size_t)ix->k)));
+        float* approx = grow<float>(s->b[B_APPROX], per * ix->k);
+        for (size_t q0 = 0; q0 < nq; q0 += per) {
+            size_t m = std::min(per, nq - q0);
+            launch_gemm_bf16_route(prep.qbf + q0 * PDIM, ix->leaves_bf, approx, (int)m, (int)ix->k, ix->k, s->st);
+            RouteArgs ra{};
+            ra.approx = approx;
+            ra.lds = ix->k;
+            ra.k = (int)ix->k;
+            ra.cents = ix->leaves;
+            ra.q256 = nullptr;
+            ra.qnorm = prep.qnorm + q0;
+            ra.cl_slots = nullptr;
+            ra.cent_norm_max = ix->cent_norm_max;
+            ra.nprobe = (int)p->nprobe;
+            ra.margin = p->margin;
+            ra.budget = 0;
+            ra.out_s = ps + q0 * pstride;
+            ra.out_c = pc + q0 * pstride;
+            ra.out_n = pn + q0;
+            ra.out_stride = pstride;
+            ra.overflow = ovf;
+            ra.l2 = 1;
+            ra.dims = D;
+            ra.cstride = D;
+            ra.cnorm = ix->leaf_norm;
+            ra.qf = prep.qf + q0 * D;
+            ra.qq = prep.qq + q0;
+            // 2 <q, c> from BF16 hi + lo halves: |err| <= 2 (2^-16 + 256 * 2^-24 * 1.01) |q| |c|
+            ra.err_coef = 2.0f * (1.0f / 65536.0f + 1.01f * 256.0f / 16777216.0f) * ix->cent_norm_max;
+            launch_route_select(ra, m, s->st);
+        }
+        SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
+        // In L2 mode the probe scores are <q, c> (the estimator's `cent`, src/bigann.rs:701).
+        ScanArgs a{};
+        a.codes = ix->codes;
+        a.mul = ix->mul;
+        a.err = ix->nx;
+        a.ids = ix->ids;
+        a.block_off = ix->block_off;
+        a.tables = prep.tables;
+        a.qab = prep.qab;
+        a.probe_s = ps;
+        a.probe_c = pc;
+        a.probe_n = pn;
+        a.probe_stride = pstride;
+        a.cand = (int)p->cand;
+        a.out_est = ce;
+        a.out_row = cr;
+        a.out_err = cx;
+        a.out_n = cn;
+        a.out_stride = (int)p->cand;
+        a.scanned = cnt;
+        const size_t lds = scan_lds_bytes(a.cand, pstride);
+        SIEVE_REQUIRE(lds <= 160 * 1024, SIEVE_ELIMIT, "scan workspace exceeds LDS");
+        launch_scan_l2(a, nq, s->st);
+        SIEVE_HIP(hipGetLastError());
+        SIEVE_HIP(hipEventRecord(s->ev[2], s->st));
+        L2RerankArgs r{};
+        r.q = dq;
+        r.base = ix->base;
+        r.cand_ids = cr;
+        r.cand_nx = cx;
+        r.cand_n = cn;
+        r.cand_stride = (int)p->cand;
+        r.k = (int)p->k;
+        r.out_s = os.dev;
+        r.out_r = orr.dev;
+        r.out_n = on.dev;
+        launch_rerank_l2(r, nq, s->st);
+        SIEVE_HIP(hipGetLastError());
+        SIEVE_HIP(hipEventRecord(s->ev[3], s->st));
+        out_done(s, os);
+        out_done(s, orr);
+        out_done(s, on);
+        auto copy_out = [&](void* dst, const void* src, size_t bytes) {
+            if (!dst) return;
+            SIEVE_HIP(hipMemcpyAsync(dst, src, bytes, on_device(dst) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                     s->st));
+        };
+        copy_out(probes, pc, nq * pstride * 4);
+        copy_out(probe_counts, pn, nq * 4);
+        copy_out(cand_rows, cr, nq * p->cand * 4);
+        copy_out(cand_est, ce, nq * p->cand * 4);
+        copy_out(cand_counts, cn, nq * 4);
+        unsigned long long hcnt[4] = {0, 0, 0, 0};
+        SIEVE_HIP(hipMemcpyAsync(hcnt, cnt, 32, hipMemcpyDeviceToHost, s->st));
+        std::vector<uint32_t> hpn(nq);
+        SIEVE_HIP(hipMemcpyAsync(hpn.data(), pn, nq * 4, hipMemcpyDeviceToHost, s->st));
+        check_overflow(s);
+        s->last = sieve_stats{};
+        s->last.queries = nq;
+        s->last.scanned = hcnt[0];
+        s->last.id_reads = hcnt[2];
+        for (uint32_t x : hpn) s->last.probed += x;
+        s->last.route_ns = elapsed_ns(s->ev[0], s->ev[1]);
+        s->last.scan_ns = elapsed_ns(s->ev[1], s->ev[2]);
+        s->last.rerank_ns = elapsed_ns(s->ev[2], s->ev[3]);
+        if (t) {
+            t->route += (uint64_t)s->last.route_ns;
+            t->scan += (uint64_t)s->last.scan_ns;
+            t->rerank += (uint64_t)s->last.rerank_ns;
+            t->scanned += hcnt[0];
             t->probed += s->last.probed;
         }
     });

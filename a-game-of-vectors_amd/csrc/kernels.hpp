@@ -22,7 +22,7 @@ void launch_query_prep(const float* q256, const float* q768, size_t nq, const fl
                        const float* quant, const QueryPrep& out, hipStream_t st);
 
 // Index re-layout at creation.
-void launch_transpose_blocks(const uint8_t* blocks, size_t nblocks, uint4* codes, hipStream_t st);
+void launch_transpose_blocks(const uint8_t* blocks, size_t nblocks, int groups, uint4* codes, hipStream_t st);
 void launch_split_factors(const float* factors, size_t nslots, float* mul, float* err, hipStream_t st);
 void launch_to_bf16(const float* x, size_t n, uint16_t* y, hipStream_t st);
 void launch_cluster_slots(const uint32_t* block_off, uint32_t k, uint32_t* slots, hipStream_t st);
@@ -47,11 +47,19 @@ struct RouteArgs {
     int nprobe;
     float margin;              // NaN = none
     uint64_t budget;           // 0 = none
-    float* out_s;
+    float* out_s;              // route score (inner product), or <q, c> in L2 mode
     uint32_t* out_c;
     uint32_t* out_n;
     int out_stride;
     uint32_t* overflow;        // set to 1 if a query had more than MAX_RESCORE near-margin centroids
+    // metric: inner product over `dims` (0) or L2 route score 2 <q, c> - |c|^2 (1)
+    int l2;
+    int dims;                  // rescoring dims (256, or 100 for L2)
+    int cstride;               // f32 centroid row stride
+    const float* cnorm;        // L2: |c|^2 per centroid
+    const float* qf;           // L2: nq x 100 f32 queries (rescoring operand)
+    const float* qq;           // L2: sum q_i^2 per query
+    float err_coef;            // |approx - exact| <= err_coef * qnorm
 };
 void launch_route_select(const RouteArgs& a, size_t nq, hipStream_t st);
 
@@ -119,5 +127,34 @@ struct RerankArgs {
 };
 size_t rerank_lds_bytes(int cand_stride, bool tier2_tables);
 void launch_rerank(const RerankArgs& a, size_t nq, hipStream_t st);
+
+// ------------------------------------------------------------------ L2 (100-d int8) ----
+struct L2Prep {
+    uint8_t* tables;   // nq x 512: 32 groups x 16 u8
+    float4* qab;       // nq: (a, b, scale, bias)
+    uint16_t* qbf;     // nq x 256 BF16 [q (100), 0 (28), q (100), 0 (28)]
+    float* qf;         // nq x 100 f32
+    float* qq;         // nq: sum q_i^2 (left to right)
+    float* qnorm;      // nq: |q|
+};
+void launch_query_prep_l2(const int8_t* q, size_t nq, const float* signs, float rot_norm, const L2Prep& out,
+                          hipStream_t st);
+void launch_split_leaves(const float* leaves, uint32_t k, uint16_t* out, hipStream_t st);
+void launch_scan_l2(const ScanArgs& a, size_t nq, hipStream_t st);
+
+struct L2RerankArgs {
+    const int8_t* q;           // nq x 100
+    const int8_t* base;        // n x 100
+    const uint32_t* cand_ids;  // nq x cand_stride
+    const float* cand_nx;
+    const uint32_t* cand_n;
+    int cand_stride;
+    int k;
+    float* out_s;
+    uint32_t* out_r;
+    uint32_t* out_n;
+};
+size_t rerank_l2_lds_bytes(int cand_stride);
+void launch_rerank_l2(const L2RerankArgs& a, size_t nq, hipStream_t st);
 
 }  // namespace sieve

@@ -133,18 +133,19 @@ void launch_query_prep(const float* q256, const float* q768, size_t nq, const fl
 }
 
 // ------------------------------------------------------------------ re-layout --------
-// blocks.bin (byte g*16+k = doc k nibble g | doc k+16 nibble g << 4) -> 32 raw code bytes
+// blocks.bin (byte g*16+k = doc k nibble g | doc k+16 nibble g << 4) -> G/2 raw code bytes
 // per slot (bit i of dim i at byte i/8, src/quant.rs:22-32), so a lane reads its slot's code
-// with two 16-byte loads.
-__global__ __launch_bounds__(256) void k_transpose_blocks(const uint8_t* __restrict__ blocks, size_t nblocks,
-                                                          uint8_t* __restrict__ codes) {
-    __shared__ uint8_t blk[BLOCK_BYTES];
+// with 16-byte loads. One workgroup of 4G threads per block; thread t writes 4 code bytes.
+template <int G>
+__global__ __launch_bounds__(4 * G) void k_transpose_blocks(const uint8_t* __restrict__ blocks, size_t nblocks,
+                                                            uint8_t* __restrict__ codes) {
+    constexpr int BB = G * 16, W = G / 8;  // block bytes, code words per slot
+    __shared__ uint8_t blk[BB];
     const size_t b = blockIdx.x;
     const int t = threadIdx.x;
-    reinterpret_cast<uint32_t*>(blk)[t] = reinterpret_cast<const uint32_t*>(blocks + b * BLOCK_BYTES)[t];
+    reinterpret_cast<uint32_t*>(blk)[t] = reinterpret_cast<const uint32_t*>(blocks + b * BB)[t];
     __syncthreads();
-    // thread t -> (slot k = t / 8, bytes 4*(t%8) .. +3)
-    const int k = t >> 3, j0 = (t & 7) * 4;
+    const int k = t / W, j0 = (t % W) * 4;
     uint32_t word = 0;
 #pragma unroll
     for (int jj = 0; jj < 4; jj++) {
@@ -159,13 +160,17 @@ __global__ __launch_bounds__(256) void k_transpose_blocks(const uint8_t* __restr
         }
         word |= (uint32_t)(lo | (hi << 4)) << (8 * jj);
     }
-    reinterpret_cast<uint32_t*>(codes + (b * BLOCK + k) * 32)[t & 7] = word;
+    reinterpret_cast<uint32_t*>(codes + (b * BLOCK + k) * (G / 2))[t % W] = word;
 }
 
-void launch_transpose_blocks(const uint8_t* blocks, size_t nblocks, uint4* codes, hipStream_t st) {
+void launch_transpose_blocks(const uint8_t* blocks, size_t nblocks, int groups, uint4* codes, hipStream_t st) {
     if (nblocks == 0) return;
-    hipLaunchKernelGGL(k_transpose_blocks, dim3((unsigned)nblocks), dim3(256), 0, st, blocks, nblocks,
-                       reinterpret_cast<uint8_t*>(codes));
+    if (groups == 64)
+        hipLaunchKernelGGL(k_transpose_blocks<64>, dim3((unsigned)nblocks), dim3(256), 0, st, blocks, nblocks,
+                           reinterpret_cast<uint8_t*>(codes));
+    else
+        hipLaunchKernelGGL(k_transpose_blocks<32>, dim3((unsigned)nblocks), dim3(128), 0, st, blocks, nblocks,
+                           reinterpret_cast<uint8_t*>(codes));
 }
 
 __global__ void k_split_factors(const float2* __restrict__ f, size_t n, float* __restrict__ mul,

@@ -218,19 +218,25 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
     const int t = threadIdx.x;
     const float* S = a.approx + qi * a.lds;
     const int n = min(a.nprobe, a.k);
-    for (int i = t; i < PDIM; i += 256) qs[i] = a.q256[qi * PDIM + i];
+    if (a.l2) {
+        for (int i = t; i < PDIM; i += 256) qs[i] = i < a.dims ? a.qf[qi * a.dims + i] : 0.0f;
+    } else {
+        for (int i = t; i < PDIM; i += 256) qs[i] = a.q256[qi * PDIM + i];
+    }
     if (t == 0) ncand = 0;
     __syncthreads();
     if (n <= 0) {
         if (t == 0) a.out_n[qi] = 0;
         return;
     }
-    uint32_t kth = block_kth_u32(a.k, n, [&](int i) { return f2key(S[i]); }, hist, sh);
-    const float u = 1.0f / 256.0f;
-    const float e = (2.0f * u + u * u + 256.0f / 16777216.0f) * a.qnorm[qi] * a.cent_norm_max;
-    const float thr = key2f(kth) - 2.2f * e - 1e-6f;
+    // approximate route score of centroid i
+    auto approx = [&](int i) -> float { return a.l2 ? 2.0f * S[i] - a.cnorm[i] : S[i]; };
+    uint32_t kth = block_kth_u32(a.k, n, [&](int i) { return f2key(approx(i) + 0.0f); }, hist, sh);
+    const float e = a.err_coef * a.qnorm[qi];
+    const float kv = key2f(kth);
+    const float thr = kv - 2.2f * e - 1e-6f * (1.0f + fabsf(kv));
     for (int i = t; i < a.k; i += 256) {
-        if (S[i] >= thr) {
+        if (approx(i) >= thr) {
             uint32_t j = atomicAdd(&ncand, 1u);
             if (j < MAX_RESCORE) cids[j] = (uint32_t)i;
         }
@@ -250,16 +256,18 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
     for (int i = t; i < n2; i += 256) {
         if (i < nc) {
             uint32_t c = cids[i];
-            const float4* row = reinterpret_cast<const float4*>(a.cents + (size_t)c * PDIM);
+            const float4* row = reinterpret_cast<const float4*>(a.cents + (size_t)c * a.cstride);
             float acc = -0.0f;
-            for (int j = 0; j < PDIM / 4; j++) {
+            for (int j = 0; j < a.dims / 4; j++) {
                 float4 x = row[j];
                 acc = acc + qs[4 * j] * x.x;
                 acc = acc + qs[4 * j + 1] * x.y;
                 acc = acc + qs[4 * j + 2] * x.z;
                 acc = acc + qs[4 * j + 3] * x.w;
             }
-            sorted[i] = make_uint2(f2key(acc), ~c);
+            // L2 route score 2 <q, c> - |c|^2 (src/bigann.rs:667)
+            float sc = a.l2 ? 2.0f * acc - a.cnorm[c] : acc;
+            sorted[i] = make_uint2(f2key(sc + 0.0f), ~c);
         } else {
             sorted[i] = make_uint2(0u, 0u);
         }
@@ -274,7 +282,16 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
         // (take clusters while the stored slots so far fit; at least one).
         int keep = min(n, nc);  // nc < n only for non-finite scores
         const float best = key2f(sorted[0].x);
-        if (a.margin == a.margin) {
+        if (a.l2) {
+            // keep |q - c|^2 <= (1 + margin) |q - c_1|^2 (src/bigann.rs:669-674)
+            if (a.margin > 0.0f) {
+                const float qq = a.qq[qi];
+                const float lim = (qq - best) * (1.0f + a.margin);
+                int kk = 0;
+                while (kk < keep && qq - key2f(sorted[kk].x) <= lim) kk++;
+                keep = kk;
+            }
+        } else if (a.margin == a.margin) {
             const float lim = best - a.margin;
             int kk = 0;
             while (kk < keep && key2f(sorted[kk].x) >= lim) kk++;
@@ -296,8 +313,23 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
     __syncthreads();
     const int keep = (int)sh[2];
     for (int i = t; i < keep; i += 256) {
-        a.out_s[qi * a.out_stride + i] = key2f(sorted[i].x);
-        a.out_c[qi * a.out_stride + i] = ~sorted[i].y;
+        const uint32_t c = ~sorted[i].y;
+        float outv = key2f(sorted[i].x);
+        if (a.l2) {
+            // the probe list carries <q, c> itself: the estimator's `cent` (src/bigann.rs:701)
+            const float4* row = reinterpret_cast<const float4*>(a.cents + (size_t)c * a.cstride);
+            float acc = -0.0f;
+            for (int j = 0; j < a.dims / 4; j++) {
+                float4 x = row[j];
+                acc = acc + qs[4 * j] * x.x;
+                acc = acc + qs[4 * j + 1] * x.y;
+                acc = acc + qs[4 * j + 2] * x.z;
+                acc = acc + qs[4 * j + 3] * x.w;
+            }
+            outv = acc;
+        }
+        a.out_s[qi * a.out_stride + i] = outv;
+        a.out_c[qi * a.out_stride + i] = c;
     }
 }
 

@@ -82,7 +82,11 @@ __device__ void scan_compact(uint2* buf, uint2* tmp, int n, int k, uint32_t* his
     __syncthreads();
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
+// G = 64 (256-bit codes, inner-product estimate) or G = 32 (128-bit codes of the padded
+// 100-d int8 vectors, L2 estimate 2 (cent + mul (a sc + b)) - |x|^2, src/bigann.rs:705-711).
+template <int G, bool L2>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_t(ScanArgs a) {
+    constexpr int CW = G / 32;  // 16-byte words per slot code
     extern __shared__ __align__(16) uint8_t lds[];
     const ScanLds L = scan_layout(a.cand, a.probe_stride);
     uint8_t* tab = lds;
@@ -96,7 +100,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
 
     const size_t qi = blockIdx.x;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    reinterpret_cast<uint32_t*>(tab)[t] = reinterpret_cast<const uint32_t*>(a.tables + qi * 1024)[t];
+    if (t < G * 4) reinterpret_cast<uint32_t*>(tab)[t] = reinterpret_cast<const uint32_t*>(a.tables + qi * G * 16)[t];
     const float4 qab = a.qab[qi];
     const int np = (int)a.probe_n[qi];
     for (int j = t; j < np; j += SCAN_THREADS) {
@@ -130,7 +134,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
     uint32_t id_reads = 0;  // slots past the threshold: their id is read
 
     for (uint32_t vb0 = 0; vb0 < total; vb0 += ROUND_BLOCKS) {
-        uint4 c0[SLOTS_PER_THREAD], c1[SLOTS_PER_THREAD];
+        uint4 cw[SLOTS_PER_THREAD][CW];
         float mul[SLOTS_PER_THREAD], cent[SLOTS_PER_THREAD];
         uint32_t gs[SLOTS_PER_THREAD];
         bool ok[SLOTS_PER_THREAD];
@@ -149,9 +153,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
                 }
                 gs[u] = (pstart[lo] + (vb - pref[lo])) * BLOCK + (t & 31);
                 cent[u] = pcent[lo];
-                const uint4* cp = a.codes + 2 * (size_t)gs[u];
-                c0[u] = cp[0];
-                c1[u] = cp[1];
+                const uint4* cp = a.codes + CW * (size_t)gs[u];
+#pragma unroll
+                for (int w = 0; w < CW; w++) cw[u][w] = cp[w];
                 mul[u] = a.mul[gs[u]];
             }
         }
@@ -162,13 +166,21 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
             bool pass = false;
             uint32_t key = 0;
             if (ok[u]) {
-                const uint32_t w[8] = {c0[u].x, c0[u].y, c0[u].z, c0[u].w, c1[u].x, c1[u].y, c1[u].z, c1[u].w};
+                uint32_t w[2 * CW * 2];
+#pragma unroll
+                for (int x = 0; x < CW; x++) {
+                    w[4 * x] = cw[u][x].x;
+                    w[4 * x + 1] = cw[u][x].y;
+                    w[4 * x + 2] = cw[u][x].z;
+                    w[4 * x + 3] = cw[u][x].w;
+                }
                 uint32_t sc = 0;
 #pragma unroll
-                for (int d = 0; d < 8; d++)
+                for (int d = 0; d < G / 8; d++)
 #pragma unroll
                     for (int j = 0; j < 8; j++) sc += tab[(8 * d + j) * 16 + ((w[d] >> (4 * j)) & 15u)];
                 float est = cent[u] + mul[u] * (qab.x * (float)sc + qab.y);
+                if constexpr (L2) est = 2.0f * est - a.err[gs[u]];
                 est = est + 0.0f;  // -0.0 -> +0.0 so keys order like the floats compare
                 key = f2key(est);
                 pass = !has_thr || key > thr;
@@ -199,8 +211,23 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
     if (lane == 0 && a.scanned) atomicAdd(a.scanned + 2, (unsigned long long)id_reads);
     if ((int)misc[0] > cand) scan_compact(buf, tmp, (int)misc[0], cand, hist, misc);
 
-    // Sort the kept slots by (row asc, est desc) and keep the first of each row.
     const int n = (int)misc[0];
+    if constexpr (L2) {
+        // Best first by estimate, ties by slot: (id, est, |x|^2) (src/bigann.rs:732-733).
+        const int n2 = pow2_at_least(max(n, 1));
+        for (int i = n + t; i < n2; i += SCAN_THREADS) buf[i] = make_uint2(0u, 0u);
+        __syncthreads();
+        block_sort_pairs(buf, nullptr, n2, false);
+        for (int i = t; i < n; i += SCAN_THREADS) {
+            const uint32_t slot = ~buf[i].y;
+            a.out_est[qi * a.out_stride + i] = key2f(buf[i].x);
+            a.out_row[qi * a.out_stride + i] = a.ids[slot];
+            a.out_err[qi * a.out_stride + i] = a.err[slot];
+        }
+        if (t == 0) a.out_n[qi] = (uint32_t)n;
+        return;
+    }
+    // Sort the kept slots by (row asc, est desc) and keep the first of each row.
     const int n2 = pow2_at_least(max(n, 1));
     uint32_t* pay = reinterpret_cast<uint32_t*>(tmp);
     for (int i = t; i < n2; i += SCAN_THREADS) {
@@ -241,8 +268,17 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
 void launch_scan(const ScanArgs& a, size_t nq, hipStream_t st) {
     if (nq == 0) return;
     const size_t lds = scan_lds_bytes(a.cand, a.probe_stride);
-    if (lds > 64 * 1024) (void)hipFuncSetAttribute((const void*)k_scan, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_scan, dim3((unsigned)nq), dim3(SCAN_THREADS), lds, st, a);
+    if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute((const void*)k_scan_t<64, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_scan_t<64, false>), dim3((unsigned)nq), dim3(SCAN_THREADS), lds, st, a);
+}
+
+void launch_scan_l2(const ScanArgs& a, size_t nq, hipStream_t st) {
+    if (nq == 0) return;
+    const size_t lds = scan_lds_bytes(a.cand, a.probe_stride);
+    if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute((const void*)k_scan_t<32, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_scan_t<32, true>), dim3((unsigned)nq), dim3(SCAN_THREADS), lds, st, a);
 }
 
 // ------------------------------------------------------------------ lut::scan seam -----

@@ -69,6 +69,18 @@ class IndexInfo(ctypes.Structure):
                 ("device_bytes", ctypes.c_uint64)]
 
 
+class L2Params(ctypes.Structure):
+    _fields_ = [("nprobe", ctypes.c_uint32), ("cand", ctypes.c_uint32), ("k", ctypes.c_uint32),
+                ("refine", ctypes.c_uint32), ("margin", ctypes.c_float)]
+
+
+class L2Desc(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_uint32), ("n", ctypes.c_uint64), ("nblocks", ctypes.c_uint64),
+                ("leaves", ctypes.c_void_p), ("block_off", ctypes.c_void_p), ("blocks", ctypes.c_void_p),
+                ("ids", ctypes.c_void_p), ("factors", ctypes.c_void_p), ("rotation", ctypes.c_void_p),
+                ("base", ctypes.c_void_p), ("borrow_base", ctypes.c_uint32)]
+
+
 _lib = None
 
 
@@ -104,6 +116,10 @@ def lib():
         "sieve_candidates": (i32, [vp, vp, vp, sz, vp, vp, vp, sz, P, V, vp, vp, vp, vp, T]),
         "sieve_rerank": (i32, [vp, vp, vp, vp, sz, vp, vp, vp, vp, sz, P, V, vp, vp, vp, T]),
         "sieve_search_batch": (i32, [vp, vp, vp, vp, sz, P, V, vp, vp, vp, T]),
+        "sieve_l2_index_create": (i32, [ctypes.POINTER(L2Desc), ctypes.POINTER(vp)]),
+        "sieve_l2_index_free": (None, [vp]),
+        "sieve_l2_search_batch": (i32, [vp, vp, vp, sz, ctypes.POINTER(L2Params), vp, vp, vp, vp, vp, vp, vp, vp,
+                                        T]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -39,9 +39,10 @@ T2_LEVELS = (-1.510, -0.4528, 0.4528, 1.510)
 # ----------------------------------------------------------------------- f32 helpers ----
 def seq_sum_sq(x):
     """Σ x_i^2 left to right from -0.0, per row (Rust `map(|x| x * x).sum()`)."""
+    xt = x.t().contiguous()  # one contiguous row per dimension
     acc = torch.full((x.shape[0],), -0.0, dtype=torch.float32, device=x.device)
-    for i in range(x.shape[1]):
-        c = x[:, i]
+    for i in range(xt.shape[0]):
+        c = xt[i]
         acc = acc + c * c
     return acc
 
@@ -132,10 +133,11 @@ def rabitq_encode(xr, cr):
     """rabitq::encode (src/rabitq.rs:86-108) row-wise: (codes n x d/8, mul n, err n)."""
     r = xr - cr
     n, d = r.shape
+    rt = r.t().contiguous()
     nr2 = torch.zeros(n, dtype=torch.float32, device=r.device)
     l1 = torch.zeros(n, dtype=torch.float32, device=r.device)
     for i in range(d):
-        c = r[:, i]
+        c = rt[i]
         nr2 = nr2 + c * c
         l1 = l1 + torch.abs(c)
     code = pack_bits(r > 0)
@@ -287,9 +289,10 @@ def tier2_encode(prefix, sigma):
     lv = torch.where(prefix < -t, 0, torch.where(prefix < 0, 1, torch.where(prefix < t, 2, 3)))
     levels = torch.tensor(T2_LEVELS, dtype=torch.float32, device=prefix.device)
     v = levels[lv] * sigma
+    vt = v.t().contiguous()
     nn = torch.zeros(prefix.shape[0], dtype=torch.float32, device=prefix.device)
     for i in range(PDIM):
-        nn = nn + v[:, i] * v[:, i]
+        nn = nn + vt[i] * vt[i]
     sh = (2 * torch.arange(4, device=prefix.device, dtype=torch.int32))
     codes = (lv.view(-1, PDIM // 4, 4).to(torch.int32) << sh).sum(-1).to(torch.uint8)
     return codes, f32_sqrt(nn)

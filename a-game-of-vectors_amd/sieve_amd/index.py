@@ -189,6 +189,59 @@ class Index:
         return s, r, n
 
 
+class L2Index:
+    """The L2 index over 100-d int8 vectors (bigann::Index, src/bigann.rs:604-747)."""
+
+    def __init__(self, arrays, borrow_base=False):
+        keep = {}
+        d = abi.L2Desc()
+        dts = dict(leaves=np.float32, block_off=np.uint64, blocks=np.uint8, ids=np.uint32, factors=np.float32,
+                   rotation=np.float32, base=np.int8)
+        for name, dt in dts.items():
+            a = arrays[name]
+            a = a.contiguous() if _is_torch(a) else np.ascontiguousarray(a, dtype=dt)
+            keep[name] = a
+            setattr(d, name, abi.ptr(a).value)
+        d.k = _numel(keep["leaves"]) // 100
+        d.n = _numel(keep["base"]) // 100
+        d.nblocks = _numel(keep["ids"]) // 32
+        d.borrow_base = 1 if borrow_base else 0
+        h = ctypes.c_void_p()
+        abi.check(abi.lib().sieve_l2_index_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self._keep = keep if borrow_base else None
+        self.k = d.k
+        self._stream = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                abi.lib().sieve_l2_index_free(self._h)
+        except Exception:
+            pass
+        self._h = None
+
+    def search(self, q, nprobe, cand, k, margin=0.0, timing=None, stream=None, details=False):
+        """bigann::Index::search for a batch of int8 queries (nq x 100): (scores, rows, counts)
+        and, with details, (probes, probe_counts, cand_rows, cand_est, cand_counts)."""
+        q = q if _is_torch(q) else np.ascontiguousarray(q, dtype=np.int8)
+        nq = q.shape[0]
+        out = Index._out
+        s, r, n = out(q, (nq, k), np.float32), out(q, (nq, k), np.uint32), out(q, (nq,), np.uint32)
+        npb = min(nprobe, self.k)
+        extra = [out(q, (nq, npb), np.uint32), out(q, (nq,), np.uint32), out(q, (nq, cand), np.uint32),
+                 out(q, (nq, cand), np.float32), out(q, (nq,), np.uint32)] if details else [None] * 5
+        t = timing if timing is not None else abi.Timing()
+        if self._stream is None:
+            self._stream = Stream()
+        st = stream or self._stream
+        p = abi.L2Params(nprobe=nprobe, cand=cand, k=k, refine=0, margin=margin)
+        abi.check(abi.lib().sieve_l2_search_batch(self._h, st._h, abi.ptr(q), nq, ctypes.byref(p), abi.ptr(s),
+                                                  abi.ptr(r), abi.ptr(n), *[abi.ptr(x) for x in extra],
+                                                  ctypes.byref(t)))
+        return (s, r, n, *extra) if details else (s, r, n)
+
+
 class ViewSpec:
     """ivf::View (src/ivf.rs:462-496): tombstones and allowlists as u64 bitsets over rows."""
 

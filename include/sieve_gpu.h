@@ -239,6 +239,56 @@ int sieve_search_batch(const sieve_index* ix, sieve_stream* s, const float* q768
                        const sieve_view* v, float* scores, uint32_t* rows, uint32_t* counts,
                        sieve_timing* t);
 
+/* --------------------------------------------- L2 over 100-d int8 vectors (1B scale) --- */
+
+/* bigann::Params (src/bigann.rs:610-622). refine must be 0: the multi-bit refinement
+ * (src/bigann.rs:565-602) is not part of this library; margin <= 0 is off. */
+typedef struct sieve_l2_params {
+    uint32_t nprobe;
+    uint32_t cand;
+    uint32_t k;
+    uint32_t refine;
+    float margin;
+} sieve_l2_params;
+
+/* The L2 index as arrays (src/bigann.rs:386-460 postings, :237-266 model):
+ *   leaves     k x 100 f32          centroids.f32 (the leaves, cell-major)
+ *   block_off  k + 1 u64            block_off.u64
+ *   blocks     nblocks x 512 B      32-group nibble-transposed codes of the rotated,
+ *                                   128-padded residuals (the postings' code section)
+ *   ids        nblocks x 32 u32     (PAD = 0xFFFFFFFF)
+ *   factors    nblocks x 32 x 2 f32 [mul, |x|^2] per slot
+ *   rotation   3 x 128 f32          rotation.f32
+ *   base       n x 100 i8           the base vectors (exact rerank rows)
+ * `base` is borrowed when it is device memory and borrow_base is nonzero. */
+typedef struct sieve_l2_desc {
+    uint32_t k;
+    uint64_t n;
+    uint64_t nblocks;
+    const float* leaves;
+    const uint64_t* block_off;
+    const uint8_t* blocks;
+    const uint32_t* ids;
+    const float* factors;
+    const float* rotation;
+    const int8_t* base;
+    uint32_t borrow_base;
+} sieve_l2_desc;
+
+typedef struct sieve_l2_index sieve_l2_index;
+
+int sieve_l2_index_create(const sieve_l2_desc* desc, sieve_l2_index** out);
+void sieve_l2_index_free(sieve_l2_index* ix);
+
+/* bigann::Index::search (src/bigann.rs:652-657) for nq queries (nq x 100 i8): the top k
+ * (score = |q|^2 - |q - x|^2 - |q|^2 = 2 <q, x> - |x|^2, row) best first. Optional outputs
+ * (NULL to skip): the probe lists (nq x nprobe leaf ids, best first, and counts), and the
+ * candidates (nq x cand ids and estimates, best first, and counts). */
+int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_t* q, size_t nq,
+                          const sieve_l2_params* p, float* scores, uint32_t* rows, uint32_t* counts,
+                          uint32_t* probes, uint32_t* probe_counts, uint32_t* cand_rows,
+                          float* cand_est, uint32_t* cand_counts, sieve_timing* t);
+
 #ifdef __cplusplus
 }
 #endif
