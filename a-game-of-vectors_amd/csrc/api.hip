@@ -17,370 +17,17 @@
 
 #include "kernels.hpp"
 
-namespace sieve {
+#include "host.hpp"
 
+namespace sieve {
 static thread_local std::string g_error;
 void set_error(const std::string& msg) { g_error = msg; }
-
-static bool on_device(const void* p) {
-    if (!p) return false;
-    hipPointerAttribute_t at;
-    hipError_t e = hipPointerGetAttributes(&at, p);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        return false;
-    }
-    return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
-}
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
-
-template <class T>
-static T* grow(DevBuf& b, size_t n) {
-    size_t bytes = std::max<size_t>(n * sizeof(T), 16);
-    if (bytes > b.cap) {
-        b.release();
-        hipError_t e = hipMalloc(&b.p, bytes);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            b.p = nullptr;
-            throw Error{SIEVE_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
-        }
-        b.cap = bytes;
-    }
-    return static_cast<T*>(b.p);
-}
-
-template <class T>
-static T* dmalloc(size_t n, size_t* acct) {
-    void* p = nullptr;
-    size_t bytes = std::max<size_t>(n * sizeof(T), 16);
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        throw Error{SIEVE_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
-    }
-    if (acct) *acct += bytes;
-    return static_cast<T*>(p);
-}
-
 }  // namespace sieve
 
 using namespace sieve;
 
-struct sieve_index {
-    int device = 0;
-    uint32_t k = 0;
-    uint64_t n = 0, nblocks = 0, nslots = 0;
-    uint32_t max_cluster_blocks = 0;
-    float* cents = nullptr;
-    uint16_t* cents_bf = nullptr;
-    float cent_norm_max = 0.0f;
-    uint32_t* block_off = nullptr;
-    uint32_t* cl_slots = nullptr;
-    uint4* codes = nullptr;
-    float* mul = nullptr;
-    float* err = nullptr;
-    uint32_t* ids = nullptr;
-    float* rotation = nullptr;
-    int8_t* rerank = nullptr;
-    bool rerank_owned = true;
-    float* quant = nullptr;
-    float* norm256 = nullptr;
-    float* t2_sigma = nullptr;
-    uint8_t* t2_codes = nullptr;
-    float* t2_norm = nullptr;
-    size_t device_bytes = 0;
-
-    ~sieve_index() {
-        void* ps[] = {cents, cents_bf, block_off, cl_slots, codes, mul, err, ids, rotation, quant, norm256, t2_sigma,
-                      t2_codes, t2_norm};
-        for (void* p : ps)
-            if (p) (void)hipFree(p);
-        if (rerank && rerank_owned) (void)hipFree(rerank);
-    }
-};
-
-struct sieve_l2_index {
-    int device = 0;
-    uint32_t k = 0;
-    uint64_t n = 0, nblocks = 0, nslots = 0;
-    float* leaves = nullptr;       // k x 100 f32
-    uint16_t* leaves_bf = nullptr; // k x 256 BF16 [hi | lo]
-    float* leaf_norm = nullptr;    // k: |c|^2 summed left to right
-    float cent_norm_max = 0.0f;
-    uint32_t* block_off = nullptr;
-    uint4* codes = nullptr;        // 16 bytes per slot
-    float* mul = nullptr;
-    float* nx = nullptr;
-    uint32_t* ids = nullptr;
-    float* rotation = nullptr;     // 3 x 128
-    int8_t* base = nullptr;
-    bool base_owned = true;
-    size_t device_bytes = 0;
-    ~sieve_l2_index() {
-        void* ps[] = {leaves, leaves_bf, leaf_norm, block_off, codes, mul, nx, ids, rotation};
-        for (void* p : ps)
-            if (p) (void)hipFree(p);
-        if (base && base_owned) (void)hipFree(base);
-    }
-};
-
-enum Slot {
-    B_Q256, B_Q768, B_TABLES, B_QAB, B_WQ768, B_WQ256, B_INV, B_QBF, B_QNORM, B_APPROX, B_PS, B_PC, B_PN, B_CE, B_CR,
-    B_CX, B_CN, B_OS, B_OR, B_ON, B_DEL, B_ALLOW, B_AFLAGS, B_CNT, B_OVF, B_TMP1, B_TMP2, B_TMP3, B_QQ, B_QF,
-    B_NSLOTS
-};
-
-struct sieve_stream {
-    hipStream_t st = nullptr;
-    int device = 0;
-    DevBuf b[B_NSLOTS];
-    hipEvent_t ev[8] = {};
-    sieve_stats last{};
-    ~sieve_stream() {
-        for (auto& x : b) x.release();
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
-        if (st) (void)hipStreamDestroy(st);
-    }
-};
-
 namespace {
 
-template <class F>
-int guarded(F&& f) {
-    try {
-        f();
-        set_error("");
-        return SIEVE_OK;
-    } catch (const Error& e) {
-        set_error(e.msg);
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        set_error("host allocation failed");
-        return SIEVE_ENOMEM;
-    } catch (const std::exception& e) {
-        set_error(e.what());
-        return SIEVE_EINVAL;
-    } catch (...) {
-        set_error("unknown error");
-        return SIEVE_EINVAL;
-    }
-}
-
-// A device view of a caller array: the array itself if it is device memory, else a copy in
-// stream scratch.
-template <class T>
-const T* dev_in(sieve_stream* s, int slot, const T* p, size_t n) {
-    if (!p || n == 0) return p;
-    if (on_device(p)) return p;
-    T* d = grow<T>(s->b[slot], n);
-    SIEVE_HIP(hipMemcpyAsync(d, p, n * sizeof(T), hipMemcpyHostToDevice, s->st));
-    return d;
-}
-
-// Where a kernel writes an output: the caller's array if it is device memory, else scratch
-// that is copied back by out_done().
-template <class T>
-struct DevOut {
-    T* user;
-    T* dev;
-    size_t n;
-    bool host;
-};
-
-template <class T>
-DevOut<T> dev_out(sieve_stream* s, int slot, T* p, size_t n) {
-    if (on_device(p)) return {p, p, n, false};
-    return {p, grow<T>(s->b[slot], n), n, true};
-}
-
-template <class T>
-void out_done(sieve_stream* s, const DevOut<T>& o) {
-    if (o.host && o.n) SIEVE_HIP(hipMemcpyAsync(o.user, o.dev, o.n * sizeof(T), hipMemcpyDeviceToHost, s->st));
-}
-
-void check_params(const sieve_params* p, bool need_probe) {
-    SIEVE_REQUIRE(p, SIEVE_EINVAL, "params is null");
-    SIEVE_REQUIRE(p->cand >= 1 && p->cand <= (uint32_t)MAX_CAND, SIEVE_ELIMIT,
-                  "cand must be in [1, " + std::to_string(MAX_CAND) + "]");
-    SIEVE_REQUIRE(p->k >= 1 && p->k <= (uint32_t)MAX_CAND, SIEVE_ELIMIT,
-                  "k must be in [1, " + std::to_string(MAX_CAND) + "]");
-    SIEVE_REQUIRE(p->rerank <= SIEVE_RERANK_SSD, SIEVE_EINVAL, "unknown rerank tag " + std::to_string(p->rerank));
-    if (need_probe)
-        SIEVE_REQUIRE(p->nprobe >= 1 && p->nprobe <= (uint32_t)MAX_PROBE, SIEVE_ELIMIT,
-                      "nprobe must be in [1, " + std::to_string(MAX_PROBE) + "]");
-    SIEVE_REQUIRE(p->prune_m <= (uint32_t)MAX_CAND, SIEVE_ELIMIT, "prune rank beyond cand limit");
-}
-
-ViewArgs make_view(sieve_stream* s, const sieve_view* v, size_t nq) {
-    ViewArgs a{};
-    if (!v) return a;
-    a.deleted = dev_in(s, B_DEL, v->deleted, v->deleted ? v->deleted_words : 0);
-    a.deleted_words = v->deleted ? v->deleted_words : 0;
-    if (v->allow) {
-        size_t words = v->allow_stride ? v->allow_stride * nq : v->allow_words;
-        a.allow = dev_in(s, B_ALLOW, v->allow, words);
-        a.allow_words = v->allow_words;
-        a.allow_stride = v->allow_stride;
-        SIEVE_REQUIRE(!v->allow_stride || v->allow_stride >= v->allow_words, SIEVE_EINVAL,
-                      "allow_stride smaller than allow_words");
-        a.allow_flags = v->allow_flags ? dev_in(s, B_AFLAGS, v->allow_flags, nq) : nullptr;
-    }
-    return a;
-}
-
-float elapsed_ns(hipEvent_t a, hipEvent_t b) {
-    float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, a, b);
-    return ms * 1e6f;
-}
-
-// Per-batch device state of the pipeline.
-struct Batch {
-    QueryPrep prep;
-    const float* q256 = nullptr;
-    const float* q768 = nullptr;
-};
-
-void prep_queries(const sieve_index* ix, sieve_stream* s, const float* q256d, const float* q768d, size_t nq,
-                  Batch& bt) {
-    bt.q256 = q256d;
-    bt.q768 = q768d;
-    bt.prep.tables = grow<uint8_t>(s->b[B_TABLES], nq * 1024);
-    bt.prep.qab = grow<float4>(s->b[B_QAB], nq);
-    bt.prep.wq768 = grow<int8_t>(s->b[B_WQ768], nq * DIM);
-    bt.prep.wq256 = grow<int8_t>(s->b[B_WQ256], nq * PDIM);
-    bt.prep.inv = grow<float2>(s->b[B_INV], nq);
-    bt.prep.qbf = grow<uint16_t>(s->b[B_QBF], nq * PDIM);
-    bt.prep.qnorm = grow<float>(s->b[B_QNORM], nq);
-    // 256^-1.5 = 2^-12 exactly.
-    launch_query_prep(q256d, q768d, nq, ix->rotation, 1.0f / 4096.0f, ix->quant, bt.prep, s->st);
-}
-
-// Route nq prepared queries into (ps, pc, pn) with row stride `stride`.
-void route_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, size_t nq, const sieve_params* p, float* ps,
-                 uint32_t* pc, uint32_t* pn, int stride) {
-    uint32_t* ovf = grow<uint32_t>(s->b[B_OVF], 1);
-    SIEVE_HIP(hipMemsetAsync(ovf, 0, 4, s->st));
-    const size_t per = std::max<size_t>(1, std::min<size_t>(nq, (size_t(1) << 30) / (4 * (size_t)ix->k)));
-    float* approx = grow<float>(s->b[B_APPROX], per * ix->k);
-    for (size_t q0 = 0; q0 < nq; q0 += per) {
-        size_t m = std::min(per, nq - q0);
-        launch_gemm_bf16_route(bt.prep.qbf + q0 * PDIM, ix->cents_bf, approx, (int)m, (int)ix->k, ix->k, s->st);
-        RouteArgs ra{};
-        ra.approx = approx;
-        ra.lds = ix->k;
-        ra.k = (int)ix->k;
-        ra.cents = ix->cents;
-        ra.q256 = bt.q256 + q0 * PDIM;
-        ra.qnorm = bt.prep.qnorm + q0;
-        ra.cl_slots = ix->cl_slots;
-        ra.cent_norm_max = ix->cent_norm_max;
-        ra.nprobe = (int)p->nprobe;
-        ra.margin = p->margin;
-        ra.budget = p->budget;
-        ra.out_s = ps + q0 * stride;
-        ra.out_c = pc + q0 * stride;
-        ra.out_n = pn + q0;
-        ra.out_stride = stride;
-        ra.overflow = ovf;
-        ra.l2 = 0;
-        ra.dims = PDIM;
-        ra.cstride = PDIM;
-        const float u = 1.0f / 256.0f;  // BF16 unit roundoff
-        ra.err_coef = (2.0f * u + u * u + 256.0f / 16777216.0f) * ix->cent_norm_max;
-        launch_route_select(ra, m, s->st);
-    }
-    SIEVE_HIP(hipGetLastError());
-}
-
-void check_overflow(sieve_stream* s) {
-    uint32_t h = 0;
-    SIEVE_HIP(hipMemcpyAsync(&h, s->b[B_OVF].p, 4, hipMemcpyDeviceToHost, s->st));
-    SIEVE_HIP(hipStreamSynchronize(s->st));
-    SIEVE_REQUIRE(h == 0, SIEVE_ELIMIT,
-                  "routing: more than " + std::to_string(MAX_RESCORE) +
-                      " centroids within the BF16 error band of the nprobe-th score");
-}
-
-void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, size_t nq, const float* ps,
-                const uint32_t* pc, const uint32_t* pn, int pstride, const sieve_params* p, const ViewArgs& view,
-                float* ce, uint32_t* cr, float* cx, uint32_t* cn, int cstride, unsigned long long* scanned) {
-    ScanArgs a{};
-    a.codes = ix->codes;
-    a.mul = ix->mul;
-    a.err = ix->err;
-    a.ids = ix->ids;
-    a.block_off = ix->block_off;
-    a.tables = bt.prep.tables;
-    a.qab = bt.prep.qab;
-    a.probe_s = ps;
-    a.probe_c = pc;
-    a.probe_n = pn;
-    a.probe_stride = pstride;
-    a.cand = (int)p->cand;
-    a.view = view;
-    a.out_est = ce;
-    a.out_row = cr;
-    a.out_err = cx;
-    a.out_n = cn;
-    a.out_stride = cstride;
-    a.scanned = scanned;
-    const size_t lds = scan_lds_bytes(a.cand, pstride);
-    SIEVE_REQUIRE(lds <= 160 * 1024, SIEVE_ELIMIT, "scan workspace exceeds LDS");
-    launch_scan(a, nq, s->st);
-    SIEVE_HIP(hipGetLastError());
-}
-
-void rerank_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, size_t nq, const float* ce,
-                  const uint32_t* cr, const float* cx, const uint32_t* cn, int cstride, const sieve_params* p,
-                  float* os, uint32_t* orr, uint32_t* on, unsigned long long* reranked) {
-    RerankArgs a{};
-    a.est = ce;
-    a.rows = cr;
-    a.errs = cx;
-    a.cand_n = cn;
-    a.cand_stride = cstride;
-    a.rerank = ix->rerank;
-    a.norm256 = ix->norm256;
-    a.t2_codes = ix->t2_codes;
-    a.t2_norm = ix->t2_norm;
-    a.t2_sigma = ix->t2_sigma;
-    a.q256 = bt.q256;
-    a.wq768 = bt.prep.wq768;
-    a.wq256 = bt.prep.wq256;
-    a.inv = bt.prep.inv;
-    a.mode = (int)p->rerank;
-    a.r = (int)p->rerank_r;
-    a.k = (int)p->k;
-    a.prune_m = (int)p->prune_m;
-    a.prune_w = p->prune_w;
-    a.out_s = os;
-    a.out_r = orr;
-    a.out_n = on;
-    a.reranked = reranked;
-    const bool t2 = a.mode == SIEVE_RERANK_TIERED && a.t2_codes;
-    SIEVE_REQUIRE(rerank_lds_bytes(cstride, t2) <= 160 * 1024, SIEVE_ELIMIT, "rerank workspace exceeds LDS");
-    launch_rerank(a, nq, s->st);
-    SIEVE_HIP(hipGetLastError());
-}
-
-void ensure_events(sieve_stream* s) {
-    for (auto& e : s->ev)
-        if (!e) SIEVE_HIP(hipEventCreate(&e));
-}
-
-// ------------------------------------------------------------------ index building ---
 struct HostCopy {
     std::vector<uint8_t> data;
 };
@@ -922,31 +569,15 @@ int sieve_search_batch(const sieve_index* ix, sieve_stream* s, const float* q768
         SIEVE_REQUIRE(ix && s && q768 && q256 && scores && rows && counts, SIEVE_EINVAL, "null argument");
         check_params(p, true);
         if (nq == 0) return;
-        const int pstride = (int)std::min<uint32_t>(p->nprobe, ix->k);
         const float* q2 = dev_in(s, B_Q256, q256, nq * PDIM);
         const float* q7 = dev_in(s, B_Q768, q768, nq * DIM);
         ViewArgs view = make_view(s, v, nq);
-        unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 4);
-        float* ps = grow<float>(s->b[B_PS], nq * pstride);
-        uint32_t* pc = grow<uint32_t>(s->b[B_PC], nq * pstride);
-        uint32_t* pn = grow<uint32_t>(s->b[B_PN], nq);
-        float* ce = grow<float>(s->b[B_CE], nq * p->cand);
-        uint32_t* cr = grow<uint32_t>(s->b[B_CR], nq * p->cand);
-        float* cx = grow<float>(s->b[B_CX], nq * p->cand);
-        uint32_t* cn = grow<uint32_t>(s->b[B_CN], nq);
         DevOut<float> os = dev_out(s, B_OS, scores, nq * p->k);
         DevOut<uint32_t> orr = dev_out(s, B_OR, rows, nq * p->k);
         DevOut<uint32_t> on = dev_out(s, B_ON, counts, nq);
-        SIEVE_HIP(hipMemsetAsync(cnt, 0, 32, s->st));
-        SIEVE_HIP(hipEventRecord(s->ev[0], s->st));
-        Batch bt;
-        prep_queries(ix, s, q2, q7, nq, bt);
-        route_batch(ix, s, bt, nq, p, ps, pc, pn, pstride);
-        SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
-        scan_batch(ix, s, bt, nq, ps, pc, pn, pstride, p, view, ce, cr, cx, cn, (int)p->cand, cnt);
-        SIEVE_HIP(hipEventRecord(s->ev[2], s->st));
-        rerank_batch(ix, s, bt, nq, ce, cr, cx, cn, (int)p->cand, p, os.dev, orr.dev, on.dev, cnt + 1);
-        SIEVE_HIP(hipEventRecord(s->ev[3], s->st));
+        enqueue_search(ix, s, q2, q7, nq, p, view, os.dev, orr.dev, on.dev);
+        unsigned long long* cnt = static_cast<unsigned long long*>(s->b[B_CNT].p);
+        const uint32_t* pn = static_cast<const uint32_t*>(s->b[B_PN].p);
         out_done(s, os);
         out_done(s, orr);
         out_done(s, on);

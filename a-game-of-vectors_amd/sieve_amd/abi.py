@@ -81,6 +81,20 @@ class L2Desc(ctypes.Structure):
                 ("base", ctypes.c_void_p), ("borrow_base", ctypes.c_uint32)]
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("max_batch", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("queue_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("batch_size", ctypes.c_uint32),
+                ("batch_id", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_uint64), ("queries", ctypes.c_uint64), ("pending", ctypes.c_uint64),
+                ("inflight", ctypes.c_uint64), ("slots", ctypes.c_uint32), ("slot_bytes", ctypes.c_uint64)]
+
+
 _lib = None
 
 
@@ -96,6 +110,14 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"libsieve_gpu.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+    # torch ships its own HIP runtime with the same soname (libamdhip64.so.7). Loading torch
+    # first makes the dynamic loader bind this library to that runtime, so a process that uses
+    # both has one runtime; loaded the other way round, torch would start a second runtime
+    # that finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, sz, i32, u32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint32
     P, T, V = ctypes.POINTER(Params), ctypes.POINTER(Timing), ctypes.POINTER(View)
@@ -116,6 +138,12 @@ def lib():
         "sieve_candidates": (i32, [vp, vp, vp, sz, vp, vp, vp, sz, P, V, vp, vp, vp, vp, T]),
         "sieve_rerank": (i32, [vp, vp, vp, vp, sz, vp, vp, vp, vp, sz, P, V, vp, vp, vp, T]),
         "sieve_search_batch": (i32, [vp, vp, vp, vp, sz, P, V, vp, vp, vp, T]),
+        "sieve_server_create": (i32, [vp, P, ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)]),
+        "sieve_server_submit": (i32, [vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "sieve_server_wait": (i32, [vp, ctypes.c_uint64, vp, vp, ctypes.POINTER(ctypes.c_uint32),
+                                    ctypes.POINTER(RequestInfo)]),
+        "sieve_server_get_stats": (i32, [vp, ctypes.POINTER(ServerStats)]),
+        "sieve_server_destroy": (None, [vp]),
         "sieve_l2_index_create": (i32, [ctypes.POINTER(L2Desc), ctypes.POINTER(vp)]),
         "sieve_l2_index_free": (None, [vp]),
         "sieve_l2_search_batch": (i32, [vp, vp, vp, sz, ctypes.POINTER(L2Params), vp, vp, vp, vp, vp, vp, vp, vp,

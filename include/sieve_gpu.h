@@ -239,6 +239,50 @@ int sieve_search_batch(const sieve_index* ix, sieve_stream* s, const float* q768
                        const sieve_view* v, float* scores, uint32_t* rows, uint32_t* counts,
                        sieve_timing* t);
 
+/* ------------------------------------------------------------- serving loop --- */
+
+/* No counterpart in the reference, whose server is a thread per connection over TCP
+ * (src/shard.rs:156-201). Continuous batching: requests submitted one at a time queue in
+ * host memory; a batcher thread launches everything pending (up to max_batch) as one device
+ * batch whenever a batch slot is free, on that slot's own stream, so batches overlap on the
+ * device; a completer thread retires them. The number of slots is bounded by hbm_budget
+ * bytes of device scratch (0: a quarter of free HBM) and by max_inflight (0: no extra
+ * bound), so device memory gates admission. One parameter set per server. */
+typedef struct sieve_server_opts {
+    uint32_t max_batch;      /* queries per device batch (0: 1024) */
+    uint32_t max_inflight;   /* device batches in flight at most (0: as many as fit) */
+    uint64_t hbm_budget;     /* device bytes the slots may hold (0: a quarter of free HBM) */
+} sieve_server_opts;
+
+typedef struct sieve_request_info {
+    double queue_ms;         /* submit -> batch launch */
+    double total_ms;         /* submit -> results on the host */
+    uint32_t batch_size;
+    uint64_t batch_id;
+} sieve_request_info;
+
+typedef struct sieve_server_stats {
+    uint64_t batches;
+    uint64_t queries;
+    uint64_t pending;
+    uint64_t inflight;
+    uint32_t slots;
+    uint64_t slot_bytes;
+} sieve_server_stats;
+
+typedef struct sieve_server sieve_server;
+
+int sieve_server_create(const sieve_index* ix, const sieve_params* p, const sieve_server_opts* o,
+                        sieve_server** out);
+/* Copies one query (768 + 256 f32, host memory) into the queue; returns a ticket. */
+int sieve_server_submit(sieve_server* s, const float* q768, const float* q256, uint64_t* ticket);
+/* Blocks until the ticket's results are ready; scores/rows hold p->k entries (count used). */
+int sieve_server_wait(sieve_server* s, uint64_t ticket, float* scores, uint32_t* rows, uint32_t* count,
+                      sieve_request_info* info);
+int sieve_server_get_stats(sieve_server* s, sieve_server_stats* out);
+/* Drains queued and in-flight work, then stops the threads and frees the slots. */
+void sieve_server_destroy(sieve_server* s);
+
 /* --------------------------------------------- L2 over 100-d int8 vectors (1B scale) --- */
 
 /* bigann::Params (src/bigann.rs:610-622). refine must be 0: the multi-bit refinement
