@@ -37,7 +37,7 @@ struct ScanLds {
 __host__ __device__ inline ScanLds scan_layout(int cand, int probe_stride) {
     ScanLds l;
     l.cap = 2 * cand + ROUND_SLOTS;
-    size_t o = 1024;  // tables
+    size_t o = 0;
     auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
     l.off_pref = o;
     o = al(o + 4 * (size_t)(probe_stride + 1));
@@ -87,9 +87,11 @@ __device__ void scan_compact(uint2* buf, uint2* tmp, int n, int k, uint32_t* his
 template <int G, bool L2>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_t(ScanArgs a) {
     constexpr int CW = G / 32;  // 16-byte words per slot code
+    // The query's tables sit in static LDS at a compile-time address, so each lookup is one
+    // ds_read_u8 with the table offset in its immediate field.
+    __shared__ __align__(16) uint8_t tab[G * 16];
     extern __shared__ __align__(16) uint8_t lds[];
     const ScanLds L = scan_layout(a.cand, a.probe_stride);
-    uint8_t* tab = lds;
     uint32_t* pref = reinterpret_cast<uint32_t*>(lds + L.off_pref);
     uint32_t* pstart = reinterpret_cast<uint32_t*>(lds + L.off_start);
     float* pcent = reinterpret_cast<float*>(lds + L.off_cent);

@@ -79,6 +79,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--latency-queries", type=int, default=300)
     ap.add_argument("--config", default=None, help="skip the sweep and time this configuration")
+    ap.add_argument("--sweep", default=None, help="comma-separated configurations to sweep instead of the "
+                                                 "workload's list")
+    ap.add_argument("--spread", type=float, default=None, help="mixture component spread (data recipe)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed steps, copy one 1 GiB device buffer (a plain copy of known size, "
+                         "the reference that scales the FETCH_SIZE/WRITE_SIZE counters of a profile)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (index sharding is a design "
@@ -94,7 +100,8 @@ def main():
     W = WORKLOADS[a.workload]
     dev = torch.device("cuda:0")
     t_build0 = time.time()
-    spec = synth.MixtureSpec(components=W["components"], seed=20241203)
+    spread = a.spread if a.spread is not None else W.get("spread", synth.MixtureSpec.spread)
+    spec = synth.MixtureSpec(components=W["components"], seed=20241203, spread=spread)
     mix = synth.Mixture(spec, device=dev)
     log(f"workload {a.workload}: ingest {W['rows']} rows")
     corpus = B.ingest(mix, W["rows"], batch=1 << 20)
@@ -119,7 +126,7 @@ def main():
 
     # Sweep: recall and a 2-step throughput estimate per configuration.
     sweep = []
-    cfgs = [a.config] if a.config else W["sweep"]
+    cfgs = [a.config] if a.config else (a.sweep.split(",") if a.sweep else W["sweep"])
     for cfg in cfgs:
         _, p = parse_config(cfg, ix.k, k=10)
         s, r, n = run(p)
@@ -169,6 +176,15 @@ def main():
         f"scan {scan_s * 1e3:.2f} ms = {achieved:.0f} GB/s; route {route_ns / 1e6 / a.steps:.2f} ms; "
         f"rerank {rerank_ns / 1e6 / a.steps:.2f} ms")
 
+    if a.copy_probe:
+        src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: 1073741824 bytes read, 1073741824 bytes written")
+        del src, dst
+
     # Single-query latency: the plain per-query GPU path (batch of one, synchronous).
     lat = []
     for i in range(min(a.latency_queries, nq)):
@@ -192,7 +208,7 @@ def main():
         "data": "synthetic: seeded anisotropic Gaussian mixture on the unit sphere (lognormal weights), "
                 "queries = mixture draws + N(0,0.05^2) noise; index built on the GPU in the reference's format",
         "config": {"workload": W["desc"], "rows": W["rows"], "clusters": ix.k, "queries_per_step": nq,
-                   "search": pick["config"], "k": 10},
+                   "search": pick["config"], "k": 10, "mixture_spread": spread},
         "recall_at_10": round(rec, 4),
         "p50_ms": round(p50, 3), "p99_ms": round(p99, 3),
         "per_query_gpu_qps": round(1e3 / float(np.mean(lat)), 1),
