@@ -710,7 +710,7 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
         a.scanned = cnt;
         const size_t lds = scan_lds_bytes(a.cand, pstride);
         SIEVE_REQUIRE(lds <= 160 * 1024, SIEVE_ELIMIT, "scan workspace exceeds LDS");
-        launch_scan_l2(a, nq, s->st);
+        launch_scan_l2(a, nq, ix->scan_variant, s->st);
         SIEVE_HIP(hipGetLastError());
         SIEVE_HIP(hipEventRecord(s->ev[2], s->st));
         L2RerankArgs r{};
@@ -760,6 +760,61 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
             t->scanned += hcnt[0];
             t->probed += s->last.probed;
         }
+    });
+}
+
+int sieve_index_autotune(sieve_index* ix, sieve_stream* s, const float* q768, const float* q256, size_t nq,
+                         const sieve_params* p, uint32_t* chosen, float* ms_per_variant) {
+    return guarded([&] {
+        SIEVE_REQUIRE(ix && s && q768 && q256 && p, SIEVE_EINVAL, "null argument");
+        check_params(p, true);
+        SIEVE_REQUIRE(nq >= 1, SIEVE_EINVAL, "autotune needs at least one query");
+        const int pstride = (int)std::min<uint32_t>(p->nprobe, ix->k);
+        const float* q2 = dev_in(s, B_Q256, q256, nq * PDIM);
+        const float* q7 = dev_in(s, B_Q768, q768, nq * DIM);
+        unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 4);
+        float* ps = grow<float>(s->b[B_PS], nq * pstride);
+        uint32_t* pc = grow<uint32_t>(s->b[B_PC], nq * pstride);
+        uint32_t* pn = grow<uint32_t>(s->b[B_PN], nq);
+        float* ce = grow<float>(s->b[B_CE], nq * p->cand);
+        uint32_t* cr = grow<uint32_t>(s->b[B_CR], nq * p->cand);
+        float* cx = grow<float>(s->b[B_CX], nq * p->cand);
+        uint32_t* cn = grow<uint32_t>(s->b[B_CN], nq);
+        Batch bt;
+        prep_queries(ix, s, q2, q7, nq, bt);
+        route_batch(ix, s, bt, nq, p, ps, pc, pn, pstride);
+        check_overflow(s);
+        int best = 0;
+        float best_ms = 0.0f;
+        const int saved = ix->scan_variant;
+        for (int v = 0; v < SCAN_VARIANTS; v++) {
+            ix->scan_variant = v;
+            float ms = 0.0f;
+            for (int rep = 0; rep < 3; rep++) {  // first run warms up; keep the faster of the rest
+                SIEVE_HIP(hipEventRecord(s->ev[0], s->st));
+                scan_batch(ix, s, bt, nq, ps, pc, pn, pstride, p, ViewArgs{}, ce, cr, cx, cn, (int)p->cand, cnt);
+                SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
+                SIEVE_HIP(hipEventSynchronize(s->ev[1]));
+                float t = elapsed_ns(s->ev[0], s->ev[1]) / 1e6f;
+                if (rep == 1 || (rep > 1 && t < ms)) ms = t;
+            }
+            if (ms_per_variant) ms_per_variant[v] = ms;
+            if (v == 0 || ms < best_ms) {
+                best = v;
+                best_ms = ms;
+            }
+        }
+        (void)saved;
+        ix->scan_variant = best;
+        if (chosen) *chosen = (uint32_t)best;
+    });
+}
+
+int sieve_index_set_scan_variant(sieve_index* ix, uint32_t variant) {
+    return guarded([&] {
+        SIEVE_REQUIRE(ix, SIEVE_EINVAL, "null index");
+        SIEVE_REQUIRE(variant < (uint32_t)SCAN_VARIANTS, SIEVE_EINVAL, "no such scan variant");
+        ix->scan_variant = (int)variant;
     });
 }
 

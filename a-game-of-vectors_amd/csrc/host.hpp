@@ -93,6 +93,7 @@ struct sieve_index {
     uint8_t* t2_codes = nullptr;
     float* t2_norm = nullptr;
     size_t device_bytes = 0;
+    int scan_variant = 0;  // scan kernel variant (sieve_index_autotune)
 
     ~sieve_index() {
         void* ps[] = {cents, cents_bf, block_off, cl_slots, codes, mul, err, ids, rotation, quant, norm256, t2_sigma,
@@ -120,6 +121,7 @@ struct sieve_l2_index {
     int8_t* base = nullptr;
     bool base_owned = true;
     size_t device_bytes = 0;
+    int scan_variant = 0;
     ~sieve_l2_index() {
         void* ps[] = {leaves, leaves_bf, leaf_norm, block_off, codes, mul, nx, ids, rotation};
         for (void* p : ps)
@@ -332,7 +334,7 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
     a.scanned = scanned;
     const size_t lds = scan_lds_bytes(a.cand, pstride);
     SIEVE_REQUIRE(lds <= 160 * 1024, SIEVE_ELIMIT, "scan workspace exceeds LDS");
-    launch_scan(a, nq, s->st);
+    launch_scan(a, nq, ix->scan_variant, s->st);
     SIEVE_HIP(hipGetLastError());
 }
 

@@ -93,8 +93,12 @@ struct ScanArgs {
     int out_stride;
     unsigned long long* scanned; // [0] += stored slots scanned, [2] += slots past the threshold
 };
+// Upper bound of the scan's LDS over all variants.
 size_t scan_lds_bytes(int cand, int probe_stride);
-void launch_scan(const ScanArgs& a, size_t nq, hipStream_t st);
+// Scan kernel variants (slots per thread, prefetch), picked per index by the autotuner.
+constexpr int SCAN_VARIANTS = 4;
+const char* scan_variant_name(int variant);
+void launch_scan(const ScanArgs& a, size_t nq, int variant, hipStream_t st);
 
 // The standalone lut::scan seam over the reference's block layout.
 void launch_lut_scan(const uint8_t* tables, int groups, const uint8_t* blocks, size_t nblocks, uint16_t* out,
@@ -140,7 +144,7 @@ struct L2Prep {
 void launch_query_prep_l2(const int8_t* q, size_t nq, const float* signs, float rot_norm, const L2Prep& out,
                           hipStream_t st);
 void launch_split_leaves(const float* leaves, uint32_t k, uint16_t* out, hipStream_t st);
-void launch_scan_l2(const ScanArgs& a, size_t nq, hipStream_t st);
+void launch_scan_l2(const ScanArgs& a, size_t nq, int variant, hipStream_t st);
 
 struct L2RerankArgs {
     const int8_t* q;           // nq x 100

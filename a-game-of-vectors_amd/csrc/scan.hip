@@ -25,18 +25,16 @@
 namespace sieve {
 
 constexpr int SCAN_THREADS = 256;
-constexpr int SLOTS_PER_THREAD = 4;
-constexpr int ROUND_BLOCKS = SCAN_THREADS * SLOTS_PER_THREAD / BLOCK;  // 32
-constexpr int ROUND_SLOTS = ROUND_BLOCKS * BLOCK;                     // 1024
 
 struct ScanLds {
     int cap;
     size_t off_pref, off_start, off_cent, off_buf, off_tmp, off_hist, off_misc, total;
 };
 
-__host__ __device__ inline ScanLds scan_layout(int cand, int probe_stride) {
+// slots: slots per thread per round (a round covers 256 * slots slots)
+__host__ __device__ inline ScanLds scan_layout(int cand, int probe_stride, int slots) {
     ScanLds l;
-    l.cap = 2 * cand + ROUND_SLOTS;
+    l.cap = 2 * cand + SCAN_THREADS * slots;
     size_t o = 0;
     auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
     l.off_pref = o;
@@ -57,7 +55,9 @@ __host__ __device__ inline ScanLds scan_layout(int cand, int probe_stride) {
     return l;
 }
 
-size_t scan_lds_bytes(int cand, int probe_stride) { return scan_layout(cand, probe_stride).total; }
+size_t scan_lds_bytes(int cand, int probe_stride) {
+    return scan_layout(cand, probe_stride, 8).total + 1024;  // largest variant, plus the static tables
+}
 
 // Cuts buf[0, n) to its best k entries (under (est key, ~slot)) and raises the threshold.
 __device__ void scan_compact(uint2* buf, uint2* tmp, int n, int k, uint32_t* hist, uint32_t* misc) {
@@ -84,14 +84,17 @@ __device__ void scan_compact(uint2* buf, uint2* tmp, int n, int k, uint32_t* his
 
 // G = 64 (256-bit codes, inner-product estimate) or G = 32 (128-bit codes of the padded
 // 100-d int8 vectors, L2 estimate 2 (cent + mul (a sc + b)) - |x|^2, src/bigann.rs:705-711).
-template <int G, bool L2>
+// SLOTS_PER_THREAD slots per thread per round; PF: prefetch the next round's codes into
+// registers while scoring the current one (more bytes in flight, more VGPRs).
+template <int G, bool L2, int SLOTS_PER_THREAD, bool PF>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_t(ScanArgs a) {
     constexpr int CW = G / 32;  // 16-byte words per slot code
+    constexpr int ROUND_BLOCKS = SCAN_THREADS * SLOTS_PER_THREAD / BLOCK;
     // The query's tables sit in static LDS at a compile-time address, so each lookup is one
     // ds_read_u8 with the table offset in its immediate field.
     __shared__ __align__(16) uint8_t tab[G * 16];
     extern __shared__ __align__(16) uint8_t lds[];
-    const ScanLds L = scan_layout(a.cand, a.probe_stride);
+    const ScanLds L = scan_layout(a.cand, a.probe_stride, SLOTS_PER_THREAD);
     uint32_t* pref = reinterpret_cast<uint32_t*>(lds + L.off_pref);
     uint32_t* pstart = reinterpret_cast<uint32_t*>(lds + L.off_start);
     float* pcent = reinterpret_cast<float*>(lds + L.off_cent);
@@ -135,79 +138,109 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_t(ScanArgs a) {
     const int cand = a.cand;
     uint32_t id_reads = 0;  // slots past the threshold: their id is read
 
-    for (uint32_t vb0 = 0; vb0 < total; vb0 += ROUND_BLOCKS) {
+    // A thread's blocks only move forward (vb0 + (t >> 5) + 8u, rounds in order), so a cursor
+    // over the probe prefix sums finds each block's probed cluster in amortised O(1).
+    int cur = 0;
+    struct Regs {
         uint4 cw[SLOTS_PER_THREAD][CW];
         float mul[SLOTS_PER_THREAD], cent[SLOTS_PER_THREAD];
         uint32_t gs[SLOTS_PER_THREAD];
         bool ok[SLOTS_PER_THREAD];
+    };
+    auto fetch = [&](uint32_t vb0, Regs& R) {
 #pragma unroll
         for (int u = 0; u < SLOTS_PER_THREAD; u++) {
-            uint32_t vb = vb0 + (t >> 5) + 8 * u;
-            ok[u] = vb < total;
-            gs[u] = 0;
-            cent[u] = 0.0f;
-            if (ok[u]) {
-                int lo = 0, hi = np - 1;  // last j with pref[j] <= vb
-                while (lo < hi) {
-                    int mid = (lo + hi + 1) >> 1;
-                    if (pref[mid] <= vb) lo = mid;
-                    else hi = mid - 1;
-                }
-                gs[u] = (pstart[lo] + (vb - pref[lo])) * BLOCK + (t & 31);
-                cent[u] = pcent[lo];
-                const uint4* cp = a.codes + CW * (size_t)gs[u];
+            const uint32_t vb = vb0 + (t >> 5) + 8 * u;
+            R.ok[u] = vb < total;
+            R.gs[u] = 0;
+            R.cent[u] = 0.0f;
+            if (R.ok[u]) {
+                while (cur + 1 < np && pref[cur + 1] <= vb) cur++;
+                R.gs[u] = (pstart[cur] + (vb - pref[cur])) * BLOCK + (t & 31);
+                R.cent[u] = pcent[cur];
+                const uint4* cp = a.codes + CW * (size_t)R.gs[u];
 #pragma unroll
-                for (int w = 0; w < CW; w++) cw[u][w] = cp[w];
-                mul[u] = a.mul[gs[u]];
+                for (int w = 0; w < CW; w++) R.cw[u][w] = cp[w];
+                R.mul[u] = a.mul[R.gs[u]];
             }
         }
+    };
+    // Scores, estimates and admits the slots in R (round `round`); the wave whose append takes
+    // the buffer to 2 * cand stamps the round so that every wave compacts after the barrier.
+    auto score = [&](const Regs& R, uint32_t round) {
         const bool has_thr = misc[1] != 0;
         const uint32_t thr = misc[2];
 #pragma unroll
         for (int u = 0; u < SLOTS_PER_THREAD; u++) {
             bool pass = false;
             uint32_t key = 0;
-            if (ok[u]) {
-                uint32_t w[2 * CW * 2];
+            if (R.ok[u]) {
+                uint32_t w[4 * CW];
 #pragma unroll
                 for (int x = 0; x < CW; x++) {
-                    w[4 * x] = cw[u][x].x;
-                    w[4 * x + 1] = cw[u][x].y;
-                    w[4 * x + 2] = cw[u][x].z;
-                    w[4 * x + 3] = cw[u][x].w;
+                    w[4 * x] = R.cw[u][x].x;
+                    w[4 * x + 1] = R.cw[u][x].y;
+                    w[4 * x + 2] = R.cw[u][x].z;
+                    w[4 * x + 3] = R.cw[u][x].w;
                 }
                 uint32_t sc = 0;
 #pragma unroll
                 for (int d = 0; d < G / 8; d++)
 #pragma unroll
                     for (int j = 0; j < 8; j++) sc += tab[(8 * d + j) * 16 + ((w[d] >> (4 * j)) & 15u)];
-                float est = cent[u] + mul[u] * (qab.x * (float)sc + qab.y);
-                if constexpr (L2) est = 2.0f * est - a.err[gs[u]];
+                float est = R.cent[u] + R.mul[u] * (qab.x * (float)sc + qab.y);
+                if constexpr (L2) est = 2.0f * est - a.err[R.gs[u]];
                 est = est + 0.0f;  // -0.0 -> +0.0 so keys order like the floats compare
                 key = f2key(est);
                 pass = !has_thr || key > thr;
                 if (pass) {
                     id_reads++;
-                    uint32_t id = a.ids[gs[u]];
+                    const uint32_t id = a.ids[R.gs[u]];
                     pass = id != PAD;
                     if (pass && a.view.deleted && bit_set(a.view.deleted, a.view.deleted_words, id)) pass = false;
                     if (pass && allow && !bit_set(allow, a.view.allow_words, id)) pass = false;
                 }
             }
-            uint64_t bal = __ballot(pass);
+            const uint64_t bal = __ballot(pass);
             if (bal) {
-                uint32_t nw = (uint32_t)__popcll(bal);
-                uint32_t before = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                const uint32_t nw = (uint32_t)__popcll(bal);
+                const uint32_t before = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
                 uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&misc[0], nw);
+                if (lane == 0) {
+                    base = atomicAdd(&misc[0], nw);
+                    if (base + nw >= 2u * (uint32_t)cand) misc[12 + (round & 1)] = round;
+                }
                 base = __shfl(base, 0, 64);
-                if (pass) buf[base + before] = make_uint2(key, ~gs[u]);
+                if (pass) buf[base + before] = make_uint2(key, ~R.gs[u]);
             }
         }
+    };
+    // One round: prefetch the next round's codes, score this one, then a single barrier. The
+    // stamp of round r lives in misc[12 + (r & 1)]: a wave already appending in round r + 1
+    // writes the other word, and round r + 2 cannot start before every wave passed barrier r + 1.
+    auto step = [&](Regs& now, Regs& next, uint32_t vb0, uint32_t round) {
+        if constexpr (PF) {
+            if (vb0 + ROUND_BLOCKS < total) fetch(vb0 + ROUND_BLOCKS, next);
+        } else {
+            fetch(vb0, now);
+        }
+        score(now, round);
         __syncthreads();
-        const int count = (int)misc[0];
-        __syncthreads();  // every wave has read the count before the next round appends
-        if (count >= 2 * cand) scan_compact(buf, tmp, count, cand, hist, misc);
+        if (misc[12 + (round & 1)] == round) scan_compact(buf, tmp, (int)misc[0], cand, hist, misc);
+    };
+    if (t == 0) {
+        misc[12] = 0xFFFFFFFFu;
+        misc[13] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    Regs ra, rb;
+    if (PF && total > 0) fetch(0, ra);
+    for (uint32_t vb0 = 0, round = 0; vb0 < total;) {
+        step(ra, rb, vb0, round++);
+        vb0 += ROUND_BLOCKS;
+        if (vb0 >= total) break;
+        step(rb, ra, vb0, round++);
+        vb0 += ROUND_BLOCKS;
     }
     id_reads = wave_sum(id_reads);
     if (lane == 0 && a.scanned) atomicAdd(a.scanned + 2, (unsigned long long)id_reads);
@@ -267,20 +300,39 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_t(ScanArgs a) {
     if (t == 0) a.out_n[qi] = tot;
 }
 
-void launch_scan(const ScanArgs& a, size_t nq, hipStream_t st) {
-    if (nq == 0) return;
-    const size_t lds = scan_lds_bytes(a.cand, a.probe_stride);
+template <int G, bool L2, int S, bool PF>
+static void launch_variant(const ScanArgs& a, size_t nq, hipStream_t st) {
+    const size_t lds = scan_layout(a.cand, a.probe_stride, S).total;
     if (lds > 64 * 1024)
-        (void)hipFuncSetAttribute((const void*)k_scan_t<64, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_scan_t<64, false>), dim3((unsigned)nq), dim3(SCAN_THREADS), lds, st, a);
+        (void)hipFuncSetAttribute((const void*)k_scan_t<G, L2, S, PF>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+    hipLaunchKernelGGL((k_scan_t<G, L2, S, PF>), dim3((unsigned)nq), dim3(SCAN_THREADS), lds, st, a);
 }
 
-void launch_scan_l2(const ScanArgs& a, size_t nq, hipStream_t st) {
+template <int G, bool L2>
+static void launch_any(const ScanArgs& a, size_t nq, int variant, hipStream_t st) {
+    switch (variant) {
+        case 0: launch_variant<G, L2, 4, false>(a, nq, st); break;
+        case 1: launch_variant<G, L2, 2, true>(a, nq, st); break;
+        case 2: launch_variant<G, L2, 4, true>(a, nq, st); break;
+        default: launch_variant<G, L2, 8, false>(a, nq, st); break;
+    }
+}
+
+const char* scan_variant_name(int v) {
+    static const char* names[SCAN_VARIANTS] = {"4 slots/thread", "2 slots/thread + prefetch",
+                                               "4 slots/thread + prefetch", "8 slots/thread"};
+    return v >= 0 && v < SCAN_VARIANTS ? names[v] : "?";
+}
+
+void launch_scan(const ScanArgs& a, size_t nq, int variant, hipStream_t st) {
     if (nq == 0) return;
-    const size_t lds = scan_lds_bytes(a.cand, a.probe_stride);
-    if (lds > 64 * 1024)
-        (void)hipFuncSetAttribute((const void*)k_scan_t<32, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_scan_t<32, true>), dim3((unsigned)nq), dim3(SCAN_THREADS), lds, st, a);
+    launch_any<64, false>(a, nq, variant, st);
+}
+
+void launch_scan_l2(const ScanArgs& a, size_t nq, int variant, hipStream_t st) {
+    if (nq == 0) return;
+    launch_any<32, true>(a, nq, variant, st);
 }
 
 // ------------------------------------------------------------------ lut::scan seam -----

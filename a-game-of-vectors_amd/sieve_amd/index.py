@@ -172,6 +172,20 @@ class Index:
                                          ctypes.byref(t)))
         return s, r, n
 
+    def autotune(self, q768, q256, p: Params, stream=None):
+        """Times every scan-kernel variant on this batch and keeps the fastest:
+        (chosen variant, ms per variant)."""
+        q768, q256 = self._in(q768, np.float32), self._in(q256, np.float32)
+        ch = ctypes.c_uint32()
+        ms = np.zeros(4, np.float32)
+        st = stream or self.stream()
+        abi.check(abi.lib().sieve_index_autotune(self._h, st._h, abi.ptr(q768), abi.ptr(q256), q256.shape[0],
+                                                 ctypes.byref(abi.make_params(p)), ctypes.byref(ch), abi.ptr(ms)))
+        return ch.value, ms.tolist()
+
+    def set_scan_variant(self, v):
+        abi.check(abi.lib().sieve_index_set_scan_variant(self._h, v))
+
     def search(self, q768, q256, p: Params, view=None, timing=None, stream=None):
         """Index::search / search_view for a batch: (scores nq x k, rows, counts)."""
         q768, q256 = self._in(q768, np.float32), self._in(q256, np.float32)

@@ -201,6 +201,17 @@ int sieve_index_create(const sieve_index_desc* desc, sieve_index** out);
 void sieve_index_free(sieve_index* ix);
 int sieve_index_get_info(const sieve_index* ix, sieve_index_info* out);
 
+/* Scan-kernel autotuner (no counterpart in the reference): times each variant of the scan
+ * kernel (slots per thread per round, prefetch of the next round's codes) on this batch of
+ * queries with these parameters, and keeps the fastest for later calls on this index. Call it
+ * before sharing the index across threads. chosen (may be NULL) gets the variant, and
+ * ms_per_variant (NULL or SIEVE_SCAN_VARIANTS floats) the time of each. */
+#define SIEVE_SCAN_VARIANTS 4
+int sieve_index_autotune(sieve_index* ix, sieve_stream* s, const float* q768, const float* q256, size_t nq,
+                         const sieve_params* p, uint32_t* chosen, float* ms_per_variant);
+/* Fixes the scan variant (0 .. SIEVE_SCAN_VARIANTS - 1) instead of tuning it. */
+int sieve_index_set_scan_variant(sieve_index* ix, uint32_t variant);
+
 /* ------------------------------------------------------------- the path --- */
 
 /* Index::route (src/ivf.rs:587-613) for nq queries. q256 is nq x 256 (unit rows).

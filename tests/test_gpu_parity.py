@@ -201,6 +201,29 @@ def test_search_matches_oracle(small, oracle_index, gpu_index, cfg):
     assert tm.probed == wt["probed"] and tm.scanned == wt["scanned"]
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_scan_variants_match_oracle(small, oracle_index, gpu_index, variant):
+    from sieve_amd import abi
+    p = Params(nprobe=16, cand=700, k=10, rerank=Rerank.FULL)
+    gpu_index.set_scan_variant(variant)
+    try:
+        tm = abi.Timing()
+        s, r, n = gpu_index.search(small.q768, small.q256, p, timing=tm)
+    finally:
+        gpu_index.set_scan_variant(0)
+    ws, wr, wn, wt = oracle_index.search_batch(small.q768, small.q256, p, threads=4)
+    for i in range(small.nq):
+        assert_same_up_to_ties(r[i, :n[i]], s[i, :n[i]], wr[i, :wn[i]], ws[i, :wn[i]])
+    assert tm.scanned == wt["scanned"]
+
+
+def test_autotune_picks_a_variant(small, gpu_index):
+    p = Params(nprobe=16, cand=700, k=10)
+    v, ms = gpu_index.autotune(small.q768, small.q256, p)
+    assert 0 <= v < 4 and len(ms) == 4 and all(x > 0 for x in ms) and ms[v] == min(ms)
+    gpu_index.set_scan_variant(0)
+
+
 def test_device_arrays_match_host_arrays(small, gpu_index):
     import torch
     p = Params(nprobe=12, cand=400, k=10)
