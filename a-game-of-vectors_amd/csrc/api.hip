@@ -170,6 +170,7 @@ sieve_l2_index* create_l2_index(const sieve_l2_desc* d) {
     std::vector<uint32_t> bo32(d->k + 1);
     for (uint32_t c = 0; c <= d->k; c++) {
         if (c) SIEVE_REQUIRE(bo[c] >= bo[c - 1], SIEVE_EINVAL, "block_off is not monotone");
+        if (c) ix->max_cluster_blocks = std::max<uint32_t>(ix->max_cluster_blocks, (uint32_t)(bo[c] - bo[c - 1]));
         bo32[c] = (uint32_t)bo[c];
     }
     const float* lv = host_view(d->leaves, (size_t)d->k * D, hc);
@@ -708,6 +709,8 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
         a.out_n = cn;
         a.out_stride = (int)p->cand;
         a.scanned = cnt;
+        set_scan_plan(a, s, nq, (uint32_t)pstride, (double)ix->nblocks / ix->k, ix->max_cluster_blocks,
+                      ix->scan_chunk_blocks);
         const size_t lds = scan_lds_bytes(a.cand, pstride);
         SIEVE_REQUIRE(lds <= 160 * 1024, SIEVE_ELIMIT, "scan workspace exceeds LDS");
         launch_scan_l2(a, nq, ix->scan_variant, s->st);
@@ -815,6 +818,20 @@ int sieve_index_set_scan_variant(sieve_index* ix, uint32_t variant) {
         SIEVE_REQUIRE(ix, SIEVE_EINVAL, "null index");
         SIEVE_REQUIRE(variant < (uint32_t)SCAN_VARIANTS, SIEVE_EINVAL, "no such scan variant");
         ix->scan_variant = (int)variant;
+    });
+}
+
+int sieve_index_set_scan_chunk(sieve_index* ix, uint32_t chunk_blocks) {
+    return guarded([&] {
+        SIEVE_REQUIRE(ix, SIEVE_EINVAL, "null index");
+        ix->scan_chunk_blocks = chunk_blocks;
+    });
+}
+
+int sieve_l2_index_set_scan_chunk(sieve_l2_index* ix, uint32_t chunk_blocks) {
+    return guarded([&] {
+        SIEVE_REQUIRE(ix, SIEVE_EINVAL, "null index");
+        ix->scan_chunk_blocks = chunk_blocks;
     });
 }
 

@@ -52,6 +52,18 @@ T* grow(DevBuf& b, size_t n) {
     return static_cast<T*>(b.p);
 }
 
+// grow() whose new allocations start zeroed (counters that kernels leave at zero).
+template <class T>
+T* grow_zeroed(DevBuf& b, size_t n, hipStream_t st) {
+    void* before = b.p;
+    T* p = grow<T>(b, n);
+    if (b.p != before) {
+        hipError_t e = hipMemsetAsync(b.p, 0, b.cap, st);
+        if (e != hipSuccess) throw Error{SIEVE_EHIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e)};
+    }
+    return p;
+}
+
 template <class T>
 T* dmalloc(size_t n, size_t* acct) {
     void* p = nullptr;
@@ -94,6 +106,7 @@ struct sieve_index {
     float* t2_norm = nullptr;
     size_t device_bytes = 0;
     int scan_variant = 0;  // scan kernel variant (sieve_index_autotune)
+    uint32_t scan_chunk_blocks = 0;  // 0: chunk size chosen per batch (sieve_index_set_scan_chunk)
 
     ~sieve_index() {
         void* ps[] = {cents, cents_bf, block_off, cl_slots, codes, mul, err, ids, rotation, quant, norm256, t2_sigma,
@@ -108,6 +121,7 @@ struct sieve_l2_index {
     int device = 0;
     uint32_t k = 0;
     uint64_t n = 0, nblocks = 0, nslots = 0;
+    uint32_t max_cluster_blocks = 0;
     float* leaves = nullptr;       // k x 100 f32
     uint16_t* leaves_bf = nullptr; // k x 256 BF16 [hi | lo]
     float* leaf_norm = nullptr;    // k: |c|^2 summed left to right
@@ -122,6 +136,7 @@ struct sieve_l2_index {
     bool base_owned = true;
     size_t device_bytes = 0;
     int scan_variant = 0;
+    uint32_t scan_chunk_blocks = 0;
     ~sieve_l2_index() {
         void* ps[] = {leaves, leaves_bf, leaf_norm, block_off, codes, mul, nx, ids, rotation};
         for (void* p : ps)
@@ -133,6 +148,7 @@ struct sieve_l2_index {
 enum Slot {
     B_Q256, B_Q768, B_TABLES, B_QAB, B_WQ768, B_WQ256, B_INV, B_QBF, B_QNORM, B_APPROX, B_PS, B_PC, B_PN, B_CE, B_CR,
     B_CX, B_CN, B_OS, B_OR, B_ON, B_DEL, B_ALLOW, B_AFLAGS, B_CNT, B_OVF, B_TMP1, B_TMP2, B_TMP3, B_QQ, B_QF,
+    B_PART, B_PARTN, B_GTHR, B_QDONE,
     B_NSLOTS
 };
 
@@ -309,6 +325,21 @@ inline void check_overflow(sieve_stream* s) {
                       " centroids within the BF16 error band of the nprobe-th score");
 }
 
+// The work split of a batch's scan and its partial-list scratch (per stream).
+inline void set_scan_plan(ScanArgs& a, sieve_stream* s, size_t nq, uint32_t nprobe, double mean_blocks,
+                          uint32_t max_cluster_blocks, uint32_t forced_chunk_blocks) {
+    const ScanPlan pl = plan_scan(nq, nprobe, mean_blocks, max_cluster_blocks, a.cand, forced_chunk_blocks);
+    a.chunk_blocks = pl.chunk_blocks;
+    a.chunk_max = pl.chunk_max;
+    SIEVE_REQUIRE(nq * pl.chunk_max < (1ull << 31), SIEVE_ELIMIT, "scan grid too large");
+    if (pl.chunk_max > 1) {
+        a.part = grow<uint2>(s->b[B_PART], nq * pl.chunk_max * (size_t)a.cand);
+        a.part_n = grow<uint32_t>(s->b[B_PARTN], nq * pl.chunk_max);
+        a.gthr = grow_zeroed<unsigned long long>(s->b[B_GTHR], nq, s->st);
+        a.qdone = grow_zeroed<uint32_t>(s->b[B_QDONE], nq, s->st);
+    }
+}
+
 inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, size_t nq, const float* ps,
                 const uint32_t* pc, const uint32_t* pn, int pstride, const sieve_params* p, const ViewArgs& view,
                 float* ce, uint32_t* cr, float* cx, uint32_t* cn, int cstride, unsigned long long* scanned) {
@@ -332,6 +363,8 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
     a.out_n = cn;
     a.out_stride = cstride;
     a.scanned = scanned;
+    set_scan_plan(a, s, nq, (uint32_t)pstride, (double)ix->nblocks / ix->k, ix->max_cluster_blocks,
+                  ix->scan_chunk_blocks);
     const size_t lds = scan_lds_bytes(a.cand, pstride);
     SIEVE_REQUIRE(lds <= 160 * 1024, SIEVE_ELIMIT, "scan workspace exceeds LDS");
     launch_scan(a, nq, ix->scan_variant, s->st);

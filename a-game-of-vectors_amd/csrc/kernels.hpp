@@ -92,7 +92,23 @@ struct ScanArgs {
     uint32_t* out_n;
     int out_stride;
     unsigned long long* scanned; // [0] += stored slots scanned, [2] += slots past the threshold
+    // Work split: query q's probed blocks (T_q) are cut into clamp(ceil(T_q / chunk_blocks), 1,
+    // chunk_max) chunks; workgroup q * chunk_max + c scans chunk c.
+    uint32_t chunk_blocks;
+    uint32_t chunk_max;
+    uint2* part;                 // (nq * chunk_max) x cand partial lists (split queries only)
+    uint32_t* part_n;            // nq * chunk_max
+    unsigned long long* gthr;    // nq, zero between launches: best shared (key, ~slot) threshold
+    uint32_t* qdone;             // nq, zero between launches: chunks finished
 };
+// Chunking of a batch's scan: fills the work-split fields of `a` (a.cand set) for nq queries of
+// up to nprobe probes over clusters of at most max_cluster_blocks blocks (mean_blocks on average);
+// forced_chunk_blocks != 0 fixes the chunk size.
+struct ScanPlan {
+    uint32_t chunk_blocks, chunk_max;
+};
+ScanPlan plan_scan(size_t nq, uint32_t nprobe, double mean_blocks, uint32_t max_cluster_blocks, int cand,
+                   uint32_t forced_chunk_blocks);
 // Upper bound of the scan's LDS over all variants.
 size_t scan_lds_bytes(int cand, int probe_stride);
 // Scan kernel variants (slots per thread, prefetch), picked per index by the autotuner.

@@ -140,6 +140,8 @@ def lib():
         "sieve_search_batch": (i32, [vp, vp, vp, vp, sz, P, V, vp, vp, vp, T]),
         "sieve_index_autotune": (i32, [vp, vp, vp, vp, sz, P, ctypes.POINTER(ctypes.c_uint32), vp]),
         "sieve_index_set_scan_variant": (i32, [vp, u32]),
+        "sieve_index_set_scan_chunk": (i32, [vp, u32]),
+        "sieve_l2_index_set_scan_chunk": (i32, [vp, u32]),
         "sieve_server_create": (i32, [vp, P, ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)]),
         "sieve_server_submit": (i32, [vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)]),
         "sieve_server_wait": (i32, [vp, ctypes.c_uint64, vp, vp, ctypes.POINTER(ctypes.c_uint32),

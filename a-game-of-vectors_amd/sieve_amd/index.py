@@ -186,6 +186,10 @@ class Index:
     def set_scan_variant(self, v):
         abi.check(abi.lib().sieve_index_set_scan_variant(self._h, v))
 
+    def set_scan_chunk(self, blocks):
+        """Chunk size (32-slot blocks) of the scan's work split; 0 = chosen per batch."""
+        abi.check(abi.lib().sieve_index_set_scan_chunk(self._h, blocks))
+
     def search(self, q768, q256, p: Params, view=None, timing=None, stream=None):
         """Index::search / search_view for a batch: (scores nq x k, rows, counts)."""
         q768, q256 = self._in(q768, np.float32), self._in(q256, np.float32)
@@ -234,6 +238,10 @@ class L2Index:
         except Exception:
             pass
         self._h = None
+
+    def set_scan_chunk(self, blocks):
+        """Chunk size (32-slot blocks) of the scan's work split; 0 = chosen per batch."""
+        abi.check(abi.lib().sieve_l2_index_set_scan_chunk(self._h, blocks))
 
     def search(self, q, nprobe, cand, k, margin=0.0, timing=None, stream=None, details=False):
         """bigann::Index::search for a batch of int8 queries (nq x 100): (scores, rows, counts)

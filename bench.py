@@ -9,7 +9,7 @@ R@10 >= 0.90 against exact flat search, p50/p99 single-query latency, the scan k
 roofline fraction (HIP events around the scan kernel, algorithmic bytes), and the CPU
 oracle (a restatement of the reference's x86 search) timed on this host on a bounded sample.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload c2]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload c3]
 """
 import argparse
 import json
@@ -27,17 +27,32 @@ HBM_PEAK_GBS = 8000.0
 # name, rows, mixture components, clusters, queries per batch, swept configurations
 # (nprobe:cand:rerank[:bM[xW]][:pD], src/bin/search.rs:180-218).
 WORKLOADS = {
+    "c3": dict(desc="113.5M x 768-d, 16384 Zipf(1.1) components, 16384 clusters, SOAR, RaBitQ 1-bit + 2-bit tier, "
+                    "87 GB int8 rows in HBM, 10k-query batches, 64 probes, tiered rerank",
+               rows=113_500_000, components=16384, clusters=16384, queries=10_000, weights="zipf", tier2=True,
+               topics=131072, topic_spread=0.08, spread=0.05,
+               sweep=["64:1000:full:b10x0.8", "64:1000:full", "64:2000:tiered200", "64:2000:tiered400",
+                      "64:1000:full:b10x0.8:p0.1", "48:1000:full:b10x0.8"]),
     "c2": dict(desc="18.4M x 768-d, 8192 clusters, SOAR, RaBitQ 1-bit, 10k-query batches, margin probing, "
-                    "500 candidates with bound cut",
+                    "bound-cut rerank",
                rows=18_400_000, components=8192, clusters=8192, queries=10_000,
-               sweep=["32:500:full:b10x0.8", "64:500:full:b10x0.8", "128:500:full:b10x0.8",
-                      "128:500:full:b10x0.8:p0.08", "128:500:full:b10x0.8:p0.12"]),
+               sweep=["32:1000:full:b10x0.8", "64:1000:full:b10x0.8", "128:1000:full:b10x0.8",
+                      "128:1000:full:b10x0.8:p0.1", "128:2000:full:b10x0.8"]),
     "c1": dict(desc="1M x 768-d, 1024 clusters, SOAR, RaBitQ 1-bit, 1000 queries, nprobe 32, 1000 candidates",
                rows=1_000_000, components=1024, clusters=1024, queries=1000,
                sweep=["16:1000:full", "32:1000:full", "64:1000:full"]),
     "tiny": dict(desc="200k x 768-d, 256 clusters (quick check)", rows=200_000, components=256, clusters=256,
                  queries=2000, sweep=["8:500:full", "16:500:full", "32:500:full"]),
 }
+# Mixture recipe: query noise N(0, 0.05^2) per dimension throughout. Flat components of spread
+# 0.1 at 18.4M rows: recall rises with nprobe and R@10 0.90 is reachable (64 probes, 1000
+# candidates); narrower components (0.02-0.045) tie the 10th/11th neighbours closer than the
+# 1-bit estimate resolves, wider ones (>= 0.14) spread the similarity outside the 256-d prefix.
+# At 113.5M rows under Zipf(1.1) weights flat components hold millions of rows each and no
+# setting reaches 0.90 (the exact 256-d prefix finds 99% of the top 10 among its best 2000, the
+# 1-bit estimate 70-78%), so that workload adds 131072 topics (spread 0.08) inside the
+# components, rows at spread 0.05 around them: R@10 0.92 at 64 probes, 1000 candidates.
+SPREAD = 0.1
 
 
 def log(msg):
@@ -72,7 +87,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--gt-queries", type=int, default=1000, help="queries with exact ground truth")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample length")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -82,6 +97,9 @@ def main():
     ap.add_argument("--sweep", default=None, help="comma-separated configurations to sweep instead of the "
                                                  "workload's list")
     ap.add_argument("--spread", type=float, default=None, help="mixture component spread (data recipe)")
+    ap.add_argument("--serve-seconds", type=float, default=3.0, help="open-loop serving run per rate (0: skip)")
+    ap.add_argument("--serve-rates", default="2000,20000", help="offered Poisson arrival rates, queries/s")
+    ap.add_argument("--serve-max-batch", type=int, default=512)
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, copy one 1 GiB device buffer (a plain copy of known size, "
                          "the reference that scales the FETCH_SIZE/WRITE_SIZE counters of a profile)")
@@ -100,19 +118,26 @@ def main():
     W = WORKLOADS[a.workload]
     dev = torch.device("cuda:0")
     t_build0 = time.time()
-    spread = a.spread if a.spread is not None else W.get("spread", synth.MixtureSpec.spread)
-    spec = synth.MixtureSpec(components=W["components"], seed=20241203, spread=spread)
+    spread = a.spread if a.spread is not None else W.get("spread", SPREAD)
+    spec = synth.MixtureSpec(components=W["components"], seed=20241203, spread=spread,
+                             weights=W.get("weights", "lognormal"), topics=W.get("topics", 0),
+                             topic_spread=W.get("topic_spread", 0.0))
     mix = synth.Mixture(spec, device=dev)
     log(f"workload {a.workload}: ingest {W['rows']} rows")
     corpus = B.ingest(mix, W["rows"], batch=1 << 20)
     log("build: k-means, SOAR, RaBitQ codes, blocks")
-    arrays = B.build(corpus, W["clusters"], sample_n=max(400_000, 64 * W["clusters"]), iters=12, chunk=1 << 19)
+    arrays = B.build(corpus, W["clusters"], sample_n=max(400_000, 64 * W["clusters"]), iters=12, chunk=1 << 19,
+                     tier2=W.get("tier2", False))
     q768, q256 = mix.queries(W["queries"])
     nq = W["queries"]
     ngt = min(a.gt_queries, nq)
     log(f"exact ground truth for {ngt} queries")
     gt_ids, _ = B.exact_topk(mix, W["rows"], q768[:ngt], k=10)
     gt = gt_ids.cpu().numpy()
+    # The library allocates with hipMalloc, outside torch's caching allocator: hand back the
+    # build's freed blocks first.
+    torch.cuda.empty_cache()
+    log(f"torch holds {torch.cuda.memory_allocated() / 1e9:.1f} GB after the build")
     ix = Index.create(device_arrays(arrays), borrow_rerank=True)
     t_build = time.time() - t_build0
     sizes = np.array(arrays.sizes)
@@ -145,6 +170,11 @@ def main():
     ok = [x for x in sweep if x["recall_at_10"] >= 0.90]
     pick = max(ok, key=lambda x: x["qps"]) if ok else max(sweep, key=lambda x: x["recall_at_10"])
     _, p = parse_config(pick["config"], ix.k, k=10)
+    variant, variant_ms = ix.autotune(q768, q256, p, stream=st)
+    log(f"autotune: scan variant {variant} ({', '.join(f'{m:.2f}' for m in variant_ms)} ms)")
+    agree = probe_agreement(ix, arrays.centroids, q256[:ngt], p, st)
+    log(f"probe sets equal to the FP64 top-{p.nprobe}: {agree['equal']:.4f} of {ngt} queries "
+        f"({agree['equal_up_to_ties']:.4f} up to f32-resolution ties)")
 
     # Timed region: W warmup + K steps of the batched search, inputs already in HBM.
     for _ in range(a.warmup):
@@ -197,6 +227,18 @@ def main():
     p50, p99 = pct(lat, 0.5), pct(lat, 0.99)
     log(f"single-query latency p50 {p50:.2f} ms p99 {p99:.2f} ms ({1e3 / np.mean(lat):.0f} QPS per-query path)")
 
+    serving = []
+    if a.serve_seconds > 0:
+        from sieve_amd.server import Server, open_loop
+        q7h, q2h = q768[:2000].cpu().numpy(), q256[:2000].cpu().numpy()
+        for rate in (float(x) for x in a.serve_rates.split(",")):
+            sv = Server(ix, p, max_batch=a.serve_max_batch)
+            res = open_loop(sv, q7h, q2h, rate=rate, seconds=a.serve_seconds)
+            sv.close()
+            serving.append(res)
+            log(f"serving at {rate:.0f} QPS offered: {res['achieved_qps']:.0f} QPS, p50 {res['p50_ms']:.2f} ms, "
+                f"p99 {res['p99_ms']:.2f} ms, mean batch {res['mean_batch']}")
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(arrays, q768, q256, p, nq, a, gt)
@@ -205,13 +247,18 @@ def main():
         "metric": METRIC, "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u8",
-        "data": "synthetic: seeded anisotropic Gaussian mixture on the unit sphere (lognormal weights), "
-                "queries = mixture draws + N(0,0.05^2) noise; index built on the GPU in the reference's format",
+        "data": f"synthetic: seeded anisotropic Gaussian mixture on the unit sphere ({spec.weights} weights"
+                + (f", {spec.topics} topics inside the components" if spec.topics else "")
+                + "), queries = mixture draws + N(0,0.05^2) noise; index built on the GPU in the reference's format",
         "config": {"workload": W["desc"], "rows": W["rows"], "clusters": ix.k, "queries_per_step": nq,
-                   "search": pick["config"], "k": 10, "mixture_spread": spread},
+                   "search": pick["config"], "k": 10, "mixture_spread": spread,
+                   "topics": spec.topics, "topic_spread": spec.topic_spread},
         "recall_at_10": round(rec, 4),
         "p50_ms": round(p50, 3), "p99_ms": round(p99, 3),
         "per_query_gpu_qps": round(1e3 / float(np.mean(lat)), 1),
+        "serving_open_loop": serving,
+        "scan_variant": variant,
+        "probe_agreement_fp64": agree,
         "stage_ms": {"route": round(route_ns / 1e6 / a.steps, 3), "scan": round(scan_s * 1e3, 3),
                      "rerank": round(rerank_ns / 1e6 / a.steps, 3)},
         "probed_per_query": round(probed / a.steps / nq, 2),
@@ -224,6 +271,33 @@ def main():
         "build_s": round(t_build, 1),
     }
     print(json.dumps(out), flush=True)
+
+
+def probe_agreement(ix, centroids, q256, p, st):
+    """Probe sets of the BF16-MFMA route (with its FP32 rescoring band) against the FP64
+    top-nprobe of <q, c>, at the configuration's nprobe without margin or budget. `equal`:
+    identical sets; `equal_up_to_ties`: every differing centroid scores within the f32 error
+    bound of a 256-term sequential dot (gamma_256 * sum |q_i c_i|) of the FP64 cut score."""
+    import dataclasses
+    import torch
+    pf = dataclasses.replace(p, margin=None, budget=None)
+    _, c, n = ix.route(q256, pf, stream=st)
+    torch.cuda.synchronize()
+    q64, c64 = q256.double(), centroids.double()
+    s64 = q64 @ c64.T
+    bound = (q64.abs() @ c64.abs().T) * (256 * 2.0 ** -24 / (1 - 256 * 2.0 ** -24))
+    top = torch.topk(s64, pf.nprobe, dim=1)
+    cut = top.values[:, -1:]
+    got = torch.zeros_like(s64, dtype=torch.bool)
+    got.scatter_(1, c.long(), True)
+    want = torch.zeros_like(got)
+    want.scatter_(1, top.indices, True)
+    diff = got ^ want
+    near = (s64 - cut).abs() <= bound + bound.gather(1, top.indices[:, -1:])
+    equal = (~diff).all(1) & (n.long() == pf.nprobe)
+    ties = (~diff | near).all(1) & (n.long() == pf.nprobe)
+    return {"queries": int(q256.shape[0]), "nprobe": pf.nprobe, "equal": round(float(equal.double().mean()), 5),
+            "equal_up_to_ties": round(float(ties.double().mean()), 5)}
 
 
 def cpu_baseline(arrays, q768, q256, p, nq, a, gt):

@@ -211,6 +211,12 @@ int sieve_index_autotune(sieve_index* ix, sieve_stream* s, const float* q768, co
                          const sieve_params* p, uint32_t* chosen, float* ms_per_variant);
 /* Fixes the scan variant (0 .. SIEVE_SCAN_VARIANTS - 1) instead of tuning it. */
 int sieve_index_set_scan_variant(sieve_index* ix, uint32_t variant);
+/* Work split of the scan: a query's probed blocks are cut into chunks of about chunk_blocks
+ * 32-slot blocks, each scanned by its own workgroup and merged on the device; 0 (the default)
+ * sizes chunks per batch (whole queries in large batches, many chunks for a lone query).
+ * Results do not depend on the split. No counterpart in the reference (one thread scans a
+ * whole query there). */
+int sieve_index_set_scan_chunk(sieve_index* ix, uint32_t chunk_blocks);
 
 /* ------------------------------------------------------------- the path --- */
 
@@ -334,6 +340,8 @@ typedef struct sieve_l2_index sieve_l2_index;
 
 int sieve_l2_index_create(const sieve_l2_desc* desc, sieve_l2_index** out);
 void sieve_l2_index_free(sieve_l2_index* ix);
+/* As sieve_index_set_scan_chunk, for the L2 index. */
+int sieve_l2_index_set_scan_chunk(sieve_l2_index* ix, uint32_t chunk_blocks);
 
 /* bigann::Index::search (src/bigann.rs:652-657) for nq queries (nq x 100 i8): the top k
  * (score = |q|^2 - |q - x|^2 - |q|^2 = 2 <q, x> - |x|^2, row) best first. Optional outputs

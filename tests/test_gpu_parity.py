@@ -266,6 +266,29 @@ def test_batch_composition_does_not_change_results(small, gpu_index):
         assert np.array_equal(r1[0], r[i]) and np.array_equal(s1[0], s[i]) and n1[0] == n[i]
 
 
+@pytest.mark.parametrize("chunk", [1, 3, 40])
+def test_scan_split_does_not_change_candidates(small, gpu_index, chunk):
+    """Cutting each query's blocks into chunks scanned by separate workgroups and merged on the
+    device gives the same candidates, bit for bit, as one workgroup per query."""
+    p = Params(nprobe=12, cand=300, k=10, rerank=Rerank.FULL)
+    ps, pc, pn = gpu_index.route(small.q256, p)
+    try:
+        gpu_index.set_scan_chunk(1 << 30)
+        want = gpu_index.candidates(small.q256, ps, pc, pn, p)
+        gpu_index.set_scan_chunk(chunk)
+        got = gpu_index.candidates(small.q256, ps, pc, pn, p)
+        got1 = gpu_index.candidates(small.q256[5:6], ps[5:6], pc[5:6], pn[5:6], p)
+    finally:
+        gpu_index.set_scan_chunk(0)
+    n = want[3]
+    assert np.array_equal(got[3], n) and got1[3][0] == n[5]
+    for i in range(small.nq):
+        for w, g in zip(want[:3], got[:3]):
+            assert np.array_equal(w[i, :n[i]], g[i, :n[i]])
+    for w, g in zip(want[:3], got1[:3]):
+        assert np.array_equal(w[5, :n[5]], g[0, :n[5]])
+
+
 def test_errors_are_reported(small, gpu_index):
     from sieve_amd.abi import SieveError
     with pytest.raises(SieveError, match="cand"):

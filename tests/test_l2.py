@@ -78,11 +78,13 @@ def test_oracle_l2_matches_brute_force(l2small, l2oracle):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("nprobe,cand,margin", [(8, 300, 0.0), (16, 1000, 0.0), (32, 500, 0.3), (1, 50, 0.0)])
-def test_gpu_l2_matches_oracle(l2small, l2oracle, nprobe, cand, margin):
+@pytest.mark.parametrize("nprobe,cand,margin,chunk", [(8, 300, 0.0, 0), (16, 1000, 0.0, 0), (32, 500, 0.3, 0),
+                                                      (1, 50, 0.0, 0), (16, 300, 0.0, 2)])
+def test_gpu_l2_matches_oracle(l2small, l2oracle, nprobe, cand, margin, chunk):
     from sieve_amd.index import L2Index
     h, qs = l2small
     ix = L2Index(h)
+    ix.set_scan_chunk(chunk)
     s, r, n, probes, pn, cr, ce, cn = ix.search(qs, nprobe, cand, 10, margin=margin, details=True)
     for i, q in enumerate(qs):
         wp, wc, we, ws, wr = l2oracle.search(q, nprobe, cand, 10, margin=margin)
