@@ -52,12 +52,13 @@ T* grow(DevBuf& b, size_t n) {
     return static_cast<T*>(b.p);
 }
 
-// grow() whose new allocations start zeroed (counters that kernels leave at zero).
+// grow() whose new allocations start zeroed (counters that kernels leave at zero). A new
+// allocation is told by its capacity: hipMalloc may hand back the address just freed.
 template <class T>
 T* grow_zeroed(DevBuf& b, size_t n, hipStream_t st) {
-    void* before = b.p;
+    const size_t cap_before = b.cap;
     T* p = grow<T>(b, n);
-    if (b.p != before) {
+    if (b.cap != cap_before) {
         hipError_t e = hipMemsetAsync(b.p, 0, b.cap, st);
         if (e != hipSuccess) throw Error{SIEVE_EHIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e)};
     }

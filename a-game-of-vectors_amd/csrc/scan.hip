@@ -333,7 +333,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_t(ScanArgs a) {
         for (uint32_t c = 0; c < nch; c++) {
             if (c == ci) continue;
             const uint2* lst = a.part + (slot0 + c) * (size_t)cand;
-            const int m = (int)__hip_atomic_load(a.part_n + slot0 + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int m = (int)min((uint32_t)cand, __hip_atomic_load(a.part_n + slot0 + c, __ATOMIC_RELAXED,
+                                                                     __HIP_MEMORY_SCOPE_AGENT));
             for (int i0 = 0; i0 < m; i0 += SCAN_THREADS * SLOTS_PER_THREAD) {
                 const bool has_thr = misc[M_HAS] != 0;
                 const uint32_t thr = misc[M_THR], thr_lo = misc[M_THR_LO];

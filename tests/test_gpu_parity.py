@@ -289,6 +289,26 @@ def test_scan_split_does_not_change_candidates(small, gpu_index, chunk):
         assert np.array_equal(w[5, :n[5]], g[0, :n[5]])
 
 
+def test_scan_split_with_growing_batches(small, gpu_index):
+    """Split scans on one stream while the batch grows (its per-query counters are reallocated
+    each time): every batch still gives the unsplit candidates."""
+    p = Params(nprobe=12, cand=300, k=10, rerank=Rerank.FULL)
+    ps, pc, pn = gpu_index.route(small.q256, p)
+    try:
+        gpu_index.set_scan_chunk(1 << 30)
+        want = gpu_index.candidates(small.q256, ps, pc, pn, p)
+        gpu_index.set_scan_chunk(3)
+        for m in (1, 3, 9, 27, small.nq):
+            got = gpu_index.candidates(small.q256[:m], ps[:m], pc[:m], pn[:m], p)
+            assert np.array_equal(got[3], want[3][:m])
+            for i in range(m):
+                n = want[3][i]
+                for w, g in zip(want[:3], got[:3]):
+                    assert np.array_equal(w[i, :n], g[i, :n])
+    finally:
+        gpu_index.set_scan_chunk(0)
+
+
 def test_errors_are_reported(small, gpu_index):
     from sieve_amd.abi import SieveError
     with pytest.raises(SieveError, match="cand"):
