@@ -53,6 +53,10 @@ WORKLOADS = {
 # 1-bit estimate 70-78%), so that workload adds 131072 topics (spread 0.08) inside the
 # components, rows at spread 0.05 around them: R@10 0.92 at 64 probes, 1000 candidates.
 SPREAD = 0.1
+# HBM bytes per scan launch measured with rocprofv3 --pmc for the profiled (workload, search
+# configuration, scan variant): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB x 1024, the factors read
+# off a 1 GiB elementwise copy (bench.py --copy-probe). Other configurations report null.
+PMC_TRAFFIC = {("c3", "64:1000:full:b10x0.8:p0.1", 1): 306.46e9}
 
 
 def log(msg):
@@ -101,8 +105,8 @@ def main():
     ap.add_argument("--serve-rates", default="2000,20000", help="offered Poisson arrival rates, queries/s")
     ap.add_argument("--serve-max-batch", type=int, default=512)
     ap.add_argument("--copy-probe", action="store_true",
-                    help="after the timed steps, copy one 1 GiB device buffer (a plain copy of known size, "
-                         "the reference that scales the FETCH_SIZE/WRITE_SIZE counters of a profile)")
+                    help="only run an elementwise kernel that reads 1 GiB and writes 1 GiB (the known size that "
+                         "scales a profile's FETCH_SIZE/WRITE_SIZE counters), then exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (index sharding is a design "
@@ -115,6 +119,9 @@ def main():
     from sieve_amd.index import Index
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if a.copy_probe:
+        copy_probe()
+        return
     W = WORKLOADS[a.workload]
     dev = torch.device("cuda:0")
     t_build0 = time.time()
@@ -206,15 +213,6 @@ def main():
         f"scan {scan_s * 1e3:.2f} ms = {achieved:.0f} GB/s; route {route_ns / 1e6 / a.steps:.2f} ms; "
         f"rerank {rerank_ns / 1e6 / a.steps:.2f} ms")
 
-    if a.copy_probe:
-        src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log("copy probe: 1073741824 bytes read, 1073741824 bytes written")
-        del src, dst
-
     # Single-query latency: the plain per-query GPU path (batch of one, synchronous).
     lat = []
     for i in range(min(a.latency_queries, nq)):
@@ -224,8 +222,11 @@ def main():
         lat.append((time.perf_counter() - t1) * 1e3)
     lat.sort()
     from sieve_amd.evaluate import pct
-    p50, p99 = pct(lat, 0.5), pct(lat, 0.99)
-    log(f"single-query latency p50 {p50:.2f} ms p99 {p99:.2f} ms ({1e3 / np.mean(lat):.0f} QPS per-query path)")
+    p50 = p99 = per_query_qps = None
+    if lat:
+        p50, p99 = round(pct(lat, 0.5), 3), round(pct(lat, 0.99), 3)
+        per_query_qps = round(1e3 / float(np.mean(lat)), 1)
+        log(f"single-query latency p50 {p50:.2f} ms p99 {p99:.2f} ms ({per_query_qps:.0f} QPS per-query path)")
 
     serving = []
     if a.serve_seconds > 0:
@@ -254,8 +255,8 @@ def main():
                    "search": pick["config"], "k": 10, "mixture_spread": spread,
                    "topics": spec.topics, "topic_spread": spec.topic_spread},
         "recall_at_10": round(rec, 4),
-        "p50_ms": round(p50, 3), "p99_ms": round(p99, 3),
-        "per_query_gpu_qps": round(1e3 / float(np.mean(lat)), 1),
+        "p50_ms": p50, "p99_ms": p99,
+        "per_query_gpu_qps": per_query_qps,
         "serving_open_loop": serving,
         "scan_variant": variant,
         "probe_agreement_fp64": agree,
@@ -264,13 +265,27 @@ def main():
         "probed_per_query": round(probed / a.steps / nq, 2),
         "scanned_per_query": round(scanned / a.steps / nq, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_scan",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": PMC_TRAFFIC.get((a.workload, pick["config"], variant)), "kernel": "k_scan_t",
                      "algorithmic_bytes_per_launch": int(scan_bytes)},
         "cpu_baseline": cpu,
         "sweep": sweep,
         "build_s": round(t_build, 1),
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_probe():
+    """One elementwise pass over a 1 GiB float buffer: reads 2^30 bytes, writes 2^30 bytes."""
+    import torch
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe": {"kernel": "elementwise neg", "launches": 3, "bytes_read": 1 << 30,
+                                     "bytes_written": 1 << 30}}), flush=True)
 
 
 def probe_agreement(ix, centroids, q256, p, st):
