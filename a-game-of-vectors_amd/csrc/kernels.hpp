@@ -112,7 +112,7 @@ ScanPlan plan_scan(size_t nq, uint32_t nprobe, double mean_blocks, uint32_t max_
 // Upper bound of the scan's LDS over all variants.
 size_t scan_lds_bytes(int cand, int probe_stride);
 // Scan kernel variants (slots per thread, prefetch), picked per index by the autotuner.
-constexpr int SCAN_VARIANTS = 4;
+constexpr int SCAN_VARIANTS = 10;
 const char* scan_variant_name(int variant);
 void launch_scan(const ScanArgs& a, size_t nq, int variant, hipStream_t st);
 

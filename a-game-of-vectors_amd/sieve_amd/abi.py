@@ -98,6 +98,9 @@ class ServerStats(ctypes.Structure):
 _lib = None
 
 
+# Scan kernel variants the autotuner times (SIEVE_SCAN_VARIANTS).
+SCAN_VARIANTS = 10
+
 def header_symbols():
     """Every function the public header declares."""
     text = open(HEADER).read()

@@ -177,7 +177,7 @@ class Index:
         (chosen variant, ms per variant)."""
         q768, q256 = self._in(q768, np.float32), self._in(q256, np.float32)
         ch = ctypes.c_uint32()
-        ms = np.zeros(4, np.float32)
+        ms = np.zeros(abi.SCAN_VARIANTS, np.float32)
         st = stream or self.stream()
         abi.check(abi.lib().sieve_index_autotune(self._h, st._h, abi.ptr(q768), abi.ptr(q256), q256.shape[0],
                                                  ctypes.byref(abi.make_params(p)), ctypes.byref(ch), abi.ptr(ms)))

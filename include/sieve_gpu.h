@@ -206,7 +206,7 @@ int sieve_index_get_info(const sieve_index* ix, sieve_index_info* out);
  * queries with these parameters, and keeps the fastest for later calls on this index. Call it
  * before sharing the index across threads. chosen (may be NULL) gets the variant, and
  * ms_per_variant (NULL or SIEVE_SCAN_VARIANTS floats) the time of each. */
-#define SIEVE_SCAN_VARIANTS 4
+#define SIEVE_SCAN_VARIANTS 10
 int sieve_index_autotune(sieve_index* ix, sieve_stream* s, const float* q768, const float* q256, size_t nq,
                          const sieve_params* p, uint32_t* chosen, float* ms_per_variant);
 /* Fixes the scan variant (0 .. SIEVE_SCAN_VARIANTS - 1) instead of tuning it. */

@@ -201,7 +201,7 @@ def test_search_matches_oracle(small, oracle_index, gpu_index, cfg):
     assert tm.probed == wt["probed"] and tm.scanned == wt["scanned"]
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", list(range(10)))
 def test_scan_variants_match_oracle(small, oracle_index, gpu_index, variant):
     from sieve_amd import abi
     p = Params(nprobe=16, cand=700, k=10, rerank=Rerank.FULL)
@@ -220,7 +220,9 @@ def test_scan_variants_match_oracle(small, oracle_index, gpu_index, variant):
 def test_autotune_picks_a_variant(small, gpu_index):
     p = Params(nprobe=16, cand=700, k=10)
     v, ms = gpu_index.autotune(small.q768, small.q256, p)
-    assert 0 <= v < 4 and len(ms) == 4 and all(x > 0 for x in ms) and ms[v] == min(ms)
+    from sieve_amd import abi
+    n = abi.SCAN_VARIANTS
+    assert 0 <= v < n and len(ms) == n and all(x > 0 for x in ms) and ms[v] == min(ms)
     gpu_index.set_scan_variant(0)
 
 
