@@ -471,9 +471,9 @@ static void launch_any(const ScanArgs& a, size_t nq, int variant, hipStream_t st
         case 3: launch_variant<G, L2, 8, false, false, 0>(a, nq, st); break;
         case 4: launch_variant<G, L2, 2, true, false, 5>(a, nq, st); break;
         case 5: launch_variant<G, L2, 2, false, false, 0>(a, nq, st); break;
-        case 6: launch_variant<G, L2, 2, false, false, 6>(a, nq, st); break;
-        case 7: launch_variant<G, L2, 1, true, false, 0>(a, nq, st); break;
-        case 8: launch_variant<G, L2, 1, false, false, 0>(a, nq, st); break;
+        case 6: launch_variant<G, L2, 1, true, false, 0>(a, nq, st); break;
+        case 7: launch_variant<G, L2, 1, false, false, 0>(a, nq, st); break;
+        case 8: launch_variant<G, L2, 2, false, false, 6>(a, nq, st); break;
         default: launch_variant<G, L2, 2, true, true, 0>(a, nq, st); break;
     }
 }
@@ -502,9 +502,8 @@ ScanPlan plan_scan(size_t nq, uint32_t nprobe, double mean_blocks, uint32_t max_
 const char* scan_variant_name(int v) {
     static const char* names[SCAN_VARIANTS] = {
         "4 slots/thread", "2 slots/thread + prefetch", "4 slots/thread + prefetch", "8 slots/thread",
-        "2 slots/thread + prefetch, interleaved", "2 slots/thread + prefetch, interleaved, >= 5 waves/SIMD",
-        "4 slots/thread, interleaved, >= 5 waves/SIMD", "4 slots/thread + prefetch, interleaved, >= 4 waves/SIMD",
-        "2 slots/thread, interleaved, >= 6 waves/SIMD", "1 slot/thread + prefetch, >= 8 waves/SIMD"};
+        "2 slots/thread + prefetch, >= 5 waves/SIMD", "2 slots/thread", "1 slot/thread + prefetch", "1 slot/thread",
+        "2 slots/thread, >= 6 waves/SIMD", "2 slots/thread + prefetch, interleaved lookups"};
     return v >= 0 && v < SCAN_VARIANTS ? names[v] : "?";
 }
 
