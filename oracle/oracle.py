@@ -37,6 +37,11 @@ Parity status of each oracle function and what pins it (tests/test_oracle_*.py):
   rerank          (ivf.rs:731-815)   pinned: brute-force int64 dots + full sort per mode
   l2 search       (bigann.rs:652-747) pinned: brute-force L2 estimate/rerank by full sort
   assign_top/soar (ivf.rs:137-186)   pinned: float64 brute force (up to near-ties)
+  Live            (live.rs)          pinned: live.rs:493-520 (delta list layout, bitset);
+                                      an inserted corpus vector encodes to the build's slots
+                                      of that row (live.rs:2-6, bit for bit); upsert/delete
+                                      semantics; live search == search over a rebuilt index
+                                      holding the same rows (brute force, up to ties)
 """
 import ctypes
 import os
@@ -137,6 +142,18 @@ def lib():
             "or_l2_create": (vp, [u32, vp, vp, vp, vp, vp, vp, vp, vp]),
             "or_l2_free": (None, [vp]),
             "or_l2_search": (sz, [vp, vp, u32, u32, u32, f32, vp, vp, vp, vp, vp, vp, vp]),
+            "or_live_create": (vp, [vp, i32, f32]),
+            "or_delta_list_fixture": (sz, [sz, vp, vp, vp, sz, vp, vp, vp]),
+            "or_live_free": (None, [vp]),
+            "or_live_insert": (None, [vp, vp, vp, sz]),
+            "or_live_delete": (None, [vp, vp, sz]),
+            "or_live_current": (u32, [vp, u32]),
+            "or_live_rows": (sz, [vp]),
+            "or_live_row": (None, [vp, sz, vp, vp, vp, vp, vp]),
+            "or_live_list": (sz, [vp, u32, sz, vp, vp, vp]),
+            "or_live_deleted": (sz, [vp, vp, sz]),
+            "or_live_search_batch": (None, [vp, vp, vp, sz, ctypes.POINTER(_Params), ctypes.POINTER(_View), i32,
+                                            vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -506,3 +523,81 @@ class L2Index:
         n = lib().or_l2_search(self._h, _p(q), nprobe, cand, k, margin, _p(probe), _p(npo), _p(ci), _p(ce), _p(nc),
                                _p(os_), _p(orr))
         return (probe[:npo[0]], ci[:nc[0]], ce[:nc[0]], os_[:n], orr[:n])
+
+
+def delta_list_fixture(codes, factors, ids):
+    """DeltaList::push of each (code, factors, id) into a fresh list: (blocks, factors, ids)."""
+    codes, factors, ids = _c(codes, np.uint8), _c(factors, np.float32), _c(ids, np.uint32)
+    n = codes.shape[0]
+    nb = lib().or_delta_list_fixture(n, _p(codes), _p(factors), _p(ids), 0, None, None, None)
+    b = np.empty((nb, 1024), np.uint8)
+    f = np.empty((nb * BLOCK, 2), np.float32)
+    i = np.empty(nb * BLOCK, np.uint32)
+    lib().or_delta_list_fixture(n, _p(codes), _p(factors), _p(ids), nb, _p(b), _p(f), _p(i))
+    return b, f, i
+
+
+class Live:
+    """Live over an oracle Index (src/live.rs): inserts into per-cluster delta lists, deletes as
+    tombstones, external ids."""
+
+    def __init__(self, index, soar_lambda=1.0):
+        self._ix = index  # keeps the index alive
+        soar = soar_lambda is not None
+        self._h = lib().or_live_create(index._h, 1 if soar else 0, float(soar_lambda) if soar else 0.0)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().or_live_free(self._h)
+        except Exception:
+            pass
+        self._h = None
+
+    def insert(self, ext, vecs):
+        ext, vecs = _c(ext, np.uint32), _c(vecs, np.float32)
+        lib().or_live_insert(self._h, _p(ext), _p(vecs), ext.shape[0])
+
+    def delete(self, ext):
+        ext = _c(ext, np.uint32)
+        lib().or_live_delete(self._h, _p(ext), ext.shape[0])
+
+    def current(self, ext):
+        r = lib().or_live_current(self._h, int(ext))
+        return None if r == PAD else r
+
+    def rows(self):
+        return lib().or_live_rows(self._h)
+
+    def row(self, j):
+        row = np.empty(DIM, np.int8)
+        t2 = np.zeros(64, np.uint8)
+        norm, t2n, ext = ctypes.c_float(), ctypes.c_float(0.0), ctypes.c_uint32()
+        lib().or_live_row(self._h, j, _p(row), ctypes.byref(norm), _p(t2), ctypes.byref(t2n), ctypes.byref(ext))
+        return row, norm.value, t2, t2n.value, ext.value
+
+    def delta_list(self, c):
+        """(raw codes n x 32, factors n x 2, row ids n) of cluster c's delta list."""
+        n = lib().or_live_list(self._h, c, 0, None, None, None)
+        codes = np.empty((n, 32), np.uint8)
+        f = np.empty((n, 2), np.float32)
+        ids = np.empty(n, np.uint32)
+        lib().or_live_list(self._h, c, n, _p(codes), _p(f), _p(ids))
+        return codes, f, ids
+
+    def deleted(self):
+        n = lib().or_live_deleted(self._h, None, 0)
+        w = np.zeros(n, np.uint64)
+        lib().or_live_deleted(self._h, _p(w), n)
+        return w
+
+    def search_batch(self, q768, q256, p, view=None, threads=1):
+        q7, q2 = _c(q768, np.float32), _c(q256, np.float32)
+        nq = q7.shape[0]
+        s = np.zeros((nq, p.k), np.float32)
+        r = np.full((nq, p.k), PAD, np.uint32)
+        cnt = np.zeros(nq, np.uint32)
+        v = view if view is not None else _View()
+        lib().or_live_search_batch(self._h, _p(q7), _p(q2), nq, ctypes.byref(make_params(p)), ctypes.byref(v),
+                                   threads, _p(s), _p(r), _p(cnt))
+        return s, r, cnt

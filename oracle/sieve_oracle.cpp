@@ -19,6 +19,7 @@
 #include <cstring>
 #include <limits>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../include/sieve_gpu.h"
@@ -387,11 +388,14 @@ float tier2_score(const float* tables, const uint8_t* code, float norm) {  // sr
 }
 
 // ---------------------------------------------------------------- ivf -----------------
-struct View {  // src/ivf.rs:462-496 (tombstones and allowlist; delta lists not modelled)
+struct Delta;
+
+struct View {  // src/ivf.rs:462-496: tombstones, allowlist, live delta segment
     const uint64_t* deleted = nullptr;
     uint64_t deleted_words = 0;
     const uint64_t* allow = nullptr;
     uint64_t allow_words = 0;
+    const Delta* delta = nullptr;
     static bool bit(const uint64_t* set, uint64_t words, uint32_t id) {  // src/ivf.rs:474-477
         uint64_t w = id >> 6;
         return w < words && ((set[w] >> (id & 63)) & 1) == 1;
@@ -419,6 +423,73 @@ struct Index {
 
     size_t slots(uint32_t c) const { return (size_t)(block_off[c + 1] - block_off[c]) * BLOCK; }  // src/ivf.rs:615-618
 };
+
+// One cluster's delta list in the base block layout (src/live.rs:37-66).
+struct DeltaList {
+    std::vector<uint8_t> blocks;
+    std::vector<float> factors;
+    std::vector<uint32_t> ids;
+    std::vector<uint8_t> tail;  // raw codes of the last block, 32 B each
+    size_t len = 0;
+    void push(const uint8_t* code, const float* f, uint32_t id) {  // DeltaList::push, src/live.rs:49-65
+        size_t k = len % BLOCK;
+        if (k == 0) {
+            blocks.resize(blocks.size() + 1024, 0);
+            factors.resize(factors.size() + BLOCK * 2, 0.0f);
+            ids.resize(ids.size() + BLOCK, PAD);
+            tail.clear();
+        }
+        tail.insert(tail.end(), code, code + 32);
+        pack_block(tail.data(), (int)(tail.size() / 32), PDIM / 4, blocks.data() + blocks.size() - 1024);
+        size_t slot = ids.size() - BLOCK + k;
+        ids[slot] = id;
+        factors[slot * 2] = f[0];
+        factors[slot * 2 + 1] = f[1];
+        len++;
+    }
+};
+
+// Rows added since the build (src/live.rs:69-121): delta lists, rows, prefix norms, tier-2
+// codes and external ids; row ids count up from `base` (the corpus size).
+struct Delta {
+    uint64_t base = 0;
+    std::vector<DeltaList> lists;
+    std::vector<int8_t> rows;
+    std::vector<float> norm256;
+    std::vector<uint8_t> t2;
+    std::vector<float> t2_norm;
+    std::vector<uint32_t> ext;
+    bool local(uint32_t id, size_t* i) const {
+        if (id < base || id - base >= ext.size()) return false;
+        *i = id - base;
+        return true;
+    }
+    const DeltaList* list(uint32_t c) const {
+        return c < lists.size() && lists[c].len > 0 ? &lists[c] : nullptr;
+    }
+};
+
+// View::row and the prefix-norm / tier-2 lookups (src/ivf.rs:485-495, src/live.rs:85-101).
+inline const int8_t* view_row(const Index& ix, const View& v, uint32_t id) {
+    size_t i;
+    if (v.delta && v.delta->local(id, &i)) return v.delta->rows.data() + i * DIM;
+    return ix.rerank + (size_t)id * DIM;
+}
+inline float view_norm256(const Index& ix, const View& v, uint32_t id) {
+    size_t i;
+    if (v.delta && v.delta->local(id, &i)) return v.delta->norm256[i];
+    return ix.norm256[id];
+}
+inline void view_tier2(const Index& ix, const View& v, uint32_t id, const uint8_t** code, float* norm) {
+    size_t i;
+    if (v.delta && !v.delta->t2_norm.empty() && v.delta->local(id, &i)) {
+        *code = v.delta->t2.data() + i * 64;
+        *norm = v.delta->t2_norm[i];
+        return;
+    }
+    *code = ix.t2_codes + (size_t)id * 64;
+    *norm = ix.t2_norm[id];
+}
 
 struct Probe {
     float score;
@@ -471,20 +542,24 @@ std::vector<Cand> candidates(const Index& ix, const float* q256, const std::vect
         query_new(ix.rotation, PDIM, q256, rq);
         TopK<float> top(p.cand);
         std::vector<std::pair<uint32_t, float>> meta;
-        for (const Probe& pr : probe) {
-            uint64_t b0 = ix.block_off[pr.cluster], b1 = ix.block_off[pr.cluster + 1];
-            size_t ns = (size_t)(b1 - b0) * BLOCK;
+        auto scan_list = [&](float cent, const uint8_t* blocks, size_t nb, const float* f, const uint32_t* ids) {
+            size_t ns = nb * BLOCK;
             scores.resize(ns);
-            scan(rq.tables.data(), PDIM / 4, ix.blocks + b0 * 1024, b1 - b0, scores.data());
+            scan(rq.tables.data(), PDIM / 4, blocks, nb, scores.data());
             *scanned += ns;
-            const float* f = ix.factors + b0 * BLOCK * 2;
-            const uint32_t* ids = ix.ids + b0 * BLOCK;
             for (size_t j = 0; j < ns; j++) {
-                float est = estimate(rq, pr.score, f[2 * j], scores[j]);
+                float est = estimate(rq, cent, f[2 * j], scores[j]);
                 if (top.rejects(est) || ids[j] == PAD || v.hidden(ids[j])) continue;
                 top.push(est, (uint32_t)meta.size());
                 meta.emplace_back(ids[j], f[2 * j + 1]);
             }
+        };
+        for (const Probe& pr : probe) {
+            uint64_t b0 = ix.block_off[pr.cluster], b1 = ix.block_off[pr.cluster + 1];
+            scan_list(pr.score, ix.blocks + b0 * 1024, b1 - b0, ix.factors + b0 * BLOCK * 2, ix.ids + b0 * BLOCK);
+            // The cluster's delta list right after its base list (src/ivf.rs:682-684).
+            if (const DeltaList* d = v.delta ? v.delta->list(pr.cluster) : nullptr)
+                scan_list(pr.score, d->blocks.data(), d->blocks.size() / 1024, d->factors.data(), d->ids.data());
         }
         for (auto& x : top.into_sorted()) cand.push_back({x.first, meta[x.second].first, meta[x.second].second});
     } else {
@@ -543,14 +618,19 @@ RerankQuery rerank_query(const Index& ix, const float* q, int n) {
 
 // read_and_rerank, src/ivf.rs:818-832 (all rows are resident, so base and delta rows are
 // one store; the pushes keep its order: base rows first).
-void read_and_rerank(const Index& ix, const float* q768, const std::vector<uint32_t>& ids, TopK<float>& out) {
+void read_and_rerank(const Index& ix, const View& v, const float* q768, const std::vector<uint32_t>& ids,
+                     TopK<float>& out) {
     RerankQuery r = rerank_query(ix, q768, DIM);
-    for (uint32_t id : ids) out.push((float)dot_i8i8(r.wq.data(), ix.rerank + (size_t)id * DIM, DIM) * r.inv, id);
+    for (int pass = 0; pass < 2; pass++)  // corpus rows first, then delta rows
+        for (uint32_t id : ids)
+            if ((id < ix.n) == (pass == 0))
+                out.push((float)dot_i8i8(r.wq.data(), view_row(ix, v, id), DIM) * r.inv, id);
 }
 
 // Index::rerank, src/ivf.rs:731-815.
 std::vector<std::pair<float, uint32_t>> rerank(const Index& ix, const float* q768, const float* q256,
-                                               std::vector<Cand> cand, const sieve_params& p, uint64_t* reranked) {
+                                               std::vector<Cand> cand, const sieve_params& p, const View& v,
+                                               uint64_t* reranked) {
     if (p.prune_m > 0) {
         float tau = prune_threshold(cand, p.prune_m, p.prune_w);
         std::vector<Cand> kept;
@@ -567,7 +647,7 @@ std::vector<std::pair<float, uint32_t>> rerank(const Index& ix, const float* q76
         case SIEVE_RERANK_FULL: {
             RerankQuery r = rerank_query(ix, q768, DIM);
             for (const Cand& c : cand)
-                out.push((float)dot_i8i8(r.wq.data(), ix.rerank + (size_t)c.row * DIM, DIM) * r.inv, c.row);
+                out.push((float)dot_i8i8(r.wq.data(), view_row(ix, v, c.row), DIM) * r.inv, c.row);
             break;
         }
         case SIEVE_RERANK_SSD: {
@@ -575,7 +655,7 @@ std::vector<std::pair<float, uint32_t>> rerank(const Index& ix, const float* q76
             for (const Cand& c : cand) best.push(c.est, c.row);
             std::vector<uint32_t> ids;
             for (auto& x : best.into_sorted()) ids.push_back(x.second);
-            read_and_rerank(ix, q768, ids, out);
+            read_and_rerank(ix, v, q768, ids, out);
             break;
         }
         default: {
@@ -583,26 +663,30 @@ std::vector<std::pair<float, uint32_t>> rerank(const Index& ix, const float* q76
                 std::vector<float> tables(64 * 256);
                 tier2_tables(ix.t2_sigma, q256, tables.data());
                 TopK<float> pre(p.rerank_r);
-                for (const Cand& c : cand)
-                    pre.push(tier2_score(tables.data(), ix.t2_codes + (size_t)c.row * 64, ix.t2_norm[c.row]), c.row);
+                for (const Cand& c : cand) {
+                    const uint8_t* code;
+                    float norm;
+                    view_tier2(ix, v, c.row, &code, &norm);
+                    pre.push(tier2_score(tables.data(), code, norm), c.row);
+                }
                 std::vector<uint32_t> ids;
                 for (auto& x : pre.into_sorted()) ids.push_back(x.second);
-                read_and_rerank(ix, q768, ids, out);
+                read_and_rerank(ix, v, q768, ids, out);
                 break;
             }
             // Prefix, or Tiered without a 2-bit tier.
             RerankQuery r = rerank_query(ix, q256, PDIM);
             TopK<float> pre(p.rerank == SIEVE_RERANK_TIERED ? p.rerank_r : p.k);
             for (const Cand& c : cand) {
-                float norm = ix.norm256[c.row];
-                pre.push((float)dot_i8i8(r.wq.data(), ix.rerank + (size_t)c.row * DIM, PDIM) * r.inv / norm, c.row);
+                float norm = view_norm256(ix, v, c.row);
+                pre.push((float)dot_i8i8(r.wq.data(), view_row(ix, v, c.row), PDIM) * r.inv / norm, c.row);
             }
             if (p.rerank == SIEVE_RERANK_PREFIX) {
                 out = pre;
             } else {
                 RerankQuery f = rerank_query(ix, q768, DIM);
                 for (auto& x : pre.into_sorted())
-                    out.push((float)dot_i8i8(f.wq.data(), ix.rerank + (size_t)x.second * DIM, DIM) * f.inv, x.second);
+                    out.push((float)dot_i8i8(f.wq.data(), view_row(ix, v, x.second), DIM) * f.inv, x.second);
             }
         }
     }
@@ -787,7 +871,7 @@ size_t or_rerank(const void* h, const float* q768, const float* q256, const floa
     std::vector<Cand> c(nc);
     for (size_t i = 0; i < nc; i++) c[i] = {est[i], rows[i], errs[i]};
     uint64_t rr = 0;
-    auto res = rerank(ix, q768, q256, c, *p, &rr);
+    auto res = rerank(ix, q768, q256, c, *p, View{}, &rr);
     if (reranked) *reranked += rr;
     for (size_t i = 0; i < res.size(); i++) {
         out_s[i] = res[i].first;
@@ -803,7 +887,7 @@ static size_t search_one(const Index& ix, const float* q768, const float* q256, 
     auto probe = route(ix, q256, p, cent);
     uint64_t scanned = 0, reranked = 0;
     auto c = candidates(ix, q256, probe, p, v, &scanned);
-    auto res = rerank(ix, q768, q256, c, p, &reranked);
+    auto res = rerank(ix, q768, q256, c, p, v, &reranked);
     if (t) {
         t->probed += probe.size();
         t->scanned += scanned;
@@ -1023,4 +1107,232 @@ size_t or_l2_search(const void* h, const int8_t* q, uint32_t nprobe, uint32_t ca
     return res.size();
 }
 
+}  // extern "C"
+
+// ---- live updates, src/live.rs ------------------------------------------------------
+// Live: the immutable index plus a Delta (inserted rows in per-cluster delta lists), a
+// tombstone bitset over row ids and the external-id map. One writer at a time.
+struct OLive {
+    const Index* ix;
+    bool soar;
+    float lambda;
+    std::vector<float> cents_rot;  // k x 256, Rotation::apply of each centroid (src/live.rs:183-192)
+    Delta delta;
+    std::vector<uint64_t> deleted;
+    std::unordered_map<uint32_t, uint32_t> by_ext;
+};
+
+static bool live_bit(const std::vector<uint64_t>& set, uint32_t id) {  // src/live.rs:123-125
+    size_t w = id >> 6;
+    return w < set.size() && ((set[w] >> (id & 63)) & 1) == 1;
+}
+static void live_set_bit(std::vector<uint64_t>& set, uint32_t id) {  // src/live.rs:127-133
+    size_t w = id >> 6;
+    if (set.size() <= w) set.resize(w + 1, 0);
+    set[w] |= 1ull << (id & 63);
+}
+// Live::current, src/live.rs:271-277: the row an external id resolves to.
+static bool live_current(const OLive& l, uint32_t e, uint32_t* id) {
+    auto it = l.by_ext.find(e);
+    if (it != l.by_ext.end()) {
+        *id = it->second;
+        return true;
+    }
+    if ((uint64_t)e < l.ix->n && !live_bit(l.deleted, e)) {
+        *id = e;
+        return true;
+    }
+    return false;
+}
+static uint32_t live_ext_of(const OLive& l, uint32_t id) {  // src/live.rs:279-284
+    size_t i;
+    return l.delta.local(id, &i) ? l.delta.ext[i] : id;
+}
+
+// Live::apply_insert, src/live.rs:331-379: encode as ivf::build encodes a row, then append.
+static void live_apply_insert(OLive& l, const uint32_t* ext, const int8_t* rows, const float* norms, size_t n) {
+    if (n == 0) return;
+    const Index& ix = *l.ix;
+    std::vector<float> xs(n * PDIM);
+    for (size_t j = 0; j < n; j++) {
+        float* x = xs.data() + j * PDIM;
+        for (int i = 0; i < PDIM; i++) x[i] = (float)rows[j * DIM + i] / ix.quant[i];
+        normalize(x, PDIM);
+    }
+    std::vector<uint32_t> pairs(n * 2, PAD);
+    if (l.soar) {
+        const size_t m = 16;  // SOAR_CAND, src/live.rs:34
+        std::vector<uint32_t> top(n * m, 0);
+        or_assign_top(xs.data(), n, ix.centroids, ix.k, m, top.data(), 1);
+        for (size_t j = 0; j < n; j++) {
+            pairs[2 * j] = top[j * m];
+            pairs[2 * j + 1] = or_soar_secondary(xs.data() + j * PDIM, ix.centroids, top.data() + j * m, m, l.lambda);
+        }
+    } else {
+        std::vector<uint32_t> top(n);
+        or_assign_top(xs.data(), n, ix.centroids, ix.k, 1, top.data(), 1);
+        for (size_t j = 0; j < n; j++) pairs[2 * j] = top[j];
+    }
+    std::vector<uint8_t> codes(n * 2 * 32, 0);
+    std::vector<float> fs(n * 2 * 2, 0.0f);
+    std::vector<float> xr(PDIM);
+    for (size_t j = 0; j < n; j++) {
+        std::copy(xs.begin() + j * PDIM, xs.begin() + (j + 1) * PDIM, xr.begin());
+        rotation_apply(ix.rotation, PDIM, xr.data());
+        for (int h = 0; h < 2; h++) {
+            uint32_t c = pairs[2 * j + h];
+            if (c == PAD) continue;
+            rabitq_encode(xr.data(), l.cents_rot.data() + (size_t)c * PDIM, PDIM, codes.data() + (2 * j + h) * 32,
+                          fs.data() + (2 * j + h) * 2);
+        }
+    }
+    if (ix.t2_sigma) {
+        size_t at = l.delta.t2.size();
+        l.delta.t2.resize(at + n * 64, 0);
+        for (size_t j = 0; j < n; j++)
+            l.delta.t2_norm.push_back(tier2_encode(ix.t2_sigma, xs.data() + j * PDIM, l.delta.t2.data() + at + j * 64));
+    }
+    for (size_t j = 0; j < n; j++) {
+        uint32_t old;
+        if (live_current(l, ext[j], &old)) live_set_bit(l.deleted, old);
+        uint32_t id = (uint32_t)(l.delta.base + l.delta.ext.size());
+        l.delta.ext.push_back(ext[j]);
+        l.delta.rows.insert(l.delta.rows.end(), rows + j * DIM, rows + (j + 1) * DIM);
+        l.delta.norm256.push_back(norms[j]);
+        l.by_ext[ext[j]] = id;
+        for (int h = 0; h < 2; h++) {
+            uint32_t c = pairs[2 * j + h];
+            if (c == PAD) continue;
+            l.delta.lists[c].push(codes.data() + (2 * j + h) * 32, fs.data() + (2 * j + h) * 2, id);
+        }
+    }
+}
+
+extern "C" {
+
+void* or_live_create(const void* index, int soar, float lambda) {
+    const Index& ix = *(const Index*)index;
+    OLive* l = new OLive();
+    l->ix = &ix;
+    l->soar = soar != 0;
+    l->lambda = lambda;
+    l->cents_rot.assign(ix.centroids, ix.centroids + (size_t)ix.k * PDIM);
+    for (uint32_t c = 0; c < ix.k; c++) rotation_apply(ix.rotation, PDIM, l->cents_rot.data() + (size_t)c * PDIM);
+    l->delta.base = ix.n;
+    l->delta.lists.resize(ix.k);
+    return l;
+}
+void or_live_free(void* h) { delete (OLive*)h; }
+
+// Live::insert, src/live.rs:287-306 (no log): int8 rows by quant.encode, prefix norms of the
+// float vectors, then apply_insert.
+void or_live_insert(void* h, const uint32_t* ext, const float* vecs, size_t n) {
+    OLive& l = *(OLive*)h;
+    std::vector<int8_t> rows(n * DIM);
+    std::vector<float> norms(n);
+    for (size_t j = 0; j < n; j++) {
+        or_int8_encode(vecs + j * DIM, l.ix->quant, DIM, rows.data() + j * DIM);
+        float s = rsum_seed();
+        for (int i = 0; i < PDIM; i++) s = s + vecs[j * DIM + i] * vecs[j * DIM + i];
+        norms[j] = std::sqrt(s);
+    }
+    live_apply_insert(l, ext, rows.data(), norms.data(), n);
+}
+
+// Live::delete / apply_delete, src/live.rs:308-326.
+void or_live_delete(void* h, const uint32_t* ext, size_t n) {
+    OLive& l = *(OLive*)h;
+    for (size_t j = 0; j < n; j++) {
+        uint32_t id;
+        if (live_current(l, ext[j], &id)) {
+            live_set_bit(l.deleted, id);
+            l.by_ext.erase(ext[j]);
+        }
+    }
+}
+
+uint32_t or_live_current(const void* h, uint32_t ext) {
+    uint32_t id;
+    return live_current(*(const OLive*)h, ext, &id) ? id : PAD;
+}
+size_t or_live_rows(const void* h) { return ((const OLive*)h)->delta.ext.size(); }
+
+// Delta row j: int8 row, prefix norm, tier-2 code and norm (if any), external id.
+void or_live_row(const void* h, size_t j, int8_t* row, float* norm, uint8_t* t2, float* t2n, uint32_t* ext) {
+    const Delta& d = ((const OLive*)h)->delta;
+    std::memcpy(row, d.rows.data() + j * DIM, DIM);
+    *norm = d.norm256[j];
+    if (!d.t2_norm.empty()) {
+        std::memcpy(t2, d.t2.data() + j * 64, 64);
+        *t2n = d.t2_norm[j];
+    }
+    *ext = d.ext[j];
+}
+
+// Cluster c's delta list: returns its slot count; copies raw codes (unpacked from the block
+// layout), [mul, err] and row ids of its first `cap` slots.
+size_t or_live_list(const void* h, uint32_t c, size_t cap, uint8_t* codes, float* factors, uint32_t* ids) {
+    const DeltaList& dl = ((const OLive*)h)->delta.lists[c];
+    for (size_t j = 0; j < dl.len && j < cap; j++) {
+        unpack_slot(dl.blocks.data() + j / BLOCK * 1024, PDIM / 4, (int)(j % BLOCK), codes + j * 32);
+        factors[2 * j] = dl.factors[2 * j];
+        factors[2 * j + 1] = dl.factors[2 * j + 1];
+        ids[j] = dl.ids[j];
+    }
+    return dl.len;
+}
+
+size_t or_live_deleted(const void* h, uint64_t* words, size_t cap) {
+    const auto& d = ((const OLive*)h)->deleted;
+    for (size_t i = 0; i < d.size() && i < cap; i++) words[i] = d[i];
+    return d.size();
+}
+
+// Live::search for nq queries (src/live.rs:381-385): search_view over tombstones and the delta
+// segment (and an optional per-query allowlist), results as external ids.
+void or_live_search_batch(const void* h, const float* q768, const float* q256, size_t nq, const sieve_params* p,
+                          const sieve_view* v, int threads, float* out_s, uint32_t* out_r, uint32_t* counts) {
+    const OLive& l = *(const OLive*)h;
+    const Index& ix = *l.ix;
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        for (;;) {
+            size_t i = next.fetch_add(1);
+            if (i >= nq) break;
+            View vw = make_view(v, i);
+            vw.deleted = l.deleted.data();
+            vw.deleted_words = l.deleted.size();
+            vw.delta = &l.delta;
+            size_t n = search_one(ix, q768 + i * DIM, q256 + i * PDIM, *p, vw, out_s + i * p->k, out_r + i * p->k,
+                                  nullptr);
+            for (size_t j = 0; j < n; j++) out_r[i * p->k + j] = live_ext_of(l, out_r[i * p->k + j]);
+            counts[i] = (uint32_t)n;
+        }
+    };
+    if (threads <= 1) {
+        work();
+    } else {
+        std::vector<std::thread> th;
+        for (int w = 0; w < threads; w++) th.emplace_back(work);
+        for (auto& x : th) x.join();
+    }
+}
+
+}  // extern "C"
+
+extern "C" {
+// DeltaList::push of n raw codes (32 B), factors and ids into a fresh list; copies out its
+// blocks (nblocks x 1024), factors and ids (nblocks x 32 each). Returns nblocks.
+size_t or_delta_list_fixture(size_t n, const uint8_t* codes, const float* factors, const uint32_t* ids,
+                             size_t cap_blocks, uint8_t* blocks, float* out_f, uint32_t* out_ids) {
+    DeltaList dl;
+    for (size_t i = 0; i < n; i++) dl.push(codes + i * 32, factors + 2 * i, ids[i]);
+    size_t nb = dl.blocks.size() / 1024;
+    if (nb <= cap_blocks) {
+        std::memcpy(blocks, dl.blocks.data(), dl.blocks.size());
+        std::memcpy(out_f, dl.factors.data(), dl.factors.size() * 4);
+        std::memcpy(out_ids, dl.ids.data(), dl.ids.size() * 4);
+    }
+    return nb;
+}
 }  // extern "C"
