@@ -577,33 +577,10 @@ int sieve_search_batch(const sieve_index* ix, sieve_stream* s, const float* q768
         DevOut<uint32_t> orr = dev_out(s, B_OR, rows, nq * p->k);
         DevOut<uint32_t> on = dev_out(s, B_ON, counts, nq);
         enqueue_search(ix, s, q2, q7, nq, p, view, os.dev, orr.dev, on.dev);
-        unsigned long long* cnt = static_cast<unsigned long long*>(s->b[B_CNT].p);
-        const uint32_t* pn = static_cast<const uint32_t*>(s->b[B_PN].p);
         out_done(s, os);
         out_done(s, orr);
         out_done(s, on);
-        unsigned long long hcnt[4] = {0, 0, 0, 0};
-        SIEVE_HIP(hipMemcpyAsync(hcnt, cnt, 32, hipMemcpyDeviceToHost, s->st));
-        std::vector<uint32_t> hpn(nq);
-        SIEVE_HIP(hipMemcpyAsync(hpn.data(), pn, nq * 4, hipMemcpyDeviceToHost, s->st));
-        check_overflow(s);
-        s->last = sieve_stats{};
-        s->last.queries = nq;
-        s->last.scanned = hcnt[0];
-        s->last.id_reads = hcnt[2];
-        s->last.reranked = hcnt[1];
-        for (uint32_t x : hpn) s->last.probed += x;
-        s->last.route_ns = elapsed_ns(s->ev[0], s->ev[1]);
-        s->last.scan_ns = elapsed_ns(s->ev[1], s->ev[2]);
-        s->last.rerank_ns = elapsed_ns(s->ev[2], s->ev[3]);
-        if (t) {
-            t->route += (uint64_t)s->last.route_ns;
-            t->scan += (uint64_t)s->last.scan_ns;
-            t->rerank += (uint64_t)s->last.rerank_ns;
-            t->scanned += hcnt[0];
-            t->reranked += hcnt[1];
-            t->probed += s->last.probed;
-        }
+        finish_search(s, nq, t);
     });
 }
 

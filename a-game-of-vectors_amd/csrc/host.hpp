@@ -106,6 +106,7 @@ struct sieve_index {
     uint8_t* t2_codes = nullptr;
     float* t2_norm = nullptr;
     size_t device_bytes = 0;
+    struct sieve_live* live = nullptr;  // attached live state (sieve_live_open)
     int scan_variant = 0;  // scan kernel variant (sieve_index_autotune)
     uint32_t scan_chunk_blocks = 0;  // 0: chunk size chosen per batch (sieve_index_set_scan_chunk)
 
@@ -326,6 +327,20 @@ inline void check_overflow(sieve_stream* s) {
                       " centroids within the BF16 error band of the nprobe-th score");
 }
 
+// The live delta segment a search reads (defaults: none).
+struct LiveArgs {
+    const uint32_t* dblock_off = nullptr;  // k + 1 delta blocks per cluster (prefix)
+    uint32_t dbase = 0;                    // first delta block in the index's slot arrays
+    uint32_t n_base = 0xFFFFFFFFu;         // corpus rows; delta row j is row id n_base + j
+    const int8_t* drows = nullptr;
+    const float* dnorm256 = nullptr;
+    const uint8_t* dt2 = nullptr;
+    const float* dt2n = nullptr;
+    const uint32_t* dext = nullptr;
+    const uint64_t* deleted = nullptr;     // tombstones over corpus and delta row ids
+    uint64_t deleted_words = 0;
+};
+
 // The work split of a batch's scan and its partial-list scratch (per stream).
 inline void set_scan_plan(ScanArgs& a, sieve_stream* s, size_t nq, uint32_t nprobe, double mean_blocks,
                           uint32_t max_cluster_blocks, uint32_t forced_chunk_blocks) {
@@ -343,8 +358,13 @@ inline void set_scan_plan(ScanArgs& a, sieve_stream* s, size_t nq, uint32_t npro
 
 inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, size_t nq, const float* ps,
                 const uint32_t* pc, const uint32_t* pn, int pstride, const sieve_params* p, const ViewArgs& view,
-                float* ce, uint32_t* cr, float* cx, uint32_t* cn, int cstride, unsigned long long* scanned) {
+                float* ce, uint32_t* cr, float* cx, uint32_t* cn, int cstride, unsigned long long* scanned,
+                const LiveArgs* lv = nullptr) {
     ScanArgs a{};
+    if (lv) {
+        a.dblock_off = lv->dblock_off;
+        a.dbase = lv->dbase;
+    }
     a.codes = ix->codes;
     a.mul = ix->mul;
     a.err = ix->err;
@@ -366,7 +386,7 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
     a.scanned = scanned;
     set_scan_plan(a, s, nq, (uint32_t)pstride, (double)ix->nblocks / ix->k, ix->max_cluster_blocks,
                   ix->scan_chunk_blocks);
-    const size_t lds = scan_lds_bytes(a.cand, pstride);
+    const size_t lds = scan_lds_bytes(a.cand, scan_segments(a));
     SIEVE_REQUIRE(lds <= 160 * 1024, SIEVE_ELIMIT, "scan workspace exceeds LDS");
     launch_scan(a, nq, ix->scan_variant, s->st);
     SIEVE_HIP(hipGetLastError());
@@ -374,8 +394,17 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
 
 inline void rerank_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, size_t nq, const float* ce,
                   const uint32_t* cr, const float* cx, const uint32_t* cn, int cstride, const sieve_params* p,
-                  float* os, uint32_t* orr, uint32_t* on, unsigned long long* reranked) {
+                  float* os, uint32_t* orr, uint32_t* on, unsigned long long* reranked, const LiveArgs* lv = nullptr) {
     RerankArgs a{};
+    a.n_base = 0xFFFFFFFFu;
+    if (lv && lv->drows) {
+        a.n_base = lv->n_base;
+        a.drows = lv->drows;
+        a.dnorm256 = lv->dnorm256;
+        a.dt2 = lv->dt2;
+        a.dt2n = lv->dt2n;
+        a.dext = lv->dext;
+    }
     a.est = ce;
     a.rows = cr;
     a.errs = cx;
@@ -410,7 +439,13 @@ inline void rerank_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt
 // (scanned, reranked, past-threshold), probe counts in B_PN, the routing overflow flag in
 // B_OVF; events ev[0..3] bracket route, scan and rerank.
 inline void enqueue_search(const sieve_index* ix, sieve_stream* s, const float* q2, const float* q7, size_t nq,
-                           const sieve_params* p, const ViewArgs& view, float* os, uint32_t* orr, uint32_t* on) {
+                           const sieve_params* p, const ViewArgs& view_in, float* os, uint32_t* orr, uint32_t* on,
+                           const LiveArgs* lv = nullptr) {
+    ViewArgs view = view_in;
+    if (lv && lv->deleted) {
+        view.deleted = lv->deleted;
+        view.deleted_words = lv->deleted_words;
+    }
     const int pstride = (int)std::min<uint32_t>(p->nprobe, ix->k);
     unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 4);
     float* ps = grow<float>(s->b[B_PS], nq * pstride);
@@ -426,10 +461,39 @@ inline void enqueue_search(const sieve_index* ix, sieve_stream* s, const float* 
     prep_queries(ix, s, q2, q7, nq, bt);
     route_batch(ix, s, bt, nq, p, ps, pc, pn, pstride);
     SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
-    scan_batch(ix, s, bt, nq, ps, pc, pn, pstride, p, view, ce, cr, cx, cn, (int)p->cand, cnt);
+    scan_batch(ix, s, bt, nq, ps, pc, pn, pstride, p, view, ce, cr, cx, cn, (int)p->cand, cnt, lv);
     SIEVE_HIP(hipEventRecord(s->ev[2], s->st));
-    rerank_batch(ix, s, bt, nq, ce, cr, cx, cn, (int)p->cand, p, os, orr, on, cnt + 1);
+    rerank_batch(ix, s, bt, nq, ce, cr, cx, cn, (int)p->cand, p, os, orr, on, cnt + 1, lv);
     SIEVE_HIP(hipEventRecord(s->ev[3], s->st));
+}
+
+// Waits for an enqueued search, fails if routing overflowed, and records its counters and
+// stage times in s->last (and adds them to *t).
+inline void finish_search(sieve_stream* s, size_t nq, sieve_timing* t) {
+    const unsigned long long* cnt = static_cast<const unsigned long long*>(s->b[B_CNT].p);
+    const uint32_t* pn = static_cast<const uint32_t*>(s->b[B_PN].p);
+    unsigned long long hcnt[4] = {0, 0, 0, 0};
+    SIEVE_HIP(hipMemcpyAsync(hcnt, cnt, 32, hipMemcpyDeviceToHost, s->st));
+    std::vector<uint32_t> hpn(nq);
+    SIEVE_HIP(hipMemcpyAsync(hpn.data(), pn, nq * 4, hipMemcpyDeviceToHost, s->st));
+    check_overflow(s);
+    s->last = sieve_stats{};
+    s->last.queries = nq;
+    s->last.scanned = hcnt[0];
+    s->last.id_reads = hcnt[2];
+    s->last.reranked = hcnt[1];
+    for (uint32_t x : hpn) s->last.probed += x;
+    s->last.route_ns = elapsed_ns(s->ev[0], s->ev[1]);
+    s->last.scan_ns = elapsed_ns(s->ev[1], s->ev[2]);
+    s->last.rerank_ns = elapsed_ns(s->ev[2], s->ev[3]);
+    if (t) {
+        t->route += (uint64_t)s->last.route_ns;
+        t->scan += (uint64_t)s->last.scan_ns;
+        t->rerank += (uint64_t)s->last.rerank_ns;
+        t->scanned += hcnt[0];
+        t->reranked += hcnt[1];
+        t->probed += s->last.probed;
+    }
 }
 
 inline void ensure_events(sieve_stream* s) {

@@ -8,6 +8,28 @@ namespace sieve {
 
 // Per-query state written by the query-prep kernel (rabitq::Query::new, src/rabitq.rs:119-125,
 // and the int8 rerank queries, src/ivf.rs:753-755 / :786-788).
+// Rotation::apply (src/rabitq.rs:59-84) on D floats in LDS by one 64-thread workgroup: three
+// rounds of (sign flip, unnormalised FWHT), then * d^-1.5 (`rot_norm`).
+template <int D>
+__device__ inline void rotate_lds(float* v, const float* signs, float rot_norm, int lane) {
+    for (int r = 0; r < 3; r++) {
+        for (int i = lane; i < D; i += 64) v[i] = v[i] * signs[r * D + i];
+        __syncthreads();
+        for (int h = 1; h < D; h <<= 1) {
+            for (int j = lane; j < D / 2; j += 64) {
+                const int blk = j / h, i = j - blk * h;
+                const int a = blk * 2 * h + i, b = a + h;
+                const float x = v[a], y = v[b];
+                v[a] = x + y;
+                v[b] = x - y;
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = lane; i < D; i += 64) v[i] = v[i] * rot_norm;
+    __syncthreads();
+}
+
 struct QueryPrep {
     uint8_t* tables;    // nq x 1024: 64 groups x 16 u8 entries
     float4* qab;        // nq: (a, b, scale, bias)
@@ -100,7 +122,13 @@ struct ScanArgs {
     uint32_t* part_n;            // nq * chunk_max
     unsigned long long* gthr;    // nq, zero between launches: best shared (key, ~slot) threshold
     uint32_t* qdone;             // nq, zero between launches: chunks finished
+    // Live delta lists (null: none): cluster c's delta blocks are [dbase + dblock_off[c],
+    // dbase + dblock_off[c + 1]) of the same slot arrays, scanned right after its base blocks.
+    const uint32_t* dblock_off;
+    uint32_t dbase;
 };
+// Block segments per probe: the base list, and with live delta lists the delta list too.
+__host__ __device__ inline int scan_segments(const ScanArgs& a) { return a.dblock_off ? 2 * a.probe_stride : a.probe_stride; }
 // Chunking of a batch's scan: fills the work-split fields of `a` (a.cand set) for nq queries of
 // up to nprobe probes over clusters of at most max_cluster_blocks blocks (mean_blocks on average);
 // forced_chunk_blocks != 0 fixes the chunk size.
@@ -144,6 +172,14 @@ struct RerankArgs {
     uint32_t* out_r;
     uint32_t* out_n;
     unsigned long long* reranked;
+    // Live delta rows (drows null: none): row ids >= n_base are delta row (id - n_base), read
+    // from drows / dnorm256 / dt2 / dt2n, and reported as dext[id - n_base].
+    uint32_t n_base;
+    const int8_t* drows;
+    const float* dnorm256;
+    const uint8_t* dt2;
+    const float* dt2n;
+    const uint32_t* dext;
 };
 size_t rerank_lds_bytes(int cand_stride, bool tier2_tables);
 void launch_rerank(const RerankArgs& a, size_t nq, hipStream_t st);

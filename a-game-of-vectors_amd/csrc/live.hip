@@ -1,0 +1,646 @@
+// Live updates on the device: Live (src/live.rs). The built index stays as it is; inserted
+// rows are encoded exactly as the build encodes a row (int8 row, prefix from the int8 row,
+// nearest centroids plus a SOAR spill, RaBitQ residual codes against each centroid, a 2-bit
+// tier code) and their slots go to per-cluster delta lists that the scan reads right after
+// each probed cluster's base list (src/ivf.rs:682-684); deletes set tombstones the scan checks
+// at admission. Row ids: corpus rows keep theirs, delta rows count up from the corpus size;
+// searches report external ids (src/live.rs:11-14).
+//
+// Device layout. The index's slot arrays (codes, mul, err, ids) get room for an arena after
+// the base slots; the arena holds the delta lists cluster by cluster, each padded to whole
+// 32-slot blocks, so the scan reads a delta list as one more block range. Every insert batch
+// appends its slots to a log (insertion order) and rebuilds the arena from the log with a
+// stable sort by cluster, which keeps each list in insertion order as DeltaList::push does.
+//
+// Concurrency: like the reference's &mut Live (src/live.rs:16-17), inserts and deletes must
+// not overlap searches of the same index.
+#include <float.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include <unordered_map>
+
+#include "host.hpp"
+
+using namespace sieve;
+
+namespace {
+
+constexpr int SOAR_CAND = 16;  // src/live.rs:34
+constexpr float EPS0 = 1.9f;   // src/rabitq.rs:28
+constexpr float T2_T = 0.9816f;
+__constant__ float T2_LEVELS[4] = {-1.510f, -0.4528f, 0.4528f, 1.510f};  // src/tier2.rs:10-11
+
+__device__ __forceinline__ float sqrt_rn(float x) { return (float)sqrt((double)x); }
+
+// quant::Int8Quantizer::encode (src/quant.rs:52-56), the prefix norm of Live::insert
+// (src/live.rs:290-293), and the dequantised, normalised prefix of apply_insert
+// (src/live.rs:336-341). One thread per row: every sum is sequential, as the reference's.
+__global__ void k_live_rows(const float* __restrict__ vecs, size_t n, const float* __restrict__ quant,
+                            int8_t* __restrict__ rows, float* __restrict__ norm256, float* __restrict__ xs) {
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const float* v = vecs + j * DIM;
+    int8_t* r = rows + j * DIM;
+    for (int i = 0; i < DIM; i++) {
+        float q = roundf(v[i] * quant[i]);
+        q = fminf(fmaxf(q, -127.0f), 127.0f);
+        r[i] = (int8_t)q;
+    }
+    float s = -0.0f;
+    for (int i = 0; i < PDIM; i++) s = s + v[i] * v[i];
+    norm256[j] = sqrt_rn(s);
+    float* x = xs + j * PDIM;
+    float ss = -0.0f;
+    for (int i = 0; i < PDIM; i++) {
+        const float t = (float)r[i] / quant[i];
+        x[i] = t;
+        ss = ss + t * t;
+    }
+    const float nn = sqrt_rn(ss);  // quant::normalize, src/quant.rs:5-13
+    if (nn > 0.0f) {
+        const float inv = 1.0f / nn;
+        for (int i = 0; i < PDIM; i++) x[i] = x[i] * inv;
+    }
+}
+
+// quant::dot_f32 (src/quant.rs:86-100): 16 lane accumulators, then their sum.
+__device__ float dot16(const float* a, const float* b) {
+    float acc[16];
+#pragma unroll
+    for (int l = 0; l < 16; l++) acc[l] = 0.0f;
+    for (int i = 0; i < PDIM; i += 16)
+#pragma unroll
+        for (int l = 0; l < 16; l++) acc[l] = acc[l] + a[i + l] * b[i + l];
+    float s = -0.0f;
+#pragma unroll
+    for (int l = 0; l < 16; l++) s = s + acc[l];
+    return s + -0.0f;
+}
+
+// The (primary, secondary) clusters of each row: assign_top's best centroid and
+// soar_secondary over its SOAR_CAND best (src/ivf.rs:164-186); PAD without SOAR. With fewer
+// than SOAR_CAND clusters, assign_top leaves the candidate list's tail at cluster 0
+// (src/ivf.rs:141, :154), and so does this.
+__global__ void k_live_pairs(const float* __restrict__ xs, size_t n, const float* __restrict__ cents,
+                             const uint32_t* __restrict__ pc, const uint32_t* __restrict__ pn, int pstride, int soar,
+                             float lambda, uint32_t* __restrict__ pairs) {
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t* cand = pc + j * pstride;
+    const int m = min((int)pn[j], pstride);
+    auto cand_at = [&](int i) -> uint32_t { return i < m ? cand[i] : 0u; };
+    pairs[2 * j] = cand_at(0);
+    if (!soar) {
+        pairs[2 * j + 1] = PAD;
+        return;
+    }
+    const float* x = xs + j * PDIM;
+    const float* c1 = cents + (size_t)cand_at(0) * PDIM;
+    const float x_c1 = dot16(x, c1);
+    const float r2 = fmaxf(2.0f - 2.0f * x_c1, 1e-12f);
+    const float r_x = 1.0f - x_c1;
+    float best = FLT_MAX;
+    uint32_t bc = cand_at(1);
+    for (int i = 1; i < SOAR_CAND; i++) {
+        const uint32_t ci = cand_at(i);
+        const float* cv = cents + (size_t)ci * PDIM;
+        const float x_c = dot16(x, cv);
+        const float r_c = x_c - dot16(c1, cv);
+        const float along = r_x - r_c;
+        const float loss = (2.0f - 2.0f * x_c) + lambda * along * along / r2;
+        if (loss < best) {
+            best = loss;
+            bc = ci;
+        }
+    }
+    pairs[2 * j + 1] = bc;
+}
+
+// Per row (one 64-thread workgroup): Rotation::apply of the prefix, then for each of its
+// clusters rabitq::encode against the rotated centroid (src/rabitq.rs:86-108) into the slot
+// log, and tier2::encode of the prefix (src/tier2.rs:39-50).
+__global__ __launch_bounds__(64) void k_live_encode(const float* __restrict__ xs, size_t n,
+                                                    const float* __restrict__ signs, float rot_norm,
+                                                    const float* __restrict__ cents_rot,
+                                                    const uint32_t* __restrict__ pairs, int spr, uint32_t row0,
+                                                    uint4* __restrict__ lcode, float* __restrict__ lmul,
+                                                    float* __restrict__ lerr, uint32_t* __restrict__ lrow,
+                                                    uint32_t* __restrict__ lcl, const float* __restrict__ t2_sigma,
+                                                    uint8_t* __restrict__ t2, float* __restrict__ t2n) {
+    __shared__ float v[PDIM];
+    __shared__ float r[PDIM];
+    __shared__ uint32_t code[8];
+    const size_t j = blockIdx.x;
+    const int lane = threadIdx.x;
+    const float* x = xs + j * PDIM;
+    for (int i = lane; i < PDIM; i += 64) v[i] = x[i];
+    __syncthreads();
+    if (t2_sigma && lane == 0) {
+        uint8_t out[PDIM / 4];
+        for (int b = 0; b < PDIM / 4; b++) out[b] = 0;
+        float nn = 0.0f;
+        for (int i = 0; i < PDIM; i++) {
+            const float s = t2_sigma[i], xi = v[i], t = T2_T * s;
+            const int l = xi < -t ? 0 : xi < 0.0f ? 1 : xi < t ? 2 : 3;
+            out[i >> 2] |= (uint8_t)(l << ((i & 3) * 2));
+            const float q = T2_LEVELS[l] * s;
+            nn = nn + q * q;
+        }
+        for (int b = 0; b < PDIM / 4; b++) t2[j * 64 + b] = out[b];
+        t2n[j] = sqrt_rn(nn);
+    }
+    __syncthreads();
+    rotate_lds<PDIM>(v, signs, rot_norm, lane);
+    for (int h = 0; h < spr; h++) {
+        const uint32_t c = pairs[2 * j + h];
+        const size_t L = j * spr + h;
+        const float* cr = cents_rot + (size_t)c * PDIM;
+        for (int i = lane; i < PDIM; i += 64) r[i] = v[i] - cr[i];
+        __syncthreads();
+        if (lane < 8) {
+            uint32_t w = 0;
+            for (int b = 0; b < 32; b++)
+                if (r[lane * 32 + b] > 0.0f) w |= 1u << b;
+            code[lane] = w;
+        }
+        if (lane == 0) {
+            float nr2 = 0.0f, l1 = 0.0f;
+            for (int i = 0; i < PDIM; i++) {
+                nr2 = nr2 + r[i] * r[i];
+                l1 = l1 + fabsf(r[i]);
+            }
+            float mul = 0.0f, err = 0.0f;
+            if (!(l1 < 1e-12f)) {
+                const float df = (float)PDIM;
+                const float nr = sqrt_rn(nr2);
+                const float od = l1 / (nr * sqrt_rn(df));
+                float inner = (1.0f - od * od) / (od * od * (df - 1.0f));
+                inner = fmaxf(inner, 0.0f);
+                err = nr * EPS0 * sqrt_rn(inner);
+                mul = nr2 / l1;
+            }
+            lmul[L] = mul;
+            lerr[L] = err;
+            lrow[L] = row0 + (uint32_t)j;
+            lcl[L] = c;
+        }
+        __syncthreads();
+        if (lane < 2) lcode[2 * L + lane] = make_uint4(code[4 * lane], code[4 * lane + 1], code[4 * lane + 2],
+                                                       code[4 * lane + 3]);
+        __syncthreads();
+    }
+}
+
+// Rotation::apply of each centroid (src/live.rs:183-192).
+__global__ __launch_bounds__(64) void k_rotate_rows(const float* __restrict__ in, const float* __restrict__ signs,
+                                                    float rot_norm, float* __restrict__ out) {
+    __shared__ float v[PDIM];
+    const size_t c = blockIdx.x;
+    const int lane = threadIdx.x;
+    for (int i = lane; i < PDIM; i += 64) v[i] = in[c * PDIM + i];
+    __syncthreads();
+    rotate_lds<PDIM>(v, signs, rot_norm, lane);
+    for (int i = lane; i < PDIM; i += 64) out[c * PDIM + i] = v[i];
+}
+
+__global__ void k_iota(uint32_t* v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = (uint32_t)i;
+}
+
+// counts[c] = slots of cluster c in the log; nb[c] = its delta blocks (counts padded to 32).
+__global__ void k_cluster_counts(const uint32_t* __restrict__ sorted_cl, size_t m, uint32_t k,
+                                 uint32_t* __restrict__ counts) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) atomicAdd(&counts[sorted_cl[i]], 1u);
+}
+__global__ void k_blocks_of(const uint32_t* __restrict__ counts, uint32_t k, uint32_t* __restrict__ nb) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c <= k) nb[c] = c < k ? (counts[c] + BLOCK - 1) / BLOCK : 0u;
+}
+
+// Arena slots [0, nslots_arena): empty (PAD id, zero code and factors).
+__global__ void k_arena_clear(uint4* __restrict__ codes, float* __restrict__ mul, float* __restrict__ err,
+                              uint32_t* __restrict__ ids, size_t slots) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= slots) return;
+    codes[2 * i] = make_uint4(0u, 0u, 0u, 0u);
+    codes[2 * i + 1] = make_uint4(0u, 0u, 0u, 0u);
+    mul[i] = 0.0f;
+    err[i] = 0.0f;
+    ids[i] = PAD;
+}
+
+// Sorted log entry i (cluster c, log index L) goes to arena slot dblock_off[c] * 32 + its rank
+// within the cluster (i - cstart[c]).
+__global__ void k_arena_scatter(const uint32_t* __restrict__ sorted_cl, const uint32_t* __restrict__ sorted_idx,
+                                size_t m, const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ dblock_off,
+                                const uint4* __restrict__ lcode, const float* __restrict__ lmul,
+                                const float* __restrict__ lerr, const uint32_t* __restrict__ lrow,
+                                uint4* __restrict__ codes, float* __restrict__ mul, float* __restrict__ err,
+                                uint32_t* __restrict__ ids) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t c = sorted_cl[i], L = sorted_idx[i];
+    const size_t dst = (size_t)dblock_off[c] * BLOCK + (i - cstart[c]);
+    codes[2 * dst] = lcode[2 * (size_t)L];
+    codes[2 * dst + 1] = lcode[2 * (size_t)L + 1];
+    mul[dst] = lmul[L];
+    err[dst] = lerr[L];
+    ids[dst] = lrow[L];
+}
+
+__global__ void k_set_words(uint64_t* __restrict__ set, const uint64_t* __restrict__ idx,
+                            const uint64_t* __restrict__ val, size_t m) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) set[idx[i]] = val[i];
+}
+
+inline unsigned grid(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+template <class T>
+void dev_realloc(T*& p, size_t old_n, size_t new_n, hipStream_t st, size_t* acct) {
+    T* q = dmalloc<T>(new_n, nullptr);
+    if (acct) *acct += (new_n - old_n) * sizeof(T);
+    if (p && old_n) SIEVE_HIP(hipMemcpyAsync(q, p, old_n * sizeof(T), hipMemcpyDeviceToDevice, st));
+    SIEVE_HIP(hipStreamSynchronize(st));
+    if (p) (void)hipFree(p);
+    p = q;
+}
+
+}  // namespace
+
+struct sieve_live {
+    sieve_index* ix = nullptr;
+    bool soar = true;
+    float lambda = 1.0f;
+    uint64_t n = 0;            // corpus rows
+    uint64_t cap_rows = 0;     // delta rows the device holds
+    uint64_t cap_slots = 0;    // arena slots after the base slots
+    uint64_t nrows = 0, nlog = 0, arena_blocks = 0, ndeleted = 0;
+    size_t device_bytes = 0;
+    float* cents_rot = nullptr;
+    int8_t* rows = nullptr;
+    float* norm256 = nullptr;
+    uint8_t* t2 = nullptr;
+    float* t2n = nullptr;
+    uint32_t* ext = nullptr;
+    uint64_t* deleted = nullptr;
+    uint64_t deleted_words = 0;
+    uint4* lcode = nullptr;
+    float* lmul = nullptr;
+    float* lerr = nullptr;
+    uint32_t* lrow = nullptr;
+    uint32_t* lcl = nullptr;
+    uint32_t* dblock_off = nullptr;  // k + 1
+    // writer-side state on the host
+    std::unordered_map<uint32_t, uint32_t> by_ext;
+    std::vector<uint64_t> deleted_host;
+    sieve_stream work;  // scratch for the encoder and the arena rebuild
+    DevBuf xs, pairs, sort_keys, sort_vals, sort_tmp, counts, nb, cstart, words_i, words_v;
+
+    ~sieve_live() {
+        void* ps[] = {cents_rot, rows, norm256, t2, t2n, ext, deleted, lcode, lmul, lerr, lrow, lcl, dblock_off};
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+        for (DevBuf* b : {&xs, &pairs, &sort_keys, &sort_vals, &sort_tmp, &counts, &nb, &cstart, &words_i, &words_v})
+            b->release();
+    }
+
+    LiveArgs args() const {
+        LiveArgs a;
+        a.dblock_off = dblock_off;
+        a.dbase = (uint32_t)ix->nblocks;
+        a.n_base = (uint32_t)n;
+        a.drows = rows;
+        a.dnorm256 = norm256;
+        a.dt2 = ix->t2_codes ? t2 : nullptr;
+        a.dt2n = ix->t2_codes ? t2n : nullptr;
+        a.dext = ext;
+        a.deleted = deleted;
+        a.deleted_words = deleted_words;
+        return a;
+    }
+
+    // Live::current (src/live.rs:271-277).
+    bool current(uint32_t e, uint32_t* id) const {
+        auto it = by_ext.find(e);
+        if (it != by_ext.end()) {
+            *id = it->second;
+            return true;
+        }
+        if ((uint64_t)e < n && !bit_host(e)) {
+            *id = e;
+            return true;
+        }
+        return false;
+    }
+    bool bit_host(uint32_t id) const {
+        const size_t w = id >> 6;
+        return w < deleted_host.size() && ((deleted_host[w] >> (id & 63)) & 1) == 1;
+    }
+};
+
+namespace {
+
+// Uploads the changed tombstone words.
+void push_words(sieve_live* l, hipStream_t st, const std::vector<uint64_t>& changed) {
+    if (changed.empty()) return;
+    std::vector<uint64_t> vals(changed.size());
+    for (size_t i = 0; i < changed.size(); i++) vals[i] = l->deleted_host[changed[i]];
+    uint64_t* di = grow<uint64_t>(l->words_i, changed.size());
+    uint64_t* dv = grow<uint64_t>(l->words_v, changed.size());
+    SIEVE_HIP(hipMemcpyAsync(di, changed.data(), changed.size() * 8, hipMemcpyHostToDevice, st));
+    SIEVE_HIP(hipMemcpyAsync(dv, vals.data(), vals.size() * 8, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_set_words, dim3(grid(changed.size(), 256)), dim3(256), 0, st, l->deleted, di, dv,
+                       changed.size());
+    SIEVE_HIP(hipGetLastError());
+    SIEVE_HIP(hipStreamSynchronize(st));
+}
+
+// Sets the tombstone of `id` on the host; records its word.
+void tombstone(sieve_live* l, uint32_t id, std::vector<uint64_t>& changed) {
+    const size_t w = id >> 6;
+    l->deleted_host[w] |= 1ull << (id & 63);
+    changed.push_back(w);
+    l->ndeleted++;
+}
+
+// Rebuilds the arena from the slot log: a stable sort of the log by cluster, delta blocks per
+// cluster, then each slot to its place.
+void rebuild_arena(sieve_live* l, hipStream_t st) {
+    sieve_index* ix = l->ix;
+    const size_t m = l->nlog;
+    const uint32_t k = ix->k;
+    uint32_t* counts = grow<uint32_t>(l->counts, k + 1);
+    uint32_t* nb = grow<uint32_t>(l->nb, k + 1);
+    uint32_t* cstart = grow<uint32_t>(l->cstart, k + 1);
+    SIEVE_HIP(hipMemsetAsync(counts, 0, (k + 1) * 4, st));
+    uint32_t* keys_out = grow<uint32_t>(l->sort_keys, std::max<size_t>(m, 1));
+    uint32_t* vals = grow<uint32_t>(l->sort_vals, 2 * std::max<size_t>(m, 1));
+    uint32_t* vals_in = vals + std::max<size_t>(m, 1);
+    if (m) {
+        hipLaunchKernelGGL(k_iota, dim3(grid(m, 256)), dim3(256), 0, st, vals_in, m);
+        int bits = 1;
+        while (bits < 32 && (1u << bits) < k) bits++;
+        size_t tmp_bytes = 0;
+        SIEVE_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, l->lcl, keys_out, vals_in, vals, (int)m, 0,
+                                                     bits, st));
+        void* tmp = grow<uint8_t>(l->sort_tmp, tmp_bytes);
+        SIEVE_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, l->lcl, keys_out, vals_in, vals, (int)m, 0, bits,
+                                                     st));
+        hipLaunchKernelGGL(k_cluster_counts, dim3(grid(m, 256)), dim3(256), 0, st, keys_out, m, k, counts);
+    }
+    hipLaunchKernelGGL(k_blocks_of, dim3(grid(k + 1, 256)), dim3(256), 0, st, counts, k, nb);
+    size_t tmp_bytes = 0;
+    SIEVE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts, cstart, (int)(k + 1), st));
+    size_t tmp2 = 0;
+    SIEVE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, nb, l->dblock_off, (int)(k + 1), st));
+    void* tmp = grow<uint8_t>(l->sort_tmp, std::max(tmp_bytes, tmp2));
+    SIEVE_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, counts, cstart, (int)(k + 1), st));
+    SIEVE_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp2, nb, l->dblock_off, (int)(k + 1), st));
+    uint32_t blocks = 0;
+    SIEVE_HIP(hipMemcpyAsync(&blocks, l->dblock_off + k, 4, hipMemcpyDeviceToHost, st));
+    SIEVE_HIP(hipStreamSynchronize(st));
+    SIEVE_REQUIRE((uint64_t)blocks * BLOCK <= l->cap_slots, SIEVE_ELIMIT, "live arena overflow");
+    const size_t base = ix->nslots;
+    const size_t clear = std::max<uint64_t>(blocks, l->arena_blocks) * BLOCK;
+    if (clear)
+        hipLaunchKernelGGL(k_arena_clear, dim3(grid(clear, 256)), dim3(256), 0, st, ix->codes + 2 * base,
+                           ix->mul + base, ix->err + base, ix->ids + base, clear);
+    if (m)
+        hipLaunchKernelGGL(k_arena_scatter, dim3(grid(m, 256)), dim3(256), 0, st, keys_out, vals, m, cstart,
+                           l->dblock_off, l->lcode, l->lmul, l->lerr, l->lrow, ix->codes + 2 * base, ix->mul + base,
+                           ix->err + base, ix->ids + base);
+    SIEVE_HIP(hipGetLastError());
+    SIEVE_HIP(hipStreamSynchronize(st));
+    l->arena_blocks = blocks;
+}
+
+void live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, size_t n) {
+    sieve_index* ix = l->ix;
+    SIEVE_REQUIRE(l->nrows + n <= l->cap_rows, SIEVE_ELIMIT,
+                  "live capacity: " + std::to_string(l->cap_rows) + " delta rows");
+    if (n == 0) return;
+    sieve_stream* s = &l->work;
+    const hipStream_t st = s->st;
+    const float* dv = dev_in(s, B_Q768, vecs, n * DIM);
+    float* xs = grow<float>(l->xs, n * PDIM);
+    const size_t r0 = l->nrows;
+    hipLaunchKernelGGL(k_live_rows, dim3(grid(n, 64)), dim3(64), 0, st, dv, n, ix->quant, l->rows + r0 * DIM,
+                       l->norm256 + r0, xs);
+    SIEVE_HIP(hipGetLastError());
+    // assign_top (src/ivf.rs:137-158) through the exact route: BF16 GEMM plus f32 rescoring of
+    // every centroid near the cut gives the reference's f32 top SOAR_CAND, ties by id.
+    const int m = (int)std::min<uint32_t>(l->soar ? SOAR_CAND : 1, ix->k);
+    sieve_params rp{};
+    rp.cand = 1;
+    rp.k = 1;
+    rp.nprobe = (uint32_t)m;
+    rp.margin = std::numeric_limits<float>::quiet_NaN();
+    rp.budget = 0;
+    Batch bt;
+    prep_queries(ix, s, xs, dv, n, bt);
+    float* ps = grow<float>(s->b[B_PS], n * m);
+    uint32_t* pc = grow<uint32_t>(s->b[B_PC], n * m);
+    uint32_t* pn = grow<uint32_t>(s->b[B_PN], n);
+    route_batch(ix, s, bt, n, &rp, ps, pc, pn, m);
+    check_overflow(s);
+    uint32_t* pairs = grow<uint32_t>(l->pairs, 2 * n);
+    hipLaunchKernelGGL(k_live_pairs, dim3(grid(n, 64)), dim3(64), 0, st, xs, n, ix->cents, pc, pn, m, l->soar ? 1 : 0,
+                       l->lambda, pairs);
+    const int spr = l->soar ? 2 : 1;
+    SIEVE_REQUIRE(l->nlog + spr * n <= 2 * l->cap_rows, SIEVE_ELIMIT, "live slot log full");
+    const size_t L0 = l->nlog;
+    hipLaunchKernelGGL(k_live_encode, dim3((unsigned)n), dim3(64), 0, st, xs, n, ix->rotation, 1.0f / 4096.0f,
+                       l->cents_rot, pairs, spr, (uint32_t)(l->n + r0), l->lcode + 2 * L0, l->lmul + L0,
+                       l->lerr + L0, l->lrow + L0, l->lcl + L0, ix->t2_sigma, l->t2 + r0 * 64, l->t2n + r0);
+    SIEVE_HIP(hipGetLastError());
+    SIEVE_HIP(hipMemcpyAsync(l->ext + r0, ext, n * 4, on_device(ext) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                             st));
+    SIEVE_HIP(hipStreamSynchronize(st));
+    // Upserts, in order (src/live.rs:369-378): a live external id's old row is tombstoned.
+    std::vector<uint32_t> he;
+    const uint32_t* eh = ext;
+    if (on_device(ext)) {
+        he.resize(n);
+        SIEVE_HIP(hipMemcpy(he.data(), ext, n * 4, hipMemcpyDeviceToHost));
+        eh = he.data();
+    }
+    std::vector<uint64_t> changed;
+    for (size_t j = 0; j < n; j++) {
+        uint32_t old;
+        if (l->current(eh[j], &old)) tombstone(l, old, changed);
+        l->by_ext[eh[j]] = (uint32_t)(l->n + r0 + j);
+    }
+    push_words(l, st, changed);
+    l->nrows += n;
+    l->nlog += spr * n;
+    rebuild_arena(l, st);
+}
+
+void live_delete(sieve_live* l, const uint32_t* ext, size_t n) {
+    std::vector<uint32_t> he;
+    const uint32_t* eh = ext;
+    if (on_device(ext)) {
+        he.resize(n);
+        SIEVE_HIP(hipMemcpy(he.data(), ext, n * 4, hipMemcpyDeviceToHost));
+        eh = he.data();
+    }
+    std::vector<uint64_t> changed;
+    for (size_t j = 0; j < n; j++) {  // Live::apply_delete, src/live.rs:317-326
+        uint32_t id;
+        if (l->current(eh[j], &id)) {
+            if (!l->bit_host(id)) tombstone(l, id, changed);
+            l->by_ext.erase(eh[j]);
+        }
+    }
+    push_words(l, l->work.st, changed);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sieve_live_open(sieve_index* ix, const sieve_live_opts* o, sieve_live** out) {
+    return guarded([&] {
+        SIEVE_REQUIRE(ix && out, SIEVE_EINVAL, "null argument");
+        SIEVE_REQUIRE(ix->rotation && ix->mul, SIEVE_EINVAL, "live updates need a RaBitQ index");
+        SIEVE_REQUIRE(!ix->live, SIEVE_EINVAL, "the index already has a live state");
+        sieve_live_opts opts = o ? *o : sieve_live_opts{};
+        SIEVE_REQUIRE(opts.max_rows >= 1, SIEVE_EINVAL, "max_rows must be >= 1");
+        SIEVE_REQUIRE(ix->n + opts.max_rows < 0xFFFFFFFFull, SIEVE_ELIMIT, "row ids exceed 32 bits");
+        std::unique_ptr<sieve_live> l(new sieve_live());
+        SIEVE_HIP(hipSetDevice(ix->device));
+        l->ix = ix;
+        l->soar = !(opts.soar_lambda != opts.soar_lambda);
+        l->lambda = l->soar ? opts.soar_lambda : 0.0f;
+        l->n = ix->n;
+        l->cap_rows = opts.max_rows;
+        l->cap_slots = 2 * opts.max_rows + (uint64_t)BLOCK * ix->k;
+        SIEVE_REQUIRE(ix->nslots + l->cap_slots < 0xFFFFFFFFull, SIEVE_ELIMIT, "slot ids exceed 32 bits");
+        SIEVE_HIP(hipStreamCreate(&l->work.st));
+        l->work.device = ix->device;
+        const hipStream_t st = l->work.st;
+        size_t* acct = &l->device_bytes;
+        // Room for the arena after the base slots.
+        const size_t ns = ix->nslots, total = ns + l->cap_slots;
+        dev_realloc(ix->codes, 2 * ns, 2 * total, st, &ix->device_bytes);
+        dev_realloc(ix->mul, ns, total, st, &ix->device_bytes);
+        dev_realloc(ix->err, ns, total, st, &ix->device_bytes);
+        dev_realloc(ix->ids, ns, total, st, &ix->device_bytes);
+        l->cents_rot = dmalloc<float>((size_t)ix->k * PDIM, acct);
+        hipLaunchKernelGGL(k_rotate_rows, dim3(ix->k), dim3(64), 0, st, ix->cents, ix->rotation, 1.0f / 4096.0f,
+                           l->cents_rot);
+        const size_t R = l->cap_rows;
+        l->rows = dmalloc<int8_t>(R * DIM, acct);
+        l->norm256 = dmalloc<float>(R, acct);
+        l->t2 = dmalloc<uint8_t>(R * 64, acct);
+        l->t2n = dmalloc<float>(R, acct);
+        l->ext = dmalloc<uint32_t>(R, acct);
+        l->deleted_words = (l->n + R + 63) / 64;
+        l->deleted = dmalloc<uint64_t>(l->deleted_words, acct);
+        SIEVE_HIP(hipMemsetAsync(l->deleted, 0, l->deleted_words * 8, st));
+        l->deleted_host.assign(l->deleted_words, 0);
+        l->lcode = dmalloc<uint4>(2 * 2 * R, acct);
+        l->lmul = dmalloc<float>(2 * R, acct);
+        l->lerr = dmalloc<float>(2 * R, acct);
+        l->lrow = dmalloc<uint32_t>(2 * R, acct);
+        l->lcl = dmalloc<uint32_t>(2 * R, acct);
+        l->dblock_off = dmalloc<uint32_t>(ix->k + 1, acct);
+        SIEVE_HIP(hipGetLastError());
+        rebuild_arena(l.get(), st);
+        ix->live = l.get();
+        *out = l.release();
+    });
+}
+
+void sieve_live_close(sieve_live* l) {
+    if (!l) return;
+    if (l->ix) l->ix->live = nullptr;
+    delete l;
+}
+
+int sieve_live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, size_t n) {
+    return guarded([&] {
+        SIEVE_REQUIRE(l && (n == 0 || (ext && vecs)), SIEVE_EINVAL, "null argument");
+        SIEVE_HIP(hipSetDevice(l->ix->device));
+        live_insert(l, ext, vecs, n);
+    });
+}
+
+int sieve_live_delete(sieve_live* l, const uint32_t* ext, size_t n) {
+    return guarded([&] {
+        SIEVE_REQUIRE(l && (n == 0 || ext), SIEVE_EINVAL, "null argument");
+        SIEVE_HIP(hipSetDevice(l->ix->device));
+        live_delete(l, ext, n);
+    });
+}
+
+int sieve_live_get_info(const sieve_live* l, sieve_live_info* out) {
+    return guarded([&] {
+        SIEVE_REQUIRE(l && out, SIEVE_EINVAL, "null argument");
+        out->rows = l->nrows;
+        out->slots = l->nlog;
+        out->arena_blocks = l->arena_blocks;
+        out->deleted = l->ndeleted;
+        out->max_rows = l->cap_rows;
+        out->device_bytes = l->device_bytes;
+    });
+}
+
+uint32_t sieve_live_current(const sieve_live* l, uint32_t ext) {
+    uint32_t id;
+    return l && l->current(ext, &id) ? id : 0xFFFFFFFFu;
+}
+
+int sieve_live_export(const sieve_live* l, int8_t* rows, float* norm256, uint8_t* t2, float* t2n, uint32_t* ext,
+                      uint32_t* dblock_off, uint8_t* codes, float* mul, float* err, uint32_t* ids) {
+    return guarded([&] {
+        SIEVE_REQUIRE(l, SIEVE_EINVAL, "null argument");
+        SIEVE_HIP(hipSetDevice(l->ix->device));
+        const size_t r = l->nrows, base = l->ix->nslots, slots = l->arena_blocks * BLOCK;
+        auto get = [&](void* dst, const void* src, size_t bytes) {
+            if (dst && bytes) SIEVE_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        };
+        get(rows, l->rows, r * DIM);
+        get(norm256, l->norm256, r * 4);
+        get(t2, l->t2, r * 64);
+        get(t2n, l->t2n, r * 4);
+        get(ext, l->ext, r * 4);
+        get(dblock_off, l->dblock_off, (l->ix->k + 1) * 4);
+        get(codes, l->ix->codes + 2 * base, slots * 32);
+        get(mul, l->ix->mul + base, slots * 4);
+        get(err, l->ix->err + base, slots * 4);
+        get(ids, l->ix->ids + base, slots * 4);
+    });
+}
+
+int sieve_live_search_batch(const sieve_live* l, sieve_stream* s, const float* q768, const float* q256, size_t nq,
+                            const sieve_params* p, const sieve_view* v, float* scores, uint32_t* ext_ids,
+                            uint32_t* counts, sieve_timing* t) {
+    return guarded([&] {
+        SIEVE_REQUIRE(l && s && q768 && q256 && p && scores && ext_ids && counts, SIEVE_EINVAL, "null argument");
+        check_params(p, true);
+        if (nq == 0) return;
+        const sieve_index* ix = l->ix;
+        SIEVE_HIP(hipSetDevice(ix->device));
+        ensure_events(s);
+        const float* q2 = dev_in(s, B_Q256, q256, nq * PDIM);
+        const float* q7 = dev_in(s, B_Q768, q768, nq * DIM);
+        ViewArgs view = make_view(s, v, nq);
+        SIEVE_REQUIRE(!view.deleted, SIEVE_EINVAL, "a live search uses the live tombstones; pass allowlists only");
+        auto os = dev_out(s, B_OS, scores, nq * p->k);
+        auto orr = dev_out(s, B_OR, ext_ids, nq * p->k);
+        auto on = dev_out(s, B_ON, counts, nq);
+        const LiveArgs la = l->args();
+        enqueue_search(ix, s, q2, q7, nq, p, view, os.dev, orr.dev, on.dev, &la);
+        out_done(s, os);
+        out_done(s, orr);
+        out_done(s, on);
+        finish_search(s, nq, t);
+    });
+}
+
+}  // extern "C"

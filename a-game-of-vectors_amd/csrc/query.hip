@@ -40,23 +40,7 @@ __global__ __launch_bounds__(64) void k_query_prep(const float* __restrict__ q25
     if (lane == 0) out.qnorm[qi] = sqrtf(nn);
     __syncthreads();
 
-    // Rotation::apply: three rounds of (sign flip, unnormalised FWHT), then * d^-1.5.
-    for (int r = 0; r < 3; r++) {
-        for (int i = lane; i < PDIM; i += WAVE) v[i] = v[i] * signs[r * PDIM + i];
-        __syncthreads();
-        for (int h = 1; h < PDIM; h <<= 1) {
-            for (int j = lane; j < PDIM / 2; j += WAVE) {
-                int blk = j / h, i = j - blk * h;
-                int a = blk * 2 * h + i, b = a + h;
-                float x = v[a], y = v[b];
-                v[a] = x + y;
-                v[b] = x - y;
-            }
-            __syncthreads();
-        }
-    }
-    for (int i = lane; i < PDIM; i += WAVE) v[i] = v[i] * rot_norm;
-    __syncthreads();
+    rotate_lds<PDIM>(v, signs, rot_norm, lane);
 
     // Lut::new: lane g builds group g's 16 sums (each a left-to-right sum from -0.0).
     const int g = lane;

@@ -57,9 +57,17 @@ size_t rerank_lds_bytes(int cand_stride, bool tier2_tables) { return rerank_layo
 
 // score[i] = dot(wq, rerank[row_of(i)][0, dims)) as f32 * inv (dims = 768 or 256), for the
 // n entries rows[0..n).
+// The int8 row of `row`: a corpus row, or (live) a delta row.
+__device__ __forceinline__ const int8_t* row_ptr(const RerankArgs& a, uint32_t row) {
+    return row < a.n_base ? a.rerank + (size_t)row * DIM : a.drows + (size_t)(row - a.n_base) * DIM;
+}
+__device__ __forceinline__ float row_norm256(const RerankArgs& a, uint32_t row) {
+    return row < a.n_base ? a.norm256[row] : a.dnorm256[row - a.n_base];
+}
+
 template <int DIMS>
-__device__ void int8_scores(const int8_t* __restrict__ rerank, const int8_t* __restrict__ wq, const uint32_t* rows,
-                            int n, float inv, float* score) {
+__device__ void int8_scores(const RerankArgs& a, const int8_t* __restrict__ wq, const uint32_t* rows, int n,
+                            float inv, float* score) {
     constexpr int CH = DIMS / 256;  // 16-byte chunks per lane
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6, sub = lane & 15;
     int wv[CH][4];
@@ -75,7 +83,7 @@ __device__ void int8_scores(const int8_t* __restrict__ rerank, const int8_t* __r
         const int i = base + (lane >> 4);
         int acc = 0;
         if (i < n) {
-            const int8_t* p = rerank + (size_t)rows[i] * DIM;
+            const int8_t* p = row_ptr(a, rows[i]);
 #pragma unroll
             for (int j = 0; j < CH; j++) {
                 uint4 x = *reinterpret_cast<const uint4*>(p + sub * 16 + 256 * j);
@@ -182,13 +190,13 @@ __global__ __launch_bounds__(RR_THREADS) void k_rerank(RerankArgs a) {
             __syncthreads();
             break;
         case SIEVE_RERANK_FULL:
-            int8_scores<DIM>(a.rerank, wq768, row, n, inv.x, score);
+            int8_scores<DIM>(a, wq768, row, n, inv.x, score);
             __syncthreads();
             break;
         case SIEVE_RERANK_PREFIX:
-            int8_scores<PDIM>(a.rerank, wq256, row, n, inv.y, score);
+            int8_scores<PDIM>(a, wq256, row, n, inv.y, score);
             __syncthreads();
-            for (int i = t; i < n; i += RR_THREADS) score[i] = score[i] / a.norm256[row[i]];
+            for (int i = t; i < n; i += RR_THREADS) score[i] = score[i] / row_norm256(a, row[i]);
             __syncthreads();
             break;
         default: {
@@ -214,7 +222,10 @@ __global__ __launch_bounds__(RR_THREADS) void k_rerank(RerankArgs a) {
                 }
                 __syncthreads();
                 for (int i = t; i < n; i += RR_THREADS) {
-                    const uint4* cp = reinterpret_cast<const uint4*>(a.t2_codes + (size_t)row[i] * 64);
+                    const uint32_t ri = row[i];
+                    const bool dr = ri >= a.n_base;
+                    const uint4* cp = reinterpret_cast<const uint4*>(dr ? a.dt2 + (size_t)(ri - a.n_base) * 64
+                                                                        : a.t2_codes + (size_t)ri * 64);
                     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
                     for (int c4 = 0; c4 < 4; c4++) {
@@ -227,16 +238,16 @@ __global__ __launch_bounds__(RR_THREADS) void k_rerank(RerankArgs a) {
                             acc[g & 3] = acc[g & 3] + t2[g * 256 + byte];
                         }
                     }
-                    score[i] = (acc[0] + acc[1] + acc[2] + acc[3]) / a.t2_norm[row[i]];
+                    score[i] = (acc[0] + acc[1] + acc[2] + acc[3]) / (dr ? a.dt2n[ri - a.n_base] : a.t2_norm[ri]);
                 }
             } else {
-                int8_scores<PDIM>(a.rerank, wq256, row, n, inv.y, score);
+                int8_scores<PDIM>(a, wq256, row, n, inv.y, score);
                 __syncthreads();
-                for (int i = t; i < n; i += RR_THREADS) score[i] = score[i] / a.norm256[row[i]];
+                for (int i = t; i < n; i += RR_THREADS) score[i] = score[i] / row_norm256(a, row[i]);
             }
             __syncthreads();
             m = best_rows(score, row, n, a.r, keys, list);
-            int8_scores<DIM>(a.rerank, wq768, list, m, inv.x, score);
+            int8_scores<DIM>(a, wq768, list, m, inv.x, score);
             __syncthreads();
             fin_rows = list;
             break;
@@ -251,7 +262,8 @@ __global__ __launch_bounds__(RR_THREADS) void k_rerank(RerankArgs a) {
     const int kk = min(a.k, m);
     for (int i = t; i < kk; i += RR_THREADS) {
         a.out_s[qi * a.k + i] = key2f(keys[i].x);
-        a.out_r[qi * a.k + i] = ~keys[i].y;
+        const uint32_t r = ~keys[i].y;
+        a.out_r[qi * a.k + i] = (a.dext && r >= a.n_base) ? a.dext[r - a.n_base] : r;
     }
     if (t == 0) a.out_n[qi] = (uint32_t)kk;
 }

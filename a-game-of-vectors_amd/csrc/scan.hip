@@ -126,7 +126,7 @@ void k_scan_t(ScanArgs a) {
     // ds_read_u8 with the table offset in its immediate field.
     __shared__ __align__(16) uint8_t tab[G * 16];
     extern __shared__ __align__(16) uint8_t lds[];
-    const ScanLds L = scan_layout(a.cand, a.probe_stride, SLOTS_PER_THREAD);
+    const ScanLds L = scan_layout(a.cand, scan_segments(a), SLOTS_PER_THREAD);
     uint32_t* pref = reinterpret_cast<uint32_t*>(lds + L.off_pref);
     uint32_t* pstart = reinterpret_cast<uint32_t*>(lds + L.off_start);
     float* pcent = reinterpret_cast<float*>(lds + L.off_cent);
@@ -138,13 +138,23 @@ void k_scan_t(ScanArgs a) {
     const size_t qi = blockIdx.x / a.chunk_max;
     const uint32_t ci = blockIdx.x % a.chunk_max;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int np = (int)a.probe_n[qi];
+    // Block segments in probe order: each probed cluster's base list, then (live) its delta
+    // list (src/ivf.rs:682-684).
+    const bool live = a.dblock_off != nullptr;
+    const int np = (live ? 2 : 1) * (int)a.probe_n[qi];
     for (int j = t; j < np; j += SCAN_THREADS) {
-        uint32_t c = a.probe_c[qi * a.probe_stride + j];
-        uint32_t b0 = a.block_off[c];
-        pstart[j] = b0;
-        pref[j] = a.block_off[c + 1] - b0;
-        pcent[j] = a.probe_s[qi * a.probe_stride + j];
+        const int pj = live ? j >> 1 : j;
+        const uint32_t c = a.probe_c[qi * a.probe_stride + pj];
+        if (!live || (j & 1) == 0) {
+            const uint32_t b0 = a.block_off[c];
+            pstart[j] = b0;
+            pref[j] = a.block_off[c + 1] - b0;
+        } else {
+            const uint32_t d0 = a.dblock_off[c];
+            pstart[j] = a.dbase + d0;
+            pref[j] = a.dblock_off[c + 1] - d0;
+        }
+        pcent[j] = a.probe_s[qi * a.probe_stride + pj];
     }
     if (t < 32) misc[t] = 0;
     __syncthreads();
@@ -453,7 +463,7 @@ void k_scan_t(ScanArgs a) {
 
 template <int G, bool L2, int S, bool PF, bool IL, int WPE>
 static void launch_variant(const ScanArgs& a, size_t nq, hipStream_t st) {
-    const size_t lds = scan_layout(a.cand, a.probe_stride, S).total;
+    const size_t lds = scan_layout(a.cand, scan_segments(a), S).total;
     if (lds > 64 * 1024)
         (void)hipFuncSetAttribute((const void*)k_scan_t<G, L2, S, PF, IL, WPE>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -81,6 +81,15 @@ class L2Desc(ctypes.Structure):
                 ("base", ctypes.c_void_p), ("borrow_base", ctypes.c_uint32)]
 
 
+class LiveOpts(ctypes.Structure):
+    _fields_ = [("max_rows", ctypes.c_uint64), ("soar_lambda", ctypes.c_float)]
+
+
+class LiveInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("rows", "slots", "arena_blocks", "deleted", "max_rows",
+                                               "device_bytes")]
+
+
 class ServerOpts(ctypes.Structure):
     _fields_ = [("max_batch", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64)]
 
@@ -155,6 +164,14 @@ def lib():
         "sieve_l2_index_free": (None, [vp]),
         "sieve_l2_search_batch": (i32, [vp, vp, vp, sz, ctypes.POINTER(L2Params), vp, vp, vp, vp, vp, vp, vp, vp,
                                         T]),
+        "sieve_live_open": (i32, [vp, ctypes.POINTER(LiveOpts), ctypes.POINTER(vp)]),
+        "sieve_live_close": (None, [vp]),
+        "sieve_live_insert": (i32, [vp, vp, vp, sz]),
+        "sieve_live_delete": (i32, [vp, vp, sz]),
+        "sieve_live_current": (u32, [vp, u32]),
+        "sieve_live_get_info": (i32, [vp, ctypes.POINTER(LiveInfo)]),
+        "sieve_live_export": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "sieve_live_search_batch": (i32, [vp, vp, vp, vp, sz, P, V, vp, vp, vp, T]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

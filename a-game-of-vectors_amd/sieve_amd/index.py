@@ -207,6 +207,78 @@ class Index:
         return s, r, n
 
 
+class Live:
+    """Live updates over an Index (src/live.rs): insert / delete / upsert by external id, and
+    searches over the base index, the delta lists and the tombstones."""
+
+    def __init__(self, index, max_rows, soar_lambda=1.0):
+        self._ix = index
+        o = abi.LiveOpts(max_rows=max_rows, soar_lambda=float("nan") if soar_lambda is None else soar_lambda)
+        h = ctypes.c_void_p()
+        abi.check(abi.lib().sieve_live_open(index._h, ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            abi.lib().sieve_live_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def insert(self, ext, vecs):
+        ext = self._ix._in(ext, np.uint32)
+        vecs = self._ix._in(vecs, np.float32)
+        abi.check(abi.lib().sieve_live_insert(self._h, abi.ptr(ext), abi.ptr(vecs), _numel(ext)))
+
+    def delete(self, ext):
+        ext = self._ix._in(ext, np.uint32)
+        abi.check(abi.lib().sieve_live_delete(self._h, abi.ptr(ext), _numel(ext)))
+
+    def current(self, ext):
+        r = abi.lib().sieve_live_current(self._h, int(ext))
+        return None if r == 0xFFFFFFFF else r
+
+    def info(self):
+        st = abi.LiveInfo()
+        abi.check(abi.lib().sieve_live_get_info(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in st._fields_}
+
+    def export(self):
+        """The delta segment on the host: rows, norm256, t2, t2n, ext, dblock_off, codes
+        (slots x 32), mul, err, ids."""
+        inf = self.info()
+        r, slots, k = inf["rows"], inf["arena_blocks"] * 32, self._ix.k
+        out = dict(rows=np.empty((r, 768), np.int8), norm256=np.empty(r, np.float32), t2=np.empty((r, 64), np.uint8),
+                   t2n=np.empty(r, np.float32), ext=np.empty(r, np.uint32), dblock_off=np.empty(k + 1, np.uint32),
+                   codes=np.empty((slots, 32), np.uint8), mul=np.empty(slots, np.float32),
+                   err=np.empty(slots, np.float32), ids=np.empty(slots, np.uint32))
+        abi.check(abi.lib().sieve_live_export(self._h, *[abi.ptr(out[f]) for f in
+                                                          ("rows", "norm256", "t2", "t2n", "ext", "dblock_off",
+                                                           "codes", "mul", "err", "ids")]))
+        return out
+
+    def search(self, q768, q256, p: Params, view=None, timing=None, stream=None):
+        """Live::search for a batch: (scores nq x k, external ids, counts)."""
+        ix = self._ix
+        q768, q256 = ix._in(q768, np.float32), ix._in(q256, np.float32)
+        nq = q256.shape[0]
+        s = ix._out(q256, (nq, p.k), np.float32)
+        r = ix._out(q256, (nq, p.k), np.uint32)
+        n = ix._out(q256, (nq,), np.uint32)
+        t = timing if timing is not None else abi.Timing()
+        st = stream or ix.stream()
+        v = view.c if view is not None else None
+        abi.check(abi.lib().sieve_live_search_batch(self._h, st._h, abi.ptr(q768), abi.ptr(q256), nq,
+                                                    ctypes.byref(abi.make_params(p)),
+                                                    ctypes.byref(v) if v is not None else None, abi.ptr(s),
+                                                    abi.ptr(r), abi.ptr(n), ctypes.byref(t)))
+        return s, r, n
+
+
 class L2Index:
     """The L2 index over 100-d int8 vectors (bigann::Index, src/bigann.rs:604-747)."""
 

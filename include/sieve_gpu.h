@@ -352,6 +352,60 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
                           uint32_t* probes, uint32_t* probe_counts, uint32_t* cand_rows,
                           float* cand_est, uint32_t* cand_counts, sieve_timing* t);
 
+/* ------------------------------------------------------------ live updates --- */
+/* Live (src/live.rs:146-447): inserts, deletes and upserts over a built index without a
+ * rebuild. Inserted rows are encoded on the device exactly as the build encodes a row and
+ * appended to per-cluster delta lists that a live search scans right after each probed
+ * cluster's base list; deletes are tombstones. Row ids: corpus rows keep theirs, delta rows
+ * count up from the corpus size; live searches report external ids. The write-ahead log and
+ * compaction (src/live.rs:212-263, :403-447) are not part of this library: the caller owns
+ * durability. Like the reference's &mut Live, inserts and deletes must not overlap searches of
+ * the same index, and an index has at most one live state. */
+typedef struct sieve_live sieve_live;
+
+typedef struct sieve_live_opts {
+    uint64_t max_rows;   /* delta rows the device reserves room for */
+    float soar_lambda;   /* SOAR spill weight of inserted rows (the build's soar_lambda);
+                          * NaN: one slot per row, as an index built without SOAR */
+} sieve_live_opts;
+
+typedef struct sieve_live_info {
+    uint64_t rows;          /* delta rows inserted */
+    uint64_t slots;         /* delta slots (rows x 2 with SOAR) */
+    uint64_t arena_blocks;  /* 32-slot blocks the delta lists occupy */
+    uint64_t deleted;       /* tombstones set */
+    uint64_t max_rows;
+    uint64_t device_bytes;  /* held by the live state (the index's slot arrays grow as well) */
+} sieve_live_info;
+
+/* Live::open (src/live.rs:175-210) without a log: the rotated centroids, room for max_rows
+ * delta rows and their slots (the index's slot arrays are re-allocated with room for them). */
+int sieve_live_open(sieve_index* ix, const sieve_live_opts* o, sieve_live** out);
+void sieve_live_close(sieve_live* l);
+/* Live::insert (src/live.rs:286-306): inserts or replaces ext[i] with the unit 768-d vector
+ * vecs[i] (n x 768, host or device memory). A live external id's old row is tombstoned. */
+int sieve_live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, size_t n);
+/* Live::delete (src/live.rs:308-326): tombstones the rows of external ids; unknown ids are
+ * ignored. */
+int sieve_live_delete(sieve_live* l, const uint32_t* ext, size_t n);
+/* Live::current (src/live.rs:270-277): the row id an external id resolves to, or 0xFFFFFFFF. */
+uint32_t sieve_live_current(const sieve_live* l, uint32_t ext);
+int sieve_live_get_info(const sieve_live* l, sieve_live_info* out);
+/* Copies the delta segment to host arrays (NULL to skip): the delta rows (rows x 768 int8),
+ * prefix norms, tier-2 codes (rows x 64) and norms, external ids; the delta lists' block
+ * offsets per cluster (k + 1), and their slots (arena_blocks x 32): raw 32-byte codes (bit i =
+ * dimension i), mul, err and row ids (0xFFFFFFFF for padding). No counterpart in the
+ * reference beyond Live::delta (src/live.rs:387-389): it is the inspection hook for tests and
+ * for a caller's own compaction. */
+int sieve_live_export(const sieve_live* l, int8_t* rows, float* norm256, uint8_t* t2, float* t2n, uint32_t* ext,
+                      uint32_t* dblock_off, uint8_t* codes, float* mul, float* err, uint32_t* ids);
+/* Live::search (src/live.rs:381-385) for nq queries: sieve_search_batch over the base index,
+ * the delta lists and the tombstones, with results as external ids. `view` may carry
+ * allowlists (filters); its tombstones must be NULL (the live state's are used). */
+int sieve_live_search_batch(const sieve_live* l, sieve_stream* s, const float* q768, const float* q256, size_t nq,
+                            const sieve_params* p, const sieve_view* view, float* scores, uint32_t* ext_ids,
+                            uint32_t* counts, sieve_timing* t);
+
 #ifdef __cplusplus
 }
 #endif
