@@ -49,7 +49,7 @@ def live_pair(small, oracle_index):
 def test_live_state_matches_oracle(small, live_pair):
     ix, live, olive, _ = live_pair
     info = live.info()
-    assert info["rows"] == olive.rows() == 678
+    assert info["rows"] == olive.rows() == 428
     d = live.export()
     for j in range(info["rows"]):
         row, norm, t2, t2n, e = olive.row(j)
@@ -75,7 +75,7 @@ def test_live_state_matches_oracle(small, live_pair):
         assert live.current(e) == olive.current(e)
 
 
-@pytest.mark.parametrize("cfg", ["12:400:full", "48:3000:full", "8:300:prefix", "12:500:tiered60:b10x1.0",
+@pytest.mark.parametrize("cfg", ["12:400:full", "48:2000:full", "8:300:prefix", "12:500:tiered60:b10x1.0",
                                  "16:400:ssd50", "12:400:none:p0.1"])
 def test_live_search_matches_oracle(small, live_pair, cfg):
     ix, live, olive, _ = live_pair
