@@ -1,0 +1,468 @@
+// Metadata filters (src/filter.rs): a filter is a bitset of allowed rows; the scan skips
+// disallowed slots at admission (sieve_view allowlists). The planner picks how to search with
+// one (Plan::resolve, src/filter.rs:60-69):
+//   * Post: the unfiltered probes, disallowed rows skipped at admission;
+//   * Expand(target): probe clusters in centroid order, skipping clusters with no allowed
+//     slot, until `target` allowed slots are covered (src/filter.rs:80-95);
+//   * Brute: exact int8 scores of every allowed row, no index (src/filter.rs:97-107).
+// Filter::new's per-cluster allowed counts and the allowed row list are built on the device.
+#include <hipcub/hipcub.hpp>
+
+#include "host.hpp"
+#include "select.hpp"
+
+using namespace sieve;
+
+namespace {
+
+constexpr uint64_t BRUTE_MAX = 250000;       // src/filter.rs:54
+constexpr double POST_MIN = 0.3;             // src/filter.rs:56
+constexpr uint64_t EXPAND_TARGET = 200000;   // src/filter.rs:58
+constexpr int EXPAND_MAX_K = 16384;          // centroids the expand cut sorts in LDS
+constexpr int FT = 256;                      // threads of the filter kernels
+
+// Allowed slots of cluster c (Filter::new, src/filter.rs:30-36).
+__global__ __launch_bounds__(FT) void k_filter_counts(const uint32_t* __restrict__ block_off,
+                                                      const uint32_t* __restrict__ ids,
+                                                      const uint64_t* __restrict__ allow, uint64_t words,
+                                                      uint32_t* __restrict__ per_cluster) {
+    __shared__ uint32_t sum;
+    const uint32_t c = blockIdx.x;
+    if (threadIdx.x == 0) sum = 0;
+    __syncthreads();
+    uint32_t cnt = 0;
+    for (size_t s = (size_t)block_off[c] * BLOCK + threadIdx.x; s < (size_t)block_off[c + 1] * BLOCK; s += FT) {
+        const uint32_t id = ids[s];
+        if (id != PAD && bit_set(allow, words, id)) cnt++;
+    }
+    cnt = wave_sum(cnt);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&sum, cnt);
+    __syncthreads();
+    if (threadIdx.x == 0) per_cluster[c] = sum;
+}
+
+__global__ void k_word_popc(const uint64_t* __restrict__ allow, uint64_t words, uint32_t* __restrict__ cnt) {
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < words) cnt[w] = (uint32_t)__popcll(allow[w]);
+}
+
+// The allowed rows, ascending.
+__global__ void k_word_rows(const uint64_t* __restrict__ allow, uint64_t words, const uint32_t* __restrict__ off,
+                            uint32_t* __restrict__ rows) {
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words) return;
+    uint64_t b = allow[w];
+    uint32_t o = off[w];
+    while (b) {
+        const int i = __ffsll((long long)b) - 1;
+        rows[o++] = (uint32_t)(w * 64 + i);
+        b &= b - 1;
+    }
+}
+
+// Brute: score[q][i] = dot(wq768[q], rerank[rows[i]]) as f32 * inv (src/filter.rs:99-104), 16
+// lanes per row on v_dot4_i32_i8, 1024 rows per workgroup.
+__global__ __launch_bounds__(FT) void k_brute_scores(const int8_t* __restrict__ rerank,
+                                                     const uint32_t* __restrict__ rows, uint64_t R,
+                                                     const int8_t* __restrict__ wq768, const float2* __restrict__ inv,
+                                                     float* __restrict__ scores) {
+    const size_t q = blockIdx.y;
+    const int t = threadIdx.x, lane = t & 63, sub = lane & 15;
+    int wv[3][4];
+    const int8_t* wq = wq768 + q * DIM;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const uint4 x = *reinterpret_cast<const uint4*>(wq + sub * 16 + 256 * j);
+        wv[j][0] = (int)x.x;
+        wv[j][1] = (int)x.y;
+        wv[j][2] = (int)x.z;
+        wv[j][3] = (int)x.w;
+    }
+    const float iv = inv[q].x;
+    const size_t r0 = (size_t)blockIdx.x * 1024;
+    const size_t r1 = min(r0 + 1024, (size_t)R);
+    // The 16 lanes of a row share `base`, so each width-16 shuffle below runs on whole groups.
+    for (size_t base = r0 + (t >> 4); base < r1; base += FT / 16) {
+        const int8_t* p = rerank + (size_t)rows[base] * DIM;
+        int acc = 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const uint4 x = *reinterpret_cast<const uint4*>(p + sub * 16 + 256 * j);
+            acc = __builtin_amdgcn_sdot4((int)x.x, wv[j][0], acc, false);
+            acc = __builtin_amdgcn_sdot4((int)x.y, wv[j][1], acc, false);
+            acc = __builtin_amdgcn_sdot4((int)x.z, wv[j][2], acc, false);
+            acc = __builtin_amdgcn_sdot4((int)x.w, wv[j][3], acc, false);
+        }
+#pragma unroll
+        for (int d = 8; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 16);
+        if (sub == 0) scores[q * R + base] = (float)acc * iv;
+    }
+}
+
+// The best k of n entries (key, ~row) kept in buf, threshold raised to the k-th.
+__device__ void topk_compact(uint2* buf, uint2* tmp, int n, int k, uint32_t* hist, uint32_t* misc) {
+    const int t = threadIdx.x;
+    const uint2 kth = block_kth_largest(buf, n, k, hist, misc + 8);
+    if (t == 0) misc[3] = 0;
+    __syncthreads();
+    for (int i = t; i < n; i += FT) {
+        const uint2 e = buf[i];
+        if (ge2(e, kth)) {
+            const uint32_t pos = atomicAdd(&misc[3], 1u);
+            if (pos < (uint32_t)k) tmp[pos] = e;
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < k; i += FT) buf[i] = tmp[i];
+    if (t == 0) {
+        misc[0] = (uint32_t)k;
+        misc[1] = 1;
+        misc[2] = kth.x;
+        misc[4 + 8] = kth.y;
+    }
+    __syncthreads();
+}
+
+// TopK over a query's n scores (score desc, row asc; src/topk.rs:4-62), streamed through LDS.
+__global__ __launch_bounds__(FT) void k_topk_scores(const float* __restrict__ scores,
+                                                    const uint32_t* __restrict__ rows, uint64_t R, int k,
+                                                    float* __restrict__ out_s, uint32_t* __restrict__ out_r,
+                                                    uint32_t* __restrict__ out_n, int out_stride) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const int cap = 2 * k + 4 * FT;
+    uint2* buf = reinterpret_cast<uint2*>(lds);
+    uint2* tmp = buf + cap;
+    uint32_t* hist = reinterpret_cast<uint32_t*>(tmp + k);
+    uint32_t* misc = hist + 256;  // 0 count, 1 has thr, 2 thr key, 3 scratch, 8..10 select, 12 thr lo, 13 stamp
+    const size_t q = blockIdx.x;
+    const int t = threadIdx.x, lane = t & 63;
+    if (t < 16) misc[t] = 0;
+    if (t == 0) misc[13] = 0xFFFFFFFFu;
+    __syncthreads();
+    const float* sc = scores + q * R;
+    uint32_t round = 0;
+    for (size_t i0 = 0; i0 < R; i0 += 4 * FT, round++) {
+        const bool has = misc[1] != 0;
+        const uint32_t thr = misc[2], thr_lo = misc[12];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const size_t i = i0 + u * FT + t;
+            uint2 e = make_uint2(0u, 0u);
+            bool pass = false;
+            if (i < R) {
+                e = make_uint2(f2key(sc[i] + 0.0f), ~rows[i]);
+                pass = !has || e.x > thr || (e.x == thr && e.y > thr_lo);
+            }
+            const uint64_t bal = __ballot(pass);
+            if (bal) {
+                const uint32_t nw = (uint32_t)__popcll(bal);
+                const uint32_t before = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                uint32_t base = 0;
+                if (lane == 0) {
+                    base = atomicAdd(&misc[0], nw);
+                    if (base + nw >= 2u * (uint32_t)k) misc[13] = round;
+                }
+                base = __shfl(base, 0, 64);
+                if (pass) buf[base + before] = e;
+            }
+        }
+        __syncthreads();
+        if (misc[13] == round) topk_compact(buf, tmp, (int)misc[0], k, hist, misc);
+    }
+    if ((int)misc[0] > k) topk_compact(buf, tmp, (int)misc[0], k, hist, misc);
+    const int n = (int)misc[0];
+    const int n2 = pow2_at_least(max(n, 1));
+    for (int i = n + t; i < n2; i += FT) buf[i] = make_uint2(0u, 0u);
+    __syncthreads();
+    block_sort_pairs(buf, nullptr, n2, false);
+    for (int i = t; i < n; i += FT) {
+        out_s[q * out_stride + i] = key2f(buf[i].x);
+        out_r[q * out_stride + i] = ~buf[i].y;
+    }
+    if (t == 0) out_n[q] = (uint32_t)n;
+}
+
+// Exact route scores of every centroid (src/blas.rs:47-53: products left to right from -0.0).
+__global__ __launch_bounds__(FT) void k_exact_route(const float* __restrict__ q256, const float* __restrict__ cents,
+                                                    int K, float* __restrict__ out) {
+    __shared__ float qs[PDIM];
+    const size_t q = blockIdx.y;
+    for (int i = threadIdx.x; i < PDIM; i += FT) qs[i] = q256[q * PDIM + i];
+    __syncthreads();
+    const int c = blockIdx.x * FT + threadIdx.x;
+    if (c >= K) return;
+    const float4* row = reinterpret_cast<const float4*>(cents + (size_t)c * PDIM);
+    float acc = -0.0f;
+    for (int j = 0; j < PDIM / 4; j++) {
+        const float4 x = row[j];
+        acc = acc + qs[4 * j] * x.x;
+        acc = acc + qs[4 * j + 1] * x.y;
+        acc = acc + qs[4 * j + 2] * x.z;
+        acc = acc + qs[4 * j + 3] * x.w;
+    }
+    out[q * K + c] = acc;
+}
+
+// Expand's probe list (src/filter.rs:80-93): all centroids by (score desc, id asc), clusters
+// with no allowed slot skipped, taken while the allowed slots covered so far are below target.
+__global__ __launch_bounds__(FT) void k_expand_cut(const float* __restrict__ scores, int K,
+                                                   const uint32_t* __restrict__ per_cluster, uint64_t target,
+                                                   int max_probe, float* __restrict__ ps, uint32_t* __restrict__ pc,
+                                                   uint32_t* __restrict__ pn, uint32_t* __restrict__ overflow) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    uint2* e = reinterpret_cast<uint2*>(lds);
+    uint32_t* scratch = reinterpret_cast<uint32_t*>(lds) + 2 * EXPAND_MAX_K;  // 16 words
+    const size_t q = blockIdx.x;
+    const int t = threadIdx.x;
+    int n2 = 1;
+    while (n2 < K) n2 <<= 1;
+    for (int i = t; i < n2; i += FT)
+        e[i] = i < K ? make_uint2(f2key(scores[q * K + i] + 0.0f), ~(uint32_t)i) : make_uint2(0u, 0u);
+    __syncthreads();
+    block_sort_pairs(e, nullptr, n2, false);
+    // Thread t owns sorted entries [t * per, (t + 1) * per): their allowed slots, then a block
+    // scan gives each entry the slots covered before it.
+    const int per = (K + FT - 1) / FT;
+    uint64_t mine = 0;
+    for (int i = t * per; i < min(K, (t + 1) * per); i++) mine += per_cluster[~e[i].y];
+    // 64-bit block prefix (the u32 scan on high and low halves would carry wrongly): a wave scan
+    // of u64 via shuffles, then across the 4 waves.
+    uint64_t incl = mine;
+    const int lane = t & 63, wave = t >> 6;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+    }
+    uint64_t* wsum = reinterpret_cast<uint64_t*>(scratch);
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint64_t before = incl - mine;
+    for (int w = 0; w < wave; w++) before += wsum[w];
+    // Kept entries: allowed slots > 0 and covered-before < target; they are a prefix of the
+    // allowed clusters in sorted order, so each thread counts its kept entries and a second scan
+    // places them.
+    uint32_t kept = 0;
+    {
+        uint64_t cov = before;
+        for (int i = t * per; i < min(K, (t + 1) * per); i++) {
+            const uint32_t w = per_cluster[~e[i].y];
+            if (w == 0) continue;
+            if (cov < target) kept++;
+            cov += w;
+        }
+    }
+    __syncthreads();
+    uint32_t tot;
+    uint32_t off = block_excl_scan(kept, scratch + 8, &tot);
+    if ((int)tot > max_probe) {
+        if (t == 0) {
+            atomicOr(overflow, 1u);
+            pn[q] = 0;
+        }
+        return;
+    }
+    uint64_t cov = before;
+    for (int i = t * per; i < min(K, (t + 1) * per); i++) {
+        const uint32_t c = ~e[i].y;
+        const uint32_t w = per_cluster[c];
+        if (w == 0) continue;
+        if (cov < target) {
+            ps[q * max_probe + off] = key2f(e[i].x);
+            pc[q * max_probe + off] = c;
+            off++;
+        }
+        cov += w;
+    }
+    if (t == 0) pn[q] = tot;
+}
+
+inline unsigned grid(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+struct sieve_filter {
+    const sieve_index* ix = nullptr;
+    uint64_t* allow = nullptr;
+    uint64_t words = 0;
+    uint32_t* rows = nullptr;
+    uint64_t nrows = 0;
+    uint32_t* per_cluster = nullptr;
+    size_t device_bytes = 0;
+    ~sieve_filter() {
+        void* ps[] = {allow, rows, per_cluster};
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+    }
+};
+
+extern "C" {
+
+int sieve_filter_create(const sieve_index* ix, const uint64_t* allow, uint64_t allow_words, sieve_filter** out) {
+    return guarded([&] {
+        SIEVE_REQUIRE(ix && allow && out, SIEVE_EINVAL, "null argument");
+        SIEVE_REQUIRE(allow_words >= 1, SIEVE_EINVAL, "empty allowlist");
+        SIEVE_HIP(hipSetDevice(ix->device));
+        std::unique_ptr<sieve_filter> f(new sieve_filter());
+        f->ix = ix;
+        f->words = allow_words;
+        size_t* acct = &f->device_bytes;
+        f->allow = dmalloc<uint64_t>(allow_words, acct);
+        SIEVE_HIP(hipMemcpy(f->allow, allow, allow_words * 8,
+                            on_device(allow) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+        f->per_cluster = dmalloc<uint32_t>(ix->k, acct);
+        hipStream_t st = nullptr;
+        hipLaunchKernelGGL(k_filter_counts, dim3(ix->k), dim3(FT), 0, st, ix->block_off, ix->ids, f->allow,
+                           allow_words, f->per_cluster);
+        uint32_t* cnt = dmalloc<uint32_t>(allow_words + 1, nullptr);
+        uint32_t* off = dmalloc<uint32_t>(allow_words + 1, nullptr);
+        SIEVE_HIP(hipMemset(cnt, 0, (allow_words + 1) * 4));
+        hipLaunchKernelGGL(k_word_popc, dim3(grid(allow_words, 256)), dim3(256), 0, st, f->allow, allow_words, cnt);
+        size_t tmp_bytes = 0;
+        SIEVE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt, off, (int)(allow_words + 1), st));
+        void* tmp = dmalloc<uint8_t>(tmp_bytes, nullptr);
+        SIEVE_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, off, (int)(allow_words + 1), st));
+        uint32_t total = 0;
+        SIEVE_HIP(hipMemcpy(&total, off + allow_words, 4, hipMemcpyDeviceToHost));
+        f->nrows = total;
+        f->rows = dmalloc<uint32_t>(std::max<uint64_t>(total, 1), acct);
+        hipLaunchKernelGGL(k_word_rows, dim3(grid(allow_words, 256)), dim3(256), 0, st, f->allow, allow_words, off,
+                           f->rows);
+        SIEVE_HIP(hipGetLastError());
+        SIEVE_HIP(hipDeviceSynchronize());
+        (void)hipFree(cnt);
+        (void)hipFree(off);
+        (void)hipFree(tmp);
+        *out = f.release();
+    });
+}
+
+void sieve_filter_free(sieve_filter* f) { delete f; }
+
+int sieve_filter_get_info(const sieve_filter* f, uint64_t* rows, uint32_t* per_cluster) {
+    return guarded([&] {
+        SIEVE_REQUIRE(f, SIEVE_EINVAL, "null filter");
+        if (rows) *rows = f->nrows;
+        if (per_cluster) SIEVE_HIP(hipMemcpy(per_cluster, f->per_cluster, f->ix->k * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+uint32_t sieve_plan_resolve(uint32_t plan, uint64_t allowed_rows, uint64_t n) {
+    if (plan != SIEVE_PLAN_AUTO) return plan;
+    if (allowed_rows <= BRUTE_MAX) return SIEVE_PLAN_BRUTE;
+    if ((double)allowed_rows >= POST_MIN * (double)n) return SIEVE_PLAN_POST;
+    return SIEVE_PLAN_EXPAND;
+}
+
+int sieve_filter_search_batch(const sieve_index* ix, const sieve_filter* f, uint32_t plan, uint64_t expand_target,
+                              sieve_stream* s, const float* q768, const float* q256, size_t nq, const sieve_params* p,
+                              float* scores, uint32_t* rows, uint32_t* counts, uint32_t* resolved, sieve_timing* t) {
+    return guarded([&] {
+        SIEVE_REQUIRE(ix && f && s && q768 && q256 && p && scores && rows && counts, SIEVE_EINVAL, "null argument");
+        SIEVE_REQUIRE(f->ix == ix, SIEVE_EINVAL, "the filter belongs to another index");
+        SIEVE_REQUIRE(plan <= SIEVE_PLAN_AUTO, SIEVE_EINVAL, "unknown plan");
+        check_params(p, true);
+        SIEVE_REQUIRE(nq <= 65535, SIEVE_ELIMIT, "a filtered batch holds at most 65535 queries");
+        SIEVE_HIP(hipSetDevice(ix->device));
+        const uint32_t rp = sieve_plan_resolve(plan, f->nrows, ix->n);
+        if (plan == SIEVE_PLAN_AUTO && rp == SIEVE_PLAN_EXPAND) expand_target = EXPAND_TARGET;
+        if (resolved) *resolved = rp;
+        if (nq == 0) return;
+        ensure_events(s);
+        const float* q2 = dev_in(s, B_Q256, q256, nq * PDIM);
+        const float* q7 = dev_in(s, B_Q768, q768, nq * DIM);
+        auto os = dev_out(s, B_OS, scores, nq * p->k);
+        auto orr = dev_out(s, B_OR, rows, nq * p->k);
+        auto on = dev_out(s, B_ON, counts, nq);
+        ViewArgs view{};
+        view.allow = f->allow;
+        view.allow_words = f->words;
+        if (rp == SIEVE_PLAN_POST) {
+            enqueue_search(ix, s, q2, q7, nq, p, view, os.dev, orr.dev, on.dev);
+            out_done(s, os);
+            out_done(s, orr);
+            out_done(s, on);
+            finish_search(s, nq, t);
+            return;
+        }
+        unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 4);
+        SIEVE_HIP(hipMemsetAsync(cnt, 0, 32, s->st));
+        SIEVE_HIP(hipEventRecord(s->ev[0], s->st));
+        Batch bt;
+        prep_queries(ix, s, q2, q7, nq, bt);
+        uint32_t* pn = grow<uint32_t>(s->b[B_PN], nq);
+        if (rp == SIEVE_PLAN_BRUTE) {
+            // All allowed rows scored for a sub-batch of queries at a time (<= 256 MB of scores).
+            const uint64_t R = f->nrows;
+            SIEVE_HIP(hipMemsetAsync(pn, 0, nq * 4, s->st));
+            SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
+            SIEVE_HIP(hipEventRecord(s->ev[2], s->st));
+            if (R == 0) {
+                SIEVE_HIP(hipMemsetAsync(on.dev, 0, nq * 4, s->st));
+            } else {
+                const size_t sub = std::max<size_t>(1, std::min<size_t>(nq, (size_t(64) << 20) / R));
+                float* sc = grow<float>(s->b[B_TMP1], sub * R);
+                const size_t lds = (size_t)(2 * p->k + 4 * FT) * 8 + (size_t)p->k * 8 + 256 * 4 + 16 * 4;
+                SIEVE_REQUIRE(lds <= 160 * 1024, SIEVE_ELIMIT, "k too large for the brute-force top-k");
+                if (lds > 64 * 1024)
+                    (void)hipFuncSetAttribute((const void*)k_topk_scores, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)lds);
+                for (size_t q0 = 0; q0 < nq; q0 += sub) {
+                    const size_t m = std::min(sub, nq - q0);
+                    hipLaunchKernelGGL(k_brute_scores, dim3(grid(R, 1024), (unsigned)m), dim3(FT), 0, s->st,
+                                       ix->rerank, f->rows, R, bt.prep.wq768 + q0 * DIM, bt.prep.inv + q0, sc);
+                    hipLaunchKernelGGL(k_topk_scores, dim3((unsigned)m), dim3(FT), lds, s->st, sc, f->rows, R,
+                                       (int)p->k, os.dev + q0 * p->k, orr.dev + q0 * p->k, on.dev + q0, (int)p->k);
+                }
+                SIEVE_HIP(hipGetLastError());
+            }
+            SIEVE_HIP(hipEventRecord(s->ev[3], s->st));
+            out_done(s, os);
+            out_done(s, orr);
+            out_done(s, on);
+            finish_search(s, nq, nullptr);
+            s->last.reranked = (unsigned long long)R * nq;  // every allowed row, per query
+            if (t) {
+                t->rerank += (uint64_t)s->last.rerank_ns;
+                t->reranked += s->last.reranked;
+            }
+            return;
+        } else {
+            // Expand: exact scores of every centroid, the weighted cut, then the filtered scan.
+            SIEVE_REQUIRE((int)ix->k <= EXPAND_MAX_K, SIEVE_ELIMIT,
+                          "expand plan sorts at most " + std::to_string(EXPAND_MAX_K) + " centroids");
+            const int maxp = (int)std::min<uint32_t>(ix->k, MAX_PROBE);
+            float* all = grow<float>(s->b[B_APPROX], nq * ix->k);
+            hipLaunchKernelGGL(k_exact_route, dim3(grid(ix->k, FT), (unsigned)nq), dim3(FT), 0, s->st, q2, ix->cents,
+                               (int)ix->k, all);
+            float* ps = grow<float>(s->b[B_PS], nq * maxp);
+            uint32_t* pc = grow<uint32_t>(s->b[B_PC], nq * maxp);
+            uint32_t* ovf = grow<uint32_t>(s->b[B_OVF], 1);
+            SIEVE_HIP(hipMemsetAsync(ovf, 0, 4, s->st));
+            const size_t lds = (size_t)EXPAND_MAX_K * 8 + 16 * 8;
+            (void)hipFuncSetAttribute((const void*)k_expand_cut, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL(k_expand_cut, dim3((unsigned)nq), dim3(FT), lds, s->st, all, (int)ix->k,
+                               f->per_cluster, expand_target, maxp, ps, pc, pn, ovf);
+            SIEVE_HIP(hipGetLastError());
+            uint32_t h = 0;
+            SIEVE_HIP(hipMemcpyAsync(&h, ovf, 4, hipMemcpyDeviceToHost, s->st));
+            SIEVE_HIP(hipStreamSynchronize(s->st));
+            SIEVE_REQUIRE(h == 0, SIEVE_ELIMIT,
+                          "expand plan needs more than " + std::to_string(maxp) + " probed clusters");
+            SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
+            float* ce = grow<float>(s->b[B_CE], nq * p->cand);
+            uint32_t* cr = grow<uint32_t>(s->b[B_CR], nq * p->cand);
+            float* cx = grow<float>(s->b[B_CX], nq * p->cand);
+            uint32_t* cn = grow<uint32_t>(s->b[B_CN], nq);
+            scan_batch(ix, s, bt, nq, ps, pc, pn, maxp, p, view, ce, cr, cx, cn, (int)p->cand, cnt);
+            SIEVE_HIP(hipEventRecord(s->ev[2], s->st));
+            rerank_batch(ix, s, bt, nq, ce, cr, cx, cn, (int)p->cand, p, os.dev, orr.dev, on.dev, cnt + 1);
+            SIEVE_HIP(hipEventRecord(s->ev[3], s->st));
+        }
+        out_done(s, os);
+        out_done(s, orr);
+        out_done(s, on);
+        finish_search(s, nq, t);
+    });
+}
+
+}  // extern "C"

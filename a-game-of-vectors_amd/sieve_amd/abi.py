@@ -107,6 +107,8 @@ class ServerStats(ctypes.Structure):
 _lib = None
 
 
+# filter::Plan tags (SIEVE_PLAN_*).
+PLANS = {"post": 0, "expand": 1, "brute": 2, "auto": 3}
 # Scan kernel variants the autotuner times (SIEVE_SCAN_VARIANTS).
 SCAN_VARIANTS = 10
 
@@ -164,6 +166,12 @@ def lib():
         "sieve_l2_index_free": (None, [vp]),
         "sieve_l2_search_batch": (i32, [vp, vp, vp, sz, ctypes.POINTER(L2Params), vp, vp, vp, vp, vp, vp, vp, vp,
                                         T]),
+        "sieve_filter_create": (i32, [vp, vp, ctypes.c_uint64, ctypes.POINTER(vp)]),
+        "sieve_filter_free": (None, [vp]),
+        "sieve_filter_get_info": (i32, [vp, ctypes.POINTER(ctypes.c_uint64), vp]),
+        "sieve_plan_resolve": (u32, [u32, ctypes.c_uint64, ctypes.c_uint64]),
+        "sieve_filter_search_batch": (i32, [vp, vp, u32, ctypes.c_uint64, vp, vp, vp, sz, P, vp, vp, vp,
+                                            ctypes.POINTER(u32), T]),
         "sieve_live_open": (i32, [vp, ctypes.POINTER(LiveOpts), ctypes.POINTER(vp)]),
         "sieve_live_close": (None, [vp]),
         "sieve_live_insert": (i32, [vp, vp, vp, sz]),

@@ -207,6 +207,54 @@ class Index:
         return s, r, n
 
 
+class Filter:
+    """filter::Filter over an Index (src/filter.rs:18-39) and filter::search with a Plan."""
+
+    def __init__(self, index, allow):
+        self._ix = index
+        allow = index._in(allow, np.uint64)
+        h = ctypes.c_void_p()
+        abi.check(abi.lib().sieve_filter_create(index._h, abi.ptr(allow), _numel(allow), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            abi.lib().sieve_filter_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def rows(self):
+        n = ctypes.c_uint64()
+        abi.check(abi.lib().sieve_filter_get_info(self._h, ctypes.byref(n), None))
+        return n.value
+
+    def per_cluster(self):
+        out = np.empty(self._ix.k, np.uint32)
+        abi.check(abi.lib().sieve_filter_get_info(self._h, None, abi.ptr(out)))
+        return out
+
+    def search(self, q768, q256, p: Params, plan="auto", target=200_000, timing=None, stream=None):
+        """filter::search for a batch sharing this filter: (scores, rows, counts, plan used)."""
+        ix = self._ix
+        q768, q256 = ix._in(q768, np.float32), ix._in(q256, np.float32)
+        nq = q256.shape[0]
+        s = ix._out(q256, (nq, p.k), np.float32)
+        r = ix._out(q256, (nq, p.k), np.uint32)
+        n = ix._out(q256, (nq,), np.uint32)
+        used = ctypes.c_uint32()
+        t = timing if timing is not None else abi.Timing()
+        st = stream or ix.stream()
+        abi.check(abi.lib().sieve_filter_search_batch(ix._h, self._h, abi.PLANS[plan], target, st._h, abi.ptr(q768),
+                                                      abi.ptr(q256), nq, ctypes.byref(abi.make_params(p)), abi.ptr(s),
+                                                      abi.ptr(r), abi.ptr(n), ctypes.byref(used), ctypes.byref(t)))
+        return s, r, n, used.value
+
+
 class Live:
     """Live updates over an Index (src/live.rs): insert / delete / upsert by external id, and
     searches over the base index, the delta lists and the tombstones."""

@@ -352,6 +352,36 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
                           uint32_t* probes, uint32_t* probe_counts, uint32_t* cand_rows,
                           float* cand_est, uint32_t* cand_counts, sieve_timing* t);
 
+/* ----------------------------------------------------------- metadata filters --- */
+/* filter::Filter (src/filter.rs:18-39): a bitset of allowed rows over the index's corpus, with
+ * the allowed slots of every cluster and the allowed rows, both built on the device. */
+typedef struct sieve_filter sieve_filter;
+
+/* filter::Plan (src/filter.rs:40-52). */
+#define SIEVE_PLAN_POST 0   /* the unfiltered probes, disallowed rows skipped at admission */
+#define SIEVE_PLAN_EXPAND 1 /* clusters in centroid order until expand_target allowed slots */
+#define SIEVE_PLAN_BRUTE 2  /* exact int8 scores of every allowed row, no index */
+#define SIEVE_PLAN_AUTO 3   /* Plan::resolve by the filter's size */
+
+/* Filter::new (src/filter.rs:27-38). allow: allow_words u64 words (host or device), bit r =
+ * row r allowed. */
+int sieve_filter_create(const sieve_index* ix, const uint64_t* allow, uint64_t allow_words, sieve_filter** out);
+void sieve_filter_free(sieve_filter* f);
+/* The number of allowed rows and (per_cluster, NULL to skip: k u32) the allowed slots per
+ * cluster. */
+int sieve_filter_get_info(const sieve_filter* f, uint64_t* rows, uint32_t* per_cluster);
+/* Plan::resolve (src/filter.rs:60-69): AUTO is BRUTE at <= 250,000 allowed rows, POST at >= 30%
+ * of the n corpus rows, EXPAND (200,000 slots) between; other plans are kept. */
+uint32_t sieve_plan_resolve(uint32_t plan, uint64_t allowed_rows, uint64_t n);
+/* filter::search (src/filter.rs:72-110) for nq (<= 65535) queries sharing one filter: the top
+ * p->k allowed rows as (score, row) best first. `expand_target` is EXPAND's slot target (AUTO
+ * uses 200,000); the plan used is written to *resolved (may be NULL). EXPAND sorts every
+ * centroid by its exact f32 score (at most 16384 centroids) and fails with SIEVE_ELIMIT if its
+ * cut needs more than 1024 clusters. */
+int sieve_filter_search_batch(const sieve_index* ix, const sieve_filter* f, uint32_t plan, uint64_t expand_target,
+                              sieve_stream* s, const float* q768, const float* q256, size_t nq, const sieve_params* p,
+                              float* scores, uint32_t* rows, uint32_t* counts, uint32_t* resolved, sieve_timing* t);
+
 /* ------------------------------------------------------------ live updates --- */
 /* Live (src/live.rs:146-447): inserts, deletes and upserts over a built index without a
  * rebuild. Inserted rows are encoded on the device exactly as the build encodes a row and

@@ -37,6 +37,9 @@ Parity status of each oracle function and what pins it (tests/test_oracle_*.py):
   rerank          (ivf.rs:731-815)   pinned: brute-force int64 dots + full sort per mode
   l2 search       (bigann.rs:652-747) pinned: brute-force L2 estimate/rerank by full sort
   assign_top/soar (ivf.rs:137-186)   pinned: float64 brute force (up to near-ties)
+  Filter/Plan     (filter.rs)        pinned: filter.rs:157-165 (auto plan); per-cluster
+                                      counts and row lists by numpy; each plan's result
+                                      against brute force over the allowed rows
   Live            (live.rs)          pinned: live.rs:493-520 (delta list layout, bitset);
                                       an inserted corpus vector encodes to the build's slots
                                       of that row (live.rs:2-6, bit for bit); upsert/delete
@@ -144,6 +147,12 @@ def lib():
             "or_l2_search": (sz, [vp, vp, u32, u32, u32, f32, vp, vp, vp, vp, vp, vp, vp]),
             "or_live_create": (vp, [vp, i32, f32]),
             "or_delta_list_fixture": (sz, [sz, vp, vp, vp, sz, vp, vp, vp]),
+            "or_filter_create": (vp, [vp, vp, sz]),
+            "or_filter_free": (None, [vp]),
+            "or_filter_rows": (sz, [vp]),
+            "or_filter_per_cluster": (None, [vp, vp]),
+            "or_plan_resolve": (u32, [u32, u64, u64]),
+            "or_filter_search": (u32, [vp, vp, u32, u64, vp, vp, ctypes.POINTER(_Params), vp, vp, vp]),
             "or_live_free": (None, [vp]),
             "or_live_insert": (None, [vp, vp, vp, sz]),
             "or_live_delete": (None, [vp, vp, sz]),
@@ -601,3 +610,51 @@ class Live:
         lib().or_live_search_batch(self._h, _p(q7), _p(q2), nq, ctypes.byref(make_params(p)), ctypes.byref(v),
                                    threads, _p(s), _p(r), _p(cnt))
         return s, r, cnt
+
+
+PLANS = {"post": 0, "expand": 1, "brute": 2, "auto": 3}
+
+
+def plan_resolve(plan, rows, n):
+    return lib().or_plan_resolve(PLANS[plan] if isinstance(plan, str) else plan, rows, n)
+
+
+class Filter:
+    """filter::Filter over an oracle Index (src/filter.rs:18-39)."""
+
+    def __init__(self, index, allow):
+        self._ix = index
+        self._allow = _c(allow, np.uint64)
+        self._h = lib().or_filter_create(index._h, _p(self._allow), self._allow.shape[0])
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().or_filter_free(self._h)
+        except Exception:
+            pass
+        self._h = None
+
+    def rows(self):
+        return lib().or_filter_rows(self._h)
+
+    def per_cluster(self):
+        out = np.empty(self._ix.k, np.uint32)
+        lib().or_filter_per_cluster(self._h, _p(out))
+        return out
+
+    def search(self, q768, q256, p, plan="auto", target=200_000):
+        """filter::search for each query: (scores nq x k, rows, counts, resolved plans)."""
+        q7, q2 = _c(q768, np.float32), _c(q256, np.float32)
+        nq = q7.shape[0]
+        s = np.zeros((nq, p.k), np.float32)
+        r = np.full((nq, p.k), PAD, np.uint32)
+        cnt = np.zeros(nq, np.uint32)
+        plans = np.zeros(nq, np.uint32)
+        pr = make_params(p)
+        c = ctypes.c_uint32()
+        for i in range(nq):
+            plans[i] = lib().or_filter_search(self._ix._h, self._h, PLANS[plan], target, _p(q7[i]), _p(q2[i]),
+                                              ctypes.byref(pr), _p(s[i]), _p(r[i]), ctypes.byref(c))
+            cnt[i] = c.value
+        return s, r, cnt, plans

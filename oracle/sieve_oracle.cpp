@@ -1336,3 +1336,99 @@ size_t or_delta_list_fixture(size_t n, const uint8_t* codes, const float* factor
     return nb;
 }
 }  // extern "C"
+
+// ---- metadata filters, src/filter.rs --------------------------------------------------
+struct OFilter {  // Filter, src/filter.rs:18-39
+    std::vector<uint64_t> allow;
+    std::vector<uint32_t> rows;         // allowed rows, ascending
+    std::vector<uint32_t> per_cluster;  // allowed slots per cluster
+};
+
+extern "C" {
+
+constexpr uint64_t BRUTE_MAX = 250000;  // src/filter.rs:54
+constexpr double POST_MIN = 0.3;        // src/filter.rs:56
+constexpr uint64_t EXPAND_TARGET = 200000;  // src/filter.rs:58
+
+// Filter::new, src/filter.rs:27-38.
+void* or_filter_create(const void* index, const uint64_t* allow, size_t words) {
+    const Index& ix = *(const Index*)index;
+    OFilter* f = new OFilter();
+    f->allow.assign(allow, allow + words);
+    for (size_t w = 0; w < words; w++)
+        for (int i = 0; i < 64; i++)
+            if ((allow[w] >> i) & 1) f->rows.push_back((uint32_t)(w * 64 + i));
+    f->per_cluster.resize(ix.k);
+    for (uint32_t c = 0; c < ix.k; c++) {
+        uint32_t cnt = 0;
+        for (uint64_t s = ix.block_off[c] * BLOCK; s < ix.block_off[c + 1] * BLOCK; s++) {
+            uint32_t id = ix.ids[s];
+            if (id != PAD && View::bit(allow, words, id)) cnt++;
+        }
+        f->per_cluster[c] = cnt;
+    }
+    return f;
+}
+void or_filter_free(void* f) { delete (OFilter*)f; }
+size_t or_filter_rows(const void* f) { return ((const OFilter*)f)->rows.size(); }
+void or_filter_per_cluster(const void* f, uint32_t* out) {
+    const auto& p = ((const OFilter*)f)->per_cluster;
+    std::copy(p.begin(), p.end(), out);
+}
+
+// Plan::resolve, src/filter.rs:60-69: plan 0 post, 1 expand (target), 2 brute, 3 auto.
+uint32_t or_plan_resolve(uint32_t plan, uint64_t rows, uint64_t n) {
+    if (plan != 3) return plan;
+    if (rows <= BRUTE_MAX) return 2;
+    if ((double)rows >= POST_MIN * (double)n) return 0;
+    return 1;
+}
+
+// filter::search, src/filter.rs:72-110, for one query; returns the resolved plan.
+uint32_t or_filter_search(const void* index, const void* filter, uint32_t plan, uint64_t target, const float* q768,
+                          const float* q256, const sieve_params* p, float* out_s, uint32_t* out_r, uint32_t* count) {
+    const Index& ix = *(const Index*)index;
+    const OFilter& f = *(const OFilter*)filter;
+    View v;
+    v.allow = f.allow.data();
+    v.allow_words = f.allow.size();
+    uint32_t rp = or_plan_resolve(plan, f.rows.size(), ix.n);
+    if (plan == 3 && rp == 1) target = EXPAND_TARGET;
+    std::vector<std::pair<float, uint32_t>> res;
+    if (rp == 0) {
+        size_t n = search_one(ix, q768, q256, *p, v, out_s, out_r, nullptr);
+        *count = (uint32_t)n;
+        return rp;
+    } else if (rp == 1) {
+        sieve_params all = *p;
+        all.nprobe = ix.k;
+        all.margin = std::numeric_limits<float>::quiet_NaN();
+        all.budget = 0;
+        std::vector<float> cent;
+        auto order = route(ix, q256, all, cent);
+        std::vector<Probe> probe;
+        uint64_t covered = 0;
+        for (const Probe& x : order) {
+            if (f.per_cluster[x.cluster] == 0) continue;
+            if (covered >= target) break;  // take_while(covered < target)
+            covered += f.per_cluster[x.cluster];
+            probe.push_back(x);
+        }
+        uint64_t sc = 0, rr = 0;
+        auto c = candidates(ix, q256, probe, *p, v, &sc);
+        res = rerank(ix, q768, q256, c, *p, v, &rr);
+    } else {
+        RerankQuery r = rerank_query(ix, q768, DIM);
+        TopK<float> top(p->k);
+        for (uint32_t row : f.rows) top.push((float)dot_i8i8(r.wq.data(), ix.rerank + (size_t)row * DIM, DIM) * r.inv, row);
+        res = top.into_sorted();
+    }
+    for (size_t i = 0; i < res.size(); i++) {
+        out_s[i] = res[i].first;
+        out_r[i] = res[i].second;
+    }
+    *count = (uint32_t)res.size();
+    return rp;
+}
+
+}  // extern "C"
