@@ -126,6 +126,8 @@ struct ScanArgs {
     // dbase + dblock_off[c + 1]) of the same slot arrays, scanned right after its base blocks.
     const uint32_t* dblock_off;
     uint32_t dbase;
+    int seg_stride;              // LDS segment table entries: scan_segments(), set by the launcher
+    int trigger;                 // buffered entries that trigger a cut to cand (set by the launcher)
 };
 // Block segments per probe: the base list, and with live delta lists the delta list too.
 __host__ __device__ inline int scan_segments(const ScanArgs& a) { return a.dblock_off ? 2 * a.probe_stride : a.probe_stride; }

@@ -43,10 +43,12 @@ struct ScanLds {
     size_t off_pref, off_start, off_cent, off_buf, off_tmp, off_hist, off_misc, total;
 };
 
-// slots: slots per thread per round (a round covers 256 * slots slots)
-__host__ __device__ inline ScanLds scan_layout(int cand, int probe_stride, int slots) {
+// slots: slots per thread per round (a round covers 256 * slots slots); trigger: buffered
+// entries at which the buffer is cut back to cand (TopK cuts at 2 * cand; a lower trigger cuts
+// more often but needs less LDS, so more workgroups fit a CU).
+__host__ __device__ inline ScanLds scan_layout(int cand, int probe_stride, int slots, int trigger) {
     ScanLds l;
-    l.cap = 2 * cand + SCAN_THREADS * slots;
+    l.cap = trigger + SCAN_THREADS * slots;
     size_t o = 0;
     auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
     l.off_pref = o;
@@ -68,7 +70,7 @@ __host__ __device__ inline ScanLds scan_layout(int cand, int probe_stride, int s
 }
 
 size_t scan_lds_bytes(int cand, int probe_stride) {
-    return scan_layout(cand, probe_stride, 8).total + 1024;  // largest variant, plus the static tables
+    return scan_layout(cand, probe_stride, 4, 2 * cand).total + 1024;  // largest variant, plus the static tables
 }
 
 // misc words: 0 count, 1 has threshold, 2 threshold key, 3 scratch count, 4..7 scan scratch,
@@ -114,10 +116,9 @@ __device__ void scan_compact(uint2* buf, uint2* tmp, int n, int k, uint32_t* his
 // G = 64 (256-bit codes, inner-product estimate) or G = 32 (128-bit codes of the padded
 // 100-d int8 vectors, L2 estimate 2 (cent + mul (a sc + b)) - |x|^2, src/bigann.rs:705-711).
 // SLOTS_PER_THREAD slots per thread per round; PF: prefetch the next round's codes into
-// registers while scoring the current one (more bytes in flight, more VGPRs); IL: interleave
-// the table lookups of a thread's slots (more LDS reads in flight per wave); WPE: minimum
+// registers while scoring the current one (more bytes in flight, more VGPRs); WPE: minimum
 // waves per SIMD the register allocation must allow (0: the compiler's choice).
-template <int G, bool L2, int SLOTS_PER_THREAD, bool PF, bool IL, int WPE>
+template <int G, bool L2, int SLOTS_PER_THREAD, bool PF, int WPE>
 __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_scan_t(ScanArgs a) {
     constexpr int CW = G / 32;  // 16-byte words per slot code
@@ -126,7 +127,7 @@ void k_scan_t(ScanArgs a) {
     // ds_read_u8 with the table offset in its immediate field.
     __shared__ __align__(16) uint8_t tab[G * 16];
     extern __shared__ __align__(16) uint8_t lds[];
-    const ScanLds L = scan_layout(a.cand, scan_segments(a), SLOTS_PER_THREAD);
+    const ScanLds L = scan_layout(a.cand, a.seg_stride, SLOTS_PER_THREAD, a.trigger);
     uint32_t* pref = reinterpret_cast<uint32_t*>(lds + L.off_pref);
     uint32_t* pstart = reinterpret_cast<uint32_t*>(lds + L.off_start);
     float* pcent = reinterpret_cast<float*>(lds + L.off_cent);
@@ -196,7 +197,7 @@ void k_scan_t(ScanArgs a) {
             uint32_t base = 0;
             if (lane == 0) {
                 base = atomicAdd(&misc[M_COUNT], nw);
-                if (base + nw >= 2u * (uint32_t)cand) misc[M_STAMP + (round & 1)] = round;
+                if (base + nw >= (uint32_t)a.trigger) misc[M_STAMP + (round & 1)] = round;
             }
             base = __shfl(base, 0, 64);
             if (pass) buf[base + before] = make_uint2(key, lo);
@@ -246,11 +247,6 @@ void k_scan_t(ScanArgs a) {
             R.ok[u] = vb < vb_hi;
             R.gs[u] = 0;
             R.cent[u] = 0.0f;
-            if constexpr (IL) {
-                R.mul[u] = 0.0f;
-#pragma unroll
-                for (int w = 0; w < CW; w++) R.cw[u][w] = make_uint4(0u, 0u, 0u, 0u);
-            }
             if (R.ok[u]) {
                 while (cur + 1 < np && pref[cur + 1] <= vb) cur++;
                 R.gs[u] = (pstart[cur] + (vb - pref[cur])) * BLOCK + (t & 31);
@@ -262,54 +258,27 @@ void k_scan_t(ScanArgs a) {
             }
         }
     };
-    // The u16 LUT sums (src/lut.rs:83-86) of all the thread's slots, interleaved; slots past
-    // the chunk have zero codes.
-    auto lut_sums = [&](const Regs& R, uint32_t* sc) {
-        auto word = [&](int u, int d) -> uint32_t {
-            const uint4 v = R.cw[u][d >> 2];
-            return (d & 3) == 0 ? v.x : (d & 3) == 1 ? v.y : (d & 3) == 2 ? v.z : v.w;
-        };
-#pragma unroll
-        for (int u = 0; u < SLOTS_PER_THREAD; u++) sc[u] = 0;
-        // One 32-bit code word (8 lookups) of every slot at a time; the scheduling fence keeps the
-        // compiler from hoisting all 64 lookups of a slot into registers.
-#pragma unroll
-        for (int d = 0; d < G / 8; d++) {
-#pragma unroll
-            for (int j = 0; j < 8; j++)
-#pragma unroll
-                for (int u = 0; u < SLOTS_PER_THREAD; u++)
-                    sc[u] += tab[(8 * d + j) * 16 + ((word(u, d) >> (4 * j)) & 15u)];
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    };
     auto score = [&](const Regs& R, uint32_t round) {
         const bool has_thr = misc[M_HAS] != 0;
         const uint32_t thr = misc[M_THR], thr_lo = misc[M_THR_LO];
-        uint32_t scs[SLOTS_PER_THREAD];
-        if constexpr (IL) lut_sums(R, scs);
 #pragma unroll
         for (int u = 0; u < SLOTS_PER_THREAD; u++) {
             bool pass = false;
             uint32_t key = 0;
             if (R.ok[u]) {
-                uint32_t sc = 0;
-                if constexpr (IL) {
-                    sc = scs[u];
-                } else {
-                    uint32_t w[4 * CW];
+                uint32_t w[4 * CW];
 #pragma unroll
-                    for (int x = 0; x < CW; x++) {
-                        w[4 * x] = R.cw[u][x].x;
-                        w[4 * x + 1] = R.cw[u][x].y;
-                        w[4 * x + 2] = R.cw[u][x].z;
-                        w[4 * x + 3] = R.cw[u][x].w;
-                    }
-#pragma unroll
-                    for (int d = 0; d < G / 8; d++)
-#pragma unroll
-                        for (int j = 0; j < 8; j++) sc += tab[(8 * d + j) * 16 + ((w[d] >> (4 * j)) & 15u)];
+                for (int x = 0; x < CW; x++) {
+                    w[4 * x] = R.cw[u][x].x;
+                    w[4 * x + 1] = R.cw[u][x].y;
+                    w[4 * x + 2] = R.cw[u][x].z;
+                    w[4 * x + 3] = R.cw[u][x].w;
                 }
+                uint32_t sc = 0;
+#pragma unroll
+                for (int d = 0; d < G / 8; d++)
+#pragma unroll
+                    for (int j = 0; j < 8; j++) sc += tab[(8 * d + j) * 16 + ((w[d] >> (4 * j)) & 15u)];
                 float est = R.cent[u] + R.mul[u] * (qab.x * (float)sc + qab.y);
                 if constexpr (L2) est = 2.0f * est - a.err[R.gs[u]];
                 est = est + 0.0f;  // -0.0 -> +0.0 so keys order like the floats compare
@@ -461,30 +430,35 @@ void k_scan_t(ScanArgs a) {
     if (t == 0) a.out_n[qi] = tot;
 }
 
-template <int G, bool L2, int S, bool PF, bool IL, int WPE>
-static void launch_variant(const ScanArgs& a, size_t nq, hipStream_t st) {
-    const size_t lds = scan_layout(a.cand, scan_segments(a), S).total;
+template <int G, bool L2, int S, bool PF, int WPE>
+static void launch_variant(const ScanArgs& a_in, size_t nq, hipStream_t st, int trig4) {
+    // The segment stride travels as a plain kernel argument: derived in the kernel from a
+    // pointer test, the LDS offsets would be recomputed in vector registers.
+    ScanArgs a = a_in;
+    a.seg_stride = scan_segments(a);
+    a.trigger = std::max(a.cand + 1, a.cand * trig4 / 4);
+    const size_t lds = scan_layout(a.cand, a.seg_stride, S, a.trigger).total;
     if (lds > 64 * 1024)
-        (void)hipFuncSetAttribute((const void*)k_scan_t<G, L2, S, PF, IL, WPE>,
+        (void)hipFuncSetAttribute((const void*)k_scan_t<G, L2, S, PF, WPE>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_scan_t<G, L2, S, PF, IL, WPE>), dim3((unsigned)(nq * a.chunk_max)), dim3(SCAN_THREADS), lds,
+    hipLaunchKernelGGL((k_scan_t<G, L2, S, PF, WPE>), dim3((unsigned)(nq * a.chunk_max)), dim3(SCAN_THREADS), lds,
                        st, a);
 }
 
-// (slots per thread, prefetch, interleaved lookups, minimum waves per SIMD)
+// (slots per thread, prefetch, minimum waves per SIMD; cut trigger in quarters of cand)
 template <int G, bool L2>
 static void launch_any(const ScanArgs& a, size_t nq, int variant, hipStream_t st) {
     switch (variant) {
-        case 0: launch_variant<G, L2, 4, false, false, 0>(a, nq, st); break;
-        case 1: launch_variant<G, L2, 2, true, false, 0>(a, nq, st); break;
-        case 2: launch_variant<G, L2, 4, true, false, 0>(a, nq, st); break;
-        case 3: launch_variant<G, L2, 8, false, false, 0>(a, nq, st); break;
-        case 4: launch_variant<G, L2, 2, true, false, 5>(a, nq, st); break;
-        case 5: launch_variant<G, L2, 2, false, false, 0>(a, nq, st); break;
-        case 6: launch_variant<G, L2, 1, true, false, 0>(a, nq, st); break;
-        case 7: launch_variant<G, L2, 1, false, false, 0>(a, nq, st); break;
-        case 8: launch_variant<G, L2, 2, false, false, 6>(a, nq, st); break;
-        default: launch_variant<G, L2, 2, true, true, 0>(a, nq, st); break;
+        case 0: launch_variant<G, L2, 4, false, 0>(a, nq, st, 8); break;
+        case 1: launch_variant<G, L2, 2, true, 0>(a, nq, st, 8); break;
+        case 2: launch_variant<G, L2, 2, false, 0>(a, nq, st, 6); break;
+        case 3: launch_variant<G, L2, 1, false, 0>(a, nq, st, 6); break;
+        case 4: launch_variant<G, L2, 2, true, 5>(a, nq, st, 8); break;
+        case 5: launch_variant<G, L2, 2, false, 0>(a, nq, st, 8); break;
+        case 6: launch_variant<G, L2, 1, true, 0>(a, nq, st, 8); break;
+        case 7: launch_variant<G, L2, 1, false, 0>(a, nq, st, 8); break;
+        case 8: launch_variant<G, L2, 1, true, 0>(a, nq, st, 6); break;
+        default: launch_variant<G, L2, 1, false, 0>(a, nq, st, 5); break;
     }
 }
 
@@ -511,9 +485,9 @@ ScanPlan plan_scan(size_t nq, uint32_t nprobe, double mean_blocks, uint32_t max_
 
 const char* scan_variant_name(int v) {
     static const char* names[SCAN_VARIANTS] = {
-        "4 slots/thread", "2 slots/thread + prefetch", "4 slots/thread + prefetch", "8 slots/thread",
+        "4 slots/thread", "2 slots/thread + prefetch", "2 slots/thread, cut at 1.5 cand", "1 slot/thread, cut at 1.5 cand",
         "2 slots/thread + prefetch, >= 5 waves/SIMD", "2 slots/thread", "1 slot/thread + prefetch", "1 slot/thread",
-        "2 slots/thread, >= 6 waves/SIMD", "2 slots/thread + prefetch, interleaved lookups"};
+        "1 slot/thread + prefetch, cut at 1.5 cand", "1 slot/thread, cut at 1.25 cand"};
     return v >= 0 && v < SCAN_VARIANTS ? names[v] : "?";
 }
 

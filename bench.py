@@ -101,6 +101,8 @@ def main():
     ap.add_argument("--sweep", default=None, help="comma-separated configurations to sweep instead of the "
                                                  "workload's list")
     ap.add_argument("--spread", type=float, default=None, help="mixture component spread (data recipe)")
+    ap.add_argument("--scan-variant", type=int, default=None,
+                    help="use this scan kernel variant instead of autotuning (profiling runs)")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="open-loop serving run per rate (0: skip)")
     ap.add_argument("--serve-rates", default="2000,20000", help="offered Poisson arrival rates, queries/s")
     ap.add_argument("--serve-max-batch", type=int, default=512)
@@ -177,8 +179,13 @@ def main():
     ok = [x for x in sweep if x["recall_at_10"] >= 0.90]
     pick = max(ok, key=lambda x: x["qps"]) if ok else max(sweep, key=lambda x: x["recall_at_10"])
     _, p = parse_config(pick["config"], ix.k, k=10)
-    variant, variant_ms = ix.autotune(q768, q256, p, stream=st)
-    log(f"autotune: scan variant {variant} ({', '.join(f'{m:.2f}' for m in variant_ms)} ms)")
+    if a.scan_variant is None:
+        variant, variant_ms = ix.autotune(q768, q256, p, stream=st)
+        log(f"autotune: scan variant {variant} ({', '.join(f'{m:.2f}' for m in variant_ms)} ms)")
+    else:
+        variant = a.scan_variant
+        ix.set_scan_variant(variant)
+        log(f"scan variant {variant} (fixed)")
     agree = probe_agreement(ix, arrays.centroids, q256[:ngt], p, st)
     log(f"probe sets equal to the FP64 top-{p.nprobe}: {agree['equal']:.4f} of {ngt} queries "
         f"({agree['equal_up_to_ties']:.4f} up to f32-resolution ties)")
