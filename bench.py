@@ -54,9 +54,10 @@ WORKLOADS = {
 # components, rows at spread 0.05 around them: R@10 0.92 at 64 probes, 1000 candidates.
 SPREAD = 0.1
 # HBM bytes per scan launch measured with rocprofv3 --pmc for the profiled (workload, search
-# configuration, scan variant): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB x 1024, the factors read
-# off a 1 GiB elementwise copy (bench.py --copy-probe). Other configurations report null.
-PMC_TRAFFIC = {("c3", "64:1000:full:b10x0.8:p0.1", 1): 306.46e9}
+# configuration): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB x 1024, the factors read off a 1 GiB
+# elementwise copy (bench.py --copy-probe). Every scan variant reads the same bytes (four
+# variants profiled within 0.1%). Other configurations report null.
+PMC_TRAFFIC = {("c3", "64:1000:full:b10x0.8:p0.1"): 306.46e9}
 
 
 def log(msg):
@@ -273,7 +274,7 @@ def main():
         "scanned_per_query": round(scanned / a.steps / nq, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": PMC_TRAFFIC.get((a.workload, pick["config"], variant)), "kernel": "k_scan_t",
+                     "traffic": PMC_TRAFFIC.get((a.workload, pick["config"])), "kernel": "k_scan_t",
                      "algorithmic_bytes_per_launch": int(scan_bytes)},
         "cpu_baseline": cpu,
         "sweep": sweep,
