@@ -57,7 +57,7 @@ SPREAD = 0.1
 # configuration): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB x 1024, the factors read off a 1 GiB
 # elementwise copy (bench.py --copy-probe). Every scan variant reads the same bytes (four
 # variants profiled within 0.1%). Other configurations report null.
-PMC_TRAFFIC = {("c3", "64:1000:full:b10x0.8:p0.1"): 306.46e9}
+PMC_TRAFFIC = {("c3", "64:1000:full:b10x0.8:p0.1"): 306.22e9}
 
 
 def log(msg):
