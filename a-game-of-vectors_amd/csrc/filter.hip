@@ -126,6 +126,7 @@ __device__ void topk_compact(uint2* buf, uint2* tmp, int n, int k, uint32_t* his
 // TopK over a query's n scores (score desc, row asc; src/topk.rs:4-62), streamed through LDS.
 __global__ __launch_bounds__(FT) void k_topk_scores(const float* __restrict__ scores,
                                                     const uint32_t* __restrict__ rows, uint64_t R, int k,
+                                                    const uint64_t* __restrict__ deleted, uint64_t deleted_words,
                                                     float* __restrict__ out_s, uint32_t* __restrict__ out_r,
                                                     uint32_t* __restrict__ out_n, int out_stride) {
     extern __shared__ __align__(16) uint8_t lds[];
@@ -152,6 +153,7 @@ __global__ __launch_bounds__(FT) void k_topk_scores(const float* __restrict__ sc
             if (i < R) {
                 e = make_uint2(f2key(sc[i] + 0.0f), ~rows[i]);
                 pass = !has || e.x > thr || (e.x == thr && e.y > thr_lo);
+                if (pass && deleted && bit_set(deleted, deleted_words, rows[i])) pass = false;
             }
             const uint64_t bal = __ballot(pass);
             if (bal) {
@@ -295,6 +297,99 @@ struct sieve_filter {
     }
 };
 
+namespace sieve {
+
+// filter::search for nq queries sharing filter f, enqueued on s with no host synchronisation:
+// outputs os / orr / on (device), counters in B_CNT, the plan's overflow flag (Expand cut
+// beyond MAX_PROBE, or routing) in B_OVF, events ev[0..3] around its stages. With a live state
+// (lv), Post and Expand read the delta lists and tombstones; Brute, as the reference's, scores
+// the allowed corpus rows (skipping tombstoned ones).
+FilterRun enqueue_filter_search(const sieve_index* ix, const sieve_filter* f, uint32_t plan, uint64_t expand_target,
+                                sieve_stream* s, const float* q2, const float* q7, size_t nq, const sieve_params* p,
+                                float* os, uint32_t* orr, uint32_t* on, const LiveArgs* lv) {
+    SIEVE_REQUIRE(f->ix == ix, SIEVE_EINVAL, "the filter belongs to another index");
+    SIEVE_REQUIRE(plan <= SIEVE_PLAN_AUTO, SIEVE_EINVAL, "unknown plan");
+    SIEVE_REQUIRE(nq <= 65535, SIEVE_ELIMIT, "a filtered batch holds at most 65535 queries");
+    FilterRun fr;
+    fr.plan = sieve_plan_resolve(plan, f->nrows, ix->n);
+    if (plan == SIEVE_PLAN_AUTO && fr.plan == SIEVE_PLAN_EXPAND) expand_target = EXPAND_TARGET;
+    uint32_t* ovf = grow<uint32_t>(s->b[B_OVF], 1);
+    ViewArgs view{};
+    view.allow = f->allow;
+    view.allow_words = f->words;
+    if (fr.plan == SIEVE_PLAN_POST) {
+        enqueue_search(ix, s, q2, q7, nq, p, view, os, orr, on, lv);
+        return fr;
+    }
+    if (lv && lv->deleted) {
+        view.deleted = lv->deleted;
+        view.deleted_words = lv->deleted_words;
+    }
+    SIEVE_HIP(hipMemsetAsync(ovf, 0, 4, s->st));
+    unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 4);
+    SIEVE_HIP(hipMemsetAsync(cnt, 0, 32, s->st));
+    SIEVE_HIP(hipEventRecord(s->ev[0], s->st));
+    Batch bt;
+    prep_queries(ix, s, q2, q7, nq, bt);
+    uint32_t* pn = grow<uint32_t>(s->b[B_PN], nq);
+    if (fr.plan == SIEVE_PLAN_BRUTE) {
+        // All allowed rows scored for a sub-batch of queries at a time (<= 256 MB of scores).
+        const uint64_t R = f->nrows;
+        fr.brute_rows = R;
+        SIEVE_HIP(hipMemsetAsync(pn, 0, nq * 4, s->st));
+        SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
+        SIEVE_HIP(hipEventRecord(s->ev[2], s->st));
+        if (R == 0) {
+            SIEVE_HIP(hipMemsetAsync(on, 0, nq * 4, s->st));
+        } else {
+            const size_t sub = std::max<size_t>(1, std::min<size_t>(nq, (size_t(64) << 20) / R));
+            float* sc = grow<float>(s->b[B_TMP1], sub * R);
+            const size_t lds = (size_t)(2 * p->k + 4 * FT) * 8 + (size_t)p->k * 8 + 256 * 4 + 16 * 4;
+            SIEVE_REQUIRE(lds <= 160 * 1024, SIEVE_ELIMIT, "k too large for the brute-force top-k");
+            if (lds > 64 * 1024)
+                (void)hipFuncSetAttribute((const void*)k_topk_scores, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)lds);
+            for (size_t q0 = 0; q0 < nq; q0 += sub) {
+                const size_t m = std::min(sub, nq - q0);
+                hipLaunchKernelGGL(k_brute_scores, dim3(grid(R, 1024), (unsigned)m), dim3(FT), 0, s->st, ix->rerank,
+                                   f->rows, R, bt.prep.wq768 + q0 * DIM, bt.prep.inv + q0, sc);
+                hipLaunchKernelGGL(k_topk_scores, dim3((unsigned)m), dim3(FT), lds, s->st, sc, f->rows, R, (int)p->k,
+                                   view.deleted, view.deleted_words, os + q0 * p->k, orr + q0 * p->k, on + q0,
+                                   (int)p->k);
+            }
+            SIEVE_HIP(hipGetLastError());
+        }
+        SIEVE_HIP(hipEventRecord(s->ev[3], s->st));
+        return fr;
+    }
+    // Expand: exact scores of every centroid, the weighted cut, then the filtered scan.
+    SIEVE_REQUIRE((int)ix->k <= EXPAND_MAX_K, SIEVE_ELIMIT,
+                  "expand plan sorts at most " + std::to_string(EXPAND_MAX_K) + " centroids");
+    const int maxp = (int)std::min<uint32_t>(ix->k, MAX_PROBE);
+    float* all = grow<float>(s->b[B_APPROX], nq * ix->k);
+    hipLaunchKernelGGL(k_exact_route, dim3(grid(ix->k, FT), (unsigned)nq), dim3(FT), 0, s->st, q2, ix->cents,
+                       (int)ix->k, all);
+    float* ps = grow<float>(s->b[B_PS], nq * maxp);
+    uint32_t* pc = grow<uint32_t>(s->b[B_PC], nq * maxp);
+    const size_t lds = (size_t)EXPAND_MAX_K * 8 + 16 * 8;
+    (void)hipFuncSetAttribute((const void*)k_expand_cut, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_expand_cut, dim3((unsigned)nq), dim3(FT), lds, s->st, all, (int)ix->k, f->per_cluster,
+                       expand_target, maxp, ps, pc, pn, ovf);
+    SIEVE_HIP(hipGetLastError());
+    SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
+    float* ce = grow<float>(s->b[B_CE], nq * p->cand);
+    uint32_t* cr = grow<uint32_t>(s->b[B_CR], nq * p->cand);
+    float* cx = grow<float>(s->b[B_CX], nq * p->cand);
+    uint32_t* cn = grow<uint32_t>(s->b[B_CN], nq);
+    scan_batch(ix, s, bt, nq, ps, pc, pn, maxp, p, view, ce, cr, cx, cn, (int)p->cand, cnt, lv);
+    SIEVE_HIP(hipEventRecord(s->ev[2], s->st));
+    rerank_batch(ix, s, bt, nq, ce, cr, cx, cn, (int)p->cand, p, os, orr, on, cnt + 1, lv);
+    SIEVE_HIP(hipEventRecord(s->ev[3], s->st));
+    return fr;
+}
+
+}  // namespace sieve
+
 extern "C" {
 
 int sieve_filter_create(const sieve_index* ix, const uint64_t* allow, uint64_t allow_words, sieve_filter** out) {
@@ -358,110 +453,36 @@ int sieve_filter_search_batch(const sieve_index* ix, const sieve_filter* f, uint
                               float* scores, uint32_t* rows, uint32_t* counts, uint32_t* resolved, sieve_timing* t) {
     return guarded([&] {
         SIEVE_REQUIRE(ix && f && s && q768 && q256 && p && scores && rows && counts, SIEVE_EINVAL, "null argument");
-        SIEVE_REQUIRE(f->ix == ix, SIEVE_EINVAL, "the filter belongs to another index");
-        SIEVE_REQUIRE(plan <= SIEVE_PLAN_AUTO, SIEVE_EINVAL, "unknown plan");
         check_params(p, true);
-        SIEVE_REQUIRE(nq <= 65535, SIEVE_ELIMIT, "a filtered batch holds at most 65535 queries");
         SIEVE_HIP(hipSetDevice(ix->device));
-        const uint32_t rp = sieve_plan_resolve(plan, f->nrows, ix->n);
-        if (plan == SIEVE_PLAN_AUTO && rp == SIEVE_PLAN_EXPAND) expand_target = EXPAND_TARGET;
-        if (resolved) *resolved = rp;
-        if (nq == 0) return;
+        if (nq == 0) {
+            if (resolved) *resolved = sieve_plan_resolve(plan, f->nrows, ix->n);
+            return;
+        }
         ensure_events(s);
         const float* q2 = dev_in(s, B_Q256, q256, nq * PDIM);
         const float* q7 = dev_in(s, B_Q768, q768, nq * DIM);
         auto os = dev_out(s, B_OS, scores, nq * p->k);
         auto orr = dev_out(s, B_OR, rows, nq * p->k);
         auto on = dev_out(s, B_ON, counts, nq);
-        ViewArgs view{};
-        view.allow = f->allow;
-        view.allow_words = f->words;
-        if (rp == SIEVE_PLAN_POST) {
-            enqueue_search(ix, s, q2, q7, nq, p, view, os.dev, orr.dev, on.dev);
-            out_done(s, os);
-            out_done(s, orr);
-            out_done(s, on);
-            finish_search(s, nq, t);
-            return;
-        }
-        unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 4);
-        SIEVE_HIP(hipMemsetAsync(cnt, 0, 32, s->st));
-        SIEVE_HIP(hipEventRecord(s->ev[0], s->st));
-        Batch bt;
-        prep_queries(ix, s, q2, q7, nq, bt);
-        uint32_t* pn = grow<uint32_t>(s->b[B_PN], nq);
-        if (rp == SIEVE_PLAN_BRUTE) {
-            // All allowed rows scored for a sub-batch of queries at a time (<= 256 MB of scores).
-            const uint64_t R = f->nrows;
-            SIEVE_HIP(hipMemsetAsync(pn, 0, nq * 4, s->st));
-            SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
-            SIEVE_HIP(hipEventRecord(s->ev[2], s->st));
-            if (R == 0) {
-                SIEVE_HIP(hipMemsetAsync(on.dev, 0, nq * 4, s->st));
-            } else {
-                const size_t sub = std::max<size_t>(1, std::min<size_t>(nq, (size_t(64) << 20) / R));
-                float* sc = grow<float>(s->b[B_TMP1], sub * R);
-                const size_t lds = (size_t)(2 * p->k + 4 * FT) * 8 + (size_t)p->k * 8 + 256 * 4 + 16 * 4;
-                SIEVE_REQUIRE(lds <= 160 * 1024, SIEVE_ELIMIT, "k too large for the brute-force top-k");
-                if (lds > 64 * 1024)
-                    (void)hipFuncSetAttribute((const void*)k_topk_scores, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                              (int)lds);
-                for (size_t q0 = 0; q0 < nq; q0 += sub) {
-                    const size_t m = std::min(sub, nq - q0);
-                    hipLaunchKernelGGL(k_brute_scores, dim3(grid(R, 1024), (unsigned)m), dim3(FT), 0, s->st,
-                                       ix->rerank, f->rows, R, bt.prep.wq768 + q0 * DIM, bt.prep.inv + q0, sc);
-                    hipLaunchKernelGGL(k_topk_scores, dim3((unsigned)m), dim3(FT), lds, s->st, sc, f->rows, R,
-                                       (int)p->k, os.dev + q0 * p->k, orr.dev + q0 * p->k, on.dev + q0, (int)p->k);
-                }
-                SIEVE_HIP(hipGetLastError());
-            }
-            SIEVE_HIP(hipEventRecord(s->ev[3], s->st));
-            out_done(s, os);
-            out_done(s, orr);
-            out_done(s, on);
-            finish_search(s, nq, nullptr);
-            s->last.reranked = (unsigned long long)R * nq;  // every allowed row, per query
-            if (t) {
-                t->rerank += (uint64_t)s->last.rerank_ns;
-                t->reranked += s->last.reranked;
-            }
-            return;
-        } else {
-            // Expand: exact scores of every centroid, the weighted cut, then the filtered scan.
-            SIEVE_REQUIRE((int)ix->k <= EXPAND_MAX_K, SIEVE_ELIMIT,
-                          "expand plan sorts at most " + std::to_string(EXPAND_MAX_K) + " centroids");
-            const int maxp = (int)std::min<uint32_t>(ix->k, MAX_PROBE);
-            float* all = grow<float>(s->b[B_APPROX], nq * ix->k);
-            hipLaunchKernelGGL(k_exact_route, dim3(grid(ix->k, FT), (unsigned)nq), dim3(FT), 0, s->st, q2, ix->cents,
-                               (int)ix->k, all);
-            float* ps = grow<float>(s->b[B_PS], nq * maxp);
-            uint32_t* pc = grow<uint32_t>(s->b[B_PC], nq * maxp);
-            uint32_t* ovf = grow<uint32_t>(s->b[B_OVF], 1);
-            SIEVE_HIP(hipMemsetAsync(ovf, 0, 4, s->st));
-            const size_t lds = (size_t)EXPAND_MAX_K * 8 + 16 * 8;
-            (void)hipFuncSetAttribute((const void*)k_expand_cut, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            hipLaunchKernelGGL(k_expand_cut, dim3((unsigned)nq), dim3(FT), lds, s->st, all, (int)ix->k,
-                               f->per_cluster, expand_target, maxp, ps, pc, pn, ovf);
-            SIEVE_HIP(hipGetLastError());
-            uint32_t h = 0;
-            SIEVE_HIP(hipMemcpyAsync(&h, ovf, 4, hipMemcpyDeviceToHost, s->st));
-            SIEVE_HIP(hipStreamSynchronize(s->st));
-            SIEVE_REQUIRE(h == 0, SIEVE_ELIMIT,
-                          "expand plan needs more than " + std::to_string(maxp) + " probed clusters");
-            SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
-            float* ce = grow<float>(s->b[B_CE], nq * p->cand);
-            uint32_t* cr = grow<uint32_t>(s->b[B_CR], nq * p->cand);
-            float* cx = grow<float>(s->b[B_CX], nq * p->cand);
-            uint32_t* cn = grow<uint32_t>(s->b[B_CN], nq);
-            scan_batch(ix, s, bt, nq, ps, pc, pn, maxp, p, view, ce, cr, cx, cn, (int)p->cand, cnt);
-            SIEVE_HIP(hipEventRecord(s->ev[2], s->st));
-            rerank_batch(ix, s, bt, nq, ce, cr, cx, cn, (int)p->cand, p, os.dev, orr.dev, on.dev, cnt + 1);
-            SIEVE_HIP(hipEventRecord(s->ev[3], s->st));
-        }
+        const FilterRun fr = enqueue_filter_search(ix, f, plan, expand_target, s, q2, q7, nq, p, os.dev, orr.dev,
+                                                   on.dev, nullptr);
+        if (resolved) *resolved = fr.plan;
         out_done(s, os);
         out_done(s, orr);
         out_done(s, on);
+        uint32_t h = 0;
+        SIEVE_HIP(hipMemcpyAsync(&h, s->b[B_OVF].p, 4, hipMemcpyDeviceToHost, s->st));
+        SIEVE_HIP(hipStreamSynchronize(s->st));
+        SIEVE_REQUIRE(h == 0, SIEVE_ELIMIT, fr.plan == SIEVE_PLAN_EXPAND
+                                                ? "expand plan needs more than " + std::to_string(MAX_PROBE) +
+                                                      " probed clusters"
+                                                : std::string("routing: too many centroids within the BF16 error band"));
         finish_search(s, nq, t);
+        if (fr.plan == SIEVE_PLAN_BRUTE) {
+            s->last.reranked = fr.brute_rows * nq;  // every allowed row, per query
+            if (t) t->reranked += s->last.reranked;
+        }
     });
 }
 

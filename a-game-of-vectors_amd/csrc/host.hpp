@@ -496,6 +496,18 @@ inline void finish_search(sieve_stream* s, size_t nq, sieve_timing* t) {
     }
 }
 
+// The delta segment a live state hands to searches (csrc/live.hip).
+LiveArgs live_args(const sieve_live* l);
+
+// filter::search enqueued without host synchronisation (csrc/filter.hip).
+struct FilterRun {
+    uint32_t plan = 0;        // the plan used (SIEVE_PLAN_*)
+    uint64_t brute_rows = 0;  // rows each query scored under the Brute plan
+};
+FilterRun enqueue_filter_search(const sieve_index* ix, const sieve_filter* f, uint32_t plan, uint64_t expand_target,
+                                sieve_stream* s, const float* q2, const float* q7, size_t nq, const sieve_params* p,
+                                float* os, uint32_t* orr, uint32_t* on, const LiveArgs* lv);
+
 inline void ensure_events(sieve_stream* s) {
     for (auto& e : s->ev)
         if (!e) SIEVE_HIP(hipEventCreate(&e));

@@ -501,6 +501,10 @@ void live_delete(sieve_live* l, const uint32_t* ext, size_t n) {
 
 }  // namespace
 
+namespace sieve {
+LiveArgs live_args(const sieve_live* l) { return l->args(); }
+}  // namespace sieve
+
 extern "C" {
 
 int sieve_live_open(sieve_index* ix, const sieve_live_opts* o, sieve_live** out) {

@@ -11,6 +11,13 @@
 //
 // Each query is scanned and reranked by its own workgroups, so a request's results do not
 // depend on which other requests share its batch.
+//
+// Filters and live writes (the serving config with filters and live updates): a filtered
+// request is batched only with queued requests carrying the same filter and plan and runs
+// filter::search on the device; with a live state attached, searches read its delta lists and
+// tombstones, and inserts / deletes queue with the searches. A write is a barrier in
+// submission order: the batcher lets every launched batch finish, applies the write, and only
+// then launches later searches, so each search sees exactly the writes submitted before it.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -26,7 +33,14 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+enum Kind { K_SEARCH = 0, K_INSERT = 1, K_DELETE = 2 };
+
 struct Request {
+    int kind = K_SEARCH;
+    const sieve_filter* filter = nullptr;
+    uint32_t plan = SIEVE_PLAN_AUTO;
+    std::vector<uint32_t> ext;  // writes
+    std::vector<float> vecs;
     std::vector<float> q768, q256;
     std::vector<float> scores;
     std::vector<uint32_t> rows;
@@ -70,6 +84,8 @@ struct BatchSlot {
 
 struct sieve_server {
     const sieve_index* ix = nullptr;
+    sieve_live* live = nullptr;
+    LiveArgs la;
     sieve_params p{};
     sieve_server_opts o{};
     std::vector<std::unique_ptr<BatchSlot>> slots;
@@ -88,6 +104,14 @@ struct sieve_server {
 
 namespace {
 
+// Applies one write request (the batcher holds no batch in flight).
+void apply_write(sieve_server* sv, Request& r) {
+    int rc = r.kind == K_INSERT ? sieve_live_insert(sv->live, r.ext.data(), r.vecs.data(), r.ext.size())
+                                : sieve_live_delete(sv->live, r.ext.data(), r.ext.size());
+    r.status = rc;
+    if (rc != 0) r.error = sieve_last_error();
+}
+
 void batcher_loop(sieve_server* sv) {
     (void)hipSetDevice(sv->device);
     const size_t K = sv->p.k;
@@ -95,6 +119,8 @@ void batcher_loop(sieve_server* sv) {
     for (;;) {
         BatchSlot* slot = nullptr;
         std::vector<uint64_t> take;
+        const sieve_filter* filter = nullptr;
+        uint32_t plan = SIEVE_PLAN_AUTO;
         {
             std::unique_lock<std::mutex> lk(sv->mu);
             auto free_slot = [&]() -> BatchSlot* {
@@ -102,14 +128,50 @@ void batcher_loop(sieve_server* sv) {
                     if (!x->busy) return x.get();
                 return nullptr;
             };
-            sv->cv_batcher.wait(lk, [&] { return sv->stop || (!sv->pending.empty() && free_slot()); });
+            auto ready = [&] {
+                if (sv->pending.empty()) return false;
+                const Request& h = sv->reqs[sv->pending.front()];
+                if (h.kind != K_SEARCH) return sv->inflight.empty();  // a write waits for the device to drain
+                return free_slot() != nullptr;
+            };
+            sv->cv_batcher.wait(lk, [&] { return sv->stop || ready(); });
             if (sv->stop) return;
-            slot = free_slot();
-            size_t m = std::min<size_t>(sv->pending.size(), sv->o.max_batch);
-            for (size_t i = 0; i < m; i++) {
-                take.push_back(sv->pending.front());
+            const uint64_t head = sv->pending.front();
+            Request& hr = sv->reqs[head];
+            if (hr.kind != K_SEARCH) {
                 sv->pending.pop_front();
+                sv->launching++;
+                hr.t_launch = Clock::now();
+                lk.unlock();
+                apply_write(sv, hr);
+                lk.lock();
+                sv->launching--;
+                hr.done = true;
+                hr.t_done = Clock::now();
+                sv->stats.writes++;
+                sv->cv_done.notify_all();
+                continue;
             }
+            slot = free_slot();
+            filter = hr.filter;
+            plan = hr.plan;
+            // Requests sharing the head's filter and plan, in order, up to the next write.
+            std::deque<uint64_t> rest;
+            while (!sv->pending.empty()) {
+                const uint64_t tk = sv->pending.front();
+                const Request& r = sv->reqs[tk];
+                if (r.kind != K_SEARCH) break;
+                sv->pending.pop_front();
+                if (take.size() < sv->o.max_batch && r.filter == filter && r.plan == plan)
+                    take.push_back(tk);
+                else
+                    rest.push_back(tk);
+            }
+            while (!rest.empty()) {
+                sv->pending.push_front(rest.back());
+                rest.pop_back();
+            }
+            const size_t m = take.size();
             slot->busy = true;
             slot->batch_id = sv->next_batch++;
             sv->launching++;
@@ -136,8 +198,13 @@ void batcher_loop(sieve_server* sv) {
         try {
             SIEVE_HIP(hipMemcpyAsync(slot->dq768, slot->q768.p, nq * DIM * 4, hipMemcpyHostToDevice, s->st));
             SIEVE_HIP(hipMemcpyAsync(slot->dq256, slot->q256.p, nq * PDIM * 4, hipMemcpyHostToDevice, s->st));
-            enqueue_search(sv->ix, s, slot->dq256, slot->dq768, nq, &sv->p, ViewArgs{}, slot->dscores, slot->drows,
-                           slot->dcounts);
+            const LiveArgs* lv = sv->live ? &sv->la : nullptr;
+            if (filter)
+                enqueue_filter_search(sv->ix, filter, plan, 0, s, slot->dq256, slot->dq768, nq, &sv->p,
+                                      slot->dscores, slot->drows, slot->dcounts, lv);
+            else
+                enqueue_search(sv->ix, s, slot->dq256, slot->dq768, nq, &sv->p, ViewArgs{}, slot->dscores,
+                               slot->drows, slot->dcounts, lv);
             SIEVE_HIP(hipMemcpyAsync(slot->scores.p, slot->dscores, nq * K * 4, hipMemcpyDeviceToHost, s->st));
             SIEVE_HIP(hipMemcpyAsync(slot->rows.p, slot->drows, nq * K * 4, hipMemcpyDeviceToHost, s->st));
             SIEVE_HIP(hipMemcpyAsync(slot->counts.p, slot->dcounts, nq * 4, hipMemcpyDeviceToHost, s->st));
@@ -237,6 +304,11 @@ int sieve_server_create(const sieve_index* ix, const sieve_params* p, const siev
         sv->ix = ix;
         sv->p = *p;
         sv->o = o ? *o : sieve_server_opts{};
+        sv->live = sv->o.live;
+        if (sv->live) {
+            SIEVE_REQUIRE(ix->live == sv->live, SIEVE_EINVAL, "opts.live is not this index's live state");
+            sv->la = live_args(sv->live);
+        }
         if (sv->o.max_batch == 0) sv->o.max_batch = 1024;
         size_t budget = sv->o.hbm_budget;
         if (budget == 0) {
@@ -306,6 +378,57 @@ int sieve_server_submit(sieve_server* sv, const float* q768, const float* q256, 
         sv->pending.push_back(tk);
         *ticket = tk;
         sv->cv_batcher.notify_one();
+    });
+}
+
+namespace {
+int submit_request(sieve_server* sv, Request&& r, uint64_t* ticket) {
+    r.t_submit = Clock::now();
+    std::lock_guard<std::mutex> lk(sv->mu);
+    SIEVE_REQUIRE(!sv->stop, SIEVE_EINVAL, "server is stopping");
+    uint64_t tk = sv->next_ticket++;
+    sv->reqs.emplace(tk, std::move(r));
+    sv->pending.push_back(tk);
+    *ticket = tk;
+    sv->cv_batcher.notify_one();
+    return 0;
+}
+}  // namespace
+
+int sieve_server_submit_filtered(sieve_server* sv, const float* q768, const float* q256, const sieve_filter* filter,
+                                 uint32_t plan, uint64_t* ticket) {
+    return guarded([&] {
+        SIEVE_REQUIRE(sv && q768 && q256 && filter && ticket, SIEVE_EINVAL, "null argument");
+        SIEVE_REQUIRE(plan <= SIEVE_PLAN_AUTO, SIEVE_EINVAL, "unknown plan");
+        Request r;
+        r.filter = filter;
+        r.plan = plan;
+        r.q768.assign(q768, q768 + DIM);
+        r.q256.assign(q256, q256 + PDIM);
+        submit_request(sv, std::move(r), ticket);
+    });
+}
+
+int sieve_server_insert(sieve_server* sv, const uint32_t* ext, const float* vecs, size_t n, uint64_t* ticket) {
+    return guarded([&] {
+        SIEVE_REQUIRE(sv && ticket && (n == 0 || (ext && vecs)), SIEVE_EINVAL, "null argument");
+        SIEVE_REQUIRE(sv->live, SIEVE_EINVAL, "the server has no live state");
+        Request r;
+        r.kind = K_INSERT;
+        r.ext.assign(ext, ext + n);
+        r.vecs.assign(vecs, vecs + n * DIM);
+        submit_request(sv, std::move(r), ticket);
+    });
+}
+
+int sieve_server_delete(sieve_server* sv, const uint32_t* ext, size_t n, uint64_t* ticket) {
+    return guarded([&] {
+        SIEVE_REQUIRE(sv && ticket && (n == 0 || ext), SIEVE_EINVAL, "null argument");
+        SIEVE_REQUIRE(sv->live, SIEVE_EINVAL, "the server has no live state");
+        Request r;
+        r.kind = K_DELETE;
+        r.ext.assign(ext, ext + n);
+        submit_request(sv, std::move(r), ticket);
     });
 }
 

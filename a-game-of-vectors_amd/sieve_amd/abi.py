@@ -91,7 +91,8 @@ class LiveInfo(ctypes.Structure):
 
 
 class ServerOpts(ctypes.Structure):
-    _fields_ = [("max_batch", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64)]
+    _fields_ = [("max_batch", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
+                ("live", ctypes.c_void_p)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -101,7 +102,8 @@ class RequestInfo(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("batches", ctypes.c_uint64), ("queries", ctypes.c_uint64), ("pending", ctypes.c_uint64),
-                ("inflight", ctypes.c_uint64), ("slots", ctypes.c_uint32), ("slot_bytes", ctypes.c_uint64)]
+                ("inflight", ctypes.c_uint64), ("slots", ctypes.c_uint32), ("slot_bytes", ctypes.c_uint64),
+                ("writes", ctypes.c_uint64)]
 
 
 _lib = None
@@ -158,6 +160,9 @@ def lib():
         "sieve_l2_index_set_scan_chunk": (i32, [vp, u32]),
         "sieve_server_create": (i32, [vp, P, ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)]),
         "sieve_server_submit": (i32, [vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "sieve_server_submit_filtered": (i32, [vp, vp, vp, vp, u32, ctypes.POINTER(ctypes.c_uint64)]),
+        "sieve_server_insert": (i32, [vp, vp, vp, sz, ctypes.POINTER(ctypes.c_uint64)]),
+        "sieve_server_delete": (i32, [vp, vp, sz, ctypes.POINTER(ctypes.c_uint64)]),
         "sieve_server_wait": (i32, [vp, ctypes.c_uint64, vp, vp, ctypes.POINTER(ctypes.c_uint32),
                                     ctypes.POINTER(RequestInfo)]),
         "sieve_server_get_stats": (i32, [vp, ctypes.POINTER(ServerStats)]),

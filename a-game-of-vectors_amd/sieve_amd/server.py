@@ -12,11 +12,13 @@ from .evaluate import pct
 
 
 class Server:
-    def __init__(self, index, params, max_batch=1024, max_inflight=0, hbm_budget=0):
+    def __init__(self, index, params, max_batch=1024, max_inflight=0, hbm_budget=0, live=None):
         self._ix = index
+        self._live = live
         self.k = params.k
         h = ctypes.c_void_p()
-        o = abi.ServerOpts(max_batch=max_batch, max_inflight=max_inflight, hbm_budget=hbm_budget)
+        o = abi.ServerOpts(max_batch=max_batch, max_inflight=max_inflight, hbm_budget=hbm_budget,
+                           live=live._h.value if live is not None else None)
         abi.check(abi.lib().sieve_server_create(index._h, ctypes.byref(abi.make_params(params)), ctypes.byref(o),
                                                 ctypes.byref(h)))
         self._h = h
@@ -26,6 +28,27 @@ class Server:
         q2 = np.ascontiguousarray(q256, np.float32)
         t = ctypes.c_uint64()
         abi.check(abi.lib().sieve_server_submit(self._h, abi.ptr(q7), abi.ptr(q2), ctypes.byref(t)))
+        return t.value
+
+    def submit_filtered(self, q768, q256, filt, plan="auto"):
+        q7 = np.ascontiguousarray(q768, np.float32)
+        q2 = np.ascontiguousarray(q256, np.float32)
+        t = ctypes.c_uint64()
+        abi.check(abi.lib().sieve_server_submit_filtered(self._h, abi.ptr(q7), abi.ptr(q2), filt._h, abi.PLANS[plan],
+                                                         ctypes.byref(t)))
+        return t.value
+
+    def insert(self, ext, vecs):
+        e = np.ascontiguousarray(ext, np.uint32)
+        v = np.ascontiguousarray(vecs, np.float32)
+        t = ctypes.c_uint64()
+        abi.check(abi.lib().sieve_server_insert(self._h, abi.ptr(e), abi.ptr(v), e.shape[0], ctypes.byref(t)))
+        return t.value
+
+    def delete(self, ext):
+        e = np.ascontiguousarray(ext, np.uint32)
+        t = ctypes.c_uint64()
+        abi.check(abi.lib().sieve_server_delete(self._h, abi.ptr(e), e.shape[0], ctypes.byref(t)))
         return t.value
 
     def wait(self, ticket):

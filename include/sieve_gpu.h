@@ -269,6 +269,8 @@ typedef struct sieve_server_opts {
     uint32_t max_batch;      /* queries per device batch (0: 1024) */
     uint32_t max_inflight;   /* device batches in flight at most (0: as many as fit) */
     uint64_t hbm_budget;     /* device bytes the slots may hold (0: a quarter of free HBM) */
+    struct sieve_live* live; /* the index's live state (NULL: none): searches see its delta lists and
+                              * tombstones and report external ids, and the server accepts writes */
 } sieve_server_opts;
 
 typedef struct sieve_request_info {
@@ -285,6 +287,7 @@ typedef struct sieve_server_stats {
     uint64_t inflight;
     uint32_t slots;
     uint64_t slot_bytes;
+    uint64_t writes;         /* insert / delete requests applied */
 } sieve_server_stats;
 
 typedef struct sieve_server sieve_server;
@@ -293,6 +296,16 @@ int sieve_server_create(const sieve_index* ix, const sieve_params* p, const siev
                         sieve_server** out);
 /* Copies one query (768 + 256 f32, host memory) into the queue; returns a ticket. */
 int sieve_server_submit(sieve_server* s, const float* q768, const float* q256, uint64_t* ticket);
+/* A query carrying a metadata filter (filter::search, src/filter.rs:72-110) under `plan`
+ * (SIEVE_PLAN_*): it is batched with queued queries that carry the same filter and plan. */
+int sieve_server_submit_filtered(sieve_server* s, const float* q768, const float* q256,
+                                 const struct sieve_filter* filter, uint32_t plan, uint64_t* ticket);
+/* Live writes through the server (needs opts.live): Live::insert / Live::delete
+ * (src/live.rs:286-326) of n rows (ext ids, n x 768 unit vectors in host memory). A write is a
+ * barrier in submission order: it is applied once every earlier search has finished, and every
+ * later search sees it. sieve_server_wait on its ticket returns when it is applied (count 0). */
+int sieve_server_insert(sieve_server* s, const uint32_t* ext, const float* vecs, size_t n, uint64_t* ticket);
+int sieve_server_delete(sieve_server* s, const uint32_t* ext, size_t n, uint64_t* ticket);
 /* Blocks until the ticket's results are ready; scores/rows hold p->k entries (count used). */
 int sieve_server_wait(sieve_server* s, uint64_t ticket, float* scores, uint32_t* rows, uint32_t* count,
                       sieve_request_info* info);

@@ -29,6 +29,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(abi.Params) == 40 and abi.Params.budget.offset == 32
     assert ctypes.sizeof(abi.Timing) == 72
     assert ctypes.sizeof(abi.LiveOpts) == 16 and ctypes.sizeof(abi.LiveInfo) == 48
+    assert ctypes.sizeof(abi.ServerOpts) == 24 and ctypes.sizeof(abi.ServerStats) == 56
     assert abi.IndexDesc.borrow_rerank.offset == 24 + 12 * 8  # after k, n, nblocks and 12 pointers
 
 

@@ -70,3 +70,69 @@ def test_open_loop_completes(small, setup):
     res = open_loop(sv, small.q768, small.q256, rate=2000, seconds=1.0)
     sv.close()
     assert res["requests"] > 500 and res["p99_ms"] >= res["p50_ms"] > 0
+
+
+def test_filters_and_live_writes_in_the_serving_loop(small, oracle_index):
+    """Searches, filtered searches, inserts and deletes through one server: a write is a barrier
+    in submission order, so every result equals the oracle's Live searched in that order."""
+    import numpy as np
+    import torch
+    from oracle import oracle as O
+    from sieve_amd.build import normalize_rows
+    from sieve_amd.index import Filter, Live, bitset
+    from test_gpu_parity import assert_same_up_to_ties
+
+    ix = Index.create(small.host)
+    live = Live(ix, max_rows=1000)
+    olive = O.Live(oracle_index, soar_lambda=1.0)
+    p = Params(nprobe=12, cand=400, k=10, rerank=Rerank.FULL)
+    rng = np.random.default_rng(21)
+    allow_rows = np.sort(rng.choice(small.n, small.n // 4, replace=False))
+    allow = bitset(allow_rows, small.n)
+    filt = Filter(ix, allow)
+    oview = O.make_view(allow=allow)
+    new, _ = normalize_rows(small.mix.rows(small.n + 500, 200))
+    new = new.numpy().astype(np.float32)
+    # queries near the new rows, so the writes change results
+    q768 = np.concatenate([small.q768[:12], new[:8]])
+    q256 = np.concatenate([small.q256[:12], normalize_rows(torch.from_numpy(new[:8, :256].copy()))[0].numpy()])
+    sv = Server(ix, p, max_batch=8, max_inflight=3, live=live)
+    expected, tickets = [], []
+
+    def searches():
+        for i in range(len(q768)):
+            tickets.append(("plain", i, sv.submit(q768[i], q256[i])))
+            if i % 3 == 0:
+                tickets.append(("post", i, sv.submit_filtered(q768[i], q256[i], filt, plan="post")))
+        s, r, n = olive.search_batch(q768, q256, p, threads=4)
+        fs, fr, fn = olive.search_batch(q768, q256, p, view=oview, threads=4)
+        expected.append(((s, r, n), (fs, fr, fn)))
+
+    ext1 = (80_000_000 + np.arange(150)).astype(np.uint32)
+    searches()
+    n0 = len(tickets)
+    w1 = sv.insert(ext1, new[:150])
+    olive.insert(ext1, new[:150])
+    searches()
+    n1 = len(tickets)
+    dels = np.concatenate([ext1[:40], allow_rows[:30]]).astype(np.uint32)
+    w2 = sv.delete(dels)
+    olive.delete(dels)
+    searches()
+    w3 = sv.insert(ext1[:5], new[150:155])  # upserts
+    olive.insert(ext1[:5], new[150:155])
+    searches()
+    # phases: tickets [0, n0) before any write, [n0, n1) after w1, then after w2, after w3
+    per_phase = (len(tickets) - n1) // 2
+    check_ranges = [(0, 0, n0), (1, n0, n1), (2, n1, n1 + per_phase), (3, n1 + per_phase, len(tickets))]
+    for phase, a, b in check_ranges:
+        (s, r, n), (fs, fr, fn) = expected[phase]
+        for kind, i, t in tickets[a:b]:
+            gs, gr, _ = sv.wait(t)
+            ws, wr, wn = (s, r, n) if kind == "plain" else (fs, fr, fn)
+            assert_same_up_to_ties(gr, gs, wr[i, :wn[i]], ws[i, :wn[i]])
+    for w in (w1, w2, w3):
+        sv.wait(w)
+    assert sv.stats()["writes"] == 3
+    sv.close()
+    live.close()
