@@ -19,6 +19,7 @@ constexpr uint64_t BRUTE_MAX = 250000;       // src/filter.rs:54
 constexpr double POST_MIN = 0.3;             // src/filter.rs:56
 constexpr uint64_t EXPAND_TARGET = 200000;   // src/filter.rs:58
 constexpr int EXPAND_MAX_K = 16384;          // centroids the expand cut sorts in LDS
+constexpr int EXPAND_MAX_PROBE = 8192;       // clusters an expand cut may probe (the scan's LDS segment table)
 constexpr int FT = 256;                      // threads of the filter kernels
 
 // Allowed slots of cluster c (Filter::new, src/filter.rs:30-36).
@@ -365,7 +366,7 @@ FilterRun enqueue_filter_search(const sieve_index* ix, const sieve_filter* f, ui
     // Expand: exact scores of every centroid, the weighted cut, then the filtered scan.
     SIEVE_REQUIRE((int)ix->k <= EXPAND_MAX_K, SIEVE_ELIMIT,
                   "expand plan sorts at most " + std::to_string(EXPAND_MAX_K) + " centroids");
-    const int maxp = (int)std::min<uint32_t>(ix->k, MAX_PROBE);
+    const int maxp = (int)std::min<uint32_t>(ix->k, EXPAND_MAX_PROBE);
     float* all = grow<float>(s->b[B_APPROX], nq * ix->k);
     hipLaunchKernelGGL(k_exact_route, dim3(grid(ix->k, FT), (unsigned)nq), dim3(FT), 0, s->st, q2, ix->cents,
                        (int)ix->k, all);
@@ -475,7 +476,7 @@ int sieve_filter_search_batch(const sieve_index* ix, const sieve_filter* f, uint
         SIEVE_HIP(hipMemcpyAsync(&h, s->b[B_OVF].p, 4, hipMemcpyDeviceToHost, s->st));
         SIEVE_HIP(hipStreamSynchronize(s->st));
         SIEVE_REQUIRE(h == 0, SIEVE_ELIMIT, fr.plan == SIEVE_PLAN_EXPAND
-                                                ? "expand plan needs more than " + std::to_string(MAX_PROBE) +
+                                                ? "expand plan needs more than " + std::to_string(EXPAND_MAX_PROBE) +
                                                       " probed clusters"
                                                 : std::string("routing: too many centroids within the BF16 error band"));
         finish_search(s, nq, t);

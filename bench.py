@@ -107,6 +107,10 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="open-loop serving run per rate (0: skip)")
     ap.add_argument("--serve-rates", default="2000,20000", help="offered Poisson arrival rates, queries/s")
     ap.add_argument("--serve-max-batch", type=int, default=512)
+    ap.add_argument("--serve-mixed", type=float, default=0.0,
+                    help="seconds of the mixed serving load (filters on 30%% of queries, live inserts at "
+                         "100k rows/s with 10%% deletes), run last on the same index (0: skip)")
+    ap.add_argument("--serve-mixed-rate", type=float, default=5000.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run an elementwise kernel that reads 1 GiB and writes 1 GiB (the known size that "
                          "scales a profile's FETCH_SIZE/WRITE_SIZE counters), then exit")
@@ -251,6 +255,7 @@ def main():
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(arrays, q768, q256, p, nq, a, gt)
+    mixed = serve_mixed(ix, mix, W["rows"], q768, q256, p, a) if a.serve_mixed > 0 else None
 
     out = {
         "metric": METRIC, "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": a.steps,
@@ -266,6 +271,7 @@ def main():
         "p50_ms": p50, "p99_ms": p99,
         "per_query_gpu_qps": per_query_qps,
         "serving_open_loop": serving,
+        "serving_mixed": mixed,
         "scan_variant": variant,
         "probe_agreement_fp64": agree,
         "stage_ms": {"route": round(route_ns / 1e6 / a.steps, 3), "scan": round(scan_s * 1e3, 3),
@@ -281,6 +287,95 @@ def main():
         "build_s": round(t_build, 1),
     }
     print(json.dumps(out), flush=True)
+
+
+def serve_mixed(ix, mix, n, q768, q256, p, a):
+    """Open-loop Poisson arrivals through the serving loop with a live state attached: 30% of
+    queries carry one of six allowlist filters (selectivity 0.1% .. 100%, log-uniform steps;
+    filter::search, plan resolved by size), and a writer inserts 1000 fresh rows every 10 ms
+    (100k rows/s) and deletes a tenth of them. Latency percentiles per request kind."""
+    import threading
+    import numpy as np
+    import torch
+    from sieve_amd import abi
+    from sieve_amd.build import normalize_rows
+    from sieve_amd.evaluate import pct
+    from sieve_amd.index import Filter, Live
+    from sieve_amd.server import Server
+    sel = [0.001, 0.004, 0.016, 0.063, 0.25, 1.0]
+    rng = np.random.default_rng(9)
+    filters, plans = [], []
+    for x in sel:
+        mask = (torch.rand(n, generator=torch.Generator().manual_seed(int(x * 1e6))) < x).numpy()
+        words = np.packbits(np.concatenate([mask, np.zeros((-n) % 64, bool)]), bitorder="little").view(np.uint64)
+        f = Filter(ix, words)
+        filters.append(f)
+        plans.append(int(abi.lib().sieve_plan_resolve(abi.PLANS["auto"], f.rows(), n)))
+    log(f"mixed serving: filters of {[f.rows() for f in filters]} rows, plans {plans}")
+    secs, rate = a.serve_mixed, a.serve_mixed_rate
+    batch_rows, period = 1000, 0.01
+    nw = int(secs / period) + 2
+    live = Live(ix, max_rows=nw * batch_rows + 1000)
+    fresh, _ = normalize_rows(mix.rows(n, nw * batch_rows).to("cuda"))
+    fresh = fresh.cpu().numpy().astype(np.float32)
+    q7h, q2h = q768[:2000].cpu().numpy(), q256[:2000].cpu().numpy()
+    sv = Server(ix, p, max_batch=a.serve_max_batch, live=live)
+    lat = {"plain": [], "filtered": [], "write": []}
+    tickets, lock, done = [], threading.Lock(), threading.Event()
+
+    def writer():
+        j = 0
+        t0 = time.perf_counter()
+        while not done.is_set() and j < nw:
+            ext = (3_000_000_000 + j * batch_rows + np.arange(batch_rows)).astype(np.uint32)
+            tk = sv.insert(ext, fresh[j * batch_rows:(j + 1) * batch_rows])
+            dels = ext[rng.choice(batch_rows, batch_rows // 10, replace=False)]
+            tk2 = sv.delete(dels)
+            with lock:
+                tickets.append(("write", tk))
+                tickets.append(("write", tk2))
+            j += 1
+            time.sleep(max(0.0, t0 + j * period - time.perf_counter()))
+
+    wt = threading.Thread(target=writer)
+    wt.start()
+    t0 = time.perf_counter()
+    nxt, i = t0, 0
+    while time.perf_counter() - t0 < secs:
+        now = time.perf_counter()
+        if now < nxt:
+            time.sleep(min(nxt - now, 0.001))
+            continue
+        j = i % len(q7h)
+        if rng.random() < 0.3:
+            tk = sv.submit_filtered(q7h[j], q2h[j], filters[rng.integers(len(filters))])
+            kind = "filtered"
+        else:
+            tk = sv.submit(q7h[j], q2h[j])
+            kind = "plain"
+        with lock:
+            tickets.append((kind, tk))
+        i += 1
+        nxt += rng.exponential(1.0 / rate)
+    done.set()
+    wt.join()
+    for kind, tk in tickets:
+        _, _, info = sv.wait(tk)
+        lat[kind].append(info.total_ms)
+    wall = time.perf_counter() - t0
+    st = sv.stats()
+    sv.close()
+    info = live.info()
+    live.close()
+    out = {"offered_qps": rate, "seconds": secs, "achieved_qps": round((len(lat["plain"]) + len(lat["filtered"])) / wall, 1),
+           "filter_selectivity": sel, "filter_plans": plans, "writes": st["writes"],
+           "rows_inserted": info["rows"], "tombstones": info["deleted"]}
+    for k, v in lat.items():
+        v.sort()
+        if v:
+            out[k] = {"requests": len(v), "p50_ms": round(pct(v, 0.5), 3), "p99_ms": round(pct(v, 0.99), 3)}
+    log(f"mixed serving: {out}")
+    return out
 
 
 def copy_probe():

@@ -390,7 +390,7 @@ uint32_t sieve_plan_resolve(uint32_t plan, uint64_t allowed_rows, uint64_t n);
  * p->k allowed rows as (score, row) best first. `expand_target` is EXPAND's slot target (AUTO
  * uses 200,000); the plan used is written to *resolved (may be NULL). EXPAND sorts every
  * centroid by its exact f32 score (at most 16384 centroids) and fails with SIEVE_ELIMIT if its
- * cut needs more than 1024 clusters. */
+ * cut needs more than 8192 clusters (or the scan's LDS at that many exceeds the device's). */
 int sieve_filter_search_batch(const sieve_index* ix, const sieve_filter* f, uint32_t plan, uint64_t expand_target,
                               sieve_stream* s, const float* q768, const float* q256, size_t nq, const sieve_params* p,
                               float* scores, uint32_t* rows, uint32_t* counts, uint32_t* resolved, sieve_timing* t);
