@@ -499,6 +499,7 @@ int sieve_candidates(const sieve_index* ix, sieve_stream* s, const float* q256, 
         s->last.queries = nq;
         s->last.scanned = hcnt[0];
         s->last.id_reads = hcnt[2];
+        s->last.rescanned = hcnt[3];
         for (size_t i = 0; i < nq; i++) s->last.probed += hn[i];
         s->last.scan_ns = elapsed_ns(s->ev[0], s->ev[1]);
         if (t) {
@@ -802,6 +803,18 @@ int sieve_index_set_scan_chunk(sieve_index* ix, uint32_t chunk_blocks) {
     return guarded([&] {
         SIEVE_REQUIRE(ix, SIEVE_EINVAL, "null index");
         ix->scan_chunk_blocks = chunk_blocks;
+    });
+}
+
+int sieve_index_set_cluster_scan(sieve_index* ix, uint32_t min_queries, uint32_t seed_slots, uint32_t ext_cap,
+                                 uint32_t chunk_blocks) {
+    return guarded([&] {
+        SIEVE_REQUIRE(ix, SIEVE_EINVAL, "null index");
+        SIEVE_REQUIRE(chunk_blocks <= (1u << 20), SIEVE_EINVAL, "chunk_blocks too large");
+        ix->cm_min_queries = min_queries ? min_queries : 64;
+        ix->cm_seed_slots = seed_slots;
+        ix->cm_ext_cap = ext_cap;
+        ix->cm_chunk_blocks = chunk_blocks ? chunk_blocks : 32;
     });
 }
 

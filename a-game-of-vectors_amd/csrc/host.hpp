@@ -109,6 +109,10 @@ struct sieve_index {
     struct sieve_live* live = nullptr;  // attached live state (sieve_live_open)
     int scan_variant = 0;  // scan kernel variant (sieve_index_autotune)
     uint32_t scan_chunk_blocks = 0;  // 0: chunk size chosen per batch (sieve_index_set_scan_chunk)
+    // Cluster-major scan (sieve_index_set_cluster_scan): batches of at least cm_min_queries
+    // unfiltered queries; seed pass of cm_seed_slots slots and appended lists of cm_ext_cap
+    // entries per query (0: 8 * cand each); cm_chunk_blocks blocks per work item.
+    uint32_t cm_min_queries = 64, cm_seed_slots = 0, cm_ext_cap = 0, cm_chunk_blocks = 32;
 
     ~sieve_index() {
         void* ps[] = {cents, cents_bf, block_off, cl_slots, codes, mul, err, ids, rotation, quant, norm256, t2_sigma,
@@ -150,7 +154,8 @@ struct sieve_l2_index {
 enum Slot {
     B_Q256, B_Q768, B_TABLES, B_QAB, B_WQ768, B_WQ256, B_INV, B_QBF, B_QNORM, B_APPROX, B_PS, B_PC, B_PN, B_CE, B_CR,
     B_CX, B_CN, B_OS, B_OR, B_ON, B_DEL, B_ALLOW, B_AFLAGS, B_CNT, B_OVF, B_TMP1, B_TMP2, B_TMP3, B_QQ, B_QF,
-    B_PART, B_PARTN, B_GTHR, B_QDONE,
+    B_PART, B_PARTN, B_GTHR, B_QDONE, B_CM_LIM, B_CM_SEGC, B_CM_QOFF, B_CM_IOFF, B_CM_Q, B_CM_CENT, B_CM_SEED,
+    B_CM_SEEDN, B_CM_THR, B_CM_EXT, B_CM_EXTN, B_CM_WORK,
     B_NSLOTS
 };
 
@@ -384,11 +389,40 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
     a.out_n = cn;
     a.out_stride = cstride;
     a.scanned = scanned;
-    set_scan_plan(a, s, nq, (uint32_t)pstride, (double)ix->nblocks / ix->k, ix->max_cluster_blocks,
-                  ix->scan_chunk_blocks);
     const size_t lds = scan_lds_bytes(a.cand, scan_segments(a));
     SIEVE_REQUIRE(lds <= 160 * 1024, SIEVE_ELIMIT, "scan workspace exceeds LDS");
-    launch_scan(a, nq, ix->scan_variant, s->st);
+    int variant = ix->scan_variant;
+    if (variant >= SCAN_VARIANT_CM) {
+        // Small or filtered batches share too little of their probed clusters: per-query scan.
+        if (nq >= ix->cm_min_queries && !view.allow) {
+            const uint32_t cand = (uint32_t)a.cand;
+            CmScratch cm{};
+            cm.k = ix->k;
+            cm.nseg = a.dblock_off ? 2 * ix->k : ix->k;
+            cm.seg_count = grow<uint32_t>(s->b[B_CM_SEGC], cm.nseg);
+            cm.seg_qoff = grow<uint32_t>(s->b[B_CM_QOFF], cm.nseg + 1);
+            cm.seg_ioff = grow<uint32_t>(s->b[B_CM_IOFF], cm.nseg + 1);
+            cm.seg_q = grow<uint32_t>(s->b[B_CM_Q], nq * (size_t)pstride * (a.dblock_off ? 2 : 1));
+            cm.seg_cent = grow<float>(s->b[B_CM_CENT], nq * (size_t)pstride * (a.dblock_off ? 2 : 1));
+            cm.probe_lim = grow<uint32_t>(s->b[B_CM_LIM], nq);
+            cm.work = grow<uint32_t>(s->b[B_CM_WORK], 1);
+            cm.seed_slots = ix->cm_seed_slots ? ix->cm_seed_slots : 8 * cand;
+            cm.chunk_blocks = std::max<uint32_t>(1, ix->cm_chunk_blocks);
+            a.ext_cap = ix->cm_ext_cap ? ix->cm_ext_cap : 8 * cand;
+            a.seed = grow<uint2>(s->b[B_CM_SEED], nq * (size_t)cand);
+            a.seed_n = grow<uint32_t>(s->b[B_CM_SEEDN], nq);
+            a.seed_thr = grow<unsigned long long>(s->b[B_CM_THR], nq);
+            a.ext = grow<uint2>(s->b[B_CM_EXT], nq * (size_t)a.ext_cap);
+            a.ext_n = grow_zeroed<uint32_t>(s->b[B_CM_EXTN], nq, s->st);  // the finish pass leaves it zero
+            launch_scan_cm(a, cm, nq, variant, s->st);
+            SIEVE_HIP(hipGetLastError());
+            return;
+        }
+        variant = 2;
+    }
+    set_scan_plan(a, s, nq, (uint32_t)pstride, (double)ix->nblocks / ix->k, ix->max_cluster_blocks,
+                  ix->scan_chunk_blocks);
+    launch_scan(a, nq, variant, s->st);
     SIEVE_HIP(hipGetLastError());
 }
 
@@ -482,6 +516,7 @@ inline void finish_search(sieve_stream* s, size_t nq, sieve_timing* t) {
     s->last.scanned = hcnt[0];
     s->last.id_reads = hcnt[2];
     s->last.reranked = hcnt[1];
+    s->last.rescanned = hcnt[3];
     for (uint32_t x : hpn) s->last.probed += x;
     s->last.route_ns = elapsed_ns(s->ev[0], s->ev[1]);
     s->last.scan_ns = elapsed_ns(s->ev[1], s->ev[2]);

@@ -128,6 +128,18 @@ struct ScanArgs {
     uint32_t dbase;
     int seg_stride;              // LDS segment table entries: scan_segments(), set by the launcher
     int trigger;                 // buffered entries that trigger a cut to cand (set by the launcher)
+    // Cluster-major scan (launch_scan_cm): the seed pass scans query q's first probe_lim[q]
+    // probes and leaves its kept entries in seed (nq x cand, seed_n) and their cand-th best
+    // (key, ~slot) in seed_thr (0: fewer than cand); the cluster pass appends every later slot
+    // above seed_thr to ext (nq x ext_cap, ext_n); the finish pass merges both lists, or rescans
+    // a query whose ext list overflowed.
+    const uint32_t* probe_lim;
+    uint2* seed;
+    uint32_t* seed_n;
+    unsigned long long* seed_thr;
+    uint2* ext;
+    uint32_t* ext_n;
+    uint32_t ext_cap;
 };
 // Block segments per probe: the base list, and with live delta lists the delta list too.
 __host__ __device__ inline int scan_segments(const ScanArgs& a) { return a.dblock_off ? 2 * a.probe_stride : a.probe_stride; }
@@ -141,10 +153,33 @@ ScanPlan plan_scan(size_t nq, uint32_t nprobe, double mean_blocks, uint32_t max_
                    uint32_t forced_chunk_blocks);
 // Upper bound of the scan's LDS over all variants.
 size_t scan_lds_bytes(int cand, int probe_stride);
-// Scan kernel variants (slots per thread, prefetch), picked per index by the autotuner.
-constexpr int SCAN_VARIANTS = 10;
+// Scan kernel variants (slots per thread, prefetch), picked per index by the autotuner; the
+// last one is the cluster-major scan (launch_scan_cm).
+constexpr int SCAN_VARIANTS = 12;
+constexpr int SCAN_VARIANT_CM = 10;  // 10: one-hot operands from LDS, 11: built in registers
 const char* scan_variant_name(int variant);
 void launch_scan(const ScanArgs& a, size_t nq, int variant, hipStream_t st);
+
+// Cluster-major scan of a batch: every (segment, 32-query tile, block chunk) work item reads
+// the segment's codes once for up to 32 queries and scores them with int8 MFMA (one-hot
+// nibbles x (table - 128)). Scratch per batch, all device memory:
+struct CmScratch {
+    uint32_t k;                  // clusters
+    uint32_t nseg;               // k, or 2k with live delta lists (segment k + c: c's delta list)
+    size_t nq;                   // set by the launcher
+    uint32_t* seg_count;         // nseg
+    uint32_t* seg_qoff;          // nseg + 1: (query, segment) pairs per segment, prefix
+    uint32_t* seg_ioff;          // nseg + 1: work items per segment, prefix
+    uint32_t* seg_q;             // nq * probe_stride (segments x their queries)
+    float* seg_cent;
+    uint32_t* probe_lim;         // nq
+    uint32_t* work;              // 1: work-item counter (reset by the launch)
+    uint32_t seed_slots;         // slots the seed pass scans per query (whole probes)
+    uint32_t chunk_blocks;       // blocks per work item
+};
+// Enqueues the seed pass, the work-list build, the cluster pass and the finish pass. `a` carries
+// the index, batch and output fields and seed / ext buffers; its chunking is ignored.
+void launch_scan_cm(const ScanArgs& a, const CmScratch& cm, size_t nq, int variant, hipStream_t st);
 
 // The standalone lut::scan seam over the reference's block layout.
 void launch_lut_scan(const uint8_t* tables, int groups, const uint8_t* blocks, size_t nblocks, uint16_t* out,

@@ -118,7 +118,12 @@ __device__ void scan_compact(uint2* buf, uint2* tmp, int n, int k, uint32_t* his
 // SLOTS_PER_THREAD slots per thread per round; PF: prefetch the next round's codes into
 // registers while scoring the current one (more bytes in flight, more VGPRs); WPE: minimum
 // waves per SIMD the register allocation must allow (0: the compiler's choice).
-template <int G, bool L2, int SLOTS_PER_THREAD, bool PF, int WPE>
+// MODE: SCAN_FULL scans every probe; SCAN_SEED scans the first probe_lim[q] probes and leaves
+// the kept list for the cluster pass; SCAN_FINISH merges the seed and cluster-pass lists (or
+// rescans the query when its cluster-pass list overflowed) and writes the output.
+constexpr int SCAN_FULL = 0, SCAN_SEED = 1, SCAN_FINISH = 2;
+
+template <int G, bool L2, int SLOTS_PER_THREAD, bool PF, int WPE, int MODE = SCAN_FULL>
 __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_scan_t(ScanArgs a) {
     constexpr int CW = G / 32;  // 16-byte words per slot code
@@ -142,7 +147,14 @@ void k_scan_t(ScanArgs a) {
     // Block segments in probe order: each probed cluster's base list, then (live) its delta
     // list (src/ivf.rs:682-684).
     const bool live = a.dblock_off != nullptr;
-    const int np = (live ? 2 : 1) * (int)a.probe_n[qi];
+    int nprobed = (int)a.probe_n[qi];
+    bool ext_ok = false;  // SCAN_FINISH: the cluster pass's list is whole (no rescan)
+    if constexpr (MODE == SCAN_SEED) nprobed = min(nprobed, (int)a.probe_lim[qi]);
+    if constexpr (MODE == SCAN_FINISH) {
+        ext_ok = a.ext_n[qi] <= a.ext_cap;
+        if (ext_ok) nprobed = 0;
+    }
+    const int np = (live ? 2 : 1) * nprobed;
     for (int j = t; j < np; j += SCAN_THREADS) {
         const int pj = live ? j >> 1 : j;
         const uint32_t c = a.probe_c[qi * a.probe_stride + pj];
@@ -318,6 +330,28 @@ void k_scan_t(ScanArgs a) {
     __syncthreads();
     Regs ra, rb;
     uint32_t round = 0;
+    // Streams m listed entries through the buffer under the running threshold.
+    auto merge_list = [&](const uint2* lst, int m) {
+        for (int i0 = 0; i0 < m; i0 += SCAN_THREADS * SLOTS_PER_THREAD) {
+            const bool has_thr = misc[M_HAS] != 0;
+            const uint32_t thr = misc[M_THR], thr_lo = misc[M_THR_LO];
+#pragma unroll
+            for (int u = 0; u < SLOTS_PER_THREAD; u++) {
+                const int i = i0 + u * SCAN_THREADS + t;
+                uint2 e = make_uint2(0u, 0u);
+                bool pass = false;
+                if (i < m) {
+                    e = lst[i];
+                    pass = !has_thr || e.x > thr || (e.x == thr && e.y > thr_lo);
+                }
+                admit(pass, e.x, e.y, round);
+            }
+            __syncthreads();
+            if (misc[M_STAMP + (round & 1)] == round)
+                scan_compact(buf, tmp, (int)misc[M_COUNT], cand, hist, misc, nullptr);
+            round++;
+        }
+    };
     if (PF && vb_lo < vb_hi) fetch(vb_lo, ra);
     for (uint32_t vb0 = vb_lo; vb0 < vb_hi;) {
         step(ra, rb, vb0, round++);
@@ -346,34 +380,48 @@ void k_scan_t(ScanArgs a) {
         // Stream the other chunks' entries through the buffer (own entries are in it already).
         for (uint32_t c = 0; c < nch; c++) {
             if (c == ci) continue;
-            const uint2* lst = a.part + (slot0 + c) * (size_t)cand;
             const int m = (int)min((uint32_t)cand, __hip_atomic_load(a.part_n + slot0 + c, __ATOMIC_RELAXED,
                                                                      __HIP_MEMORY_SCOPE_AGENT));
-            for (int i0 = 0; i0 < m; i0 += SCAN_THREADS * SLOTS_PER_THREAD) {
-                const bool has_thr = misc[M_HAS] != 0;
-                const uint32_t thr = misc[M_THR], thr_lo = misc[M_THR_LO];
-#pragma unroll
-                for (int u = 0; u < SLOTS_PER_THREAD; u++) {
-                    const int i = i0 + u * SCAN_THREADS + t;
-                    uint2 e = make_uint2(0u, 0u);
-                    bool pass = false;
-                    if (i < m) {
-                        e = lst[i];
-                        pass = !has_thr || e.x > thr || (e.x == thr && e.y > thr_lo);
-                    }
-                    admit(pass, e.x, e.y, round);
-                }
-                __syncthreads();
-                if (misc[M_STAMP + (round & 1)] == round)
-                    scan_compact(buf, tmp, (int)misc[M_COUNT], cand, hist, misc, nullptr);
-                round++;
-            }
+            merge_list(a.part + (slot0 + c) * (size_t)cand, m);
         }
         if ((int)misc[M_COUNT] > cand) scan_compact(buf, tmp, (int)misc[M_COUNT], cand, hist, misc, nullptr);
         if (t == 0) {  // ready for the next launch
             __hip_atomic_store(a.qdone + qi, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(gthr, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+    }
+    if constexpr (MODE == SCAN_FINISH) {
+        if (ext_ok) {
+            merge_list(a.seed + qi * (size_t)cand, (int)min((uint32_t)cand, a.seed_n[qi]));
+            merge_list(a.ext + qi * (size_t)a.ext_cap, (int)a.ext_n[qi]);
+            if ((int)misc[M_COUNT] > cand) scan_compact(buf, tmp, (int)misc[M_COUNT], cand, hist, misc, nullptr);
+        } else if (t == 0 && a.scanned) {
+            atomicAdd(a.scanned + 3, 1ull);  // queries rescanned
+        }
+        __syncthreads();
+        if (t == 0) a.ext_n[qi] = 0;  // ready for the next launch
+    }
+    if constexpr (MODE == SCAN_SEED) {
+        // The kept list and, when it is full, its worst entry: cand admitted slots compare at
+        // least as high, so a later slot at or below it cannot enter the final list.
+        const int n = (int)misc[M_COUNT];
+        uint2* dst = a.seed + qi * (size_t)cand;
+        unsigned long long lo = ~0ull;
+        for (int i = t; i < n; i += SCAN_THREADS) {
+            dst[i] = buf[i];
+            lo = min(lo, pack_pair(buf[i]));
+        }
+        lo = wave_min_u64(lo);
+        __syncthreads();
+        unsigned long long* red = reinterpret_cast<unsigned long long*>(hist);
+        if (lane == 0) red[wave] = lo;
+        __syncthreads();
+        if (t == 0) {
+            for (int w = 1; w < SCAN_THREADS / 64; w++) lo = min(lo, red[w]);
+            a.seed_n[qi] = (uint32_t)n;
+            a.seed_thr[qi] = n >= cand ? lo : 0ull;
+        }
+        return;
     }
 
     const int n = (int)misc[M_COUNT];
@@ -487,7 +535,8 @@ const char* scan_variant_name(int v) {
     static const char* names[SCAN_VARIANTS] = {
         "4 slots/thread", "2 slots/thread + prefetch", "2 slots/thread, cut at 1.5 cand", "1 slot/thread, cut at 1.5 cand",
         "2 slots/thread + prefetch, >= 5 waves/SIMD", "2 slots/thread", "1 slot/thread + prefetch", "1 slot/thread",
-        "1 slot/thread + prefetch, cut at 1.5 cand", "1 slot/thread, cut at 1.25 cand"};
+        "1 slot/thread + prefetch, cut at 1.5 cand", "1 slot/thread, cut at 1.25 cand",
+        "cluster-major MFMA, one-hot from LDS", "cluster-major MFMA, one-hot in registers"};
     return v >= 0 && v < SCAN_VARIANTS ? names[v] : "?";
 }
 
@@ -499,6 +548,373 @@ void launch_scan(const ScanArgs& a, size_t nq, int variant, hipStream_t st) {
 void launch_scan_l2(const ScanArgs& a, size_t nq, int variant, hipStream_t st) {
     if (nq == 0) return;
     launch_any<32, true>(a, nq, variant, st);
+}
+
+// ------------------------------------------------------------ cluster-major scan -----
+// In a large batch a stored slot is probed by many queries (tens, at 10,000 queries over 16,384
+// clusters), and the per-query scan reads its code once for each. The cluster-major pass reads
+// a block chunk of one segment (a cluster's base or delta list) once for a tile of up to 32 of
+// the queries that probe it and scores the tile with int8 MFMA: for 16 slots x 16 queries x 4
+// groups per v_mfma_i32_16x16x64_i8, A holds the one-hot bytes of each slot's nibble per group
+// and B the query's table entries minus 128, so the product sums T[g][nibble_g] - 128 over the
+// 64 groups: score_code (src/lut.rs:83-86) minus 64 * 128, exactly (|sum| <= 8192 in i32).
+//
+// The pass needs each query's admission threshold up front. The seed pass (k_scan_t in
+// SCAN_SEED mode) scans the query's nearest probes (whole clusters, at least seed_slots slots)
+// and keeps their best cand entries; the cand-th of those is a lower bound of the final
+// cand-th best, so a later slot at or below it never enters the result, and the cluster pass
+// appends only (est, slot) pairs above it. The finish pass merges the seed list and the
+// appended list through the same TopK buffer as the per-query scan (the union holds every
+// final entry, and the order (est desc, slot asc) is total, so the kept set is the same), or,
+// if the appended list overflowed its capacity, rescans the query in full.
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+struct CmArgs {
+    const uint4* codes;
+    const float* mul;
+    const uint32_t* ids;
+    const uint32_t* block_off;
+    const uint32_t* dblock_off;
+    uint32_t dbase;
+    uint32_t k;
+    const uint8_t* tables;
+    const float4* qab;
+    const uint32_t* seg_qoff;
+    const uint32_t* seg_ioff;
+    const uint32_t* seg_q;
+    const float* seg_cent;
+    uint32_t nseg;
+    const unsigned long long* thr;
+    ViewArgs view;
+    uint2* ext;
+    uint32_t* ext_n;
+    uint32_t ext_cap;
+    uint32_t* work;
+    unsigned long long* scanned;
+    uint32_t chunk_blocks;
+};
+
+__device__ __forceinline__ void cm_segment(const uint32_t* block_off, const uint32_t* dblock_off, uint32_t dbase,
+                                           uint32_t k, uint32_t sg, uint32_t& b0, uint32_t& nb) {
+    if (sg < k) {
+        b0 = block_off[sg];
+        nb = block_off[sg + 1] - b0;
+    } else {
+        const uint32_t d0 = dblock_off[sg - k];
+        b0 = dbase + d0;
+        nb = dblock_off[sg - k + 1] - d0;
+    }
+}
+
+__global__ void k_cm_zero(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// One wave per query: the seed pass's probe count (whole probes, in probe order, until
+// seed_slots stored slots) and the per-segment counts of the later probes.
+__global__ __launch_bounds__(256) void k_cm_prep(ScanArgs a, CmScratch cm) {
+    const size_t q = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (q >= cm.nq) return;
+    const int pn = (int)a.probe_n[q];
+    uint32_t run = 0, lim = (uint32_t)pn;
+    bool found = false;
+    for (int base = 0; base < pn && !found; base += 64) {
+        const int p = base + lane;
+        uint32_t nb = 0;
+        if (p < pn) {
+            const uint32_t c = a.probe_c[q * a.probe_stride + p];
+            nb = a.block_off[c + 1] - a.block_off[c];
+            if (a.dblock_off) nb += a.dblock_off[c + 1] - a.dblock_off[c];
+        }
+        const uint32_t incl = wave_incl_scan(nb) + run;
+        const uint64_t hit = __ballot(p < pn && (uint64_t)incl * BLOCK >= cm.seed_slots);
+        if (hit) {
+            lim = (uint32_t)(base + __ffsll((long long)hit));  // through the first probe that reaches it
+            found = true;
+        }
+        run = __shfl(incl, 63, 64);
+    }
+    if (lane == 0) cm.probe_lim[q] = lim;
+    for (int p = (int)lim + lane; p < pn; p += 64) {
+        const uint32_t c = a.probe_c[q * a.probe_stride + p];
+        if (a.block_off[c + 1] > a.block_off[c]) atomicAdd(cm.seg_count + c, 1u);
+        if (a.dblock_off && a.dblock_off[c + 1] > a.dblock_off[c]) atomicAdd(cm.seg_count + cm.k + c, 1u);
+    }
+}
+
+// One workgroup: prefix sums of the per-segment query counts and work items; the counts are
+// cleared for the scatter, which uses them as cursors.
+__global__ __launch_bounds__(SEL_THREADS) void k_cm_offsets(ScanArgs a, CmScratch cm) {
+    __shared__ uint32_t scratch[16];
+    const int t = threadIdx.x;
+    const uint32_t n = cm.nseg, per = (n + SEL_THREADS - 1) / SEL_THREADS;
+    const uint32_t lo = min(n, t * per), hi = min(n, lo + per);
+    uint32_t sq = 0, si = 0;
+    for (uint32_t sg = lo; sg < hi; sg++) {
+        uint32_t b0, nb;
+        cm_segment(a.block_off, a.dblock_off, a.dbase, cm.k, sg, b0, nb);
+        const uint32_t qc = cm.seg_count[sg];
+        sq += qc;
+        si += ((qc + 31) / 32) * ((nb + cm.chunk_blocks - 1) / cm.chunk_blocks);
+    }
+    uint32_t tq, ti;
+    uint32_t oq = block_excl_scan(sq, scratch, &tq);
+    uint32_t oi = block_excl_scan(si, scratch + 8, &ti);
+    for (uint32_t sg = lo; sg < hi; sg++) {
+        uint32_t b0, nb;
+        cm_segment(a.block_off, a.dblock_off, a.dbase, cm.k, sg, b0, nb);
+        const uint32_t qc = cm.seg_count[sg];
+        cm.seg_qoff[sg] = oq;
+        cm.seg_ioff[sg] = oi;
+        oq += qc;
+        oi += ((qc + 31) / 32) * ((nb + cm.chunk_blocks - 1) / cm.chunk_blocks);
+        cm.seg_count[sg] = 0;
+    }
+    if (t == 0) {
+        cm.seg_qoff[n] = tq;
+        cm.seg_ioff[n] = ti;
+        *cm.work = 0;
+    }
+}
+
+// One wave per query: lists each later probe's query and centroid score under its segment.
+__global__ __launch_bounds__(256) void k_cm_scatter(ScanArgs a, CmScratch cm) {
+    const size_t q = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (q >= cm.nq) return;
+    const int pn = (int)a.probe_n[q];
+    for (int p = (int)cm.probe_lim[q] + lane; p < pn; p += 64) {
+        const uint32_t c = a.probe_c[q * a.probe_stride + p];
+        const float cent = a.probe_s[q * a.probe_stride + p];
+        if (a.block_off[c + 1] > a.block_off[c]) {
+            const uint32_t pos = cm.seg_qoff[c] + atomicAdd(cm.seg_count + c, 1u);
+            cm.seg_q[pos] = (uint32_t)q;
+            cm.seg_cent[pos] = cent;
+        }
+        if (a.dblock_off && a.dblock_off[c + 1] > a.dblock_off[c]) {
+            const uint32_t sg = cm.k + c;
+            const uint32_t pos = cm.seg_qoff[sg] + atomicAdd(cm.seg_count + sg, 1u);
+            cm.seg_q[pos] = (uint32_t)q;
+            cm.seg_cent[pos] = cent;
+        }
+    }
+}
+
+// The per-query constants of one C column (query) of a tile half.
+struct CmCol {
+    float a, b, cent, thrf;
+    unsigned long long thr;
+    uint32_t q;
+};
+
+// ONEHOT_LDS: the one-hot operand is read from a 16-entry LDS table (2 VALU + 1 LDS read per
+// MFMA step); otherwise it is built with shifts and selects (about 8 VALU).
+template <bool ONEHOT_LDS>
+__global__ __launch_bounds__(256) void k_scan_cm(CmArgs a) {
+    __shared__ uint4 onehot[16];
+    const int t = threadIdx.x, lane = t & 63;
+    if (t < 16) {
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+        w[t >> 2] = 1u << (8 * (t & 3));
+        onehot[t] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    __syncthreads();
+    const int r = lane >> 4, j = lane & 15;
+    const uint32_t total = a.seg_ioff[a.nseg];
+    uint32_t id_reads = 0;
+    for (;;) {
+        uint32_t idx = 0;
+        if (lane == 0) idx = atomicAdd(a.work, 1u);
+        idx = __shfl(idx, 0, 64);
+        if (idx >= total) break;
+        // The segment holding work item idx (the last with seg_ioff <= idx is non-empty).
+        uint32_t lo = 0, hi = a.nseg - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (a.seg_ioff[mid] <= idx) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t sg = lo;
+        uint32_t b0, nb;
+        cm_segment(a.block_off, a.dblock_off, a.dbase, a.k, sg, b0, nb);
+        const uint32_t q0 = a.seg_qoff[sg], qc = a.seg_qoff[sg + 1] - q0;
+        const uint32_t tiles = (qc + 31) / 32;
+        const uint32_t local = idx - a.seg_ioff[sg];
+        const uint32_t tile = local % tiles, chunk = local / tiles;
+        const uint32_t blo = chunk * a.chunk_blocks, bhi = min(nb, blo + a.chunk_blocks);
+        const uint32_t nvalid = min(32u, qc - tile * 32);
+        const bool two = nvalid > 16;
+        // B operands: lane (j, r) holds query j's table entries of groups 4 st + r, minus 128.
+        uint4 B0[16], B1[16];
+        CmCol col[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t n = 16 * h + j;
+            const bool valid = n < nvalid;
+            const uint32_t pos = q0 + tile * 32 + (valid ? n : 0);
+            const uint32_t q = a.seg_q[pos];
+            const float4 qab = a.qab[q];
+            col[h].a = qab.x;
+            col[h].b = qab.y;
+            col[h].cent = a.seg_cent[pos];
+            col[h].q = q;
+            col[h].thr = valid ? a.thr[q] : ~0ull;
+            col[h].thrf = !valid ? INFINITY : col[h].thr == 0 ? -INFINITY : key2f((uint32_t)(col[h].thr >> 32));
+            if (h == 1 && !two) continue;
+            const uint4* tp = reinterpret_cast<const uint4*>(a.tables + (size_t)q * 1024);
+#pragma unroll
+            for (int st = 0; st < 16; st++) {
+                uint4 v = tp[4 * st + r];
+                v.x ^= 0x80808080u;
+                v.y ^= 0x80808080u;
+                v.z ^= 0x80808080u;
+                v.w ^= 0x80808080u;
+                if (h == 0) B0[st] = v;
+                else B1[st] = v;
+            }
+        }
+        if (lane == 0 && a.scanned) atomicAdd(a.scanned, (unsigned long long)(bhi - blo) * BLOCK * nvalid);
+
+        // Lane (j, r) of C holds slots 4 r .. 4 r + 3 of the 16-slot tile for query column j.
+        auto epilogue = [&](const v4i& acc, const CmCol& cc, uint32_t slot0, const float4& mu) {
+            const float m4[4] = {mu.x, mu.y, mu.z, mu.w};
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint32_t sc = (uint32_t)(acc[i] + 64 * 128);
+                float est = cc.cent + m4[i] * (cc.a * (float)sc + cc.b);
+                est = est + 0.0f;
+                if (est >= cc.thrf) {
+                    const uint32_t slot = slot0 + 4 * r + i;
+                    const uint32_t key = f2key(est);
+                    if ((((unsigned long long)key << 32) | (unsigned long long)(~slot)) > cc.thr) {
+                        id_reads++;
+                        const uint32_t id = a.ids[slot];
+                        bool ok = id != PAD;
+                        if (ok && a.view.deleted && bit_set(a.view.deleted, a.view.deleted_words, id)) ok = false;
+                        if (ok) {
+                            const uint32_t at = atomicAdd(a.ext_n + cc.q, 1u);
+                            if (at < a.ext_cap) a.ext[(size_t)cc.q * a.ext_cap + at] = make_uint2(key, ~slot);
+                        }
+                    }
+                }
+            }
+        };
+        auto tile16 = [&](const uint4 (&cw)[2], const float4& mu, uint32_t slot0, auto two_c) {
+            constexpr bool TWO = decltype(two_c)::value;
+            const uint32_t w[8] = {cw[0].x, cw[0].y, cw[0].z, cw[0].w, cw[1].x, cw[1].y, cw[1].z, cw[1].w};
+            v4i acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+#pragma unroll
+            for (int st = 0; st < 16; st++) {
+                // Group 4 st + r of the slot: word st / 2, bits 16 (st & 1) + 4 r.
+                const uint32_t nib = (w[st >> 1] >> (16 * (st & 1) + 4 * r)) & 15u;
+                v4i av;
+                if constexpr (ONEHOT_LDS) {
+                    const uint4 oh = onehot[nib];
+                    av = v4i{(int)oh.x, (int)oh.y, (int)oh.z, (int)oh.w};
+                } else {
+                    const unsigned long long v = 1ull << ((nib & 7u) * 8u);
+                    const bool hi8 = nib >= 8u;
+                    av = v4i{hi8 ? 0 : (int)(uint32_t)v, hi8 ? 0 : (int)(uint32_t)(v >> 32), hi8 ? (int)(uint32_t)v : 0,
+                             hi8 ? (int)(uint32_t)(v >> 32) : 0};
+                }
+                const v4i b0v = {(int)B0[st].x, (int)B0[st].y, (int)B0[st].z, (int)B0[st].w};
+                acc0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b0v, acc0, 0, 0, 0);
+                if constexpr (TWO) {
+                    const v4i b1v = {(int)B1[st].x, (int)B1[st].y, (int)B1[st].z, (int)B1[st].w};
+                    acc1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b1v, acc1, 0, 0, 0);
+                }
+            }
+            epilogue(acc0, col[0], slot0, mu);
+            if constexpr (TWO) epilogue(acc1, col[1], slot0, mu);
+        };
+        // 16-slot tiles of the chunk in order; the next tile's codes load during this one.
+        const uint32_t s_lo = (b0 + blo) * BLOCK, s_hi = (b0 + bhi) * BLOCK;
+        uint4 cw[2], nw[2];
+        float4 mu, nmu;
+        auto load = [&](uint32_t slot0, uint4 (&c)[2], float4& m) {
+            const uint4* cp = a.codes + 2 * (size_t)(slot0 + j);
+            c[0] = cp[0];
+            c[1] = cp[1];
+            m = *reinterpret_cast<const float4*>(a.mul + slot0 + 4 * r);
+        };
+        if (s_lo < s_hi) load(s_lo, cw, mu);
+        for (uint32_t s0 = s_lo; s0 < s_hi; s0 += 16) {
+            if (s0 + 16 < s_hi) load(s0 + 16, nw, nmu);
+            if (two) tile16(cw, mu, s0, std::true_type{});
+            else tile16(cw, mu, s0, std::false_type{});
+            cw[0] = nw[0];
+            cw[1] = nw[1];
+            mu = nmu;
+        }
+    }
+    id_reads = wave_sum(id_reads);
+    if (lane == 0 && a.scanned) atomicAdd(a.scanned + 2, (unsigned long long)id_reads);
+}
+
+void launch_scan_cm(const ScanArgs& a_in, const CmScratch& cm_in, size_t nq, int variant, hipStream_t st) {
+    if (nq == 0) return;
+    CmScratch cm = cm_in;
+    cm.nq = nq;
+    ScanArgs a = a_in;
+    a.chunk_max = 1;
+    a.chunk_blocks = 0xFFFFFFFFu;
+    a.seg_stride = scan_segments(a);
+    a.trigger = std::max(a.cand + 1, a.cand * 6 / 4);
+    a.probe_lim = cm.probe_lim;
+    const size_t lds = scan_layout(a.cand, a.seg_stride, 2, a.trigger).total;
+    hipLaunchKernelGGL(k_cm_zero, dim3((cm.nseg + 255) / 256), dim3(256), 0, st, cm.seg_count, cm.nseg);
+    const unsigned qgrid = (unsigned)((nq + 3) / 4);
+    hipLaunchKernelGGL(k_cm_prep, dim3(qgrid), dim3(256), 0, st, a, cm);
+    hipLaunchKernelGGL(k_cm_offsets, dim3(1), dim3(SEL_THREADS), 0, st, a, cm);
+    hipLaunchKernelGGL(k_cm_scatter, dim3(qgrid), dim3(256), 0, st, a, cm);
+    if (lds > 64 * 1024) {
+        (void)hipFuncSetAttribute((const void*)k_scan_t<64, false, 2, false, 0, SCAN_SEED>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipFuncSetAttribute((const void*)k_scan_t<64, false, 2, false, 0, SCAN_FINISH>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    }
+    hipLaunchKernelGGL((k_scan_t<64, false, 2, false, 0, SCAN_SEED>), dim3((unsigned)nq), dim3(SCAN_THREADS), lds,
+                       st, a);
+
+    CmArgs c{};
+    c.codes = a.codes;
+    c.mul = a.mul;
+    c.ids = a.ids;
+    c.block_off = a.block_off;
+    c.dblock_off = a.dblock_off;
+    c.dbase = a.dbase;
+    c.k = cm.k;
+    c.tables = a.tables;
+    c.qab = a.qab;
+    c.seg_qoff = cm.seg_qoff;
+    c.seg_ioff = cm.seg_ioff;
+    c.seg_q = cm.seg_q;
+    c.seg_cent = cm.seg_cent;
+    c.nseg = cm.nseg;
+    c.thr = a.seed_thr;
+    c.view = a.view;
+    c.ext = a.ext;
+    c.ext_n = a.ext_n;
+    c.ext_cap = a.ext_cap;
+    c.work = cm.work;
+    c.scanned = a.scanned;
+    c.chunk_blocks = cm.chunk_blocks;
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (cus <= 0) cus = 256;
+    }
+    if (variant == SCAN_VARIANT_CM)
+        hipLaunchKernelGGL(k_scan_cm<true>, dim3((unsigned)cus * 4), dim3(256), 0, st, c);
+    else
+        hipLaunchKernelGGL(k_scan_cm<false>, dim3((unsigned)cus * 4), dim3(256), 0, st, c);
+    hipLaunchKernelGGL((k_scan_t<64, false, 2, false, 0, SCAN_FINISH>), dim3((unsigned)nq), dim3(SCAN_THREADS), lds,
+                       st, a);
 }
 
 // ------------------------------------------------------------------ lut::scan seam -----

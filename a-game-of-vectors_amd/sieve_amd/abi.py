@@ -57,7 +57,7 @@ class View(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("queries", ctypes.c_uint64), ("probed", ctypes.c_uint64), ("scanned", ctypes.c_uint64),
                 ("id_reads", ctypes.c_uint64), ("reranked", ctypes.c_uint64), ("route_ns", ctypes.c_double),
-                ("scan_ns", ctypes.c_double), ("rerank_ns", ctypes.c_double)]
+                ("scan_ns", ctypes.c_double), ("rerank_ns", ctypes.c_double), ("rescanned", ctypes.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -112,7 +112,9 @@ _lib = None
 # filter::Plan tags (SIEVE_PLAN_*).
 PLANS = {"post": 0, "expand": 1, "brute": 2, "auto": 3}
 # Scan kernel variants the autotuner times (SIEVE_SCAN_VARIANTS).
-SCAN_VARIANTS = 10
+SCAN_VARIANTS = 12
+# The cluster-major scan variants (sieve_index_set_cluster_scan).
+SCAN_VARIANTS_CM = (10, 11)
 
 def header_symbols():
     """Every function the public header declares."""
@@ -157,6 +159,7 @@ def lib():
         "sieve_index_autotune": (i32, [vp, vp, vp, vp, sz, P, ctypes.POINTER(ctypes.c_uint32), vp]),
         "sieve_index_set_scan_variant": (i32, [vp, u32]),
         "sieve_index_set_scan_chunk": (i32, [vp, u32]),
+        "sieve_index_set_cluster_scan": (i32, [vp, u32, u32, u32, u32]),
         "sieve_l2_index_set_scan_chunk": (i32, [vp, u32]),
         "sieve_server_create": (i32, [vp, P, ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)]),
         "sieve_server_submit": (i32, [vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)]),

@@ -148,6 +148,7 @@ typedef struct sieve_stats {
     double route_ns;
     double scan_ns;
     double rerank_ns;
+    uint64_t rescanned;  /* cluster-major scan: queries whose appended list overflowed       */
 } sieve_stats;
 
 typedef struct sieve_index sieve_index;
@@ -202,11 +203,12 @@ void sieve_index_free(sieve_index* ix);
 int sieve_index_get_info(const sieve_index* ix, sieve_index_info* out);
 
 /* Scan-kernel autotuner (no counterpart in the reference): times each variant of the scan
- * kernel (slots per thread per round, prefetch of the next round's codes) on this batch of
- * queries with these parameters, and keeps the fastest for later calls on this index. Call it
- * before sharing the index across threads. chosen (may be NULL) gets the variant, and
- * ms_per_variant (NULL or SIEVE_SCAN_VARIANTS floats) the time of each. */
-#define SIEVE_SCAN_VARIANTS 10
+ * kernel (slots per thread per round, prefetch of the next round's codes; variants 10 and 11
+ * are the cluster-major scan) on this batch of queries with these parameters, and keeps the
+ * fastest for later calls on this index. Call it before sharing the index across threads.
+ * chosen (may be NULL) gets the variant, and ms_per_variant (NULL or SIEVE_SCAN_VARIANTS
+ * floats) the time of each. */
+#define SIEVE_SCAN_VARIANTS 12
 int sieve_index_autotune(sieve_index* ix, sieve_stream* s, const float* q768, const float* q256, size_t nq,
                          const sieve_params* p, uint32_t* chosen, float* ms_per_variant);
 /* Fixes the scan variant (0 .. SIEVE_SCAN_VARIANTS - 1) instead of tuning it. */
@@ -217,6 +219,18 @@ int sieve_index_set_scan_variant(sieve_index* ix, uint32_t variant);
  * Results do not depend on the split. No counterpart in the reference (one thread scans a
  * whole query there). */
 int sieve_index_set_scan_chunk(sieve_index* ix, uint32_t chunk_blocks);
+/* The cluster-major scan (scan variants 10 and 11; no counterpart in the reference, where one
+ * thread scans one query): a batch's (cluster, query) pairs are grouped by cluster, and each
+ * block chunk of a cluster is read once for up to 32 of its queries and scored with int8
+ * MFMA. A seed pass first scans each query's nearest probes (whole clusters, at least
+ * seed_slots stored slots) for an admission threshold; later slots above it are appended to a
+ * per-query list of ext_cap entries, and a query whose list overflows is rescanned alone
+ * (sieve_stats.rescanned). Used for batches of at least min_queries queries without an
+ * allow filter (smaller or filtered batches run scan variant 2). Results equal the per-query
+ * scan's. 0 keeps a setting's default (min_queries 64, seed_slots and ext_cap 8 * cand,
+ * chunk_blocks 32). */
+int sieve_index_set_cluster_scan(sieve_index* ix, uint32_t min_queries, uint32_t seed_slots, uint32_t ext_cap,
+                                 uint32_t chunk_blocks);
 
 /* ------------------------------------------------------------- the path --- */
 
