@@ -807,14 +807,16 @@ int sieve_index_set_scan_chunk(sieve_index* ix, uint32_t chunk_blocks) {
 }
 
 int sieve_index_set_cluster_scan(sieve_index* ix, uint32_t min_queries, uint32_t seed_slots, uint32_t ext_cap,
-                                 uint32_t chunk_blocks) {
+                                 uint32_t chunk_blocks, uint32_t phases) {
     return guarded([&] {
         SIEVE_REQUIRE(ix, SIEVE_EINVAL, "null index");
         SIEVE_REQUIRE(chunk_blocks <= (1u << 20), SIEVE_EINVAL, "chunk_blocks too large");
+        SIEVE_REQUIRE(phases <= 64, SIEVE_EINVAL, "at most 64 phases");
         ix->cm_min_queries = min_queries ? min_queries : 64;
         ix->cm_seed_slots = seed_slots;
         ix->cm_ext_cap = ext_cap;
-        ix->cm_chunk_blocks = chunk_blocks ? chunk_blocks : 32;
+        ix->cm_chunk_blocks = chunk_blocks ? chunk_blocks : 64;
+        ix->cm_phases = phases ? phases : 3;
     });
 }
 

@@ -96,6 +96,32 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+// Slot codes of 64 groups (256-bit RaBitQ codes) are stored lane-chunked: 8-byte chunk r
+// (r = 0..3) holds the nibbles of groups 4 st + r, nibble st at bits 4 st (st = 0..15). A
+// lane that scores groups 4 st + r for one slot (the int8 MFMA operand layout) reads 8
+// contiguous bytes. code_word_group(d, j): the group of nibble j of 32-bit code word d.
+__host__ __device__ constexpr int code_word_group(int d, int j) { return 4 * (8 * (d & 1) + j) + (d >> 1); }
+
+// Raw code (bit i = sign of dimension i, group g = bits 4g .. 4g + 3) <-> lane-chunked code.
+__host__ __device__ inline void code_to_chunked(const uint32_t raw[8], uint32_t out[8]) {
+    for (int d = 0; d < 8; d++) {
+        uint32_t w = 0;
+        for (int j = 0; j < 8; j++) {
+            const int g = code_word_group(d, j);
+            w |= ((raw[g >> 3] >> (4 * (g & 7))) & 15u) << (4 * j);
+        }
+        out[d] = w;
+    }
+}
+__host__ __device__ inline void code_from_chunked(const uint32_t ch[8], uint32_t raw[8]) {
+    for (int d = 0; d < 8; d++) raw[d] = 0;
+    for (int d = 0; d < 8; d++)
+        for (int j = 0; j < 8; j++) {
+            const int g = code_word_group(d, j);
+            raw[g >> 3] |= ((ch[d] >> (4 * j)) & 15u) << (4 * (g & 7));
+        }
+}
+
 __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {

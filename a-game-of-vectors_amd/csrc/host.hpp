@@ -111,8 +111,9 @@ struct sieve_index {
     uint32_t scan_chunk_blocks = 0;  // 0: chunk size chosen per batch (sieve_index_set_scan_chunk)
     // Cluster-major scan (sieve_index_set_cluster_scan): batches of at least cm_min_queries
     // unfiltered queries; seed pass of cm_seed_slots slots and appended lists of cm_ext_cap
-    // entries per query (0: 8 * cand each); cm_chunk_blocks blocks per work item.
-    uint32_t cm_min_queries = 64, cm_seed_slots = 0, cm_ext_cap = 0, cm_chunk_blocks = 32;
+    // entries per query (0: 32 * cand each); cm_chunk_blocks blocks per work item; the work
+    // list in cm_phases launches.
+    uint32_t cm_min_queries = 64, cm_seed_slots = 0, cm_ext_cap = 0, cm_chunk_blocks = 64, cm_phases = 3;
 
     ~sieve_index() {
         void* ps[] = {cents, cents_bf, block_off, cl_slots, codes, mul, err, ids, rotation, quant, norm256, t2_sigma,
@@ -155,7 +156,7 @@ enum Slot {
     B_Q256, B_Q768, B_TABLES, B_QAB, B_WQ768, B_WQ256, B_INV, B_QBF, B_QNORM, B_APPROX, B_PS, B_PC, B_PN, B_CE, B_CR,
     B_CX, B_CN, B_OS, B_OR, B_ON, B_DEL, B_ALLOW, B_AFLAGS, B_CNT, B_OVF, B_TMP1, B_TMP2, B_TMP3, B_QQ, B_QF,
     B_PART, B_PARTN, B_GTHR, B_QDONE, B_CM_LIM, B_CM_SEGC, B_CM_QOFF, B_CM_IOFF, B_CM_Q, B_CM_CENT, B_CM_SEED,
-    B_CM_SEEDN, B_CM_THR, B_CM_EXT, B_CM_EXTN, B_CM_WORK,
+    B_CM_SEEDN, B_CM_THR, B_CM_EXT, B_CM_EXTN, B_CM_WORK, B_CM_ITEMS,
     B_NSLOTS
 };
 
@@ -336,6 +337,7 @@ inline void check_overflow(sieve_stream* s) {
 struct LiveArgs {
     const uint32_t* dblock_off = nullptr;  // k + 1 delta blocks per cluster (prefix)
     uint32_t dbase = 0;                    // first delta block in the index's slot arrays
+    uint32_t dblocks = 0;                  // delta blocks in all
     uint32_t n_base = 0xFFFFFFFFu;         // corpus rows; delta row j is row id n_base + j
     const int8_t* drows = nullptr;
     const float* dnorm256 = nullptr;
@@ -405,10 +407,21 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
             cm.seg_q = grow<uint32_t>(s->b[B_CM_Q], nq * (size_t)pstride * (a.dblock_off ? 2 : 1));
             cm.seg_cent = grow<float>(s->b[B_CM_CENT], nq * (size_t)pstride * (a.dblock_off ? 2 : 1));
             cm.probe_lim = grow<uint32_t>(s->b[B_CM_LIM], nq);
-            cm.work = grow<uint32_t>(s->b[B_CM_WORK], 1);
-            cm.seed_slots = ix->cm_seed_slots ? ix->cm_seed_slots : 8 * cand;
+            // Work items: sum over segments of ceil(queries / 32) * ceil(blocks / chunk) <= the
+            // bound below (pairs * max blocks / (32 chunk) + pairs / 32 + blocks / chunk + segments).
+            const double pairs = (double)nq * pstride * (a.dblock_off ? 2 : 1);
+            const double chunk = (double)std::max<uint32_t>(1, ix->cm_chunk_blocks);
+            const double maxnb = std::max<double>(ix->max_cluster_blocks, lv ? lv->dblocks : 0);
+            const double allnb = (double)ix->nblocks + (lv ? lv->dblocks : 0);
+            const double bound = pairs * maxnb / (32.0 * chunk) + pairs / 32.0 + allnb / chunk + cm.nseg + 2.0;
+            SIEVE_REQUIRE(bound < 4.0e9, SIEVE_ELIMIT, "cluster scan: too many work items");
+            cm.items_cap = (uint32_t)bound;
+            cm.items = grow<uint4>(s->b[B_CM_ITEMS], cm.items_cap);
+            cm.phases = std::max<uint32_t>(1, ix->cm_phases);
+            cm.work = grow<uint32_t>(s->b[B_CM_WORK], cm.phases);
+            cm.seed_slots = ix->cm_seed_slots ? ix->cm_seed_slots : 32 * cand;
             cm.chunk_blocks = std::max<uint32_t>(1, ix->cm_chunk_blocks);
-            a.ext_cap = ix->cm_ext_cap ? ix->cm_ext_cap : 8 * cand;
+            a.ext_cap = ix->cm_ext_cap ? ix->cm_ext_cap : 32 * cand;
             a.seed = grow<uint2>(s->b[B_CM_SEED], nq * (size_t)cand);
             a.seed_n = grow<uint32_t>(s->b[B_CM_SEEDN], nq);
             a.seed_thr = grow<unsigned long long>(s->b[B_CM_THR], nq);

@@ -173,9 +173,12 @@ struct CmScratch {
     uint32_t* seg_q;             // nq * probe_stride (segments x their queries)
     float* seg_cent;
     uint32_t* probe_lim;         // nq
-    uint32_t* work;              // 1: work-item counter (reset by the launch)
+    uint4* items;                // work items: (first slot, slots, first seg_q entry, queries)
+    uint32_t items_cap;
+    uint32_t* work;              // phases: work-item counters (reset by the launch)
     uint32_t seed_slots;         // slots the seed pass scans per query (whole probes)
     uint32_t chunk_blocks;       // blocks per work item
+    uint32_t phases;             // the work list runs in this many launches, thresholds raised between
 };
 // Enqueues the seed pass, the work-list build, the cluster pass and the finish pass. `a` carries
 // the index, batch and output fields and seed / ext buffers; its chunking is ignored.

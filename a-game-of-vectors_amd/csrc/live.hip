@@ -15,6 +15,7 @@
 // Concurrency: like the reference's &mut Live (src/live.rs:16-17), inserts and deletes must
 // not overlap searches of the same index.
 #include <float.h>
+#include <cstring>
 
 #include <hipcub/hipcub.hpp>
 
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(64) void k_live_encode(const float* __restrict__ xs
                                                     uint8_t* __restrict__ t2, float* __restrict__ t2n) {
     __shared__ float v[PDIM];
     __shared__ float r[PDIM];
-    __shared__ uint32_t code[8];
+    __shared__ uint32_t code[8], chunked[8];
     const size_t j = blockIdx.x;
     const int lane = threadIdx.x;
     const float* x = xs + j * PDIM;
@@ -186,8 +187,10 @@ __global__ __launch_bounds__(64) void k_live_encode(const float* __restrict__ xs
             lcl[L] = c;
         }
         __syncthreads();
-        if (lane < 2) lcode[2 * L + lane] = make_uint4(code[4 * lane], code[4 * lane + 1], code[4 * lane + 2],
-                                                       code[4 * lane + 3]);
+        if (lane == 0) code_to_chunked(code, chunked);
+        __syncthreads();
+        if (lane < 2) lcode[2 * L + lane] = make_uint4(chunked[4 * lane], chunked[4 * lane + 1], chunked[4 * lane + 2],
+                                                       chunked[4 * lane + 3]);
         __syncthreads();
     }
 }
@@ -312,6 +315,7 @@ struct sieve_live {
         LiveArgs a;
         a.dblock_off = dblock_off;
         a.dbase = (uint32_t)ix->nblocks;
+        a.dblocks = (uint32_t)arena_blocks;
         a.n_base = (uint32_t)n;
         a.drows = rows;
         a.dnorm256 = norm256;
@@ -615,6 +619,13 @@ int sieve_live_export(const sieve_live* l, int8_t* rows, float* norm256, uint8_t
         get(ext, l->ext, r * 4);
         get(dblock_off, l->dblock_off, (l->ix->k + 1) * 4);
         get(codes, l->ix->codes + 2 * base, slots * 32);
+        if (codes)  // the device keeps them lane-chunked; the export gives raw codes
+            for (size_t i = 0; i < slots; i++) {
+                uint32_t ch[8], raw[8];
+                std::memcpy(ch, codes + 32 * i, 32);
+                code_from_chunked(ch, raw);
+                std::memcpy(codes + 32 * i, raw, 32);
+            }
         get(mul, l->ix->mul + base, slots * 4);
         get(err, l->ix->err + base, slots * 4);
         get(ids, l->ix->ids + base, slots * 4);

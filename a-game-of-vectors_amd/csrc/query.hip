@@ -117,9 +117,10 @@ void launch_query_prep(const float* q256, const float* q768, size_t nq, const fl
 }
 
 // ------------------------------------------------------------------ re-layout --------
-// blocks.bin (byte g*16+k = doc k nibble g | doc k+16 nibble g << 4) -> G/2 raw code bytes
-// per slot (bit i of dim i at byte i/8, src/quant.rs:22-32), so a lane reads its slot's code
-// with 16-byte loads. One workgroup of 4G threads per block; thread t writes 4 code bytes.
+// blocks.bin (byte g*16+k = doc k nibble g | doc k+16 nibble g << 4) -> G/2 code bytes per
+// slot, so a lane reads its slot's code with 16-byte loads: G = 32 codes raw (bit i of dim i at
+// byte i/8, src/quant.rs:22-32), G = 64 codes lane-chunked (code_word_group). One workgroup of
+// 4G threads per block; thread t writes code word t % (G / 8) of slot t / (G / 8).
 template <int G>
 __global__ __launch_bounds__(4 * G) void k_transpose_blocks(const uint8_t* __restrict__ blocks, size_t nblocks,
                                                             uint8_t* __restrict__ codes) {
@@ -129,22 +130,15 @@ __global__ __launch_bounds__(4 * G) void k_transpose_blocks(const uint8_t* __res
     const int t = threadIdx.x;
     reinterpret_cast<uint32_t*>(blk)[t] = reinterpret_cast<const uint32_t*>(blocks + b * BB)[t];
     __syncthreads();
-    const int k = t / W, j0 = (t % W) * 4;
+    const int k = t / W, d = t % W;
     uint32_t word = 0;
 #pragma unroll
-    for (int jj = 0; jj < 4; jj++) {
-        int j = j0 + jj;  // code byte j holds groups 2j (low nibble) and 2j+1 (high nibble)
-        uint8_t lo, hi;
-        if (k < 16) {
-            lo = blk[(2 * j) * 16 + k] & 0x0F;
-            hi = blk[(2 * j + 1) * 16 + k] & 0x0F;
-        } else {
-            lo = blk[(2 * j) * 16 + k - 16] >> 4;
-            hi = blk[(2 * j + 1) * 16 + k - 16] >> 4;
-        }
-        word |= (uint32_t)(lo | (hi << 4)) << (8 * jj);
+    for (int j = 0; j < 8; j++) {  // nibble j of word d
+        const int g = G == 64 ? code_word_group(d, j) : 8 * d + j;
+        const uint8_t x = blk[g * 16 + (k & 15)];
+        word |= (uint32_t)(k < 16 ? x & 0x0F : x >> 4) << (4 * j);
     }
-    reinterpret_cast<uint32_t*>(codes + (b * BLOCK + k) * (G / 2))[t % W] = word;
+    reinterpret_cast<uint32_t*>(codes + (b * BLOCK + k) * (G / 2))[d] = word;
 }
 
 void launch_transpose_blocks(const uint8_t* blocks, size_t nblocks, int groups, uint4* codes, hipStream_t st) {

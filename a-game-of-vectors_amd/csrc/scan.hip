@@ -119,9 +119,13 @@ __device__ void scan_compact(uint2* buf, uint2* tmp, int n, int k, uint32_t* his
 // registers while scoring the current one (more bytes in flight, more VGPRs); WPE: minimum
 // waves per SIMD the register allocation must allow (0: the compiler's choice).
 // MODE: SCAN_FULL scans every probe; SCAN_SEED scans the first probe_lim[q] probes and leaves
-// the kept list for the cluster pass; SCAN_FINISH merges the seed and cluster-pass lists (or
-// rescans the query when its cluster-pass list overflowed) and writes the output.
-constexpr int SCAN_FULL = 0, SCAN_SEED = 1, SCAN_FINISH = 2;
+// the kept list for the cluster pass; SCAN_TIGHTEN (between cluster-pass phases) merges the
+// kept list with the entries appended since and keeps the best cand, raising the threshold;
+// SCAN_FINISH merges them one last time (or rescans the query when an appended list
+// overflowed) and writes the output.
+constexpr int SCAN_FULL = 0, SCAN_SEED = 1, SCAN_FINISH = 2, SCAN_TIGHTEN = 3;
+// seed_n flag: an appended list overflowed, so the query is rescanned by the finish pass.
+constexpr uint32_t SEED_LOST = 0x80000000u;
 
 template <int G, bool L2, int SLOTS_PER_THREAD, bool PF, int WPE, int MODE = SCAN_FULL>
 __global__ __launch_bounds__(SCAN_THREADS) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
@@ -148,11 +152,11 @@ void k_scan_t(ScanArgs a) {
     // list (src/ivf.rs:682-684).
     const bool live = a.dblock_off != nullptr;
     int nprobed = (int)a.probe_n[qi];
-    bool ext_ok = false;  // SCAN_FINISH: the cluster pass's list is whole (no rescan)
+    bool ext_ok = false;  // SCAN_FINISH / SCAN_TIGHTEN: no appended entry was lost
     if constexpr (MODE == SCAN_SEED) nprobed = min(nprobed, (int)a.probe_lim[qi]);
-    if constexpr (MODE == SCAN_FINISH) {
-        ext_ok = a.ext_n[qi] <= a.ext_cap;
-        if (ext_ok) nprobed = 0;
+    if constexpr (MODE == SCAN_FINISH || MODE == SCAN_TIGHTEN) {
+        ext_ok = a.ext_n[qi] <= a.ext_cap && !(a.seed_n[qi] & SEED_LOST);
+        if (ext_ok || MODE == SCAN_TIGHTEN) nprobed = 0;
     }
     const int np = (live ? 2 : 1) * nprobed;
     for (int j = t; j < np; j += SCAN_THREADS) {
@@ -290,7 +294,11 @@ void k_scan_t(ScanArgs a) {
 #pragma unroll
                 for (int d = 0; d < G / 8; d++)
 #pragma unroll
-                    for (int j = 0; j < 8; j++) sc += tab[(8 * d + j) * 16 + ((w[d] >> (4 * j)) & 15u)];
+                    for (int j = 0; j < 8; j++) {
+                        constexpr bool chunked = G == 64;  // lane-chunked codes
+                        const int g = chunked ? code_word_group(d, j) : 8 * d + j;
+                        sc += tab[g * 16 + ((w[d] >> (4 * j)) & 15u)];
+                    }
                 float est = R.cent[u] + R.mul[u] * (qab.x * (float)sc + qab.y);
                 if constexpr (L2) est = 2.0f * est - a.err[R.gs[u]];
                 est = est + 0.0f;  // -0.0 -> +0.0 so keys order like the floats compare
@@ -330,8 +338,10 @@ void k_scan_t(ScanArgs a) {
     __syncthreads();
     Regs ra, rb;
     uint32_t round = 0;
-    // Streams m listed entries through the buffer under the running threshold.
-    auto merge_list = [&](const uint2* lst, int m) {
+    // Streams m listed entries through the buffer under the running threshold; with check_ids
+    // (entries the cluster pass appended unchecked) an entry past the threshold is admitted only
+    // if its slot holds a row the view shows (src/ivf.rs:669).
+    auto merge_list = [&](const uint2* lst, int m, bool check_ids) {
         for (int i0 = 0; i0 < m; i0 += SCAN_THREADS * SLOTS_PER_THREAD) {
             const bool has_thr = misc[M_HAS] != 0;
             const uint32_t thr = misc[M_THR], thr_lo = misc[M_THR_LO];
@@ -343,6 +353,12 @@ void k_scan_t(ScanArgs a) {
                 if (i < m) {
                     e = lst[i];
                     pass = !has_thr || e.x > thr || (e.x == thr && e.y > thr_lo);
+                    if (pass && check_ids) {
+                        id_reads++;
+                        const uint32_t id = a.ids[~e.y];
+                        pass = id != PAD;
+                        if (pass && a.view.deleted && bit_set(a.view.deleted, a.view.deleted_words, id)) pass = false;
+                    }
                 }
                 admit(pass, e.x, e.y, round);
             }
@@ -362,6 +378,7 @@ void k_scan_t(ScanArgs a) {
     }
     id_reads = wave_sum(id_reads);
     if (lane == 0 && a.scanned) atomicAdd(a.scanned + 2, (unsigned long long)id_reads);
+    id_reads = 0;
     if ((int)misc[M_COUNT] > cand) scan_compact(buf, tmp, (int)misc[M_COUNT], cand, hist, misc, gthr);
 
     if (nch > 1) {
@@ -382,7 +399,7 @@ void k_scan_t(ScanArgs a) {
             if (c == ci) continue;
             const int m = (int)min((uint32_t)cand, __hip_atomic_load(a.part_n + slot0 + c, __ATOMIC_RELAXED,
                                                                      __HIP_MEMORY_SCOPE_AGENT));
-            merge_list(a.part + (slot0 + c) * (size_t)cand, m);
+            merge_list(a.part + (slot0 + c) * (size_t)cand, m, false);
         }
         if ((int)misc[M_COUNT] > cand) scan_compact(buf, tmp, (int)misc[M_COUNT], cand, hist, misc, nullptr);
         if (t == 0) {  // ready for the next launch
@@ -390,18 +407,29 @@ void k_scan_t(ScanArgs a) {
             __hip_atomic_store(gthr, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
-    if constexpr (MODE == SCAN_FINISH) {
+    if constexpr (MODE == SCAN_FINISH || MODE == SCAN_TIGHTEN) {
         if (ext_ok) {
-            merge_list(a.seed + qi * (size_t)cand, (int)min((uint32_t)cand, a.seed_n[qi]));
-            merge_list(a.ext + qi * (size_t)a.ext_cap, (int)a.ext_n[qi]);
+            merge_list(a.seed + qi * (size_t)cand, (int)min((uint32_t)cand, a.seed_n[qi]), false);
+            merge_list(a.ext + qi * (size_t)a.ext_cap, (int)a.ext_n[qi], true);
             if ((int)misc[M_COUNT] > cand) scan_compact(buf, tmp, (int)misc[M_COUNT], cand, hist, misc, nullptr);
-        } else if (t == 0 && a.scanned) {
+            id_reads = wave_sum(id_reads);
+            if (lane == 0 && a.scanned) atomicAdd(a.scanned + 2, (unsigned long long)id_reads);
+        } else if (MODE == SCAN_FINISH && t == 0 && a.scanned) {
             atomicAdd(a.scanned + 3, 1ull);  // queries rescanned
         }
         __syncthreads();
-        if (t == 0) a.ext_n[qi] = 0;  // ready for the next launch
+        if (t == 0) a.ext_n[qi] = 0;  // ready for the next phase / launch
+        if constexpr (MODE == SCAN_TIGHTEN) {
+            if (!ext_ok) {  // lost: later phases append nothing for it
+                if (t == 0) {
+                    a.seed_n[qi] |= SEED_LOST;
+                    a.seed_thr[qi] = ~0ull;
+                }
+                return;
+            }
+        }
     }
-    if constexpr (MODE == SCAN_SEED) {
+    if constexpr (MODE == SCAN_SEED || MODE == SCAN_TIGHTEN) {
         // The kept list and, when it is full, its worst entry: cand admitted slots compare at
         // least as high, so a later slot at or below it cannot enter the final list.
         const int n = (int)misc[M_COUNT];
@@ -574,13 +602,9 @@ struct CmArgs {
     const uint4* codes;
     const float* mul;
     const uint32_t* ids;
-    const uint32_t* block_off;
-    const uint32_t* dblock_off;
-    uint32_t dbase;
-    uint32_t k;
     const uint8_t* tables;
     const float4* qab;
-    const uint32_t* seg_qoff;
+    const uint4* items;          // (first slot, slots, first seg_q entry, queries)
     const uint32_t* seg_ioff;
     const uint32_t* seg_q;
     const float* seg_cent;
@@ -590,9 +614,9 @@ struct CmArgs {
     uint2* ext;
     uint32_t* ext_n;
     uint32_t ext_cap;
-    uint32_t* work;
+    uint32_t* work;              // phases counters
     unsigned long long* scanned;
-    uint32_t chunk_blocks;
+    uint32_t phase, phases;      // this launch runs work items [total * phase / phases, ...)
 };
 
 __device__ __forceinline__ void cm_segment(const uint32_t* block_off, const uint32_t* dblock_off, uint32_t dbase,
@@ -676,8 +700,8 @@ __global__ __launch_bounds__(SEL_THREADS) void k_cm_offsets(ScanArgs a, CmScratc
     if (t == 0) {
         cm.seg_qoff[n] = tq;
         cm.seg_ioff[n] = ti;
-        *cm.work = 0;
     }
+    if (t < (int)cm.phases) cm.work[t] = 0;
 }
 
 // One wave per query: lists each later probe's query and centroid score under its segment.
@@ -703,6 +727,27 @@ __global__ __launch_bounds__(256) void k_cm_scatter(ScanArgs a, CmScratch cm) {
     }
 }
 
+// One wave per segment: the segment's work items, 32-query tiles fastest (the tiles of one
+// block chunk run close together and share its reads in L2).
+__global__ __launch_bounds__(256) void k_cm_items(ScanArgs a, CmScratch cm) {
+    const uint32_t sg = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (sg >= cm.nseg) return;
+    uint32_t b0, nb;
+    cm_segment(a.block_off, a.dblock_off, a.dbase, cm.k, sg, b0, nb);
+    const uint32_t q0 = cm.seg_qoff[sg], qc = cm.seg_qoff[sg + 1] - q0;
+    const uint32_t i0 = cm.seg_ioff[sg], n = cm.seg_ioff[sg + 1] - i0;
+    if (n == 0) return;
+    const uint32_t tiles = (qc + 31) / 32;
+    for (uint32_t i = lane; i < n; i += 64) {
+        const uint32_t tile = i % tiles, chunk = i / tiles;
+        const uint32_t blo = chunk * cm.chunk_blocks, bhi = min(nb, blo + cm.chunk_blocks);
+        if (i0 + i < cm.items_cap)
+            cm.items[i0 + i] = make_uint4((b0 + blo) * BLOCK, (bhi - blo) * BLOCK, q0 + tile * 32,
+                                          min(32u, qc - tile * 32));
+    }
+}
+
 // The per-query constants of one C column (query) of a tile half.
 struct CmCol {
     float a, b, cent, thrf;
@@ -712,10 +757,26 @@ struct CmCol {
 
 // ONEHOT_LDS: the one-hot operand is read from a 16-entry LDS table (2 VALU + 1 LDS read per
 // MFMA step); otherwise it is built with shifts and selects (about 8 VALU).
+// Per-wave staging of appended entries, flushed once per work item (or when full): a past-the-
+// threshold slot costs an LDS write instead of a global atomic round trip inside the tile loop.
+constexpr int CM_STAGE = 256;
+// 16-slot tiles whose codes a wave has in flight ahead of the one it scores.
+constexpr int CM_PREFETCH = 4;
+
+// Orders this wave's LDS accesses (lanes exchange data through the wave's staging area).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <bool ONEHOT_LDS>
-__global__ __launch_bounds__(256) void k_scan_cm(CmArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_scan_cm(CmArgs a) {
     __shared__ uint4 onehot[16];
-    const int t = threadIdx.x, lane = t & 63;
+    __shared__ uint2 stage_e[4][CM_STAGE];       // (key, ~slot)
+    __shared__ uint8_t stage_c[4][CM_STAGE];     // tile column (query) 0..31
+    __shared__ uint32_t stage_cnt[4][32], stage_base[4][32], stage_q[4][32];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     if (t < 16) {
         uint32_t w[4] = {0u, 0u, 0u, 0u};
         w[t >> 2] = 1u << (8 * (t & 3));
@@ -724,28 +785,45 @@ __global__ __launch_bounds__(256) void k_scan_cm(CmArgs a) {
     __syncthreads();
     const int r = lane >> 4, j = lane & 15;
     const uint32_t total = a.seg_ioff[a.nseg];
-    uint32_t id_reads = 0;
+    const uint32_t i_lo = (uint32_t)((uint64_t)total * a.phase / a.phases);
+    const uint32_t i_hi = (uint32_t)((uint64_t)total * (a.phase + 1) / a.phases);
+    uint32_t sn = 0;  // staged entries (the same in every lane)
+    // Appends the staged entries to their queries' lists: one global atomic per column.
+    auto flush = [&]() {
+        if (sn == 0) return;
+        if (lane < 32) stage_cnt[wave][lane] = 0;
+        wave_lds_sync();
+        uint32_t loc[CM_STAGE / 64];
+#pragma unroll
+        for (int k = 0; k < CM_STAGE / 64; k++) {
+            const uint32_t i = lane + 64 * k;
+            loc[k] = i < sn ? atomicAdd(&stage_cnt[wave][stage_c[wave][i]], 1u) : 0u;
+        }
+        wave_lds_sync();
+        if (lane < 32) {
+            const uint32_t cnt = stage_cnt[wave][lane];
+            stage_base[wave][lane] = cnt ? atomicAdd(a.ext_n + stage_q[wave][lane], cnt) : 0u;
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int k = 0; k < CM_STAGE / 64; k++) {
+            const uint32_t i = lane + 64 * k;
+            if (i < sn) {
+                const uint32_t c = stage_c[wave][i];
+                const uint32_t at = stage_base[wave][c] + loc[k];
+                if (at < a.ext_cap) a.ext[(size_t)stage_q[wave][c] * a.ext_cap + at] = stage_e[wave][i];
+            }
+        }
+        wave_lds_sync();
+        sn = 0;
+    };
     for (;;) {
         uint32_t idx = 0;
-        if (lane == 0) idx = atomicAdd(a.work, 1u);
+        if (lane == 0) idx = i_lo + atomicAdd(a.work + a.phase, 1u);
         idx = __shfl(idx, 0, 64);
-        if (idx >= total) break;
-        // The segment holding work item idx (the last with seg_ioff <= idx is non-empty).
-        uint32_t lo = 0, hi = a.nseg - 1;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (a.seg_ioff[mid] <= idx) lo = mid;
-            else hi = mid - 1;
-        }
-        const uint32_t sg = lo;
-        uint32_t b0, nb;
-        cm_segment(a.block_off, a.dblock_off, a.dbase, a.k, sg, b0, nb);
-        const uint32_t q0 = a.seg_qoff[sg], qc = a.seg_qoff[sg + 1] - q0;
-        const uint32_t tiles = (qc + 31) / 32;
-        const uint32_t local = idx - a.seg_ioff[sg];
-        const uint32_t tile = local % tiles, chunk = local / tiles;
-        const uint32_t blo = chunk * a.chunk_blocks, bhi = min(nb, blo + a.chunk_blocks);
-        const uint32_t nvalid = min(32u, qc - tile * 32);
+        if (idx >= i_hi) break;
+        const uint4 item = a.items[idx];
+        const uint32_t s_lo = item.x, s_hi = item.x + item.y, qpos0 = item.z, nvalid = item.w;
         const bool two = nvalid > 16;
         // B operands: lane (j, r) holds query j's table entries of groups 4 st + r, minus 128.
         uint4 B0[16], B1[16];
@@ -754,15 +832,18 @@ __global__ __launch_bounds__(256) void k_scan_cm(CmArgs a) {
         for (int h = 0; h < 2; h++) {
             const uint32_t n = 16 * h + j;
             const bool valid = n < nvalid;
-            const uint32_t pos = q0 + tile * 32 + (valid ? n : 0);
+            const uint32_t pos = qpos0 + (valid ? n : 0);
             const uint32_t q = a.seg_q[pos];
             const float4 qab = a.qab[q];
             col[h].a = qab.x;
             col[h].b = qab.y;
             col[h].cent = a.seg_cent[pos];
             col[h].q = q;
+            // ~0: nothing passes (a padding column, or a query whose list was lost).
             col[h].thr = valid ? a.thr[q] : ~0ull;
-            col[h].thrf = !valid ? INFINITY : col[h].thr == 0 ? -INFINITY : key2f((uint32_t)(col[h].thr >> 32));
+            col[h].thrf = col[h].thr == ~0ull ? INFINITY
+                          : col[h].thr == 0   ? -INFINITY
+                                              : key2f((uint32_t)(col[h].thr >> 32));
             if (h == 1 && !two) continue;
             const uint4* tp = reinterpret_cast<const uint4*>(a.tables + (size_t)q * 1024);
 #pragma unroll
@@ -776,11 +857,19 @@ __global__ __launch_bounds__(256) void k_scan_cm(CmArgs a) {
                 else B1[st] = v;
             }
         }
-        if (lane == 0 && a.scanned) atomicAdd(a.scanned, (unsigned long long)(bhi - blo) * BLOCK * nvalid);
+        if (lane == 0 && a.scanned) atomicAdd(a.scanned, (unsigned long long)item.y * nvalid);
+        if (lane < 16) {
+            stage_q[wave][lane] = col[0].q;
+            stage_q[wave][lane + 16] = col[1].q;
+        }
+        wave_lds_sync();
 
-        // Lane (j, r) of C holds slots 4 r .. 4 r + 3 of the 16-slot tile for query column j.
-        auto epilogue = [&](const v4i& acc, const CmCol& cc, uint32_t slot0, const float4& mu) {
+        // Lane (j, r) of C holds slots 4 r .. 4 r + 3 of the 16-slot tile for query column j
+        // (column 16 h + j of the tile). Entries past the threshold are staged unchecked: the
+        // merge passes drop padding and hidden rows.
+        auto epilogue = [&](const v4i& acc, const CmCol& cc, uint32_t slot0, const float4& mu, uint32_t colx) {
             const float m4[4] = {mu.x, mu.y, mu.z, mu.w};
+            uint32_t keys[4] = {0u, 0u, 0u, 0u}, mask = 0;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const uint32_t sc = (uint32_t)(acc[i] + 64 * 128);
@@ -790,36 +879,54 @@ __global__ __launch_bounds__(256) void k_scan_cm(CmArgs a) {
                     const uint32_t slot = slot0 + 4 * r + i;
                     const uint32_t key = f2key(est);
                     if ((((unsigned long long)key << 32) | (unsigned long long)(~slot)) > cc.thr) {
-                        id_reads++;
-                        const uint32_t id = a.ids[slot];
-                        bool ok = id != PAD;
-                        if (ok && a.view.deleted && bit_set(a.view.deleted, a.view.deleted_words, id)) ok = false;
-                        if (ok) {
-                            const uint32_t at = atomicAdd(a.ext_n + cc.q, 1u);
-                            if (at < a.ext_cap) a.ext[(size_t)cc.q * a.ext_cap + at] = make_uint2(key, ~slot);
-                        }
+                        mask |= 1u << i;
+                        keys[i] = key;
                     }
                 }
             }
-        };
-        auto tile16 = [&](const uint4 (&cw)[2], const float4& mu, uint32_t slot0, auto two_c) {
-            constexpr bool TWO = decltype(two_c)::value;
-            const uint32_t w[8] = {cw[0].x, cw[0].y, cw[0].z, cw[0].w, cw[1].x, cw[1].y, cw[1].z, cw[1].w};
-            v4i acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+            if (!__ballot(mask != 0)) return;
+            const uint32_t c = (uint32_t)__popc(mask);
+            const uint32_t incl = wave_incl_scan(c);
+            const uint32_t tot = __shfl(incl, 63, 64);
+            if (sn + tot > (uint32_t)CM_STAGE) flush();
+            uint32_t at = sn + incl - c;
 #pragma unroll
-            for (int st = 0; st < 16; st++) {
-                // Group 4 st + r of the slot: word st / 2, bits 16 (st & 1) + 4 r.
-                const uint32_t nib = (w[st >> 1] >> (16 * (st & 1) + 4 * r)) & 15u;
-                v4i av;
+            for (int i = 0; i < 4; i++) {
+                if ((mask >> i) & 1u) {
+                    stage_e[wave][at] = make_uint2(keys[i], ~(slot0 + 4 * r + i));
+                    stage_c[wave][at] = (uint8_t)colx;
+                    at++;
+                }
+            }
+            sn += tot;
+        };
+        auto tile16 = [&](const uint2& cw, const float4& mu, uint32_t slot0, auto two_c) {
+            constexpr bool TWO = decltype(two_c)::value;
+            const uint32_t w[2] = {cw.x, cw.y};
+            v4i acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+            // The A operand of step st: the one-hot bytes of group 4 st + r of the slot, nibble st
+            // of the lane's 8-byte chunk of the code.
+            auto operand = [&](int st) -> v4i {
+                const uint32_t nib = (w[st >> 3] >> (4 * (st & 7))) & 15u;
                 if constexpr (ONEHOT_LDS) {
                     const uint4 oh = onehot[nib];
-                    av = v4i{(int)oh.x, (int)oh.y, (int)oh.z, (int)oh.w};
+                    return v4i{(int)oh.x, (int)oh.y, (int)oh.z, (int)oh.w};
                 } else {
                     const unsigned long long v = 1ull << ((nib & 7u) * 8u);
                     const bool hi8 = nib >= 8u;
-                    av = v4i{hi8 ? 0 : (int)(uint32_t)v, hi8 ? 0 : (int)(uint32_t)(v >> 32), hi8 ? (int)(uint32_t)v : 0,
-                             hi8 ? (int)(uint32_t)(v >> 32) : 0};
+                    return v4i{hi8 ? 0 : (int)(uint32_t)v, hi8 ? 0 : (int)(uint32_t)(v >> 32),
+                               hi8 ? (int)(uint32_t)v : 0, hi8 ? (int)(uint32_t)(v >> 32) : 0};
                 }
+            };
+            // LDS reads run PIPE steps ahead of the MFMAs that consume them.
+            constexpr int PIPE = ONEHOT_LDS ? 6 : 1;
+            v4i pipe[PIPE];
+#pragma unroll
+            for (int st = 0; st < PIPE; st++) pipe[st] = operand(st);
+#pragma unroll
+            for (int st = 0; st < 16; st++) {
+                const v4i av = pipe[st % PIPE];
+                if (st + PIPE < 16) pipe[st % PIPE] = operand(st + PIPE);
                 const v4i b0v = {(int)B0[st].x, (int)B0[st].y, (int)B0[st].z, (int)B0[st].w};
                 acc0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b0v, acc0, 0, 0, 0);
                 if constexpr (TWO) {
@@ -827,31 +934,38 @@ __global__ __launch_bounds__(256) void k_scan_cm(CmArgs a) {
                     acc1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b1v, acc1, 0, 0, 0);
                 }
             }
-            epilogue(acc0, col[0], slot0, mu);
-            if constexpr (TWO) epilogue(acc1, col[1], slot0, mu);
+            epilogue(acc0, col[0], slot0, mu, (uint32_t)j);
+            if constexpr (TWO) epilogue(acc1, col[1], slot0, mu, 16u + j);
         };
-        // 16-slot tiles of the chunk in order; the next tile's codes load during this one.
-        const uint32_t s_lo = (b0 + blo) * BLOCK, s_hi = (b0 + bhi) * BLOCK;
-        uint4 cw[2], nw[2];
-        float4 mu, nmu;
-        auto load = [&](uint32_t slot0, uint4 (&c)[2], float4& m) {
-            const uint4* cp = a.codes + 2 * (size_t)(slot0 + j);
-            c[0] = cp[0];
-            c[1] = cp[1];
+        // 16-slot tiles of the chunk in order, the codes and multipliers of the next CM_PREFETCH
+        // tiles in flight (lane (j, r): chunk r of slot j's code, mul of slots 4 r .. 4 r + 3).
+        constexpr int PF = ONEHOT_LDS ? CM_PREFETCH : CM_PREFETCH / 2;  // registers: no spills
+        const uint32_t ntiles = (s_hi - s_lo) / 16;
+        uint2 cr[PF];
+        float4 mr[PF];
+        auto load = [&](uint32_t ti, uint2& c, float4& m) {
+            const uint32_t slot0 = s_lo + 16 * ti;
+            c = reinterpret_cast<const uint2*>(a.codes + 2 * (size_t)(slot0 + j))[r];
             m = *reinterpret_cast<const float4*>(a.mul + slot0 + 4 * r);
         };
-        if (s_lo < s_hi) load(s_lo, cw, mu);
-        for (uint32_t s0 = s_lo; s0 < s_hi; s0 += 16) {
-            if (s0 + 16 < s_hi) load(s0 + 16, nw, nmu);
-            if (two) tile16(cw, mu, s0, std::true_type{});
-            else tile16(cw, mu, s0, std::false_type{});
-            cw[0] = nw[0];
-            cw[1] = nw[1];
-            mu = nmu;
+#pragma unroll
+        for (int p = 0; p < PF; p++)
+            if ((uint32_t)p < ntiles) load(p, cr[p], mr[p]);
+        for (uint32_t t0 = 0; t0 < ntiles; t0 += PF) {
+#pragma unroll
+            for (int p = 0; p < PF; p++) {
+                const uint32_t ti = t0 + p;
+                if (ti < ntiles) {
+                    const uint2 c = cr[p];
+                    const float4 m = mr[p];
+                    if (ti + PF < ntiles) load(ti + PF, cr[p], mr[p]);
+                    if (two) tile16(c, m, s_lo + 16 * ti, std::true_type{});
+                    else tile16(c, m, s_lo + 16 * ti, std::false_type{});
+                }
+            }
         }
+        flush();  // before the next item replaces the column queries
     }
-    id_reads = wave_sum(id_reads);
-    if (lane == 0 && a.scanned) atomicAdd(a.scanned + 2, (unsigned long long)id_reads);
 }
 
 void launch_scan_cm(const ScanArgs& a_in, const CmScratch& cm_in, size_t nq, int variant, hipStream_t st) {
@@ -870,6 +984,7 @@ void launch_scan_cm(const ScanArgs& a_in, const CmScratch& cm_in, size_t nq, int
     hipLaunchKernelGGL(k_cm_prep, dim3(qgrid), dim3(256), 0, st, a, cm);
     hipLaunchKernelGGL(k_cm_offsets, dim3(1), dim3(SEL_THREADS), 0, st, a, cm);
     hipLaunchKernelGGL(k_cm_scatter, dim3(qgrid), dim3(256), 0, st, a, cm);
+    hipLaunchKernelGGL(k_cm_items, dim3((cm.nseg + 3) / 4), dim3(256), 0, st, a, cm);
     if (lds > 64 * 1024) {
         (void)hipFuncSetAttribute((const void*)k_scan_t<64, false, 2, false, 0, SCAN_SEED>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -883,13 +998,9 @@ void launch_scan_cm(const ScanArgs& a_in, const CmScratch& cm_in, size_t nq, int
     c.codes = a.codes;
     c.mul = a.mul;
     c.ids = a.ids;
-    c.block_off = a.block_off;
-    c.dblock_off = a.dblock_off;
-    c.dbase = a.dbase;
-    c.k = cm.k;
     c.tables = a.tables;
     c.qab = a.qab;
-    c.seg_qoff = cm.seg_qoff;
+    c.items = cm.items;
     c.seg_ioff = cm.seg_ioff;
     c.seg_q = cm.seg_q;
     c.seg_cent = cm.seg_cent;
@@ -901,18 +1012,26 @@ void launch_scan_cm(const ScanArgs& a_in, const CmScratch& cm_in, size_t nq, int
     c.ext_cap = a.ext_cap;
     c.work = cm.work;
     c.scanned = a.scanned;
-    c.chunk_blocks = cm.chunk_blocks;
-    static int cus = 0;
-    if (!cus) {
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (cus <= 0) cus = 256;
+    c.phases = cm.phases;
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (cus <= 0) cus = 256;
+    if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute((const void*)k_scan_t<64, false, 2, false, 0, SCAN_TIGHTEN>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    // Phases of the work list; between them each query's kept list absorbs what was appended
+    // and its threshold rises, so later phases append less.
+    for (uint32_t ph = 0; ph < cm.phases; ph++) {
+        c.phase = ph;
+        if (variant == SCAN_VARIANT_CM)
+            hipLaunchKernelGGL(k_scan_cm<true>, dim3((unsigned)cus * 4), dim3(256), 0, st, c);
+        else
+            hipLaunchKernelGGL(k_scan_cm<false>, dim3((unsigned)cus * 4), dim3(256), 0, st, c);
+        if (ph + 1 < cm.phases)
+            hipLaunchKernelGGL((k_scan_t<64, false, 2, false, 0, SCAN_TIGHTEN>), dim3((unsigned)nq),
+                               dim3(SCAN_THREADS), lds, st, a);
     }
-    if (variant == SCAN_VARIANT_CM)
-        hipLaunchKernelGGL(k_scan_cm<true>, dim3((unsigned)cus * 4), dim3(256), 0, st, c);
-    else
-        hipLaunchKernelGGL(k_scan_cm<false>, dim3((unsigned)cus * 4), dim3(256), 0, st, c);
     hipLaunchKernelGGL((k_scan_t<64, false, 2, false, 0, SCAN_FINISH>), dim3((unsigned)nq), dim3(SCAN_THREADS), lds,
                        st, a);
 }

@@ -159,7 +159,7 @@ def lib():
         "sieve_index_autotune": (i32, [vp, vp, vp, vp, sz, P, ctypes.POINTER(ctypes.c_uint32), vp]),
         "sieve_index_set_scan_variant": (i32, [vp, u32]),
         "sieve_index_set_scan_chunk": (i32, [vp, u32]),
-        "sieve_index_set_cluster_scan": (i32, [vp, u32, u32, u32, u32]),
+        "sieve_index_set_cluster_scan": (i32, [vp, u32, u32, u32, u32, u32]),
         "sieve_l2_index_set_scan_chunk": (i32, [vp, u32]),
         "sieve_server_create": (i32, [vp, P, ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)]),
         "sieve_server_submit": (i32, [vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)]),

@@ -190,9 +190,10 @@ class Index:
         """Chunk size (32-slot blocks) of the scan's work split; 0 = chosen per batch."""
         abi.check(abi.lib().sieve_index_set_scan_chunk(self._h, blocks))
 
-    def set_cluster_scan(self, min_queries=0, seed_slots=0, ext_cap=0, chunk_blocks=0):
+    def set_cluster_scan(self, min_queries=0, seed_slots=0, ext_cap=0, chunk_blocks=0, phases=0):
         """Settings of the cluster-major scan (scan variants 10, 11); 0 keeps a default."""
-        abi.check(abi.lib().sieve_index_set_cluster_scan(self._h, min_queries, seed_slots, ext_cap, chunk_blocks))
+        abi.check(abi.lib().sieve_index_set_cluster_scan(self._h, min_queries, seed_slots, ext_cap, chunk_blocks,
+                                                         phases))
 
     def search(self, q768, q256, p: Params, view=None, timing=None, stream=None):
         """Index::search / search_view for a batch: (scores nq x k, rows, counts)."""

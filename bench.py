@@ -111,6 +111,11 @@ def main():
                     help="seconds of the mixed serving load (filters on 30%% of queries, live inserts at "
                          "100k rows/s with 10%% deletes), run last on the same index (0: skip)")
     ap.add_argument("--serve-mixed-rate", type=float, default=5000.0)
+    ap.add_argument("--cluster-scan", default=None,
+                    help="cluster-major scan settings seed_slots:ext_cap:chunk_blocks:phases (0 = default)")
+    ap.add_argument("--cm-sweep", default=None,
+                    help="comma-separated cluster-major scan settings (as --cluster-scan) to time on the picked "
+                         "configuration with scan variants 10 and 11 before the timed run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run an elementwise kernel that reads 1 GiB and writes 1 GiB (the known size that "
                          "scales a profile's FETCH_SIZE/WRITE_SIZE counters), then exit")
@@ -184,13 +189,49 @@ def main():
     ok = [x for x in sweep if x["recall_at_10"] >= 0.90]
     pick = max(ok, key=lambda x: x["qps"]) if ok else max(sweep, key=lambda x: x["recall_at_10"])
     _, p = parse_config(pick["config"], ix.k, k=10)
+    def cm_settings(spec):
+        v = [int(x) for x in spec.split(":")]
+        return dict(seed_slots=v[0], ext_cap=v[1], chunk_blocks=v[2], phases=v[3])
+
+    cm_sweep = []
+    if a.cm_sweep:
+        for setting in a.cm_sweep.split(","):
+            ix.set_cluster_scan(**cm_settings(setting))
+            for v in (10, 11):
+                ix.set_scan_variant(v)
+                run(p)
+                ms, resc = [], 0
+                for _ in range(3):
+                    run(p)
+                    stt = st.last_stats()
+                    ms.append(stt["scan_ns"] / 1e6)
+                    resc += stt["rescanned"]
+                cm_sweep.append(dict(settings=setting, variant=v, scan_ms=round(min(ms), 3), rescanned=resc / 3,
+                                     id_reads_per_query=round(stt["id_reads"] / nq, 1)))
+                log(f"cluster scan {setting} variant {v}: scan {min(ms):.2f} ms, {resc / 3:.0f} rescans, "
+                    f"{stt['id_reads'] / nq:.0f} past-threshold slots per query")
+        ix.set_cluster_scan()
+        ix.set_scan_variant(0)
+    if a.cluster_scan:
+        ix.set_cluster_scan(**cm_settings(a.cluster_scan))
+    variant_ms = None
     if a.scan_variant is None:
         variant, variant_ms = ix.autotune(q768, q256, p, stream=st)
+        variant_ms = [round(float(m), 3) for m in variant_ms]
         log(f"autotune: scan variant {variant} ({', '.join(f'{m:.2f}' for m in variant_ms)} ms)")
     else:
         variant = a.scan_variant
         ix.set_scan_variant(variant)
         log(f"scan variant {variant} (fixed)")
+    same_as_per_query = None
+    if variant >= 10:
+        # The cluster-major scan must return the per-query scan's results bit for bit.
+        got = [x.cpu().numpy() for x in run(p)]
+        ix.set_scan_variant(2)
+        want = [x.cpu().numpy() for x in run(p)]
+        ix.set_scan_variant(variant)
+        same_as_per_query = all(np.array_equal(x.view(np.uint32), y.view(np.uint32)) for x, y in zip(got, want))
+        log(f"results of scan variant {variant} equal the per-query scan's: {same_as_per_query}")
     agree = probe_agreement(ix, arrays.centroids, q256[:ngt], p, st)
     log(f"probe sets equal to the FP64 top-{p.nprobe}: {agree['equal']:.4f} of {ngt} queries "
         f"({agree['equal_up_to_ties']:.4f} up to f32-resolution ties)")
@@ -199,7 +240,7 @@ def main():
     for _ in range(a.warmup):
         run(p)
     torch.cuda.synchronize()
-    scan_ns, route_ns, rerank_ns, scanned, id_reads, probed = 0.0, 0.0, 0.0, 0, 0, 0
+    scan_ns, route_ns, rerank_ns, scanned, id_reads, probed, rescanned = 0.0, 0.0, 0.0, 0, 0, 0, 0
     t0 = time.perf_counter()
     for _ in range(a.steps):
         run(p)
@@ -210,6 +251,7 @@ def main():
         scanned += stt["scanned"]
         id_reads += stt["id_reads"]
         probed += stt["probed"]
+        rescanned += stt["rescanned"]
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / a.steps
@@ -273,6 +315,11 @@ def main():
         "serving_open_loop": serving,
         "serving_mixed": mixed,
         "scan_variant": variant,
+        "scan_variant_ms": variant_ms,
+        "cluster_scan": a.cluster_scan,
+        "results_equal_per_query_scan": same_as_per_query,
+        "cluster_scan_sweep": cm_sweep or None,
+        "rescanned_per_step": rescanned / a.steps,
         "probe_agreement_fp64": agree,
         "stage_ms": {"route": round(route_ns / 1e6 / a.steps, 3), "scan": round(scan_s * 1e3, 3),
                      "rerank": round(rerank_ns / 1e6 / a.steps, 3)},

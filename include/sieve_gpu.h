@@ -224,13 +224,14 @@ int sieve_index_set_scan_chunk(sieve_index* ix, uint32_t chunk_blocks);
  * block chunk of a cluster is read once for up to 32 of its queries and scored with int8
  * MFMA. A seed pass first scans each query's nearest probes (whole clusters, at least
  * seed_slots stored slots) for an admission threshold; later slots above it are appended to a
- * per-query list of ext_cap entries, and a query whose list overflows is rescanned alone
- * (sieve_stats.rescanned). Used for batches of at least min_queries queries without an
- * allow filter (smaller or filtered batches run scan variant 2). Results equal the per-query
- * scan's. 0 keeps a setting's default (min_queries 64, seed_slots and ext_cap 8 * cand,
- * chunk_blocks 32). */
+ * per-query list of ext_cap entries. The work list runs in `phases` launches; after each, a
+ * query's best cand entries so far raise its threshold. A query whose list overflows is
+ * rescanned alone (sieve_stats.rescanned). Used for batches of at least min_queries queries
+ * without an allow filter (smaller or filtered batches run scan variant 2). Results equal the
+ * per-query scan's. 0 keeps a setting's default (min_queries 64, seed_slots and ext_cap
+ * 32 * cand, chunk_blocks 64, phases 3). */
 int sieve_index_set_cluster_scan(sieve_index* ix, uint32_t min_queries, uint32_t seed_slots, uint32_t ext_cap,
-                                 uint32_t chunk_blocks);
+                                 uint32_t chunk_blocks, uint32_t phases);
 
 /* ------------------------------------------------------------- the path --- */
 

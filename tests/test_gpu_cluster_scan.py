@@ -46,21 +46,22 @@ def _same_candidates(want, got):
 
 
 @pytest.mark.parametrize("variant", CM)
-@pytest.mark.parametrize("cand,nprobe,seed,cap,chunk", [
-    (300, 16, 0, 0, 32),      # defaults
-    (300, 32, 1, 0, 1),       # seed = the nearest probe alone; one block per work item
-    (1000, 48, 0, 0, 7),      # every cluster probed
-    (37, 8, 0, 0, 32),
-    (150, 24, 1, 40, 3),      # tiny appended lists: overflowing queries are rescanned
-    (300, 16, 1 << 30, 0, 32),  # the seed pass covers every probe: nothing left for the MFMA pass
+@pytest.mark.parametrize("cand,nprobe,seed,cap,chunk,phases", [
+    (300, 16, 0, 0, 32, 0),      # defaults
+    (300, 32, 1, 0, 1, 1),       # seed = the nearest probe alone; one block per work item; one phase
+    (1000, 48, 0, 0, 7, 3),      # every cluster probed
+    (37, 8, 0, 0, 32, 8),
+    (150, 24, 1, 40, 3, 1),      # tiny appended lists: overflowing queries are rescanned
+    (150, 24, 1, 60, 5, 5),      # ... also when a list overflows in an early phase
+    (300, 16, 1 << 30, 0, 32, 0),  # the seed pass covers every probe: nothing left for the MFMA pass
 ])
-def test_cluster_scan_equals_per_query_scan(small, cm_index, variant, cand, nprobe, seed, cap, chunk):
+def test_cluster_scan_equals_per_query_scan(small, cm_index, variant, cand, nprobe, seed, cap, chunk, phases):
     q768, q256 = _queries(small)
     p = Params(nprobe=nprobe, cand=cand, k=10, rerank=Rerank.FULL)
     probes = cm_index.route(q256, p)
     t0, t1 = abi.Timing(), abi.Timing()
     want = _cands(cm_index, q256, probes, p, 0, timing=t0)
-    cm_index.set_cluster_scan(min_queries=1, seed_slots=seed, ext_cap=cap, chunk_blocks=chunk)
+    cm_index.set_cluster_scan(min_queries=1, seed_slots=seed, ext_cap=cap, chunk_blocks=chunk, phases=phases)
     s = cm_index.stream()
     try:
         got = _cands(cm_index, q256, probes, p, variant, timing=t1, stream=s)
