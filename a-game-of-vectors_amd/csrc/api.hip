@@ -816,7 +816,7 @@ int sieve_index_set_cluster_scan(sieve_index* ix, uint32_t min_queries, uint32_t
         ix->cm_seed_slots = seed_slots;
         ix->cm_ext_cap = ext_cap;
         ix->cm_chunk_blocks = chunk_blocks ? chunk_blocks : 64;
-        ix->cm_phases = phases ? phases : 3;
+        ix->cm_phases = phases ? phases : 2;
     });
 }
 

@@ -366,7 +366,13 @@ FilterRun enqueue_filter_search(const sieve_index* ix, const sieve_filter* f, ui
     // Expand: exact scores of every centroid, the weighted cut, then the filtered scan.
     SIEVE_REQUIRE((int)ix->k <= EXPAND_MAX_K, SIEVE_ELIMIT,
                   "expand plan sorts at most " + std::to_string(EXPAND_MAX_K) + " centroids");
-    const int maxp = (int)std::min<uint32_t>(ix->k, EXPAND_MAX_PROBE);
+    // As many probes as the scan's LDS segment table holds (two segments per probe with live
+    // delta lists), at most EXPAND_MAX_PROBE.
+    int maxp = (int)std::min<uint32_t>(ix->k, EXPAND_MAX_PROBE);
+    const int seg_per_probe = lv && lv->dblock_off ? 2 : 1;
+    while (maxp > 1 && scan_lds_bytes((int)p->cand, maxp * seg_per_probe) > 160 * 1024) maxp -= 64;
+    maxp = std::max(maxp, 1);
+    fr.max_probe = (uint32_t)maxp;
     float* all = grow<float>(s->b[B_APPROX], nq * ix->k);
     hipLaunchKernelGGL(k_exact_route, dim3(grid(ix->k, FT), (unsigned)nq), dim3(FT), 0, s->st, q2, ix->cents,
                        (int)ix->k, all);
@@ -476,7 +482,7 @@ int sieve_filter_search_batch(const sieve_index* ix, const sieve_filter* f, uint
         SIEVE_HIP(hipMemcpyAsync(&h, s->b[B_OVF].p, 4, hipMemcpyDeviceToHost, s->st));
         SIEVE_HIP(hipStreamSynchronize(s->st));
         SIEVE_REQUIRE(h == 0, SIEVE_ELIMIT, fr.plan == SIEVE_PLAN_EXPAND
-                                                ? "expand plan needs more than " + std::to_string(EXPAND_MAX_PROBE) +
+                                                ? "expand plan needs more than " + std::to_string(fr.max_probe) +
                                                       " probed clusters"
                                                 : std::string("routing: too many centroids within the BF16 error band"));
         finish_search(s, nq, t);

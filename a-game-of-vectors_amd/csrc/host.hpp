@@ -113,7 +113,7 @@ struct sieve_index {
     // unfiltered queries; seed pass of cm_seed_slots slots and appended lists of cm_ext_cap
     // entries per query (0: 32 * cand each); cm_chunk_blocks blocks per work item; the work
     // list in cm_phases launches.
-    uint32_t cm_min_queries = 64, cm_seed_slots = 0, cm_ext_cap = 0, cm_chunk_blocks = 64, cm_phases = 3;
+    uint32_t cm_min_queries = 64, cm_seed_slots = 0, cm_ext_cap = 0, cm_chunk_blocks = 64, cm_phases = 2;
 
     ~sieve_index() {
         void* ps[] = {cents, cents_bf, block_off, cl_slots, codes, mul, err, ids, rotation, quant, norm256, t2_sigma,
@@ -551,6 +551,7 @@ LiveArgs live_args(const sieve_live* l);
 struct FilterRun {
     uint32_t plan = 0;        // the plan used (SIEVE_PLAN_*)
     uint64_t brute_rows = 0;  // rows each query scored under the Brute plan
+    uint32_t max_probe = 0;   // Expand: clusters a cut may probe
 };
 FilterRun enqueue_filter_search(const sieve_index* ix, const sieve_filter* f, uint32_t plan, uint64_t expand_target,
                                 sieve_stream* s, const float* q2, const float* q7, size_t nq, const sieve_params* p,

@@ -156,7 +156,7 @@ size_t scan_lds_bytes(int cand, int probe_stride);
 // Scan kernel variants (slots per thread, prefetch), picked per index by the autotuner; the
 // last one is the cluster-major scan (launch_scan_cm).
 constexpr int SCAN_VARIANTS = 12;
-constexpr int SCAN_VARIANT_CM = 10;  // 10: one-hot operands from LDS, 11: built in registers
+constexpr int SCAN_VARIANT_CM = 10;  // 10: one-hot operands from LDS, 11: from LDS and registers
 const char* scan_variant_name(int variant);
 void launch_scan(const ScanArgs& a, size_t nq, int variant, hipStream_t st);
 

@@ -564,7 +564,7 @@ const char* scan_variant_name(int v) {
         "4 slots/thread", "2 slots/thread + prefetch", "2 slots/thread, cut at 1.5 cand", "1 slot/thread, cut at 1.5 cand",
         "2 slots/thread + prefetch, >= 5 waves/SIMD", "2 slots/thread", "1 slot/thread + prefetch", "1 slot/thread",
         "1 slot/thread + prefetch, cut at 1.5 cand", "1 slot/thread, cut at 1.25 cand",
-        "cluster-major MFMA, one-hot from LDS", "cluster-major MFMA, one-hot in registers"};
+        "cluster-major MFMA, one-hot from LDS", "cluster-major MFMA, one-hot from LDS and registers"};
     return v >= 0 && v < SCAN_VARIANTS ? names[v] : "?";
 }
 
@@ -755,8 +755,6 @@ struct CmCol {
     uint32_t q;
 };
 
-// ONEHOT_LDS: the one-hot operand is read from a 16-entry LDS table (2 VALU + 1 LDS read per
-// MFMA step); otherwise it is built with shifts and selects (about 8 VALU).
 // Per-wave staging of appended entries, flushed once per work item (or when full): a past-the-
 // threshold slot costs an LDS write instead of a global atomic round trip inside the tile loop.
 constexpr int CM_STAGE = 256;
@@ -770,7 +768,11 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <bool ONEHOT_LDS>
+// ONEHOT: where the one-hot A operands come from: CM_OH_LDS a 16-entry LDS table (2 VALU +
+// 1 LDS read per step), CM_OH_MIXED the table on even steps and shifts and selects (about 7
+// VALU) on odd ones, halving the LDS reads.
+constexpr int CM_OH_LDS = 1, CM_OH_MIXED = 2;
+template <int ONEHOT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_scan_cm(CmArgs a) {
     __shared__ uint4 onehot[16];
     __shared__ uint2 stage_e[4][CM_STAGE];       // (key, ~slot)
@@ -817,12 +819,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         wave_lds_sync();
         sn = 0;
     };
+    // The next work item is claimed, and its descriptor read, while the current one runs.
+    auto claim = [&]() {
+        uint32_t v = 0;
+        if (lane == 0) v = i_lo + atomicAdd(a.work + a.phase, 1u);
+        return (uint32_t)__shfl(v, 0, 64);
+    };
+    uint32_t nidx = claim();
+    uint4 nitem = nidx < i_hi ? a.items[nidx] : make_uint4(0u, 0u, 0u, 0u);
     for (;;) {
-        uint32_t idx = 0;
-        if (lane == 0) idx = i_lo + atomicAdd(a.work + a.phase, 1u);
-        idx = __shfl(idx, 0, 64);
+        const uint32_t idx = nidx;
         if (idx >= i_hi) break;
-        const uint4 item = a.items[idx];
+        const uint4 item = nitem;
+        nidx = claim();
+        if (nidx < i_hi) nitem = a.items[nidx];
         const uint32_t s_lo = item.x, s_hi = item.x + item.y, qpos0 = item.z, nvalid = item.w;
         const bool two = nvalid > 16;
         // B operands: lane (j, r) holds query j's table entries of groups 4 st + r, minus 128.
@@ -872,7 +882,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             uint32_t keys[4] = {0u, 0u, 0u, 0u}, mask = 0;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const uint32_t sc = (uint32_t)(acc[i] + 64 * 128);
+                const uint32_t sc = (uint32_t)acc[i];
                 float est = cc.cent + m4[i] * (cc.a * (float)sc + cc.b);
                 est = est + 0.0f;
                 if (est >= cc.thrf) {
@@ -903,12 +913,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         auto tile16 = [&](const uint2& cw, const float4& mu, uint32_t slot0, auto two_c) {
             constexpr bool TWO = decltype(two_c)::value;
             const uint32_t w[2] = {cw.x, cw.y};
-            v4i acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+            // The accumulators start at 64 * 128, so they end at score_code itself.
+            v4i acc0 = {64 * 128, 64 * 128, 64 * 128, 64 * 128}, acc1 = acc0;
             // The A operand of step st: the one-hot bytes of group 4 st + r of the slot, nibble st
             // of the lane's 8-byte chunk of the code.
             auto operand = [&](int st) -> v4i {
                 const uint32_t nib = (w[st >> 3] >> (4 * (st & 7))) & 15u;
-                if constexpr (ONEHOT_LDS) {
+                if (ONEHOT == CM_OH_LDS || (st & 1) == 0) {
                     const uint4 oh = onehot[nib];
                     return v4i{(int)oh.x, (int)oh.y, (int)oh.z, (int)oh.w};
                 } else {
@@ -919,7 +930,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 }
             };
             // LDS reads run PIPE steps ahead of the MFMAs that consume them.
-            constexpr int PIPE = ONEHOT_LDS ? 6 : 1;
+            constexpr int PIPE = 6;
             v4i pipe[PIPE];
 #pragma unroll
             for (int st = 0; st < PIPE; st++) pipe[st] = operand(st);
@@ -939,7 +950,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         };
         // 16-slot tiles of the chunk in order, the codes and multipliers of the next CM_PREFETCH
         // tiles in flight (lane (j, r): chunk r of slot j's code, mul of slots 4 r .. 4 r + 3).
-        constexpr int PF = ONEHOT_LDS ? CM_PREFETCH : CM_PREFETCH / 2;  // registers: no spills
+        constexpr int PF = ONEHOT == CM_OH_LDS ? CM_PREFETCH : CM_PREFETCH - 1;  // registers: no spills
         const uint32_t ntiles = (s_hi - s_lo) / 16;
         uint2 cr[PF];
         float4 mr[PF];
@@ -1025,9 +1036,9 @@ void launch_scan_cm(const ScanArgs& a_in, const CmScratch& cm_in, size_t nq, int
     for (uint32_t ph = 0; ph < cm.phases; ph++) {
         c.phase = ph;
         if (variant == SCAN_VARIANT_CM)
-            hipLaunchKernelGGL(k_scan_cm<true>, dim3((unsigned)cus * 4), dim3(256), 0, st, c);
+            hipLaunchKernelGGL(k_scan_cm<CM_OH_LDS>, dim3((unsigned)cus * 4), dim3(256), 0, st, c);
         else
-            hipLaunchKernelGGL(k_scan_cm<false>, dim3((unsigned)cus * 4), dim3(256), 0, st, c);
+            hipLaunchKernelGGL(k_scan_cm<CM_OH_MIXED>, dim3((unsigned)cus * 4), dim3(256), 0, st, c);
         if (ph + 1 < cm.phases)
             hipLaunchKernelGGL((k_scan_t<64, false, 2, false, 0, SCAN_TIGHTEN>), dim3((unsigned)nq),
                                dim3(SCAN_THREADS), lds, st, a);

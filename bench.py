@@ -57,7 +57,9 @@ SPREAD = 0.1
 # configuration): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB x 1024, the factors read off a 1 GiB
 # elementwise copy (bench.py --copy-probe). Every scan variant reads the same bytes (four
 # variants profiled within 0.1%). Other configurations report null.
-PMC_TRAFFIC = {("c3", "64:1000:full:b10x0.8:p0.1"): 306.22e9}
+# HBM bytes per scan (rocprofv3 FETCH_SIZE and WRITE_SIZE over the scan's kernels, scaled as
+# measured on the copy probe), keyed by workload, configuration and scan path.
+PMC_TRAFFIC = {("c3", "64:1000:full:b10x0.8:p0.1", "per-query"): 306.22e9}
 
 
 def log(msg):
@@ -107,10 +109,10 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="open-loop serving run per rate (0: skip)")
     ap.add_argument("--serve-rates", default="2000,20000", help="offered Poisson arrival rates, queries/s")
     ap.add_argument("--serve-max-batch", type=int, default=512)
-    ap.add_argument("--serve-mixed", type=float, default=0.0,
+    ap.add_argument("--serve-mixed", type=float, default=3.0,
                     help="seconds of the mixed serving load (filters on 30%% of queries, live inserts at "
                          "100k rows/s with 10%% deletes), run last on the same index (0: skip)")
-    ap.add_argument("--serve-mixed-rate", type=float, default=5000.0)
+    ap.add_argument("--serve-mixed-rates", default="500,2000", help="offered rates of the mixed serving load")
     ap.add_argument("--cluster-scan", default=None,
                     help="cluster-major scan settings seed_slots:ext_cap:chunk_blocks:phases (0 = default)")
     ap.add_argument("--cm-sweep", default=None,
@@ -259,6 +261,7 @@ def main():
     s, r, n = run(p)
     torch.cuda.synchronize()
     rec = recall10(r[:ngt].cpu().numpy(), n[:ngt].cpu().numpy().view(np.uint32), gt)
+    scan_path = "cluster-major" if variant >= 10 and nq >= 64 else "per-query"
     # Scan roofline: 32 code + 4 mul bytes per scanned slot, 8 (id, err) per slot past the threshold.
     scan_bytes = (scanned * 36 + id_reads * 8) / a.steps
     scan_s = scan_ns / 1e9 / a.steps
@@ -297,7 +300,13 @@ def main():
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(arrays, q768, q256, p, nq, a, gt)
-    mixed = serve_mixed(ix, mix, W["rows"], q768, q256, p, a) if a.serve_mixed > 0 else None
+    mixed = None
+    if a.serve_mixed > 0:
+        try:
+            mixed = serve_mixed(ix, mix, W["rows"], q768, q256, p, a)
+        except Exception as e:  # reported, not fatal: the batched measurement above stands
+            mixed = {"error": f"{type(e).__name__}: {e}"}
+            log(f"mixed serving failed: {mixed['error']}")
 
     out = {
         "metric": METRIC, "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": a.steps,
@@ -325,10 +334,16 @@ def main():
                      "rerank": round(rerank_ns / 1e6 / a.steps, 3)},
         "probed_per_query": round(probed / a.steps / nq, 2),
         "scanned_per_query": round(scanned / a.steps / nq, 1),
+        # The scan stage on its stream: k_scan_t alone (per-query path), or the cluster-major
+        # path's seed / MFMA / merge kernels. Algorithmic bytes are the per-(query, slot)
+        # stream either way; the cluster-major path reads each code once per 32-query tile,
+        # so it can exceed the HBM peak on that count.
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": PMC_TRAFFIC.get((a.workload, pick["config"])), "kernel": "k_scan_t",
-                     "algorithmic_bytes_per_launch": int(scan_bytes)},
+                     "traffic": PMC_TRAFFIC.get((a.workload, pick["config"], scan_path)),
+                     "kernel": ("scan stage: k_scan_t seed/merge + k_scan_cm (cluster-major, int8 MFMA)"
+                                if scan_path == "cluster-major" else "k_scan_t"),
+                     "scan_path": scan_path, "algorithmic_bytes_per_launch": int(scan_bytes)},
         "cpu_baseline": cpu,
         "sweep": sweep,
         "build_s": round(t_build, 1),
@@ -337,10 +352,11 @@ def main():
 
 
 def serve_mixed(ix, mix, n, q768, q256, p, a):
-    """Open-loop Poisson arrivals through the serving loop with a live state attached: 30% of
-    queries carry one of six allowlist filters (selectivity 0.1% .. 100%, log-uniform steps;
-    filter::search, plan resolved by size), and a writer inserts 1000 fresh rows every 10 ms
-    (100k rows/s) and deletes a tenth of them. Latency percentiles per request kind."""
+    """Open-loop Poisson arrivals through the serving loop with a live state attached, at each
+    offered rate: 30% of queries carry one of six allowlist filters (selectivity 0.1% .. 100%,
+    log-uniform steps; filter::search, plan resolved by size), and a writer inserts 1000 fresh
+    rows every 10 ms (100k rows/s) and deletes a tenth of them. Latency percentiles per request
+    kind; before the load, the time of one 1000-row insert and one 100-row delete alone."""
     import threading
     import numpy as np
     import torch
@@ -359,69 +375,85 @@ def serve_mixed(ix, mix, n, q768, q256, p, a):
         filters.append(f)
         plans.append(int(abi.lib().sieve_plan_resolve(abi.PLANS["auto"], f.rows(), n)))
     log(f"mixed serving: filters of {[f.rows() for f in filters]} rows, plans {plans}")
-    secs, rate = a.serve_mixed, a.serve_mixed_rate
+    secs = a.serve_mixed
+    rates = [float(x) for x in a.serve_mixed_rates.split(",")]
     batch_rows, period = 1000, 0.01
     nw = int(secs / period) + 2
-    live = Live(ix, max_rows=nw * batch_rows + 1000)
-    fresh, _ = normalize_rows(mix.rows(n, nw * batch_rows).to("cuda"))
+    live = Live(ix, max_rows=(len(rates) * nw + 2) * batch_rows)
+    fresh, _ = normalize_rows(mix.rows(n, (len(rates) * nw + 1) * batch_rows).to("cuda"))
     fresh = fresh.cpu().numpy().astype(np.float32)
     q7h, q2h = q768[:2000].cpu().numpy(), q256[:2000].cpu().numpy()
-    sv = Server(ix, p, max_batch=a.serve_max_batch, live=live)
-    lat = {"plain": [], "filtered": [], "write": []}
-    tickets, lock, done = [], threading.Lock(), threading.Event()
+    out = {"filter_selectivity": sel, "filter_plans": plans, "seconds_per_rate": secs, "rates": []}
+    # One write of each kind alone (the device encoder, arena rebuild and tombstone upload).
+    ext0 = (2_900_000_000 + np.arange(batch_rows)).astype(np.uint32)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    live.insert(ext0, fresh[:batch_rows])
+    out["insert_1000_rows_ms"] = round((time.perf_counter() - t1) * 1e3, 3)
+    t1 = time.perf_counter()
+    live.delete(ext0[:batch_rows // 10])
+    out["delete_100_rows_ms"] = round((time.perf_counter() - t1) * 1e3, 3)
+    used = 1
+    for rate in rates:
+        sv = Server(ix, p, max_batch=a.serve_max_batch, live=live)
+        lat = {"plain": [], "filtered": [], "write": []}
+        tickets, lock, done = [], threading.Lock(), threading.Event()
+        base = used
 
-    def writer():
-        j = 0
+        def writer():
+            j = 0
+            t0w = time.perf_counter()
+            while not done.is_set() and j < nw:
+                b = base + j
+                ext = (3_000_000_000 + b * batch_rows + np.arange(batch_rows)).astype(np.uint32)
+                tk = sv.insert(ext, fresh[b * batch_rows:(b + 1) * batch_rows])
+                tk2 = sv.delete(ext[rng.choice(batch_rows, batch_rows // 10, replace=False)])
+                with lock:
+                    tickets.append(("write", tk))
+                    tickets.append(("write", tk2))
+                j += 1
+                time.sleep(max(0.0, t0w + j * period - time.perf_counter()))
+
+        wt = threading.Thread(target=writer)
+        wt.start()
         t0 = time.perf_counter()
-        while not done.is_set() and j < nw:
-            ext = (3_000_000_000 + j * batch_rows + np.arange(batch_rows)).astype(np.uint32)
-            tk = sv.insert(ext, fresh[j * batch_rows:(j + 1) * batch_rows])
-            dels = ext[rng.choice(batch_rows, batch_rows // 10, replace=False)]
-            tk2 = sv.delete(dels)
+        nxt, i = t0, 0
+        while time.perf_counter() - t0 < secs:
+            now = time.perf_counter()
+            if now < nxt:
+                time.sleep(min(nxt - now, 0.001))
+                continue
+            j = i % len(q7h)
+            if rng.random() < 0.3:
+                tk = sv.submit_filtered(q7h[j], q2h[j], filters[rng.integers(len(filters))])
+                kind = "filtered"
+            else:
+                tk = sv.submit(q7h[j], q2h[j])
+                kind = "plain"
             with lock:
-                tickets.append(("write", tk))
-                tickets.append(("write", tk2))
-            j += 1
-            time.sleep(max(0.0, t0 + j * period - time.perf_counter()))
-
-    wt = threading.Thread(target=writer)
-    wt.start()
-    t0 = time.perf_counter()
-    nxt, i = t0, 0
-    while time.perf_counter() - t0 < secs:
-        now = time.perf_counter()
-        if now < nxt:
-            time.sleep(min(nxt - now, 0.001))
-            continue
-        j = i % len(q7h)
-        if rng.random() < 0.3:
-            tk = sv.submit_filtered(q7h[j], q2h[j], filters[rng.integers(len(filters))])
-            kind = "filtered"
-        else:
-            tk = sv.submit(q7h[j], q2h[j])
-            kind = "plain"
-        with lock:
-            tickets.append((kind, tk))
-        i += 1
-        nxt += rng.exponential(1.0 / rate)
-    done.set()
-    wt.join()
-    for kind, tk in tickets:
-        _, _, info = sv.wait(tk)
-        lat[kind].append(info.total_ms)
-    wall = time.perf_counter() - t0
-    st = sv.stats()
-    sv.close()
+                tickets.append((kind, tk))
+            i += 1
+            nxt += rng.exponential(1.0 / rate)
+        done.set()
+        wt.join()
+        used = base + nw
+        for kind, tk in tickets:
+            _, _, info = sv.wait(tk)
+            lat[kind].append(info.total_ms)
+        wall = time.perf_counter() - t0
+        st = sv.stats()
+        sv.close()
+        r = {"offered_qps": rate, "achieved_qps": round((len(lat["plain"]) + len(lat["filtered"])) / wall, 1),
+             "writes": st["writes"]}
+        for k, v in lat.items():
+            v.sort()
+            if v:
+                r[k] = {"requests": len(v), "p50_ms": round(pct(v, 0.5), 3), "p99_ms": round(pct(v, 0.99), 3)}
+        out["rates"].append(r)
+        log(f"mixed serving at {rate:.0f} QPS offered: {r}")
     info = live.info()
+    out["rows_inserted"], out["tombstones"] = info["rows"], info["deleted"]
     live.close()
-    out = {"offered_qps": rate, "seconds": secs, "achieved_qps": round((len(lat["plain"]) + len(lat["filtered"])) / wall, 1),
-           "filter_selectivity": sel, "filter_plans": plans, "writes": st["writes"],
-           "rows_inserted": info["rows"], "tombstones": info["deleted"]}
-    for k, v in lat.items():
-        v.sort()
-        if v:
-            out[k] = {"requests": len(v), "p50_ms": round(pct(v, 0.5), 3), "p99_ms": round(pct(v, 0.99), 3)}
-    log(f"mixed serving: {out}")
     return out
 
 
