@@ -59,7 +59,8 @@ SPREAD = 0.1
 # variants profiled within 0.1%). Other configurations report null.
 # HBM bytes per scan (rocprofv3 FETCH_SIZE and WRITE_SIZE over the scan's kernels, scaled as
 # measured on the copy probe), keyed by workload, configuration and scan path.
-PMC_TRAFFIC = {("c3", "64:1000:full:b10x0.8:p0.1", "per-query"): 306.22e9}
+PMC_TRAFFIC = {("c3", "64:1000:full:b10x0.8:p0.1", "per-query"): 306.22e9,
+               ("c3", "64:1000:full:b10x0.8:p0.1", "cluster-major"): 36.12e9}
 
 
 def log(msg):
