@@ -404,12 +404,12 @@ void rebuild_arena(sieve_live* l, hipStream_t st) {
     void* tmp = grow<uint8_t>(l->sort_tmp, std::max(tmp_bytes, tmp2));
     SIEVE_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, counts, cstart, (int)(k + 1), st));
     SIEVE_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp2, nb, l->dblock_off, (int)(k + 1), st));
-    uint32_t blocks = 0;
-    SIEVE_HIP(hipMemcpyAsync(&blocks, l->dblock_off + k, 4, hipMemcpyDeviceToHost, st));
-    SIEVE_HIP(hipStreamSynchronize(st));
-    SIEVE_REQUIRE((uint64_t)blocks * BLOCK <= l->cap_slots, SIEVE_ELIMIT, "live arena overflow");
+    // The arena's blocks, sum over clusters of ceil(count / 32), are at most m / 32 + k, and it
+    // holds 2 max_rows + 32 k slots with m <= 2 max_rows: it cannot overflow, and the clear
+    // covers the old arena and the bound of the new one without reading the count back first.
+    const uint64_t bound = std::min<uint64_t>(l->cap_slots / BLOCK, (m + BLOCK - 1) / BLOCK + k);
     const size_t base = ix->nslots;
-    const size_t clear = std::max<uint64_t>(blocks, l->arena_blocks) * BLOCK;
+    const size_t clear = std::max<uint64_t>(bound, l->arena_blocks) * BLOCK;
     if (clear)
         hipLaunchKernelGGL(k_arena_clear, dim3(grid(clear, 256)), dim3(256), 0, st, ix->codes + 2 * base,
                            ix->mul + base, ix->err + base, ix->ids + base, clear);
@@ -418,6 +418,8 @@ void rebuild_arena(sieve_live* l, hipStream_t st) {
                            l->dblock_off, l->lcode, l->lmul, l->lerr, l->lrow, ix->codes + 2 * base, ix->mul + base,
                            ix->err + base, ix->ids + base);
     SIEVE_HIP(hipGetLastError());
+    uint32_t blocks = 0;
+    SIEVE_HIP(hipMemcpyAsync(&blocks, l->dblock_off + k, 4, hipMemcpyDeviceToHost, st));
     SIEVE_HIP(hipStreamSynchronize(st));
     l->arena_blocks = blocks;
 }
@@ -450,7 +452,6 @@ void live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, size_t n
     uint32_t* pc = grow<uint32_t>(s->b[B_PC], n * m);
     uint32_t* pn = grow<uint32_t>(s->b[B_PN], n);
     route_batch(ix, s, bt, n, &rp, ps, pc, pn, m);
-    check_overflow(s);
     uint32_t* pairs = grow<uint32_t>(l->pairs, 2 * n);
     hipLaunchKernelGGL(k_live_pairs, dim3(grid(n, 64)), dim3(64), 0, st, xs, n, ix->cents, pc, pn, m, l->soar ? 1 : 0,
                        l->lambda, pairs);
@@ -463,7 +464,9 @@ void live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, size_t n
     SIEVE_HIP(hipGetLastError());
     SIEVE_HIP(hipMemcpyAsync(l->ext + r0, ext, n * 4, on_device(ext) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                              st));
-    SIEVE_HIP(hipStreamSynchronize(st));
+    // The routing overflow check synchronises the stream; nothing is committed before it (the
+    // log and row counts advance below).
+    check_overflow(s);
     // Upserts, in order (src/live.rs:369-378): a live external id's old row is tombstoned.
     std::vector<uint32_t> he;
     const uint32_t* eh = ext;

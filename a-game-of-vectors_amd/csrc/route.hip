@@ -187,6 +187,48 @@ __device__ uint32_t block_kth_u32(int n, int k, KeyFn key, uint32_t* hist, uint3
     return prefix;
 }
 
+// The same over keys the threads hold in registers: thread t holds keys of i = t + 256 j
+// (j < VALS, i < n), each read once from memory.
+template <int VALS>
+__device__ uint32_t block_kth_regs(const uint32_t (&x)[VALS], int n, int k, uint32_t* hist, uint32_t* sh) {
+    const int t = threadIdx.x;
+    uint32_t prefix = 0, mask = 0;
+    int kk = k;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        hist[t] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < VALS; j++)
+            if (t + 256 * j < n && (x[j] & mask) == prefix) atomicAdd(&hist[(x[j] >> shift) & 255], 1u);
+        __syncthreads();
+        if (t < 64) {
+            uint32_t c0 = hist[255 - 4 * t], c1 = hist[254 - 4 * t], c2 = hist[253 - 4 * t], c3 = hist[252 - 4 * t];
+            uint32_t s = c0 + c1 + c2 + c3;
+            uint32_t incl = wave_incl_scan(s);
+            uint64_t hit = __ballot(incl >= (uint32_t)kk);
+            int f = __ffsll((long long)hit) - 1;
+            if (t == f) {
+                uint32_t cum = incl - s;
+                uint32_t cs[4] = {c0, c1, c2, c3};
+                for (int j = 0; j < 4; j++) {
+                    if (cum + cs[j] >= (uint32_t)kk) {
+                        sh[0] = 255 - 4 * t - j;
+                        sh[1] = kk - cum;
+                        break;
+                    }
+                    cum += cs[j];
+                }
+            }
+        }
+        __syncthreads();
+        prefix |= sh[0] << shift;
+        mask |= 0xFFu << shift;
+        kk = (int)sh[1];
+        __syncthreads();
+    }
+    return prefix;
+}
+
 // Bitonic sort, descending, of n2 (a power of two) (key, ~id) pairs in LDS.
 __device__ void block_sort_desc(uint2* a, int n2) {
     for (int size = 2; size <= n2; size <<= 1) {
@@ -207,6 +249,9 @@ __device__ void block_sort_desc(uint2* a, int n2) {
     }
 }
 
+// REG_VALS > 0: the approximate scores (k <= 256 REG_VALS) are read once into registers and
+// the select and the band pass run over them; 0: over memory (any k).
+template <int REG_VALS>
 __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
     __shared__ float qs[PDIM];
     __shared__ uint32_t hist[256];
@@ -231,14 +276,38 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
     }
     // approximate route score of centroid i
     auto approx = [&](int i) -> float { return a.l2 ? 2.0f * S[i] - a.cnorm[i] : S[i]; };
-    uint32_t kth = block_kth_u32(a.k, n, [&](int i) { return f2key(approx(i) + 0.0f); }, hist, sh);
-    const float e = a.err_coef * a.qnorm[qi];
-    const float kv = key2f(kth);
-    const float thr = kv - 2.2f * e - 1e-6f * (1.0f + fabsf(kv));
-    for (int i = t; i < a.k; i += 256) {
-        if (approx(i) >= thr) {
-            uint32_t j = atomicAdd(&ncand, 1u);
-            if (j < MAX_RESCORE) cids[j] = (uint32_t)i;
+    uint32_t kth;
+    float thr;
+    if constexpr (REG_VALS > 0) {
+        uint32_t x[REG_VALS];
+#pragma unroll
+        for (int j = 0; j < REG_VALS; j++) {
+            const int i = t + 256 * j;
+            x[j] = i < a.k ? f2key(approx(i) + 0.0f) : 0u;
+        }
+        kth = block_kth_regs<REG_VALS>(x, a.k, n, hist, sh);
+        const float e = a.err_coef * a.qnorm[qi];
+        const float kv = key2f(kth);
+        thr = kv - 2.2f * e - 1e-6f * (1.0f + fabsf(kv));
+#pragma unroll
+        for (int j = 0; j < REG_VALS; j++) {
+            const int i = t + 256 * j;
+            // key2f(key(v)) == v + 0.0f, which compares with thr as v does
+            if (i < a.k && key2f(x[j]) >= thr) {
+                uint32_t jj = atomicAdd(&ncand, 1u);
+                if (jj < MAX_RESCORE) cids[jj] = (uint32_t)i;
+            }
+        }
+    } else {
+        kth = block_kth_u32(a.k, n, [&](int i) { return f2key(approx(i) + 0.0f); }, hist, sh);
+        const float e = a.err_coef * a.qnorm[qi];
+        const float kv = key2f(kth);
+        thr = kv - 2.2f * e - 1e-6f * (1.0f + fabsf(kv));
+        for (int i = t; i < a.k; i += 256) {
+            if (approx(i) >= thr) {
+                uint32_t j = atomicAdd(&ncand, 1u);
+                if (j < MAX_RESCORE) cids[j] = (uint32_t)i;
+            }
         }
     }
     __syncthreads();
@@ -335,7 +404,12 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
 
 void launch_route_select(const RouteArgs& a, size_t nq, hipStream_t st) {
     if (nq == 0) return;
-    hipLaunchKernelGGL(k_route_select, dim3((unsigned)nq), dim3(256), 0, st, a);
+    if (a.k <= 256 * 16)
+        hipLaunchKernelGGL(k_route_select<16>, dim3((unsigned)nq), dim3(256), 0, st, a);
+    else if (a.k <= 256 * 64)
+        hipLaunchKernelGGL(k_route_select<64>, dim3((unsigned)nq), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_route_select<0>, dim3((unsigned)nq), dim3(256), 0, st, a);
 }
 
 }  // namespace sieve

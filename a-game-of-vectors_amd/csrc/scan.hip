@@ -540,9 +540,10 @@ static void launch_any(const ScanArgs& a, size_t nq, int variant, hipStream_t st
 
 ScanPlan plan_scan(size_t nq, uint32_t nprobe, double mean_blocks, uint32_t max_cluster_blocks, int cand,
                    uint32_t forced_chunk_blocks) {
-    // A chunk holds at least 8 * cand slots (and 64 blocks), so a split query's merge reads at
-    // most an eighth of what its chunks scanned.
-    const double chmin = std::max(64.0, std::ceil(8.0 * cand / BLOCK));
+    // A chunk holds at least 24 * cand slots (and 64 blocks): a split query's last chunk merges
+    // up to cand entries from each of the others alone, so fewer, larger chunks balance that
+    // serial merge against the parallel scan (a lone 113.5M query: ~35 chunks).
+    const double chmin = std::max(64.0, std::ceil(24.0 * cand / BLOCK));
     // About 2048 workgroups per batch (8 per CU), and in large batches queries of more than
     // twice the expected work split, so the batch's tail is not one heavy query.
     const double expect = std::max(1.0, (double)nprobe * mean_blocks);
