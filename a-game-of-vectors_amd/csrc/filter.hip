@@ -305,6 +305,14 @@ namespace sieve {
 // beyond MAX_PROBE, or routing) in B_OVF, events ev[0..3] around its stages. With a live state
 // (lv), Post and Expand read the delta lists and tombstones; Brute, as the reference's, scores
 // the allowed corpus rows (skipping tombstoned ones).
+// Smallest scan chunk for a query filtered to f->nrows of the index's rows: a chunk should hold
+// about 2 * cand allowed slots, so each chunk reaches cand admitted entries and publishes a
+// threshold, and the query's last chunk merges lists the shared threshold mostly rejects.
+static uint32_t filtered_chunk_blocks(const sieve_index* ix, const sieve_filter* f, const sieve_params* p) {
+    const double frac = std::max(1e-6, (double)f->nrows / (double)std::max<uint64_t>(ix->n, 1));
+    return (uint32_t)std::min(1.0e6, std::ceil(2.0 * p->cand / (frac * BLOCK)));
+}
+
 FilterRun enqueue_filter_search(const sieve_index* ix, const sieve_filter* f, uint32_t plan, uint64_t expand_target,
                                 sieve_stream* s, const float* q2, const float* q7, size_t nq, const sieve_params* p,
                                 float* os, uint32_t* orr, uint32_t* on, const LiveArgs* lv) {
@@ -388,7 +396,8 @@ FilterRun enqueue_filter_search(const sieve_index* ix, const sieve_filter* f, ui
     uint32_t* cr = grow<uint32_t>(s->b[B_CR], nq * p->cand);
     float* cx = grow<float>(s->b[B_CX], nq * p->cand);
     uint32_t* cn = grow<uint32_t>(s->b[B_CN], nq);
-    scan_batch(ix, s, bt, nq, ps, pc, pn, maxp, p, view, ce, cr, cx, cn, (int)p->cand, cnt, lv);
+    scan_batch(ix, s, bt, nq, ps, pc, pn, maxp, p, view, ce, cr, cx, cn, (int)p->cand, cnt, lv,
+               filtered_chunk_blocks(ix, f, p));
     SIEVE_HIP(hipEventRecord(s->ev[2], s->st));
     rerank_batch(ix, s, bt, nq, ce, cr, cx, cn, (int)p->cand, p, os, orr, on, cnt + 1, lv);
     SIEVE_HIP(hipEventRecord(s->ev[3], s->st));

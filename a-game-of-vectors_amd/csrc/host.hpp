@@ -350,8 +350,9 @@ struct LiveArgs {
 
 // The work split of a batch's scan and its partial-list scratch (per stream).
 inline void set_scan_plan(ScanArgs& a, sieve_stream* s, size_t nq, uint32_t nprobe, double mean_blocks,
-                          uint32_t max_cluster_blocks, uint32_t forced_chunk_blocks) {
-    const ScanPlan pl = plan_scan(nq, nprobe, mean_blocks, max_cluster_blocks, a.cand, forced_chunk_blocks);
+                          uint32_t max_cluster_blocks, uint32_t forced_chunk_blocks, uint32_t min_chunk_blocks = 0) {
+    const ScanPlan pl =
+        plan_scan(nq, nprobe, mean_blocks, max_cluster_blocks, a.cand, forced_chunk_blocks, min_chunk_blocks);
     a.chunk_blocks = pl.chunk_blocks;
     a.chunk_max = pl.chunk_max;
     SIEVE_REQUIRE(nq * pl.chunk_max < (1ull << 31), SIEVE_ELIMIT, "scan grid too large");
@@ -366,7 +367,7 @@ inline void set_scan_plan(ScanArgs& a, sieve_stream* s, size_t nq, uint32_t npro
 inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, size_t nq, const float* ps,
                 const uint32_t* pc, const uint32_t* pn, int pstride, const sieve_params* p, const ViewArgs& view,
                 float* ce, uint32_t* cr, float* cx, uint32_t* cn, int cstride, unsigned long long* scanned,
-                const LiveArgs* lv = nullptr) {
+                const LiveArgs* lv = nullptr, uint32_t min_chunk_blocks = 0) {
     ScanArgs a{};
     if (lv) {
         a.dblock_off = lv->dblock_off;
@@ -434,7 +435,7 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
         variant = 2;
     }
     set_scan_plan(a, s, nq, (uint32_t)pstride, (double)ix->nblocks / ix->k, ix->max_cluster_blocks,
-                  ix->scan_chunk_blocks);
+                  ix->scan_chunk_blocks, min_chunk_blocks);
     launch_scan(a, nq, variant, s->st);
     SIEVE_HIP(hipGetLastError());
 }

@@ -145,12 +145,12 @@ struct ScanArgs {
 __host__ __device__ inline int scan_segments(const ScanArgs& a) { return a.dblock_off ? 2 * a.probe_stride : a.probe_stride; }
 // Chunking of a batch's scan: fills the work-split fields of `a` (a.cand set) for nq queries of
 // up to nprobe probes over clusters of at most max_cluster_blocks blocks (mean_blocks on average);
-// forced_chunk_blocks != 0 fixes the chunk size.
+// forced_chunk_blocks != 0 fixes the chunk size; min_chunk_blocks raises its floor.
 struct ScanPlan {
     uint32_t chunk_blocks, chunk_max;
 };
 ScanPlan plan_scan(size_t nq, uint32_t nprobe, double mean_blocks, uint32_t max_cluster_blocks, int cand,
-                   uint32_t forced_chunk_blocks);
+                   uint32_t forced_chunk_blocks, uint32_t min_chunk_blocks = 0);
 // Upper bound of the scan's LDS over all variants.
 size_t scan_lds_bytes(int cand, int probe_stride);
 // Scan kernel variants (slots per thread, prefetch), picked per index by the autotuner; the

@@ -539,11 +539,11 @@ static void launch_any(const ScanArgs& a, size_t nq, int variant, hipStream_t st
 }
 
 ScanPlan plan_scan(size_t nq, uint32_t nprobe, double mean_blocks, uint32_t max_cluster_blocks, int cand,
-                   uint32_t forced_chunk_blocks) {
+                   uint32_t forced_chunk_blocks, uint32_t min_chunk_blocks) {
     // A chunk holds at least 24 * cand slots (and 64 blocks): a split query's last chunk merges
     // up to cand entries from each of the others alone, so fewer, larger chunks balance that
     // serial merge against the parallel scan (a lone 113.5M query: ~35 chunks).
-    const double chmin = std::max(64.0, std::ceil(24.0 * cand / BLOCK));
+    const double chmin = std::max({64.0, std::ceil(24.0 * cand / BLOCK), (double)min_chunk_blocks});
     // About 2048 workgroups per batch (8 per CU), and in large batches queries of more than
     // twice the expected work split, so the batch's tail is not one heavy query.
     const double expect = std::max(1.0, (double)nprobe * mean_blocks);
@@ -606,6 +606,7 @@ struct CmArgs {
     const uint8_t* tables;
     const float4* qab;
     const uint4* items;          // (first slot, slots, first seg_q entry, queries)
+    uint32_t items_cap;
     const uint32_t* seg_ioff;
     const uint32_t* seg_q;
     const float* seg_cent;
@@ -787,7 +788,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
     __syncthreads();
     const int r = lane >> 4, j = lane & 15;
-    const uint32_t total = a.seg_ioff[a.nseg];
+    // The host sizes the descriptor array by a bound of the work items; never read past it.
+    const uint32_t total = min(a.seg_ioff[a.nseg], a.items_cap);
     const uint32_t i_lo = (uint32_t)((uint64_t)total * a.phase / a.phases);
     const uint32_t i_hi = (uint32_t)((uint64_t)total * (a.phase + 1) / a.phases);
     uint32_t sn = 0;  // staged entries (the same in every lane)
@@ -1013,6 +1015,7 @@ void launch_scan_cm(const ScanArgs& a_in, const CmScratch& cm_in, size_t nq, int
     c.tables = a.tables;
     c.qab = a.qab;
     c.items = cm.items;
+    c.items_cap = cm.items_cap;
     c.seg_ioff = cm.seg_ioff;
     c.seg_q = cm.seg_q;
     c.seg_cent = cm.seg_cent;

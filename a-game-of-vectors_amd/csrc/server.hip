@@ -104,12 +104,14 @@ struct sieve_server {
 
 namespace {
 
-// Applies one write request (the batcher holds no batch in flight).
+// Applies one write request (the batcher holds no batch in flight), then refreshes the delta
+// segment later batches read (its block count grows with inserts).
 void apply_write(sieve_server* sv, Request& r) {
     int rc = r.kind == K_INSERT ? sieve_live_insert(sv->live, r.ext.data(), r.vecs.data(), r.ext.size())
                                 : sieve_live_delete(sv->live, r.ext.data(), r.ext.size());
     r.status = rc;
     if (rc != 0) r.error = sieve_last_error();
+    sv->la = live_args(sv->live);
 }
 
 void batcher_loop(sieve_server* sv) {
@@ -139,16 +141,24 @@ void batcher_loop(sieve_server* sv) {
             const uint64_t head = sv->pending.front();
             Request& hr = sv->reqs[head];
             if (hr.kind != K_SEARCH) {
-                sv->pending.pop_front();
+                // The writes queued back to back at the head share one drain, applied in order.
+                std::vector<Request*> writes;
+                while (!sv->pending.empty() && sv->reqs[sv->pending.front()].kind != K_SEARCH) {
+                    Request& w = sv->reqs[sv->pending.front()];
+                    sv->pending.pop_front();
+                    w.t_launch = Clock::now();
+                    writes.push_back(&w);  // node-based map: stable until waited out
+                }
                 sv->launching++;
-                hr.t_launch = Clock::now();
                 lk.unlock();
-                apply_write(sv, hr);
+                for (Request* w : writes) apply_write(sv, *w);
                 lk.lock();
                 sv->launching--;
-                hr.done = true;
-                hr.t_done = Clock::now();
-                sv->stats.writes++;
+                for (Request* w : writes) {
+                    w->done = true;
+                    w->t_done = Clock::now();
+                }
+                sv->stats.writes += writes.size();
                 sv->cv_done.notify_all();
                 continue;
             }

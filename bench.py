@@ -114,6 +114,8 @@ def main():
                     help="seconds of the mixed serving load (filters on 30%% of queries, live inserts at "
                          "100k rows/s with 10%% deletes), run last on the same index (0: skip)")
     ap.add_argument("--serve-mixed-rates", default="500,2000", help="offered rates of the mixed serving load")
+    ap.add_argument("--serve-mixed-filtered", type=float, default=0.3, help="fraction of filtered queries")
+    ap.add_argument("--serve-mixed-writes", type=int, default=1, help="1: live inserts and deletes, 0: none")
     ap.add_argument("--cluster-scan", default=None,
                     help="cluster-major scan settings seed_slots:ext_cap:chunk_blocks:phases (0 = default)")
     ap.add_argument("--cm-sweep", default=None,
@@ -384,7 +386,8 @@ def serve_mixed(ix, mix, n, q768, q256, p, a):
     fresh, _ = normalize_rows(mix.rows(n, (len(rates) * nw + 1) * batch_rows).to("cuda"))
     fresh = fresh.cpu().numpy().astype(np.float32)
     q7h, q2h = q768[:2000].cpu().numpy(), q256[:2000].cpu().numpy()
-    out = {"filter_selectivity": sel, "filter_plans": plans, "seconds_per_rate": secs, "rates": []}
+    out = {"filter_selectivity": sel, "filter_plans": plans, "seconds_per_rate": secs,
+           "filtered_fraction": a.serve_mixed_filtered, "writes": bool(a.serve_mixed_writes), "rates": []}
     # One write of each kind alone (the device encoder, arena rebuild and tombstone upload).
     ext0 = (2_900_000_000 + np.arange(batch_rows)).astype(np.uint32)
     torch.cuda.synchronize()
@@ -404,7 +407,7 @@ def serve_mixed(ix, mix, n, q768, q256, p, a):
         def writer():
             j = 0
             t0w = time.perf_counter()
-            while not done.is_set() and j < nw:
+            while a.serve_mixed_writes and not done.is_set() and j < nw:
                 b = base + j
                 ext = (3_000_000_000 + b * batch_rows + np.arange(batch_rows)).astype(np.uint32)
                 tk = sv.insert(ext, fresh[b * batch_rows:(b + 1) * batch_rows])
@@ -425,7 +428,7 @@ def serve_mixed(ix, mix, n, q768, q256, p, a):
                 time.sleep(min(nxt - now, 0.001))
                 continue
             j = i % len(q7h)
-            if rng.random() < 0.3:
+            if rng.random() < a.serve_mixed_filtered:
                 tk = sv.submit_filtered(q7h[j], q2h[j], filters[rng.integers(len(filters))])
                 kind = "filtered"
             else:
