@@ -933,7 +933,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 }
             };
             // LDS reads run PIPE steps ahead of the MFMAs that consume them.
-            constexpr int PIPE = 6;
+            constexpr int PIPE = 4;
             v4i pipe[PIPE];
 #pragma unroll
             for (int st = 0; st < PIPE; st++) pipe[st] = operand(st);
@@ -948,12 +948,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     acc1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b1v, acc1, 0, 0, 0);
                 }
             }
+            if constexpr (ONEHOT == CM_OH_LDS) {
+                // Keep the table reads PIPE steps ahead: the reads and address arithmetic of the
+                // prologue first, then per step one read, its address, and the step's MFMAs.
+                __builtin_amdgcn_sched_group_barrier(0x002, 2 * PIPE, 0);  // VALU
+                __builtin_amdgcn_sched_group_barrier(0x100, PIPE, 0);      // DS read
+#pragma unroll
+                for (int st = 0; st < 16; st++) {
+                    if (st + PIPE < 16) {
+                        __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
+                        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                    }
+                    __builtin_amdgcn_sched_group_barrier(0x008, TWO ? 2 : 1, 0);  // MFMA
+                }
+            }
             epilogue(acc0, col[0], slot0, mu, (uint32_t)j);
             if constexpr (TWO) epilogue(acc1, col[1], slot0, mu, 16u + j);
         };
         // 16-slot tiles of the chunk in order, the codes and multipliers of the next CM_PREFETCH
         // tiles in flight (lane (j, r): chunk r of slot j's code, mul of slots 4 r .. 4 r + 3).
-        constexpr int PF = ONEHOT == CM_OH_LDS ? CM_PREFETCH : CM_PREFETCH - 1;  // registers: no spills
+        constexpr int PF = ONEHOT == CM_OH_LDS ? CM_PREFETCH : CM_PREFETCH - 2;  // registers: no spills
         const uint32_t ntiles = (s_hi - s_lo) / 16;
         uint2 cr[PF];
         float4 mr[PF];

@@ -99,6 +99,28 @@ def test_route_matches_oracle(small, oracle_index, gpu_index, cfg):
             assert np.array_equal(gc, wc) and _flt_eq(gs, ws)
 
 
+@pytest.fixture(scope="module")
+def wide():
+    """An index of 6000 clusters over the small corpus (4 rows per cluster on average): routing
+    selects over more than 4096 centroids, the register-held select with 64 keys per thread."""
+    from conftest import SmallWorkload
+    from oracle import oracle as O
+    print("[gpu] building a 6000-cluster index", flush=True)
+    w = SmallWorkload(k=6000, nq=24, tier2=False)
+    return w, O.Index(w.host), Index.create(w.host)
+
+
+@pytest.mark.parametrize("cfg", [dict(nprobe=64), dict(nprobe=256, margin=0.05), dict(nprobe=9)])
+def test_route_over_many_centroids_matches_oracle(wide, cfg):
+    w, oix, gix = wide
+    p = Params(cand=100, **cfg)
+    s, c, n = gix.route(w.q256, p)
+    for i in range(w.nq):
+        ws, wc = oix.route(w.q256[i], p)
+        assert n[i] == len(wc)
+        assert_same_up_to_ties(c[i, :n[i]], s[i, :n[i]], wc, ws)
+
+
 # ---------------------------------------------------------------------- candidates ------
 def _oracle_probes(oracle_index, q256, p, nq):
     stride = p.nprobe
