@@ -111,7 +111,7 @@ struct sieve_index {
     uint32_t scan_chunk_blocks = 0;  // 0: chunk size chosen per batch (sieve_index_set_scan_chunk)
     // Cluster-major scan (sieve_index_set_cluster_scan): batches of at least cm_min_queries
     // unfiltered queries; seed pass of cm_seed_slots slots and appended lists of cm_ext_cap
-    // entries per query (0: 32 and 64 * cand); cm_chunk_blocks blocks per work item; the work
+    // entries per query (0: 24 and 64 * cand); cm_chunk_blocks blocks per work item; the work
     // list in cm_phases launches.
     uint32_t cm_min_queries = 64, cm_seed_slots = 0, cm_ext_cap = 0, cm_chunk_blocks = 64, cm_phases = 2;
 
@@ -420,7 +420,7 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
             cm.items = grow<uint4>(s->b[B_CM_ITEMS], cm.items_cap);
             cm.phases = std::max<uint32_t>(1, ix->cm_phases);
             cm.work = grow<uint32_t>(s->b[B_CM_WORK], cm.phases);
-            cm.seed_slots = ix->cm_seed_slots ? ix->cm_seed_slots : 32 * cand;
+            cm.seed_slots = ix->cm_seed_slots ? ix->cm_seed_slots : 24 * cand;
             cm.chunk_blocks = std::max<uint32_t>(1, ix->cm_chunk_blocks);
             a.ext_cap = ix->cm_ext_cap ? ix->cm_ext_cap : 64 * cand;
             a.seed = grow<uint2>(s->b[B_CM_SEED], nq * (size_t)cand);

@@ -115,4 +115,175 @@ def build_l2(base, k, iters=8, seed=7, chunk=1 << 18):
                     factors=factors, rotation=signs, base=base)
 
 
-__all__ = ["L2Arrays", "int8_mixture", "build_l2", "f32_div"]
+class L2Mixture:
+    """The 100-d int8 mixture of BASELINE config 4 at any size: component means N(0, 40^2) per
+    dimension; each component holds `topics` sub-centres at N(0, spread^2) around its mean, and
+    a row is a uniformly drawn topic's centre + N(0, topic_spread^2), rounded and clipped to
+    [-127, 127] (topics = 0: rows around the component mean at `spread`). Rows come in 2^22-row
+    chunks, chunk c from its own seed, so any range can be regenerated; queries are further
+    draws plus uniform integer noise in [-noise, noise]."""
+
+    CHUNK = 1 << 22
+
+    def __init__(self, components, seed, device="cuda", spread=12.0, topics=0, topic_spread=4.0):
+        self.components, self.seed, self.device, self.spread = components, seed, device, spread
+        self.topics, self.topic_spread = topics, topic_spread
+        g = torch.Generator(device=device)
+        g.manual_seed(sub_seed(seed, 11))
+        self.means = torch.randn(components, D, generator=g, device=device) * 40.0
+        self.centres = None
+        if topics:
+            tc = torch.empty(components * topics, D, device=device)
+            for c0 in range(0, components, 8192):
+                m = self.means[c0:c0 + 8192].repeat_interleave(topics, 0)
+                tc[c0 * topics:c0 * topics + m.shape[0]] = m + spread * torch.randn(m.shape, generator=g,
+                                                                                      device=device)
+            self.centres = tc
+
+    def _draw(self, n, g):
+        if self.centres is None:
+            comp = torch.randint(0, self.components, (n,), generator=g, device=self.device)
+            x = self.means[comp] + self.spread * torch.randn(n, D, generator=g, device=self.device)
+        else:
+            t = torch.randint(0, self.centres.shape[0], (n,), generator=g, device=self.device)
+            x = self.centres[t] + self.topic_spread * torch.randn(n, D, generator=g, device=self.device)
+        return torch.clamp(torch.round(x), -127, 127).to(torch.int8)
+
+    def fill(self, out, start=0):
+        """Rows [start, start + len(out)) into the int8 tensor `out` (n x 100)."""
+        n = out.shape[0]
+        c0, c1 = start // self.CHUNK, (start + n + self.CHUNK - 1) // self.CHUNK
+        g = torch.Generator(device=self.device)
+        for c in range(c0, c1):
+            g.manual_seed(sub_seed(self.seed, 12, c))
+            x = self._draw(self.CHUNK, g)
+            lo, hi = max(start - c * self.CHUNK, 0), min(start + n - c * self.CHUNK, self.CHUNK)
+            out[c * self.CHUNK + lo - start:c * self.CHUNK + hi - start] = x[lo:hi]
+        return out
+
+    def queries(self, nq, noise=6):
+        g = torch.Generator(device=self.device)
+        g.manual_seed(sub_seed(self.seed, 13))
+        q = self._draw(nq, g).to(torch.int32)
+        q = q + torch.randint(-noise, noise + 1, q.shape, generator=g, device=self.device, dtype=torch.int32)
+        return torch.clamp(q, -127, 127).to(torch.int8)
+
+
+def two_level_leaves(x, coarse, fine, iters, seed):
+    """Leaves from a two-level k-means as bigann trains them (src/bigann.rs:175-226): `coarse`
+    cells over the sample, then `fine` leaves per cell over the cell's rows (a cell with fewer
+    rows repeats its rows). Returns (coarse centroids, leaves coarse * fine x 100)."""
+    cents = kmeans_l2(x, coarse, iters, seed)
+    a = (2.0 * x @ cents.T - (cents * cents).sum(1)).argmax(1)
+    leaves = torch.empty(coarse * fine, D, dtype=torch.float32, device=x.device)
+    order = torch.argsort(a)
+    counts = torch.bincount(a, minlength=coarse).tolist()
+    o = 0
+    for c in range(coarse):
+        xc = x[order[o:o + counts[c]]]
+        o += counts[c]
+        if xc.shape[0] == 0:
+            xc = cents[c:c + 1]
+        if xc.shape[0] < fine:
+            xc = xc.repeat((fine + xc.shape[0] - 1) // xc.shape[0], 1)
+        leaves[c * fine:(c + 1) * fine] = kmeans_l2(xc, fine, iters, sub_seed(seed, 14, c) & 0x7FFFFFFF)
+    return cents, leaves
+
+
+def build_l2_large(base, coarse=512, fine=256, iters=8, seed=7, sample=1 << 22, chunk=1 << 22, log=None):
+    """build_l2 for base tables too large for whole-table temporaries (1B x 100 on one GPU):
+    two-level leaves; each row assigned to the nearest of its coarse cell's leaves (rows sorted
+    by cell, one GEMM per cell); codes, mul and |x|^2 per chunk; slots sorted by (leaf, row)
+    and packed into 512-byte blocks a chunk of blocks at a time. The arrays keep the layout of
+    build_l2's. Peak device memory is about the base plus 90 bytes per row."""
+    say = log or (lambda m: None)
+    dev = base.device
+    n = base.shape[0]
+    k = coarse * fine
+    xs = base[torch.randint(0, n, (min(n, sample),), device=dev)].to(torch.float32)
+    cents, leaves = two_level_leaves(xs, coarse, fine, iters, seed)
+    del xs
+    say(f"l2 build: {coarse} x {fine} leaves")
+    cnorm = (cents * cents).sum(1)
+    lnorm = (leaves * leaves).sum(1)
+    cell = torch.empty(n, dtype=torch.int16, device=dev)
+    for s0 in range(0, n, chunk):
+        x = base[s0:s0 + chunk].to(torch.float32)
+        cell[s0:s0 + x.shape[0]] = (2.0 * x @ cents.T - cnorm).argmax(1).to(torch.int16)
+    order = torch.argsort(cell.to(torch.int32), stable=True)
+    counts = torch.bincount(cell.to(torch.int32), minlength=coarse).tolist()
+    del cell
+    leaf = torch.empty(n, dtype=torch.int32, device=dev)
+    o = 0
+    for c in range(coarse):
+        for s0 in range(o, o + counts[c], chunk):
+            rows = order[s0:min(s0 + chunk, o + counts[c])]
+            x = base[rows].to(torch.float32)
+            lc = leaves[c * fine:(c + 1) * fine]
+            leaf[rows] = ((2.0 * x @ lc.T - lnorm[c * fine:(c + 1) * fine]).argmax(1) + c * fine).to(torch.int32)
+        o += counts[c]
+    del order
+    say("l2 build: rows assigned")
+    signs = rotation_signs(11, PD)
+    leaves_rot = rotate(pad(leaves), signs)
+    codes = torch.empty(n, PD // 8, dtype=torch.uint8, device=dev)
+    mul = torch.empty(n, dtype=torch.float32, device=dev)
+    nx = torch.empty(n, dtype=torch.float32, device=dev)
+    for s0 in range(0, n, chunk):
+        x = base[s0:s0 + chunk].to(torch.float32)
+        lf = leaf[s0:s0 + x.shape[0]].to(torch.int64)
+        c_, m_, _ = rabitq_encode(rotate(pad(x), signs), leaves_rot[lf])
+        codes[s0:s0 + x.shape[0]], mul[s0:s0 + x.shape[0]], nx[s0:s0 + x.shape[0]] = c_, m_, seq_norm2(x)
+    say("l2 build: encoded")
+    order = torch.argsort(leaf, stable=True)            # rows of each leaf ascending
+    sizes = torch.bincount(leaf, minlength=k)
+    del leaf
+    nbl = (sizes + BLOCK - 1) // BLOCK
+    block_off = torch.zeros(k + 1, dtype=torch.int64, device=dev)
+    block_off[1:] = torch.cumsum(nbl, 0)
+    first = torch.zeros(k + 1, dtype=torch.int64, device=dev)
+    first[1:] = torch.cumsum(sizes, 0)
+    S = int(block_off[-1]) * BLOCK
+    ids = torch.full((S,), -1, dtype=torch.int32, device=dev)   # PAD
+    slot_codes = torch.zeros(S, PD // 8, dtype=torch.uint8, device=dev)
+    factors = torch.zeros(S, 2, dtype=torch.float32, device=dev)
+    lsorted = torch.repeat_interleave(torch.arange(k, device=dev), sizes)
+    for s0 in range(0, n, chunk):
+        rows = order[s0:s0 + chunk]
+        lf = lsorted[s0:s0 + rows.shape[0]]
+        pos = block_off[lf] * BLOCK + torch.arange(s0, s0 + rows.shape[0], device=dev) - first[lf]
+        ids[pos] = rows.to(torch.int32)
+        slot_codes[pos] = codes[rows]
+        factors[pos, 0] = mul[rows]
+        factors[pos, 1] = nx[rows]
+    del order, lsorted, codes, mul, nx
+    nb = S // BLOCK
+    blocks = torch.empty(nb, 512, dtype=torch.uint8, device=dev)
+    step = chunk // BLOCK
+    for b0 in range(0, nb, step):
+        blocks[b0:b0 + step] = pack_blocks(slot_codes[b0 * BLOCK:(b0 + step) * BLOCK], PD // 4)
+    del slot_codes
+    say("l2 build: packed")
+    return L2Arrays(leaves=leaves, block_off=block_off, blocks=blocks, ids=ids, factors=factors, rotation=signs,
+                    base=base)
+
+
+def exact_l2_topk(base, q, k=32, chunk=1 << 22):
+    """Exact top-k by 2 <q, x> - |x|^2 (nearest in L2) over every row: integer-valued products and
+    sums below 2^24 are exact in f32. Returns (ids, scores) nq x k, best first."""
+    qf = q.to(torch.float32)
+    best_s = torch.full((q.shape[0], k), -float("inf"), device=q.device)
+    best_i = torch.zeros((q.shape[0], k), dtype=torch.int64, device=q.device)
+    for s0 in range(0, base.shape[0], chunk):
+        x = base[s0:s0 + chunk].to(torch.float32)
+        sc = 2.0 * qf @ x.T - (x * x).sum(1)
+        v, i = torch.topk(sc, min(k, x.shape[0]), dim=1)
+        cs = torch.cat([best_s, v], 1)
+        ci = torch.cat([best_i, i + s0], 1)
+        # best first, ties by row id (stable over the concatenation order)
+        o = torch.argsort(cs, dim=1, descending=True, stable=True)[:, :k]
+        best_s, best_i = torch.gather(cs, 1, o), torch.gather(ci, 1, o)
+    return best_i, best_s
+
+
+__all__ = ["L2Arrays", "L2Mixture", "int8_mixture", "build_l2", "build_l2_large", "exact_l2_topk", "f32_div"]

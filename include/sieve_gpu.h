@@ -228,7 +228,7 @@ int sieve_index_set_scan_chunk(sieve_index* ix, uint32_t chunk_blocks);
  * query's best cand entries so far raise its threshold. A query whose list overflows is
  * rescanned alone (sieve_stats.rescanned). Used for batches of at least min_queries queries
  * without an allow filter (smaller or filtered batches run scan variant 2). Results equal the
- * per-query scan's. 0 keeps a setting's default (min_queries 64, seed_slots 32 * cand, ext_cap
+ * per-query scan's. 0 keeps a setting's default (min_queries 64, seed_slots 24 * cand, ext_cap
  * 64 * cand, chunk_blocks 64, phases 2). */
 int sieve_index_set_cluster_scan(sieve_index* ix, uint32_t min_queries, uint32_t seed_slots, uint32_t ext_cap,
                                  uint32_t chunk_blocks, uint32_t phases);
