@@ -19,6 +19,7 @@
 #include <float.h>
 
 #include "kernels.hpp"
+#include "select.hpp"
 
 namespace sieve {
 
@@ -299,7 +300,51 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
             }
         }
     } else {
-        kth = block_kth_u32(a.k, n, [&](int i) { return f2key(approx(i) + 0.0f); }, hist, sh);
+        // Large K: a histogram of the keys' top 12 bits (in cids) finds the bin of the n-th
+        // largest; the keys from that bin up (at most MAX_RESCORE, in `sorted`) are selected in
+        // LDS. Three passes over the K scores instead of five; the 8-bit radix over memory
+        // remains for a bin too crowded to collect.
+        static_assert(MAX_RESCORE == 4096, "the 12-bit histogram lives in cids");
+        auto key = [&](int i) { return f2key(approx(i) + 0.0f); };
+        for (int i = t; i < 4096; i += 256) cids[i] = 0;
+        __syncthreads();
+        for (int i = t; i < a.k; i += 256) atomicAdd(&cids[key(i) >> 20], 1u);
+        __syncthreads();
+        if (t < 64) {
+            // lane l sums bins 4095 - 64 l .. 4032 - 64 l (top first)
+            uint32_t sl = 0;
+            for (int b = 0; b < 64; b++) sl += cids[4095 - 64 * t - b];
+            const uint32_t incl = wave_incl_scan(sl);
+            const uint64_t hit = __ballot(incl >= (uint32_t)n);
+            const int f = __ffsll((long long)hit) - 1;
+            if (t == f) {
+                uint32_t cum = incl - sl;
+                for (int b = 0; b < 64; b++) {
+                    const uint32_t c = cids[4095 - 64 * t - b];
+                    if (cum + c >= (uint32_t)n) {
+                        sh[0] = 4095 - 64 * t - b;  // the bin
+                        sh[1] = cum + c;            // keys in it and above
+                        break;
+                    }
+                    cum += c;
+                }
+            }
+        }
+        if (t == 0) ncand = 0;
+        __syncthreads();
+        const uint32_t bin = sh[0], upto = sh[1];
+        if (upto <= (uint32_t)MAX_RESCORE) {
+            for (int i = t; i < a.k; i += 256) {
+                const uint32_t x = key(i);
+                if ((x >> 20) >= bin) sorted[atomicAdd(&ncand, 1u)] = make_uint2(x, ~(uint32_t)i);
+            }
+            __syncthreads();
+            kth = block_kth_largest(sorted, (int)upto, n, hist, sh).x;
+        } else {
+            kth = block_kth_u32(a.k, n, key, hist, sh);
+        }
+        if (t == 0) ncand = 0;
+        __syncthreads();
         const float e = a.err_coef * a.qnorm[qi];
         const float kv = key2f(kth);
         thr = kv - 2.2f * e - 1e-6f * (1.0f + fabsf(kv));

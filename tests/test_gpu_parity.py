@@ -121,6 +121,39 @@ def test_route_over_many_centroids_matches_oracle(wide, cfg):
         assert_same_up_to_ties(c[i, :n[i]], s[i, :n[i]], wc, ws)
 
 
+@pytest.fixture(scope="module")
+def huge_k(small):
+    """The small index with 20000 more, empty clusters (random unit centroids, some of them
+    near the queries): routing selects over 20048 centroids, the histogram-and-collect select."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(21)
+    extra = 20000
+    h = dict(small.host)
+    c = rng.standard_normal((extra, 256)).astype(np.float32)
+    c[:200] += 3.0 * small.q256[rng.integers(0, small.nq, 200)]  # some close to the queries
+    c /= np.linalg.norm(c, axis=1, keepdims=True)
+    h["centroids"] = np.concatenate([h["centroids"], c.astype(np.float32)])
+    h["block_off"] = np.concatenate([h["block_off"], np.full(extra, h["block_off"][-1], np.uint64)])
+    return h, O.Index(h), Index.create(h)
+
+
+@pytest.mark.parametrize("cfg", [dict(nprobe=32), dict(nprobe=200, margin=0.1), dict(nprobe=1)])
+def test_route_over_20k_centroids_matches_oracle(small, huge_k, cfg):
+    h, oix, gix = huge_k
+    p = Params(cand=100, **cfg)
+    s, c, n = gix.route(small.q256, p)
+    for i in range(small.nq):
+        ws, wc = oix.route(small.q256[i], p)
+        assert n[i] == len(wc)
+        assert_same_up_to_ties(c[i, :n[i]], s[i, :n[i]], wc, ws)
+    # the searches over it still equal the oracle's (empty clusters scan nothing)
+    p = Params(nprobe=24, cand=300, k=10, rerank=Rerank.FULL)
+    gs, gr, gn = gix.search(small.q768, small.q256, p)
+    ws, wr, wn, _ = oix.search_batch(small.q768, small.q256, p, threads=4)
+    for i in range(small.nq):
+        assert_same_up_to_ties(gr[i, :gn[i]], gs[i, :gn[i]], wr[i, :wn[i]], ws[i, :wn[i]])
+
+
 # ---------------------------------------------------------------------- candidates ------
 def _oracle_probes(oracle_index, q256, p, nq):
     stride = p.nprobe
