@@ -820,6 +820,14 @@ int sieve_index_set_cluster_scan(sieve_index* ix, uint32_t min_queries, uint32_t
     });
 }
 
+int sieve_l2_index_set_scan_variant(sieve_l2_index* ix, uint32_t variant) {
+    return guarded([&] {
+        SIEVE_REQUIRE(ix, SIEVE_EINVAL, "null index");
+        SIEVE_REQUIRE(variant < (uint32_t)SCAN_VARIANT_CM, SIEVE_EINVAL, "no such scan variant for the L2 index");
+        ix->scan_variant = (int)variant;
+    });
+}
+
 int sieve_l2_index_set_scan_chunk(sieve_l2_index* ix, uint32_t chunk_blocks) {
     return guarded([&] {
         SIEVE_REQUIRE(ix, SIEVE_EINVAL, "null index");

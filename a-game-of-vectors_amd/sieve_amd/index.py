@@ -368,6 +368,10 @@ class L2Index:
         """Chunk size (32-slot blocks) of the scan's work split; 0 = chosen per batch."""
         abi.check(abi.lib().sieve_l2_index_set_scan_chunk(self._h, blocks))
 
+    def set_scan_variant(self, v):
+        """One of the per-query scan variants (0 .. 9)."""
+        abi.check(abi.lib().sieve_l2_index_set_scan_variant(self._h, v))
+
     def search(self, q, nprobe, cand, k, margin=0.0, timing=None, stream=None, details=False):
         """bigann::Index::search for a batch of int8 queries (nq x 100): (scores, rows, counts)
         and, with details, (probes, probe_counts, cand_rows, cand_est, cand_counts)."""

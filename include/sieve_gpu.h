@@ -370,6 +370,9 @@ int sieve_l2_index_create(const sieve_l2_desc* desc, sieve_l2_index** out);
 void sieve_l2_index_free(sieve_l2_index* ix);
 /* As sieve_index_set_scan_chunk, for the L2 index. */
 int sieve_l2_index_set_scan_chunk(sieve_l2_index* ix, uint32_t chunk_blocks);
+/* As sieve_index_set_scan_variant, for the L2 index: one of the per-query scan variants
+ * (0 .. 9; the cluster-major scan serves the inner-product index only). */
+int sieve_l2_index_set_scan_variant(sieve_l2_index* ix, uint32_t variant);
 
 /* bigann::Index::search (src/bigann.rs:652-657) for nq queries (nq x 100 i8): the top k
  * (score = |q|^2 - |q - x|^2 - |q|^2 = 2 <q, x> - |x|^2, row) best first. Optional outputs

@@ -92,3 +92,17 @@ def test_gpu_l2_matches_oracle(l2small, l2oracle, nprobe, cand, margin, chunk):
         assert set(probes[i, :pn[i]].tolist()) == set(wp.tolist())
         assert _ties_equal(cr[i, :cn[i]], ce[i, :cn[i]], wc, we)
         assert _ties_equal(r[i, :n[i]], s[i, :n[i]], wr, ws)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", list(range(10)))
+def test_gpu_l2_scan_variants_match_oracle(l2small, l2oracle, variant):
+    from sieve_amd.index import L2Index
+    h, qs = l2small
+    ix = L2Index(h)
+    ix.set_scan_variant(variant)
+    s, r, n, probes, pn, cr, ce, cn = ix.search(qs, 16, 700, 10, details=True)
+    for i, q in enumerate(qs):
+        wp, wc, we, ws, wr = l2oracle.search(q, 16, 700, 10)
+        assert _ties_equal(cr[i, :cn[i]], ce[i, :cn[i]], wc, we)
+        assert _ties_equal(r[i, :n[i]], s[i, :n[i]], wr, ws)
