@@ -300,11 +300,13 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
             }
         }
     } else {
-        // Large K: a histogram of the keys' top 12 bits (in cids) finds the bin of the n-th
-        // largest; the keys from that bin up (at most MAX_RESCORE, in `sorted`) are selected in
-        // LDS. Three passes over the K scores instead of five; the 8-bit radix over memory
-        // remains for a bin too crowded to collect.
-        static_assert(MAX_RESCORE == 4096, "the 12-bit histogram lives in cids");
+        // Large K: a 4096-bin histogram of the keys' top 12 bits (in cids) finds the bin of the
+        // n-th largest score. Every score the band keeps (>= kth - 2.2 e - 1e-6 (1 + |kth|)) is
+        // at least that bin's lower edge minus the same margin, so one more pass collects a
+        // superset of the top n and the band into `sorted`; the n-th and the band then come from
+        // LDS: two passes over the K scores. A collection that does not fit falls back to the
+        // 8-bit radix over memory and a band pass.
+        static_assert(MAX_RESCORE == 4096, "the histogram lives in cids");
         auto key = [&](int i) { return f2key(approx(i) + 0.0f); };
         for (int i = t; i < 4096; i += 256) cids[i] = 0;
         __syncthreads();
@@ -323,7 +325,7 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
                     const uint32_t c = cids[4095 - 64 * t - b];
                     if (cum + c >= (uint32_t)n) {
                         sh[0] = 4095 - 64 * t - b;  // the bin
-                        sh[1] = cum + c;            // keys in it and above
+                        sh[1] = cum + c;            // scores in it and above
                         break;
                     }
                     cum += c;
@@ -333,25 +335,38 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
         if (t == 0) ncand = 0;
         __syncthreads();
         const uint32_t bin = sh[0], upto = sh[1];
-        if (upto <= (uint32_t)MAX_RESCORE) {
-            for (int i = t; i < a.k; i += 256) {
-                const uint32_t x = key(i);
-                if ((x >> 20) >= bin) sorted[atomicAdd(&ncand, 1u)] = make_uint2(x, ~(uint32_t)i);
+        const float e = a.err_coef * a.qnorm[qi];
+        const float lo = key2f(bin << 20), hi = key2f((bin << 20) | 0xFFFFFu);
+        const float lim = lo - 2.2f * e - 1e-6f * (1.0f + fmaxf(fabsf(lo), fabsf(hi)));
+        for (int i = t; i < a.k; i += 256) {
+            const float v = approx(i);
+            if (v >= lim) {
+                const uint32_t j = atomicAdd(&ncand, 1u);
+                if (j < (uint32_t)MAX_RESCORE) sorted[j] = make_uint2(f2key(v + 0.0f), ~(uint32_t)i);
             }
+        }
+        __syncthreads();
+        const uint32_t got = ncand;
+        __syncthreads();
+        if (got <= (uint32_t)MAX_RESCORE && got >= upto) {
+            kth = block_kth_largest(sorted, (int)got, n, hist, sh).x;
+            const float kv = key2f(kth);
+            thr = kv - 2.2f * e - 1e-6f * (1.0f + fabsf(kv));
+            if (t == 0) ncand = 0;
             __syncthreads();
-            kth = block_kth_largest(sorted, (int)upto, n, hist, sh).x;
+            for (uint32_t j = t; j < got; j += 256)
+                if (key2f(sorted[j].x) >= thr) cids[atomicAdd(&ncand, 1u)] = ~sorted[j].y;
         } else {
             kth = block_kth_u32(a.k, n, key, hist, sh);
-        }
-        if (t == 0) ncand = 0;
-        __syncthreads();
-        const float e = a.err_coef * a.qnorm[qi];
-        const float kv = key2f(kth);
-        thr = kv - 2.2f * e - 1e-6f * (1.0f + fabsf(kv));
-        for (int i = t; i < a.k; i += 256) {
-            if (approx(i) >= thr) {
-                uint32_t j = atomicAdd(&ncand, 1u);
-                if (j < MAX_RESCORE) cids[j] = (uint32_t)i;
+            const float kv = key2f(kth);
+            thr = kv - 2.2f * e - 1e-6f * (1.0f + fabsf(kv));
+            if (t == 0) ncand = 0;
+            __syncthreads();
+            for (int i = t; i < a.k; i += 256) {
+                if (approx(i) >= thr) {
+                    uint32_t j = atomicAdd(&ncand, 1u);
+                    if (j < MAX_RESCORE) cids[j] = (uint32_t)i;
+                }
             }
         }
     }
