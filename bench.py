@@ -128,6 +128,8 @@ def main():
     ap.add_argument("--cm-sweep", default=None,
                     help="comma-separated cluster-major scan settings (as --cluster-scan) to time on the picked "
                          "configuration with scan variants 10 and 11 before the timed run")
+    ap.add_argument("--streams", type=int, default=1,
+                    help="host threads, each driving its own stream, over which the timed steps are split")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run an elementwise kernel that reads 1 GiB and writes 1 GiB (the known size that "
                          "scales a profile's FETCH_SIZE/WRITE_SIZE counters), then exit")
@@ -250,26 +252,49 @@ def main():
     log(f"probe sets equal to the FP64 top-{p.nprobe}: {agree['equal']:.4f} of {ngt} queries "
         f"({agree['equal_up_to_ties']:.4f} up to f32-resolution ties)")
 
-    # Timed region: W warmup + K steps of the batched search, inputs already in HBM.
-    for _ in range(a.warmup):
-        run(p)
+    # Timed region: W warmup + K steps of the batched search, inputs already in HBM. With
+    # --streams N, N host threads each drive their own stream (the index is shared read-only,
+    # as the C ABI allows), so up to N batches are on the GPU at once; the K steps are split
+    # among them.
+    import threading
+    streams = [st] + [ix.stream() for _ in range(a.streams - 1)]
+    for sx in streams:
+        for _ in range(a.warmup):
+            ix.search(q768, q256, p, stream=sx)
     torch.cuda.synchronize()
-    scan_ns, route_ns, rerank_ns, scanned, id_reads, probed, rescanned = 0.0, 0.0, 0.0, 0, 0, 0, 0
+    step_stats, lock = [], threading.Lock()
+
+    def worker(sx, count):
+        for _ in range(count):
+            ix.search(q768, q256, p, stream=sx)
+            stt = sx.last_stats()
+            with lock:
+                step_stats.append(stt)
+
+    counts = [a.steps // a.streams + (1 if i < a.steps % a.streams else 0) for i in range(a.streams)]
+    workers = [threading.Thread(target=worker, args=(streams[i], counts[i])) for i in range(a.streams)]
     t0 = time.perf_counter()
-    for _ in range(a.steps):
-        run(p)
-        stt = st.last_stats()
-        scan_ns += stt["scan_ns"]
-        route_ns += stt["route_ns"]
-        rerank_ns += stt["rerank_ns"]
-        scanned += stt["scanned"]
-        id_reads += stt["id_reads"]
-        probed += stt["probed"]
-        rescanned += stt["rescanned"]
+    for w in workers:
+        w.start()
+    for w in workers:
+        w.join()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / a.steps
     qps = nq * a.steps / wall
+    if a.streams > 1:
+        # Stage times of the kernels alone (the roofline): a few steps on one stream.
+        step_stats = []
+        for _ in range(3):
+            worker(st, 1)
+    k_steps = len(step_stats)
+    scan_ns = sum(x["scan_ns"] for x in step_stats) * a.steps / k_steps
+    route_ns = sum(x["route_ns"] for x in step_stats) * a.steps / k_steps
+    rerank_ns = sum(x["rerank_ns"] for x in step_stats) * a.steps / k_steps
+    scanned = sum(x["scanned"] for x in step_stats) * a.steps // k_steps
+    id_reads = sum(x["id_reads"] for x in step_stats) * a.steps // k_steps
+    probed = sum(x["probed"] for x in step_stats) * a.steps // k_steps
+    rescanned = sum(x["rescanned"] for x in step_stats) * a.steps // k_steps
     s, r, n = run(p)
     torch.cuda.synchronize()
     rec = recall10(r[:ngt].cpu().numpy(), n[:ngt].cpu().numpy().view(np.uint32), gt)
@@ -335,6 +360,7 @@ def main():
         "per_query_gpu_qps": per_query_qps,
         "serving_open_loop": serving,
         "serving_mixed": mixed,
+        "streams": a.streams,
         "scan_variant": variant,
         "scan_variant_ms": variant_ms,
         "cluster_scan": a.cluster_scan,
