@@ -1013,14 +1013,17 @@ void launch_scan_cm(const ScanArgs& a_in, const CmScratch& cm_in, size_t nq, int
     hipLaunchKernelGGL(k_cm_offsets, dim3(1), dim3(SEL_THREADS), 0, st, a, cm);
     hipLaunchKernelGGL(k_cm_scatter, dim3(qgrid), dim3(256), 0, st, a, cm);
     hipLaunchKernelGGL(k_cm_items, dim3((cm.nseg + 3) / 4), dim3(256), 0, st, a, cm);
-    if (lds > 64 * 1024) {
-        (void)hipFuncSetAttribute((const void*)k_scan_t<64, false, 2, false, 0, SCAN_SEED>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    // The seed pass streams each query's nearest probes: 4 slots per thread keep more of its
+    // short scan in flight.
+    const size_t lds_seed = scan_layout(a.cand, a.seg_stride, 4, a.trigger).total;
+    if (lds_seed > 64 * 1024)
+        (void)hipFuncSetAttribute((const void*)k_scan_t<64, false, 4, false, 0, SCAN_SEED>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_seed);
+    if (lds > 64 * 1024)
         (void)hipFuncSetAttribute((const void*)k_scan_t<64, false, 2, false, 0, SCAN_FINISH>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    }
-    hipLaunchKernelGGL((k_scan_t<64, false, 2, false, 0, SCAN_SEED>), dim3((unsigned)nq), dim3(SCAN_THREADS), lds,
-                       st, a);
+    hipLaunchKernelGGL((k_scan_t<64, false, 4, false, 0, SCAN_SEED>), dim3((unsigned)nq), dim3(SCAN_THREADS),
+                       lds_seed, st, a);
 
     CmArgs c{};
     c.codes = a.codes;
