@@ -190,7 +190,7 @@ def two_level_leaves(x, coarse, fine, iters, seed):
     return cents, leaves
 
 
-def build_l2_large(base, coarse=512, fine=256, iters=8, seed=7, sample=1 << 22, chunk=1 << 22, log=None):
+def build_l2_large(base, coarse=512, fine=256, iters=8, seed=7, sample=1 << 24, chunk=1 << 22, log=None):
     """build_l2 for base tables too large for whole-table temporaries (1B x 100 on one GPU):
     two-level leaves; each row assigned to the nearest of its coarse cell's leaves (rows sorted
     by cell, one GEMM per cell); codes, mul and |x|^2 per chunk; slots sorted by (leaf, row)
