@@ -636,12 +636,15 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
         launch_query_prep_l2(dq, nq, ix->rotation, std::pow(128.0f, -1.5f), prep, s->st);
         const size_t per = std::max<size_t>(1, std::min<size_t>(nq, (size_t(1) << 30) / (4 * (size_t)ix->k)));
         float* approx = grow<float>(s->b[B_APPROX], per * ix->k);
+        float* tmax = grow<float>(s->b[B_TMAX], per * ((ix->k + 15) / 16));
         for (size_t q0 = 0; q0 < nq; q0 += per) {
             size_t m = std::min(per, nq - q0);
-            launch_gemm_bf16_route(prep.qbf + q0 * PDIM, ix->leaves_bf, approx, (int)m, (int)ix->k, ix->k, s->st);
+            launch_gemm_bf16_route(prep.qbf + q0 * PDIM, ix->leaves_bf, ix->leaf_norm, approx, tmax, (int)m,
+                                   (int)ix->k, ix->k, s->st);
             RouteArgs ra{};
             ra.approx = approx;
             ra.lds = ix->k;
+            ra.tmax = tmax;
             ra.k = (int)ix->k;
             ra.cents = ix->leaves;
             ra.q256 = nullptr;

@@ -156,7 +156,7 @@ enum Slot {
     B_Q256, B_Q768, B_TABLES, B_QAB, B_WQ768, B_WQ256, B_INV, B_QBF, B_QNORM, B_APPROX, B_PS, B_PC, B_PN, B_CE, B_CR,
     B_CX, B_CN, B_OS, B_OR, B_ON, B_DEL, B_ALLOW, B_AFLAGS, B_CNT, B_OVF, B_TMP1, B_TMP2, B_TMP3, B_QQ, B_QF,
     B_PART, B_PARTN, B_GTHR, B_QDONE, B_CM_LIM, B_CM_SEGC, B_CM_QOFF, B_CM_IOFF, B_CM_Q, B_CM_CENT, B_CM_SEED,
-    B_CM_SEEDN, B_CM_THR, B_CM_EXT, B_CM_EXTN, B_CM_WORK, B_CM_ITEMS,
+    B_CM_SEEDN, B_CM_THR, B_CM_EXT, B_CM_EXTN, B_CM_WORK, B_CM_ITEMS, B_TMAX,
     B_NSLOTS
 };
 
@@ -294,12 +294,15 @@ inline void route_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt,
     SIEVE_HIP(hipMemsetAsync(ovf, 0, 4, s->st));
     const size_t per = std::max<size_t>(1, std::min<size_t>(nq, (size_t(1) << 30) / (4 * (size_t)ix->k)));
     float* approx = grow<float>(s->b[B_APPROX], per * ix->k);
+    float* tmax = grow<float>(s->b[B_TMAX], per * ((ix->k + 15) / 16));
     for (size_t q0 = 0; q0 < nq; q0 += per) {
         size_t m = std::min(per, nq - q0);
-        launch_gemm_bf16_route(bt.prep.qbf + q0 * PDIM, ix->cents_bf, approx, (int)m, (int)ix->k, ix->k, s->st);
+        launch_gemm_bf16_route(bt.prep.qbf + q0 * PDIM, ix->cents_bf, nullptr, approx, tmax, (int)m, (int)ix->k, ix->k,
+                               s->st);
         RouteArgs ra{};
         ra.approx = approx;
         ra.lds = ix->k;
+        ra.tmax = tmax;
         ra.k = (int)ix->k;
         ra.cents = ix->cents;
         ra.q256 = bt.q256 + q0 * PDIM;

@@ -54,12 +54,16 @@ void launch_count_bad_ids(const uint32_t* ids, size_t nslots, uint64_t n, unsign
 // C = A * B^T in f32 (m x k, n x k -> m x n, row-major), f32 MFMA.
 void launch_gemm_f32_abt(const float* a, const float* b, float* c, int m, int n, int k, hipStream_t st);
 // S = Qbf * Cbf^T (m x 256, n x 256 BF16 -> m x n f32, row stride lds), BF16 MFMA.
-void launch_gemm_bf16_route(const uint16_t* qbf, const uint16_t* cbf, float* s, int m, int n, size_t lds,
-                            hipStream_t st);
+// S (row stride lds) = qbf (m x 256 BF16) * cbf (n x 256 BF16)^T; tmax (m x ceil(n / 16)): the
+// largest approximate route score of each 16 consecutive centroids (S, or 2 S - cnorm when
+// cnorm is given).
+void launch_gemm_bf16_route(const uint16_t* qbf, const uint16_t* cbf, const float* cnorm, float* s, float* tmax,
+                            int m, int n, size_t lds, hipStream_t st);
 
 struct RouteArgs {
     const float* approx;       // nq x k approximate scores (row stride lds)
     size_t lds;
+    const float* tmax;         // nq x ceil(k / 16) tile maxima of the approximate scores, or null
     int k;
     const float* cents;        // k x 256 f32
     const float* q256;         // nq x 256

@@ -1,9 +1,11 @@
 // Centroid routing: Index::route (src/ivf.rs:587-613) over a batch of queries.
 //
 // 1. k_gemm_bf16_route: S = Q * C^T with v_mfma_f32_16x16x32_bf16 (BF16 inputs, f32
-//    accumulation) for every (query, centroid).
-// 2. k_route_select, one workgroup per query: the nprobe-th largest approximate score by a
-//    4-pass radix select; every centroid whose approximate score lies within twice the BF16
+//    accumulation) for every (query, centroid), and the maximum of every 16 consecutive
+//    scores of a query.
+// 2. k_route_select, one workgroup per query: the nprobe-th largest approximate score (a radix
+//    select over the tile maxima, then over the scores of the tiles that can hold the top
+//    nprobe); every centroid whose approximate score lies within twice the BF16
 //    error bound of it is rescored in f32 exactly as the reference's gemm_abt fallback sums
 //    (src/blas.rs:47-53: products left to right from -0.0, no FMA); the rescored scores are
 //    sorted (score desc, id asc) as TopK::into_sorted (src/topk.rs:57-61); then the margin
@@ -18,6 +20,9 @@
 // sieve_gemm_abt (blas::gemm_abt, src/blas.rs:5-10) uses k_gemm_f32_abt: v_mfma_f32_16x16x4_f32.
 #include <float.h>
 
+#include <algorithm>
+#include <type_traits>
+
 #include "kernels.hpp"
 #include "select.hpp"
 
@@ -27,61 +32,126 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------ BF16 routing GEMM ---
-// Workgroup tile 128 x 128 (4 waves in 2 x 2, each 64 x 64 = 4 x 4 MFMA tiles), K = 256.
-// MFMA 16x16x32 operand layout: lane l supplies row (l % 16) and k-octet (l / 16) of each
-// operand; the accumulator holds rows 4 * (l / 16) + i, column l % 16.
-__global__ __launch_bounds__(256) void k_gemm_bf16_route(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B,
-                                                         float* __restrict__ S, int m, int n, size_t lds) {
+// S = Q C^T for every (query, centroid) with v_mfma_f32_16x16x32_bf16, K = 256. Each wave
+// owns 64 centroids, whose BF16 rows stay in registers (4 MFMA tiles x 8 k-steps, 128 VGPRs),
+// and walks the 16-query tiles of its workgroup's query range, loading the next tile's
+// operand while the current tile's 32 MFMAs run: centroid rows are read once per workgroup
+// and the kernel is bound by writing S. The centroids are the MFMA's first operand, so lane
+// l holds centroids 4 (l / 16) + v, v < 4, of tile j for query l % 16 (one float4 store).
+// tmax (row stride ceil(n / 16)) receives the largest approximate route score of each 16
+// consecutive centroids: S itself, or the L2 route score 2 S - |c|^2 (L2).
+template <bool L2>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_bf16_route(
+    const uint16_t* __restrict__ Q, const uint16_t* __restrict__ C, const float* __restrict__ cnorm,
+    float* __restrict__ S, float* __restrict__ tmax, int m, int n, size_t lds, int qtiles_per) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tm = blockIdx.y * 128 + (wave >> 1) * 64;
-    const int tn = blockIdx.x * 128 + (wave & 1) * 64;
-    const int r16 = lane & 15, ko = (lane >> 4) * 8;
-    f32x4 acc[4][4];
+    const int cb = blockIdx.x * 256 + wave * 64;
+    const int qtiles = (m + 15) / 16;
+    const int qt0 = blockIdx.y * qtiles_per, qt1 = min(qt0 + qtiles_per, qtiles);
+    if (cb >= n || qt0 >= qt1) return;  // wave-uniform; the kernel has no block barrier
+    const int r16 = lane & 15, ko = (lane >> 4) * 8, c4 = (lane >> 4) * 4;
+    const int ntile = (n + 15) / 16;
+    bf16x8 bc[4][8];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) {
+        const uint16_t* row = C + (size_t)min(cb + j * 16 + r16, n - 1) * PDIM + ko;
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const uint16_t* arow[4];
-    const uint16_t* brow[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        int r = min(tm + i * 16 + r16, m - 1);
-        int c = min(tn + i * 16 + r16, n - 1);
-        arow[i] = A + (size_t)r * PDIM + ko;
-        brow[i] = B + (size_t)c * PDIM + ko;
+        for (int s = 0; s < 8; s++) bc[j][s] = *reinterpret_cast<const bf16x8*>(row + 32 * s);
     }
-#pragma unroll 2
-    for (int k0 = 0; k0 < PDIM; k0 += 32) {
-        bf16x8 a[4], b[4];
+    float cn[4][4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            a[i] = *reinterpret_cast<const bf16x8*>(arow[i] + k0);
-            b[i] = *reinterpret_cast<const bf16x8*>(brow[i] + k0);
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int c = cb + j * 16 + c4 + v;
+            cn[j][v] = (L2 && c < n) ? cnorm[c] : 0.0f;
         }
+    const bool vec = (lds & 3) == 0;
+    auto load_q = [&](bf16x8(&dst)[8], int qt) {
+        const uint16_t* row = Q + (size_t)min(qt * 16 + r16, m - 1) * PDIM + ko;
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int s = 0; s < 8; s++) dst[s] = *reinterpret_cast<const bf16x8*>(row + 32 * s);
+    };
+    // One 16-query tile: 32 MFMAs, then S and the tile maxima. FULL: every query and centroid
+    // of the tile exists and rows are float4-aligned, so every store is unconditional (all
+    // four lanes of a query store its same tile maximum), which keeps the memory waits exact.
+    auto tile = [&](auto full, const bf16x8(&cur)[8], int qt) {
+        constexpr bool FULL = decltype(full)::value;
+        f32x4 acc[4];
 #pragma unroll
-            for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-    }
+        for (int j = 0; j < 4; j++) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+        for (int s = 0; s < 8; s++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bc[j][s], cur[s], acc[j], 0, 0, 0);
+        const int q = qt * 16 + r16;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            int col = tn + j * 16 + r16;
-            if (col >= n) continue;
+            const int c0 = cb + j * 16 + c4;
+            float* dst = S + (size_t)q * lds + c0;
+            float mx = -INFINITY;
+            if constexpr (FULL) {
+                *reinterpret_cast<f32x4*>(dst) = acc[j];
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                int row = tm + i * 16 + (lane >> 4) * 4 + v;
-                if (row < m) S[(size_t)row * lds + col] = acc[i][j][v];
+                for (int v = 0; v < 4; v++) mx = fmaxf(mx, L2 ? 2.0f * acc[j][v] - cn[j][v] : acc[j][v]);
+            } else {
+                if (q < m) {
+                    if (vec && c0 + 3 < n) {
+                        *reinterpret_cast<f32x4*>(dst) = acc[j];
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < 4; v++)
+                            if (c0 + v < n) dst[v] = acc[j][v];
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    if (c0 + v < n) mx = fmaxf(mx, L2 ? 2.0f * acc[j][v] - cn[j][v] : acc[j][v]);
             }
+            mx = fmaxf(mx, __shfl_xor(mx, 16, 64));
+            mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+            const int tl = (cb >> 4) + j;
+            if (FULL || (lane < 16 && q < m && tl < ntile)) tmax[(size_t)q * ntile + tl] = mx;
         }
+    };
+    // Full tiles in pairs, the two operand buffers taking turns (each tile's loads are issued
+    // a whole tile ahead); then the rest one by one.
+    const int qfull = (vec && cb + 64 <= n) ? min(qt1, m / 16) : qt0;
+    bf16x8 aq[8], an[8];
+    load_q(aq, qt0);
+    int qt = qt0;
+    for (; qt + 1 < qfull; qt += 2) {
+        load_q(an, qt + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        tile(std::true_type{}, aq, qt);
+        load_q(aq, min(qt + 2, qt1 - 1));
+        __builtin_amdgcn_sched_barrier(0);
+        tile(std::true_type{}, an, qt + 1);
+    }
+    for (; qt < qt1; qt++) {
+        load_q(an, min(qt + 1, qt1 - 1));
+        __builtin_amdgcn_sched_barrier(0);
+        tile(std::false_type{}, aq, qt);
+#pragma unroll
+        for (int s = 0; s < 8; s++) aq[s] = an[s];
+    }
 }
 
-void launch_gemm_bf16_route(const uint16_t* qbf, const uint16_t* cbf, float* s, int m, int n, size_t lds,
-                            hipStream_t st) {
+void launch_gemm_bf16_route(const uint16_t* qbf, const uint16_t* cbf, const float* cnorm, float* s, float* tmax,
+                            int m, int n, size_t lds, hipStream_t st) {
     if (m == 0 || n == 0) return;
-    dim3 grid((n + 127) / 128, (m + 127) / 128);
-    hipLaunchKernelGGL(k_gemm_bf16_route, grid, dim3(256), 0, st, qbf, cbf, s, m, n, lds);
+    const int gx = (n + 255) / 256;
+    const int qtiles = (m + 15) / 16;
+    // about 1024 workgroups: two rounds of two per CU
+    int gy = std::max(1, std::min(qtiles, (1024 + gx - 1) / gx));
+    const int per = (qtiles + gy - 1) / gy;
+    gy = (qtiles + per - 1) / per;
+    if (cnorm)
+        hipLaunchKernelGGL(k_gemm_bf16_route<true>, dim3(gx, gy), dim3(256), 0, st, qbf, cbf, cnorm, s, tmax, m, n,
+                           lds, per);
+    else
+        hipLaunchKernelGGL(k_gemm_bf16_route<false>, dim3(gx, gy), dim3(256), 0, st, qbf, cbf, cnorm, s, tmax, m,
+                           n, lds, per);
 }
 
 // ------------------------------------------------------------------ f32 GEMM -----------
@@ -199,8 +269,20 @@ __device__ uint32_t block_kth_regs(const uint32_t (&x)[VALS], int n, int k, uint
         hist[t] = 0;
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < VALS; j++)
-            if (t + 256 * j < n && (x[j] & mask) == prefix) atomicAdd(&hist[(x[j] >> shift) & 255], 1u);
+        for (int j = 0; j < VALS; j++) {
+            // Scores near the top share their high digits: the lanes that hold the first active
+            // lane's digit add once for all of them.
+            const bool in = t + 256 * j < n && (x[j] & mask) == prefix;
+            const uint32_t d = (x[j] >> shift) & 255;
+            const uint64_t act = __ballot(in);
+            if (act) {
+                const int first = __ffsll((long long)act) - 1;
+                const uint32_t d0 = __shfl(d, first, 64);
+                const uint64_t same = __ballot(in && d == d0);
+                const bool lead = (t & 63) == first;
+                if (in && (lead || d != d0)) atomicAdd(&hist[d], lead ? (uint32_t)__popcll(same) : 1u);
+            }
+        }
         __syncthreads();
         if (t < 64) {
             uint32_t c0 = hist[255 - 4 * t], c1 = hist[254 - 4 * t], c2 = hist[253 - 4 * t], c3 = hist[252 - 4 * t];
@@ -250,14 +332,19 @@ __device__ void block_sort_desc(uint2* a, int n2) {
     }
 }
 
-// REG_VALS > 0: the approximate scores (k <= 256 REG_VALS) are read once into registers and
-// the select and the band pass run over them; 0: over memory (any k).
-template <int REG_VALS>
+// Route select modes. ROUTE_REGS: the approximate scores (k <= 256 VALS) are read once into
+// registers and the select and the band pass run over them. ROUTE_MEM: over memory (any k).
+// ROUTE_TILES: over the GEMM's maxima of 16 consecutive scores (ceil(k / 16) <= 256 VALS,
+// at least nprobe of them), then only the tiles that can hold the top nprobe or the band.
+enum { ROUTE_REGS = 0, ROUTE_MEM = 1, ROUTE_TILES = 2 };
+
+template <int MODE, int VALS>
 __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
     __shared__ float qs[PDIM];
     __shared__ uint32_t hist[256];
     __shared__ uint32_t sh[4];
-    __shared__ uint32_t ncand;
+    __shared__ float red[4];
+    __shared__ uint32_t ncand, ntl;
     __shared__ uint32_t cids[MAX_RESCORE];
     __shared__ uint2 sorted[MAX_RESCORE];
     const size_t qi = blockIdx.x;
@@ -269,7 +356,10 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
     } else {
         for (int i = t; i < PDIM; i += 256) qs[i] = a.q256[qi * PDIM + i];
     }
-    if (t == 0) ncand = 0;
+    if (t == 0) {
+        ncand = 0;
+        ntl = 0;
+    }
     __syncthreads();
     if (n <= 0) {
         if (t == 0) a.out_n[qi] = 0;
@@ -277,21 +367,43 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
     }
     // approximate route score of centroid i
     auto approx = [&](int i) -> float { return a.l2 ? 2.0f * S[i] - a.cnorm[i] : S[i]; };
-    uint32_t kth;
-    float thr;
-    if constexpr (REG_VALS > 0) {
-        uint32_t x[REG_VALS];
-#pragma unroll
-        for (int j = 0; j < REG_VALS; j++) {
-            const int i = t + 256 * j;
-            x[j] = i < a.k ? f2key(approx(i) + 0.0f) : 0u;
+    auto key = [&](int i) { return f2key(approx(i) + 0.0f); };
+    const float e = a.err_coef * a.qnorm[qi];
+    // The band of the n-th largest approximate score kv: every centroid whose exact score can
+    // reach the exact top n.
+    auto band_of = [&](float kv) { return kv - 2.2f * e - 1e-6f * (1.0f + fabsf(kv)); };
+    // 8-bit radix over memory, then a band pass: any k.
+    auto radix_over_memory = [&]() {
+        const uint32_t kth = block_kth_u32(a.k, n, key, hist, sh);
+        const float thr = band_of(key2f(kth));
+        if (t == 0) ncand = 0;
+        __syncthreads();
+        for (int i = t; i < a.k; i += 256) {
+            if (approx(i) >= thr) {
+                const uint32_t j = atomicAdd(&ncand, 1u);
+                if (j < MAX_RESCORE) cids[j] = (uint32_t)i;
+            }
         }
-        kth = block_kth_regs<REG_VALS>(x, a.k, n, hist, sh);
-        const float e = a.err_coef * a.qnorm[qi];
-        const float kv = key2f(kth);
-        thr = kv - 2.2f * e - 1e-6f * (1.0f + fabsf(kv));
+    };
+    // `got` candidates (key, ~id) in `sorted` hold the top n and every score >= the band's
+    // floor: the n-th largest and the band from LDS into cids.
+    auto band_from_sorted = [&](uint32_t got) {
+        const float thr = band_of(key2f(block_kth_largest(sorted, (int)got, n, hist, sh).x));
+        if (t == 0) ncand = 0;
+        __syncthreads();
+        for (uint32_t j = t; j < got; j += 256)
+            if (key2f(sorted[j].x) >= thr) cids[atomicAdd(&ncand, 1u)] = ~sorted[j].y;
+    };
+    if constexpr (MODE == ROUTE_REGS) {
+        uint32_t x[VALS];
 #pragma unroll
-        for (int j = 0; j < REG_VALS; j++) {
+        for (int j = 0; j < VALS; j++) {
+            const int i = t + 256 * j;
+            x[j] = i < a.k ? key(i) : 0u;
+        }
+        const float thr = band_of(key2f(block_kth_regs<VALS>(x, a.k, n, hist, sh)));
+#pragma unroll
+        for (int j = 0; j < VALS; j++) {
             const int i = t + 256 * j;
             // key2f(key(v)) == v + 0.0f, which compares with thr as v does
             if (i < a.k && key2f(x[j]) >= thr) {
@@ -299,6 +411,56 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
                 if (jj < MAX_RESCORE) cids[jj] = (uint32_t)i;
             }
         }
+    } else if constexpr (MODE == ROUTE_TILES) {
+        // T0, the n-th largest tile maximum, is at most the n-th largest score (n tiles hold a
+        // score >= T0), and the band's floor over kth in [T0, max] is >= lim. Every score the
+        // band or the top n needs is >= lim, and so is its tile's maximum: the tiles whose
+        // maximum is >= lim hold all of them.
+        const int nt = (a.k + 15) >> 4;
+        const float* T = a.tmax + qi * (size_t)nt;
+        uint32_t tk[VALS];
+        float mx = -INFINITY;
+#pragma unroll
+        for (int j = 0; j < VALS; j++) {
+            const int i = t + 256 * j;
+            const float v = i < nt ? T[i] : -INFINITY;
+            tk[j] = i < nt ? f2key(v + 0.0f) : 0u;
+            mx = fmaxf(mx, v);
+        }
+        mx = wave_max(mx);
+        if ((t & 63) == 0) red[t >> 6] = mx;
+        const float t0 = key2f(block_kth_regs<VALS>(tk, nt, n, hist, sh));  // ends with a barrier
+        const float top = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        const float lim = t0 - 2.2f * e - 1e-6f * (1.0f + fmaxf(fabsf(t0), fabsf(top)));
+#pragma unroll
+        for (int j = 0; j < VALS; j++) {
+            const int i = t + 256 * j;
+            if (i < nt && key2f(tk[j]) >= lim) {
+                const uint32_t p = atomicAdd(&ntl, 1u);
+                if (p < MAX_RESCORE) cids[p] = (uint32_t)i;
+            }
+        }
+        __syncthreads();
+        const uint32_t nl = ntl;
+        if (nl <= (uint32_t)MAX_RESCORE) {
+            for (uint32_t f = t; f < nl * 16; f += 256) {
+                const int c = (int)cids[f >> 4] * 16 + (int)(f & 15);
+                if (c < a.k) {
+                    const float v = approx(c);
+                    if (v >= lim) {
+                        const uint32_t j = atomicAdd(&ncand, 1u);
+                        if (j < (uint32_t)MAX_RESCORE) sorted[j] = make_uint2(f2key(v + 0.0f), ~(uint32_t)c);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t got = ncand;
+        __syncthreads();
+        if (nl <= (uint32_t)MAX_RESCORE && got <= (uint32_t)MAX_RESCORE && got >= (uint32_t)n)
+            band_from_sorted(got);
+        else
+            radix_over_memory();
     } else {
         // Large K: a 4096-bin histogram of the keys' top 12 bits (in cids) finds the bin of the
         // n-th largest score. Every score the band keeps (>= kth - 2.2 e - 1e-6 (1 + |kth|)) is
@@ -307,7 +469,6 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
         // LDS: two passes over the K scores. A collection that does not fit falls back to the
         // 8-bit radix over memory and a band pass.
         static_assert(MAX_RESCORE == 4096, "the histogram lives in cids");
-        auto key = [&](int i) { return f2key(approx(i) + 0.0f); };
         for (int i = t; i < 4096; i += 256) cids[i] = 0;
         __syncthreads();
         for (int i = t; i < a.k; i += 256) atomicAdd(&cids[key(i) >> 20], 1u);
@@ -332,10 +493,8 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
                 }
             }
         }
-        if (t == 0) ncand = 0;
         __syncthreads();
         const uint32_t bin = sh[0], upto = sh[1];
-        const float e = a.err_coef * a.qnorm[qi];
         const float lo = key2f(bin << 20), hi = key2f((bin << 20) | 0xFFFFFu);
         const float lim = lo - 2.2f * e - 1e-6f * (1.0f + fmaxf(fabsf(lo), fabsf(hi)));
         for (int i = t; i < a.k; i += 256) {
@@ -348,27 +507,10 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
         __syncthreads();
         const uint32_t got = ncand;
         __syncthreads();
-        if (got <= (uint32_t)MAX_RESCORE && got >= upto) {
-            kth = block_kth_largest(sorted, (int)got, n, hist, sh).x;
-            const float kv = key2f(kth);
-            thr = kv - 2.2f * e - 1e-6f * (1.0f + fabsf(kv));
-            if (t == 0) ncand = 0;
-            __syncthreads();
-            for (uint32_t j = t; j < got; j += 256)
-                if (key2f(sorted[j].x) >= thr) cids[atomicAdd(&ncand, 1u)] = ~sorted[j].y;
-        } else {
-            kth = block_kth_u32(a.k, n, key, hist, sh);
-            const float kv = key2f(kth);
-            thr = kv - 2.2f * e - 1e-6f * (1.0f + fabsf(kv));
-            if (t == 0) ncand = 0;
-            __syncthreads();
-            for (int i = t; i < a.k; i += 256) {
-                if (approx(i) >= thr) {
-                    uint32_t j = atomicAdd(&ncand, 1u);
-                    if (j < MAX_RESCORE) cids[j] = (uint32_t)i;
-                }
-            }
-        }
+        if (got <= (uint32_t)MAX_RESCORE && got >= upto)
+            band_from_sorted(got);
+        else
+            radix_over_memory();
     }
     __syncthreads();
     const int nc = (int)ncand;
@@ -464,12 +606,24 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
 
 void launch_route_select(const RouteArgs& a, size_t nq, hipStream_t st) {
     if (nq == 0) return;
-    if (a.k <= 256 * 16)
-        hipLaunchKernelGGL(k_route_select<16>, dim3((unsigned)nq), dim3(256), 0, st, a);
-    else if (a.k <= 256 * 64)
-        hipLaunchKernelGGL(k_route_select<64>, dim3((unsigned)nq), dim3(256), 0, st, a);
-    else
-        hipLaunchKernelGGL(k_route_select<0>, dim3((unsigned)nq), dim3(256), 0, st, a);
+    const int nt = (a.k + 15) / 16;
+    const dim3 g((unsigned)nq), b(256);
+    if (a.tmax && nt >= std::min(a.nprobe, a.k) && nt <= 256 * 64) {
+        if (nt <= 256 * 4)
+            hipLaunchKernelGGL((k_route_select<ROUTE_TILES, 4>), g, b, 0, st, a);
+        else if (nt <= 256 * 16)
+            hipLaunchKernelGGL((k_route_select<ROUTE_TILES, 16>), g, b, 0, st, a);
+        else if (nt <= 256 * 32)
+            hipLaunchKernelGGL((k_route_select<ROUTE_TILES, 32>), g, b, 0, st, a);
+        else
+            hipLaunchKernelGGL((k_route_select<ROUTE_TILES, 64>), g, b, 0, st, a);
+    } else if (a.k <= 256 * 16) {
+        hipLaunchKernelGGL((k_route_select<ROUTE_REGS, 16>), g, b, 0, st, a);
+    } else if (a.k <= 256 * 64) {
+        hipLaunchKernelGGL((k_route_select<ROUTE_REGS, 64>), g, b, 0, st, a);
+    } else {
+        hipLaunchKernelGGL((k_route_select<ROUTE_MEM, 0>), g, b, 0, st, a);
+    }
 }
 
 }  // namespace sieve

@@ -361,7 +361,9 @@ int sieve_server_create(const sieve_index* ix, const sieve_params* p, const siev
             grow<float2>(s->b[B_INV], mb);
             grow<uint16_t>(s->b[B_QBF], mb * PDIM);
             grow<float>(s->b[B_QNORM], mb);
-            grow<float>(s->b[B_APPROX], std::min<size_t>(mb, (size_t(1) << 30) / (4 * (size_t)ix->k)) * ix->k);
+            const size_t route_rows = std::min<size_t>(mb, (size_t(1) << 30) / (4 * (size_t)ix->k));
+            grow<float>(s->b[B_APPROX], route_rows * ix->k);
+            grow<float>(s->b[B_TMAX], route_rows * ((ix->k + 15) / 16));
             grow<uint32_t>(s->b[B_OVF], 1);
             sv->slots.push_back(std::move(sl));
         }
