@@ -295,6 +295,18 @@ int sieve_stream_create(sieve_stream** out) {
 
 void sieve_stream_destroy(sieve_stream* s) { delete s; }
 
+int sieve_stream_set_graphs(sieve_stream* s, int enable) {
+    return guarded([&] {
+        SIEVE_REQUIRE(s, SIEVE_EINVAL, "stream is null");
+        s->graphs = enable != 0;
+        if (!s->graphs) {
+            for (auto& g : s->gcache)
+                if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            s->gcache.clear();
+        }
+    });
+}
+
 int sieve_stream_sync(sieve_stream* s) {
     return guarded([&] {
         SIEVE_REQUIRE(s, SIEVE_EINVAL, "stream is null");
@@ -564,6 +576,98 @@ int sieve_rerank(const sieve_index* ix, sieve_stream* s, const float* q768, cons
     });
 }
 
+}  // extern "C"
+
+namespace sieve {
+
+// enqueue_search as a HIP graph. The first call with a given key (index, its scan settings,
+// batch size, parameters, view and device addresses) runs eagerly and sizes the stream's
+// scratch; the next one captures the launches and every later one replays them, as long as no
+// scratch was reallocated since (the graph holds scratch addresses). Returns 0 (eager),
+// 1 (captured and launched) or 2 (replayed).
+static uint64_t search_graphed(const sieve_index* ix, sieve_stream* s, const float* q2, const float* q7, size_t nq,
+                               const sieve_params* p, const ViewArgs& view, float* os, uint32_t* orr, uint32_t* on) {
+    std::string key;
+    auto put = [&](const auto& v) { key.append(reinterpret_cast<const char*>(&v), sizeof v); };
+    put(ix->uid);
+    put(ix->scan_variant);
+    put(ix->scan_chunk_blocks);
+    put(ix->cm_min_queries);
+    put(ix->cm_seed_slots);
+    put(ix->cm_ext_cap);
+    put(ix->cm_chunk_blocks);
+    put(ix->cm_phases);
+    put(nq);
+    put(*p);
+    put(view.deleted);
+    put(view.deleted_words);
+    put(view.allow);
+    put(view.allow_words);
+    put(view.allow_stride);
+    put(view.allow_flags);
+    put(q2);
+    put(q7);
+    put(os);
+    put(orr);
+    put(on);
+    auto it = std::find_if(s->gcache.begin(), s->gcache.end(), [&](const GraphEntry& g) { return g.key == key; });
+    auto eager = [&](GraphEntry* g) {
+        enqueue_search(ix, s, q2, q7, nq, p, view, os, orr, on);
+        if (g) g->gen = g_scratch_gen.load();
+        return (uint64_t)0;
+    };
+    if (it == s->gcache.end()) {
+        if (s->gcache.size() >= 32) {
+            if (s->gcache.front().exec) (void)hipGraphExecDestroy(s->gcache.front().exec);
+            s->gcache.erase(s->gcache.begin());
+        }
+        s->gcache.push_back(GraphEntry{key});
+        return eager(&s->gcache.back());
+    }
+    GraphEntry& g = *it;
+    if (g.failed) return eager(nullptr);
+    if (g.gen != g_scratch_gen.load()) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+        return eager(&g);
+    }
+    if (g.exec) {
+        SIEVE_HIP(hipGraphLaunch(g.exec, s->st));
+        return 2;
+    }
+    // Capture. Relaxed mode: the stage functions may query function attributes; nothing in
+    // them synchronises or allocates at this size (checked by the scratch generation).
+    SIEVE_HIP(hipStreamBeginCapture(s->st, hipStreamCaptureModeRelaxed));
+    hipGraph_t graph = nullptr;
+    try {
+        enqueue_search(ix, s, q2, q7, nq, p, view, os, orr, on);
+    } catch (...) {
+        (void)hipStreamEndCapture(s->st, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        g.failed = true;
+        return eager(nullptr);
+    }
+    hipError_t e = hipStreamEndCapture(s->st, &graph);
+    hipGraphExec_t exec = nullptr;
+    if (e == hipSuccess && graph && g.gen == g_scratch_gen.load())
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess || !exec) {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        (void)hipGetLastError();
+        g.failed = true;
+        return eager(nullptr);
+    }
+    g.exec = exec;
+    SIEVE_HIP(hipGraphLaunch(g.exec, s->st));
+    return 1;
+}
+
+}  // namespace sieve
+
+extern "C" {
+
 int sieve_search_batch(const sieve_index* ix, sieve_stream* s, const float* q768, const float* q256, size_t nq,
                        const sieve_params* p, const sieve_view* v, float* scores, uint32_t* rows, uint32_t* counts,
                        sieve_timing* t) {
@@ -577,11 +681,16 @@ int sieve_search_batch(const sieve_index* ix, sieve_stream* s, const float* q768
         DevOut<float> os = dev_out(s, B_OS, scores, nq * p->k);
         DevOut<uint32_t> orr = dev_out(s, B_OR, rows, nq * p->k);
         DevOut<uint32_t> on = dev_out(s, B_ON, counts, nq);
-        enqueue_search(ix, s, q2, q7, nq, p, view, os.dev, orr.dev, on.dev);
+        uint64_t graph = 0;
+        if (s->graphs)
+            graph = search_graphed(ix, s, q2, q7, nq, p, view, os.dev, orr.dev, on.dev);
+        else
+            enqueue_search(ix, s, q2, q7, nq, p, view, os.dev, orr.dev, on.dev);
         out_done(s, os);
         out_done(s, orr);
         out_done(s, on);
         finish_search(s, nq, t);
+        s->last.graph = graph;
     });
 }
 

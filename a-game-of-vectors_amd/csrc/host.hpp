@@ -3,6 +3,7 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -36,10 +37,19 @@ struct DevBuf {
     }
 };
 
+// Reallocations of stream scratch so far: a captured graph holds scratch addresses, so one
+// captured before the latest reallocation is stale.
+inline std::atomic<uint64_t> g_scratch_gen{0};
+inline uint64_t next_index_uid() {
+    static std::atomic<uint64_t> c{0};
+    return ++c;
+}
+
 template <class T>
 T* grow(DevBuf& b, size_t n) {
     size_t bytes = std::max<size_t>(n * sizeof(T), 16);
     if (bytes > b.cap) {
+        g_scratch_gen.fetch_add(1);
         b.release();
         hipError_t e = hipMalloc(&b.p, bytes);
         if (e != hipSuccess) {
@@ -84,6 +94,7 @@ using sieve::DevBuf;
 
 
 struct sieve_index {
+    uint64_t uid = sieve::next_index_uid();  // never reused: keys of captured graphs
     int device = 0;
     uint32_t k = 0;
     uint64_t n = 0, nblocks = 0, nslots = 0;
@@ -160,13 +171,28 @@ enum Slot {
     B_NSLOTS
 };
 
+namespace sieve {
+// A search captured as a HIP graph: the call's arguments (key), the executable graph and the
+// scratch generation its addresses belong to.
+struct GraphEntry {
+    std::string key;
+    hipGraphExec_t exec = nullptr;
+    uint64_t gen = 0;
+    bool failed = false;  // capture failed: this call runs eagerly
+};
+}  // namespace sieve
+
 struct sieve_stream {
     hipStream_t st = nullptr;
     int device = 0;
     DevBuf b[B_NSLOTS];
     hipEvent_t ev[8] = {};
     sieve_stats last{};
+    bool graphs = false;               // sieve_stream_set_graphs
+    std::vector<sieve::GraphEntry> gcache;  // most recent last
     ~sieve_stream() {
+        for (auto& g : gcache)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
         for (auto& x : b) x.release();
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -261,7 +287,10 @@ inline ViewArgs make_view(sieve_stream* s, const sieve_view* v, size_t nq) {
 
 inline float elapsed_ns(hipEvent_t a, hipEvent_t b) {
     float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, a, b);
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();  // leave no error behind for the next launch check
+        return 0.0f;
+    }
     return ms * 1e6f;
 }
 
@@ -291,7 +320,7 @@ inline void prep_queries(const sieve_index* ix, sieve_stream* s, const float* q2
 inline void route_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, size_t nq, const sieve_params* p, float* ps,
                  uint32_t* pc, uint32_t* pn, int stride) {
     uint32_t* ovf = grow<uint32_t>(s->b[B_OVF], 1);
-    SIEVE_HIP(hipMemsetAsync(ovf, 0, 4, s->st));
+    launch_zero_words(ovf, 1, s->st);
     const size_t per = std::max<size_t>(1, std::min<size_t>(nq, (size_t(1) << 30) / (4 * (size_t)ix->k)));
     float* approx = grow<float>(s->b[B_APPROX], per * ix->k);
     float* tmax = grow<float>(s->b[B_TMAX], per * ((ix->k + 15) / 16));
@@ -506,7 +535,7 @@ inline void enqueue_search(const sieve_index* ix, sieve_stream* s, const float* 
     uint32_t* cr = grow<uint32_t>(s->b[B_CR], nq * p->cand);
     float* cx = grow<float>(s->b[B_CX], nq * p->cand);
     uint32_t* cn = grow<uint32_t>(s->b[B_CN], nq);
-    SIEVE_HIP(hipMemsetAsync(cnt, 0, 32, s->st));
+    launch_zero_words(reinterpret_cast<uint32_t*>(cnt), 8, s->st);  // a kernel: graph-capturable
     SIEVE_HIP(hipEventRecord(s->ev[0], s->st));
     Batch bt;
     prep_queries(ix, s, q2, q7, nq, bt);

@@ -88,6 +88,8 @@ struct RouteArgs {
     float err_coef;            // |approx - exact| <= err_coef * qnorm
 };
 void launch_route_select(const RouteArgs& a, size_t nq, hipStream_t st);
+// p[0, n) = 0 by a kernel (a memset node is not replayed reliably in a captured graph).
+void launch_zero_words(uint32_t* p, uint32_t n, hipStream_t st);
 
 struct ViewArgs {
     const uint64_t* deleted;

@@ -638,6 +638,11 @@ __global__ void k_cm_zero(uint32_t* p, uint32_t n) {
     if (i < n) p[i] = 0;
 }
 
+void launch_zero_words(uint32_t* p, uint32_t n, hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_cm_zero, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+}
+
 // One wave per query: the seed pass's probe count (whole probes, in probe order, until
 // seed_slots stored slots) and the per-segment counts of the later probes.
 __global__ __launch_bounds__(256) void k_cm_prep(ScanArgs a, CmScratch cm) {

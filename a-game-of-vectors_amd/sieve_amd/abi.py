@@ -57,7 +57,8 @@ class View(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("queries", ctypes.c_uint64), ("probed", ctypes.c_uint64), ("scanned", ctypes.c_uint64),
                 ("id_reads", ctypes.c_uint64), ("reranked", ctypes.c_uint64), ("route_ns", ctypes.c_double),
-                ("scan_ns", ctypes.c_double), ("rerank_ns", ctypes.c_double), ("rescanned", ctypes.c_uint64)]
+                ("scan_ns", ctypes.c_double), ("rerank_ns", ctypes.c_double), ("rescanned", ctypes.c_uint64),
+                ("graph", ctypes.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -144,6 +145,7 @@ def lib():
         "sieve_stream_create": (i32, [ctypes.POINTER(vp)]),
         "sieve_stream_destroy": (None, [vp]),
         "sieve_stream_sync": (i32, [vp]),
+        "sieve_stream_set_graphs": (i32, [vp, ctypes.c_int]),
         "sieve_stream_handle": (vp, [vp]),
         "sieve_stream_last_stats": (i32, [vp, ctypes.POINTER(Stats)]),
         "sieve_gemm_abt": (i32, [vp, vp, sz, sz, sz, vp]),

@@ -31,6 +31,10 @@ class Stream:
     def sync(self):
         abi.check(abi.lib().sieve_stream_sync(self._h))
 
+    def set_graphs(self, enable=True):
+        """Run batched searches on this stream as captured HIP graphs (sieve_stream_set_graphs)."""
+        abi.check(abi.lib().sieve_stream_set_graphs(self._h, int(bool(enable))))
+
     def last_stats(self):
         """Counters of the last call on this stream (sieve_stats) as a dict."""
         st = abi.Stats()
@@ -195,13 +199,18 @@ class Index:
         abi.check(abi.lib().sieve_index_set_cluster_scan(self._h, min_queries, seed_slots, ext_cap, chunk_blocks,
                                                          phases))
 
-    def search(self, q768, q256, p: Params, view=None, timing=None, stream=None):
-        """Index::search / search_view for a batch: (scores nq x k, rows, counts)."""
+    def search(self, q768, q256, p: Params, view=None, timing=None, stream=None, out=None):
+        """Index::search / search_view for a batch: (scores nq x k, rows, counts). `out`: the
+        (scores, rows, counts) arrays to write, else new ones."""
         q768, q256 = self._in(q768, np.float32), self._in(q256, np.float32)
         nq = q256.shape[0]
-        s = self._out(q256, (nq, p.k), np.float32)
-        r = self._out(q256, (nq, p.k), np.uint32)
-        n = self._out(q256, (nq,), np.uint32)
+        if out is not None:
+            s, r, n = out
+            assert _numel(s) == nq * p.k and _numel(r) == nq * p.k and _numel(n) == nq
+        else:
+            s = self._out(q256, (nq, p.k), np.float32)
+            r = self._out(q256, (nq, p.k), np.uint32)
+            n = self._out(q256, (nq,), np.uint32)
         t = timing if timing is not None else abi.Timing()
         st = stream or self.stream()
         v = view.c if view is not None else None

@@ -149,6 +149,7 @@ typedef struct sieve_stats {
     double scan_ns;
     double rerank_ns;
     uint64_t rescanned;  /* cluster-major scan: queries whose appended list overflowed       */
+    uint64_t graph;      /* sieve_search_batch on a stream with graphs: 0 eager, 1 captured, 2 replayed */
 } sieve_stats;
 
 typedef struct sieve_index sieve_index;
@@ -173,6 +174,12 @@ const char* sieve_last_error(void);
 int sieve_stream_create(sieve_stream** out);
 void sieve_stream_destroy(sieve_stream* s);
 int sieve_stream_sync(sieve_stream* s);
+/* Runs sieve_search_batch on this stream as HIP graphs (enable != 0): a call repeated with the
+ * same index settings, batch size, parameters, view and device addresses replays the captured
+ * launches (the first such call runs eagerly, the second captures). Results are those of the
+ * eager path. Meant for serving loops with fixed staging buffers; no counterpart in the
+ * reference (a launch-overhead knob of this library). 0 disables and frees the graphs. */
+int sieve_stream_set_graphs(sieve_stream* s, int enable);
 /* The underlying hipStream_t, for callers that order their own work with it. */
 void* sieve_stream_handle(sieve_stream* s);
 /* Counters of the last route / candidates / rerank / search call on this stream. */
