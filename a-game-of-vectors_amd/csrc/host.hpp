@@ -125,6 +125,9 @@ struct sieve_index {
     // entries per query (0: 24 and 64 * cand); cm_chunk_blocks blocks per work item; the work
     // list in cm_phases launches.
     uint32_t cm_min_queries = 64, cm_seed_slots = 0, cm_ext_cap = 0, cm_chunk_blocks = 64, cm_phases = 2;
+    // Below cm_min_queries, unfiltered batches of at most small_max_queries queries use the
+    // small-batch scan (per-chunk bounds, then appended lists); larger ones the per-query scan.
+    uint32_t small_max_queries = 16;
 
     ~sieve_index() {
         void* ps[] = {cents, cents_bf, block_off, cl_slots, codes, mul, err, ids, rotation, quant, norm256, t2_sigma,
@@ -461,6 +464,28 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
             a.ext = grow<uint2>(s->b[B_CM_EXT], nq * (size_t)a.ext_cap);
             a.ext_n = grow_zeroed<uint32_t>(s->b[B_CM_EXTN], nq, s->st);  // the finish pass leaves it zero
             launch_scan_cm(a, cm, nq, variant, s->st);
+            SIEVE_HIP(hipGetLastError());
+            return;
+        }
+        if (nq <= ix->small_max_queries && !view.allow) {
+            // Per-chunk bounds, a threshold per query, appended lists, the finish merge.
+            const uint32_t cand = (uint32_t)a.cand;
+            const double worst = (double)pstride * std::max<uint32_t>(ix->max_cluster_blocks, 1) +
+                                 (lv ? (double)lv->dblocks : 0.0);
+            const uint32_t bound_chunks = (uint32_t)std::max(1.0, std::ceil(worst / 32.0));
+            const uint32_t append_blocks = 64;
+            const uint32_t append_chunks = (uint32_t)std::max(1.0, std::ceil(worst / append_blocks));
+            SIEVE_REQUIRE(nq * (double)bound_chunks < 2.0e9 && nq * (double)append_chunks < 2.0e9, SIEVE_ELIMIT,
+                          "scan grid too large");
+            a.part = grow<uint2>(s->b[B_PART], nq * (size_t)bound_chunks * 16);
+            a.part_n = grow<uint32_t>(s->b[B_PARTN], nq * (size_t)bound_chunks);
+            a.ext_cap = ix->cm_ext_cap ? ix->cm_ext_cap : 64 * cand;
+            a.seed = grow<uint2>(s->b[B_CM_SEED], nq * (size_t)cand);
+            a.seed_n = grow<uint32_t>(s->b[B_CM_SEEDN], nq);
+            a.seed_thr = grow<unsigned long long>(s->b[B_CM_THR], nq);
+            a.ext = grow<uint2>(s->b[B_CM_EXT], nq * (size_t)a.ext_cap);
+            a.ext_n = grow_zeroed<uint32_t>(s->b[B_CM_EXTN], nq, s->st);  // the finish pass leaves it zero
+            launch_scan_small(a, nq, bound_chunks, append_blocks, append_chunks, s->st);
             SIEVE_HIP(hipGetLastError());
             return;
         }

@@ -123,7 +123,12 @@ __device__ void scan_compact(uint2* buf, uint2* tmp, int n, int k, uint32_t* his
 // kept list with the entries appended since and keeps the best cand, raising the threshold;
 // SCAN_FINISH merges them one last time (or rescans the query when an appended list
 // overflowed) and writes the output.
-constexpr int SCAN_FULL = 0, SCAN_SEED = 1, SCAN_FINISH = 2, SCAN_TIGHTEN = 3;
+// SCAN_BOUND (small batches): one round per chunk; writes the chunk's best min(16, cand)
+// admitted entries to part (16 per chunk), a subset of the query's admissible entries.
+// SCAN_APPEND (small batches): appends every slot above the query's threshold (seed_thr) to its
+// appended list unchecked, for the finish pass.
+constexpr int SCAN_FULL = 0, SCAN_SEED = 1, SCAN_FINISH = 2, SCAN_TIGHTEN = 3, SCAN_BOUND = 4, SCAN_APPEND = 5;
+constexpr int BOUND_PER_CHUNK = 16;
 // seed_n flag: an appended list overflowed, so the query is rescanned by the finish pass.
 constexpr uint32_t SEED_LOST = 0x80000000u;
 
@@ -189,13 +194,18 @@ void k_scan_t(ScanArgs a) {
     __syncthreads();
     const uint32_t total = pref[np];
     const uint32_t nch = min(a.chunk_max, max(1u, (uint32_t)((total + (uint64_t)a.chunk_blocks - 1) / a.chunk_blocks)));
-    if (ci >= nch) return;  // the whole workgroup: this query has fewer chunks
+    if (ci >= nch) {  // the whole workgroup: this query has fewer chunks
+        if (MODE == SCAN_BOUND && t == 0) a.part_n[qi * a.chunk_max + ci] = 0;
+        return;
+    }
     const uint32_t vb_lo = (uint32_t)((uint64_t)total * ci / nch);
     const uint32_t vb_hi = (uint32_t)((uint64_t)total * (ci + 1) / nch);
     if (t < G * 4) reinterpret_cast<uint32_t*>(tab)[t] = reinterpret_cast<const uint32_t*>(a.tables + qi * G * 16)[t];
     const float4 qab = a.qab[qi];
-    if (t == 0 && a.scanned) atomicAdd(a.scanned, (unsigned long long)(vb_hi - vb_lo) * BLOCK);
-    unsigned long long* gthr = nch > 1 ? a.gthr + qi : nullptr;
+    if (MODE != SCAN_BOUND && t == 0 && a.scanned) atomicAdd(a.scanned, (unsigned long long)(vb_hi - vb_lo) * BLOCK);
+    unsigned long long* gthr = MODE == SCAN_FULL && nch > 1 ? a.gthr + qi : nullptr;
+    // SCAN_APPEND: the query's threshold (0: none)
+    const unsigned long long athr = MODE == SCAN_APPEND ? a.seed_thr[qi] : 0ull;
 
     const uint64_t* allow = nullptr;
     if (a.view.allow && (a.view.allow_stride == 0 || !a.view.allow_flags || a.view.allow_flags[qi]))
@@ -303,8 +313,12 @@ void k_scan_t(ScanArgs a) {
                 if constexpr (L2) est = 2.0f * est - a.err[R.gs[u]];
                 est = est + 0.0f;  // -0.0 -> +0.0 so keys order like the floats compare
                 key = f2key(est);
-                pass = !has_thr || key > thr || (key == thr && ~R.gs[u] > thr_lo);
-                if (pass) {
+                if constexpr (MODE == SCAN_APPEND) {
+                    pass = ((((unsigned long long)key) << 32) | (unsigned long long)(~R.gs[u])) > athr;
+                } else {
+                    pass = !has_thr || key > thr || (key == thr && ~R.gs[u] > thr_lo);
+                }
+                if (MODE != SCAN_APPEND && pass) {
                     id_reads++;
                     const uint32_t id = a.ids[R.gs[u]];
                     pass = id != PAD;
@@ -312,7 +326,20 @@ void k_scan_t(ScanArgs a) {
                     if (pass && allow && !bit_set(allow, a.view.allow_words, id)) pass = false;
                 }
             }
-            admit(pass, key, ~R.gs[u], round);
+            if constexpr (MODE == SCAN_APPEND) {
+                const uint64_t bal = __ballot(pass);
+                if (bal) {
+                    const uint32_t nw = (uint32_t)__popcll(bal);
+                    const uint32_t before = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(a.ext_n + qi, nw);
+                    base = __shfl(base, 0, 64);
+                    if (pass && base + before < a.ext_cap)
+                        a.ext[qi * (size_t)a.ext_cap + base + before] = make_uint2(key, ~R.gs[u]);
+                }
+            } else {
+                admit(pass, key, ~R.gs[u], round);
+            }
         }
     };
     // One round: prefetch the next round's codes, score this one, then a single barrier. The
@@ -375,6 +402,17 @@ void k_scan_t(ScanArgs a) {
         if (vb0 >= vb_hi) break;
         step(rb, ra, vb0, round++);
         vb0 += ROUND_BLOCKS;
+    }
+    if constexpr (MODE == SCAN_APPEND) return;
+    if constexpr (MODE == SCAN_BOUND) {
+        // The chunk's best entries: every one is an admissible slot (id read, view checked).
+        const int b = min(BOUND_PER_CHUNK, cand);
+        if ((int)misc[M_COUNT] > b) scan_compact(buf, tmp, (int)misc[M_COUNT], b, hist, misc, nullptr);
+        const int n = min((int)misc[M_COUNT], b);
+        const size_t at = qi * a.chunk_max + ci;
+        if (t < n) a.part[at * BOUND_PER_CHUNK + t] = buf[t];
+        if (t == 0) a.part_n[at] = (uint32_t)n;
+        return;
     }
     id_reads = wave_sum(id_reads);
     if (lane == 0 && a.scanned) atomicAdd(a.scanned + 2, (unsigned long long)id_reads);
@@ -1071,6 +1109,85 @@ void launch_scan_cm(const ScanArgs& a_in, const CmScratch& cm_in, size_t nq, int
     }
     hipLaunchKernelGGL((k_scan_t<64, false, 2, false, 0, SCAN_FINISH>), dim3((unsigned)nq), dim3(SCAN_THREADS), lds,
                        st, a);
+}
+
+// ------------------------------------------------------------------ small batches ------
+// A few queries alone cannot fill the GPU through per-query workgroups that keep their own
+// candidate lists: a chunk's workgroup admits and cuts thousands of entries under a low
+// threshold, and the last chunk merges every chunk's list. For small batches the scan runs
+// in four steps instead: (1) SCAN_BOUND: every 32-block chunk of a query's probes keeps its
+// best 16 admissible entries; (2) k_scan_bound_select: the query's threshold is the
+// (cand + 1)-th best of their union, a subset of the admissible entries, so it lies strictly
+// below the cand-th best of all of them; (3) SCAN_APPEND: every slot above the threshold is
+// appended to the query's list, which therefore holds the top cand; (4) SCAN_FINISH merges the
+// list (reading ids and the view there, as after the cluster pass) or, when the list
+// overflowed, rescans the query.
+constexpr int BOUND_MAX = 8192;  // union entries one query's select reads (the first chunks')
+
+__global__ __launch_bounds__(SEL_THREADS) void k_scan_bound_select(ScanArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    uint2* ent = reinterpret_cast<uint2*>(lds);
+    __shared__ uint32_t hist[256], sh[4], cnt;
+    const size_t qi = blockIdx.x;
+    const int t = threadIdx.x;
+    if (t == 0) cnt = 0;
+    __syncthreads();
+    const uint32_t chunks = min(a.chunk_max, (uint32_t)(BOUND_MAX / BOUND_PER_CHUNK));
+    for (uint32_t c = t; c < chunks; c += SEL_THREADS) {
+        const size_t at = qi * a.chunk_max + c;
+        const uint32_t m = a.part_n[at];
+        if (m) {
+            const uint32_t o = atomicAdd(&cnt, m);
+            for (uint32_t i = 0; i < m; i++) ent[o + i] = a.part[at * BOUND_PER_CHUNK + i];
+        }
+    }
+    __syncthreads();
+    const int n = (int)cnt;
+    const int k = a.cand + 1;
+    unsigned long long thr = 0;
+    if (n >= k) thr = pack_pair(block_kth_largest(ent, n, k, hist, sh));
+    if (t == 0) {
+        a.seed_thr[qi] = thr;
+        a.seed_n[qi] = 0;
+    }
+}
+
+void launch_scan_small(const ScanArgs& a_in, size_t nq, uint32_t bound_chunks, uint32_t append_chunk_blocks,
+                       uint32_t append_chunks, hipStream_t st) {
+    if (nq == 0) return;
+    ScanArgs a = a_in;
+    a.seg_stride = scan_segments(a);
+    a.trigger = std::max(a.cand + 1, a.cand * 6 / 4);
+    const size_t lds4 = scan_layout(a.cand, a.seg_stride, 4, a.trigger).total;
+    const size_t lds2 = scan_layout(a.cand, a.seg_stride, 2, a.trigger).total;
+    if (lds4 > 64 * 1024) {
+        (void)hipFuncSetAttribute((const void*)k_scan_t<64, false, 4, false, 0, SCAN_BOUND>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds4);
+        (void)hipFuncSetAttribute((const void*)k_scan_t<64, false, 4, false, 0, SCAN_APPEND>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds4);
+    }
+    if (lds2 > 64 * 1024)
+        (void)hipFuncSetAttribute((const void*)k_scan_t<64, false, 2, false, 0, SCAN_FINISH>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2);
+    ScanArgs b = a;  // one round of 4 slots per thread = 32 blocks per chunk
+    b.chunk_blocks = 32;
+    b.chunk_max = bound_chunks;
+    hipLaunchKernelGGL((k_scan_t<64, false, 4, false, 0, SCAN_BOUND>), dim3((unsigned)(nq * b.chunk_max)),
+                       dim3(SCAN_THREADS), lds4, st, b);
+    const size_t sel_lds = (size_t)std::min<uint32_t>(bound_chunks * BOUND_PER_CHUNK, BOUND_MAX) * sizeof(uint2);
+    (void)hipFuncSetAttribute((const void*)k_scan_bound_select, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)sel_lds);
+    hipLaunchKernelGGL(k_scan_bound_select, dim3((unsigned)nq), dim3(SEL_THREADS), sel_lds, st, b);
+    ScanArgs p = a;
+    p.chunk_blocks = append_chunk_blocks;
+    p.chunk_max = append_chunks;
+    hipLaunchKernelGGL((k_scan_t<64, false, 4, false, 0, SCAN_APPEND>), dim3((unsigned)(nq * p.chunk_max)),
+                       dim3(SCAN_THREADS), lds4, st, p);
+    ScanArgs f = a;
+    f.chunk_max = 1;
+    f.chunk_blocks = 0xFFFFFFFFu;
+    hipLaunchKernelGGL((k_scan_t<64, false, 2, false, 0, SCAN_FINISH>), dim3((unsigned)nq), dim3(SCAN_THREADS), lds2,
+                       st, f);
 }
 
 // ------------------------------------------------------------------ lut::scan seam -----

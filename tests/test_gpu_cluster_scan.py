@@ -154,8 +154,8 @@ def test_cluster_scan_over_live_delta_lists(small):
     live.close()
 
 
-def test_cluster_scan_small_batches_use_the_per_query_scan(small, cm_index):
-    """Below min_queries the variant falls back to the per-query scan (same results)."""
+def test_cluster_scan_small_batches_equal_the_per_query_scan(small, cm_index):
+    """Below min_queries a batch takes the small-batch scan (same results)."""
     p = Params(nprobe=12, cand=300, k=10, rerank=Rerank.FULL)
     probes = cm_index.route(small.q256[:5], p)
     want = _cands(cm_index, small.q256[:5], probes, p, 0)
