@@ -932,6 +932,13 @@ int sieve_index_set_cluster_scan(sieve_index* ix, uint32_t min_queries, uint32_t
     });
 }
 
+int sieve_index_set_small_batch(sieve_index* ix, uint32_t max_queries) {
+    return guarded([&] {
+        SIEVE_REQUIRE(ix, SIEVE_EINVAL, "null index");
+        ix->small_max_queries = max_queries;
+    });
+}
+
 int sieve_l2_index_set_scan_variant(sieve_l2_index* ix, uint32_t variant) {
     return guarded([&] {
         SIEVE_REQUIRE(ix, SIEVE_EINVAL, "null index");

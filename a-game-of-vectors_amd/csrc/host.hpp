@@ -126,8 +126,9 @@ struct sieve_index {
     // list in cm_phases launches.
     uint32_t cm_min_queries = 64, cm_seed_slots = 0, cm_ext_cap = 0, cm_chunk_blocks = 64, cm_phases = 2;
     // Below cm_min_queries, unfiltered batches of at most small_max_queries queries use the
-    // small-batch scan (per-chunk bounds, then appended lists); larger ones the per-query scan.
-    uint32_t small_max_queries = 16;
+    // small-batch scan (per-chunk bounds, then appended lists); larger ones the per-query scan
+    // (on the 113.5M index the small-batch scan is faster up to about 8 queries).
+    uint32_t small_max_queries = 8;
 
     ~sieve_index() {
         void* ps[] = {cents, cents_bf, block_off, cl_slots, codes, mul, err, ids, rotation, quant, norm256, t2_sigma,
@@ -170,7 +171,7 @@ enum Slot {
     B_Q256, B_Q768, B_TABLES, B_QAB, B_WQ768, B_WQ256, B_INV, B_QBF, B_QNORM, B_APPROX, B_PS, B_PC, B_PN, B_CE, B_CR,
     B_CX, B_CN, B_OS, B_OR, B_ON, B_DEL, B_ALLOW, B_AFLAGS, B_CNT, B_OVF, B_TMP1, B_TMP2, B_TMP3, B_QQ, B_QF,
     B_PART, B_PARTN, B_GTHR, B_QDONE, B_CM_LIM, B_CM_SEGC, B_CM_QOFF, B_CM_IOFF, B_CM_Q, B_CM_CENT, B_CM_SEED,
-    B_CM_SEEDN, B_CM_THR, B_CM_EXT, B_CM_EXTN, B_CM_WORK, B_CM_ITEMS, B_TMAX,
+    B_CM_SEEDN, B_CM_THR, B_CM_EXT, B_CM_EXTN, B_CM_WORK, B_CM_ITEMS, B_TMAX, B_STATS,
     B_NSLOTS
 };
 
@@ -191,12 +192,14 @@ struct sieve_stream {
     DevBuf b[B_NSLOTS];
     hipEvent_t ev[8] = {};
     sieve_stats last{};
+    unsigned long long* hstats = nullptr;  // pinned: the packed counters of the last search
     bool graphs = false;               // sieve_stream_set_graphs
     std::vector<sieve::GraphEntry> gcache;  // most recent last
     ~sieve_stream() {
         for (auto& g : gcache)
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
         for (auto& x : b) x.release();
+        if (hstats) (void)hipHostFree(hstats);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
@@ -553,6 +556,7 @@ inline void enqueue_search(const sieve_index* ix, sieve_stream* s, const float* 
     }
     const int pstride = (int)std::min<uint32_t>(p->nprobe, ix->k);
     unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 4);
+    grow<unsigned long long>(s->b[B_STATS], 6);  // finish_search's packed counters, sized with the rest
     float* ps = grow<float>(s->b[B_PS], nq * pstride);
     uint32_t* pc = grow<uint32_t>(s->b[B_PC], nq * pstride);
     uint32_t* pn = grow<uint32_t>(s->b[B_PN], nq);
@@ -575,20 +579,25 @@ inline void enqueue_search(const sieve_index* ix, sieve_stream* s, const float* 
 // Waits for an enqueued search, fails if routing overflowed, and records its counters and
 // stage times in s->last (and adds them to *t).
 inline void finish_search(sieve_stream* s, size_t nq, sieve_timing* t) {
+    // One small copy into pinned memory: the counters, the probe total and the overflow flag.
     const unsigned long long* cnt = static_cast<const unsigned long long*>(s->b[B_CNT].p);
     const uint32_t* pn = static_cast<const uint32_t*>(s->b[B_PN].p);
-    unsigned long long hcnt[4] = {0, 0, 0, 0};
-    SIEVE_HIP(hipMemcpyAsync(hcnt, cnt, 32, hipMemcpyDeviceToHost, s->st));
-    std::vector<uint32_t> hpn(nq);
-    SIEVE_HIP(hipMemcpyAsync(hpn.data(), pn, nq * 4, hipMemcpyDeviceToHost, s->st));
-    check_overflow(s);
+    unsigned long long* packed = grow<unsigned long long>(s->b[B_STATS], 6);
+    launch_pack_stats(cnt, pn, nq, static_cast<const uint32_t*>(s->b[B_OVF].p), packed, s->st);
+    if (!s->hstats) SIEVE_HIP(hipHostMalloc((void**)&s->hstats, 6 * sizeof(unsigned long long), hipHostMallocDefault));
+    SIEVE_HIP(hipMemcpyAsync(s->hstats, packed, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
+    SIEVE_HIP(hipStreamSynchronize(s->st));
+    const unsigned long long* hcnt = s->hstats;
+    SIEVE_REQUIRE(hcnt[5] == 0, SIEVE_ELIMIT,
+                  "routing: more than " + std::to_string(MAX_RESCORE) +
+                      " centroids within the BF16 error band of the nprobe-th score");
     s->last = sieve_stats{};
     s->last.queries = nq;
     s->last.scanned = hcnt[0];
     s->last.id_reads = hcnt[2];
     s->last.reranked = hcnt[1];
     s->last.rescanned = hcnt[3];
-    for (uint32_t x : hpn) s->last.probed += x;
+    s->last.probed = hcnt[4];
     s->last.route_ns = elapsed_ns(s->ev[0], s->ev[1]);
     s->last.scan_ns = elapsed_ns(s->ev[1], s->ev[2]);
     s->last.rerank_ns = elapsed_ns(s->ev[2], s->ev[3]);

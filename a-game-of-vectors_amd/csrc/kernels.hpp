@@ -88,6 +88,9 @@ struct RouteArgs {
     float err_coef;            // |approx - exact| <= err_coef * qnorm
 };
 void launch_route_select(const RouteArgs& a, size_t nq, hipStream_t st);
+// out[0..3] = cnt[0..3], out[4] = sum of pn[0, nq), out[5] = *ovf (one workgroup).
+void launch_pack_stats(const unsigned long long* cnt, const uint32_t* pn, size_t nq, const uint32_t* ovf,
+                       unsigned long long* out, hipStream_t st);
 // p[0, n) = 0 by a kernel (a memset node is not replayed reliably in a captured graph).
 void launch_zero_words(uint32_t* p, uint32_t n, hipStream_t st);
 

@@ -79,23 +79,38 @@ __device__ void int8_scores(const RerankArgs& a, const int8_t* __restrict__ wq, 
         wv[j][2] = (int)x.z;
         wv[j][3] = (int)x.w;
     }
-    for (int base = wave * 4; base < n; base += RR_THREADS / 16) {
-        const int i = base + (lane >> 4);
-        int acc = 0;
-        if (i < n) {
-            const int8_t* p = row_ptr(a, rows[i]);
+    // 16 lanes per row, four rows per lane group at a time (their loads in flight together):
+    // rows 64 it + 16 r + 4 wave + lane / 16, r < 4. The i32 sums are exact in any order.
+    constexpr int R = 4;
+    for (int b0 = wave * 4; b0 < n; b0 += R * (RR_THREADS / 16)) {  // wave-uniform
+        const int base = b0 + (lane >> 4);
+        uint4 x[R][CH];
 #pragma unroll
-            for (int j = 0; j < CH; j++) {
-                uint4 x = *reinterpret_cast<const uint4*>(p + sub * 16 + 256 * j);
-                acc = __builtin_amdgcn_sdot4((int)x.x, wv[j][0], acc, false);
-                acc = __builtin_amdgcn_sdot4((int)x.y, wv[j][1], acc, false);
-                acc = __builtin_amdgcn_sdot4((int)x.z, wv[j][2], acc, false);
-                acc = __builtin_amdgcn_sdot4((int)x.w, wv[j][3], acc, false);
+        for (int r = 0; r < R; r++) {
+            const int i = base + r * (RR_THREADS / 16);
+            if (i < n) {
+                const int8_t* p = row_ptr(a, rows[i]);
+#pragma unroll
+                for (int j = 0; j < CH; j++) x[r][j] = *reinterpret_cast<const uint4*>(p + sub * 16 + 256 * j);
             }
         }
 #pragma unroll
-        for (int d = 8; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
-        if (sub == 0 && i < n) score[i] = (float)acc * inv;
+        for (int r = 0; r < R; r++) {
+            const int i = base + r * (RR_THREADS / 16);
+            int acc = 0;
+            if (i < n) {
+#pragma unroll
+                for (int j = 0; j < CH; j++) {
+                    acc = __builtin_amdgcn_sdot4((int)x[r][j].x, wv[j][0], acc, false);
+                    acc = __builtin_amdgcn_sdot4((int)x[r][j].y, wv[j][1], acc, false);
+                    acc = __builtin_amdgcn_sdot4((int)x[r][j].z, wv[j][2], acc, false);
+                    acc = __builtin_amdgcn_sdot4((int)x[r][j].w, wv[j][3], acc, false);
+                }
+            }
+#pragma unroll
+            for (int d = 8; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
+            if (sub == 0 && i < n) score[i] = (float)acc * inv;
+        }
     }
 }
 

@@ -372,23 +372,29 @@ void k_scan_t(ScanArgs a) {
         for (int i0 = 0; i0 < m; i0 += SCAN_THREADS * SLOTS_PER_THREAD) {
             const bool has_thr = misc[M_HAS] != 0;
             const uint32_t thr = misc[M_THR], thr_lo = misc[M_THR_LO];
+            // The round's entries, then all their id reads at once, then the admissions.
+            uint2 e[SLOTS_PER_THREAD];
+            bool pass[SLOTS_PER_THREAD];
 #pragma unroll
             for (int u = 0; u < SLOTS_PER_THREAD; u++) {
                 const int i = i0 + u * SCAN_THREADS + t;
-                uint2 e = make_uint2(0u, 0u);
-                bool pass = false;
-                if (i < m) {
-                    e = lst[i];
-                    pass = !has_thr || e.x > thr || (e.x == thr && e.y > thr_lo);
-                    if (pass && check_ids) {
-                        id_reads++;
-                        const uint32_t id = a.ids[~e.y];
-                        pass = id != PAD;
-                        if (pass && a.view.deleted && bit_set(a.view.deleted, a.view.deleted_words, id)) pass = false;
-                    }
-                }
-                admit(pass, e.x, e.y, round);
+                e[u] = i < m ? lst[i] : make_uint2(0u, 0u);
+                pass[u] = i < m && (!has_thr || e[u].x > thr || (e[u].x == thr && e[u].y > thr_lo));
             }
+            if (check_ids) {
+                uint32_t id[SLOTS_PER_THREAD];
+#pragma unroll
+                for (int u = 0; u < SLOTS_PER_THREAD; u++) id[u] = pass[u] ? a.ids[~e[u].y] : PAD;
+#pragma unroll
+                for (int u = 0; u < SLOTS_PER_THREAD; u++) {
+                    if (!pass[u]) continue;
+                    id_reads++;
+                    pass[u] = id[u] != PAD &&
+                              !(a.view.deleted && bit_set(a.view.deleted, a.view.deleted_words, id[u]));
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < SLOTS_PER_THREAD; u++) admit(pass[u], e[u].x, e[u].y, round);
             __syncthreads();
             if (misc[M_STAMP + (round & 1)] == round)
                 scan_compact(buf, tmp, (int)misc[M_COUNT], cand, hist, misc, nullptr);
@@ -674,6 +680,26 @@ __device__ __forceinline__ void cm_segment(const uint32_t* block_off, const uint
 __global__ void k_cm_zero(uint32_t* p, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_pack_stats(const unsigned long long* cnt, const uint32_t* pn, size_t nq,
+                                                    const uint32_t* ovf, unsigned long long* out) {
+    __shared__ unsigned long long part[4];
+    unsigned long long sum = 0;
+    for (size_t i = threadIdx.x; i < nq; i += 256) sum += pn[i];
+    for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x < 4) out[threadIdx.x] = cnt[threadIdx.x];
+    if (threadIdx.x == 0) {
+        out[4] = part[0] + part[1] + part[2] + part[3];
+        out[5] = *ovf;
+    }
+}
+
+void launch_pack_stats(const unsigned long long* cnt, const uint32_t* pn, size_t nq, const uint32_t* ovf,
+                       unsigned long long* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_pack_stats, dim3(1), dim3(256), 0, st, cnt, pn, nq, ovf, out);
 }
 
 void launch_zero_words(uint32_t* p, uint32_t n, hipStream_t st) {
@@ -1159,16 +1185,15 @@ void launch_scan_small(const ScanArgs& a_in, size_t nq, uint32_t bound_chunks, u
     a.seg_stride = scan_segments(a);
     a.trigger = std::max(a.cand + 1, a.cand * 6 / 4);
     const size_t lds4 = scan_layout(a.cand, a.seg_stride, 4, a.trigger).total;
-    const size_t lds2 = scan_layout(a.cand, a.seg_stride, 2, a.trigger).total;
     if (lds4 > 64 * 1024) {
         (void)hipFuncSetAttribute((const void*)k_scan_t<64, false, 4, false, 0, SCAN_BOUND>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds4);
         (void)hipFuncSetAttribute((const void*)k_scan_t<64, false, 4, false, 0, SCAN_APPEND>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds4);
     }
-    if (lds2 > 64 * 1024)
-        (void)hipFuncSetAttribute((const void*)k_scan_t<64, false, 2, false, 0, SCAN_FINISH>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2);
+    if (lds4 > 64 * 1024)
+        (void)hipFuncSetAttribute((const void*)k_scan_t<64, false, 4, false, 0, SCAN_FINISH>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds4);
     ScanArgs b = a;  // one round of 4 slots per thread = 32 blocks per chunk
     b.chunk_blocks = 32;
     b.chunk_max = bound_chunks;
@@ -1186,7 +1211,7 @@ void launch_scan_small(const ScanArgs& a_in, size_t nq, uint32_t bound_chunks, u
     ScanArgs f = a;
     f.chunk_max = 1;
     f.chunk_blocks = 0xFFFFFFFFu;
-    hipLaunchKernelGGL((k_scan_t<64, false, 2, false, 0, SCAN_FINISH>), dim3((unsigned)nq), dim3(SCAN_THREADS), lds2,
+    hipLaunchKernelGGL((k_scan_t<64, false, 4, false, 0, SCAN_FINISH>), dim3((unsigned)nq), dim3(SCAN_THREADS), lds4,
                        st, f);
 }
 

@@ -162,6 +162,7 @@ def lib():
         "sieve_index_set_scan_variant": (i32, [vp, u32]),
         "sieve_index_set_scan_chunk": (i32, [vp, u32]),
         "sieve_index_set_cluster_scan": (i32, [vp, u32, u32, u32, u32, u32]),
+        "sieve_index_set_small_batch": (i32, [vp, u32]),
         "sieve_l2_index_set_scan_chunk": (i32, [vp, u32]),
         "sieve_l2_index_set_scan_variant": (i32, [vp, u32]),
         "sieve_server_create": (i32, [vp, P, ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)]),

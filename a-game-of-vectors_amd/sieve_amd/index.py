@@ -199,6 +199,11 @@ class Index:
         abi.check(abi.lib().sieve_index_set_cluster_scan(self._h, min_queries, seed_slots, ext_cap, chunk_blocks,
                                                          phases))
 
+    def set_small_batch(self, max_queries=8):
+        """Largest unfiltered batch for the small-batch scan of the cluster-major variants
+        (sieve_index_set_small_batch); 0 disables it."""
+        abi.check(abi.lib().sieve_index_set_small_batch(self._h, max_queries))
+
     def search(self, q768, q256, p: Params, view=None, timing=None, stream=None, out=None):
         """Index::search / search_view for a batch: (scores nq x k, rows, counts). `out`: the
         (scores, rows, counts) arrays to write, else new ones."""

@@ -23,7 +23,9 @@ CM = abi.SCAN_VARIANTS_CM
 @pytest.fixture(scope="module")
 def sb_index(small):
     print(f"[gpu] small-batch index k={small.k} n={small.n}", flush=True)
-    return Index.create(small.host)
+    ix = Index.create(small.host)
+    ix.set_small_batch(16)
+    return ix
 
 
 @pytest.mark.parametrize("nq", [1, 2, 7, 16])
@@ -83,6 +85,7 @@ def test_small_batch_search_matches_oracle(small, oracle_index, sb_index):
 
 def test_small_batches_over_live_delta_lists(small):
     ix = Index.create(small.host)
+    ix.set_small_batch(16)
     live = Live(ix, max_rows=1000, soar_lambda=1.0)
     new, _ = normalize_rows(small.mix.rows(small.n, 600))
     live.insert((80_000_000 + np.arange(600)).astype(np.uint32), new.numpy().astype(np.float32))
