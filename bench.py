@@ -60,14 +60,13 @@ WORKLOADS = {
 # 1-bit estimate 70-78%), so that workload adds 131072 topics (spread 0.08) inside the
 # components, rows at spread 0.05 around them: R@10 0.92 at 64 probes, 1000 candidates.
 SPREAD = 0.1
-# HBM bytes per scan launch measured with rocprofv3 --pmc for the profiled (workload, search
-# configuration): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB x 1024, the factors read off a 1 GiB
-# elementwise copy (bench.py --copy-probe). Every scan variant reads the same bytes (four
-# variants profiled within 0.1%). Other configurations report null.
-# HBM bytes per scan (rocprofv3 FETCH_SIZE and WRITE_SIZE over the scan's kernels, scaled as
-# measured on the copy probe), keyed by workload, configuration and scan path.
+# HBM bytes per scan stage measured with rocprofv3 --pmc for the profiled (workload, search
+# configuration, scan path): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB x 1024 summed over the
+# stage's kernels, the factors read off a 1 GiB elementwise copy (bench.py --copy-probe). Every
+# per-query scan variant reads the same bytes (four variants profiled within 0.1%). Other
+# configurations report null.
 PMC_TRAFFIC = {("c3", "64:1000:full:b10x0.8:p0.1", "per-query"): 306.22e9,
-               ("c3", "64:1000:full:b10x0.8:p0.1", "cluster-major"): 36.12e9}
+               ("c3", "64:1000:full:b10x0.8:p0.1", "cluster-major"): 33.66e9}
 
 
 def log(msg):
