@@ -164,7 +164,16 @@ __global__ __launch_bounds__(RR_THREADS) void k_rerank(RerankArgs a) {
             keys[i] = make_uint2(f2key(lo + 0.0f), (uint32_t)i);
         }
         __syncthreads();
-        uint2 kth = block_kth_largest(keys, n, a.prune_m, hist, misc + 8);
+        uint2 kth;
+        if (a.prune_m <= 32) {  // the m-th largest by m rounds of a block maximum
+            // hist (1 KiB): 8 u64 of maxima, then up to 32 entries
+            uint2* best = reinterpret_cast<uint2*>(hist) + 8;
+            block_topk_small(keys, n, a.prune_m, best, reinterpret_cast<unsigned long long*>(hist));
+            kth = best[a.prune_m - 1];
+            __syncthreads();
+        } else {
+            kth = block_kth_largest(keys, n, a.prune_m, hist, misc + 8);
+        }
         const float tau = key2f(kth.x);
         // keep est + w*err >= tau, in order: each thread a contiguous run
         const int per = (n + RR_THREADS - 1) / RR_THREADS;
@@ -269,12 +278,21 @@ __global__ __launch_bounds__(RR_THREADS) void k_rerank(RerankArgs a) {
         }
     }
     // Final top k by (score desc, row asc).
-    const int n2 = pow2_at_least(max(m, 1));
-    for (int i = t; i < n2; i += RR_THREADS)
-        keys[i] = i < m ? make_uint2(f2key(score[i] + 0.0f), ~fin_rows[i]) : make_uint2(0u, 0u);
-    __syncthreads();
-    block_sort_pairs(keys, nullptr, n2, false);
     const int kk = min(a.k, m);
+    if (a.k <= 32) {
+        for (int i = t; i < m; i += RR_THREADS) keys[i] = make_uint2(f2key(score[i] + 0.0f), ~fin_rows[i]);
+        __syncthreads();
+        uint2* best = reinterpret_cast<uint2*>(hist) + 8;  // after the 8 u64 of maxima
+        block_topk_small(keys, m, a.k, best, reinterpret_cast<unsigned long long*>(hist));
+        for (int i = t; i < kk; i += RR_THREADS) keys[i] = best[i];
+        __syncthreads();
+    } else {
+        const int n2 = pow2_at_least(max(m, 1));
+        for (int i = t; i < n2; i += RR_THREADS)
+            keys[i] = i < m ? make_uint2(f2key(score[i] + 0.0f), ~fin_rows[i]) : make_uint2(0u, 0u);
+        __syncthreads();
+        block_sort_pairs(keys, nullptr, n2, false);
+    }
     for (int i = t; i < kk; i += RR_THREADS) {
         a.out_s[qi * a.k + i] = key2f(keys[i].x);
         const uint32_t r = ~keys[i].y;

@@ -124,6 +124,44 @@ __device__ __forceinline__ void block_sort_pairs(uint2* key, uint32_t* pay, int 
     }
 }
 
+// The best min(k, n) of n unique entries buf[0, n) (n <= 2048) under the order (hi, lo), best
+// first, into out: k rounds of a block-wide maximum over the entries the threads hold in
+// registers (i = t + 256 j), one barrier per round. For small k it replaces a radix select or
+// a full sort. `red` holds 8 u64 (two rounds' per-wave maxima). An entry (0, 0) cannot occur
+// (a key of 0 is the bit pattern of a NaN).
+__device__ __forceinline__ int block_topk_small(const uint2* buf, int n, int k, uint2* out, unsigned long long* red) {
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    unsigned long long v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int i = t + SEL_THREADS * j;
+        v[j] = i < n ? (((unsigned long long)buf[i].x << 32) | buf[i].y) : 0ull;
+    }
+    const int m = min(k, n);
+    for (int r = 0; r < m; r++) {
+        unsigned long long b = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) b = v[j] > b ? v[j] : b;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            const unsigned long long o = __shfl_xor(b, d, 64);
+            b = o > b ? o : b;
+        }
+        unsigned long long* slot = red + 4 * (r & 1);
+        if (lane == 0) slot[wave] = b;
+        __syncthreads();  // the other parity's slots were read before the previous barrier
+        b = slot[0];
+#pragma unroll
+        for (int w = 1; w < SEL_THREADS / 64; w++) b = slot[w] > b ? slot[w] : b;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (v[j] == b) v[j] = 0;
+        if (t == 0) out[r] = make_uint2((uint32_t)(b >> 32), (uint32_t)b);
+    }
+    __syncthreads();
+    return m;
+}
+
 __device__ __forceinline__ int pow2_at_least(int n) {
     int p = 1;
     while (p < n) p <<= 1;
