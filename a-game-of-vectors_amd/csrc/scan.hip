@@ -287,10 +287,14 @@ void k_scan_t(ScanArgs a) {
     auto score = [&](const Regs& R, uint32_t round) {
         const bool has_thr = misc[M_HAS] != 0;
         const uint32_t thr = misc[M_THR], thr_lo = misc[M_THR_LO];
+        // Every slot's estimate and threshold test, then the id reads of all passing slots at
+        // once (and the view words they need), then the admissions.
+        uint32_t key[SLOTS_PER_THREAD];
+        bool pass[SLOTS_PER_THREAD];
 #pragma unroll
         for (int u = 0; u < SLOTS_PER_THREAD; u++) {
-            bool pass = false;
-            uint32_t key = 0;
+            pass[u] = false;
+            key[u] = 0;
             if (R.ok[u]) {
                 uint32_t w[4 * CW];
 #pragma unroll
@@ -312,33 +316,54 @@ void k_scan_t(ScanArgs a) {
                 float est = R.cent[u] + R.mul[u] * (qab.x * (float)sc + qab.y);
                 if constexpr (L2) est = 2.0f * est - a.err[R.gs[u]];
                 est = est + 0.0f;  // -0.0 -> +0.0 so keys order like the floats compare
-                key = f2key(est);
+                key[u] = f2key(est);
                 if constexpr (MODE == SCAN_APPEND) {
-                    pass = ((((unsigned long long)key) << 32) | (unsigned long long)(~R.gs[u])) > athr;
+                    pass[u] = ((((unsigned long long)key[u]) << 32) | (unsigned long long)(~R.gs[u])) > athr;
                 } else {
-                    pass = !has_thr || key > thr || (key == thr && ~R.gs[u] > thr_lo);
-                }
-                if (MODE != SCAN_APPEND && pass) {
-                    id_reads++;
-                    const uint32_t id = a.ids[R.gs[u]];
-                    pass = id != PAD;
-                    if (pass && a.view.deleted && bit_set(a.view.deleted, a.view.deleted_words, id)) pass = false;
-                    if (pass && allow && !bit_set(allow, a.view.allow_words, id)) pass = false;
+                    pass[u] = !has_thr || key[u] > thr || (key[u] == thr && ~R.gs[u] > thr_lo);
                 }
             }
+        }
+        if constexpr (MODE != SCAN_APPEND) {
+            uint32_t id[SLOTS_PER_THREAD];
+#pragma unroll
+            for (int u = 0; u < SLOTS_PER_THREAD; u++) id[u] = pass[u] ? a.ids[R.gs[u]] : PAD;
+#pragma unroll
+            for (int u = 0; u < SLOTS_PER_THREAD; u++) {
+                if (pass[u]) id_reads++;
+                pass[u] = pass[u] && id[u] != PAD;
+            }
+            if (a.view.deleted || allow) {
+                uint64_t dw[SLOTS_PER_THREAD], aw[SLOTS_PER_THREAD];
+#pragma unroll
+                for (int u = 0; u < SLOTS_PER_THREAD; u++) {
+                    const uint64_t wi = id[u] >> 6;
+                    dw[u] = pass[u] && a.view.deleted && wi < a.view.deleted_words ? a.view.deleted[wi] : 0ull;
+                    aw[u] = pass[u] && allow && wi < a.view.allow_words ? allow[wi] : 0ull;
+                }
+#pragma unroll
+                for (int u = 0; u < SLOTS_PER_THREAD; u++) {
+                    const uint64_t bit = 1ull << (id[u] & 63);
+                    if (dw[u] & bit) pass[u] = false;
+                    if (allow && !(aw[u] & bit)) pass[u] = false;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SLOTS_PER_THREAD; u++) {
             if constexpr (MODE == SCAN_APPEND) {
-                const uint64_t bal = __ballot(pass);
+                const uint64_t bal = __ballot(pass[u]);
                 if (bal) {
                     const uint32_t nw = (uint32_t)__popcll(bal);
                     const uint32_t before = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
                     uint32_t base = 0;
                     if (lane == 0) base = atomicAdd(a.ext_n + qi, nw);
                     base = __shfl(base, 0, 64);
-                    if (pass && base + before < a.ext_cap)
-                        a.ext[qi * (size_t)a.ext_cap + base + before] = make_uint2(key, ~R.gs[u]);
+                    if (pass[u] && base + before < a.ext_cap)
+                        a.ext[qi * (size_t)a.ext_cap + base + before] = make_uint2(key[u], ~R.gs[u]);
                 }
             } else {
-                admit(pass, key, ~R.gs[u], round);
+                admit(pass[u], key[u], ~R.gs[u], round);
             }
         }
     };
