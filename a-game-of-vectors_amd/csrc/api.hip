@@ -799,11 +799,29 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
         a.out_n = cn;
         a.out_stride = (int)p->cand;
         a.scanned = cnt;
-        set_scan_plan(a, s, nq, (uint32_t)pstride, (double)ix->nblocks / ix->k, ix->max_cluster_blocks,
-                      ix->scan_chunk_blocks);
         const size_t lds = scan_lds_bytes(a.cand, pstride);
         SIEVE_REQUIRE(lds <= 160 * 1024, SIEVE_ELIMIT, "scan workspace exceeds LDS");
-        launch_scan_l2(a, nq, ix->scan_variant, s->st);
+        if (nq <= ix->small_max_queries) {
+            // A few queries: per-chunk bounds, a threshold per query, appended lists, the merge.
+            const uint32_t cand = (uint32_t)a.cand;
+            const double worst = (double)pstride * std::max<uint32_t>(ix->max_cluster_blocks, 1);
+            const uint32_t bound_chunks = (uint32_t)std::max(1.0, std::ceil(worst / 32.0));
+            const uint32_t append_chunks = (uint32_t)std::max(1.0, std::ceil(worst / 64.0));
+            SIEVE_REQUIRE(nq * (double)bound_chunks < 2.0e9, SIEVE_ELIMIT, "scan grid too large");
+            a.part = grow<uint2>(s->b[B_PART], nq * (size_t)bound_chunks * 16);
+            a.part_n = grow<uint32_t>(s->b[B_PARTN], nq * (size_t)bound_chunks);
+            a.ext_cap = 64 * cand;
+            a.seed = grow<uint2>(s->b[B_CM_SEED], nq * (size_t)cand);
+            a.seed_n = grow<uint32_t>(s->b[B_CM_SEEDN], nq);
+            a.seed_thr = grow<unsigned long long>(s->b[B_CM_THR], nq);
+            a.ext = grow<uint2>(s->b[B_CM_EXT], nq * (size_t)a.ext_cap);
+            a.ext_n = grow_zeroed<uint32_t>(s->b[B_CM_EXTN], nq, s->st);  // the finish pass leaves it zero
+            launch_scan_small(a, nq, true, bound_chunks, 64, append_chunks, s->st);
+        } else {
+            set_scan_plan(a, s, nq, (uint32_t)pstride, (double)ix->nblocks / ix->k, ix->max_cluster_blocks,
+                          ix->scan_chunk_blocks);
+            launch_scan_l2(a, nq, ix->scan_variant, s->st);
+        }
         SIEVE_HIP(hipGetLastError());
         SIEVE_HIP(hipEventRecord(s->ev[2], s->st));
         L2RerankArgs r{};
@@ -833,16 +851,23 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
         copy_out(cand_rows, cr, nq * p->cand * 4);
         copy_out(cand_est, ce, nq * p->cand * 4);
         copy_out(cand_counts, cn, nq * 4);
-        unsigned long long hcnt[4] = {0, 0, 0, 0};
-        SIEVE_HIP(hipMemcpyAsync(hcnt, cnt, 32, hipMemcpyDeviceToHost, s->st));
-        std::vector<uint32_t> hpn(nq);
-        SIEVE_HIP(hipMemcpyAsync(hpn.data(), pn, nq * 4, hipMemcpyDeviceToHost, s->st));
-        check_overflow(s);
+        // One small copy into pinned memory: the counters, the probe total, the overflow flag.
+        unsigned long long* packed = grow<unsigned long long>(s->b[B_STATS], 6);
+        launch_pack_stats(cnt, pn, nq, ovf, packed, s->st);
+        if (!s->hstats)
+            SIEVE_HIP(hipHostMalloc((void**)&s->hstats, 6 * sizeof(unsigned long long), hipHostMallocDefault));
+        SIEVE_HIP(hipMemcpyAsync(s->hstats, packed, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
+        SIEVE_HIP(hipStreamSynchronize(s->st));
+        const unsigned long long* hcnt = s->hstats;
+        SIEVE_REQUIRE(hcnt[5] == 0, SIEVE_ELIMIT,
+                      "routing: more than " + std::to_string(MAX_RESCORE) +
+                          " centroids within the BF16 error band of the nprobe-th score");
         s->last = sieve_stats{};
         s->last.queries = nq;
         s->last.scanned = hcnt[0];
         s->last.id_reads = hcnt[2];
-        for (uint32_t x : hpn) s->last.probed += x;
+        s->last.rescanned = hcnt[3];
+        s->last.probed = hcnt[4];
         s->last.route_ns = elapsed_ns(s->ev[0], s->ev[1]);
         s->last.scan_ns = elapsed_ns(s->ev[1], s->ev[2]);
         s->last.rerank_ns = elapsed_ns(s->ev[2], s->ev[3]);
@@ -933,6 +958,13 @@ int sieve_index_set_cluster_scan(sieve_index* ix, uint32_t min_queries, uint32_t
 }
 
 int sieve_index_set_small_batch(sieve_index* ix, uint32_t max_queries) {
+    return guarded([&] {
+        SIEVE_REQUIRE(ix, SIEVE_EINVAL, "null index");
+        ix->small_max_queries = max_queries;
+    });
+}
+
+int sieve_l2_index_set_small_batch(sieve_l2_index* ix, uint32_t max_queries) {
     return guarded([&] {
         SIEVE_REQUIRE(ix, SIEVE_EINVAL, "null index");
         ix->small_max_queries = max_queries;

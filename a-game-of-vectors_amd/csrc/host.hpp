@@ -141,6 +141,7 @@ struct sieve_index {
 
 struct sieve_l2_index {
     int device = 0;
+    uint32_t small_max_queries = 8;  // batches of at most this many queries: the small-batch scan
     uint32_t k = 0;
     uint64_t n = 0, nblocks = 0, nslots = 0;
     uint32_t max_cluster_blocks = 0;
@@ -488,7 +489,7 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
             a.seed_thr = grow<unsigned long long>(s->b[B_CM_THR], nq);
             a.ext = grow<uint2>(s->b[B_CM_EXT], nq * (size_t)a.ext_cap);
             a.ext_n = grow_zeroed<uint32_t>(s->b[B_CM_EXTN], nq, s->st);  // the finish pass leaves it zero
-            launch_scan_small(a, nq, bound_chunks, append_blocks, append_chunks, s->st);
+            launch_scan_small(a, nq, false, bound_chunks, append_blocks, append_chunks, s->st);
             SIEVE_HIP(hipGetLastError());
             return;
         }

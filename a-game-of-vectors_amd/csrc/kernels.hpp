@@ -191,10 +191,11 @@ struct CmScratch {
 };
 // Enqueues the seed pass, the work-list build, the cluster pass and the finish pass. `a` carries
 // the index, batch and output fields and seed / ext buffers; its chunking is ignored.
-// The scan of a small unfiltered batch (64-group codes): per-chunk bounds, a threshold per
-// query, the appended lists above it, the finish merge. part / part_n hold bound_chunks x 16
-// entries per query; seed, seed_n, seed_thr, ext, ext_n, ext_cap as for the cluster pass.
-void launch_scan_small(const ScanArgs& a, size_t nq, uint32_t bound_chunks, uint32_t append_chunk_blocks,
+// The scan of a small unfiltered batch (64-group inner-product codes, or with l2 the 32-group
+// L2 codes): per-chunk bounds, a threshold per query, the appended lists above it, the finish
+// merge. part / part_n hold bound_chunks x 16 entries per query; seed, seed_n, seed_thr, ext,
+// ext_n, ext_cap as for the cluster pass.
+void launch_scan_small(const ScanArgs& a, size_t nq, bool l2, uint32_t bound_chunks, uint32_t append_chunk_blocks,
                        uint32_t append_chunks, hipStream_t st);
 void launch_scan_cm(const ScanArgs& a, const CmScratch& cm, size_t nq, int variant, hipStream_t st);
 

@@ -1203,26 +1203,27 @@ __global__ __launch_bounds__(SEL_THREADS) void k_scan_bound_select(ScanArgs a) {
     }
 }
 
-void launch_scan_small(const ScanArgs& a_in, size_t nq, uint32_t bound_chunks, uint32_t append_chunk_blocks,
-                       uint32_t append_chunks, hipStream_t st) {
+template <int G, bool L2>
+static void launch_small_t(const ScanArgs& a_in, size_t nq, uint32_t bound_chunks, uint32_t append_chunk_blocks,
+                           uint32_t append_chunks, hipStream_t st) {
     if (nq == 0) return;
     ScanArgs a = a_in;
     a.seg_stride = scan_segments(a);
     a.trigger = std::max(a.cand + 1, a.cand * 6 / 4);
     const size_t lds4 = scan_layout(a.cand, a.seg_stride, 4, a.trigger).total;
     if (lds4 > 64 * 1024) {
-        (void)hipFuncSetAttribute((const void*)k_scan_t<64, false, 4, false, 0, SCAN_BOUND>,
+        (void)hipFuncSetAttribute((const void*)k_scan_t<G, L2, 4, false, 0, SCAN_BOUND>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds4);
-        (void)hipFuncSetAttribute((const void*)k_scan_t<64, false, 4, false, 0, SCAN_APPEND>,
+        (void)hipFuncSetAttribute((const void*)k_scan_t<G, L2, 4, false, 0, SCAN_APPEND>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds4);
     }
     if (lds4 > 64 * 1024)
-        (void)hipFuncSetAttribute((const void*)k_scan_t<64, false, 4, false, 0, SCAN_FINISH>,
+        (void)hipFuncSetAttribute((const void*)k_scan_t<G, L2, 4, false, 0, SCAN_FINISH>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds4);
     ScanArgs b = a;  // one round of 4 slots per thread = 32 blocks per chunk
     b.chunk_blocks = 32;
     b.chunk_max = bound_chunks;
-    hipLaunchKernelGGL((k_scan_t<64, false, 4, false, 0, SCAN_BOUND>), dim3((unsigned)(nq * b.chunk_max)),
+    hipLaunchKernelGGL((k_scan_t<G, L2, 4, false, 0, SCAN_BOUND>), dim3((unsigned)(nq * b.chunk_max)),
                        dim3(SCAN_THREADS), lds4, st, b);
     const size_t sel_lds = (size_t)std::min<uint32_t>(bound_chunks * BOUND_PER_CHUNK, BOUND_MAX) * sizeof(uint2);
     (void)hipFuncSetAttribute((const void*)k_scan_bound_select, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1231,13 +1232,21 @@ void launch_scan_small(const ScanArgs& a_in, size_t nq, uint32_t bound_chunks, u
     ScanArgs p = a;
     p.chunk_blocks = append_chunk_blocks;
     p.chunk_max = append_chunks;
-    hipLaunchKernelGGL((k_scan_t<64, false, 4, false, 0, SCAN_APPEND>), dim3((unsigned)(nq * p.chunk_max)),
+    hipLaunchKernelGGL((k_scan_t<G, L2, 4, false, 0, SCAN_APPEND>), dim3((unsigned)(nq * p.chunk_max)),
                        dim3(SCAN_THREADS), lds4, st, p);
     ScanArgs f = a;
     f.chunk_max = 1;
     f.chunk_blocks = 0xFFFFFFFFu;
-    hipLaunchKernelGGL((k_scan_t<64, false, 4, false, 0, SCAN_FINISH>), dim3((unsigned)nq), dim3(SCAN_THREADS), lds4,
+    hipLaunchKernelGGL((k_scan_t<G, L2, 4, false, 0, SCAN_FINISH>), dim3((unsigned)nq), dim3(SCAN_THREADS), lds4,
                        st, f);
+}
+
+void launch_scan_small(const ScanArgs& a, size_t nq, bool l2, uint32_t bound_chunks, uint32_t append_chunk_blocks,
+                       uint32_t append_chunks, hipStream_t st) {
+    if (l2)
+        launch_small_t<32, true>(a, nq, bound_chunks, append_chunk_blocks, append_chunks, st);
+    else
+        launch_small_t<64, false>(a, nq, bound_chunks, append_chunk_blocks, append_chunks, st);
 }
 
 // ------------------------------------------------------------------ lut::scan seam -----

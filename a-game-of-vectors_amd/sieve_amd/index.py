@@ -386,6 +386,10 @@ class L2Index:
         """One of the per-query scan variants (0 .. 9)."""
         abi.check(abi.lib().sieve_l2_index_set_scan_variant(self._h, v))
 
+    def set_small_batch(self, max_queries=8):
+        """Largest batch for the small-batch scan (sieve_l2_index_set_small_batch); 0 disables it."""
+        abi.check(abi.lib().sieve_l2_index_set_small_batch(self._h, max_queries))
+
     def search(self, q, nprobe, cand, k, margin=0.0, timing=None, stream=None, details=False):
         """bigann::Index::search for a batch of int8 queries (nq x 100): (scores, rows, counts)
         and, with details, (probes, probe_counts, cand_rows, cand_est, cand_counts)."""

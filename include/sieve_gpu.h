@@ -388,6 +388,10 @@ int sieve_l2_index_set_scan_chunk(sieve_l2_index* ix, uint32_t chunk_blocks);
 /* As sieve_index_set_scan_variant, for the L2 index: one of the per-query scan variants
  * (0 .. 9; the cluster-major scan serves the inner-product index only). */
 int sieve_l2_index_set_scan_variant(sieve_l2_index* ix, uint32_t variant);
+/* As sieve_index_set_small_batch, for the L2 index (no counterpart in the reference): batches
+ * of at most max_queries queries run the small-batch scan, larger ones the selected per-query
+ * variant. Results equal the per-query scan's. 0 disables it (default 8). */
+int sieve_l2_index_set_small_batch(sieve_l2_index* ix, uint32_t max_queries);
 
 /* bigann::Index::search (src/bigann.rs:652-657) for nq queries (nq x 100 i8): the top k
  * (score = |q|^2 - |q - x|^2 - |q|^2 = 2 <q, x> - |x|^2, row) best first. Optional outputs

@@ -132,3 +132,23 @@ def test_device_buffers_replay(small, gix):
         modes.append(s.last_stats()["graph"])
         _same(want, got)
     assert modes == [0, 1, 2, 2]
+
+
+def test_small_batch_scan_replays(small, gix):
+    """A batch of a few queries on the cluster-major variant takes the small-batch scan (bound,
+    threshold, append and finish launches): captured and replayed, every result equal."""
+    p = Params(nprobe=16, cand=300, k=10, rerank=Rerank.FULL)
+    gix.set_scan_variant(10)
+    try:
+        for nq in (1, 5):
+            want = gix.search(small.q768[:nq], small.q256[:nq], p)
+            s = _graph_stream()
+            modes = []
+            for _ in range(4):
+                got = gix.search(small.q768[:nq], small.q256[:nq], p, stream=s)
+                modes.append(s.last_stats()["graph"])
+                _same(want, got)
+                torch.cuda.synchronize()
+            assert modes == [0, 1, 2, 2]
+    finally:
+        gix.set_scan_variant(0)

@@ -106,3 +106,29 @@ def test_gpu_l2_scan_variants_match_oracle(l2small, l2oracle, variant):
         wp, wc, we, ws, wr = l2oracle.search(q, 16, 700, 10)
         assert _ties_equal(cr[i, :cn[i]], ce[i, :cn[i]], wc, we)
         assert _ties_equal(r[i, :n[i]], s[i, :n[i]], wr, ws)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nq", [1, 3, 8])
+@pytest.mark.parametrize("nprobe,cand", [(8, 300), (16, 1000), (1, 50), (32, 12)])
+def test_gpu_l2_small_batches_equal_the_per_query_scan(l2small, l2oracle, nq, nprobe, cand):
+    """Batches of at most 8 queries take the small-batch scan (per-chunk bounds, a threshold per
+    query, appended lists, the merge): candidates and results equal the per-query scan's bit for
+    bit, and the oracle's up to ties."""
+    from sieve_amd.index import L2Index
+    h, qs = l2small
+    ix = L2Index(h)
+    got = ix.search(qs[:nq], nprobe, cand, 10, details=True)
+    ix.set_small_batch(0)
+    want = ix.search(qs[:nq], nprobe, cand, 10, details=True)
+    ws_, wr_, wn_, _, _, wcr, wce, wcn = want
+    s, r, n, probes, pn, cr, ce, cn = got
+    assert np.array_equal(wn_, n) and np.array_equal(wcn, cn)
+    for i in range(nq):
+        assert np.array_equal(wcr[i, :cn[i]], cr[i, :cn[i]])
+        assert np.array_equal(wce[i, :cn[i]].view(np.uint32), ce[i, :cn[i]].view(np.uint32))
+        assert np.array_equal(wr_[i, :n[i]], r[i, :n[i]])
+        assert np.array_equal(ws_[i, :n[i]].view(np.uint32), s[i, :n[i]].view(np.uint32))
+        wp, wc, we, ws, wr = l2oracle.search(qs[i], nprobe, cand, 10)
+        assert _ties_equal(cr[i, :cn[i]], ce[i, :cn[i]], wc, we)
+        assert _ties_equal(r[i, :n[i]], s[i, :n[i]], wr, ws)
