@@ -805,8 +805,9 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
             // A few queries: per-chunk bounds, a threshold per query, appended lists, the merge.
             const uint32_t cand = (uint32_t)a.cand;
             const double worst = (double)pstride * std::max<uint32_t>(ix->max_cluster_blocks, 1);
-            const uint32_t bound_chunks = (uint32_t)std::max(1.0, std::ceil(worst / 32.0));
-            const uint32_t append_chunks = (uint32_t)std::max(1.0, std::ceil(worst / 64.0));
+            // At most 512 bound and 256 append chunks per query (larger chunks take more rounds).
+            const uint32_t bound_chunks = (uint32_t)std::min(512.0, std::max(1.0, std::ceil(worst / 32.0)));
+            const uint32_t append_chunks = (uint32_t)std::min(256.0, std::max(1.0, std::ceil(worst / 64.0)));
             SIEVE_REQUIRE(nq * (double)bound_chunks < 2.0e9, SIEVE_ELIMIT, "scan grid too large");
             a.part = grow<uint2>(s->b[B_PART], nq * (size_t)bound_chunks * 16);
             a.part_n = grow<uint32_t>(s->b[B_PARTN], nq * (size_t)bound_chunks);

@@ -476,9 +476,13 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
             const uint32_t cand = (uint32_t)a.cand;
             const double worst = (double)pstride * std::max<uint32_t>(ix->max_cluster_blocks, 1) +
                                  (lv ? (double)lv->dblocks : 0.0);
-            const uint32_t bound_chunks = (uint32_t)std::max(1.0, std::ceil(worst / 32.0));
+            // Grids of at most 512 bound chunks (all the threshold select reads) and 256 append
+            // chunks per query: a larger query's chunks take more rounds, and the workgroups past
+            // a query's own chunk count, which only exit, stay few.
+            const uint32_t bound_chunks = (uint32_t)std::min(512.0, std::max(1.0, std::ceil(worst / 32.0)));
             const uint32_t append_blocks = 64;
-            const uint32_t append_chunks = (uint32_t)std::max(1.0, std::ceil(worst / append_blocks));
+            const uint32_t append_chunks =
+                (uint32_t)std::min(256.0, std::max(1.0, std::ceil(worst / append_blocks)));
             SIEVE_REQUIRE(nq * (double)bound_chunks < 2.0e9 && nq * (double)append_chunks < 2.0e9, SIEVE_ELIMIT,
                           "scan grid too large");
             a.part = grow<uint2>(s->b[B_PART], nq * (size_t)bound_chunks * 16);

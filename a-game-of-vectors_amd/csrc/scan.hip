@@ -123,8 +123,9 @@ __device__ void scan_compact(uint2* buf, uint2* tmp, int n, int k, uint32_t* his
 // kept list with the entries appended since and keeps the best cand, raising the threshold;
 // SCAN_FINISH merges them one last time (or rescans the query when an appended list
 // overflowed) and writes the output.
-// SCAN_BOUND (small batches): one round per chunk; writes the chunk's best min(16, cand)
-// admitted entries to part (16 per chunk), a subset of the query's admissible entries.
+// SCAN_BOUND (small batches): one round per 32-block chunk (more for queries with more chunks
+// than the grid holds); writes the chunk's best min(16, cand) admitted entries to part (16 per
+// chunk), a subset of the query's admissible entries.
 // SCAN_APPEND (small batches): appends every slot above the query's threshold (seed_thr) to its
 // appended list unchecked, for the finish pass.
 constexpr int SCAN_FULL = 0, SCAN_SEED = 1, SCAN_FINISH = 2, SCAN_TIGHTEN = 3, SCAN_BOUND = 4, SCAN_APPEND = 5;
