@@ -352,7 +352,8 @@ int sieve_server_create(const sieve_index* ix, const sieve_params* p, const siev
             sl->dscores = dmalloc<float>(mb * K, nullptr);
             sl->drows = dmalloc<uint32_t>(mb * K, nullptr);
             sl->dcounts = dmalloc<uint32_t>(mb, nullptr);
-            // Pre-grow the stream scratch to a full batch so admission never allocates later.
+            // Pre-grow the query-prep and routing scratch to a full batch; the scan's scratch grows
+            // with the first batches that need it (its size depends on the path a batch takes).
             sieve_stream* s = sl->s.get();
             grow<uint8_t>(s->b[B_TABLES], mb * 1024);
             grow<float4>(s->b[B_QAB], mb);
