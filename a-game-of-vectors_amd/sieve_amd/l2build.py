@@ -36,7 +36,7 @@ class L2Arrays:
 
 
 def int8_mixture(n, components, seed, device="cpu", spread=12.0):
-    """n x 100 int8 rows from a seeded mixture, clipped to [-127, 127]."""
+    """n x 100 int8 rows from a seeded mixture, clipped to [-127, 127] (the 100-d int8 L2 workload)."""
     g = torch.Generator(device=device)
     g.manual_seed(sub_seed(seed, 11))
     means = torch.randn(components, D, generator=g, device=device) * 40.0
@@ -116,7 +116,7 @@ def build_l2(base, k, iters=8, seed=7, chunk=1 << 18):
 
 
 class L2Mixture:
-    """The 100-d int8 mixture of BASELINE config 4 at any size: component means N(0, 40^2) per
+    """The 100-d int8 mixture of the 1B-row L2 workload at any size: component means N(0, 40^2) per
     dimension; each component holds `topics` sub-centres at N(0, spread^2) around its mean, and
     a row is a uniformly drawn topic's centre + N(0, topic_spread^2), rounded and clipped to
     [-127, 127] (topics = 0: rows around the component mean at `spread`). Rows come in 2^22-row
