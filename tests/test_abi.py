@@ -50,3 +50,16 @@ def test_make_params_tags():
     c = abi.make_params(Params(nprobe=8, cand=100, rerank=Rerank.tiered(20), k=10, prune=(10, 0.5)))
     assert (c.rerank, c.rerank_r, c.prune_m, c.prune_w, c.nprobe, c.budget) == (3, 20, 10, 0.5, 8, 0)
     assert math.isnan(c.margin)
+
+
+def test_stats_mirror_matches_header_field_order():
+    """sieve_stats grows by appending fields; the ctypes mirror must list the header's fields in
+    the header's order (every field is 8 bytes)."""
+    import os
+    import re
+    hdr = open(os.path.join(os.path.dirname(__file__), "..", "include", "sieve_gpu.h")).read()
+    body = re.search(r"typedef struct sieve_stats \{(.*?)\} sieve_stats;", hdr, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = re.findall(r"(?:uint64_t|double)\s+(\w+);", body)
+    assert names == [f for f, _ in abi.Stats._fields_]
+    assert ctypes.sizeof(abi.Stats) == 8 * len(names)
