@@ -20,7 +20,7 @@ namespace sieve {
 constexpr int RR_THREADS = 256;
 
 struct RerankLds {
-    size_t off_est, off_row, off_err, off_score, off_keys, off_pay, off_list, off_t2, off_hist, off_misc, total;
+    size_t off_est, off_row, off_err, off_score, off_keys, off_list, off_t2, off_hist, off_misc, total;
 };
 
 __host__ __device__ inline RerankLds rerank_layout(int cs, bool t2) {
@@ -39,8 +39,6 @@ __host__ __device__ inline RerankLds rerank_layout(int cs, bool t2) {
     o = al(o + 4 * (size_t)cs);
     l.off_keys = o;
     o = al(o + 8 * (size_t)p2);
-    l.off_pay = o;
-    o = al(o + 4 * (size_t)p2);
     l.off_list = o;
     o = al(o + 4 * (size_t)cs);
     l.off_t2 = o;
