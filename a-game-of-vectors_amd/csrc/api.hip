@@ -452,7 +452,7 @@ int sieve_route(const sieve_index* ix, sieve_stream* s, const float* q256, size_
         out_done(s, os);
         out_done(s, oc);
         out_done(s, on);
-        check_overflow(s);
+        SIEVE_HIP(hipStreamSynchronize(s->st));
         s->last = sieve_stats{};
         s->last.queries = nq;
         s->last.route_ns = elapsed_ns(s->ev[0], s->ev[1]);
@@ -597,6 +597,7 @@ static uint64_t search_graphed(const sieve_index* ix, sieve_stream* s, const flo
     put(ix->cm_ext_cap);
     put(ix->cm_chunk_blocks);
     put(ix->cm_phases);
+    put(ix->small_max_queries);
     put(nq);
     put(*p);
     put(view.deleted);
@@ -767,7 +768,6 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
             ra.out_c = pc + q0 * pstride;
             ra.out_n = pn + q0;
             ra.out_stride = pstride;
-            ra.overflow = ovf;
             ra.l2 = 1;
             ra.dims = D;
             ra.cstride = D;
@@ -852,7 +852,7 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
         copy_out(cand_rows, cr, nq * p->cand * 4);
         copy_out(cand_est, ce, nq * p->cand * 4);
         copy_out(cand_counts, cn, nq * 4);
-        // One small copy into pinned memory: the counters, the probe total, the overflow flag.
+        // One small copy into pinned memory: the counters and the probe total.
         unsigned long long* packed = grow<unsigned long long>(s->b[B_STATS], 6);
         launch_pack_stats(cnt, pn, nq, ovf, packed, s->st);
         if (!s->hstats)
@@ -860,9 +860,6 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
         SIEVE_HIP(hipMemcpyAsync(s->hstats, packed, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
         SIEVE_HIP(hipStreamSynchronize(s->st));
         const unsigned long long* hcnt = s->hstats;
-        SIEVE_REQUIRE(hcnt[5] == 0, SIEVE_ELIMIT,
-                      "routing: more than " + std::to_string(MAX_RESCORE) +
-                          " centroids within the BF16 error band of the nprobe-th score");
         s->last = sieve_stats{};
         s->last.queries = nq;
         s->last.scanned = hcnt[0];
@@ -902,7 +899,6 @@ int sieve_index_autotune(sieve_index* ix, sieve_stream* s, const float* q768, co
         Batch bt;
         prep_queries(ix, s, q2, q7, nq, bt);
         route_batch(ix, s, bt, nq, p, ps, pc, pn, pstride);
-        check_overflow(s);
         int best = 0;
         float best_ms = 0.0f;
         const int saved = ix->scan_variant;

@@ -301,8 +301,8 @@ struct sieve_filter {
 namespace sieve {
 
 // filter::search for nq queries sharing filter f, enqueued on s with no host synchronisation:
-// outputs os / orr / on (device), counters in B_CNT, the plan's overflow flag (Expand cut
-// beyond MAX_PROBE, or routing) in B_OVF, events ev[0..3] around its stages. With a live state
+// outputs os / orr / on (device), counters in B_CNT, the plan's overflow flag (an Expand cut
+// beyond its probe limit) in B_OVF, events ev[0..3] around its stages. With a live state
 // (lv), Post and Expand read the delta lists and tombstones; Brute, as the reference's, scores
 // the allowed corpus rows (skipping tombstoned ones).
 // Smallest scan chunk for a query filtered to f->nrows of the index's rows: a chunk should hold
@@ -322,7 +322,7 @@ FilterRun enqueue_filter_search(const sieve_index* ix, const sieve_filter* f, ui
     FilterRun fr;
     fr.plan = sieve_plan_resolve(plan, f->nrows, ix->n);
     if (plan == SIEVE_PLAN_AUTO && fr.plan == SIEVE_PLAN_EXPAND) expand_target = EXPAND_TARGET;
-    uint32_t* ovf = grow<uint32_t>(s->b[B_OVF], 1);
+    uint32_t* ovf = sgrow<uint32_t>(s, B_OVF, 1);
     ViewArgs view{};
     view.allow = f->allow;
     view.allow_words = f->words;
@@ -334,25 +334,31 @@ FilterRun enqueue_filter_search(const sieve_index* ix, const sieve_filter* f, ui
         view.deleted = lv->deleted;
         view.deleted_words = lv->deleted_words;
     }
-    SIEVE_HIP(hipMemsetAsync(ovf, 0, 4, s->st));
-    unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 4);
-    SIEVE_HIP(hipMemsetAsync(cnt, 0, 32, s->st));
-    SIEVE_HIP(hipEventRecord(s->ev[0], s->st));
+    unsigned long long* cnt = sgrow<unsigned long long>(s, B_CNT, 4);
+    if (!s->sizing) {
+        SIEVE_HIP(hipMemsetAsync(ovf, 0, 4, s->st));
+        SIEVE_HIP(hipMemsetAsync(cnt, 0, 32, s->st));
+    }
+    record(s, 0);
     Batch bt;
     prep_queries(ix, s, q2, q7, nq, bt);
-    uint32_t* pn = grow<uint32_t>(s->b[B_PN], nq);
+    uint32_t* pn = sgrow<uint32_t>(s, B_PN, nq);
     if (fr.plan == SIEVE_PLAN_BRUTE) {
         // All allowed rows scored for a sub-batch of queries at a time (<= 256 MB of scores).
         const uint64_t R = f->nrows;
         fr.brute_rows = R;
-        SIEVE_HIP(hipMemsetAsync(pn, 0, nq * 4, s->st));
-        SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
-        SIEVE_HIP(hipEventRecord(s->ev[2], s->st));
+        if (!s->sizing) SIEVE_HIP(hipMemsetAsync(pn, 0, nq * 4, s->st));
+        record(s, 1);
+        record(s, 2);
+        if (s->sizing) {
+            if (R) sgrow<float>(s, B_TMP1, std::max<size_t>(1, std::min<size_t>(nq, (size_t(64) << 20) / R)) * R);
+            return fr;
+        }
         if (R == 0) {
             SIEVE_HIP(hipMemsetAsync(on, 0, nq * 4, s->st));
         } else {
             const size_t sub = std::max<size_t>(1, std::min<size_t>(nq, (size_t(64) << 20) / R));
-            float* sc = grow<float>(s->b[B_TMP1], sub * R);
+            float* sc = sgrow<float>(s, B_TMP1, sub * R);
             const size_t lds = (size_t)(2 * p->k + 4 * FT) * 8 + (size_t)p->k * 8 + 256 * 4 + 16 * 4;
             SIEVE_REQUIRE(lds <= 160 * 1024, SIEVE_ELIMIT, "k too large for the brute-force top-k");
             if (lds > 64 * 1024)
@@ -381,27 +387,61 @@ FilterRun enqueue_filter_search(const sieve_index* ix, const sieve_filter* f, ui
     while (maxp > 1 && scan_lds_bytes((int)p->cand, maxp * seg_per_probe) > 160 * 1024) maxp -= 64;
     maxp = std::max(maxp, 1);
     fr.max_probe = (uint32_t)maxp;
-    float* all = grow<float>(s->b[B_APPROX], nq * ix->k);
+    float* all = sgrow<float>(s, B_APPROX, nq * ix->k);
+    float* ps = sgrow<float>(s, B_PS, nq * maxp);
+    uint32_t* pc = sgrow<uint32_t>(s, B_PC, nq * maxp);
+    float* ce = sgrow<float>(s, B_CE, nq * p->cand);
+    uint32_t* cr = sgrow<uint32_t>(s, B_CR, nq * p->cand);
+    float* cx = sgrow<float>(s, B_CX, nq * p->cand);
+    uint32_t* cn = sgrow<uint32_t>(s, B_CN, nq);
+    if (s->sizing) {
+        scan_batch(ix, s, bt, nq, ps, pc, pn, maxp, p, view, ce, cr, cx, cn, (int)p->cand, cnt, lv,
+                   filtered_chunk_blocks(ix, f, p));
+        rerank_batch(ix, s, bt, nq, ce, cr, cx, cn, (int)p->cand, p, os, orr, on, cnt + 1, lv);
+        return fr;
+    }
     hipLaunchKernelGGL(k_exact_route, dim3(grid(ix->k, FT), (unsigned)nq), dim3(FT), 0, s->st, q2, ix->cents,
                        (int)ix->k, all);
-    float* ps = grow<float>(s->b[B_PS], nq * maxp);
-    uint32_t* pc = grow<uint32_t>(s->b[B_PC], nq * maxp);
     const size_t lds = (size_t)EXPAND_MAX_K * 8 + 16 * 8;
     (void)hipFuncSetAttribute((const void*)k_expand_cut, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_expand_cut, dim3((unsigned)nq), dim3(FT), lds, s->st, all, (int)ix->k, f->per_cluster,
                        expand_target, maxp, ps, pc, pn, ovf);
     SIEVE_HIP(hipGetLastError());
     SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
-    float* ce = grow<float>(s->b[B_CE], nq * p->cand);
-    uint32_t* cr = grow<uint32_t>(s->b[B_CR], nq * p->cand);
-    float* cx = grow<float>(s->b[B_CX], nq * p->cand);
-    uint32_t* cn = grow<uint32_t>(s->b[B_CN], nq);
     scan_batch(ix, s, bt, nq, ps, pc, pn, maxp, p, view, ce, cr, cx, cn, (int)p->cand, cnt, lv,
                filtered_chunk_blocks(ix, f, p));
     SIEVE_HIP(hipEventRecord(s->ev[2], s->st));
     rerank_batch(ix, s, bt, nq, ce, cr, cx, cn, (int)p->cand, p, os, orr, on, cnt + 1, lv);
     SIEVE_HIP(hipEventRecord(s->ev[3], s->st));
     return fr;
+}
+
+// The filtered paths a batch of nq queries can take, on a stream in sizing mode (their scratch is
+// recorded, nothing is enqueued): Post, Expand over a filter of every row (its scan has the
+// smallest chunks, so the most partial lists) and Brute at Plan::resolve's largest filter.
+void size_filter_search(const sieve_index* ix, sieve_stream* s, size_t nq, const sieve_params* p,
+                        const LiveArgs* lv) {
+    SIEVE_REQUIRE(s->sizing, SIEVE_EINVAL, "size_filter_search needs a stream in sizing mode");
+    sieve_filter f;
+    f.ix = ix;
+    f.allow = reinterpret_cast<uint64_t*>(uintptr_t(256));  // never dereferenced in sizing mode
+    f.words = (ix->n + 63) / 64;
+    const float* q = reinterpret_cast<const float*>(uintptr_t(256));
+    float* os = reinterpret_cast<float*>(uintptr_t(256));
+    uint32_t* ou = reinterpret_cast<uint32_t*>(uintptr_t(256));
+    const std::pair<uint32_t, uint64_t> runs[] = {{SIEVE_PLAN_POST, ix->n},
+                                                  {SIEVE_PLAN_EXPAND, ix->n},
+                                                  {SIEVE_PLAN_BRUTE, std::min<uint64_t>(ix->n, BRUTE_MAX)}};
+    try {
+        for (const auto& r : runs) {
+            f.nrows = r.second;
+            enqueue_filter_search(ix, &f, r.first, EXPAND_TARGET, s, q, q, nq, p, os, ou, ou, lv);
+        }
+    } catch (...) {
+        f.allow = nullptr;
+        throw;
+    }
+    f.allow = nullptr;
 }
 
 }  // namespace sieve
@@ -490,10 +530,8 @@ int sieve_filter_search_batch(const sieve_index* ix, const sieve_filter* f, uint
         uint32_t h = 0;
         SIEVE_HIP(hipMemcpyAsync(&h, s->b[B_OVF].p, 4, hipMemcpyDeviceToHost, s->st));
         SIEVE_HIP(hipStreamSynchronize(s->st));
-        SIEVE_REQUIRE(h == 0, SIEVE_ELIMIT, fr.plan == SIEVE_PLAN_EXPAND
-                                                ? "expand plan needs more than " + std::to_string(fr.max_probe) +
-                                                      " probed clusters"
-                                                : std::string("routing: too many centroids within the BF16 error band"));
+        SIEVE_REQUIRE(h == 0, SIEVE_ELIMIT,
+                      "expand plan needs more than " + std::to_string(fr.max_probe) + " probed clusters");
         finish_search(s, nq, t);
         if (fr.plan == SIEVE_PLAN_BRUTE) {
             s->last.reranked = fr.brute_rows * nq;  // every allowed row, per query

@@ -191,6 +191,9 @@ struct sieve_stream {
     hipStream_t st = nullptr;
     int device = 0;
     DevBuf b[B_NSLOTS];
+    // Sizing mode (sizing != nullptr, B_NSLOTS entries): the pipeline functions record the bytes
+    // each scratch slot would need and enqueue nothing, so a caller can reserve a worst case.
+    size_t* sizing = nullptr;
     hipEvent_t ev[8] = {};
     sieve_stats last{};
     unsigned long long* hstats = nullptr;  // pinned: the packed counters of the last search
@@ -208,6 +211,26 @@ struct sieve_stream {
 };
 
 namespace sieve {
+
+// Stream scratch slot `slot` holding at least n T's (grow), or in sizing mode the record of that
+// need and a placeholder address that no host code dereferences.
+template <class T>
+T* sgrow(sieve_stream* s, int slot, size_t n) {
+    if (s->sizing) {
+        s->sizing[slot] = std::max(s->sizing[slot], std::max<size_t>(n * sizeof(T), 16));
+        return reinterpret_cast<T*>(uintptr_t(256));
+    }
+    return grow<T>(s->b[slot], n);
+}
+template <class T>
+T* sgrow_zeroed(sieve_stream* s, int slot, size_t n) {
+    if (s->sizing) return sgrow<T>(s, slot, n);
+    return grow_zeroed<T>(s->b[slot], n, s->st);
+}
+// Event records and launches are skipped in sizing mode.
+inline void record(sieve_stream* s, int i) {
+    if (!s->sizing) SIEVE_HIP(hipEventRecord(s->ev[i], s->st));
+}
 
 template <class F>
 int guarded(F&& f) {
@@ -312,25 +335,26 @@ inline void prep_queries(const sieve_index* ix, sieve_stream* s, const float* q2
                   Batch& bt) {
     bt.q256 = q256d;
     bt.q768 = q768d;
-    bt.prep.tables = grow<uint8_t>(s->b[B_TABLES], nq * 1024);
-    bt.prep.qab = grow<float4>(s->b[B_QAB], nq);
-    bt.prep.wq768 = grow<int8_t>(s->b[B_WQ768], nq * DIM);
-    bt.prep.wq256 = grow<int8_t>(s->b[B_WQ256], nq * PDIM);
-    bt.prep.inv = grow<float2>(s->b[B_INV], nq);
-    bt.prep.qbf = grow<uint16_t>(s->b[B_QBF], nq * PDIM);
-    bt.prep.qnorm = grow<float>(s->b[B_QNORM], nq);
+    bt.prep.tables = sgrow<uint8_t>(s, B_TABLES, nq * 1024);
+    bt.prep.qab = sgrow<float4>(s, B_QAB, nq);
+    bt.prep.wq768 = sgrow<int8_t>(s, B_WQ768, nq * DIM);
+    bt.prep.wq256 = sgrow<int8_t>(s, B_WQ256, nq * PDIM);
+    bt.prep.inv = sgrow<float2>(s, B_INV, nq);
+    bt.prep.qbf = sgrow<uint16_t>(s, B_QBF, nq * PDIM);
+    bt.prep.qnorm = sgrow<float>(s, B_QNORM, nq);
     // 256^-1.5 = 2^-12 exactly.
-    launch_query_prep(q256d, q768d, nq, ix->rotation, 1.0f / 4096.0f, ix->quant, bt.prep, s->st);
+    if (!s->sizing) launch_query_prep(q256d, q768d, nq, ix->rotation, 1.0f / 4096.0f, ix->quant, bt.prep, s->st);
 }
 
 // Route nq prepared queries into (ps, pc, pn) with row stride `stride`.
 inline void route_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, size_t nq, const sieve_params* p, float* ps,
                  uint32_t* pc, uint32_t* pn, int stride) {
-    uint32_t* ovf = grow<uint32_t>(s->b[B_OVF], 1);
-    launch_zero_words(ovf, 1, s->st);
+    uint32_t* ovf = sgrow<uint32_t>(s, B_OVF, 1);
+    if (!s->sizing) launch_zero_words(ovf, 1, s->st);  // the filter plans' overflow flag; routing never sets it
     const size_t per = std::max<size_t>(1, std::min<size_t>(nq, (size_t(1) << 30) / (4 * (size_t)ix->k)));
-    float* approx = grow<float>(s->b[B_APPROX], per * ix->k);
-    float* tmax = grow<float>(s->b[B_TMAX], per * ((ix->k + 15) / 16));
+    float* approx = sgrow<float>(s, B_APPROX, per * ix->k);
+    float* tmax = sgrow<float>(s, B_TMAX, per * ((ix->k + 15) / 16));
+    if (s->sizing) return;
     for (size_t q0 = 0; q0 < nq; q0 += per) {
         size_t m = std::min(per, nq - q0);
         launch_gemm_bf16_route(bt.prep.qbf + q0 * PDIM, ix->cents_bf, nullptr, approx, tmax, (int)m, (int)ix->k, ix->k,
@@ -352,7 +376,6 @@ inline void route_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt,
         ra.out_c = pc + q0 * stride;
         ra.out_n = pn + q0;
         ra.out_stride = stride;
-        ra.overflow = ovf;
         ra.l2 = 0;
         ra.dims = PDIM;
         ra.cstride = PDIM;
@@ -361,15 +384,6 @@ inline void route_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt,
         launch_route_select(ra, m, s->st);
     }
     SIEVE_HIP(hipGetLastError());
-}
-
-inline void check_overflow(sieve_stream* s) {
-    uint32_t h = 0;
-    SIEVE_HIP(hipMemcpyAsync(&h, s->b[B_OVF].p, 4, hipMemcpyDeviceToHost, s->st));
-    SIEVE_HIP(hipStreamSynchronize(s->st));
-    SIEVE_REQUIRE(h == 0, SIEVE_ELIMIT,
-                  "routing: more than " + std::to_string(MAX_RESCORE) +
-                      " centroids within the BF16 error band of the nprobe-th score");
 }
 
 // The live delta segment a search reads (defaults: none).
@@ -396,10 +410,10 @@ inline void set_scan_plan(ScanArgs& a, sieve_stream* s, size_t nq, uint32_t npro
     a.chunk_max = pl.chunk_max;
     SIEVE_REQUIRE(nq * pl.chunk_max < (1ull << 31), SIEVE_ELIMIT, "scan grid too large");
     if (pl.chunk_max > 1) {
-        a.part = grow<uint2>(s->b[B_PART], nq * pl.chunk_max * (size_t)a.cand);
-        a.part_n = grow<uint32_t>(s->b[B_PARTN], nq * pl.chunk_max);
-        a.gthr = grow_zeroed<unsigned long long>(s->b[B_GTHR], nq, s->st);
-        a.qdone = grow_zeroed<uint32_t>(s->b[B_QDONE], nq, s->st);
+        a.part = sgrow<uint2>(s, B_PART, nq * pl.chunk_max * (size_t)a.cand);
+        a.part_n = sgrow<uint32_t>(s, B_PARTN, nq * pl.chunk_max);
+        a.gthr = sgrow_zeroed<unsigned long long>(s, B_GTHR, nq);
+        a.qdone = sgrow_zeroed<uint32_t>(s, B_QDONE, nq);
     }
 }
 
@@ -441,12 +455,12 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
             CmScratch cm{};
             cm.k = ix->k;
             cm.nseg = a.dblock_off ? 2 * ix->k : ix->k;
-            cm.seg_count = grow<uint32_t>(s->b[B_CM_SEGC], cm.nseg);
-            cm.seg_qoff = grow<uint32_t>(s->b[B_CM_QOFF], cm.nseg + 1);
-            cm.seg_ioff = grow<uint32_t>(s->b[B_CM_IOFF], cm.nseg + 1);
-            cm.seg_q = grow<uint32_t>(s->b[B_CM_Q], nq * (size_t)pstride * (a.dblock_off ? 2 : 1));
-            cm.seg_cent = grow<float>(s->b[B_CM_CENT], nq * (size_t)pstride * (a.dblock_off ? 2 : 1));
-            cm.probe_lim = grow<uint32_t>(s->b[B_CM_LIM], nq);
+            cm.seg_count = sgrow<uint32_t>(s, B_CM_SEGC, cm.nseg);
+            cm.seg_qoff = sgrow<uint32_t>(s, B_CM_QOFF, cm.nseg + 1);
+            cm.seg_ioff = sgrow<uint32_t>(s, B_CM_IOFF, cm.nseg + 1);
+            cm.seg_q = sgrow<uint32_t>(s, B_CM_Q, nq * (size_t)pstride * (a.dblock_off ? 2 : 1));
+            cm.seg_cent = sgrow<float>(s, B_CM_CENT, nq * (size_t)pstride * (a.dblock_off ? 2 : 1));
+            cm.probe_lim = sgrow<uint32_t>(s, B_CM_LIM, nq);
             // Work items: sum over segments of ceil(queries / 32) * ceil(blocks / chunk) <= the
             // bound below (pairs * max blocks / (32 chunk) + pairs / 32 + blocks / chunk + segments).
             const double pairs = (double)nq * pstride * (a.dblock_off ? 2 : 1);
@@ -456,17 +470,18 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
             const double bound = pairs * maxnb / (32.0 * chunk) + pairs / 32.0 + allnb / chunk + cm.nseg + 2.0;
             SIEVE_REQUIRE(bound < 4.0e9, SIEVE_ELIMIT, "cluster scan: too many work items");
             cm.items_cap = (uint32_t)bound;
-            cm.items = grow<uint4>(s->b[B_CM_ITEMS], cm.items_cap);
+            cm.items = sgrow<uint4>(s, B_CM_ITEMS, cm.items_cap);
             cm.phases = std::max<uint32_t>(1, ix->cm_phases);
-            cm.work = grow<uint32_t>(s->b[B_CM_WORK], cm.phases);
+            cm.work = sgrow<uint32_t>(s, B_CM_WORK, cm.phases);
             cm.seed_slots = ix->cm_seed_slots ? ix->cm_seed_slots : 24 * cand;
             cm.chunk_blocks = std::max<uint32_t>(1, ix->cm_chunk_blocks);
             a.ext_cap = ix->cm_ext_cap ? ix->cm_ext_cap : 64 * cand;
-            a.seed = grow<uint2>(s->b[B_CM_SEED], nq * (size_t)cand);
-            a.seed_n = grow<uint32_t>(s->b[B_CM_SEEDN], nq);
-            a.seed_thr = grow<unsigned long long>(s->b[B_CM_THR], nq);
-            a.ext = grow<uint2>(s->b[B_CM_EXT], nq * (size_t)a.ext_cap);
-            a.ext_n = grow_zeroed<uint32_t>(s->b[B_CM_EXTN], nq, s->st);  // the finish pass leaves it zero
+            a.seed = sgrow<uint2>(s, B_CM_SEED, nq * (size_t)cand);
+            a.seed_n = sgrow<uint32_t>(s, B_CM_SEEDN, nq);
+            a.seed_thr = sgrow<unsigned long long>(s, B_CM_THR, nq);
+            a.ext = sgrow<uint2>(s, B_CM_EXT, nq * (size_t)a.ext_cap);
+            a.ext_n = sgrow_zeroed<uint32_t>(s, B_CM_EXTN, nq);  // the finish pass leaves it zero
+            if (s->sizing) return;
             launch_scan_cm(a, cm, nq, variant, s->st);
             SIEVE_HIP(hipGetLastError());
             return;
@@ -485,14 +500,15 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
                 (uint32_t)std::min(256.0, std::max(1.0, std::ceil(worst / append_blocks)));
             SIEVE_REQUIRE(nq * (double)bound_chunks < 2.0e9 && nq * (double)append_chunks < 2.0e9, SIEVE_ELIMIT,
                           "scan grid too large");
-            a.part = grow<uint2>(s->b[B_PART], nq * (size_t)bound_chunks * 16);
-            a.part_n = grow<uint32_t>(s->b[B_PARTN], nq * (size_t)bound_chunks);
+            a.part = sgrow<uint2>(s, B_PART, nq * (size_t)bound_chunks * 16);
+            a.part_n = sgrow<uint32_t>(s, B_PARTN, nq * (size_t)bound_chunks);
             a.ext_cap = ix->cm_ext_cap ? ix->cm_ext_cap : 64 * cand;
-            a.seed = grow<uint2>(s->b[B_CM_SEED], nq * (size_t)cand);
-            a.seed_n = grow<uint32_t>(s->b[B_CM_SEEDN], nq);
-            a.seed_thr = grow<unsigned long long>(s->b[B_CM_THR], nq);
-            a.ext = grow<uint2>(s->b[B_CM_EXT], nq * (size_t)a.ext_cap);
-            a.ext_n = grow_zeroed<uint32_t>(s->b[B_CM_EXTN], nq, s->st);  // the finish pass leaves it zero
+            a.seed = sgrow<uint2>(s, B_CM_SEED, nq * (size_t)cand);
+            a.seed_n = sgrow<uint32_t>(s, B_CM_SEEDN, nq);
+            a.seed_thr = sgrow<unsigned long long>(s, B_CM_THR, nq);
+            a.ext = sgrow<uint2>(s, B_CM_EXT, nq * (size_t)a.ext_cap);
+            a.ext_n = sgrow_zeroed<uint32_t>(s, B_CM_EXTN, nq);  // the finish pass leaves it zero
+            if (s->sizing) return;
             launch_scan_small(a, nq, false, bound_chunks, append_blocks, append_chunks, s->st);
             SIEVE_HIP(hipGetLastError());
             return;
@@ -501,6 +517,7 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
     }
     set_scan_plan(a, s, nq, (uint32_t)pstride, (double)ix->nblocks / ix->k, ix->max_cluster_blocks,
                   ix->scan_chunk_blocks, min_chunk_blocks);
+    if (s->sizing) return;
     launch_scan(a, nq, variant, s->st);
     SIEVE_HIP(hipGetLastError());
 }
@@ -543,14 +560,15 @@ inline void rerank_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt
     a.reranked = reranked;
     const bool t2 = a.mode == SIEVE_RERANK_TIERED && a.t2_codes;
     SIEVE_REQUIRE(rerank_lds_bytes(cstride, t2) <= 160 * 1024, SIEVE_ELIMIT, "rerank workspace exceeds LDS");
+    if (s->sizing) return;
     launch_rerank(a, nq, s->st);
     SIEVE_HIP(hipGetLastError());
 }
 
 // The whole batched search (prep, route, scan, rerank) of nq queries already in device
 // memory, enqueued on s's stream with no host synchronisation. Counters land in B_CNT
-// (scanned, reranked, past-threshold), probe counts in B_PN, the routing overflow flag in
-// B_OVF; events ev[0..3] bracket route, scan and rerank.
+// (scanned, reranked, past-threshold), probe counts in B_PN; events ev[0..3] bracket route,
+// scan and rerank.
 inline void enqueue_search(const sieve_index* ix, sieve_stream* s, const float* q2, const float* q7, size_t nq,
                            const sieve_params* p, const ViewArgs& view_in, float* os, uint32_t* orr, uint32_t* on,
                            const LiveArgs* lv = nullptr) {
@@ -560,29 +578,29 @@ inline void enqueue_search(const sieve_index* ix, sieve_stream* s, const float* 
         view.deleted_words = lv->deleted_words;
     }
     const int pstride = (int)std::min<uint32_t>(p->nprobe, ix->k);
-    unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 4);
-    grow<unsigned long long>(s->b[B_STATS], 6);  // finish_search's packed counters, sized with the rest
-    float* ps = grow<float>(s->b[B_PS], nq * pstride);
-    uint32_t* pc = grow<uint32_t>(s->b[B_PC], nq * pstride);
-    uint32_t* pn = grow<uint32_t>(s->b[B_PN], nq);
-    float* ce = grow<float>(s->b[B_CE], nq * p->cand);
-    uint32_t* cr = grow<uint32_t>(s->b[B_CR], nq * p->cand);
-    float* cx = grow<float>(s->b[B_CX], nq * p->cand);
-    uint32_t* cn = grow<uint32_t>(s->b[B_CN], nq);
-    launch_zero_words(reinterpret_cast<uint32_t*>(cnt), 8, s->st);  // a kernel: graph-capturable
-    SIEVE_HIP(hipEventRecord(s->ev[0], s->st));
+    unsigned long long* cnt = sgrow<unsigned long long>(s, B_CNT, 4);
+    sgrow<unsigned long long>(s, B_STATS, 6);  // finish_search's packed counters, sized with the rest
+    float* ps = sgrow<float>(s, B_PS, nq * pstride);
+    uint32_t* pc = sgrow<uint32_t>(s, B_PC, nq * pstride);
+    uint32_t* pn = sgrow<uint32_t>(s, B_PN, nq);
+    float* ce = sgrow<float>(s, B_CE, nq * p->cand);
+    uint32_t* cr = sgrow<uint32_t>(s, B_CR, nq * p->cand);
+    float* cx = sgrow<float>(s, B_CX, nq * p->cand);
+    uint32_t* cn = sgrow<uint32_t>(s, B_CN, nq);
+    if (!s->sizing) launch_zero_words(reinterpret_cast<uint32_t*>(cnt), 8, s->st);  // a kernel: graph-capturable
+    record(s, 0);
     Batch bt;
     prep_queries(ix, s, q2, q7, nq, bt);
     route_batch(ix, s, bt, nq, p, ps, pc, pn, pstride);
-    SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
+    record(s, 1);
     scan_batch(ix, s, bt, nq, ps, pc, pn, pstride, p, view, ce, cr, cx, cn, (int)p->cand, cnt, lv);
-    SIEVE_HIP(hipEventRecord(s->ev[2], s->st));
+    record(s, 2);
     rerank_batch(ix, s, bt, nq, ce, cr, cx, cn, (int)p->cand, p, os, orr, on, cnt + 1, lv);
-    SIEVE_HIP(hipEventRecord(s->ev[3], s->st));
+    record(s, 3);
 }
 
-// Waits for an enqueued search, fails if routing overflowed, and records its counters and
-// stage times in s->last (and adds them to *t).
+// Waits for an enqueued search and records its counters and stage times in s->last (and adds
+// them to *t).
 inline void finish_search(sieve_stream* s, size_t nq, sieve_timing* t) {
     // One small copy into pinned memory: the counters, the probe total and the overflow flag.
     const unsigned long long* cnt = static_cast<const unsigned long long*>(s->b[B_CNT].p);
@@ -593,9 +611,6 @@ inline void finish_search(sieve_stream* s, size_t nq, sieve_timing* t) {
     SIEVE_HIP(hipMemcpyAsync(s->hstats, packed, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
     SIEVE_HIP(hipStreamSynchronize(s->st));
     const unsigned long long* hcnt = s->hstats;
-    SIEVE_REQUIRE(hcnt[5] == 0, SIEVE_ELIMIT,
-                  "routing: more than " + std::to_string(MAX_RESCORE) +
-                      " centroids within the BF16 error band of the nprobe-th score");
     s->last = sieve_stats{};
     s->last.queries = nq;
     s->last.scanned = hcnt[0];
@@ -616,8 +631,10 @@ inline void finish_search(sieve_stream* s, size_t nq, sieve_timing* t) {
     }
 }
 
-// The delta segment a live state hands to searches (csrc/live.hip).
+// The delta segment a live state hands to searches (csrc/live.hip), and the same with the delta
+// lists at the largest size the live state reserves (for sizing scratch).
 LiveArgs live_args(const sieve_live* l);
+LiveArgs live_args_max(const sieve_live* l);
 
 // filter::search enqueued without host synchronisation (csrc/filter.hip).
 struct FilterRun {
@@ -628,6 +645,9 @@ struct FilterRun {
 FilterRun enqueue_filter_search(const sieve_index* ix, const sieve_filter* f, uint32_t plan, uint64_t expand_target,
                                 sieve_stream* s, const float* q2, const float* q7, size_t nq, const sieve_params* p,
                                 float* os, uint32_t* orr, uint32_t* on, const LiveArgs* lv);
+
+void size_filter_search(const sieve_index* ix, sieve_stream* s, size_t nq, const sieve_params* p,
+                        const LiveArgs* lv);
 
 inline void ensure_events(sieve_stream* s) {
     for (auto& e : s->ev)

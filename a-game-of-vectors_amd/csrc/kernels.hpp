@@ -61,7 +61,8 @@ void launch_gemm_bf16_route(const uint16_t* qbf, const uint16_t* cbf, const floa
                             int m, int n, size_t lds, hipStream_t st);
 
 struct RouteArgs {
-    const float* approx;       // nq x k approximate scores (row stride lds)
+    float* approx;             // nq x k approximate scores (row stride lds); a query whose band
+                               // overflows the rescoring buffer gets its exact scores written here
     size_t lds;
     const float* tmax;         // nq x ceil(k / 16) tile maxima of the approximate scores, or null
     int k;
@@ -77,7 +78,6 @@ struct RouteArgs {
     uint32_t* out_c;
     uint32_t* out_n;
     int out_stride;
-    uint32_t* overflow;        // set to 1 if a query had more than MAX_RESCORE near-margin centroids
     // metric: inner product over `dims` (0) or L2 route score 2 <q, c> - |c|^2 (1)
     int l2;
     int dims;                  // rescoring dims (256, or 100 for L2)

@@ -464,9 +464,9 @@ void live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, size_t n
     SIEVE_HIP(hipGetLastError());
     SIEVE_HIP(hipMemcpyAsync(l->ext + r0, ext, n * 4, on_device(ext) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                              st));
-    // The routing overflow check synchronises the stream; nothing is committed before it (the
-    // log and row counts advance below).
-    check_overflow(s);
+    // The encoder's launches finish before anything is committed (the log and row counts advance
+    // below).
+    SIEVE_HIP(hipStreamSynchronize(s->st));
     // Upserts, in order (src/live.rs:369-378): a live external id's old row is tombstoned.
     std::vector<uint32_t> he;
     const uint32_t* eh = ext;
@@ -510,6 +510,11 @@ void live_delete(sieve_live* l, const uint32_t* ext, size_t n) {
 
 namespace sieve {
 LiveArgs live_args(const sieve_live* l) { return l->args(); }
+LiveArgs live_args_max(const sieve_live* l) {
+    LiveArgs a = l->args();
+    a.dblocks = (uint32_t)(l->cap_slots / BLOCK);  // every delta list padded to whole blocks fits
+    return a;
+}
 }  // namespace sieve
 
 extern "C" {

@@ -513,14 +513,45 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
             radix_over_memory();
     }
     __syncthreads();
-    const int nc = (int)ncand;
-    if (nc > MAX_RESCORE) {
-        if (t == 0) {
-            atomicOr(a.overflow, 1u);
-            a.out_n[qi] = 0;
+    if (ncand > (uint32_t)MAX_RESCORE) {
+        // More centroids in the band than the rescoring buffer holds (a degenerate query: a zero
+        // q256 ties every score): this query alone takes the exact route. Its row of S gets the
+        // exact f32 score of every centroid (summed as below), and the top n by (score desc, id
+        // asc) are selected from it directly: the n-th largest score, then among the scores equal
+        // to it the smallest ids. The result is the one the rescoring path would give.
+        __syncthreads();
+        float* Sw = a.approx + qi * a.lds;
+        for (int i = t; i < a.k; i += 256) {
+            const float4* row = reinterpret_cast<const float4*>(a.cents + (size_t)i * a.cstride);
+            float acc = -0.0f;
+            for (int j = 0; j < a.dims / 4; j++) {
+                float4 x = row[j];
+                acc = acc + qs[4 * j] * x.x;
+                acc = acc + qs[4 * j + 1] * x.y;
+                acc = acc + qs[4 * j + 2] * x.z;
+                acc = acc + qs[4 * j + 3] * x.w;
+            }
+            Sw[i] = acc;
         }
-        return;
+        __syncthreads();
+        const uint32_t kth = block_kth_u32(a.k, n, key, hist, sh);
+        if (t == 0) {
+            ncand = 0;
+            ntl = 0;
+        }
+        __syncthreads();
+        for (int i = t; i < a.k; i += 256)
+            if (key(i) > kth) atomicAdd(&ntl, 1u);
+        __syncthreads();
+        const int need = n - (int)ntl;  // >= 1: at least n keys are >= kth
+        const uint32_t idk = block_kth_u32(a.k, need, [&](int i) { return key(i) == kth ? ~(uint32_t)i : 0u; }, hist, sh);
+        for (int i = t; i < a.k; i += 256) {
+            const uint32_t x = key(i);
+            if (x > kth || (x == kth && ~(uint32_t)i >= idk)) cids[atomicAdd(&ncand, 1u)] = (uint32_t)i;
+        }
+        __syncthreads();
     }
+    const int nc = (int)ncand;
     int n2 = 1;
     while (n2 < nc) n2 <<= 1;
     // Exact f32 rescoring: products summed left to right from -0.0 (src/blas.rs:51).

@@ -6,7 +6,8 @@
 // on the slot's own stream, so several batches are on the device at once and overlap. A slot
 // holds the device scratch of one batch; the number of slots is bounded by the HBM budget,
 // which is how memory gates admission: a request waits in the host queue until a slot, i.e.
-// device memory, frees up. A completer thread retires batches in launch order and hands each
+// device memory, frees up. A slot's scratch is reserved at creation for the worst case of every
+// path a batch can take (sizing mode of the pipeline), so batches never allocate. A completer thread retires batches in launch order and hands each
 // request its own results.
 //
 // Each query is scanned and reranked by its own workgroups, so a request's results do not
@@ -269,7 +270,7 @@ void completer_loop(sieve_server* sv) {
                 r.error = hipGetErrorString(e);
             } else if (ovf) {
                 r.status = SIEVE_ELIMIT;
-                r.error = "routing: too many centroids within the BF16 error band";
+                r.error = "expand plan: the cut needs more clusters than its probe limit";
             } else {
                 const uint32_t n = static_cast<uint32_t*>(slot->counts.p)[i];
                 r.count = n;
@@ -291,17 +292,39 @@ void pinned(HostPinned& h, size_t n) {
     SIEVE_HIP(hipHostMalloc(&h.p, std::max<size_t>(n * sizeof(T), 16), hipHostMallocDefault));
 }
 
-// Device bytes one slot holds for a batch of mb queries (inputs, outputs and stream scratch).
-size_t slot_bytes(const sieve_index* ix, const sieve_params& p, size_t mb) {
-    const size_t ps = std::min<uint32_t>(p.nprobe, ix->k);
-    size_t b = mb * (DIM + PDIM) * 4 + mb * (p.k * 8 + 4);                // inputs, outputs
-    b += mb * (1024 + 16 + DIM + PDIM + 8 + PDIM * 2 + 4);                 // query prep
-    b += std::min<size_t>(mb, (size_t(1) << 30) / (4 * (size_t)ix->k)) * ix->k * 4;  // route scores
-    b += mb * (ps * 8 + 4) + mb * (p.cand * 12 + 4) + 64;                  // probes, candidates
-    return b;
+// Device bytes of a slot's own staging (inputs and outputs of a batch of mb queries).
+size_t slot_io_bytes(const sieve_params& p, size_t mb) { return mb * (DIM + PDIM) * 4 + mb * (p.k * 8 + 4) + 16; }
+
+// The scratch a slot's stream needs for every path a batch of 1 .. mb queries can take: the plain
+// search (small-batch, per-query or cluster-major scan by batch size), Post, Expand and Brute
+// filters, with the live delta lists at their largest. Recorded per scratch slot in `need`
+// (B_NSLOTS entries) by running the pipeline in sizing mode, which enqueues nothing.
+void slot_scratch_need(const sieve_server* sv, size_t mb, size_t* need) {
+    sieve_stream z;
+    z.sizing = need;
+    LiveArgs lmax;
+    const LiveArgs* lv = nullptr;
+    if (sv->live) {
+        lmax = live_args_max(sv->live);
+        lv = &lmax;
+    }
+    const float* q = reinterpret_cast<const float*>(uintptr_t(256));
+    float* os = reinterpret_cast<float*>(uintptr_t(256));
+    uint32_t* ou = reinterpret_cast<uint32_t*>(uintptr_t(256));
+    for (size_t m = 1; m <= mb; m++) {
+        enqueue_search(sv->ix, &z, q, q, m, &sv->p, ViewArgs{}, os, ou, ou, lv);
+        try {
+            size_filter_search(sv->ix, &z, m, &sv->p, lv);
+        } catch (const Error&) {
+            // a plan these parameters cannot run fails its requests with that error instead
+        }
+    }
+    z.sizing = nullptr;
 }
 
 }  // namespace
+
+extern "C" {}  // namespace
 
 extern "C" {
 
@@ -326,7 +349,12 @@ int sieve_server_create(const sieve_index* ix, const sieve_params* p, const siev
             SIEVE_HIP(hipMemGetInfo(&fr, &tot));
             budget = fr / 4;
         }
-        const size_t per = slot_bytes(ix, *p, sv->o.max_batch);
+        // Admission by HBM: a slot holds the worst-case scratch of every path a batch can take,
+        // reserved here, so no batch allocates device memory once the server runs.
+        std::vector<size_t> need(B_NSLOTS, 0);
+        slot_scratch_need(sv.get(), sv->o.max_batch, need.data());
+        size_t per = slot_io_bytes(*p, sv->o.max_batch);
+        for (size_t b : need) per += b;
         size_t nslots = budget / per;
         if (sv->o.max_inflight) nslots = std::min<size_t>(nslots, sv->o.max_inflight);
         SIEVE_REQUIRE(nslots >= 1, SIEVE_ENOMEM,
@@ -352,20 +380,11 @@ int sieve_server_create(const sieve_index* ix, const sieve_params* p, const siev
             sl->dscores = dmalloc<float>(mb * K, nullptr);
             sl->drows = dmalloc<uint32_t>(mb * K, nullptr);
             sl->dcounts = dmalloc<uint32_t>(mb, nullptr);
-            // Pre-grow the query-prep and routing scratch to a full batch; the scan's scratch grows
-            // with the first batches that need it (its size depends on the path a batch takes).
+            // The reserved scratch, zeroed (counters that kernels leave at zero start zeroed).
             sieve_stream* s = sl->s.get();
-            grow<uint8_t>(s->b[B_TABLES], mb * 1024);
-            grow<float4>(s->b[B_QAB], mb);
-            grow<int8_t>(s->b[B_WQ768], mb * DIM);
-            grow<int8_t>(s->b[B_WQ256], mb * PDIM);
-            grow<float2>(s->b[B_INV], mb);
-            grow<uint16_t>(s->b[B_QBF], mb * PDIM);
-            grow<float>(s->b[B_QNORM], mb);
-            const size_t route_rows = std::min<size_t>(mb, (size_t(1) << 30) / (4 * (size_t)ix->k));
-            grow<float>(s->b[B_APPROX], route_rows * ix->k);
-            grow<float>(s->b[B_TMAX], route_rows * ((ix->k + 15) / 16));
-            grow<uint32_t>(s->b[B_OVF], 1);
+            for (int b = 0; b < B_NSLOTS; b++)
+                if (need[b]) grow_zeroed<uint8_t>(s->b[b], need[b], s->st);
+            SIEVE_HIP(hipStreamSynchronize(s->st));
             sv->slots.push_back(std::move(sl));
         }
         sv->stats.slots = (uint32_t)nslots;
@@ -454,8 +473,10 @@ int sieve_server_wait(sieve_server* sv, uint64_t ticket, float* scores, uint32_t
         std::unique_lock<std::mutex> lk(sv->mu);
         auto it = sv->reqs.find(ticket);
         SIEVE_REQUIRE(it != sv->reqs.end(), SIEVE_EINVAL, "unknown ticket " + std::to_string(ticket));
-        sv->cv_done.wait(lk, [&] { return it->second.done; });
+        // Submits may rehash the map while this thread waits: keep the element (whose address
+        // is stable), not the iterator, and find the ticket again to erase it.
         Request& r = it->second;
+        sv->cv_done.wait(lk, [&] { return r.done; });
         *count = r.count;
         if (scores) std::memcpy(scores, r.scores.data(), r.count * 4);
         if (rows) std::memcpy(rows, r.rows.data(), r.count * 4);
@@ -468,7 +489,7 @@ int sieve_server_wait(sieve_server* sv, uint64_t ticket, float* scores, uint32_t
         }
         rc = r.status;
         err = r.error;
-        sv->reqs.erase(it);
+        sv->reqs.erase(ticket);
     });
     if (g != 0) return g;
     if (rc != 0) {

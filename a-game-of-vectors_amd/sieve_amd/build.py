@@ -322,6 +322,20 @@ class IndexArrays:
     def nblocks(self):
         return self.blocks.shape[0]
 
+    def device(self):
+        """The arrays as device tensors in the on-disk dtypes (u32 / u64 as same-width ints),
+        for Index.create with the rows borrowed in place."""
+        c = self.corpus
+        ids = torch.where(self.ids == PAD, torch.full_like(self.ids, -1), self.ids).to(torch.int32)
+        d = dict(centroids=self.centroids.contiguous(), block_off=self.block_off.contiguous(),
+                 blocks=self.blocks.contiguous(), ids=ids.contiguous(), factors=self.factors.contiguous(),
+                 rotation=torch.as_tensor(self.rotation, device=c.rerank.device), rerank=c.rerank,
+                 quant=c.quant.contiguous(), norm256=c.norm256.contiguous())
+        if self.tier2_sigma is not None:
+            d.update(tier2_sigma=self.tier2_sigma.contiguous(), tier2_codes=self.tier2_codes.contiguous(),
+                     tier2_norm=self.tier2_norm.contiguous())
+        return d
+
     def numpy(self):
         """Host arrays in the on-disk dtypes (the layout both the device index and the oracle
         read)."""

@@ -73,22 +73,6 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def device_arrays(arrays):
-    """The built index as device tensors in the on-disk dtypes (u32/u64 as same-width ints)."""
-    import torch
-    from sieve_amd.build import PAD
-    c = arrays.corpus
-    ids = torch.where(arrays.ids == PAD, torch.full_like(arrays.ids, -1), arrays.ids).to(torch.int32)
-    d = dict(centroids=arrays.centroids.contiguous(), block_off=arrays.block_off.contiguous(),
-             blocks=arrays.blocks.contiguous(), ids=ids.contiguous(), factors=arrays.factors.contiguous(),
-             rotation=torch.as_tensor(arrays.rotation, device=c.rerank.device), rerank=c.rerank,
-             quant=c.quant.contiguous(), norm256=c.norm256.contiguous())
-    if arrays.tier2_sigma is not None:
-        d.update(tier2_sigma=arrays.tier2_sigma.contiguous(), tier2_codes=arrays.tier2_codes.contiguous(),
-                 tier2_norm=arrays.tier2_norm.contiguous())
-    return d
-
-
 def recall10(rows, counts, gt):
     import numpy as np
     from sieve_amd.evaluate import recall_at
@@ -172,7 +156,7 @@ def main():
     # build's freed blocks first.
     torch.cuda.empty_cache()
     log(f"torch holds {torch.cuda.memory_allocated() / 1e9:.1f} GB after the build")
-    ix = Index.create(device_arrays(arrays), borrow_rerank=True)
+    ix = Index.create(arrays.device(), borrow_rerank=True)
     t_build = time.time() - t_build0
     sizes = np.array(arrays.sizes)
     log(f"index on device: {ix.device_bytes / 1e9:.1f} GB + {corpus.rerank.numel() / 1e9:.1f} GB rows; "

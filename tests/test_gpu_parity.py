@@ -124,7 +124,8 @@ def test_route_over_many_centroids_matches_oracle(wide, cfg):
 @pytest.fixture(scope="module")
 def huge_k(small):
     """The small index with 20000 more, empty clusters (random unit centroids, some of them
-    near the queries): routing selects over 20048 centroids, the histogram-and-collect select."""
+    near the queries): routing selects over 20048 centroids, from the 1253 maxima of 16
+    consecutive scores (the select over tile maxima with 16 maxima per thread)."""
     from oracle import oracle as O
     rng = np.random.default_rng(21)
     extra = 20000
@@ -376,3 +377,34 @@ def test_errors_are_reported(small, gpu_index):
     with pytest.raises(SieveError, match="out of range"):
         gpu_index.candidates(small.q256, np.zeros((small.nq, 4), np.float32), bad, np.full(small.nq, 4, np.uint32),
                              Params(nprobe=4, cand=50))
+
+
+# ---------------------------------------------------------------------- degenerate queries ---
+def test_zero_query_routes_exactly_and_leaves_the_batch_alone(small, oracle_index, gpu_index):
+    """A zero q256 ties every centroid score, so its BF16 error band holds every centroid of a
+    wide index: that query alone takes the exact route (all scores in f32, then the top n by
+    score desc, id asc), and the other queries of its batch get their usual results."""
+    from conftest import SmallWorkload
+    w = SmallWorkload(k=6000, nq=12, tier2=False)
+    gix = Index.create(w.host)
+    oix = O.Index(w.host)
+    q256 = w.q256.copy()
+    q768 = w.q768.copy()
+    q256[3] = 0.0
+    q768[3] = 0.0
+    p = Params(nprobe=40, cand=200, k=10, rerank=Rerank.FULL)
+    s, c, n = gix.route(q256, p)
+    assert n[3] == 40 and np.all(s[3, :40] == 0.0)
+    assert np.array_equal(c[3, :40], np.arange(40))  # ties broken by id asc
+    for i in range(w.nq):
+        ws, wc = oix.route(q256[i], p)
+        assert n[i] == len(wc)
+        assert_same_up_to_ties(c[i, :n[i]], s[i, :n[i]], wc, ws)
+    gs, gr, gn = gix.search(q768, q256, p)
+    alone = [gix.search(q768[i:i + 1], q256[i:i + 1], p) for i in range(w.nq)]
+    for i in range(w.nq):
+        assert np.array_equal(gr[i, :gn[i]], alone[i][1][0, :alone[i][2][0]])
+        assert _flt_eq(gs[i, :gn[i]], alone[i][0][0, :alone[i][2][0]])
+    ws, wr, wn, _ = oix.search_batch(q768, q256, p, threads=4)
+    for i in range(w.nq):
+        assert_same_up_to_ties(gr[i, :gn[i]], gs[i, :gn[i]], wr[i, :wn[i]], ws[i, :wn[i]])

@@ -136,3 +136,46 @@ def test_filters_and_live_writes_in_the_serving_loop(small, oracle_index):
     assert sv.stats()["writes"] == 3
     sv.close()
     live.close()
+
+
+def test_zero_query_in_a_batch_does_not_fail_the_others(small):
+    """A degenerate request (zero vectors: every centroid ties, so its routing takes the exact
+    path) batched with normal requests: every request succeeds, and each normal one gets the
+    result it gets alone."""
+    from conftest import SmallWorkload
+    w = SmallWorkload(k=6000, nq=10, tier2=False)
+    ix = Index.create(w.host)
+    p = Params(nprobe=40, cand=200, k=10, rerank=Rerank.FULL)
+    q768, q256 = w.q768.copy(), w.q256.copy()
+    q768[4] = 0.0
+    q256[4] = 0.0
+    alone = [ix.search(q768[i:i + 1], q256[i:i + 1], p) for i in range(w.nq)]
+    sv = Server(ix, p, max_batch=16, max_inflight=1)
+    tickets = [sv.submit(q768[i], q256[i]) for i in range(w.nq)]
+    for i, t in enumerate(tickets):
+        s, r, info = sv.wait(t)
+        ws, wr, wn = alone[i]
+        assert np.array_equal(r, wr[0, :wn[0]]) and np.array_equal(s, ws[0, :wn[0]])
+    sv.close()
+
+
+def test_server_reserves_scratch_for_every_path(small, setup):
+    """The slot footprint counts the worst-case scan scratch: larger than the inputs, outputs,
+    routing scores and candidates alone, and a mixed run (batch sizes from 1 up, so every scan
+    path) allocates nothing more on the device than the slots hold after creation."""
+    import torch
+    _, p, _ = setup
+    ix = Index.create(small.host)
+    ix.set_scan_variant(10)  # batches of 1-8: small-batch scan, 9-63: per-query, 64+: cluster-major
+    sv = Server(ix, p, max_batch=96, max_inflight=2)
+    st = sv.stats()
+    base = 96 * ((768 + 256) * 4 + p.k * 8 + 4) + 96 * ix.k * 4 + 96 * p.cand * 12
+    assert st["slot_bytes"] > base
+    free0 = torch.cuda.mem_get_info()[0]
+    for m in (1, 5, 9, 40, 96):
+        ts = [sv.submit(small.q768[i % small.nq], small.q256[i % small.nq]) for i in range(m)]
+        for t in ts:
+            sv.wait(t)
+    free1 = torch.cuda.mem_get_info()[0]
+    sv.close()
+    assert free0 - free1 < 2 << 20, (free0, free1)
