@@ -289,6 +289,7 @@ int sieve_stream_create(sieve_stream** out) {
         SIEVE_HIP(hipGetDevice(&s->device));
         SIEVE_HIP(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
         ensure_events(s.get());
+        noalloc_setup(s->device);
         *out = s.release();
     });
 }

@@ -194,6 +194,7 @@ struct sieve_stream {
     // Sizing mode (sizing != nullptr, B_NSLOTS entries): the pipeline functions record the bytes
     // each scratch slot would need and enqueue nothing, so a caller can reserve a worst case.
     size_t* sizing = nullptr;
+    uint8_t* sizing_zeroed = nullptr;  // sizing mode: slots whose buffers must start zeroed
     hipEvent_t ev[8] = {};
     sieve_stats last{};
     unsigned long long* hstats = nullptr;  // pinned: the packed counters of the last search
@@ -224,7 +225,10 @@ T* sgrow(sieve_stream* s, int slot, size_t n) {
 }
 template <class T>
 T* sgrow_zeroed(sieve_stream* s, int slot, size_t n) {
-    if (s->sizing) return sgrow<T>(s, slot, n);
+    if (s->sizing) {
+        if (s->sizing_zeroed) s->sizing_zeroed[slot] = 1;
+        return sgrow<T>(s, slot, n);
+    }
     return grow_zeroed<T>(s->b[slot], n, s->st);
 }
 // Event records and launches are skipped in sizing mode.
@@ -648,6 +652,9 @@ FilterRun enqueue_filter_search(const sieve_index* ix, const sieve_filter* f, ui
 
 void size_filter_search(const sieve_index* ix, sieve_stream* s, size_t nq, const sieve_params* p,
                         const LiveArgs* lv);
+
+// Resolves the no-allocation entry point's fixed scratch array on device dev (csrc/noalloc.hip).
+void noalloc_setup(int dev);
 
 inline void ensure_events(sieve_stream* s) {
     for (auto& e : s->ev)

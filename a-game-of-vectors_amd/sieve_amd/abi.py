@@ -158,6 +158,7 @@ def lib():
         "sieve_candidates": (i32, [vp, vp, vp, sz, vp, vp, vp, sz, P, V, vp, vp, vp, vp, T]),
         "sieve_rerank": (i32, [vp, vp, vp, vp, sz, vp, vp, vp, vp, sz, P, V, vp, vp, vp, T]),
         "sieve_search_batch": (i32, [vp, vp, vp, vp, sz, P, V, vp, vp, vp, T]),
+        "sieve_search_batch_noalloc": (i32, [vp, vp, vp, vp, sz, P, V, vp, vp, vp]),
         "sieve_index_autotune": (i32, [vp, vp, vp, vp, sz, P, ctypes.POINTER(ctypes.c_uint32), vp]),
         "sieve_index_set_scan_variant": (i32, [vp, u32]),
         "sieve_index_set_scan_chunk": (i32, [vp, u32]),

@@ -286,6 +286,21 @@ int sieve_search_batch(const sieve_index* ix, sieve_stream* s, const float* q768
                        const sieve_view* v, float* scores, uint32_t* rows, uint32_t* counts,
                        sieve_timing* t);
 
+/* sieve_search_batch without any device allocation. No counterpart in the reference: it is
+ * for callers that cannot call hipMalloc (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation). Every scratch buffer of the search is carved from one
+ * fixed device array of 64 MiB that each GPU the library runs on holds until the process ends;
+ * a batch whose buffers do not fit fails with SIEVE_ELIMIT (at 16384 clusters and 1000
+ * candidates the limit is around a hundred queries per call; fewer with larger probe lists).
+ * Queries, view bitsets and outputs must be device memory. The search is enqueued on s's stream
+ * and the call returns without waiting: results are ready once that stream is synchronised
+ * (sieve_stream_sync), and no counters are recorded. Results equal sieve_search_batch's. All
+ * calls on one GPU share the array, so calls in flight at the same time on one device race:
+ * serialise them (one stream, or events). */
+int sieve_search_batch_noalloc(const sieve_index* ix, sieve_stream* s, const float* q768,
+                               const float* q256, size_t nq, const sieve_params* p,
+                               const sieve_view* v, float* scores, uint32_t* rows, uint32_t* counts);
+
 /* ------------------------------------------------------------- serving loop --- */
 
 /* No counterpart in the reference, whose server is a thread per connection over TCP
