@@ -233,6 +233,30 @@ sieve_l2_index* create_l2_index(const sieve_l2_desc* d) {
         SIEVE_REQUIRE(hbad == 0, SIEVE_EINVAL, std::to_string(hbad) + " slot ids are beyond the base");
     }
     ix->rotation = upload(d->rotation, 3 * PD, acct);
+    if (d->refine_bits > 1) {
+        // The per-slot extra section into two plane words and f per slot, through a staging
+        // buffer when it is host memory.
+        SIEVE_REQUIRE(d->refine_bits <= SIEVE_REFINE_MAX_BITS && d->refine, SIEVE_EINVAL,
+                      "refinement: refine_bits must be 2 or 3, with its extra section");
+        ix->bits = (int)d->refine_bits;
+        const size_t rb = SIEVE_REFINE_BYTES(d->refine_bits);
+        ix->rplanes = dmalloc<uint4>(ix->nslots * 2, acct);
+        ix->rf = dmalloc<float>(ix->nslots, acct);
+        const bool dev = on_device(d->refine);
+        const size_t chunk = size_t(1) << 22;  // slots per pass
+        uint8_t* stage = dev ? nullptr : dmalloc<uint8_t>(std::min<size_t>(chunk, ix->nslots) * rb, nullptr);
+        for (size_t s0 = 0; s0 < ix->nslots; s0 += chunk) {
+            const size_t m = std::min<size_t>(chunk, ix->nslots - s0);
+            const uint8_t* src = d->refine + s0 * rb;
+            if (!dev) {
+                SIEVE_HIP(hipMemcpy(stage, src, m * rb, hipMemcpyHostToDevice));
+                src = stage;
+            }
+            launch_unpack_refine(src, m, ix->bits, ix->rplanes + 2 * s0, ix->rf + s0, st);
+            SIEVE_HIP(hipDeviceSynchronize());
+        }
+        if (stage) (void)hipFree(stage);
+    }
     if (d->borrow_base && on_device(d->base)) {
         ix->base = const_cast<int8_t*>(d->base);
         ix->base_owned = false;
@@ -489,8 +513,8 @@ int sieve_candidates(const sieve_index* ix, sieve_stream* s, const float* q256, 
         const uint32_t* pc = dev_in(s, B_PC, probe_clusters, nq * nprobe_stride);
         const uint32_t* pn = dev_in(s, B_PN, probe_counts, nq);
         ViewArgs view = make_view(s, v, nq);
-        unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 4);
-        SIEVE_HIP(hipMemsetAsync(cnt, 0, 32, s->st));
+        unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 8);
+        SIEVE_HIP(hipMemsetAsync(cnt, 0, 64, s->st));
         Batch bt;
         prep_queries(ix, s, q2, q7, nq, bt);
         DevOut<float> oe = dev_out(s, B_CE, est, nq * p->cand);
@@ -550,8 +574,8 @@ int sieve_rerank(const sieve_index* ix, sieve_stream* s, const float* q768, cons
         const uint32_t* cr = dev_in(s, B_CR, rows, nq * cand_stride);
         const float* cx = dev_in(s, B_CX, errs, nq * cand_stride);
         const uint32_t* cn = dev_in(s, B_CN, cand_counts, nq);
-        unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 4);
-        SIEVE_HIP(hipMemsetAsync(cnt, 0, 32, s->st));
+        unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 8);
+        SIEVE_HIP(hipMemsetAsync(cnt, 0, 64, s->st));
         Batch bt;
         prep_queries(ix, s, q2, q7, nq, bt);
         DevOut<float> os = dev_out(s, B_OS, scores, nq * p->k);
@@ -615,7 +639,10 @@ static uint64_t search_graphed(const sieve_index* ix, sieve_stream* s, const flo
     auto it = std::find_if(s->gcache.begin(), s->gcache.end(), [&](const GraphEntry& g) { return g.key == key; });
     auto eager = [&](GraphEntry* g) {
         enqueue_search(ix, s, q2, q7, nq, p, view, os, orr, on);
-        if (g) g->gen = g_scratch_gen.load();
+        if (g) {
+            g->gen = g_scratch_gen.load();
+            g->cm_timed = s->cm_timed;
+        }
         return (uint64_t)0;
     };
     if (it == s->gcache.end()) {
@@ -635,6 +662,7 @@ static uint64_t search_graphed(const sieve_index* ix, sieve_stream* s, const flo
     }
     if (g.exec) {
         SIEVE_HIP(hipGraphLaunch(g.exec, s->st));
+        s->cm_timed = g.cm_timed;
         return 2;
     }
     // Capture. Relaxed mode: the stage functions may query function attributes; nothing in
@@ -662,6 +690,7 @@ static uint64_t search_graphed(const sieve_index* ix, sieve_stream* s, const flo
         return eager(nullptr);
     }
     g.exec = exec;
+    g.cm_timed = s->cm_timed;
     SIEVE_HIP(hipGraphLaunch(g.exec, s->st));
     return 1;
 }
@@ -712,7 +741,6 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
     return guarded([&] {
         constexpr int D = 100;
         SIEVE_REQUIRE(ix && s && q && p && scores && rows && counts, SIEVE_EINVAL, "null argument");
-        SIEVE_REQUIRE(p->refine == 0, SIEVE_EINVAL, "multi-bit refinement is not supported (refine must be 0)");
         SIEVE_REQUIRE(p->nprobe >= 1 && p->nprobe <= (uint32_t)MAX_PROBE, SIEVE_ELIMIT,
                       "nprobe must be in [1, " + std::to_string(MAX_PROBE) + "]");
         SIEVE_REQUIRE(p->cand >= 1 && p->cand <= (uint32_t)MAX_CAND, SIEVE_ELIMIT,
@@ -720,15 +748,27 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
         SIEVE_REQUIRE(p->k >= 1 && p->k <= (uint32_t)MAX_CAND, SIEVE_ELIMIT, "k out of range");
         if (nq == 0) return;
         const int pstride = (int)std::min<uint32_t>(p->nprobe, ix->k);
+        // src/bigann.rs:688: refinement only on an index with bits > 1 and refine > 0.
+        const bool refining = ix->bits > 1 && p->refine > 0;
+        // A shortlist whose buffer (2 K) outgrows every slot a query can probe is never cut, so
+        // it needs no storage: K = 0 tells the kernel so.
+        uint64_t K = std::max<uint32_t>(p->refine, p->cand);
+        if (2 * K > (uint64_t)pstride * std::max<uint32_t>(ix->max_cluster_blocks, 1) * BLOCK) K = 0;
+        if (refining)
+            SIEVE_REQUIRE(K <= 2 * (uint64_t)MAX_CAND && refine_lds_bytes((int)K, (int)p->cand, pstride) <= 160 * 1024,
+                          SIEVE_ELIMIT,
+                          "refine: a shortlist of " + std::to_string(K) + " slots exceeds the scan's LDS (at most " +
+                              std::to_string(2 * MAX_CAND) + ")");
         const int8_t* dq = dev_in(s, B_Q768, q, nq * D);
         L2Prep prep;
+        prep.qrot = refining ? grow<float>(s->b[B_QROT], nq * 128) : nullptr;
         prep.tables = grow<uint8_t>(s->b[B_TABLES], nq * 512);
         prep.qab = grow<float4>(s->b[B_QAB], nq);
         prep.qbf = grow<uint16_t>(s->b[B_QBF], nq * PDIM);
         prep.qf = grow<float>(s->b[B_QF], nq * D);
         prep.qq = grow<float>(s->b[B_QQ], nq);
         prep.qnorm = grow<float>(s->b[B_QNORM], nq);
-        unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 4);
+        unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 8);
         uint32_t* ovf = grow<uint32_t>(s->b[B_OVF], 1);
         float* ps = grow<float>(s->b[B_PS], nq * pstride);
         uint32_t* pc = grow<uint32_t>(s->b[B_PC], nq * pstride);
@@ -740,7 +780,7 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
         DevOut<float> os = dev_out(s, B_OS, scores, nq * p->k);
         DevOut<uint32_t> orr = dev_out(s, B_OR, rows, nq * p->k);
         DevOut<uint32_t> on = dev_out(s, B_ON, counts, nq);
-        SIEVE_HIP(hipMemsetAsync(cnt, 0, 32, s->st));
+        SIEVE_HIP(hipMemsetAsync(cnt, 0, 64, s->st));
         SIEVE_HIP(hipMemsetAsync(ovf, 0, 4, s->st));
         SIEVE_HIP(hipEventRecord(s->ev[0], s->st));
         // 128^-1.5 as (d as f32).powf(-1.5) (src/rabitq.rs:67)
@@ -802,7 +842,35 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
         a.scanned = cnt;
         const size_t lds = scan_lds_bytes(a.cand, pstride);
         SIEVE_REQUIRE(lds <= 160 * 1024, SIEVE_ELIMIT, "scan workspace exceeds LDS");
-        if (nq <= ix->small_max_queries) {
+        if (refining) {
+            // The 1-bit shortlist replayed in probe order, refined estimates for its entrants,
+            // the best cand of those (src/bigann.rs:683-729).
+            L2RefineArgs ra{};
+            ra.codes = ix->codes;
+            ra.mul = ix->mul;
+            ra.nx = ix->nx;
+            ra.ids = ix->ids;
+            ra.block_off = ix->block_off;
+            ra.planes = ix->rplanes;
+            ra.rf = ix->rf;
+            ra.bits = ix->bits;
+            ra.tables = prep.tables;
+            ra.qab = prep.qab;
+            ra.qrot = prep.qrot;
+            ra.probe_s = ps;
+            ra.probe_c = pc;
+            ra.probe_n = pn;
+            ra.probe_stride = pstride;
+            ra.cand = (int)p->cand;
+            ra.K = (int)K;
+            ra.out_est = ce;
+            ra.out_row = cr;
+            ra.out_nx = cx;
+            ra.out_n = cn;
+            ra.out_stride = (int)p->cand;
+            ra.scanned = cnt;
+            launch_scan_l2_refine(ra, nq, s->st);
+        } else if (nq <= ix->small_max_queries) {
             // A few queries: per-chunk bounds, a threshold per query, appended lists, the merge.
             const uint32_t cand = (uint32_t)a.cand;
             const double worst = (double)pstride * std::max<uint32_t>(ix->max_cluster_blocks, 1);
@@ -854,11 +922,11 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
         copy_out(cand_est, ce, nq * p->cand * 4);
         copy_out(cand_counts, cn, nq * 4);
         // One small copy into pinned memory: the counters and the probe total.
-        unsigned long long* packed = grow<unsigned long long>(s->b[B_STATS], 6);
+        unsigned long long* packed = grow<unsigned long long>(s->b[B_STATS], 8);
         launch_pack_stats(cnt, pn, nq, ovf, packed, s->st);
         if (!s->hstats)
-            SIEVE_HIP(hipHostMalloc((void**)&s->hstats, 6 * sizeof(unsigned long long), hipHostMallocDefault));
-        SIEVE_HIP(hipMemcpyAsync(s->hstats, packed, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
+            SIEVE_HIP(hipHostMalloc((void**)&s->hstats, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+        SIEVE_HIP(hipMemcpyAsync(s->hstats, packed, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
         SIEVE_HIP(hipStreamSynchronize(s->st));
         const unsigned long long* hcnt = s->hstats;
         s->last = sieve_stats{};
@@ -889,7 +957,7 @@ int sieve_index_autotune(sieve_index* ix, sieve_stream* s, const float* q768, co
         const int pstride = (int)std::min<uint32_t>(p->nprobe, ix->k);
         const float* q2 = dev_in(s, B_Q256, q256, nq * PDIM);
         const float* q7 = dev_in(s, B_Q768, q768, nq * DIM);
-        unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 4);
+        unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 8);
         float* ps = grow<float>(s->b[B_PS], nq * pstride);
         uint32_t* pc = grow<uint32_t>(s->b[B_PC], nq * pstride);
         uint32_t* pn = grow<uint32_t>(s->b[B_PN], nq);
@@ -974,6 +1042,39 @@ int sieve_l2_index_set_scan_variant(sieve_l2_index* ix, uint32_t variant) {
         SIEVE_REQUIRE(ix, SIEVE_EINVAL, "null index");
         SIEVE_REQUIRE(variant < (uint32_t)SCAN_VARIANT_CM, SIEVE_EINVAL, "no such scan variant for the L2 index");
         ix->scan_variant = (int)variant;
+    });
+}
+
+int sieve_l2_refine_encode(const float* r, size_t n, uint32_t bits, uint8_t* planes, float* f) {
+    return guarded([&] {
+        SIEVE_REQUIRE(r && planes && f, SIEVE_EINVAL, "null argument");
+        SIEVE_REQUIRE(bits >= 2 && bits <= SIEVE_REFINE_MAX_BITS, SIEVE_EINVAL, "bits must be 2 or 3");
+        if (n == 0) return;
+        hipStream_t st = nullptr;
+        std::vector<void*> tmp;
+        auto dev_copy_in = [&](const float* src) -> const float* {
+            if (on_device(src)) return src;
+            float* d = dmalloc<float>(n * 128, nullptr);
+            tmp.push_back(d);
+            SIEVE_HIP(hipMemcpy(d, src, n * 128 * 4, hipMemcpyHostToDevice));
+            return d;
+        };
+        try {
+            const float* dr = dev_copy_in(r);
+            uint4* dp = on_device(planes) ? reinterpret_cast<uint4*>(planes) : nullptr;
+            float* df = on_device(f) ? f : nullptr;
+            if (!dp) tmp.push_back(dp = dmalloc<uint4>(2 * n, nullptr));
+            if (!df) tmp.push_back(df = dmalloc<float>(n, nullptr));
+            launch_refine_encode(dr, n, (int)bits, dp, df, st);
+            SIEVE_HIP(hipGetLastError());
+            SIEVE_HIP(hipDeviceSynchronize());
+            if ((void*)dp != (void*)planes) SIEVE_HIP(hipMemcpy(planes, dp, n * 32, hipMemcpyDeviceToHost));
+            if (df != f) SIEVE_HIP(hipMemcpy(f, df, n * 4, hipMemcpyDeviceToHost));
+        } catch (...) {
+            for (void* x : tmp) (void)hipFree(x);
+            throw;
+        }
+        for (void* x : tmp) (void)hipFree(x);
     });
 }
 

@@ -25,6 +25,8 @@ constexpr int MAX_PROBE = 1024;
 // Centroids within the BF16 error band of the nprobe-th routing score that one query may
 // rescore in f32.
 constexpr int MAX_RESCORE = 4096;
+// Phases of the cluster-major scan whose MFMA launches are timed with HIP events.
+constexpr uint32_t CM_TIMED_PHASES = 8;
 
 // ------------------------------------------------------------------ errors (host) ---
 void set_error(const std::string& msg);

@@ -334,10 +334,10 @@ FilterRun enqueue_filter_search(const sieve_index* ix, const sieve_filter* f, ui
         view.deleted = lv->deleted;
         view.deleted_words = lv->deleted_words;
     }
-    unsigned long long* cnt = sgrow<unsigned long long>(s, B_CNT, 4);
+    unsigned long long* cnt = sgrow<unsigned long long>(s, B_CNT, 8);
     if (!s->sizing) {
         SIEVE_HIP(hipMemsetAsync(ovf, 0, 4, s->st));
-        SIEVE_HIP(hipMemsetAsync(cnt, 0, 32, s->st));
+        SIEVE_HIP(hipMemsetAsync(cnt, 0, 64, s->st));
     }
     record(s, 0);
     Batch bt;

@@ -160,8 +160,11 @@ struct sieve_l2_index {
     size_t device_bytes = 0;
     int scan_variant = 0;
     uint32_t scan_chunk_blocks = 0;
+    int bits = 1;                  // refinement code width (1: none)
+    uint4* rplanes = nullptr;      // 2 x 16 bytes per slot
+    float* rf = nullptr;           // per slot
     ~sieve_l2_index() {
-        void* ps[] = {leaves, leaves_bf, leaf_norm, block_off, codes, mul, nx, ids, rotation};
+        void* ps[] = {leaves, leaves_bf, leaf_norm, block_off, codes, mul, nx, ids, rotation, rplanes, rf};
         for (void* p : ps)
             if (p) (void)hipFree(p);
         if (base && base_owned) (void)hipFree(base);
@@ -172,7 +175,7 @@ enum Slot {
     B_Q256, B_Q768, B_TABLES, B_QAB, B_WQ768, B_WQ256, B_INV, B_QBF, B_QNORM, B_APPROX, B_PS, B_PC, B_PN, B_CE, B_CR,
     B_CX, B_CN, B_OS, B_OR, B_ON, B_DEL, B_ALLOW, B_AFLAGS, B_CNT, B_OVF, B_TMP1, B_TMP2, B_TMP3, B_QQ, B_QF,
     B_PART, B_PARTN, B_GTHR, B_QDONE, B_CM_LIM, B_CM_SEGC, B_CM_QOFF, B_CM_IOFF, B_CM_Q, B_CM_CENT, B_CM_SEED,
-    B_CM_SEEDN, B_CM_THR, B_CM_EXT, B_CM_EXTN, B_CM_WORK, B_CM_ITEMS, B_TMAX, B_STATS,
+    B_CM_SEEDN, B_CM_THR, B_CM_EXT, B_CM_EXTN, B_CM_WORK, B_CM_ITEMS, B_TMAX, B_STATS, B_QROT,
     B_NSLOTS
 };
 
@@ -183,6 +186,7 @@ struct GraphEntry {
     std::string key;
     hipGraphExec_t exec = nullptr;
     uint64_t gen = 0;
+    uint32_t cm_timed = 0;  // the stream's cm_timed for this call (a replay runs no host code)
     bool failed = false;  // capture failed: this call runs eagerly
 };
 }  // namespace sieve
@@ -195,7 +199,10 @@ struct sieve_stream {
     // each scratch slot would need and enqueue nothing, so a caller can reserve a worst case.
     size_t* sizing = nullptr;
     uint8_t* sizing_zeroed = nullptr;  // sizing mode: slots whose buffers must start zeroed
-    hipEvent_t ev[8] = {};
+    // ev[0..3]: around route, scan and rerank; ev[4 + 2 ph], ev[5 + 2 ph]: around phase ph's
+    // cluster-major MFMA launch (cm_timed of them in the last search).
+    hipEvent_t ev[4 + 2 * sieve::CM_TIMED_PHASES] = {};
+    uint32_t cm_timed = 0;
     sieve_stats last{};
     unsigned long long* hstats = nullptr;  // pinned: the packed counters of the last search
     bool graphs = false;               // sieve_stream_set_graphs
@@ -425,6 +432,7 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
                 const uint32_t* pc, const uint32_t* pn, int pstride, const sieve_params* p, const ViewArgs& view,
                 float* ce, uint32_t* cr, float* cx, uint32_t* cn, int cstride, unsigned long long* scanned,
                 const LiveArgs* lv = nullptr, uint32_t min_chunk_blocks = 0) {
+    if (!s->sizing) s->cm_timed = 0;
     ScanArgs a{};
     if (lv) {
         a.dblock_off = lv->dblock_off;
@@ -486,6 +494,10 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
             a.ext = sgrow<uint2>(s, B_CM_EXT, nq * (size_t)a.ext_cap);
             a.ext_n = sgrow_zeroed<uint32_t>(s, B_CM_EXTN, nq);  // the finish pass leaves it zero
             if (s->sizing) return;
+            // HIP events around each phase's MFMA launch (the first CM_TIMED_PHASES phases).
+            s->cm_timed = std::min<uint32_t>(cm.phases, CM_TIMED_PHASES);
+            cm.ev = s->ev[4] ? s->ev + 4 : nullptr;
+            cm.nev = 2 * s->cm_timed;
             launch_scan_cm(a, cm, nq, variant, s->st);
             SIEVE_HIP(hipGetLastError());
             return;
@@ -582,8 +594,8 @@ inline void enqueue_search(const sieve_index* ix, sieve_stream* s, const float* 
         view.deleted_words = lv->deleted_words;
     }
     const int pstride = (int)std::min<uint32_t>(p->nprobe, ix->k);
-    unsigned long long* cnt = sgrow<unsigned long long>(s, B_CNT, 4);
-    sgrow<unsigned long long>(s, B_STATS, 6);  // finish_search's packed counters, sized with the rest
+    unsigned long long* cnt = sgrow<unsigned long long>(s, B_CNT, 8);
+    sgrow<unsigned long long>(s, B_STATS, 8);  // finish_search's packed counters, sized with the rest
     float* ps = sgrow<float>(s, B_PS, nq * pstride);
     uint32_t* pc = sgrow<uint32_t>(s, B_PC, nq * pstride);
     uint32_t* pn = sgrow<uint32_t>(s, B_PN, nq);
@@ -591,7 +603,7 @@ inline void enqueue_search(const sieve_index* ix, sieve_stream* s, const float* 
     uint32_t* cr = sgrow<uint32_t>(s, B_CR, nq * p->cand);
     float* cx = sgrow<float>(s, B_CX, nq * p->cand);
     uint32_t* cn = sgrow<uint32_t>(s, B_CN, nq);
-    if (!s->sizing) launch_zero_words(reinterpret_cast<uint32_t*>(cnt), 8, s->st);  // a kernel: graph-capturable
+    if (!s->sizing) launch_zero_words(reinterpret_cast<uint32_t*>(cnt), 16, s->st);  // a kernel: graph-capturable
     record(s, 0);
     Batch bt;
     prep_queries(ix, s, q2, q7, nq, bt);
@@ -609,10 +621,10 @@ inline void finish_search(sieve_stream* s, size_t nq, sieve_timing* t) {
     // One small copy into pinned memory: the counters, the probe total and the overflow flag.
     const unsigned long long* cnt = static_cast<const unsigned long long*>(s->b[B_CNT].p);
     const uint32_t* pn = static_cast<const uint32_t*>(s->b[B_PN].p);
-    unsigned long long* packed = grow<unsigned long long>(s->b[B_STATS], 6);
+    unsigned long long* packed = grow<unsigned long long>(s->b[B_STATS], 8);
     launch_pack_stats(cnt, pn, nq, static_cast<const uint32_t*>(s->b[B_OVF].p), packed, s->st);
-    if (!s->hstats) SIEVE_HIP(hipHostMalloc((void**)&s->hstats, 6 * sizeof(unsigned long long), hipHostMallocDefault));
-    SIEVE_HIP(hipMemcpyAsync(s->hstats, packed, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
+    if (!s->hstats) SIEVE_HIP(hipHostMalloc((void**)&s->hstats, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+    SIEVE_HIP(hipMemcpyAsync(s->hstats, packed, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
     SIEVE_HIP(hipStreamSynchronize(s->st));
     const unsigned long long* hcnt = s->hstats;
     s->last = sieve_stats{};
@@ -625,6 +637,9 @@ inline void finish_search(sieve_stream* s, size_t nq, sieve_timing* t) {
     s->last.route_ns = elapsed_ns(s->ev[0], s->ev[1]);
     s->last.scan_ns = elapsed_ns(s->ev[1], s->ev[2]);
     s->last.rerank_ns = elapsed_ns(s->ev[2], s->ev[3]);
+    s->last.cm_pairs = hcnt[6];
+    s->last.cm_launches = s->cm_timed;
+    for (uint32_t ph = 0; ph < s->cm_timed; ph++) s->last.cm_ns += elapsed_ns(s->ev[4 + 2 * ph], s->ev[5 + 2 * ph]);
     if (t) {
         t->route += (uint64_t)s->last.route_ns;
         t->scan += (uint64_t)s->last.scan_ns;

@@ -88,7 +88,7 @@ struct RouteArgs {
     float err_coef;            // |approx - exact| <= err_coef * qnorm
 };
 void launch_route_select(const RouteArgs& a, size_t nq, hipStream_t st);
-// out[0..3] = cnt[0..3], out[4] = sum of pn[0, nq), out[5] = *ovf (one workgroup).
+// out[0..3] = cnt[0..3], out[4] = sum of pn[0, nq), out[5] = *ovf, out[6] = cnt[4] (one workgroup).
 void launch_pack_stats(const unsigned long long* cnt, const uint32_t* pn, size_t nq, const uint32_t* ovf,
                        unsigned long long* out, hipStream_t st);
 // p[0, n) = 0 by a kernel (a memset node is not replayed reliably in a captured graph).
@@ -122,7 +122,8 @@ struct ScanArgs {
     float* out_err;
     uint32_t* out_n;
     int out_stride;
-    unsigned long long* scanned; // [0] += stored slots scanned, [2] += slots past the threshold
+    unsigned long long* scanned; // [0] += stored slots scanned, [2] += slots past the threshold,
+                                 // [4] += (query, slot) pairs the cluster-major MFMA pass scores
     // Work split: query q's probed blocks (T_q) are cut into clamp(ceil(T_q / chunk_blocks), 1,
     // chunk_max) chunks; workgroup q * chunk_max + c scans chunk c.
     uint32_t chunk_blocks;
@@ -188,6 +189,8 @@ struct CmScratch {
     uint32_t seed_slots;         // slots the seed pass scans per query (whole probes)
     uint32_t chunk_blocks;       // blocks per work item
     uint32_t phases;             // the work list runs in this many launches, thresholds raised between
+    hipEvent_t* ev;              // null, or events recorded around phase ph's MFMA launch: ev[2 ph],
+    uint32_t nev;                // ev[2 ph + 1], for 2 ph + 1 < nev
 };
 // Enqueues the seed pass, the work-list build, the cluster pass and the finish pass. `a` carries
 // the index, batch and output fields and seed / ext buffers; its chunking is ignored.
@@ -247,6 +250,7 @@ struct L2Prep {
     float* qf;         // nq x 100 f32
     float* qq;         // nq: sum q_i^2 (left to right)
     float* qnorm;      // nq: |q|
+    float* qrot;       // nq x 128: the rotated, zero-padded query (refinement tables), or null
 };
 void launch_query_prep_l2(const int8_t* q, size_t nq, const float* signs, float rot_norm, const L2Prep& out,
                           hipStream_t st);
@@ -267,5 +271,37 @@ struct L2RerankArgs {
 };
 size_t rerank_l2_lds_bytes(int cand_stride);
 void launch_rerank_l2(const L2RerankArgs& a, size_t nq, hipStream_t st);
+
+// Multi-bit refinement (refine.hip). planes: 2 x 16 bytes per row (plane 0, plane 1 or zeros).
+void launch_refine_encode(const float* r, size_t n, int bits, uint4* planes, float* f, hipStream_t st);
+// The record's extra section (per slot: bits - 1 planes of 16 B, then f) -> planes, f.
+void launch_unpack_refine(const uint8_t* extra, size_t nslots, int bits, uint4* planes, float* f, hipStream_t st);
+struct L2RefineArgs {
+    const uint4* codes;          // raw 16-byte codes per slot
+    const float* mul;
+    const float* nx;
+    const uint32_t* ids;
+    const uint32_t* block_off;   // k + 1
+    const uint4* planes;         // 2 per slot: the low planes (plane p = bit p of u)
+    const float* rf;             // f per slot
+    int bits;
+    const uint8_t* tables;       // nq x 512
+    const float4* qab;           // nq: (a, b, ...)
+    const float* qrot;           // nq x 128 rotated queries
+    const float* probe_s;        // <q, c> per probe
+    const uint32_t* probe_c;
+    const uint32_t* probe_n;
+    int probe_stride;
+    int cand;
+    int K;                       // shortlist length, max(refine, cand); 0: never cut (no storage)
+    float* out_est;              // nq x out_stride: refined estimates, best first
+    uint32_t* out_row;
+    float* out_nx;
+    uint32_t* out_n;
+    int out_stride;
+    unsigned long long* scanned;
+};
+size_t refine_lds_bytes(int K, int cand, int nprobe);
+void launch_scan_l2_refine(const L2RefineArgs& a, size_t nq, hipStream_t st);
 
 }  // namespace sieve

@@ -1,5 +1,5 @@
 // The 1B-scale L2 path over 100-d int8 vectors: bigann::Index::{route, candidates, rerank}
-// (src/bigann.rs:652-747), without the multi-bit refinement.
+// (src/bigann.rs:652-747); the multi-bit refinement of the candidates is in refine.hip.
 //
 //   prep:   q as f32 zero-padded to 128 dims, Rotation (128-d, src/rabitq.rs:59-69), the
 //           32-group LUT and (a, b) (rabitq::Query::<32>::new); the routing operand
@@ -69,6 +69,8 @@ __global__ __launch_bounds__(64) void k_query_prep_l2(const int8_t* __restrict__
     }
     for (int i = lane; i < L2_PD; i += WAVE) v[i] = v[i] * rot_norm;
     __syncthreads();
+    if (out.qrot)
+        for (int i = lane; i < L2_PD; i += WAVE) out.qrot[qi * L2_PD + i] = v[i];
     const int g = lane;
     float mn = FLT_MAX, mx = -FLT_MAX;
     float f[16];

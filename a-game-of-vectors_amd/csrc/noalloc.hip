@@ -68,7 +68,7 @@ struct BorrowedStream {
     BorrowedStream(const sieve_stream* s) {
         t.st = s->st;
         t.device = s->device;
-        for (int e = 0; e < 8; e++) t.ev[e] = s->ev[e];
+        for (size_t e = 0; e < sizeof t.ev / sizeof t.ev[0]; e++) t.ev[e] = s->ev[e];
     }
     ~BorrowedStream() {
         for (auto& b : t.b) {

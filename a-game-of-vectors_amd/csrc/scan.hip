@@ -688,6 +688,7 @@ struct CmArgs {
     uint32_t ext_cap;
     uint32_t* work;              // phases counters
     unsigned long long* scanned;
+    unsigned long long* pairs;   // (query, slot) pairs scored, the MFMA pass's own count
     uint32_t phase, phases;      // this launch runs work items [total * phase / phases, ...)
 };
 
@@ -720,6 +721,8 @@ __global__ __launch_bounds__(256) void k_pack_stats(const unsigned long long* cn
     if (threadIdx.x == 0) {
         out[4] = part[0] + part[1] + part[2] + part[3];
         out[5] = *ovf;
+        out[6] = cnt[4];
+        out[7] = 0;
     }
 }
 
@@ -965,7 +968,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 else B1[st] = v;
             }
         }
-        if (lane == 0 && a.scanned) atomicAdd(a.scanned, (unsigned long long)item.y * nvalid);
+        if (lane == 0 && a.scanned) {
+            atomicAdd(a.scanned, (unsigned long long)item.y * nvalid);
+            atomicAdd(a.pairs, (unsigned long long)item.y * nvalid);
+        }
         if (lane < 16) {
             stage_q[wave][lane] = col[0].q;
             stage_q[wave][lane + 16] = col[1].q;
@@ -1139,6 +1145,7 @@ void launch_scan_cm(const ScanArgs& a_in, const CmScratch& cm_in, size_t nq, int
     c.ext_cap = a.ext_cap;
     c.work = cm.work;
     c.scanned = a.scanned;
+    c.pairs = a.scanned ? a.scanned + 4 : nullptr;
     c.phases = cm.phases;
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
@@ -1151,10 +1158,13 @@ void launch_scan_cm(const ScanArgs& a_in, const CmScratch& cm_in, size_t nq, int
     // and its threshold rises, so later phases append less.
     for (uint32_t ph = 0; ph < cm.phases; ph++) {
         c.phase = ph;
+        const bool timed = cm.ev && 2 * ph + 1 < cm.nev;
+        if (timed) (void)hipEventRecord(cm.ev[2 * ph], st);
         if (variant == SCAN_VARIANT_CM)
             hipLaunchKernelGGL(k_scan_cm<CM_OH_LDS>, dim3((unsigned)cus * 4), dim3(256), 0, st, c);
         else
             hipLaunchKernelGGL(k_scan_cm<CM_OH_MIXED>, dim3((unsigned)cus * 4), dim3(256), 0, st, c);
+        if (timed) (void)hipEventRecord(cm.ev[2 * ph + 1], st);
         if (ph + 1 < cm.phases)
             hipLaunchKernelGGL((k_scan_t<64, false, 2, false, 0, SCAN_TIGHTEN>), dim3((unsigned)nq),
                                dim3(SCAN_THREADS), lds, st, a);

@@ -58,7 +58,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("queries", ctypes.c_uint64), ("probed", ctypes.c_uint64), ("scanned", ctypes.c_uint64),
                 ("id_reads", ctypes.c_uint64), ("reranked", ctypes.c_uint64), ("route_ns", ctypes.c_double),
                 ("scan_ns", ctypes.c_double), ("rerank_ns", ctypes.c_double), ("rescanned", ctypes.c_uint64),
-                ("graph", ctypes.c_uint64)]
+                ("graph", ctypes.c_uint64), ("cm_pairs", ctypes.c_uint64), ("cm_ns", ctypes.c_double),
+                ("cm_launches", ctypes.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -79,7 +80,8 @@ class L2Desc(ctypes.Structure):
     _fields_ = [("k", ctypes.c_uint32), ("n", ctypes.c_uint64), ("nblocks", ctypes.c_uint64),
                 ("leaves", ctypes.c_void_p), ("block_off", ctypes.c_void_p), ("blocks", ctypes.c_void_p),
                 ("ids", ctypes.c_void_p), ("factors", ctypes.c_void_p), ("rotation", ctypes.c_void_p),
-                ("base", ctypes.c_void_p), ("borrow_base", ctypes.c_uint32)]
+                ("base", ctypes.c_void_p), ("borrow_base", ctypes.c_uint32), ("refine_bits", ctypes.c_uint32),
+                ("refine", ctypes.c_void_p)]
 
 
 class LiveOpts(ctypes.Structure):
@@ -178,6 +180,7 @@ def lib():
         "sieve_server_destroy": (None, [vp]),
         "sieve_l2_index_create": (i32, [ctypes.POINTER(L2Desc), ctypes.POINTER(vp)]),
         "sieve_l2_index_free": (None, [vp]),
+        "sieve_l2_refine_encode": (i32, [vp, sz, u32, vp, vp]),
         "sieve_l2_search_batch": (i32, [vp, vp, vp, sz, ctypes.POINTER(L2Params), vp, vp, vp, vp, vp, vp, vp, vp,
                                         T]),
         "sieve_filter_create": (i32, [vp, vp, ctypes.c_uint64, ctypes.POINTER(vp)]),

@@ -350,15 +350,21 @@ class L2Index:
     """The L2 index over 100-d int8 vectors (bigann::Index, src/bigann.rs:604-747)."""
 
     def __init__(self, arrays, borrow_base=False):
+        """arrays: leaves, block_off, blocks, ids, factors, rotation, base; optionally refine_bits and
+        refine (the per-slot extra section of a multi-bit index, src/bigann.rs:430-436)."""
         keep = {}
         d = abi.L2Desc()
         dts = dict(leaves=np.float32, block_off=np.uint64, blocks=np.uint8, ids=np.uint32, factors=np.float32,
                    rotation=np.float32, base=np.int8)
+        if arrays.get("refine") is not None and int(arrays.get("refine_bits", 1)) > 1:
+            dts["refine"] = np.uint8
+            d.refine_bits = int(arrays["refine_bits"])
         for name, dt in dts.items():
             a = arrays[name]
             a = a.contiguous() if _is_torch(a) else np.ascontiguousarray(a, dtype=dt)
             keep[name] = a
             setattr(d, name, abi.ptr(a).value)
+        self.bits = max(1, int(d.refine_bits))
         d.k = _numel(keep["leaves"]) // 100
         d.n = _numel(keep["base"]) // 100
         d.nblocks = _numel(keep["ids"]) // 32
@@ -390,9 +396,10 @@ class L2Index:
         """Largest batch for the small-batch scan (sieve_l2_index_set_small_batch); 0 disables it."""
         abi.check(abi.lib().sieve_l2_index_set_small_batch(self._h, max_queries))
 
-    def search(self, q, nprobe, cand, k, margin=0.0, timing=None, stream=None, details=False):
+    def search(self, q, nprobe, cand, k, margin=0.0, timing=None, stream=None, details=False, refine=0):
         """bigann::Index::search for a batch of int8 queries (nq x 100): (scores, rows, counts)
-        and, with details, (probes, probe_counts, cand_rows, cand_est, cand_counts)."""
+        and, with details, (probes, probe_counts, cand_rows, cand_est, cand_counts). refine > 0 on
+        a multi-bit index re-estimates the 1-bit shortlist of max(refine, cand) slots."""
         q = q if _is_torch(q) else np.ascontiguousarray(q, dtype=np.int8)
         nq = q.shape[0]
         out = Index._out
@@ -404,7 +411,7 @@ class L2Index:
         if self._stream is None:
             self._stream = Stream()
         st = stream or self._stream
-        p = abi.L2Params(nprobe=nprobe, cand=cand, k=k, refine=0, margin=margin)
+        p = abi.L2Params(nprobe=nprobe, cand=cand, k=k, refine=refine, margin=margin)
         abi.check(abi.lib().sieve_l2_search_batch(self._h, st._h, abi.ptr(q), nq, ctypes.byref(p), abi.ptr(s),
                                                   abi.ptr(r), abi.ptr(n), *[abi.ptr(x) for x in extra],
                                                   ctypes.byref(t)))

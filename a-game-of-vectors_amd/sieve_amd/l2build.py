@@ -115,6 +115,43 @@ def build_l2(base, k, iters=8, seed=7, chunk=1 << 18):
                     factors=factors, rotation=signs, base=base)
 
 
+def refine_section(arr, bits, chunk=1 << 22):
+    """The per-slot extra section of a multi-bit index (src/bigann.rs:331-345, 430-436): for every
+    slot, refine_encode (src/bigann.rs:70-108, on the device through sieve_l2_refine_encode) of
+    its rotated residual, the row padded and rotated minus its leaf padded and rotated; stored
+    as the low bits - 1 planes (16 bytes each) followed by the scale f as a little-endian f32,
+    refine_bytes(bits) bytes per slot in slot order, zero for padding slots."""
+    from . import abi
+    assert bits in (2, 3)
+    dev = arr.base.device
+    ids = arr.ids.to(torch.int64)
+    S = ids.shape[0]
+    rb = (bits - 1) * 16 + 4
+    out = torch.zeros(S, rb, dtype=torch.uint8, device=dev)
+    k = arr.leaves.shape[0]
+    nb = torch.diff(arr.block_off.to(torch.int64))
+    leaf_of_slot = torch.repeat_interleave(torch.arange(k, device=dev), nb * BLOCK)
+    leaves_rot = rotate(pad(arr.leaves.reshape(k, D).to(torch.float32)), arr.rotation)
+    for s0 in range(0, S, chunk):
+        sid = ids[s0:s0 + chunk]
+        at = torch.nonzero((sid >= 0) & (sid != PAD)).squeeze(1)
+        if at.numel() == 0:
+            continue
+        x = arr.base[sid[at]].to(torch.float32)
+        r = (rotate(pad(x), arr.rotation) - leaves_rot[leaf_of_slot[s0 + at]]).contiguous()
+        m = r.shape[0]
+        on_dev = r.is_cuda
+        planes = torch.empty(m, 32, dtype=torch.uint8, device=dev)
+        f = torch.empty(m, dtype=torch.float32, device=dev)
+        if on_dev:
+            abi.check(abi.lib().sieve_l2_refine_encode(abi.ptr(r), m, bits, abi.ptr(planes), abi.ptr(f)))
+        else:
+            rn, pn, fn = r.numpy(), planes.numpy(), f.numpy()
+            abi.check(abi.lib().sieve_l2_refine_encode(abi.ptr(rn), m, bits, abi.ptr(pn), abi.ptr(fn)))
+        out[s0 + at] = torch.cat([planes[:, :(bits - 1) * 16], f.view(torch.uint8).reshape(m, 4)], 1)
+    return out
+
+
 class L2Mixture:
     """The 100-d int8 mixture of the 1B-row L2 workload at any size: component means N(0, 40^2) per
     dimension; each component holds `topics` sub-centres at N(0, spread^2) around its mean, and
@@ -286,4 +323,5 @@ def exact_l2_topk(base, q, k=32, chunk=1 << 22):
     return best_i, best_s
 
 
-__all__ = ["L2Arrays", "L2Mixture", "int8_mixture", "build_l2", "build_l2_large", "exact_l2_topk", "f32_div"]
+__all__ = ["L2Arrays", "L2Mixture", "int8_mixture", "build_l2", "build_l2_large", "exact_l2_topk", "f32_div",
+           "refine_section"]

@@ -77,14 +77,17 @@ class Server:
             pass
 
 
-def open_loop(server, q768, q256, rate, seconds, seed=0):
+def open_loop(server, q768, q256, rate, seconds, seed=0, truth=None):
     """Submits queries (cycling through the given ones) at Poisson arrivals of `rate`/s for
-    `seconds`, waits for every result, and returns latency percentiles and throughput."""
+    `seconds`, waits for every result, and returns latency percentiles and throughput. With
+    `truth` (exact top-10 row lists of the first len(truth) queries), also the mean R@10 of the
+    results served for those queries (eval::recall_at, src/eval.rs:7-10)."""
+    from .evaluate import recall_at
     rng = np.random.default_rng(seed)
     tickets = []
     lock = threading.Lock()
     done = threading.Event()
-    lat, batch = [], []
+    lat, batch, rec = [], [], []
 
     def collect():
         i = 0
@@ -100,9 +103,12 @@ def open_loop(server, q768, q256, rate, seconds, seed=0):
                     continue
                 time.sleep(0.0002)
                 continue
-            _, _, info = server.wait(t)
+            _, rows, info = server.wait(t)
             lat.append(info.total_ms)
             batch.append(info.batch_size)
+            j = i % len(q768)
+            if truth is not None and j < len(truth):
+                rec.append(recall_at(10, rows, truth[j]))
             i += 1
 
     th = threading.Thread(target=collect)
@@ -127,6 +133,10 @@ def open_loop(server, q768, q256, rate, seconds, seed=0):
     th.join()
     wall = time.perf_counter() - t0
     lat.sort()
-    return dict(offered_qps=rate, achieved_qps=round(len(lat) / wall, 1), requests=len(lat),
-                p50_ms=round(pct(lat, 0.5), 3), p99_ms=round(pct(lat, 0.99), 3),
-                mean_batch=round(float(np.mean(batch)), 1))
+    out = dict(offered_qps=rate, achieved_qps=round(len(lat) / wall, 1), requests=len(lat),
+               p50_ms=round(pct(lat, 0.5), 3), p99_ms=round(pct(lat, 0.99), 3),
+               mean_batch=round(float(np.mean(batch)), 1))
+    if rec:
+        out["recall_at_10"] = round(float(np.mean(rec)), 4)
+        out["recall_requests"] = len(rec)
+    return out

@@ -150,6 +150,9 @@ typedef struct sieve_stats {
     double rerank_ns;
     uint64_t rescanned;  /* cluster-major scan: queries whose appended list overflowed       */
     uint64_t graph;      /* sieve_search_batch on a stream with graphs: 0 eager, 1 captured, 2 replayed */
+    uint64_t cm_pairs;   /* cluster-major scan: (query, slot) pairs its MFMA pass scored          */
+    double cm_ns;        /* GPU time of the MFMA pass's launches (HIP events around each)         */
+    uint64_t cm_launches; /* MFMA pass launches timed (one per work-list phase)                   */
 } sieve_stats;
 
 typedef struct sieve_index sieve_index;
@@ -360,8 +363,10 @@ void sieve_server_destroy(sieve_server* s);
 
 /* --------------------------------------------- L2 over 100-d int8 vectors (1B scale) --- */
 
-/* bigann::Params (src/bigann.rs:610-622). refine must be 0: the multi-bit refinement
- * (src/bigann.rs:565-602) is not part of this library; margin <= 0 is off. */
+/* bigann::Params (src/bigann.rs:610-622). refine: on an index with refinement bits > 1, the
+ * 1-bit shortlist of max(refine, cand) slots whose members are re-estimated from all planes
+ * (src/bigann.rs:683-729); 0, or an index without refinement, ranks by the 1-bit estimate alone.
+ * margin <= 0 is off. */
 typedef struct sieve_l2_params {
     uint32_t nprobe;
     uint32_t cand;
@@ -392,7 +397,17 @@ typedef struct sieve_l2_desc {
     const float* rotation;
     const int8_t* base;
     uint32_t borrow_base;
+    /* Multi-bit refinement (src/bigann.rs:38-113; bits.u64): refine_bits 0 or 1 = none, 2 or 3
+     * = the record's per-slot extra section in `refine`, nblocks x 32 x SIEVE_REFINE_BYTES(bits)
+     * bytes in slot order: the low bits - 1 code planes (16 B each, bit i = dimension i), then
+     * the scale f as a little-endian f32 (src/postings.rs:70-76, src/bigann.rs:430-436). */
+    uint32_t refine_bits;
+    const uint8_t* refine;
 } sieve_l2_desc;
+
+/* refine_bytes (src/bigann.rs:110-113): bytes per slot of the refinement section. */
+#define SIEVE_REFINE_BYTES(bits) ((bits) > 1 ? ((bits) - 1) * 16 + 4 : 0)
+#define SIEVE_REFINE_MAX_BITS 3
 
 typedef struct sieve_l2_index sieve_l2_index;
 
@@ -400,6 +415,12 @@ int sieve_l2_index_create(const sieve_l2_desc* desc, sieve_l2_index** out);
 void sieve_l2_index_free(sieve_l2_index* ix);
 /* As sieve_index_set_scan_chunk, for the L2 index. */
 int sieve_l2_index_set_scan_chunk(sieve_l2_index* ix, uint32_t chunk_blocks);
+/* refine_encode (src/bigann.rs:70-108) for n rotated, 128-dim residuals r (n x 128 f32, the
+ * rotated padded row minus its rotated leaf, host or device): the low bits - 1 planes of each
+ * residual's bits-bit code into planes (n x 32 bytes: plane 0, then plane 1 or zeros; bit i of
+ * a plane = dimension i) and the scale into f (n f32). Bit-exact with the reference's f32 steps;
+ * bits is 2 or 3. */
+int sieve_l2_refine_encode(const float* r, size_t n, uint32_t bits, uint8_t* planes, float* f);
 /* As sieve_index_set_scan_variant, for the L2 index: one of the per-query scan variants
  * (0 .. 9; the cluster-major scan serves the inner-product index only). */
 int sieve_l2_index_set_scan_variant(sieve_l2_index* ix, uint32_t variant);

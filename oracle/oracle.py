@@ -145,6 +145,13 @@ def lib():
             "or_l2_create": (vp, [u32, vp, vp, vp, vp, vp, vp, vp, vp]),
             "or_l2_free": (None, [vp]),
             "or_l2_search": (sz, [vp, vp, u32, u32, u32, f32, vp, vp, vp, vp, vp, vp, vp]),
+            "or_l2_search_batch": (None, [vp, vp, sz, u32, u32, u32, f32, u32, i32, vp, vp, vp]),
+            "or_l2_search_latency": (None, [vp, vp, sz, u32, u32, u32, f32, u32, vp]),
+            "or_l2_search_refine": (sz, [vp, vp, u32, u32, u32, f32, u32, vp, vp, vp, vp, vp, vp, vp]),
+            "or_l2_set_refine": (None, [vp, i32, vp]),
+            "or_refine_encode": (None, [vp, i32, vp, vp]),
+            "or_refine_estimate": (f32, [vp, i32, vp, vp, f32]),
+            "or_l2_hot_leaves": (sz, [vp, vp, sz, u32, f32, vp, ctypes.c_uint64, vp]),
             "or_live_create": (vp, [vp, i32, f32]),
             "or_delta_list_fixture": (sz, [sz, vp, vp, vp, sz, vp, vp, vp]),
             "or_filter_create": (vp, [vp, vp, sz]),
@@ -520,7 +527,38 @@ class L2Index:
             pass
         self._h = None
 
-    def search(self, q, nprobe, cand, k, margin=0.0):
+    def set_refine(self, bits, extra):
+        """The multi-bit refinement section (src/bigann.rs:38-113): `bits` and, per slot in slot
+        order, refine_bytes(bits) bytes (the low bits - 1 planes, then f as LE f32)."""
+        self._extra = _c(extra, np.uint8)
+        lib().or_l2_set_refine(self._h, int(bits), _p(self._extra))
+
+    def search_batch(self, q, nprobe, cand, k, margin=0.0, threads=1, refine=0):
+        """or_l2_search over a batch (nq x 100), whole queries per worker: (scores, rows, counts)."""
+        q = _c(q, np.int8)
+        nq = q.shape[0]
+        s = np.zeros((nq, k), np.float32)
+        r = np.full((nq, k), PAD, np.uint32)
+        cnt = np.zeros(nq, np.uint32)
+        lib().or_l2_search_batch(self._h, _p(q), nq, nprobe, cand, k, margin, refine, threads, _p(s), _p(r), _p(cnt))
+        return s, r, cnt
+
+    def search_latency(self, q, nprobe, cand, k, margin=0.0, refine=0):
+        """Single-thread latency of each query, ns."""
+        q = _c(q, np.int8)
+        lat = np.zeros(q.shape[0], np.uint64)
+        lib().or_l2_search_latency(self._h, _p(q), q.shape[0], nprobe, cand, k, margin, refine, _p(lat))
+        return lat
+
+    def hot_leaves(self, q, nprobe, margin, record_bytes, budget):
+        """hot_leaves (src/bigann.rs:537-563): leaf ids, hottest per byte first, within budget."""
+        q = _c(q, np.int8)
+        rb = _c(record_bytes, np.uint64)
+        out = np.zeros(self.k, np.uint32)
+        n = lib().or_l2_hot_leaves(self._h, _p(q), q.shape[0], nprobe, margin, _p(rb), budget, _p(out))
+        return out[:n]
+
+    def search(self, q, nprobe, cand, k, margin=0.0, refine=0):
         q = _c(q, np.int8)
         probe = np.empty(max(nprobe, 1), np.uint32)
         npo = np.zeros(1, np.uint32)
@@ -529,9 +567,29 @@ class L2Index:
         nc = np.zeros(1, np.uint32)
         os_ = np.empty(max(k, 1), np.float32)
         orr = np.empty(max(k, 1), np.uint32)
-        n = lib().or_l2_search(self._h, _p(q), nprobe, cand, k, margin, _p(probe), _p(npo), _p(ci), _p(ce), _p(nc),
-                               _p(os_), _p(orr))
+        n = lib().or_l2_search_refine(self._h, _p(q), nprobe, cand, k, margin, refine, _p(probe), _p(npo), _p(ci),
+                                      _p(ce), _p(nc), _p(os_), _p(orr))
         return (probe[:npo[0]], ci[:nc[0]], ce[:nc[0]], os_[:n], orr[:n])
+
+
+def refine_bytes(bits):
+    """Bytes per slot of the refinement section (src/bigann.rs:110-113)."""
+    return (bits - 1) * 16 + 4 if bits > 1 else 0
+
+
+def refine_encode(r, bits):
+    """refine_encode (src/bigann.rs:70-108) of one rotated 128-d residual: (planes 32 B, f)."""
+    r = _c(r, np.float32)
+    planes = np.zeros(32, np.uint8)
+    f = np.zeros(1, np.float32)
+    lib().or_refine_encode(_p(r), int(bits), _p(planes), _p(f))
+    return planes, float(f[0])
+
+
+def refine_estimate(qr, bits, sign, planes, f):
+    """Refiner::new(qr, bits).estimate(sign, planes, f) (src/bigann.rs:565-602)."""
+    qr, sign, planes = _c(qr, np.float32), _c(sign, np.uint8), _c(planes, np.uint8)
+    return float(lib().or_refine_estimate(_p(qr), int(bits), _p(sign), _p(planes), np.float32(f)))
 
 
 def delta_list_fixture(codes, factors, ids):

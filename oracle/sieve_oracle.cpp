@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -1013,6 +1014,89 @@ struct L2Index {
     const uint32_t* ids;
     const float* rotation;
     const int8_t* base;
+    // Multi-bit refinement (src/bigann.rs:38-113): `bits` (1 = none) and, per slot in slot order,
+    // refine_bytes(bits) bytes: the low bits - 1 planes of the code, then the scale f (LE f32).
+    int bits = 1;
+    const uint8_t* extra = nullptr;
+};
+
+// ---------------------------------------------------------------- refinement ----------
+constexpr int RF_PD = 128, RF_CODE = RF_PD / 8, RF_MAX_BITS = 3;
+
+// refine_bytes, src/bigann.rs:110-113.
+size_t refine_bytes(int bits) { return bits > 1 ? (size_t)(bits - 1) * RF_CODE + 4 : 0; }
+
+// refine_encode, src/bigann.rs:70-108: the uniform `bits`-bit code u of the rotated residual r
+// (levels u - h, h = (2^bits - 1) / 2) at the step of 24 tried that maximises cos(r, u - h), its
+// top bit forced to the sign; writes the low bits - 1 planes (plane p = bit p of u, dim i at
+// byte i / 8, bit i % 8) and returns f = |r|^2 / <r, u - h> (0 when that is not positive).
+float refine_encode(const float* r, int bits, uint8_t* planes) {
+    std::memset(planes, 0, (RF_MAX_BITS - 1) * RF_CODE);
+    const uint32_t top = 1u << (bits - 1);
+    const float h = (float)(2 * top - 1) / 2.0f;
+    float amax = 0.0f;
+    for (int i = 0; i < RF_PD; i++) amax = std::fmax(amax, std::fabs(r[i]));
+    if (amax < 1e-12f) return 0.0f;
+    auto level = [&](float v, float delta) -> uint32_t {
+        float x = std::floor(v / delta + h + 0.5f);
+        x = x < 0.0f ? 0.0f : (x > (float)(2 * top - 1) ? (float)(2 * top - 1) : x);  // f32::clamp
+        const uint32_t u = x != x ? 0u : (uint32_t)x;                                  // `as u32`
+        return v > 0.0f ? (u | top) : (u & (top - 1));
+    };
+    float best = -std::numeric_limits<float>::max(), best_delta = amax / h;
+    for (int st = 0; st < 24; st++) {
+        const float delta = amax / h * (0.3f + 0.035f * (float)st);
+        float dot = 0.0f, n2 = 0.0f;
+        for (int i = 0; i < RF_PD; i++) {
+            const float w = (float)level(r[i], delta) - h;
+            dot = dot + r[i] * w;
+            n2 = n2 + w * w;
+        }
+        const float cosv = dot / std::sqrt(n2);
+        if (cosv > best) {
+            best = cosv;
+            best_delta = delta;
+        }
+    }
+    float rr = 0.0f, dot = 0.0f;
+    for (int i = 0; i < RF_PD; i++) {
+        const uint32_t u = level(r[i], best_delta);
+        for (int p = 0; p < bits - 1; p++) planes[p * RF_CODE + (i >> 3)] |= (uint8_t)(((u >> p) & 1u) << (i & 7));
+        rr = rr + r[i] * r[i];
+        dot = dot + r[i] * ((float)u - h);
+    }
+    return dot > 0.0f ? rr / dot : 0.0f;
+}
+
+// Refiner, src/bigann.rs:565-602: per code byte k and byte value v, the sum of the rotated query
+// over the dims whose bits are set in v (t[v] = t[v & (v - 1)] + q[8 k + lowest set bit]), and
+// off = h * sum(q).
+struct Refiner {
+    std::vector<float> tab;  // RF_CODE x 256
+    float off = 0.0f;
+    int bits = 1;
+    Refiner(const float* qr, int bits_) : tab((size_t)RF_CODE * 256, 0.0f), bits(bits_) {
+        for (int k = 0; k < RF_CODE; k++) {
+            float* t = tab.data() + k * 256;
+            t[0] = 0.0f;
+            for (int v = 1; v < 256; v++) t[v] = t[v & (v - 1)] + qr[8 * k + __builtin_ctz((unsigned)v)];
+        }
+        const float h = (float)((1u << bits) - 1) / 2.0f;
+        float sum = rsum_seed();
+        for (int i = 0; i < RF_PD; i++) sum = sum + qr[i];
+        off = h * sum;
+    }
+    float plane(const uint8_t* p) const {
+        float s = rsum_seed();
+        for (int k = 0; k < RF_CODE; k++) s = s + tab[(size_t)k * 256 + p[k]];
+        return s;
+    }
+    // <q, r> ~ f * (<q, u> - h * sum q), u = 2^(bits - 1) sign + sum_p 2^p plane_p.
+    float estimate(const uint8_t* sign, const uint8_t* planes, float f) const {
+        float u = (float)(1u << (bits - 1)) * plane(sign);
+        for (int p = 0; p < bits - 1; p++) u = u + (float)(1u << p) * plane(planes + p * RF_CODE);
+        return f * (u - off);
+    }
 };
 
 void or_l2_leaf_norms(const float* leaves, size_t k, float* out) {  // norms, src/bigann.rs:135-137
@@ -1032,9 +1116,23 @@ void or_l2_free(void* h) { delete (L2Index*)h; }
 
 // bigann::Index::search without refinement (src/bigann.rs:652-747): route, candidates, rerank.
 // Writes probes (count returned in *nprobe_out), candidates and the final top-k.
+size_t or_l2_search_refine(const void* h, const int8_t* q, uint32_t nprobe, uint32_t cand, uint32_t k, float margin,
+                           uint32_t refine, uint32_t* probe_out, uint32_t* nprobe_out, uint32_t* cand_ids,
+                           float* cand_est, uint32_t* ncand_out, float* out_s, uint32_t* out_r);
 size_t or_l2_search(const void* h, const int8_t* q, uint32_t nprobe, uint32_t cand, uint32_t k, float margin,
                     uint32_t* probe_out, uint32_t* nprobe_out, uint32_t* cand_ids, float* cand_est,
                     uint32_t* ncand_out, float* out_s, uint32_t* out_r) {
+    return or_l2_search_refine(h, q, nprobe, cand, k, margin, 0, probe_out, nprobe_out, cand_ids, cand_est, ncand_out,
+                               out_s, out_r);
+}
+
+// bigann::Index::search (src/bigann.rs:652-747) with the refinement parameter: on an index with
+// bits > 1 and refine > 0, the 1-bit shortlist of max(refine, cand) gates which slots are
+// re-estimated from all planes (at the moment each is scanned, in probe order, as
+// src/bigann.rs:699-729 does), and the best cand by the refined estimate are reranked.
+size_t or_l2_search_refine(const void* h, const int8_t* q, uint32_t nprobe, uint32_t cand, uint32_t k, float margin,
+                           uint32_t refine, uint32_t* probe_out, uint32_t* nprobe_out, uint32_t* cand_ids,
+                           float* cand_est, uint32_t* ncand_out, float* out_s, uint32_t* out_r) {
     const L2Index& ix = *(const L2Index*)h;
     const int D = 100, PD = 128, G = 32;
     // route, src/bigann.rs:659-678
@@ -1056,15 +1154,27 @@ size_t or_l2_search(const void* h, const int8_t* q, uint32_t nprobe, uint32_t ca
     }
     *nprobe_out = (uint32_t)probe.size();
     for (size_t i = 0; i < probe.size(); i++) probe_out[i] = probe[i].second;
-    // candidates, src/bigann.rs:683-734 (refine off)
+    // candidates, src/bigann.rs:683-734
     std::vector<float> qp(PD, 0.0f);
     for (int i = 0; i < D; i++) qp[i] = (float)q[i];
     QueryState rq;
     query_new(ix.rotation, PD, qp.data(), rq);
+    const bool refining = ix.bits > 1 && refine > 0;
+    std::unique_ptr<Refiner> rf;
+    if (refining) {
+        std::vector<float> qr(qp);
+        rotation_apply(ix.rotation, PD, qr.data());  // pad_rotate, src/bigann.rs:249-254
+        rf.reset(new Refiner(qr.data(), ix.bits));
+    }
+    const size_t extra = refine_bytes(ix.bits);
     std::vector<std::pair<uint32_t, float>> info;  // (id, |x|^2); est lives in the TopK
     std::vector<float> info_est;
-    TopK<float> ct(cand);
+    // Without refinement `ct` is the candidate list; with it, the 1-bit shortlist that gates
+    // which slots are re-estimated, and `best` keeps the top cand by the refined estimate.
+    TopK<float> ct(refining ? std::max(refine, cand) : cand);
+    TopK<float> best(cand);
     std::vector<uint16_t> scores;
+    uint8_t sign[RF_CODE];
     for (auto& pr : probe) {
         uint32_t c = pr.second;
         float cent = dots[c];
@@ -1080,12 +1190,20 @@ size_t or_l2_search(const void* h, const int8_t* q, uint32_t nprobe, uint32_t ca
             float est = 2.0f * (cent + mul * (rq.a * (float)scores[j] + rq.b)) - nx;
             if (!ct.has_thr || est > ct.thr) {
                 ct.push(est, (uint32_t)info.size());
+                if (refining) {
+                    unpack_slot(ix.blocks + (b0 + j / BLOCK) * 512, G, (int)(j % BLOCK), sign);
+                    const uint8_t* e = ix.extra + (b0 * BLOCK + j) * extra;
+                    float fs;
+                    std::memcpy(&fs, e + extra - 4, 4);
+                    est = 2.0f * (cent + rf->estimate(sign, e, fs)) - nx;
+                    best.push(est, (uint32_t)info.size());
+                }
                 info.emplace_back(ids[j], nx);
                 info_est.push_back(est);
             }
         }
     }
-    auto cs = ct.into_sorted();
+    auto cs = refining ? best.into_sorted() : ct.into_sorted();
     *ncand_out = (uint32_t)cs.size();
     // rerank, src/bigann.rs:736-747
     TopK<float> out(k);
@@ -1105,6 +1223,90 @@ size_t or_l2_search(const void* h, const int8_t* q, uint32_t nprobe, uint32_t ca
         out_r[i] = res[i].second;
     }
     return res.size();
+}
+
+// Attaches a multi-bit refinement section (per slot, refine_bytes(bits) bytes) to an L2 index.
+void or_l2_set_refine(void* h, int bits, const uint8_t* extra) {
+    L2Index& ix = *(L2Index*)h;
+    ix.bits = bits;
+    ix.extra = extra;
+}
+
+void or_refine_encode(const float* r, int bits, uint8_t* planes, float* f) { *f = refine_encode(r, bits, planes); }
+
+// Refiner::estimate for one rotated query and one slot (sign code, planes, f).
+float or_refine_estimate(const float* qr, int bits, const uint8_t* sign, const uint8_t* planes, float f) {
+    return Refiner(qr, bits).estimate(sign, planes, f);
+}
+
+// hot_leaves, src/bigann.rs:537-563: the leaves to keep resident under `budget` bytes, hottest
+// per byte first: how often the queries route to a leaf (nprobe, cut by margin) over its record's
+// bytes (record_bytes[c]); leaves no query probes are left out. Returns the count written.
+size_t or_l2_hot_leaves(const void* h, const int8_t* q, size_t nq, uint32_t nprobe, float margin,
+                        const uint64_t* record_bytes, uint64_t budget, uint32_t* out) {
+    const L2Index& ix = *(const L2Index*)h;
+    std::vector<uint32_t> hits(ix.k, 0), probe(std::max<uint32_t>(nprobe, 1)), ci(1);
+    std::vector<float> ce(1), s(10);
+    std::vector<uint32_t> r(10);
+    uint32_t np = 0, nc = 0;
+    for (size_t i = 0; i < nq; i++) {
+        or_l2_search_refine(h, q + i * 100, nprobe, 1, 10, margin, 0, probe.data(), &np, ci.data(), ce.data(), &nc,
+                            s.data(), r.data());
+        for (uint32_t j = 0; j < np; j++) hits[probe[j]]++;
+    }
+    auto heat = [&](uint32_t c) { return (double)hits[c] / (double)std::max<uint64_t>(record_bytes[c], 1); };
+    std::vector<uint32_t> order;
+    for (uint32_t c = 0; c < ix.k; c++)
+        if (hits[c] > 0) order.push_back(c);
+    // slice::sort_by is stable: equal heat keeps leaf order
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return heat(b) < heat(a); });
+    uint64_t used = 0;
+    size_t n = 0;
+    for (uint32_t c : order) {
+        used += record_bytes[c];
+        if (used > budget) break;
+        out[n++] = c;
+    }
+    return n;
+}
+
+// or_l2_search over nq queries (nq x 100 i8), whole queries per worker thread as the reference's
+// search bench runs them (src/bin/search.rs:310-332); top-k into out_s / out_r (nq x k).
+void or_l2_search_batch(const void* h, const int8_t* q, size_t nq, uint32_t nprobe, uint32_t cand, uint32_t k,
+                        float margin, uint32_t refine, int threads, float* out_s, uint32_t* out_r, uint32_t* counts) {
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        std::vector<uint32_t> probe(std::max<uint32_t>(nprobe, 1)), ci(std::max<uint32_t>(cand, 1));
+        std::vector<float> ce(std::max<uint32_t>(cand, 1));
+        uint32_t np = 0, nc = 0;
+        for (;;) {
+            size_t i = next.fetch_add(1);
+            if (i >= nq) break;
+            counts[i] = (uint32_t)or_l2_search_refine(h, q + i * 100, nprobe, cand, k, margin, refine, probe.data(), &np,
+                                                      ci.data(), ce.data(), &nc, out_s + i * k, out_r + i * k);
+        }
+    };
+    if (threads <= 1) {
+        work();
+    } else {
+        std::vector<std::thread> th;
+        for (int w = 0; w < threads; w++) th.emplace_back(work);
+        for (auto& x : th) x.join();
+    }
+}
+
+// Single-thread latency of each L2 query in ns (src/bin/search.rs:730-738).
+void or_l2_search_latency(const void* h, const int8_t* q, size_t nq, uint32_t nprobe, uint32_t cand, uint32_t k,
+                          float margin, uint32_t refine, uint64_t* lat_ns) {
+    std::vector<uint32_t> probe(std::max<uint32_t>(nprobe, 1)), ci(std::max<uint32_t>(cand, 1)), r(k);
+    std::vector<float> ce(std::max<uint32_t>(cand, 1)), s(k);
+    uint32_t np = 0, nc = 0;
+    for (size_t i = 0; i < nq; i++) {
+        auto t0 = std::chrono::steady_clock::now();
+        or_l2_search_refine(h, q + i * 100, nprobe, cand, k, margin, refine, probe.data(), &np, ci.data(), ce.data(), &nc,
+                            s.data(), r.data());
+        lat_ns[i] = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count();
+    }
 }
 
 }  // extern "C"

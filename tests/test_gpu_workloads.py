@@ -337,3 +337,33 @@ def test_c4_routing_shape_matches_oracle(c4shape, nprobe, cand):
     for i in (0, 77):
         s1, r1, n1 = gix.search(qh[i:i + 1], nprobe, cand, 10)
         assert np.array_equal(r1[0, :n1[0]], r[i, :n[i]]) and _flt_eq(s1[0, :n1[0]], s[i, :n[i]])
+
+
+@pytest.mark.timeout(600)
+def test_c4_shape_refined_search_matches_oracle(c4shape):
+    """The reference's best 1B operating point, 2-bit refinement with a 1000-slot shortlist, 100
+    candidates and margin probing (128:100:1000:m0.4), on the 131,072-leaf index."""
+    from sieve_amd.index import L2Index
+    from sieve_amd.l2build import refine_section
+    arr, qh, oix = c4shape["arr"], c4shape["qh"], c4shape["oix"]
+    ext = refine_section(arr, 2)
+    gix = L2Index(dict(leaves=arr.leaves, block_off=arr.block_off, blocks=arr.blocks, ids=arr.ids,
+                       factors=arr.factors, rotation=arr.rotation, base=arr.base, refine_bits=2, refine=ext),
+                  borrow_base=True)
+    oix.set_refine(2, ext.cpu().numpy())
+    s, r, n, probes, pn, cr, ce, cn = gix.search(c4shape["q"], 128, 100, 10, margin=0.4, details=True, refine=1000)
+    s, r, n = s.cpu().numpy(), r.cpu().numpy().view(np.uint32), n.cpu().numpy().view(np.uint32)
+    cr, ce, cn = cr.cpu().numpy().view(np.uint32), ce.cpu().numpy(), cn.cpu().numpy().view(np.uint32)
+    pn = pn.cpu().numpy().view(np.uint32)
+    bad_c = bad_k = 0
+    m = 60
+    for i in range(m):
+        wp, wci, wce, ws, wr = oix.search(qh[i], 128, 100, 10, margin=0.4, refine=1000)
+        assert pn[i] == len(wp)
+        if not _same_up_to_ties(cr[i, :cn[i]], ce[i, :cn[i]], wci, wce):
+            bad_c += 1
+        if not _same_up_to_ties(r[i, :n[i]], s[i, :n[i]], wr, ws):
+            bad_k += 1
+    say(f"c4 shape refined 128:100:1000:m0.4: probes per query {int(pn.min())} .. {int(pn.max())}; of {m} "
+        f"queries, {bad_c} candidate lists and {bad_k} top-10 lists differ")
+    assert bad_c == 0 and bad_k == 0

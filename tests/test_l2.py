@@ -77,6 +77,18 @@ def test_oracle_l2_matches_brute_force(l2small, l2oracle):
         assert r.tolist() == [int(cids[j]) for j in o2]
 
 
+def test_oracle_l2_batch_equals_single_queries(l2small, l2oracle):
+    """The threaded batch form (the CPU baseline's) returns each query's single search exactly."""
+    h, qs = l2small
+    s, r, n = l2oracle.search_batch(qs, 8, 300, 10, threads=4)
+    for i, q in enumerate(qs):
+        _, _, _, ws, wr = l2oracle.search(q, 8, 300, 10)
+        assert n[i] == len(wr) and np.array_equal(r[i, :n[i]], wr)
+        assert np.array_equal(s[i, :n[i]].view(np.uint32), ws.view(np.uint32))
+    lat = l2oracle.search_latency(qs[:5], 8, 300, 10)
+    assert lat.shape == (5,) and np.all(lat > 0)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("nprobe,cand,margin,chunk", [(8, 300, 0.0, 0), (16, 1000, 0.0, 0), (32, 500, 0.3, 0),
                                                       (1, 50, 0.0, 0), (16, 300, 0.0, 2)])
