@@ -150,7 +150,10 @@ sieve_index* create_index(const sieve_index_desc* d) {
     return ix.release();
 }
 
-sieve_l2_index* create_l2_index(const sieve_l2_desc* d) {
+}  // namespace
+
+sieve_l2_index* sieve::create_l2_index(const sieve_l2_desc* d) {
+    using namespace sieve;
     constexpr int D = 100, PD = 128, BB = 512;
     SIEVE_REQUIRE(d, SIEVE_EINVAL, "desc is null");
     SIEVE_REQUIRE(d->k >= 1, SIEVE_EINVAL, "k must be >= 1");
@@ -195,6 +198,7 @@ sieve_l2_index* create_l2_index(const sieve_l2_desc* d) {
     launch_split_leaves(ix->leaves, d->k, ix->leaves_bf, st);
     ix->leaf_norm = upload(ln.data(), d->k, acct);
     ix->block_off = upload(bo32.data(), d->k + 1, acct);
+    ix->hblock_off = bo32;
     ix->codes = dmalloc<uint4>(ix->nslots, acct);
     {
         const uint8_t* src = d->blocks;
@@ -267,37 +271,7 @@ sieve_l2_index* create_l2_index(const sieve_l2_desc* d) {
     return ix.release();
 }
 
-struct Mapped {
-    void* p = nullptr;
-    size_t len = 0;
-    ~Mapped() {
-        if (p && len) munmap(p, len);
-    }
-};
-
-bool map_file(const std::string& path, Mapped& m, bool required) {
-    int fd = ::open(path.c_str(), O_RDONLY);
-    if (fd < 0) {
-        SIEVE_REQUIRE(!required, SIEVE_EIO, "cannot open " + path);
-        return false;
-    }
-    struct stat stt;
-    if (fstat(fd, &stt) != 0) {
-        ::close(fd);
-        throw Error{SIEVE_EIO, "cannot stat " + path};
-    }
-    m.len = (size_t)stt.st_size;
-    if (m.len) {
-        m.p = mmap(nullptr, m.len, PROT_READ, MAP_PRIVATE, fd, 0);
-        if (m.p == MAP_FAILED) {
-            m.p = nullptr;
-            ::close(fd);
-            throw Error{SIEVE_EIO, "cannot map " + path};
-        }
-    }
-    ::close(fd);
-    return true;
-}
+namespace {
 
 }  // namespace
 
@@ -697,6 +671,68 @@ static uint64_t search_graphed(const sieve_index* ix, sieve_stream* s, const flo
 
 }  // namespace sieve
 
+
+namespace sieve {
+// Query prep and routing of nq L2 queries already in device memory (bigann::Index::route,
+// src/bigann.rs:660-678): the probe lists (<q, c> per probe, leaf ids, counts; row stride
+// min(nprobe, k)) in stream scratch, enqueued on s's stream. with_qrot keeps the rotated queries
+// (the refinement tables' operand) in prep.qrot.
+void l2_route(const sieve_l2_index* ix, sieve_stream* s, const int8_t* dq, size_t nq, uint32_t nprobe, float margin,
+              bool with_qrot, L2Prep& prep, float** ps_out, uint32_t** pc_out, uint32_t** pn_out) {
+    constexpr int D = 100;
+    const int pstride = (int)std::min<uint32_t>(nprobe, ix->k);
+    prep.qrot = with_qrot ? grow<float>(s->b[B_QROT], nq * 128) : nullptr;
+    prep.tables = grow<uint8_t>(s->b[B_TABLES], nq * 512);
+    prep.qab = grow<float4>(s->b[B_QAB], nq);
+    prep.qbf = grow<uint16_t>(s->b[B_QBF], nq * PDIM);
+    prep.qf = grow<float>(s->b[B_QF], nq * D);
+    prep.qq = grow<float>(s->b[B_QQ], nq);
+    prep.qnorm = grow<float>(s->b[B_QNORM], nq);
+    float* ps = grow<float>(s->b[B_PS], nq * pstride);
+    uint32_t* pc = grow<uint32_t>(s->b[B_PC], nq * pstride);
+    uint32_t* pn = grow<uint32_t>(s->b[B_PN], nq);
+    // 128^-1.5 as (d as f32).powf(-1.5) (src/rabitq.rs:67)
+    launch_query_prep_l2(dq, nq, ix->rotation, std::pow(128.0f, -1.5f), prep, s->st);
+    const size_t per = std::max<size_t>(1, std::min<size_t>(nq, (size_t(1) << 30) / (4 * (size_t)ix->k)));
+    float* approx = grow<float>(s->b[B_APPROX], per * ix->k);
+    float* tmax = grow<float>(s->b[B_TMAX], per * ((ix->k + 15) / 16));
+    for (size_t q0 = 0; q0 < nq; q0 += per) {
+        size_t m = std::min(per, nq - q0);
+        launch_gemm_bf16_route(prep.qbf + q0 * PDIM, ix->leaves_bf, ix->leaf_norm, approx, tmax, (int)m,
+                               (int)ix->k, ix->k, s->st);
+        RouteArgs ra{};
+        ra.approx = approx;
+        ra.lds = ix->k;
+        ra.tmax = tmax;
+        ra.k = (int)ix->k;
+        ra.cents = ix->leaves;
+        ra.q256 = nullptr;
+        ra.qnorm = prep.qnorm + q0;
+        ra.cl_slots = nullptr;
+        ra.cent_norm_max = ix->cent_norm_max;
+        ra.nprobe = (int)nprobe;
+        ra.margin = margin;
+        ra.budget = 0;
+        ra.out_s = ps + q0 * pstride;
+        ra.out_c = pc + q0 * pstride;
+        ra.out_n = pn + q0;
+        ra.out_stride = pstride;
+        ra.l2 = 1;
+        ra.dims = D;
+        ra.cstride = D;
+        ra.cnorm = ix->leaf_norm;
+        ra.qf = prep.qf + q0 * D;
+        ra.qq = prep.qq + q0;
+        // 2 <q, c> from BF16 hi + lo halves: |err| <= 2 (2^-16 + 256 * 2^-24 * 1.01) |q| |c|
+        ra.err_coef = 2.0f * (1.0f / 65536.0f + 1.01f * 256.0f / 16777216.0f) * ix->cent_norm_max;
+        launch_route_select(ra, m, s->st);
+    }
+    *ps_out = ps;
+    *pc_out = pc;
+    *pn_out = pn;
+}
+}  // namespace sieve
+
 extern "C" {
 
 int sieve_search_batch(const sieve_index* ix, sieve_stream* s, const float* q768, const float* q256, size_t nq,
@@ -760,19 +796,8 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
                           "refine: a shortlist of " + std::to_string(K) + " slots exceeds the scan's LDS (at most " +
                               std::to_string(2 * MAX_CAND) + ")");
         const int8_t* dq = dev_in(s, B_Q768, q, nq * D);
-        L2Prep prep;
-        prep.qrot = refining ? grow<float>(s->b[B_QROT], nq * 128) : nullptr;
-        prep.tables = grow<uint8_t>(s->b[B_TABLES], nq * 512);
-        prep.qab = grow<float4>(s->b[B_QAB], nq);
-        prep.qbf = grow<uint16_t>(s->b[B_QBF], nq * PDIM);
-        prep.qf = grow<float>(s->b[B_QF], nq * D);
-        prep.qq = grow<float>(s->b[B_QQ], nq);
-        prep.qnorm = grow<float>(s->b[B_QNORM], nq);
         unsigned long long* cnt = grow<unsigned long long>(s->b[B_CNT], 8);
         uint32_t* ovf = grow<uint32_t>(s->b[B_OVF], 1);
-        float* ps = grow<float>(s->b[B_PS], nq * pstride);
-        uint32_t* pc = grow<uint32_t>(s->b[B_PC], nq * pstride);
-        uint32_t* pn = grow<uint32_t>(s->b[B_PN], nq);
         float* ce = grow<float>(s->b[B_CE], nq * p->cand);
         uint32_t* cr = grow<uint32_t>(s->b[B_CR], nq * p->cand);
         float* cx = grow<float>(s->b[B_CX], nq * p->cand);
@@ -783,42 +808,10 @@ int sieve_l2_search_batch(const sieve_l2_index* ix, sieve_stream* s, const int8_
         SIEVE_HIP(hipMemsetAsync(cnt, 0, 64, s->st));
         SIEVE_HIP(hipMemsetAsync(ovf, 0, 4, s->st));
         SIEVE_HIP(hipEventRecord(s->ev[0], s->st));
-        // 128^-1.5 as (d as f32).powf(-1.5) (src/rabitq.rs:67)
-        launch_query_prep_l2(dq, nq, ix->rotation, std::pow(128.0f, -1.5f), prep, s->st);
-        const size_t per = std::max<size_t>(1, std::min<size_t>(nq, (size_t(1) << 30) / (4 * (size_t)ix->k)));
-        float* approx = grow<float>(s->b[B_APPROX], per * ix->k);
-        float* tmax = grow<float>(s->b[B_TMAX], per * ((ix->k + 15) / 16));
-        for (size_t q0 = 0; q0 < nq; q0 += per) {
-            size_t m = std::min(per, nq - q0);
-            launch_gemm_bf16_route(prep.qbf + q0 * PDIM, ix->leaves_bf, ix->leaf_norm, approx, tmax, (int)m,
-                                   (int)ix->k, ix->k, s->st);
-            RouteArgs ra{};
-            ra.approx = approx;
-            ra.lds = ix->k;
-            ra.tmax = tmax;
-            ra.k = (int)ix->k;
-            ra.cents = ix->leaves;
-            ra.q256 = nullptr;
-            ra.qnorm = prep.qnorm + q0;
-            ra.cl_slots = nullptr;
-            ra.cent_norm_max = ix->cent_norm_max;
-            ra.nprobe = (int)p->nprobe;
-            ra.margin = p->margin;
-            ra.budget = 0;
-            ra.out_s = ps + q0 * pstride;
-            ra.out_c = pc + q0 * pstride;
-            ra.out_n = pn + q0;
-            ra.out_stride = pstride;
-            ra.l2 = 1;
-            ra.dims = D;
-            ra.cstride = D;
-            ra.cnorm = ix->leaf_norm;
-            ra.qf = prep.qf + q0 * D;
-            ra.qq = prep.qq + q0;
-            // 2 <q, c> from BF16 hi + lo halves: |err| <= 2 (2^-16 + 256 * 2^-24 * 1.01) |q| |c|
-            ra.err_coef = 2.0f * (1.0f / 65536.0f + 1.01f * 256.0f / 16777216.0f) * ix->cent_norm_max;
-            launch_route_select(ra, m, s->st);
-        }
+        L2Prep prep;
+        float* ps;
+        uint32_t *pc, *pn;
+        l2_route(ix, s, dq, nq, p->nprobe, p->margin, refining, prep, &ps, &pc, &pn);
         SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
         // In L2 mode the probe scores are <q, c> (the estimator's `cent`, src/bigann.rs:701).
         ScanArgs a{};

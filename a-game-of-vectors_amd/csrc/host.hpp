@@ -2,6 +2,11 @@
 // device buffers, the index and stream objects, and the batched search pipeline stages.
 #pragma once
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -160,6 +165,7 @@ struct sieve_l2_index {
     size_t device_bytes = 0;
     int scan_variant = 0;
     uint32_t scan_chunk_blocks = 0;
+    std::vector<uint32_t> hblock_off;  // host copy of block_off (record sizes for the hot-leaf policy)
     int bits = 1;                  // refinement code width (1: none)
     uint4* rplanes = nullptr;      // 2 x 16 bytes per slot
     float* rf = nullptr;           // per slot
@@ -670,6 +676,50 @@ void size_filter_search(const sieve_index* ix, sieve_stream* s, size_t nq, const
 
 // Resolves the no-allocation entry point's fixed scratch array on device dev (csrc/noalloc.hip).
 void noalloc_setup(int dev);
+
+// The L2 index from arrays (csrc/api.hip), used by sieve_l2_index_create and the reader of the
+// on-disk form (csrc/l2io.hip).
+sieve_l2_index* create_l2_index(const sieve_l2_desc* d);
+
+// Query prep and routing of nq L2 queries in device memory (csrc/api.hip): probe lists in stream
+// scratch (row stride min(nprobe, k)).
+void l2_route(const sieve_l2_index* ix, sieve_stream* s, const int8_t* dq, size_t nq, uint32_t nprobe, float margin,
+              bool with_qrot, L2Prep& prep, float** ps_out, uint32_t** pc_out, uint32_t** pn_out);
+
+// A read-only memory map of a whole file.
+struct Mapped {
+    void* p = nullptr;
+    size_t len = 0;
+    ~Mapped() {
+        if (p && len) munmap(p, len);
+    }
+};
+
+inline bool map_file(const std::string& path, Mapped& m, bool required) {
+    int fd = ::open(path.c_str(), O_RDONLY);
+    if (fd < 0) {
+        SIEVE_REQUIRE(!required, SIEVE_EIO, "cannot open " + path);
+        return false;
+    }
+    struct stat stt;
+    if (fstat(fd, &stt) != 0) {
+        ::close(fd);
+        throw Error{SIEVE_EIO, "cannot stat " + path};
+    }
+    m.len = (size_t)stt.st_size;
+    if (m.len) {
+        m.p = mmap(nullptr, m.len, PROT_READ, MAP_PRIVATE, fd, 0);
+        if (m.p == MAP_FAILED) {
+            m.p = nullptr;
+            ::close(fd);
+            throw Error{SIEVE_EIO, "cannot map " + path};
+        }
+    }
+    ::close(fd);
+    return true;
+}
+
+
 
 inline void ensure_events(sieve_stream* s) {
     for (auto& e : s->ev)

@@ -181,6 +181,9 @@ def lib():
         "sieve_l2_index_create": (i32, [ctypes.POINTER(L2Desc), ctypes.POINTER(vp)]),
         "sieve_l2_index_free": (None, [vp]),
         "sieve_l2_refine_encode": (i32, [vp, sz, u32, vp, vp]),
+        "sieve_l2_index_open": (i32, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]),
+        "sieve_l2_hot_leaves": (i32, [vp, vp, vp, sz, u32, ctypes.c_float, ctypes.c_uint64, vp,
+                                      ctypes.POINTER(u32)]),
         "sieve_l2_search_batch": (i32, [vp, vp, vp, sz, ctypes.POINTER(L2Params), vp, vp, vp, vp, vp, vp, vp, vp,
                                         T]),
         "sieve_filter_create": (i32, [vp, vp, ctypes.c_uint64, ctypes.POINTER(vp)]),

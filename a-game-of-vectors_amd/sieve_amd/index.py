@@ -376,6 +376,34 @@ class L2Index:
         self.k = d.k
         self._stream = None
 
+    @classmethod
+    def open(cls, index_dir, base_dir):
+        """bigann::Index::open over the reference's files (sieve_l2_index_open)."""
+        self = cls.__new__(cls)
+        h = ctypes.c_void_p()
+        abi.check(abi.lib().sieve_l2_index_open(index_dir.encode(), base_dir.encode(), ctypes.byref(h)))
+        self._h = h
+        self._keep = None
+        self._stream = None
+        import os
+        self.k = os.path.getsize(os.path.join(index_dir, "centroids.f32")) // 400
+        bits_path = os.path.join(index_dir, "bits.u64")
+        self.bits = int(np.fromfile(bits_path, np.uint64)[0]) if os.path.exists(bits_path) else 1
+        return self
+
+    def hot_leaves(self, q, nprobe, margin, budget, stream=None):
+        """hot_leaves (src/bigann.rs:537-563): leaf ids to keep resident under `budget` record
+        bytes, hottest per byte first, for these queries."""
+        q = np.ascontiguousarray(q, dtype=np.int8)
+        out = np.zeros(self.k, np.uint32)
+        n = ctypes.c_uint32()
+        if self._stream is None:
+            self._stream = Stream()
+        st = stream or self._stream
+        abi.check(abi.lib().sieve_l2_hot_leaves(self._h, st._h, abi.ptr(q), q.shape[0], nprobe, margin, budget,
+                                                abi.ptr(out), ctypes.byref(n)))
+        return out[:n.value]
+
     def __del__(self):
         try:
             if getattr(self, "_h", None):

@@ -152,6 +152,63 @@ def refine_section(arr, bits, chunk=1 << 22):
     return out
 
 
+CHUNK_ROWS = 100_000_000  # rows per base.NN file (src/bigann.rs:57-58)
+RECORD_ALIGN = 4096       # postings records (src/postings.rs:38)
+
+
+def write_l2_index(arr, index_dir, base_dir, refine_bits=1, refine=None, coarse=None, k2=None,
+                   chunk_rows=CHUNK_ROWS):
+    """Writes an L2 index in the reference's on-disk form: the Model files (src/bigann.rs:257-266:
+    centroids.f32, coarse.f32, k2.u64, rotation.f32, bits.u64 when refine_bits > 1), the postings
+    (src/bigann.rs:386-460: postings.bin with one 4 KiB-aligned record per leaf, its code blocks,
+    [mul, |x|^2] per slot, ids and the refinement section, then postings.off and block_off.u64),
+    and the base rows as base.NN chunks of chunk_rows rows with base.NN.done markers
+    (src/bigann.rs:465-489). A flat index (no coarse cells given) is written as one cell per
+    leaf: coarse = the leaves, k2 = 1."""
+    import os
+    os.makedirs(index_dir, exist_ok=True)
+    os.makedirs(base_dir, exist_ok=True)
+    h = arr.numpy()
+    k = h["leaves"].reshape(-1, D).shape[0]
+    leaves = np.ascontiguousarray(h["leaves"], np.float32).reshape(-1)
+    coarse = leaves if coarse is None else np.ascontiguousarray(coarse, np.float32).reshape(-1)
+    k2 = 1 if k2 is None else int(k2)
+    leaves.tofile(os.path.join(index_dir, "centroids.f32"))
+    coarse.tofile(os.path.join(index_dir, "coarse.f32"))
+    np.array([k2], np.uint64).tofile(os.path.join(index_dir, "k2.u64"))
+    np.ascontiguousarray(h["rotation"], np.float32).tofile(os.path.join(index_dir, "rotation.f32"))
+    if refine_bits > 1:
+        np.array([refine_bits], np.uint64).tofile(os.path.join(index_dir, "bits.u64"))
+        ext = refine.cpu().numpy() if hasattr(refine, "cpu") else np.asarray(refine, np.uint8)
+        rb = (refine_bits - 1) * 16 + 4
+        ext = ext.reshape(-1, rb)
+    bo = h["block_off"].astype(np.uint64)
+    blocks = h["blocks"].reshape(-1, 512)
+    fac = h["factors"].reshape(-1, 2).astype(np.float32)
+    ids = h["ids"].astype(np.uint32)
+    off = np.zeros(k + 1, np.uint64)
+    pos = 0
+    with open(os.path.join(index_dir, "postings.bin"), "wb") as f:
+        for c in range(k):
+            b0, b1 = int(bo[c]), int(bo[c + 1])
+            off[c] = pos
+            rec = [blocks[b0:b1].tobytes(), fac[b0 * 32:b1 * 32].tobytes(), ids[b0 * 32:b1 * 32].tobytes()]
+            if refine_bits > 1:
+                rec.append(ext[b0 * 32:b1 * 32].tobytes())
+            body = b"".join(rec)
+            padn = (-len(body)) % RECORD_ALIGN
+            f.write(body + b"\0" * padn)
+            pos += len(body) + padn
+    off[k] = pos
+    off.tofile(os.path.join(index_dir, "postings.off"))
+    bo.tofile(os.path.join(index_dir, "block_off.u64"))
+    base = h["base"].reshape(-1, D)
+    for c, r0 in enumerate(range(0, base.shape[0], chunk_rows)):
+        name = os.path.join(base_dir, f"base.{c:02d}")
+        np.ascontiguousarray(base[r0:r0 + chunk_rows], np.int8).tofile(name)
+        open(name + ".done", "w").close()
+
+
 class L2Mixture:
     """The 100-d int8 mixture of the 1B-row L2 workload at any size: component means N(0, 40^2) per
     dimension; each component holds `topics` sub-centres at N(0, spread^2) around its mean, and

@@ -413,6 +413,20 @@ typedef struct sieve_l2_index sieve_l2_index;
 
 int sieve_l2_index_create(const sieve_l2_desc* desc, sieve_l2_index** out);
 void sieve_l2_index_free(sieve_l2_index* ix);
+/* bigann::Index::open (src/bigann.rs:620-628) over the reference's files: Model::open
+ * (src/bigann.rs:257-266: centroids.f32, coarse.f32, k2.u64, rotation.f32, optional bits.u64),
+ * Postings::open_ext (src/postings.rs:243-262: postings.off, block_off.u64, postings.bin with
+ * 4 KiB-aligned records of code blocks, [mul, |x|^2], ids and the refinement section) in
+ * index_dir, and Base::open (src/bigann.rs:477-489: base.00, base.01, ... with .done markers,
+ * 100,000,000 rows each) in base_dir; everything is uploaded to the current device. */
+int sieve_l2_index_open(const char* index_dir, const char* base_dir, sieve_l2_index** out);
+/* hot_leaves (src/bigann.rs:537-563): of the leaves the nq queries route to (nprobe, cut by
+ * margin), those worth keeping resident under `budget` bytes of postings records, hottest per
+ * byte first (probes over the record's bytes, 4 KiB aligned); leaves (room for k) gets them and
+ * *count their number. The index itself is wholly resident in HBM; the policy is for callers
+ * that tier the records (host RAM, a second device). */
+int sieve_l2_hot_leaves(const sieve_l2_index* ix, sieve_stream* s, const int8_t* q, size_t nq, uint32_t nprobe,
+                        float margin, uint64_t budget, uint32_t* leaves, uint32_t* count);
 /* As sieve_index_set_scan_chunk, for the L2 index. */
 int sieve_l2_index_set_scan_chunk(sieve_l2_index* ix, uint32_t chunk_blocks);
 /* refine_encode (src/bigann.rs:70-108) for n rotated, 128-dim residuals r (n x 128 f32, the

@@ -94,3 +94,48 @@ def test_refine_needs_a_multibit_index(refined_l2):
     b = ix.search(qs, 8, 100, 10, refine=500)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("bits", [1, 2])
+def test_l2_index_open_from_the_reference_files(refined_l2, bits, tmp_path):
+    """sieve_l2_index_open over the files write_l2_index writes (postings records, Model files,
+    base chunks): searches equal the index created from the same arrays, bit for bit."""
+    from sieve_amd.index import L2Index
+    from sieve_amd.l2build import write_l2_index
+    arr, h, ext, qs = refined_l2
+    write_l2_index(arr, str(tmp_path / "ix"), str(tmp_path / "base"), refine_bits=bits,
+                   refine=ext[bits] if bits > 1 else None)
+    fix = L2Index.open(str(tmp_path / "ix"), str(tmp_path / "base"))
+    aix = L2Index(dict(h, refine_bits=bits, refine=ext[bits]) if bits > 1 else h)
+    assert fix.k == aix.k and fix.bits == bits
+    for nprobe, cand, refine in ((8, 300, 0), (16, 100, 500)):
+        a = aix.search(qs, nprobe, cand, 10, details=True, refine=refine)
+        b = fix.search(qs, nprobe, cand, 10, details=True, refine=refine)
+        for x, y in zip(a, b):
+            assert np.array_equal(np.asarray(x).view(np.uint32), np.asarray(y).view(np.uint32))
+
+
+def test_l2_open_rejects_a_mismatched_postings_offset_file(refined_l2, tmp_path):
+    from sieve_amd.abi import SieveError
+    from sieve_amd.index import L2Index
+    from sieve_amd.l2build import write_l2_index
+    arr, _, _, _ = refined_l2
+    write_l2_index(arr, str(tmp_path / "ix"), str(tmp_path / "base"))
+    off = np.fromfile(str(tmp_path / "ix" / "postings.off"), np.uint64)
+    off[:-1].tofile(str(tmp_path / "ix" / "postings.off"))
+    with pytest.raises(SieveError, match="postings.off"):
+        L2Index.open(str(tmp_path / "ix"), str(tmp_path / "base"))
+
+
+@pytest.mark.parametrize("bits,margin", [(1, 0.0), (2, 0.3)])
+def test_hot_leaves_match_the_oracle(refined_l2, bits, margin):
+    from sieve_amd.index import L2Index
+    arr, h, ext, qs = refined_l2
+    ix = L2Index(dict(h, refine_bits=bits, refine=ext[bits]) if bits > 1 else h)
+    oix = O.L2Index(h["leaves"], h["block_off"], h["blocks"], h["factors"], h["ids"], h["rotation"], h["base"])
+    rec = 512 + 32 * 8 + 32 * 4 + 32 * O.refine_bytes(bits)
+    rb = (np.diff(h["block_off"].astype(np.int64)) * rec + 4095) // 4096 * 4096
+    for budget in (0, int(rb.max()), int(rb.sum() // 3), int(rb.sum()) * 2):
+        got = ix.hot_leaves(qs, 8, margin, budget)
+        want = oix.hot_leaves(qs, 8, margin, rb.astype(np.uint64), budget)
+        assert got.tolist() == want.tolist(), budget

@@ -107,3 +107,40 @@ def test_oracle_recall_on_the_built_index(built):
     rec = [recall10_ties(oix.search(q.numpy()[i], 16, 1000, 10)[4], gi[i].numpy(), gs[i].numpy())
            for i in range(40)]
     assert np.mean(rec) > 0.9, np.mean(rec)
+
+
+def test_l2_index_files_have_the_reference_layout(tmp_path):
+    """write_l2_index: the postings records of src/bigann.rs:386-460 (4 KiB-aligned, code blocks,
+    [mul, |x|^2], ids, refinement section), the Model files and the base chunks, read back by an
+    independent parse."""
+    import os
+    import numpy as np
+    from sieve_amd.l2build import build_l2, int8_mixture, write_l2_index
+    base = int8_mixture(3000, 12, seed=3)
+    arr = build_l2(base, 12, iters=4)
+    h = arr.numpy()
+    S = h["ids"].shape[0]
+    rb = 16 + 4
+    ext = np.random.default_rng(1).integers(0, 256, (S, rb), dtype=np.uint8)
+    write_l2_index(arr, str(tmp_path / "ix"), str(tmp_path / "base"), refine_bits=2, refine=ext, chunk_rows=1000)
+    rd = lambda name, dt: np.fromfile(str(tmp_path / "ix" / name), dt)
+    assert np.array_equal(rd("centroids.f32", np.float32), h["leaves"].reshape(-1))
+    assert np.array_equal(rd("coarse.f32", np.float32), h["leaves"].reshape(-1)) and rd("k2.u64", np.uint64)[0] == 1
+    assert rd("bits.u64", np.uint64)[0] == 2 and rd("rotation.f32", np.float32).shape == (384,)
+    bo = rd("block_off.u64", np.uint64).astype(np.int64)
+    off = rd("postings.off", np.uint64).astype(np.int64)
+    post = np.fromfile(str(tmp_path / "ix" / "postings.bin"), np.uint8)
+    assert np.array_equal(bo, h["block_off"].astype(np.int64)) and off[-1] == post.shape[0]
+    rec = 512 + 32 * 8 + 32 * 4 + 32 * rb
+    for c in range(12):
+        nb = int(bo[c + 1] - bo[c])
+        assert off[c] % 4096 == 0 and off[c + 1] - off[c] == -(-nb * rec // 4096) * 4096
+        r = post[off[c]:off[c] + nb * rec]
+        b0 = int(bo[c])
+        assert np.array_equal(r[:nb * 512], h["blocks"].reshape(-1, 512)[b0:b0 + nb].reshape(-1))
+        assert np.array_equal(r[nb * 512:nb * 768].view(np.float32), h["factors"].reshape(-1)[b0 * 64:(b0 + nb) * 64])
+        assert np.array_equal(r[nb * 768:nb * 896].view(np.uint32), h["ids"][b0 * 32:(b0 + nb) * 32].astype(np.uint32))
+        assert np.array_equal(r[nb * 896:].reshape(-1, rb), ext[b0 * 32:(b0 + nb) * 32])
+    rows = [np.fromfile(str(tmp_path / "base" / f"base.{i:02d}"), np.int8) for i in range(3)]
+    assert all(os.path.exists(str(tmp_path / "base" / f"base.{i:02d}.done")) for i in range(3))
+    assert np.array_equal(np.concatenate(rows), h["base"].reshape(-1))
