@@ -47,7 +47,9 @@ T* upload(const T* src, size_t n, size_t* acct) {
     return d;
 }
 
-sieve_index* create_index(const sieve_index_desc* d) {
+// seg_rows: rows of a compacted index's live segment, which its lists may reference as row ids
+// n .. n + seg_rows - 1.
+sieve_index* create_index(const sieve_index_desc* d, uint64_t seg_rows = 0) {
     SIEVE_REQUIRE(d, SIEVE_EINVAL, "desc is null");
     SIEVE_REQUIRE(d->k >= 1, SIEVE_EINVAL, "k must be >= 1");
     SIEVE_REQUIRE(d->centroids && d->block_off && d->blocks && d->ids && d->rerank && d->quant && d->norm256,
@@ -126,12 +128,14 @@ sieve_index* create_index(const sieve_index_desc* d) {
     {
         unsigned long long* bad = dmalloc<unsigned long long>(1, nullptr);
         SIEVE_HIP(hipMemset(bad, 0, 8));
-        launch_count_bad_ids(ix->ids, ix->nslots, d->n, bad, st);
+        launch_count_bad_ids(ix->ids, ix->nslots, d->n + seg_rows, bad, st);
         unsigned long long hbad = 0;
         SIEVE_HIP(hipMemcpy(&hbad, bad, 8, hipMemcpyDeviceToHost));
         (void)hipFree(bad);
-        SIEVE_REQUIRE(hbad == 0, SIEVE_EINVAL, std::to_string(hbad) + " slot ids are beyond the corpus (ids.u32)");
+        SIEVE_REQUIRE(hbad == 0, SIEVE_EINVAL,
+                      std::to_string(hbad) + " slot ids are beyond the corpus and its live segment (ids.u32)");
     }
+    ix->seg_rows = seg_rows;
     ix->rotation = upload(d->rotation, 3 * PDIM, acct);
     if (d->borrow_rerank && on_device(d->rerank)) {
         ix->rerank = const_cast<int8_t*>(d->rerank);
@@ -367,7 +371,7 @@ int sieve_index_open(const char* index_dir, const char* ingest_dir, sieve_index*
     return guarded([&] {
         SIEVE_REQUIRE(index_dir && ingest_dir && out, SIEVE_EINVAL, "null argument");
         std::string I(index_dir), C(ingest_dir);
-        Mapped cents, bo, blocks, ids, factors, rot, rr, quant, norm, t2s, t2c, t2n;
+        Mapped cents, bo, blocks, ids, factors, rot, rr, quant, norm, t2s, t2c, t2n, seg;
         map_file(I + "/centroids.f32", cents, true);
         map_file(I + "/block_off.u64", bo, true);
         map_file(I + "/blocks.bin", blocks, true);
@@ -409,7 +413,11 @@ int sieve_index_open(const char* index_dir, const char* ingest_dir, sieve_index*
             d.tier2_norm = (const float*)t2n.p;
             d.tier2_sigma = (const float*)t2s.p;
         }
-        *out = create_index(&d);
+        // A compacted index (src/live.rs:395-468) carries its row segment: its lists reference
+        // the segment's rows.
+        const bool compacted = map_file(I + "/live.ext.u32", seg, false);
+        SIEVE_REQUIRE(!compacted || seg.len % 4 == 0, SIEVE_EIO, "live.ext.u32 size");
+        *out = create_index(&d, compacted ? seg.len / 4 : 0);
     });
 }
 
@@ -467,6 +475,8 @@ int sieve_candidates(const sieve_index* ix, sieve_stream* s, const float* q256, 
         SIEVE_REQUIRE(ix && s && q256 && probe_scores && probe_clusters && probe_counts && est && rows && errs && counts,
                       SIEVE_EINVAL, "null argument");
         check_params(p, false);
+        SIEVE_REQUIRE(ix->seg_rows == 0, SIEVE_EINVAL,
+                      "the index's lists hold rows of its live segment (live.*): search it through its live state");
         SIEVE_REQUIRE(nprobe_stride >= 1 && nprobe_stride <= (size_t)MAX_PROBE, SIEVE_ELIMIT, "probe stride limit");
         if (nq == 0) return;
         // Probe lists come from the caller: check them on the host before any kernel reads them.
@@ -530,6 +540,8 @@ int sieve_rerank(const sieve_index* ix, sieve_stream* s, const float* q768, cons
         SIEVE_REQUIRE(ix && s && q768 && q256 && est && rows && errs && cand_counts && scores && out_rows && counts,
                       SIEVE_EINVAL, "null argument");
         check_params(p, false);
+        SIEVE_REQUIRE(ix->seg_rows == 0, SIEVE_EINVAL,
+                      "the index's lists hold rows of its live segment (live.*): search it through its live state");
         SIEVE_REQUIRE(cand_stride >= 1 && cand_stride <= (size_t)MAX_CAND, SIEVE_ELIMIT, "candidate stride limit");
         if (nq == 0) return;
         std::vector<uint32_t> hn(nq), hr(nq * cand_stride);
@@ -741,6 +753,8 @@ int sieve_search_batch(const sieve_index* ix, sieve_stream* s, const float* q768
     return guarded([&] {
         SIEVE_REQUIRE(ix && s && q768 && q256 && scores && rows && counts, SIEVE_EINVAL, "null argument");
         check_params(p, true);
+        SIEVE_REQUIRE(ix->seg_rows == 0, SIEVE_EINVAL,
+                      "the index's lists hold rows of its live segment (live.*): search it through its live state");
         if (nq == 0) return;
         const float* q2 = dev_in(s, B_Q256, q256, nq * PDIM);
         const float* q7 = dev_in(s, B_Q768, q768, nq * DIM);

@@ -15,9 +15,9 @@ using namespace sieve;
 
 namespace {
 
-constexpr uint64_t BRUTE_MAX = 250000;       // src/filter.rs:54
-constexpr double POST_MIN = 0.3;             // src/filter.rs:56
-constexpr uint64_t EXPAND_TARGET = 200000;   // src/filter.rs:58
+constexpr uint64_t BRUTE_MAX = 250000;       // src/filter.rs:55
+constexpr double POST_MIN = 0.3;             // src/filter.rs:57
+constexpr uint64_t EXPAND_TARGET = 200000;   // src/filter.rs:59
 constexpr int EXPAND_MAX_K = 16384;          // centroids the expand cut sorts in LDS
 constexpr int EXPAND_MAX_PROBE = 8192;       // clusters an expand cut may probe (the scan's LDS segment table)
 constexpr int FT = 256;                      // threads of the filter kernels
@@ -317,6 +317,8 @@ FilterRun enqueue_filter_search(const sieve_index* ix, const sieve_filter* f, ui
                                 sieve_stream* s, const float* q2, const float* q7, size_t nq, const sieve_params* p,
                                 float* os, uint32_t* orr, uint32_t* on, const LiveArgs* lv) {
     SIEVE_REQUIRE(f->ix == ix, SIEVE_EINVAL, "the filter belongs to another index");
+    SIEVE_REQUIRE(ix->seg_rows == 0 || lv, SIEVE_EINVAL,
+                  "the index's lists hold rows of its live segment (live.*): search it through its live state");
     SIEVE_REQUIRE(plan <= SIEVE_PLAN_AUTO, SIEVE_EINVAL, "unknown plan");
     SIEVE_REQUIRE(nq <= 65535, SIEVE_ELIMIT, "a filtered batch holds at most 65535 queries");
     FilterRun fr;

@@ -123,6 +123,9 @@ struct sieve_index {
     float* t2_norm = nullptr;
     size_t device_bytes = 0;
     struct sieve_live* live = nullptr;  // attached live state (sieve_live_open)
+    // Rows of the live segment (live.*) that the lists of a compacted index reference: row ids
+    // n .. n + seg_rows - 1 resolve through the live state, so searches need one.
+    uint64_t seg_rows = 0;
     int scan_variant = 0;  // scan kernel variant (sieve_index_autotune)
     uint32_t scan_chunk_blocks = 0;  // 0: chunk size chosen per batch (sieve_index_set_scan_chunk)
     // Cluster-major scan (sieve_index_set_cluster_scan): batches of at least cm_min_queries
@@ -594,6 +597,8 @@ inline void rerank_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt
 inline void enqueue_search(const sieve_index* ix, sieve_stream* s, const float* q2, const float* q7, size_t nq,
                            const sieve_params* p, const ViewArgs& view_in, float* os, uint32_t* orr, uint32_t* on,
                            const LiveArgs* lv = nullptr) {
+    SIEVE_REQUIRE(ix->seg_rows == 0 || lv, SIEVE_EINVAL,
+                  "the index's lists hold rows of its live segment (live.*): search it through its live state");
     ViewArgs view = view_in;
     if (lv && lv->deleted) {
         view.deleted = lv->deleted;

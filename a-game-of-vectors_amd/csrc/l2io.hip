@@ -1,15 +1,15 @@
 // The 1B L2 index in the reference's on-disk form, and its hot-leaf policy.
 //
-// sieve_l2_index_open reads what bigann::Index::open reads (src/bigann.rs:620-628):
+// sieve_l2_index_open reads what bigann::Index::open reads (src/bigann.rs:634-641):
 //   Model::open (src/bigann.rs:257-266): centroids.f32 (the k leaves x 100 f32, cell major),
 //     coarse.f32 and k2.u64 (the two-level model the builder assigns rows with; required as
 //     there, not used by search), rotation.f32 (3 x 128 signs), bits.u64 (refinement width,
-//     absent = 1, src/bigann.rs:245-250);
-//   Postings::open_ext (src/postings.rs:243-262): postings.off (k + 1 record offsets, equal in
+//     absent = 1, src/bigann.rs:229-234);
+//   Postings::open_ext (src/postings.rs:209-229): postings.off (k + 1 record offsets, equal in
 //     length to block_off.u64), block_off.u64, postings.bin: record c at postings.off[c], 4 KiB
 //     aligned, holds for its nb blocks the 512-byte code blocks, nb x 32 x [mul, |x|^2] f32, nb x
-//     32 u32 row ids and nb x 32 x refine_bytes(bits) extra bytes (src/postings.rs:20-34, 62-76);
-//   Base::open (src/bigann.rs:465-489): base.00, base.01, ... in base_dir, each with a
+//     32 u32 row ids and nb x 32 x refine_bytes(bits) extra bytes (src/postings.rs:26-35, 53-75);
+//   Base::open (src/bigann.rs:476-489): base.00, base.01, ... in base_dir, each with a
 //     base.NN.done marker, 100,000,000 rows of 100 int8 each (the last may be shorter).
 // The sections are gathered record by record into the library's arrays through pinned
 // staging, the base rows are streamed into HBM, and the index is then created as from arrays.
@@ -137,7 +137,7 @@ int sieve_l2_index_open(const char* index_dir, const char* base_dir, sieve_l2_in
             n += chunks.back()->len / L2D;
         }
         SIEVE_REQUIRE(!chunks.empty(), SIEVE_EIO, "no base.NN chunks in " + B);
-        // Row id r lives in chunk r / CHUNK_ROWS (src/bigann.rs:503-506): every chunk but the last
+        // Row id r lives in chunk r / CHUNK_ROWS (src/bigann.rs:58, :509): every chunk but the last
         // is full.
         for (size_t c = 0; c + 1 < chunks.size(); c++)
             SIEVE_REQUIRE(chunks[c]->len / L2D == CHUNK_ROWS, SIEVE_EINVAL,

@@ -19,26 +19,23 @@
 
 #include <hipcub/hipcub.hpp>
 
-#include <unordered_map>
-
-#include "host.hpp"
+#include "live.hpp"
 
 using namespace sieve;
 
 namespace {
 
-constexpr int SOAR_CAND = 16;  // src/live.rs:34
+constexpr int SOAR_CAND = 16;  // src/live.rs:35
 constexpr float EPS0 = 1.9f;   // src/rabitq.rs:28
 constexpr float T2_T = 0.9816f;
 __constant__ float T2_LEVELS[4] = {-1.510f, -0.4528f, 0.4528f, 1.510f};  // src/tier2.rs:10-11
 
 __device__ __forceinline__ float sqrt_rn(float x) { return (float)sqrt((double)x); }
 
-// quant::Int8Quantizer::encode (src/quant.rs:52-56), the prefix norm of Live::insert
-// (src/live.rs:290-293), and the dequantised, normalised prefix of apply_insert
-// (src/live.rs:336-341). One thread per row: every sum is sequential, as the reference's.
-__global__ void k_live_rows(const float* __restrict__ vecs, size_t n, const float* __restrict__ quant,
-                            int8_t* __restrict__ rows, float* __restrict__ norm256, float* __restrict__ xs) {
+// quant::Int8Quantizer::encode (src/quant.rs:52-56) and the prefix norm of Live::insert
+// (src/live.rs:280-283). One thread per row: every sum is sequential, as the reference's.
+__global__ void k_live_quant(const float* __restrict__ vecs, size_t n, const float* __restrict__ quant,
+                             int8_t* __restrict__ rows, float* __restrict__ norm256) {
     const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const float* v = vecs + j * DIM;
@@ -51,6 +48,15 @@ __global__ void k_live_rows(const float* __restrict__ vecs, size_t n, const floa
     float s = -0.0f;
     for (int i = 0; i < PDIM; i++) s = s + v[i] * v[i];
     norm256[j] = sqrt_rn(s);
+}
+
+// The dequantised, normalised prefix of apply_insert (src/live.rs:325-329), from the int8 row
+// alone, so a replayed log gives the same codes.
+__global__ void k_live_prefix(const int8_t* __restrict__ rows, size_t n, const float* __restrict__ quant,
+                              float* __restrict__ xs) {
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int8_t* r = rows + j * DIM;
     float* x = xs + j * PDIM;
     float ss = -0.0f;
     for (int i = 0; i < PDIM; i++) {
@@ -120,7 +126,7 @@ __global__ void k_live_pairs(const float* __restrict__ xs, size_t n, const float
 
 // Per row (one 64-thread workgroup): Rotation::apply of the prefix, then for each of its
 // clusters rabitq::encode against the rotated centroid (src/rabitq.rs:86-108) into the slot
-// log, and tier2::encode of the prefix (src/tier2.rs:39-50).
+// log, and tier2::encode of the prefix (src/tier2.rs:36-47).
 __global__ __launch_bounds__(64) void k_live_encode(const float* __restrict__ xs, size_t n,
                                                     const float* __restrict__ signs, float rot_norm,
                                                     const float* __restrict__ cents_rot,
@@ -195,7 +201,7 @@ __global__ __launch_bounds__(64) void k_live_encode(const float* __restrict__ xs
     }
 }
 
-// Rotation::apply of each centroid (src/live.rs:183-192).
+// Rotation::apply of each centroid (src/live.rs:172-180).
 __global__ __launch_bounds__(64) void k_rotate_rows(const float* __restrict__ in, const float* __restrict__ signs,
                                                     float rot_norm, float* __restrict__ out) {
     __shared__ float v[PDIM];
@@ -272,96 +278,10 @@ void dev_realloc(T*& p, size_t old_n, size_t new_n, hipStream_t st, size_t* acct
     p = q;
 }
 
-}  // namespace
-
-struct sieve_live {
-    sieve_index* ix = nullptr;
-    bool soar = true;
-    float lambda = 1.0f;
-    uint64_t n = 0;            // corpus rows
-    uint64_t cap_rows = 0;     // delta rows the device holds
-    uint64_t cap_slots = 0;    // arena slots after the base slots
-    uint64_t nrows = 0, nlog = 0, arena_blocks = 0, ndeleted = 0;
-    size_t device_bytes = 0;
-    float* cents_rot = nullptr;
-    int8_t* rows = nullptr;
-    float* norm256 = nullptr;
-    uint8_t* t2 = nullptr;
-    float* t2n = nullptr;
-    uint32_t* ext = nullptr;
-    uint64_t* deleted = nullptr;
-    uint64_t deleted_words = 0;
-    uint4* lcode = nullptr;
-    float* lmul = nullptr;
-    float* lerr = nullptr;
-    uint32_t* lrow = nullptr;
-    uint32_t* lcl = nullptr;
-    uint32_t* dblock_off = nullptr;  // k + 1
-    // writer-side state on the host
-    std::unordered_map<uint32_t, uint32_t> by_ext;
-    std::vector<uint64_t> deleted_host;
-    sieve_stream work;  // scratch for the encoder and the arena rebuild
-    DevBuf xs, pairs, sort_keys, sort_vals, sort_tmp, counts, nb, cstart, words_i, words_v;
-
-    ~sieve_live() {
-        void* ps[] = {cents_rot, rows, norm256, t2, t2n, ext, deleted, lcode, lmul, lerr, lrow, lcl, dblock_off};
-        for (void* p : ps)
-            if (p) (void)hipFree(p);
-        for (DevBuf* b : {&xs, &pairs, &sort_keys, &sort_vals, &sort_tmp, &counts, &nb, &cstart, &words_i, &words_v})
-            b->release();
-    }
-
-    LiveArgs args() const {
-        LiveArgs a;
-        a.dblock_off = dblock_off;
-        a.dbase = (uint32_t)ix->nblocks;
-        a.dblocks = (uint32_t)arena_blocks;
-        a.n_base = (uint32_t)n;
-        a.drows = rows;
-        a.dnorm256 = norm256;
-        a.dt2 = ix->t2_codes ? t2 : nullptr;
-        a.dt2n = ix->t2_codes ? t2n : nullptr;
-        a.dext = ext;
-        a.deleted = deleted;
-        a.deleted_words = deleted_words;
-        return a;
-    }
-
-    // Live::current (src/live.rs:271-277).
-    bool current(uint32_t e, uint32_t* id) const {
-        auto it = by_ext.find(e);
-        if (it != by_ext.end()) {
-            *id = it->second;
-            return true;
-        }
-        if ((uint64_t)e < n && !bit_host(e)) {
-            *id = e;
-            return true;
-        }
-        return false;
-    }
-    bool bit_host(uint32_t id) const {
-        const size_t w = id >> 6;
-        return w < deleted_host.size() && ((deleted_host[w] >> (id & 63)) & 1) == 1;
-    }
-};
-
-namespace {
-
-// Uploads the changed tombstone words.
-void push_words(sieve_live* l, hipStream_t st, const std::vector<uint64_t>& changed) {
-    if (changed.empty()) return;
-    std::vector<uint64_t> vals(changed.size());
-    for (size_t i = 0; i < changed.size(); i++) vals[i] = l->deleted_host[changed[i]];
-    uint64_t* di = grow<uint64_t>(l->words_i, changed.size());
-    uint64_t* dv = grow<uint64_t>(l->words_v, changed.size());
-    SIEVE_HIP(hipMemcpyAsync(di, changed.data(), changed.size() * 8, hipMemcpyHostToDevice, st));
-    SIEVE_HIP(hipMemcpyAsync(dv, vals.data(), vals.size() * 8, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_set_words, dim3(grid(changed.size(), 256)), dim3(256), 0, st, l->deleted, di, dv,
-                       changed.size());
-    SIEVE_HIP(hipGetLastError());
-    SIEVE_HIP(hipStreamSynchronize(st));
-}
+// Rows per encoder pass: bounds the encoder's and the router's scratch. Rows are applied in
+// order and each row's codes depend on the row alone, so passes give the state one pass over
+// all rows would.
+constexpr size_t APPLY_ROWS = 8192;
 
 // Sets the tombstone of `id` on the host; records its word.
 void tombstone(sieve_live* l, uint32_t id, std::vector<uint64_t>& changed) {
@@ -424,21 +344,18 @@ void rebuild_arena(sieve_live* l, hipStream_t st) {
     l->arena_blocks = blocks;
 }
 
-void live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, size_t n) {
+// One encoder pass over delta rows r0 .. r0 + n (already in l->rows and l->norm256): prefix,
+// clusters, slot codes into the log at l->nlog, tier-2 codes, external ids; then the upserts.
+void apply_pass(sieve_live* l, const uint32_t* eh, size_t n, size_t r0, std::vector<uint64_t>& changed) {
     sieve_index* ix = l->ix;
-    SIEVE_REQUIRE(l->nrows + n <= l->cap_rows, SIEVE_ELIMIT,
-                  "live capacity: " + std::to_string(l->cap_rows) + " delta rows");
-    if (n == 0) return;
     sieve_stream* s = &l->work;
     const hipStream_t st = s->st;
-    const float* dv = dev_in(s, B_Q768, vecs, n * DIM);
     float* xs = grow<float>(l->xs, n * PDIM);
-    const size_t r0 = l->nrows;
-    hipLaunchKernelGGL(k_live_rows, dim3(grid(n, 64)), dim3(64), 0, st, dv, n, ix->quant, l->rows + r0 * DIM,
-                       l->norm256 + r0, xs);
+    hipLaunchKernelGGL(k_live_prefix, dim3(grid(n, 64)), dim3(64), 0, st, l->rows + r0 * DIM, n, ix->quant, xs);
     SIEVE_HIP(hipGetLastError());
     // assign_top (src/ivf.rs:137-158) through the exact route: BF16 GEMM plus f32 rescoring of
-    // every centroid near the cut gives the reference's f32 top SOAR_CAND, ties by id.
+    // every centroid near the cut gives the reference's f32 top SOAR_CAND, ties by id. Routing
+    // reads the prefix only; the 768-d query the prep also takes is a zero stand-in.
     const int m = (int)std::min<uint32_t>(l->soar ? SOAR_CAND : 1, ix->k);
     sieve_params rp{};
     rp.cand = 1;
@@ -446,8 +363,11 @@ void live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, size_t n
     rp.nprobe = (uint32_t)m;
     rp.margin = std::numeric_limits<float>::quiet_NaN();
     rp.budget = 0;
+    const size_t zq_cap = l->zq.cap;
+    float* zq = grow<float>(l->zq, n * DIM);
+    if (l->zq.cap != zq_cap) SIEVE_HIP(hipMemsetAsync(zq, 0, l->zq.cap, st));
     Batch bt;
-    prep_queries(ix, s, xs, dv, n, bt);
+    prep_queries(ix, s, xs, zq, n, bt);
     float* ps = grow<float>(s->b[B_PS], n * m);
     uint32_t* pc = grow<uint32_t>(s->b[B_PC], n * m);
     uint32_t* pn = grow<uint32_t>(s->b[B_PN], n);
@@ -456,65 +376,162 @@ void live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, size_t n
     hipLaunchKernelGGL(k_live_pairs, dim3(grid(n, 64)), dim3(64), 0, st, xs, n, ix->cents, pc, pn, m, l->soar ? 1 : 0,
                        l->lambda, pairs);
     const int spr = l->soar ? 2 : 1;
-    SIEVE_REQUIRE(l->nlog + spr * n <= 2 * l->cap_rows, SIEVE_ELIMIT, "live slot log full");
     const size_t L0 = l->nlog;
     hipLaunchKernelGGL(k_live_encode, dim3((unsigned)n), dim3(64), 0, st, xs, n, ix->rotation, 1.0f / 4096.0f,
                        l->cents_rot, pairs, spr, (uint32_t)(l->n + r0), l->lcode + 2 * L0, l->lmul + L0,
                        l->lerr + L0, l->lrow + L0, l->lcl + L0, ix->t2_sigma, l->t2 + r0 * 64, l->t2n + r0);
     SIEVE_HIP(hipGetLastError());
-    SIEVE_HIP(hipMemcpyAsync(l->ext + r0, ext, n * 4, on_device(ext) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                             st));
+    SIEVE_HIP(hipMemcpyAsync(l->ext + r0, eh, n * 4, hipMemcpyHostToDevice, st));
     // The encoder's launches finish before anything is committed (the log and row counts advance
     // below).
-    SIEVE_HIP(hipStreamSynchronize(s->st));
-    // Upserts, in order (src/live.rs:369-378): a live external id's old row is tombstoned.
-    std::vector<uint32_t> he;
-    const uint32_t* eh = ext;
-    if (on_device(ext)) {
-        he.resize(n);
-        SIEVE_HIP(hipMemcpy(he.data(), ext, n * 4, hipMemcpyDeviceToHost));
-        eh = he.data();
-    }
-    std::vector<uint64_t> changed;
+    SIEVE_HIP(hipStreamSynchronize(st));
+    // Upserts, in order (src/live.rs:361-373): a live external id's old row is tombstoned.
     for (size_t j = 0; j < n; j++) {
         uint32_t old;
         if (l->current(eh[j], &old)) tombstone(l, old, changed);
         l->by_ext[eh[j]] = (uint32_t)(l->n + r0 + j);
     }
-    push_words(l, st, changed);
     l->nrows += n;
     l->nlog += spr * n;
-    rebuild_arena(l, st);
 }
 
-void live_delete(sieve_live* l, const uint32_t* ext, size_t n) {
+// Live::insert (src/live.rs:275-294): quantise on the device, log the records (op, external
+// id, prefix norm, int8 row) and wait for them to be durable, then apply.
+void live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, size_t n) {
+    sieve_index* ix = l->ix;
+    SIEVE_REQUIRE(l->nrows + n <= l->cap_rows, SIEVE_ELIMIT,
+                  "live capacity: " + std::to_string(l->cap_rows) + " delta rows");
+    if (n == 0) return;
+    sieve_stream* s = &l->work;
+    const hipStream_t st = s->st;
+    const float* dv = dev_in(s, B_Q768, vecs, n * DIM);
+    const size_t r0 = l->nrows;
+    hipLaunchKernelGGL(k_live_quant, dim3(grid(n, 64)), dim3(64), 0, st, dv, n, ix->quant, l->rows + r0 * DIM,
+                       l->norm256 + r0);
+    SIEVE_HIP(hipGetLastError());
     std::vector<uint32_t> he;
     const uint32_t* eh = ext;
     if (on_device(ext)) {
         he.resize(n);
-        SIEVE_HIP(hipMemcpy(he.data(), ext, n * 4, hipMemcpyDeviceToHost));
+        SIEVE_HIP(hipMemcpyAsync(he.data(), ext, n * 4, hipMemcpyDeviceToHost, st));
         eh = he.data();
     }
-    std::vector<uint64_t> changed;
-    for (size_t j = 0; j < n; j++) {  // Live::apply_delete, src/live.rs:317-326
-        uint32_t id;
-        if (l->current(eh[j], &id)) {
-            if (!l->bit_host(id)) tombstone(l, id, changed);
-            l->by_ext.erase(eh[j]);
-        }
+    if (l->wal_fd >= 0) {
+        std::vector<int8_t> rows(n * DIM);
+        std::vector<float> norms(n);
+        SIEVE_HIP(hipMemcpyAsync(rows.data(), l->rows + r0 * DIM, n * DIM, hipMemcpyDeviceToHost, st));
+        SIEVE_HIP(hipMemcpyAsync(norms.data(), l->norm256 + r0, n * 4, hipMemcpyDeviceToHost, st));
+        SIEVE_HIP(hipStreamSynchronize(st));
+        std::vector<uint8_t> rec(n * WAL_INSERT_BYTES);
+        for (size_t j = 0; j < n; j++) wal_insert_record(eh[j], norms[j], rows.data() + j * DIM, &rec[j * WAL_INSERT_BYTES]);
+        live_log(l, rec);
+    } else {
+        SIEVE_HIP(hipStreamSynchronize(st));
     }
-    push_words(l, l->work.st, changed);
+    live_apply_rows(l, eh, n);
 }
 
 }  // namespace
 
 namespace sieve {
+
 LiveArgs live_args(const sieve_live* l) { return l->args(); }
 LiveArgs live_args_max(const sieve_live* l) {
     LiveArgs a = l->args();
     a.dblocks = (uint32_t)(l->cap_slots / BLOCK);  // every delta list padded to whole blocks fits
     return a;
 }
+
+void live_push_words(sieve_live* l, hipStream_t st, const std::vector<uint64_t>& changed) {
+    if (changed.empty()) return;
+    std::vector<uint64_t> vals(changed.size());
+    for (size_t i = 0; i < changed.size(); i++) vals[i] = l->deleted_host[changed[i]];
+    uint64_t* di = grow<uint64_t>(l->words_i, changed.size());
+    uint64_t* dv = grow<uint64_t>(l->words_v, changed.size());
+    SIEVE_HIP(hipMemcpyAsync(di, changed.data(), changed.size() * 8, hipMemcpyHostToDevice, st));
+    SIEVE_HIP(hipMemcpyAsync(dv, vals.data(), vals.size() * 8, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_set_words, dim3(grid(changed.size(), 256)), dim3(256), 0, st, l->deleted, di, dv,
+                       changed.size());
+    SIEVE_HIP(hipGetLastError());
+    SIEVE_HIP(hipStreamSynchronize(st));
+}
+
+void live_apply_rows(sieve_live* l, const uint32_t* eh, size_t n) {
+    SIEVE_REQUIRE(l->nrows + n <= l->cap_rows, SIEVE_ELIMIT,
+                  "live capacity: " + std::to_string(l->cap_rows) + " delta rows");
+    if (n == 0) return;
+    const int spr = l->soar ? 2 : 1;
+    SIEVE_REQUIRE(l->nlog + spr * n <= 2 * l->cap_rows, SIEVE_ELIMIT, "live slot log full");
+    std::vector<uint64_t> changed;
+    for (size_t j0 = 0; j0 < n; j0 += APPLY_ROWS) {
+        const size_t m = std::min(APPLY_ROWS, n - j0);
+        apply_pass(l, eh + j0, m, l->nrows, changed);
+    }
+    live_push_words(l, l->work.st, changed);
+    rebuild_arena(l, l->work.st);
+}
+
+void live_apply_delete(sieve_live* l, const uint32_t* eh, size_t n) {
+    std::vector<uint64_t> changed;
+    for (size_t j = 0; j < n; j++) {  // Live::apply_delete, src/live.rs:308-315
+        uint32_t id;
+        if (l->current(eh[j], &id)) {
+            if (!l->bit_host(id)) tombstone(l, id, changed);
+            l->by_ext.erase(eh[j]);
+        }
+    }
+    live_push_words(l, l->work.st, changed);
+}
+
+sieve_live* live_create(sieve_index* ix, const sieve_live_opts& opts) {
+    SIEVE_REQUIRE(ix->rotation && ix->mul, SIEVE_EINVAL, "live updates need a RaBitQ index");
+    SIEVE_REQUIRE(!ix->live, SIEVE_EINVAL, "the index already has a live state");
+    SIEVE_REQUIRE(opts.max_rows >= 1, SIEVE_EINVAL, "max_rows must be >= 1");
+    SIEVE_REQUIRE(ix->n + opts.max_rows < 0xFFFFFFFFull, SIEVE_ELIMIT, "row ids exceed 32 bits");
+    std::unique_ptr<sieve_live> l(new sieve_live());
+    SIEVE_HIP(hipSetDevice(ix->device));
+    l->ix = ix;
+    l->soar = !(opts.soar_lambda != opts.soar_lambda);
+    l->lambda = l->soar ? opts.soar_lambda : 0.0f;
+    l->n = ix->n;
+    l->cap_rows = opts.max_rows;
+    l->cap_slots = 2 * opts.max_rows + (uint64_t)BLOCK * ix->k;
+    SIEVE_REQUIRE(ix->nslots + l->cap_slots < 0xFFFFFFFFull, SIEVE_ELIMIT, "slot ids exceed 32 bits");
+    SIEVE_HIP(hipStreamCreate(&l->work.st));
+    l->work.device = ix->device;
+    const hipStream_t st = l->work.st;
+    size_t* acct = &l->device_bytes;
+    // Room for the arena after the base slots.
+    const size_t ns = ix->nslots, total = ns + l->cap_slots;
+    dev_realloc(ix->codes, 2 * ns, 2 * total, st, &ix->device_bytes);
+    dev_realloc(ix->mul, ns, total, st, &ix->device_bytes);
+    dev_realloc(ix->err, ns, total, st, &ix->device_bytes);
+    dev_realloc(ix->ids, ns, total, st, &ix->device_bytes);
+    l->cents_rot = dmalloc<float>((size_t)ix->k * PDIM, acct);
+    hipLaunchKernelGGL(k_rotate_rows, dim3(ix->k), dim3(64), 0, st, ix->cents, ix->rotation, 1.0f / 4096.0f,
+                       l->cents_rot);
+    const size_t R = l->cap_rows;
+    l->rows = dmalloc<int8_t>(R * DIM, acct);
+    l->norm256 = dmalloc<float>(R, acct);
+    l->t2 = dmalloc<uint8_t>(R * 64, acct);
+    l->t2n = dmalloc<float>(R, acct);
+    l->ext = dmalloc<uint32_t>(R, acct);
+    l->deleted_words = (l->n + R + 63) / 64;
+    l->deleted = dmalloc<uint64_t>(l->deleted_words, acct);
+    SIEVE_HIP(hipMemsetAsync(l->deleted, 0, l->deleted_words * 8, st));
+    l->deleted_host.assign(l->deleted_words, 0);
+    l->lcode = dmalloc<uint4>(2 * 2 * R, acct);
+    l->lmul = dmalloc<float>(2 * R, acct);
+    l->lerr = dmalloc<float>(2 * R, acct);
+    l->lrow = dmalloc<uint32_t>(2 * R, acct);
+    l->lcl = dmalloc<uint32_t>(2 * R, acct);
+    l->dblock_off = dmalloc<uint32_t>(ix->k + 1, acct);
+    SIEVE_HIP(hipGetLastError());
+    rebuild_arena(l.get(), st);
+    ix->live = l.get();
+    return l.release();
+}
+
 }  // namespace sieve
 
 extern "C" {
@@ -522,53 +539,9 @@ extern "C" {
 int sieve_live_open(sieve_index* ix, const sieve_live_opts* o, sieve_live** out) {
     return guarded([&] {
         SIEVE_REQUIRE(ix && out, SIEVE_EINVAL, "null argument");
-        SIEVE_REQUIRE(ix->rotation && ix->mul, SIEVE_EINVAL, "live updates need a RaBitQ index");
-        SIEVE_REQUIRE(!ix->live, SIEVE_EINVAL, "the index already has a live state");
-        sieve_live_opts opts = o ? *o : sieve_live_opts{};
-        SIEVE_REQUIRE(opts.max_rows >= 1, SIEVE_EINVAL, "max_rows must be >= 1");
-        SIEVE_REQUIRE(ix->n + opts.max_rows < 0xFFFFFFFFull, SIEVE_ELIMIT, "row ids exceed 32 bits");
-        std::unique_ptr<sieve_live> l(new sieve_live());
-        SIEVE_HIP(hipSetDevice(ix->device));
-        l->ix = ix;
-        l->soar = !(opts.soar_lambda != opts.soar_lambda);
-        l->lambda = l->soar ? opts.soar_lambda : 0.0f;
-        l->n = ix->n;
-        l->cap_rows = opts.max_rows;
-        l->cap_slots = 2 * opts.max_rows + (uint64_t)BLOCK * ix->k;
-        SIEVE_REQUIRE(ix->nslots + l->cap_slots < 0xFFFFFFFFull, SIEVE_ELIMIT, "slot ids exceed 32 bits");
-        SIEVE_HIP(hipStreamCreate(&l->work.st));
-        l->work.device = ix->device;
-        const hipStream_t st = l->work.st;
-        size_t* acct = &l->device_bytes;
-        // Room for the arena after the base slots.
-        const size_t ns = ix->nslots, total = ns + l->cap_slots;
-        dev_realloc(ix->codes, 2 * ns, 2 * total, st, &ix->device_bytes);
-        dev_realloc(ix->mul, ns, total, st, &ix->device_bytes);
-        dev_realloc(ix->err, ns, total, st, &ix->device_bytes);
-        dev_realloc(ix->ids, ns, total, st, &ix->device_bytes);
-        l->cents_rot = dmalloc<float>((size_t)ix->k * PDIM, acct);
-        hipLaunchKernelGGL(k_rotate_rows, dim3(ix->k), dim3(64), 0, st, ix->cents, ix->rotation, 1.0f / 4096.0f,
-                           l->cents_rot);
-        const size_t R = l->cap_rows;
-        l->rows = dmalloc<int8_t>(R * DIM, acct);
-        l->norm256 = dmalloc<float>(R, acct);
-        l->t2 = dmalloc<uint8_t>(R * 64, acct);
-        l->t2n = dmalloc<float>(R, acct);
-        l->ext = dmalloc<uint32_t>(R, acct);
-        l->deleted_words = (l->n + R + 63) / 64;
-        l->deleted = dmalloc<uint64_t>(l->deleted_words, acct);
-        SIEVE_HIP(hipMemsetAsync(l->deleted, 0, l->deleted_words * 8, st));
-        l->deleted_host.assign(l->deleted_words, 0);
-        l->lcode = dmalloc<uint4>(2 * 2 * R, acct);
-        l->lmul = dmalloc<float>(2 * R, acct);
-        l->lerr = dmalloc<float>(2 * R, acct);
-        l->lrow = dmalloc<uint32_t>(2 * R, acct);
-        l->lcl = dmalloc<uint32_t>(2 * R, acct);
-        l->dblock_off = dmalloc<uint32_t>(ix->k + 1, acct);
-        SIEVE_HIP(hipGetLastError());
-        rebuild_arena(l.get(), st);
-        ix->live = l.get();
-        *out = l.release();
+        SIEVE_REQUIRE(ix->seg_rows == 0, SIEVE_EINVAL,
+                      "the index's lists hold rows of its live segment (live.*): open it with sieve_live_open_dir");
+        *out = live_create(ix, o ? *o : sieve_live_opts{});
     });
 }
 
@@ -586,11 +559,24 @@ int sieve_live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, siz
     });
 }
 
+// Live::delete (src/live.rs:296-306): logs the records, then applies them.
 int sieve_live_delete(sieve_live* l, const uint32_t* ext, size_t n) {
     return guarded([&] {
         SIEVE_REQUIRE(l && (n == 0 || ext), SIEVE_EINVAL, "null argument");
         SIEVE_HIP(hipSetDevice(l->ix->device));
-        live_delete(l, ext, n);
+        std::vector<uint32_t> he;
+        const uint32_t* eh = ext;
+        if (n && on_device(ext)) {
+            he.resize(n);
+            SIEVE_HIP(hipMemcpy(he.data(), ext, n * 4, hipMemcpyDeviceToHost));
+            eh = he.data();
+        }
+        if (l->wal_fd >= 0 && n) {
+            std::vector<uint8_t> rec(n * WAL_DELETE_BYTES);
+            for (size_t j = 0; j < n; j++) wal_delete_record(eh[j], &rec[j * WAL_DELETE_BYTES]);
+            live_log(l, rec);
+        }
+        live_apply_delete(l, eh, n);
     });
 }
 

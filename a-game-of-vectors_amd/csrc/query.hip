@@ -1,8 +1,8 @@
 // Per-query preparation and index re-layout kernels.
 //
 // Query prep restates, per query, rabitq::Query::new (src/rabitq.rs:119-125): the rotation
-// (src/rabitq.rs:59-84), the u8 LUT (lut::Lut::new, src/lut.rs:58-81) and the affine map
-// (a, b); and the int8 rerank queries (quant::query_i8, src/quant.rs:102-110, over
+// (src/rabitq.rs:59-84), the u8 LUT (lut::Lut::new, src/lut.rs:33-56) and the affine map
+// (a, b); and the int8 rerank queries (quant::query_i8, src/quant.rs:104-111, over
 // w_i = q_i / scale_i, src/ivf.rs:753-755 and :786-788). Every f32 operation is the
 // reference's, in the reference's order, and the library is built with -ffp-contract=off, so
 // tables, a, b and the int8 queries are bit-identical to the reference's.
@@ -119,7 +119,7 @@ void launch_query_prep(const float* q256, const float* q768, size_t nq, const fl
 // ------------------------------------------------------------------ re-layout --------
 // blocks.bin (byte g*16+k = doc k nibble g | doc k+16 nibble g << 4) -> G/2 code bytes per
 // slot, so a lane reads its slot's code with 16-byte loads: G = 32 codes raw (bit i of dim i at
-// byte i/8, src/quant.rs:22-32), G = 64 codes lane-chunked (code_word_group). One workgroup of
+// byte i/8, src/quant.rs:24-34), G = 64 codes lane-chunked (code_word_group). One workgroup of
 // 4G threads per block; thread t writes code word t % (G / 8) of slot t / (G / 8).
 template <int G>
 __global__ __launch_bounds__(4 * G) void k_transpose_blocks(const uint8_t* __restrict__ blocks, size_t nblocks,

@@ -9,7 +9,7 @@
 //   ssd(r): the best r by estimate, then full;
 //   tiered(r): the best r by the 2-bit tier score (Tier2::tables / score, src/tier2.rs:52-75)
 //         when the index has a 2-bit tier, else by the prefix score; then full;
-//   final: the best k by (score desc, row asc) (TopK::into_sorted, src/topk.rs:57-61).
+//   final: the best k by (score desc, row asc) (TopK::into_sorted, src/topk.rs:47-66).
 // Integer dots run on v_dot4_i32_i8 with 16 lanes per row (3 x 16 coalesced bytes per lane),
 // and are exact; the f32 steps after them are the reference's, in its order.
 #include "kernels.hpp"

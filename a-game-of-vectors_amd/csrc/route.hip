@@ -8,7 +8,7 @@
 //    nprobe); every centroid whose approximate score lies within twice the BF16
 //    error bound of it is rescored in f32 exactly as the reference's gemm_abt fallback sums
 //    (src/blas.rs:47-53: products left to right from -0.0, no FMA); the rescored scores are
-//    sorted (score desc, id asc) as TopK::into_sorted (src/topk.rs:57-61); then the margin
+//    sorted (score desc, id asc) as TopK::into_sorted (src/topk.rs:47-66); then the margin
 //    and budget cuts (src/ivf.rs:598-611).
 //
 // Error bound: BF16 round-to-nearest has unit roundoff u = 2^-8, so each product carries a

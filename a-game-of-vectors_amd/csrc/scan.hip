@@ -1,5 +1,5 @@
 // The scan: Index::candidates (src/ivf.rs:643-729) for a batch, plus the standalone lut::scan
-// seam (src/lut.rs:121-152).
+// seam (src/lut.rs:96-127).
 //
 // Work items. A query's probed clusters, taken in probe order, form one sequence of blocks;
 // it is cut into C_q = clamp(ceil(T_q / chunk_blocks), 1, chunk_max) chunks of nearly equal
@@ -8,7 +8,7 @@
 //   * score: the 64 nibbles of the slot's 32-byte code index the query's 64 16-entry u8
 //     tables, held in LDS (all lanes of a wave read the same 16-byte table row at a time, so
 //     the reads are broadcast and conflict-free); the u16 sum equals score_code
-//     (src/lut.rs:83-86) exactly;
+//     (src/lut.rs:58-61) exactly;
 //   * estimate: cent + mul * (a * sc + b), f32, no FMA (rabitq::Query::estimate,
 //     src/rabitq.rs:127-130);
 //   * admission: rejected if (est, slot) is not above the threshold, if the slot is padding,
@@ -656,7 +656,7 @@ void launch_scan_l2(const ScanArgs& a, size_t nq, int variant, hipStream_t st) {
 // the queries that probe it and scores the tile with int8 MFMA: for 16 slots x 16 queries x 4
 // groups per v_mfma_i32_16x16x64_i8, A holds the one-hot bytes of each slot's nibble per group
 // and B the query's table entries minus 128, so the product sums T[g][nibble_g] - 128 over the
-// 64 groups: score_code (src/lut.rs:83-86) minus 64 * 128, exactly (|sum| <= 8192 in i32).
+// 64 groups: score_code (src/lut.rs:58-61) minus 64 * 128, exactly (|sum| <= 8192 in i32).
 //
 // The pass needs each query's admission threshold up front. The seed pass (k_scan_t in
 // SCAN_SEED mode) scans the query's nearest probes (whole clusters, at least seed_slots slots)

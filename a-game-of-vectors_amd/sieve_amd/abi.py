@@ -93,6 +93,16 @@ class LiveInfo(ctypes.Structure):
                                                "device_bytes")]
 
 
+class ReplayStats(ctypes.Structure):
+    _fields_ = [("inserts", ctypes.c_uint64), ("deletes", ctypes.c_uint64), ("torn", ctypes.c_uint64),
+                ("seconds", ctypes.c_double)]
+
+
+class CompactStats(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_uint64), ("dropped", ctypes.c_uint64), ("blocks", ctypes.c_uint64),
+                ("seconds", ctypes.c_double)]
+
+
 class ServerOpts(ctypes.Structure):
     _fields_ = [("max_batch", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
                 ("live", ctypes.c_void_p)]
@@ -193,6 +203,10 @@ def lib():
         "sieve_filter_search_batch": (i32, [vp, vp, u32, ctypes.c_uint64, vp, vp, vp, sz, P, vp, vp, vp,
                                             ctypes.POINTER(u32), T]),
         "sieve_live_open": (i32, [vp, ctypes.POINTER(LiveOpts), ctypes.POINTER(vp)]),
+        "sieve_live_open_dir": (i32, [vp, ctypes.POINTER(LiveOpts), ctypes.c_char_p, ctypes.c_char_p,
+                                      ctypes.POINTER(ReplayStats), ctypes.POINTER(vp)]),
+        "sieve_live_compact": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(CompactStats)]),
+        "sieve_live_truncate_wal": (i32, [vp]),
         "sieve_live_close": (None, [vp]),
         "sieve_live_insert": (i32, [vp, vp, vp, sz]),
         "sieve_live_delete": (i32, [vp, vp, sz]),

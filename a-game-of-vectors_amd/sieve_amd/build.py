@@ -10,7 +10,7 @@ is given (the MI355X for large corpora). It follows the reference's recipe:
            (src/ivf.rs:188-225), 2-bit tier fit/encode (src/tier2.rs:23-50), assign_top(16) +
            SOAR secondary (src/ivf.rs:137-186), Rotation(11) + RaBitQ residual codes per
            (row, cluster) (src/rabitq.rs:86-108), cluster-major 32-slot blocks
-           (src/ivf.rs:352-391, src/lut.rs:98-108).
+           (src/ivf.rs:352-391, src/lut.rs:73-83).
 
 Element-wise steps reproduce the reference's f32 arithmetic exactly (sums run sequentially in
 the reference's order; no fused multiply-add), so the codes and [mul, err] factors written
@@ -19,6 +19,7 @@ cluster assignment run on torch's f32 GEMM, whose summation order differs from t
 reference's sgemm; rows within rounding of a tie between two centroids may be assigned
 differently.
 """
+import json
 import os
 from dataclasses import dataclass, field
 
@@ -87,7 +88,7 @@ def round_half_away(x):
 
 
 def pack_bits(bits):
-    """bits (n x d bool), bit i -> byte i/8 position i%8 (src/quant.rs:22-32)."""
+    """bits (n x d bool), bit i -> byte i/8 position i%8 (src/quant.rs:24-34)."""
     n, d = bits.shape
     w = (1 << torch.arange(8, device=bits.device, dtype=torch.int32))
     return (bits.view(n, d // 8, 8).to(torch.int32) * w).sum(-1).to(torch.uint8)
@@ -156,7 +157,7 @@ def rabitq_encode(xr, cr):
 
 # ----------------------------------------------------------------------- pack ----------
 def pack_blocks(codes, G):
-    """lut::pack_block (src/lut.rs:98-108) for slots laid out 32 per block: codes is
+    """lut::pack_block (src/lut.rs:73-83) for slots laid out 32 per block: codes is
     (nblocks*32) x (G/2) bytes, returns nblocks x (G*16) bytes."""
     S = codes.shape[0]
     nb = S // BLOCK
@@ -284,7 +285,7 @@ def tier2_fit(sample):
 
 
 def tier2_encode(prefix, sigma):
-    """Tier2::encode (src/tier2.rs:39-50): (codes n x 64, reconstruction norms n)."""
+    """Tier2::encode (src/tier2.rs:36-47): (codes n x 64, reconstruction norms n)."""
     t = np.float32(T2_T) * sigma
     lv = torch.where(prefix < -t, 0, torch.where(prefix < 0, 1, torch.where(prefix < t, 2, 3)))
     levels = torch.tensor(T2_LEVELS, dtype=torch.float32, device=prefix.device)
@@ -313,6 +314,7 @@ class IndexArrays:
     tier2_codes: torch.Tensor = None
     tier2_norm: torch.Tensor = None
     sizes: list = field(default_factory=list)
+    soar_lambda: float = 1.0    # None: built without SOAR spills
 
     @property
     def k(self):
@@ -419,7 +421,7 @@ def build(corpus: Corpus, k, sample_n=None, iters=12, soar_lambda=1.0, rabitq=Tr
     return IndexArrays(centroids=cents, block_off=block_off, blocks=blocks, ids=ids,
                        factors=factors if rabitq else None, rotation=signs if rabitq else None, corpus=corpus,
                        tier2_sigma=t2_sigma, tier2_codes=t2_codes, tier2_norm=t2_norm,
-                       sizes=sizes.cpu().tolist())
+                       sizes=sizes.cpu().tolist(), soar_lambda=soar_lambda)
 
 
 # ----------------------------------------------------------------------- ground truth --
@@ -464,6 +466,12 @@ def write_index(arrays: IndexArrays, index_dir, ingest_dir, queries=None, gt=Non
         _w(os.path.join(index_dir, "tier2.sigma"), a["tier2_sigma"])
         _w(os.path.join(index_dir, "tier2.bin"), a["tier2_codes"])
         _w(os.path.join(index_dir, "tier2.norm"), a["tier2_norm"])
+    # meta.json as src/bin/index.rs:53-60 writes it (serde_json's sorted keys, two-space indent);
+    # Live::open reads its soar_lambda (src/live.rs:170-171).
+    meta = {"k": int(arrays.k), "rabitq": arrays.factors is not None, "rows": int(arrays.corpus.n),
+            "soar_lambda": None if arrays.soar_lambda is None else float(arrays.soar_lambda)}
+    with open(os.path.join(index_dir, "meta.json"), "w") as f:
+        f.write(json.dumps(meta, indent=2, sort_keys=True))
     _w(os.path.join(ingest_dir, "rerank.i8"), a["rerank"])
     _w(os.path.join(ingest_dir, "quant.f32"), a["quant"])
     _w(os.path.join(ingest_dir, "norm256.f32"), a["norm256"])

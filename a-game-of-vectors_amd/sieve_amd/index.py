@@ -278,12 +278,34 @@ class Live:
     """Live updates over an Index (src/live.rs): insert / delete / upsert by external id, and
     searches over the base index, the delta lists and the tombstones."""
 
-    def __init__(self, index, max_rows, soar_lambda=1.0):
+    def __init__(self, index, max_rows, soar_lambda=1.0, index_dir=None, wal=None):
+        """Live::open (src/live.rs:166-207). Without index_dir and wal: an index from arrays with
+        the given soar_lambda. With index_dir (the directory the index was opened from):
+        soar_lambda from its meta.json and a compacted index's row segment; with wal: the log
+        replayed (self.replay holds its ReplayStats) and appended to from then on."""
         self._ix = index
         o = abi.LiveOpts(max_rows=max_rows, soar_lambda=float("nan") if soar_lambda is None else soar_lambda)
         h = ctypes.c_void_p()
-        abi.check(abi.lib().sieve_live_open(index._h, ctypes.byref(o), ctypes.byref(h)))
+        self.replay = None
+        if index_dir is None and wal is None:
+            abi.check(abi.lib().sieve_live_open(index._h, ctypes.byref(o), ctypes.byref(h)))
+        else:
+            st = abi.ReplayStats()
+            abi.check(abi.lib().sieve_live_open_dir(index._h, ctypes.byref(o),
+                                                    None if index_dir is None else str(index_dir).encode(),
+                                                    None if wal is None else str(wal).encode(), ctypes.byref(st),
+                                                    ctypes.byref(h)))
+            self.replay = {f: getattr(st, f) for f, _ in st._fields_}
         self._h = h
+
+    def compact(self, out_dir):
+        """Live::compact (src/live.rs:395-468) into out_dir: its CompactStats."""
+        st = abi.CompactStats()
+        abi.check(abi.lib().sieve_live_compact(self._h, str(out_dir).encode(), ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in st._fields_}
+
+    def truncate_wal(self):
+        abi.check(abi.lib().sieve_live_truncate_wal(self._h))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-/* Compile-time widths (src/lib.rs:21-26, src/lut.rs:42-44, src/ivf.rs:18). */
+/* Compile-time widths (src/lib.rs:21-26, src/lut.rs:16-18, src/ivf.rs:18). */
 #define SIEVE_DIM 768
 #define SIEVE_PDIM 256
 #define SIEVE_CODE_BYTES 32
@@ -87,7 +87,7 @@ typedef struct sieve_timing {
  * (src/ivf.rs:354-391) and the ingest output (src/bin/ingest.rs:99-166):
  *   centroids  k x 256 f32           centroids.f32
  *   block_off  k + 1 u64             block_off.u64
- *   blocks     nblocks x 1024 B      blocks.bin   (nibble-transposed, src/lut.rs:98-108)
+ *   blocks     nblocks x 1024 B      blocks.bin   (nibble-transposed, src/lut.rs:73-83)
  *   ids        nblocks x 32 u32      ids.u32      (PAD = 0xFFFFFFFF)
  *   factors    nblocks x 32 x 2 f32  factors.f32  ([mul, err] per slot; NULL = global
  *                                                  sign codes, no RaBitQ)
@@ -197,7 +197,7 @@ int sieve_gemm_abt(const float* a, const float* b, size_t m, size_t n, size_t k,
 
 /* Scores every 32-slot block of nibble-transposed codes against G 16-entry u8 tables:
  * out[32*b + j] = sum_g tables[g][nibble_g(slot j of block b)].
- * Replaces lut::scan (src/lut.rs:121-128); bit-exact with score_code (src/lut.rs:83-86).
+ * Replaces lut::scan (src/lut.rs:96-103); bit-exact with score_code (src/lut.rs:58-61).
  * groups is 64 (256-bit codes) or 32 (128-bit codes, src/bigann.rs:53-56). */
 int sieve_lut_scan(const uint8_t* tables, uint32_t groups, const uint8_t* blocks,
                    size_t nblocks, uint16_t* out);
@@ -205,7 +205,9 @@ int sieve_lut_scan(const uint8_t* tables, uint32_t groups, const uint8_t* blocks
 /* ------------------------------------------------------------ the index --- */
 
 /* Index::open (src/ivf.rs:553-578) + Corpus::open (src/ivf.rs:75-90): reads the build
- * directory and the ingest directory and uploads them to the current device. */
+ * directory and the ingest directory and uploads them to the current device. The directory of
+ * a compacted index (sieve_live_compact) holds a row segment (live.*) whose rows its lists
+ * reference: such an index is searched through the live state sieve_live_open_dir opens. */
 int sieve_index_open(const char* index_dir, const char* ingest_dir, sieve_index** out);
 /* The same from arrays already in memory (host or device). */
 int sieve_index_create(const sieve_index_desc* desc, sieve_index** out);
@@ -282,7 +284,7 @@ int sieve_rerank(const sieve_index* ix, sieve_stream* s, const float* q768, cons
                  const sieve_view* v, float* scores, uint32_t* out_rows, uint32_t* counts,
                  sieve_timing* t);
 
-/* Index::search / search_view (src/ivf.rs:620-633) for a batch: route, scan and rerank
+/* Index::search / search_view (src/ivf.rs:627-636) for a batch: route, scan and rerank
  * of nq queries in one call, all stages on the device. Output as sieve_rerank. */
 int sieve_search_batch(const sieve_index* ix, sieve_stream* s, const float* q768,
                        const float* q256, size_t nq, const sieve_params* p,
@@ -349,7 +351,7 @@ int sieve_server_submit(sieve_server* s, const float* q768, const float* q256, u
 int sieve_server_submit_filtered(sieve_server* s, const float* q768, const float* q256,
                                  const struct sieve_filter* filter, uint32_t plan, uint64_t* ticket);
 /* Live writes through the server (needs opts.live): Live::insert / Live::delete
- * (src/live.rs:286-326) of n rows (ext ids, n x 768 unit vectors in host memory). A write is a
+ * (src/live.rs:275-315) of n rows (ext ids, n x 768 unit vectors in host memory). A write is a
  * barrier in submission order: it is applied once every earlier search has finished, and every
  * later search sees it. sieve_server_wait on its ticket returns when it is applied (count 0). */
 int sieve_server_insert(sieve_server* s, const uint32_t* ext, const float* vecs, size_t n, uint64_t* ticket);
@@ -413,9 +415,9 @@ typedef struct sieve_l2_index sieve_l2_index;
 
 int sieve_l2_index_create(const sieve_l2_desc* desc, sieve_l2_index** out);
 void sieve_l2_index_free(sieve_l2_index* ix);
-/* bigann::Index::open (src/bigann.rs:620-628) over the reference's files: Model::open
+/* bigann::Index::open (src/bigann.rs:634-641) over the reference's files: Model::open
  * (src/bigann.rs:257-266: centroids.f32, coarse.f32, k2.u64, rotation.f32, optional bits.u64),
- * Postings::open_ext (src/postings.rs:243-262: postings.off, block_off.u64, postings.bin with
+ * Postings::open_ext (src/postings.rs:209-229: postings.off, block_off.u64, postings.bin with
  * 4 KiB-aligned records of code blocks, [mul, |x|^2], ids and the refinement section) in
  * index_dir, and Base::open (src/bigann.rs:477-489: base.00, base.01, ... with .done markers,
  * 100,000,000 rows each) in base_dir; everything is uploaded to the current device. */
@@ -483,14 +485,16 @@ int sieve_filter_search_batch(const sieve_index* ix, const sieve_filter* f, uint
                               float* scores, uint32_t* rows, uint32_t* counts, uint32_t* resolved, sieve_timing* t);
 
 /* ------------------------------------------------------------ live updates --- */
-/* Live (src/live.rs:146-447): inserts, deletes and upserts over a built index without a
+/* Live (src/live.rs:151-480): inserts, deletes and upserts over a built index without a
  * rebuild. Inserted rows are encoded on the device exactly as the build encodes a row and
  * appended to per-cluster delta lists that a live search scans right after each probed
  * cluster's base list; deletes are tombstones. Row ids: corpus rows keep theirs, delta rows
- * count up from the corpus size; live searches report external ids. The write-ahead log and
- * compaction (src/live.rs:212-263, :403-447) are not part of this library: the caller owns
- * durability. Like the reference's &mut Live, inserts and deletes must not overlap searches of
- * the same index, and an index has at most one live state. */
+ * count up from the corpus size; live searches report external ids. Durability as the
+ * reference's: with a write-ahead log (sieve_live_open_dir) every insert and delete is logged
+ * and made durable before it is applied, a restart replays the log, and compaction writes a new
+ * index directory with the delta lists merged into the base lists. Like the reference's
+ * &mut Live, inserts and deletes must not overlap searches of the same index, and an index has
+ * at most one live state. */
 typedef struct sieve_live sieve_live;
 
 typedef struct sieve_live_opts {
@@ -508,17 +512,56 @@ typedef struct sieve_live_info {
     uint64_t device_bytes;  /* held by the live state (the index's slot arrays grow as well) */
 } sieve_live_info;
 
-/* Live::open (src/live.rs:175-210) without a log: the rotated centroids, room for max_rows
- * delta rows and their slots (the index's slot arrays are re-allocated with room for them). */
+typedef struct sieve_replay_stats {  /* Live::replay's ReplayStats (src/live.rs:137-143) */
+    uint64_t inserts;
+    uint64_t deletes;
+    uint64_t torn;     /* bytes of a torn final record, cut off the log */
+    double seconds;
+} sieve_replay_stats;
+
+typedef struct sieve_compact_stats {  /* Live::compact's CompactStats (src/live.rs:145-149) */
+    uint64_t slots;    /* slots of the new lists (padding aside) */
+    uint64_t dropped;  /* tombstoned slots left out */
+    uint64_t blocks;   /* 32-slot blocks of the new lists */
+    double seconds;
+} sieve_compact_stats;
+
+/* Live::open (src/live.rs:166-207) without a directory or a log: the rotated centroids, room for
+ * max_rows delta rows and their slots (the index's slot arrays are re-allocated with room for
+ * them). Refused for a compacted index (its lists reference its row segment). */
 int sieve_live_open(sieve_index* ix, const sieve_live_opts* o, sieve_live** out);
+/* Live::open (src/live.rs:166-207) of an index opened from index_dir: soar_lambda is read from
+ * index_dir/meta.json (o->soar_lambda is not used; null or absent: no SOAR spill); a compacted
+ * index's row segment (live.rows.i8, live.norm256.f32, live.t2.bin, live.t2.norm, live.ext.u32,
+ * live.deleted.u64) is loaded, and its rows count against o->max_rows. With wal_path, the log is
+ * replayed (Live::replay, src/live.rs:209-249: consecutive inserts as one batch, a torn final
+ * record cut off the file, an unknown op an SIEVE_EIO error naming its byte offset), *replay
+ * filled (if not NULL), and every later sieve_live_insert / sieve_live_delete appends its records
+ * (op u8 = 1 insert / 2 delete, external id u32 LE, then for an insert the prefix norm f32 LE
+ * and the 768 int8 row bytes) and waits for fdatasync before it is applied. index_dir NULL: an
+ * index created from arrays, with o->soar_lambda as given. */
+int sieve_live_open_dir(sieve_index* ix, const sieve_live_opts* o, const char* index_dir, const char* wal_path,
+                        sieve_replay_stats* replay, sieve_live** out);
+/* Live::compact (src/live.rs:395-468): writes a new index directory out_dir (created if needed;
+ * it must not be the directory the index was opened from): per cluster the base list's then the
+ * delta list's slots, padding and tombstoned slots left out, re-packed into blocks (blocks.bin,
+ * factors.f32, ids.u32, block_off.u64), the same records as postings.bin / postings.off
+ * (postings::pack, src/postings.rs:89-114), meta.json (the source's plus compacted_from,
+ * live_rows, deleted), the row segment (live.*), and centroids.f32, rotation.f32 and tier2.*
+ * symlinked to the source directory's (written from the device for an index from arrays).
+ * sieve_index_open + sieve_live_open_dir on out_dir give the same searches as this live state. */
+int sieve_live_compact(const sieve_live* l, const char* out_dir, sieve_compact_stats* stats);
+/* Live::truncate_wal (src/live.rs:470-479): empties the log (after a compaction made its
+ * records redundant) and syncs it. No-op without a log. */
+int sieve_live_truncate_wal(sieve_live* l);
 void sieve_live_close(sieve_live* l);
-/* Live::insert (src/live.rs:286-306): inserts or replaces ext[i] with the unit 768-d vector
+/* Live::insert (src/live.rs:275-294): inserts or replaces ext[i] with the unit 768-d vector
  * vecs[i] (n x 768, host or device memory). A live external id's old row is tombstoned. */
 int sieve_live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, size_t n);
-/* Live::delete (src/live.rs:308-326): tombstones the rows of external ids; unknown ids are
+/* Live::delete (src/live.rs:296-315): tombstones the rows of external ids; unknown ids are
  * ignored. */
 int sieve_live_delete(sieve_live* l, const uint32_t* ext, size_t n);
-/* Live::current (src/live.rs:270-277): the row id an external id resolves to, or 0xFFFFFFFF. */
+/* Live::current (src/live.rs:260-266): the row id an external id resolves to, or 0xFFFFFFFF. */
 uint32_t sieve_live_current(const sieve_live* l, uint32_t ext);
 int sieve_live_get_info(const sieve_live* l, sieve_live_info* out);
 /* Copies the delta segment to host arrays (NULL to skip): the delta rows (rows x 768 int8),

@@ -14,30 +14,30 @@ Parity status of each oracle function and what pins it (tests/test_oracle_*.py):
   gemm_abt        (blas.rs:43-54)    pinned: blas.rs:58-65 known answer; float64 bound;
                                       bit-exact against a sequential float32 numpy sum
   Rng             (ivf.rs:120-135)   pinned: published splitmix64 output for seed 0/1234567
-  lut_new         (lut.rs:58-81)     pinned: lut.rs:223-234 float bound; table invariants
+  lut_new         (lut.rs:33-56)     pinned: lut.rs:198-209 float bound; table invariants
                                       (min entry 0 per group, 255 in the widest group)
-  score_code/scan (lut.rs:84-152)    pinned: lut.rs:204-221, :237-266 (kernel == score_code,
+  score_code/scan (lut.rs:59-127)    pinned: lut.rs:179-196, :212-241 (kernel == score_code,
                                       G=32 fixture, partial block); brute-force bit sum
-  pack/unpack     (lut.rs:98-119)    pinned: round trip (lut.rs:246-252); bit-level brute force
-  hamming         (lut.rs:182-191)   pinned: lut.rs:268-273 known answer; numpy popcount
+  pack/unpack     (lut.rs:73-94)    pinned: round trip (lut.rs:221-227); bit-level brute force
+  hamming         (lut.rs:157-166)   pinned: lut.rs:243-248 known answer; numpy popcount
   fwht/rotation   (rabitq.rs:34-84)  pinned: explicit Hadamard matrix product (float64),
-                                      orthogonality rabitq.rs:147-157
-  rabitq_encode   (rabitq.rs:86-108) pinned: closed form of mul/err in float64;
-                                      estimate bound rabitq.rs:159-187
+                                      orthogonality rabitq.rs:147-158
+  rabitq_encode   (rabitq.rs:88-110) pinned: closed form of mul/err in float64;
+                                      estimate bound rabitq.rs:159-188
   query_new/estimate (rabitq.rs:112-131) pinned: estimator identity
                                       a*lut+b == sum(sign_i * (Pq)_i) within LUT rounding
-  normalize/sign_code/int8 (quant.rs) pinned: quant.rs:147-171 known answers
-  query_i8/dot_i8i8 (quant.rs:102-125) pinned: integer identity vs numpy int64
-  TopK            (topk.rs:4-62)     pinned: topk.rs:257-270 (== full sort)
-  tier2           (tier2.rs)         pinned: tier2.rs:326-341 (table score == reconstruction)
-  prune_threshold (ivf.rs:540-550)   pinned: ivf.rs:856-868 known answer
-  route           (ivf.rs:587-613)   pinned: float64 brute-force probe set (up to f32 ties)
-  candidates      (ivf.rs:643-729)   pinned: brute-force estimate over all probed slots,
+  normalize/sign_code/int8 (quant.rs) pinned: quant.rs:147-172 known answers
+  query_i8/dot_i8i8 (quant.rs:104-135) pinned: integer identity vs numpy int64
+  TopK            (topk.rs:4-62)     pinned: topk.rs:69-82 (== full sort)
+  tier2           (tier2.rs)         pinned: tier2.rs:81-93 (table score == reconstruction)
+  prune_threshold (ivf.rs:542-551)   pinned: ivf.rs:856-868 known answer
+  route           (ivf.rs:589-614)   pinned: float64 brute-force probe set (up to f32 ties)
+  candidates      (ivf.rs:647-733)   pinned: brute-force estimate over all probed slots,
                                       top-cand by full sort, dedup by row
-  rerank          (ivf.rs:731-815)   pinned: brute-force int64 dots + full sort per mode
-  l2 search       (bigann.rs:652-747) pinned: brute-force L2 estimate/rerank by full sort
-  assign_top/soar (ivf.rs:137-186)   pinned: float64 brute force (up to near-ties)
-  Filter/Plan     (filter.rs)        pinned: filter.rs:157-165 (auto plan); per-cluster
+  rerank          (ivf.rs:734-818)   pinned: brute-force int64 dots + full sort per mode
+  l2 search       (bigann.rs:653-747) pinned: brute-force L2 estimate/rerank by full sort
+  assign_top/soar (ivf.rs:138-192)   pinned: float64 brute force (up to near-ties)
+  Filter/Plan     (filter.rs)        pinned: filter.rs:157-166 (auto plan); per-cluster
                                       counts and row lists by numpy; each plan's result
                                       against brute force over the allowed rows
   Live            (live.rs)          pinned: live.rs:493-520 (delta list layout, bitset);
@@ -45,6 +45,12 @@ Parity status of each oracle function and what pins it (tests/test_oracle_*.py):
                                       of that row (live.rs:2-6, bit for bit); upsert/delete
                                       semantics; live search == search over a rebuilt index
                                       holding the same rows (brute force, up to ties)
+  Live log        (live.rs:131-135, :209-258, :470-479) pinned: record layout against bytes
+                                      written by hand; replay == the writes applied directly
+                                      (state and searches); torn tails cut, bad ops refused
+  Live compact    (live.rs:395-468)  pinned: the merged lists against a rebuild of the same
+                                      live rows' slots by numpy (order, padding, drops);
+                                      searches of the compacted index == live searches
 """
 import ctypes
 import os
@@ -161,8 +167,12 @@ def lib():
             "or_plan_resolve": (u32, [u32, u64, u64]),
             "or_filter_search": (u32, [vp, vp, u32, u64, vp, vp, ctypes.POINTER(_Params), vp, vp, vp]),
             "or_live_free": (None, [vp]),
-            "or_live_insert": (None, [vp, vp, vp, sz]),
-            "or_live_delete": (None, [vp, vp, sz]),
+            "or_live_insert": (i32, [vp, vp, vp, sz]),
+            "or_live_delete": (i32, [vp, vp, sz]),
+            "or_live_open_wal": (i32, [vp, ctypes.c_char_p, vp]),
+            "or_live_truncate_wal": (i32, [vp]),
+            "or_live_load_segment": (None, [vp, vp, vp, vp, vp, vp, sz, vp, sz]),
+            "or_live_compact": (sz, [vp, sz, vp, vp, vp, vp, vp]),
             "or_live_current": (u32, [vp, u32]),
             "or_live_rows": (sz, [vp]),
             "or_live_row": (None, [vp, sz, vp, vp, vp, vp, vp]),
@@ -623,11 +633,53 @@ class Live:
 
     def insert(self, ext, vecs):
         ext, vecs = _c(ext, np.uint32), _c(vecs, np.float32)
-        lib().or_live_insert(self._h, _p(ext), _p(vecs), ext.shape[0])
+        if lib().or_live_insert(self._h, _p(ext), _p(vecs), ext.shape[0]) != 0:
+            raise OSError("oracle Live.insert: log write failed")
 
     def delete(self, ext):
         ext = _c(ext, np.uint32)
-        lib().or_live_delete(self._h, _p(ext), ext.shape[0])
+        if lib().or_live_delete(self._h, _p(ext), ext.shape[0]) != 0:
+            raise OSError("oracle Live.delete: log write failed")
+
+    def open_wal(self, path):
+        """Live::replay of `path`, then appends every later write to it: {inserts, deletes, torn}.
+        Raises ValueError (with the byte offset) on an unknown op."""
+        st = np.zeros(4, np.uint64)
+        rc = lib().or_live_open_wal(self._h, str(path).encode(), _p(st))
+        if rc == -1:
+            raise ValueError(f"{path}: bad op at byte {int(st[3])}")
+        if rc != 0:
+            raise OSError(f"{path}: replay failed")
+        return dict(inserts=int(st[0]), deletes=int(st[1]), torn=int(st[2]))
+
+    def truncate_wal(self):
+        if lib().or_live_truncate_wal(self._h) != 0:
+            raise OSError("truncate_wal failed")
+
+    def load_segment(self, rows, norm256, t2, t2n, ext, deleted):
+        """The row segment of a compacted index (live.* files) as Live::open reads it."""
+        rows, norm256, ext = _c(rows, np.int8), _c(norm256, np.float32), _c(ext, np.uint32)
+        deleted = _c(deleted, np.uint64)
+        has_t2 = t2 is not None and len(t2) > 0
+        t2 = _c(t2, np.uint8) if has_t2 else None
+        t2n = _c(t2n, np.float32) if has_t2 else None
+        self._seg = (rows, norm256, t2, t2n, ext, deleted)
+        lib().or_live_load_segment(self._h, _p(rows), _p(norm256), _p(t2) if has_t2 else None,
+                                   _p(t2n) if has_t2 else None, _p(ext), ext.shape[0], _p(deleted),
+                                   deleted.shape[0])
+
+    def compact(self):
+        """Live::compact's merged lists: dict(block_off u64 k+1, blocks nb x 1024, factors
+        nb*32 x 2, ids nb*32, slots, dropped)."""
+        st = np.zeros(2, np.uint64)
+        nb = lib().or_live_compact(self._h, 0, None, None, None, None, _p(st))
+        k = self._ix.k
+        off = np.zeros(k + 1, np.uint64)
+        b = np.zeros((nb, 1024), np.uint8)
+        f = np.zeros((nb * BLOCK, 2), np.float32)
+        i = np.zeros(nb * BLOCK, np.uint32)
+        lib().or_live_compact(self._h, nb, _p(off), _p(b), _p(f), _p(i), _p(st))
+        return dict(block_off=off, blocks=b, factors=f, ids=i, slots=int(st[0]), dropped=int(st[1]))
 
     def current(self, ext):
         r = lib().or_live_current(self._h, int(ext))

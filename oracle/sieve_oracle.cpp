@@ -16,11 +16,15 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
+#include <fcntl.h>
 #include <limits>
 #include <memory>
 #include <thread>
 #include <unordered_map>
+#include <string>
+#include <unistd.h>
 #include <vector>
 
 #include "../include/sieve_gpu.h"
@@ -115,11 +119,11 @@ void gemm_abt(const float* a, const float* b, size_t m, size_t n, size_t k, floa
 }
 
 // ---------------------------------------------------------------- lut -----------------
-inline uint8_t nibble(const uint8_t* code, int g) {  // src/lut.rs:53-56
+inline uint8_t nibble(const uint8_t* code, int g) {  // src/lut.rs:28-30
     return (code[g >> 1] >> ((g & 1) * 4)) & 0x0F;
 }
 
-// Lut::new, src/lut.rs:58-81. tables is G x 16.
+// Lut::new, src/lut.rs:33-56. tables is G x 16.
 void lut_new(const float* q, int G, uint8_t* tables, float* scale_out, float* bias_out) {
     std::vector<float> f((size_t)G * 16);
     for (int g = 0; g < G; g++)
@@ -154,13 +158,13 @@ void lut_new(const float* q, int G, uint8_t* tables, float* scale_out, float* bi
     *bias_out = bias;
 }
 
-uint16_t score_code(const uint8_t* tables, int G, const uint8_t* code) {  // src/lut.rs:84-86
+uint16_t score_code(const uint8_t* tables, int G, const uint8_t* code) {  // src/lut.rs:59-61
     uint16_t s = 0;
     for (int g = 0; g < G; g++) s = (uint16_t)(s + tables[g * 16 + nibble(code, g)]);
     return s;
 }
 
-// pack_block, src/lut.rs:98-108: codes is n x (G/2) bytes, n <= 32, out G x 16 bytes.
+// pack_block, src/lut.rs:73-83: codes is n x (G/2) bytes, n <= 32, out G x 16 bytes.
 void pack_block(const uint8_t* codes, int n, int G, uint8_t* out) {
     int cb = G / 2;
     auto get = [&](int k, int g) -> uint8_t { return k < n ? nibble(codes + (size_t)k * cb, g) : 0; };
@@ -168,7 +172,7 @@ void pack_block(const uint8_t* codes, int n, int G, uint8_t* out) {
         for (int k = 0; k < 16; k++) out[g * 16 + k] = (uint8_t)(get(k, g) | (get(k + 16, g) << 4));
 }
 
-void unpack_slot(const uint8_t* block, int G, int k, uint8_t* code) {  // src/lut.rs:110-119
+void unpack_slot(const uint8_t* block, int G, int k, uint8_t* code) {  // src/lut.rs:85-94
     std::memset(code, 0, G / 2);
     for (int g = 0; g < G; g++) {
         uint8_t b = block[g * 16 + (k & 15)];
@@ -177,7 +181,7 @@ void unpack_slot(const uint8_t* block, int G, int k, uint8_t* code) {  // src/lu
     }
 }
 
-// scan_block_scalar, src/lut.rs:142-152, over nblocks blocks (scan, src/lut.rs:121-128).
+// scan_block_scalar, src/lut.rs:117-127, over nblocks blocks (scan, src/lut.rs:96-103).
 void scan(const uint8_t* tables, int G, const uint8_t* blocks, size_t nblocks, uint16_t* out) {
     for (size_t b = 0; b < nblocks; b++) {
         const uint8_t* blk = blocks + b * (size_t)G * 16;
@@ -278,7 +282,7 @@ float normalize(float* v, int n) {  // src/quant.rs:5-13
     return nn;
 }
 
-void sign_code(const float* p, int n, uint8_t* out) {  // src/quant.rs:22-32
+void sign_code(const float* p, int n, uint8_t* out) {  // src/quant.rs:24-34
     std::memset(out, 0, n / 8);
     for (int i = 0; i < n; i++)
         if (p[i] > 0.0f) out[i >> 3] |= (uint8_t)(1u << (i & 7));
@@ -317,7 +321,7 @@ inline int8_t sat_i8(float v) {  // Rust `f32 as i8`: saturating, NaN -> 0
     return (int8_t)v;
 }
 
-float query_i8(const float* w, int n, int8_t* out) {  // src/quant.rs:102-110
+float query_i8(const float* w, int n, int8_t* out) {  // src/quant.rs:104-111
     float m = 1e-12f;
     for (int i = 0; i < n; i++) m = std::fmax(m, std::fabs(w[i]));
     float s = 127.0f / m;
@@ -346,7 +350,7 @@ void tier2_fit(const float* rows, size_t nrows, float* sigma) {  // src/tier2.rs
     for (int i = 0; i < PDIM; i++) sigma[i] = std::fmax((float)std::sqrt(ss[i] / (double)n), 1e-6f);
 }
 
-inline uint8_t tier2_level(float x, float s) {  // src/tier2.rs:33-37
+inline uint8_t tier2_level(float x, float s) {  // src/tier2.rs:30-34
     float t = T2_T * s;
     if (x < -t) return 0;
     if (x < 0.0f) return 1;
@@ -354,7 +358,7 @@ inline uint8_t tier2_level(float x, float s) {  // src/tier2.rs:33-37
     return 3;
 }
 
-float tier2_encode(const float* sigma, const float* prefix, uint8_t* out) {  // src/tier2.rs:39-50
+float tier2_encode(const float* sigma, const float* prefix, uint8_t* out) {  // src/tier2.rs:36-47
     std::memset(out, 0, PDIM / 4);
     float nn = 0.0f;
     for (int i = 0; i < PDIM; i++) {
@@ -450,7 +454,7 @@ struct DeltaList {
     }
 };
 
-// Rows added since the build (src/live.rs:69-121): delta lists, rows, prefix norms, tier-2
+// Rows added since the build (src/live.rs:68-117): delta lists, rows, prefix norms, tier-2
 // codes and external ids; row ids count up from `base` (the corpus size).
 struct Delta {
     uint64_t base = 0;
@@ -470,7 +474,7 @@ struct Delta {
     }
 };
 
-// View::row and the prefix-norm / tier-2 lookups (src/ivf.rs:485-495, src/live.rs:85-101).
+// View::row and the prefix-norm / tier-2 lookups (src/ivf.rs:485-495, src/live.rs:84-103).
 inline const int8_t* view_row(const Index& ix, const View& v, uint32_t id) {
     size_t i;
     if (v.delta && v.delta->local(id, &i)) return v.delta->rows.data() + i * DIM;
@@ -714,7 +718,7 @@ void or_gemm_abt(const float* a, const float* b, size_t m, size_t n, size_t k, f
 
 void or_lut_new(const float* q, int G, uint8_t* tables, float* scale, float* bias) { lut_new(q, G, tables, scale, bias); }
 uint16_t or_score_code(const uint8_t* tables, int G, const uint8_t* code) { return score_code(tables, G, code); }
-float or_float_score(const float* q, int d, const uint8_t* code) {  // src/lut.rs:93-96
+float or_float_score(const float* q, int d, const uint8_t* code) {  // src/lut.rs:68-71
     float s = rsum_seed();
     for (int i = 0; i < d; i++)
         if ((code[i >> 3] >> (i & 7)) & 1) s = s + q[i];
@@ -725,7 +729,7 @@ void or_unpack_slot(const uint8_t* block, int G, int k, uint8_t* code) { unpack_
 void or_scan(const uint8_t* tables, int G, const uint8_t* blocks, size_t nblocks, uint16_t* out) {
     scan(tables, G, blocks, nblocks, out);
 }
-uint32_t or_hamming(const uint8_t* a, const uint8_t* b, int nbytes) {  // src/lut.rs:182-191
+uint32_t or_hamming(const uint8_t* a, const uint8_t* b, int nbytes) {  // src/lut.rs:157-166
     uint32_t s = 0;
     for (int i = 0; i < nbytes; i++) s += (uint32_t)__builtin_popcount((unsigned)(a[i] ^ b[i]));
     return s;
@@ -881,7 +885,7 @@ size_t or_rerank(const void* h, const float* q768, const float* q256, const floa
     return res.size();
 }
 
-// Index::search_view for one query (src/ivf.rs:620-633).
+// Index::search_view for one query (src/ivf.rs:627-636).
 static size_t search_one(const Index& ix, const float* q768, const float* q256, const sieve_params& p, const View& v,
                          float* out_s, uint32_t* out_r, sieve_timing* t) {
     std::vector<float> cent;
@@ -1318,22 +1322,50 @@ struct OLive {
     const Index* ix;
     bool soar;
     float lambda;
-    std::vector<float> cents_rot;  // k x 256, Rotation::apply of each centroid (src/live.rs:183-192)
+    std::vector<float> cents_rot;  // k x 256, Rotation::apply of each centroid (src/live.rs:172-180)
     Delta delta;
     std::vector<uint64_t> deleted;
     std::unordered_map<uint32_t, uint32_t> by_ext;
+    std::string wal;  // the log (src/live.rs:161), "" without one
+    int wal_fd = -1;
+    ~OLive() {
+        if (wal_fd >= 0) ::close(wal_fd);
+    }
 };
 
-static bool live_bit(const std::vector<uint64_t>& set, uint32_t id) {  // src/live.rs:123-125
+// Live::log (src/live.rs:251-258): the records, then fdatasync.
+static bool live_log(OLive& l, const std::vector<uint8_t>& rec) {
+    if (l.wal_fd < 0) return true;
+    size_t done = 0;
+    while (done < rec.size()) {
+        ssize_t w = ::write(l.wal_fd, rec.data() + done, rec.size() - done);
+        if (w <= 0) return false;
+        done += (size_t)w;
+    }
+    return ::fdatasync(l.wal_fd) == 0;
+}
+
+// OP_INSERT / OP_DELETE and the record sizes, src/live.rs:131-135.
+constexpr uint8_t OP_INSERT = 1, OP_DELETE = 2;
+constexpr size_t INSERT_BYTES = 1 + 4 + 4 + DIM, DELETE_BYTES = 1 + 4;
+
+static void le32_put(std::vector<uint8_t>& v, uint32_t x) {
+    for (int i = 0; i < 4; i++) v.push_back((uint8_t)(x >> (8 * i)));
+}
+static uint32_t le32_get(const uint8_t* p) {
+    return (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24;
+}
+
+static bool live_bit(const std::vector<uint64_t>& set, uint32_t id) {  // src/live.rs:119-121
     size_t w = id >> 6;
     return w < set.size() && ((set[w] >> (id & 63)) & 1) == 1;
 }
-static void live_set_bit(std::vector<uint64_t>& set, uint32_t id) {  // src/live.rs:127-133
+static void live_set_bit(std::vector<uint64_t>& set, uint32_t id) {  // src/live.rs:123-129
     size_t w = id >> 6;
     if (set.size() <= w) set.resize(w + 1, 0);
     set[w] |= 1ull << (id & 63);
 }
-// Live::current, src/live.rs:271-277: the row an external id resolves to.
+// Live::current, src/live.rs:260-266: the row an external id resolves to.
 static bool live_current(const OLive& l, uint32_t e, uint32_t* id) {
     auto it = l.by_ext.find(e);
     if (it != l.by_ext.end()) {
@@ -1346,12 +1378,12 @@ static bool live_current(const OLive& l, uint32_t e, uint32_t* id) {
     }
     return false;
 }
-static uint32_t live_ext_of(const OLive& l, uint32_t id) {  // src/live.rs:279-284
+static uint32_t live_ext_of(const OLive& l, uint32_t id) {  // src/live.rs:268-273
     size_t i;
     return l.delta.local(id, &i) ? l.delta.ext[i] : id;
 }
 
-// Live::apply_insert, src/live.rs:331-379: encode as ivf::build encodes a row, then append.
+// Live::apply_insert, src/live.rs:317-374: encode as ivf::build encodes a row, then append.
 static void live_apply_insert(OLive& l, const uint32_t* ext, const int8_t* rows, const float* norms, size_t n) {
     if (n == 0) return;
     const Index& ix = *l.ix;
@@ -1363,7 +1395,7 @@ static void live_apply_insert(OLive& l, const uint32_t* ext, const int8_t* rows,
     }
     std::vector<uint32_t> pairs(n * 2, PAD);
     if (l.soar) {
-        const size_t m = 16;  // SOAR_CAND, src/live.rs:34
+        const size_t m = 16;  // SOAR_CAND, src/live.rs:35
         std::vector<uint32_t> top(n * m, 0);
         or_assign_top(xs.data(), n, ix.centroids, ix.k, m, top.data(), 1);
         for (size_t j = 0; j < n; j++) {
@@ -1426,9 +1458,9 @@ void* or_live_create(const void* index, int soar, float lambda) {
 }
 void or_live_free(void* h) { delete (OLive*)h; }
 
-// Live::insert, src/live.rs:287-306 (no log): int8 rows by quant.encode, prefix norms of the
+// Live::insert, src/live.rs:275-294 (no log): int8 rows by quant.encode, prefix norms of the
 // float vectors, then apply_insert.
-void or_live_insert(void* h, const uint32_t* ext, const float* vecs, size_t n) {
+int or_live_insert(void* h, const uint32_t* ext, const float* vecs, size_t n) {
     OLive& l = *(OLive*)h;
     std::vector<int8_t> rows(n * DIM);
     std::vector<float> norms(n);
@@ -1438,12 +1470,22 @@ void or_live_insert(void* h, const uint32_t* ext, const float* vecs, size_t n) {
         for (int i = 0; i < PDIM; i++) s = s + vecs[j * DIM + i] * vecs[j * DIM + i];
         norms[j] = std::sqrt(s);
     }
+    std::vector<uint8_t> rec;
+    for (size_t j = 0; j < n; j++) {  // src/live.rs:284-290
+        rec.push_back(OP_INSERT);
+        le32_put(rec, ext[j]);
+        uint32_t nb;
+        std::memcpy(&nb, &norms[j], 4);
+        le32_put(rec, nb);
+        rec.insert(rec.end(), reinterpret_cast<const uint8_t*>(rows.data() + j * DIM),
+                   reinterpret_cast<const uint8_t*>(rows.data() + (j + 1) * DIM));
+    }
+    if (!live_log(l, rec)) return -1;
     live_apply_insert(l, ext, rows.data(), norms.data(), n);
+    return 0;
 }
 
-// Live::delete / apply_delete, src/live.rs:308-326.
-void or_live_delete(void* h, const uint32_t* ext, size_t n) {
-    OLive& l = *(OLive*)h;
+static void live_apply_delete(OLive& l, const uint32_t* ext, size_t n) {  // src/live.rs:308-315
     for (size_t j = 0; j < n; j++) {
         uint32_t id;
         if (live_current(l, ext[j], &id)) {
@@ -1451,6 +1493,19 @@ void or_live_delete(void* h, const uint32_t* ext, size_t n) {
             l.by_ext.erase(ext[j]);
         }
     }
+}
+
+// Live::delete, src/live.rs:296-306: log, then apply_delete.
+int or_live_delete(void* h, const uint32_t* ext, size_t n) {
+    OLive& l = *(OLive*)h;
+    std::vector<uint8_t> rec;
+    for (size_t j = 0; j < n; j++) {
+        rec.push_back(OP_DELETE);
+        le32_put(rec, ext[j]);
+    }
+    if (!live_log(l, rec)) return -1;
+    live_apply_delete(l, ext, n);
+    return 0;
 }
 
 uint32_t or_live_current(const void* h, uint32_t ext) {
@@ -1488,6 +1543,161 @@ size_t or_live_deleted(const void* h, uint64_t* words, size_t cap) {
     const auto& d = ((const OLive*)h)->deleted;
     for (size_t i = 0; i < d.size() && i < cap; i++) words[i] = d[i];
     return d.size();
+}
+
+// Live::replay (src/live.rs:209-249) and the log opened for appending (src/live.rs:197-205).
+// stats: inserts, deletes, torn bytes, and the byte offset of a bad op. Returns 0, -1 for an
+// unknown op (nothing after it applied, the file untouched), -2 for an I/O error.
+int or_live_open_wal(void* h, const char* path, uint64_t* stats) {
+    OLive& l = *(OLive*)h;
+    std::vector<uint8_t> buf;
+    if (FILE* f = std::fopen(path, "rb")) {
+        uint8_t tmp[65536];
+        size_t r;
+        while ((r = std::fread(tmp, 1, sizeof tmp, f)) > 0) buf.insert(buf.end(), tmp, tmp + r);
+        std::fclose(f);
+    }
+    stats[0] = stats[1] = stats[2] = stats[3] = 0;
+    size_t pos = 0;
+    std::vector<uint32_t> ext;
+    std::vector<int8_t> rows;
+    std::vector<float> norms;
+    while (pos < buf.size()) {
+        size_t need;
+        if (buf[pos] == OP_INSERT)
+            need = INSERT_BYTES;
+        else if (buf[pos] == OP_DELETE)
+            need = DELETE_BYTES;
+        else {
+            stats[3] = pos;
+            return -1;
+        }
+        if (pos + need > buf.size()) break;
+        const uint8_t* r = buf.data() + pos;
+        if (r[0] == OP_INSERT) {
+            ext.push_back(le32_get(r + 1));
+            uint32_t nb = le32_get(r + 5);
+            float nf;
+            std::memcpy(&nf, &nb, 4);
+            norms.push_back(nf);
+            rows.insert(rows.end(), reinterpret_cast<const int8_t*>(r + 9), reinterpret_cast<const int8_t*>(r + 9) + DIM);
+            stats[0]++;
+        } else {
+            live_apply_insert(l, ext.data(), rows.data(), norms.data(), ext.size());
+            ext.clear();
+            rows.clear();
+            norms.clear();
+            uint32_t e = le32_get(r + 1);
+            live_apply_delete(l, &e, 1);
+            stats[1]++;
+        }
+        pos += need;
+    }
+    live_apply_insert(l, ext.data(), rows.data(), norms.data(), ext.size());
+    stats[2] = buf.size() - pos;
+    if (stats[2] && ::truncate(path, (off_t)pos) != 0) return -2;
+    l.wal = path;
+    l.wal_fd = ::open(path, O_WRONLY | O_CREAT | O_APPEND, 0644);
+    return l.wal_fd >= 0 ? 0 : -2;
+}
+
+// Live::truncate_wal, src/live.rs:470-479.
+int or_live_truncate_wal(void* h) {
+    OLive& l = *(OLive*)h;
+    if (l.wal_fd < 0) return 0;
+    return ::ftruncate(l.wal_fd, 0) == 0 && ::fsync(l.wal_fd) == 0 ? 0 : -2;
+}
+
+// The row segment of a compacted index, Live::open (src/live.rs:183-195): rows, prefix norms,
+// tier-2 codes and norms (t2 NULL: none), external ids, tombstone words; every delta row whose
+// tombstone is clear is its external id's row.
+void or_live_load_segment(void* h, const int8_t* rows, const float* norm256, const uint8_t* t2, const float* t2n,
+                          const uint32_t* ext, size_t nrows, const uint64_t* deleted, size_t words) {
+    OLive& l = *(OLive*)h;
+    Delta& d = l.delta;
+    d.rows.assign(rows, rows + nrows * DIM);
+    d.norm256.assign(norm256, norm256 + nrows);
+    if (t2) {
+        d.t2.assign(t2, t2 + nrows * 64);
+        d.t2_norm.assign(t2n, t2n + nrows);
+    } else {
+        d.t2.clear();
+        d.t2_norm.clear();
+    }
+    d.ext.assign(ext, ext + nrows);
+    l.deleted.assign(deleted, deleted + words);
+    l.by_ext.clear();
+    for (size_t i = 0; i < nrows; i++) {
+        uint32_t id = (uint32_t)(l.ix->n + i);
+        if (!live_bit(l.deleted, id)) l.by_ext[ext[i]] = id;
+    }
+}
+
+// Live::compact's merge (src/live.rs:402-448): per cluster the base list's then the delta list's
+// slots, PAD and tombstoned ones left out, packed into blocks. Writes the new lists if they fit
+// in cap_blocks; returns their block count. stats: slots kept, slots dropped.
+size_t or_live_compact(const void* h, size_t cap_blocks, uint64_t* block_off, uint8_t* blocks, float* factors,
+                       uint32_t* ids, uint64_t* stats) {
+    const OLive& l = *(const OLive*)h;
+    const Index& ix = *l.ix;
+    const size_t BB = 1024;  // BLOCK_BYTES of the 256-d codes
+    struct Slot {
+        uint8_t code[32];
+        float f[2];
+        uint32_t id;
+    };
+    std::vector<Slot> live;
+    size_t nblocks = 0;
+    uint64_t slots = 0, dropped = 0;
+    for (uint32_t c = 0; c < ix.k; c++) {
+        live.clear();
+        const size_t b0 = ix.block_off[c], b1 = ix.block_off[c + 1];
+        const DeltaList& dl = l.delta.lists[c];
+        struct Src {
+            const uint8_t* bl;
+            const float* fs;
+            const uint32_t* is;
+            size_t n;
+        } srcs[2] = {{ix.blocks + b0 * BB, ix.factors + b0 * BLOCK * 2, ix.ids + b0 * BLOCK, (b1 - b0) * BLOCK},
+                     {dl.blocks.data(), dl.factors.data(), dl.ids.data(), dl.len ? dl.ids.size() : 0}};
+        for (const Src& sr : srcs)
+            for (size_t j = 0; j < sr.n; j++) {
+                uint32_t id = sr.is[j];
+                if (id == PAD) continue;
+                if (live_bit(l.deleted, id)) {
+                    dropped++;
+                    continue;
+                }
+                Slot x;
+                unpack_slot(sr.bl + j / BLOCK * BB, PDIM / 4, (int)(j % BLOCK), x.code);
+                x.f[0] = sr.fs[j * 2];
+                x.f[1] = sr.fs[j * 2 + 1];
+                x.id = id;
+                live.push_back(x);
+            }
+        if (block_off && nblocks <= cap_blocks) block_off[c] = nblocks;
+        for (size_t o = 0; o < live.size(); o += BLOCK) {
+            size_t m = std::min<size_t>(BLOCK, live.size() - o);
+            if (nblocks < cap_blocks) {
+                std::vector<uint8_t> codes(m * 32);
+                for (size_t q = 0; q < m; q++) std::memcpy(codes.data() + q * 32, live[o + q].code, 32);
+                pack_block(codes.data(), (int)m, PDIM / 4, blocks + nblocks * BB);
+                for (size_t q = 0; q < (size_t)BLOCK; q++) {
+                    factors[(nblocks * BLOCK + q) * 2] = q < m ? live[o + q].f[0] : 0.0f;
+                    factors[(nblocks * BLOCK + q) * 2 + 1] = q < m ? live[o + q].f[1] : 0.0f;
+                    ids[nblocks * BLOCK + q] = q < m ? live[o + q].id : PAD;
+                }
+            }
+            nblocks++;
+        }
+        slots += live.size();
+    }
+    if (block_off && nblocks <= cap_blocks) block_off[ix.k] = nblocks;
+    if (stats) {
+        stats[0] = slots;
+        stats[1] = dropped;
+    }
+    return nblocks;
 }
 
 // Live::search for nq queries (src/live.rs:381-385): search_view over tombstones and the delta
@@ -1548,9 +1758,9 @@ struct OFilter {  // Filter, src/filter.rs:18-39
 
 extern "C" {
 
-constexpr uint64_t BRUTE_MAX = 250000;  // src/filter.rs:54
-constexpr double POST_MIN = 0.3;        // src/filter.rs:56
-constexpr uint64_t EXPAND_TARGET = 200000;  // src/filter.rs:58
+constexpr uint64_t BRUTE_MAX = 250000;  // src/filter.rs:55
+constexpr double POST_MIN = 0.3;        // src/filter.rs:57
+constexpr uint64_t EXPAND_TARGET = 200000;  // src/filter.rs:59
 
 // Filter::new, src/filter.rs:27-38.
 void* or_filter_create(const void* index, const uint64_t* allow, size_t words) {

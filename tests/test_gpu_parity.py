@@ -73,7 +73,7 @@ def test_lut_scan_bit_exact(rng, G):
 
 
 def test_lut_scan_narrow_fixture():
-    # lut.rs:237-253 fixture through the GPU seam.
+    # lut.rs:212-228 fixture through the GPU seam.
     from sieve_amd import abi
     q = np.cos(np.arange(128, dtype=np.float32) * np.float32(0.91)).astype(np.float32)
     tables, _, _ = O.lut_new(q)

@@ -80,7 +80,7 @@ def test_codes_and_factors_are_the_encoder_output(built):
     x = base[torch.as_tensor(rows.astype(np.int64))].to(torch.float32)
     cr = rotate(pad(arr.leaves[c:c + 1]), signs).repeat(x.shape[0], 1)
     code, mul, _ = rabitq_encode(rotate(pad(x), signs), cr)
-    # unpack the slots' codes from the reference's nibble-transposed blocks (src/lut.rs:98-108)
+    # unpack the slots' codes from the reference's nibble-transposed blocks (src/lut.rs:73-83)
     blk = arr.blocks.numpy()[slots // 32]
     k = slots % 32
     v = np.take_along_axis(blk, np.arange(32)[None, :] * 16 + (k % 16)[:, None], axis=1)  # group g of slot k

@@ -54,7 +54,7 @@ def test_gemm_is_sequential_f32_and_within_f64_bound(rng):
 
 # ----------------------------------------------------------------------- LUT -----------
 def test_lut_kernel_matches_reference(rng):
-    # lut.rs:204-221 (kernel_matches_reference), plus an independent bit-level sum.
+    # lut.rs:179-196 (kernel_matches_reference), plus an independent bit-level sum.
     for _ in range(20):
         q = rng.uniform(-1, 1, PDIM).astype(np.float32)
         codes = rng.integers(0, 256, (BLOCK, 32), dtype=np.uint8)
@@ -69,7 +69,7 @@ def test_lut_kernel_matches_reference(rng):
 
 
 def test_lut_quantized_score_tracks_float(rng):
-    # lut.rs:223-234 (quantized_score_tracks_float); float_score against float64.
+    # lut.rs:198-209 (quantized_score_tracks_float); float_score against float64.
     for _ in range(200):
         q = rng.uniform(-0.2, 0.2, PDIM).astype(np.float32)
         code = rng.integers(0, 256, 32, dtype=np.uint8)
@@ -100,7 +100,7 @@ def test_lut_table_invariants(rng):
 
 
 def test_lut_narrow_codes_fixture():
-    # lut.rs:237-253: G = 32, q_i = cos(0.91 i), code byte i of doc k = (37k + 11i) mod 256.
+    # lut.rs:212-228: G = 32, q_i = cos(0.91 i), code byte i of doc k = (37k + 11i) mod 256.
     q = np.cos(np.arange(128, dtype=np.float32) * np.float32(0.91)).astype(np.float32)
     tables, _, _ = O.lut_new(q)
     codes = np.array([[(k * 37 + i * 11) % 256 for i in range(16)] for k in range(BLOCK)], np.uint8)
@@ -112,7 +112,7 @@ def test_lut_narrow_codes_fixture():
 
 
 def test_lut_partial_block_pads_with_zero_codes():
-    # lut.rs:255-266
+    # lut.rs:230-241
     q = np.sin(np.arange(PDIM, dtype=np.float32) * np.float32(0.37)).astype(np.float32)
     tables, _, _ = O.lut_new(q)
     c = np.full((1, 32), 0xA5, np.uint8)
@@ -122,7 +122,7 @@ def test_lut_partial_block_pads_with_zero_codes():
 
 
 def test_pack_block_layout_bitwise(rng):
-    # Byte g*16+k holds doc k's nibble g (low) and doc k+16's (high) (lut.rs:29-35).
+    # Byte g*16+k holds doc k's nibble g (low) and doc k+16's (high) (lut.rs:7-9).
     codes = rng.integers(0, 256, (BLOCK, 32), dtype=np.uint8)
     block = O.pack_block(codes, 64)
     nib = nibbles_np(codes, 64)
@@ -132,7 +132,7 @@ def test_pack_block_layout_bitwise(rng):
 
 
 def test_hamming_counts_bits(rng):
-    # lut.rs:268-273, plus numpy popcount.
+    # lut.rs:243-248, plus numpy popcount.
     assert O.hamming(np.full(32, 0xFF, np.uint8), np.full(32, 0x0F, np.uint8)) == 4 * 32
     a = rng.integers(0, 256, 32, dtype=np.uint8)
     b = rng.integers(0, 256, 32, dtype=np.uint8)
