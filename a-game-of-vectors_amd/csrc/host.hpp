@@ -649,6 +649,7 @@ inline void finish_search(sieve_stream* s, size_t nq, sieve_timing* t) {
     s->last.scan_ns = elapsed_ns(s->ev[1], s->ev[2]);
     s->last.rerank_ns = elapsed_ns(s->ev[2], s->ev[3]);
     s->last.cm_pairs = hcnt[6];
+    s->last.cm_staged = hcnt[7];
     s->last.cm_launches = s->cm_timed;
     for (uint32_t ph = 0; ph < s->cm_timed; ph++) s->last.cm_ns += elapsed_ns(s->ev[4 + 2 * ph], s->ev[5 + 2 * ph]);
     if (t) {

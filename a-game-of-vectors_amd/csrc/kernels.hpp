@@ -123,7 +123,8 @@ struct ScanArgs {
     uint32_t* out_n;
     int out_stride;
     unsigned long long* scanned; // [0] += stored slots scanned, [2] += slots past the threshold,
-                                 // [4] += (query, slot) pairs the cluster-major MFMA pass scores
+                                 // [4] += (query, slot) pairs the cluster-major MFMA pass scores,
+                                 // [5] += pairs its integer bound stages for the exact estimate
     // Work split: query q's probed blocks (T_q) are cut into clamp(ceil(T_q / chunk_blocks), 1,
     // chunk_max) chunks; workgroup q * chunk_max + c scans chunk c.
     uint32_t chunk_blocks;
@@ -166,7 +167,7 @@ size_t scan_lds_bytes(int cand, int probe_stride);
 // Scan kernel variants (slots per thread, prefetch), picked per index by the autotuner; the
 // last one is the cluster-major scan (launch_scan_cm).
 constexpr int SCAN_VARIANTS = 12;
-constexpr int SCAN_VARIANT_CM = 10;  // 10: one-hot operands from LDS, 11: from LDS and registers
+constexpr int SCAN_VARIANT_CM = 10;  // 10: k_scan_cm2, 11: k_scan_cm (one accumulator chain per column half)
 const char* scan_variant_name(int variant);
 void launch_scan(const ScanArgs& a, size_t nq, int variant, hipStream_t st);
 

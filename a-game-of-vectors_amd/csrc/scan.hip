@@ -635,7 +635,7 @@ const char* scan_variant_name(int v) {
         "4 slots/thread", "2 slots/thread + prefetch", "2 slots/thread, cut at 1.5 cand", "1 slot/thread, cut at 1.5 cand",
         "2 slots/thread + prefetch, >= 5 waves/SIMD", "2 slots/thread", "1 slot/thread + prefetch", "1 slot/thread",
         "1 slot/thread + prefetch, cut at 1.5 cand", "1 slot/thread, cut at 1.25 cand",
-        "cluster-major MFMA, one-hot from LDS", "cluster-major MFMA, one-hot from LDS and registers"};
+        "cluster-major MFMA, split chains, integer bound epilogue", "cluster-major MFMA, one chain per column half, f32 epilogue"};
     return v >= 0 && v < SCAN_VARIANTS ? names[v] : "?";
 }
 
@@ -689,6 +689,7 @@ struct CmArgs {
     uint32_t* work;              // phases counters
     unsigned long long* scanned;
     unsigned long long* pairs;   // (query, slot) pairs scored, the MFMA pass's own count
+    unsigned long long* staged;  // pairs its integer bound passed to the exact estimate
     uint32_t phase, phases;      // this launch runs work items [total * phase / phases, ...)
 };
 
@@ -722,7 +723,7 @@ __global__ __launch_bounds__(256) void k_pack_stats(const unsigned long long* cn
         out[4] = part[0] + part[1] + part[2] + part[3];
         out[5] = *ovf;
         out[6] = cnt[4];
-        out[7] = 0;
+        out[7] = cnt[5];
     }
 }
 
@@ -1097,6 +1098,276 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
 }
 
+// The cluster-major MFMA pass, second form (scan variant 10). Against k_scan_cm:
+//  - A dependent int8 MFMA waits out the full latency of the one before it (about three issue
+//    intervals), so a 16-query tile accumulates its 16 steps in two chains (even and odd steps)
+//    summed at the end, and a 32-query tile runs its two column halves as two chains.
+//  - The one-hot operands come from the 16-entry table (one LDS row, so reads never conflict)
+//    at an offset taken from the code with one masked byte select.
+//  - The epilogue does no float work per (query, slot) pair. Each lane compares its four scores
+//    with an integer bound tau for its column: a score below tau gives an estimate below the
+//    query's threshold. est = cent + m (a sc + b) rises with sc (a > 0, m >= 0); with m between
+//    the lane's least and greatest multiplier mn <= m <= mx and T = thr - cent - err (err bounds
+//    the f32 rounding of the estimate and of T: 2^-21 (|cent| + |thr| + mx (4 * 16320 a + 3 |b|))
+//    is eight times its first-order bound), est >= thr needs m (a sc + b) >= T, hence
+//    a sc + b >= T / mx when T > 0 and a sc + b >= T / mn otherwise; tau is that bound on sc
+//    less 2 (the error of computing it is far below one unit). Scores at or above tau are staged
+//    as (slot, score, column); the flush computes their estimates in the f32 steps of the
+//    per-query scan and appends those past the threshold, so the appended lists are exactly
+//    k_scan_cm's.
+constexpr int CM2_STAGE = 512;
+
+// Bits 4..7 of byte k of x as a byte offset (0, 16, ..., 240): one masked byte select (SDWA).
+__device__ __forceinline__ uint32_t nib_off(uint32_t x, int k) {
+    const uint32_t m = 0xF0u;
+    uint32_t r;
+    switch (k) {
+        case 0: return x & m;
+        case 1: asm("v_and_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_1 src1_sel:DWORD"
+                    : "=v"(r) : "v"(x), "s"(m));
+                return r;
+        case 2: asm("v_and_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_2 src1_sel:DWORD"
+                    : "=v"(r) : "v"(x), "s"(m));
+                return r;
+        default: asm("v_and_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_3 src1_sel:DWORD"
+                     : "=v"(r) : "v"(x), "s"(m));
+                 return r;
+    }
+}
+
+struct CmColC {  // a tile column's query constants, for the flush
+    float a, b, cent, thrf;
+    unsigned long long thr;
+    uint32_t q, pad;
+};
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_scan_cm2(CmArgs a) {
+    __shared__ uint4 onehot[16];              // 256 B: one LDS row, conflict-free reads
+    __shared__ uint2 stage[4][CM2_STAGE];     // (slot, score << 5 | column)
+    __shared__ CmColC cols[4][32];
+    __shared__ uint32_t ccnt[4][32], cbase[4][32];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (t < 16) {
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+        w[t >> 2] = 1u << (8 * (t & 3));
+        onehot[t] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    __syncthreads();
+    const int r = lane >> 4, j = lane & 15;
+    // The host sizes the descriptor array by a bound of the work items; never read past it.
+    const uint32_t total = min(a.seg_ioff[a.nseg], a.items_cap);
+    const uint32_t i_lo = (uint32_t)((uint64_t)total * a.phase / a.phases);
+    const uint32_t i_hi = (uint32_t)((uint64_t)total * (a.phase + 1) / a.phases);
+    uint32_t sn = 0;  // staged entries (the same in every lane)
+    // Exact estimates of the staged entries; those past their column's threshold are appended,
+    // one global atomic per column and chunk of 64.
+    auto flush = [&]() {
+        if (lane == 0 && sn && a.staged) atomicAdd(a.staged, (unsigned long long)sn);
+        for (uint32_t c0 = 0; c0 < sn; c0 += 64) {
+            if (lane < 32) ccnt[wave][lane] = 0;
+            wave_lds_sync();
+            const uint32_t i = c0 + lane;
+            bool pass = false;
+            uint32_t key = 0, slot = 0, col = 0, loc = 0;
+            if (i < sn) {
+                const uint2 e = stage[wave][i];
+                slot = e.x;
+                col = e.y & 31u;
+                const uint32_t sc = e.y >> 5;
+                const CmColC cc = cols[wave][col];
+                const float m = a.mul[slot];
+                float est = cc.cent + m * (cc.a * (float)sc + cc.b);
+                est = est + 0.0f;
+                if (est >= cc.thrf) {
+                    key = f2key(est);
+                    pass = (((unsigned long long)key << 32) | (unsigned long long)(~slot)) > cc.thr;
+                }
+                if (pass) loc = atomicAdd(&ccnt[wave][col], 1u);
+            }
+            wave_lds_sync();
+            if (lane < 32) {
+                const uint32_t cnt = ccnt[wave][lane];
+                cbase[wave][lane] = cnt ? atomicAdd(a.ext_n + cols[wave][lane].q, cnt) : 0u;
+            }
+            wave_lds_sync();
+            if (pass) {
+                const uint32_t at = cbase[wave][col] + loc;
+                if (at < a.ext_cap) a.ext[(size_t)cols[wave][col].q * a.ext_cap + at] = make_uint2(key, ~slot);
+            }
+            wave_lds_sync();
+        }
+        sn = 0;
+    };
+    // The next work item is claimed, and its descriptor read, while the current one runs.
+    auto claim = [&]() {
+        uint32_t v = 0;
+        if (lane == 0) v = i_lo + atomicAdd(a.work + a.phase, 1u);
+        return (uint32_t)__shfl(v, 0, 64);
+    };
+    uint32_t nidx = claim();
+    uint4 nitem = nidx < i_hi ? a.items[nidx] : make_uint4(0u, 0u, 0u, 0u);
+    for (;;) {
+        const uint32_t idx = nidx;
+        if (idx >= i_hi) break;
+        const uint4 item = nitem;
+        nidx = claim();
+        if (nidx < i_hi) nitem = a.items[nidx];
+        const uint32_t s_lo = item.x, s_hi = item.x + item.y, qpos0 = item.z, nvalid = item.w;
+        const bool two = nvalid > 16;
+        // B operands: lane (j, r) holds query j's table entries of groups 4 st + r, minus 128.
+        uint4 B0[16], B1[16];
+        // Per column half: T = thr - cent, the rounding-margin terms, -b and 1 / a.
+        float cT[2], cS[2], cK[2], cnb[2], cia[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t n = 16 * h + j;
+            const bool valid = n < nvalid;
+            const uint32_t pos = qpos0 + (valid ? n : 0);
+            const uint32_t q = a.seg_q[pos];
+            const float4 qab = a.qab[q];
+            const float cent = a.seg_cent[pos];
+            // ~0: nothing passes (a padding column, or a query whose list was lost).
+            const unsigned long long thr = valid ? a.thr[q] : ~0ull;
+            const float thrf = thr == ~0ull ? INFINITY : thr == 0 ? -INFINITY : key2f((uint32_t)(thr >> 32));
+            if (lane < 16) {
+                CmColC cc;
+                cc.a = qab.x;
+                cc.b = qab.y;
+                cc.cent = cent;
+                cc.thrf = thrf;
+                cc.thr = thr;
+                cc.q = q;
+                cc.pad = 0;
+                cols[wave][n] = cc;
+            }
+            const bool finite = thr != ~0ull && thr != 0;
+            cT[h] = thr == ~0ull ? INFINITY : thr == 0 ? -INFINITY : thrf - cent;
+            // 1e-30: the margin is never exactly zero.
+            cS[h] = finite ? fabsf(cent) + fabsf(thrf) + 1.0e-30f : 0.0f;
+            cK[h] = 65280.0f * qab.x + 3.0f * fabsf(qab.y);
+            cnb[h] = -qab.y;
+            cia[h] = __builtin_amdgcn_rcpf(qab.x);
+            if (h == 1 && !two) continue;
+            const uint4* tp = reinterpret_cast<const uint4*>(a.tables + (size_t)q * 1024);
+#pragma unroll
+            for (int st = 0; st < 16; st++) {
+                uint4 v = tp[4 * st + r];
+                v.x ^= 0x80808080u;
+                v.y ^= 0x80808080u;
+                v.z ^= 0x80808080u;
+                v.w ^= 0x80808080u;
+                if (h == 0) B0[st] = v;
+                else B1[st] = v;
+            }
+        }
+        if (lane == 0 && a.scanned) {
+            atomicAdd(a.scanned, (unsigned long long)item.y * nvalid);
+            atomicAdd(a.pairs, (unsigned long long)item.y * nvalid);
+        }
+        wave_lds_sync();
+
+        // Lane (j, r) of C holds slots 4 r .. 4 r + 3 of the 16-slot tile for query column j
+        // (column 16 h + j of the tile).
+        // The bound is compared in f32 (scores are exact there): a NaN bound (0 x inf, a zero
+        // multiplier against a margin-adjusted threshold of exactly 0) stages everything.
+        auto stage_half = [&](const v4i& acc, int h, uint32_t slot0, float mx, float rmx, float rmn) {
+            if (sn > (uint32_t)(CM2_STAGE - 256)) flush();  // a half adds at most 256 entries
+            const float tpv = fmaf(-0x1p-21f, fmaf(mx, cK[h], cS[h]), cT[h]);
+            const float inv = tpv > 0.0f ? rmx : rmn;
+            const float tf = fmaf(fmaf(tpv, inv, cnb[h]), cia[h], -2.0f);
+            const uint32_t colx = (uint32_t)(16 * h + j), sb = slot0 + 4 * r;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const bool p = !((float)acc[i] < tf);
+                const unsigned long long m = __ballot(p);
+                if (m) {
+                    if (p) {
+                        const uint32_t at =
+                            sn + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                        stage[wave][at] = make_uint2(sb + i, ((uint32_t)acc[i] << 5) | colx);
+                    }
+                    sn += (uint32_t)__popcll(m);
+                }
+            }
+        };
+        auto tile16 = [&](const uint2& cw, const float4& mu, uint32_t slot0, auto two_c) {
+            constexpr bool TWO = decltype(two_c)::value;
+            // The even chain starts at 64 * 128, so the sum of the chains is score_code itself.
+            v4i e0 = {64 * 128, 64 * 128, 64 * 128, 64 * 128}, o0 = {0, 0, 0, 0}, e1 = e0;
+            // Operand of step st: the one-hot bytes of group 4 st + r of the slot, nibble st of
+            // the lane's 8-byte chunk of the code. Its table offset (nibble x 16) is bits 4..7 of
+            // a code byte: of byte st / 2 itself for the high nibble, of the word shifted left by
+            // 4 for the low one (one masked byte select per operand).
+            const uint32_t w[2] = {cw.x, cw.y}, w4[2] = {cw.x << 4, cw.y << 4};
+            auto operand = [&](int st) -> v4i {
+                const uint32_t off = nib_off((st & 1) ? w[st >> 3] : w4[st >> 3], (st >> 1) & 3);
+                const uint4 oh = *reinterpret_cast<const uint4*>(reinterpret_cast<const uint8_t*>(onehot) + off);
+                return v4i{(int)oh.x, (int)oh.y, (int)oh.z, (int)oh.w};
+            };
+            constexpr int PIPE = 4;
+            v4i pipe[PIPE];
+#pragma unroll
+            for (int st = 0; st < PIPE; st++) pipe[st] = operand(st);
+#pragma unroll
+            for (int st = 0; st < 16; st++) {
+                const v4i av = pipe[st % PIPE];
+                if (st + PIPE < 16) pipe[st % PIPE] = operand(st + PIPE);
+                const v4i b0v = {(int)B0[st].x, (int)B0[st].y, (int)B0[st].z, (int)B0[st].w};
+                if constexpr (TWO) {
+                    // Two halves: already two chains.
+                    const v4i b1v = {(int)B1[st].x, (int)B1[st].y, (int)B1[st].z, (int)B1[st].w};
+                    e0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b0v, e0, 0, 0, 0);
+                    e1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b1v, e1, 0, 0, 0);
+                } else if (st & 1) {
+                    o0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b0v, o0, 0, 0, 0);
+                } else {
+                    e0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b0v, e0, 0, 0, 0);
+                }
+            }
+            // Multipliers are >= +0, where the f32 order is the order of the bit patterns.
+            const uint32_t ux = __float_as_uint(mu.x), uy = __float_as_uint(mu.y), uz = __float_as_uint(mu.z),
+                           uw = __float_as_uint(mu.w);
+            const float mx = __uint_as_float(max(max(ux, uy), max(uz, uw)));
+            const float mn = __uint_as_float(min(min(ux, uy), min(uz, uw)));
+            const float rmx = __builtin_amdgcn_rcpf(mx), rmn = __builtin_amdgcn_rcpf(mn);
+            if constexpr (TWO) {
+                stage_half(e0, 0, slot0, mx, rmx, rmn);
+                stage_half(e1, 1, slot0, mx, rmx, rmn);
+            } else {
+                stage_half(e0 + o0, 0, slot0, mx, rmx, rmn);
+            }
+        };
+        // 16-slot tiles of the chunk in order, the codes and multipliers of the next CM_PREFETCH
+        // tiles in flight (lane (j, r): chunk r of slot j's code, mul of slots 4 r .. 4 r + 3).
+        constexpr int PF = CM_PREFETCH;
+        const uint32_t ntiles = (s_hi - s_lo) / 16;
+        uint2 cr[PF];
+        float4 mr[PF];
+        auto load = [&](uint32_t ti, uint2& c, float4& m) {
+            const uint32_t slot0 = s_lo + 16 * ti;
+            c = reinterpret_cast<const uint2*>(a.codes + 2 * (size_t)(slot0 + j))[r];
+            m = *reinterpret_cast<const float4*>(a.mul + slot0 + 4 * r);
+        };
+#pragma unroll
+        for (int p = 0; p < PF; p++)
+            if ((uint32_t)p < ntiles) load(p, cr[p], mr[p]);
+        for (uint32_t t0 = 0; t0 < ntiles; t0 += PF) {
+#pragma unroll
+            for (int p = 0; p < PF; p++) {
+                const uint32_t ti = t0 + p;
+                if (ti < ntiles) {
+                    const uint2 c = cr[p];
+                    const float4 m = mr[p];
+                    if (ti + PF < ntiles) load(ti + PF, cr[p], mr[p]);
+                    if (two) tile16(c, m, s_lo + 16 * ti, std::true_type{});
+                    else tile16(c, m, s_lo + 16 * ti, std::false_type{});
+                }
+            }
+        }
+        flush();  // before the next item replaces the column constants
+    }
+}
+
 void launch_scan_cm(const ScanArgs& a_in, const CmScratch& cm_in, size_t nq, int variant, hipStream_t st) {
     if (nq == 0) return;
     CmScratch cm = cm_in;
@@ -1146,6 +1417,7 @@ void launch_scan_cm(const ScanArgs& a_in, const CmScratch& cm_in, size_t nq, int
     c.work = cm.work;
     c.scanned = a.scanned;
     c.pairs = a.scanned ? a.scanned + 4 : nullptr;
+    c.staged = a.scanned ? a.scanned + 5 : nullptr;
     c.phases = cm.phases;
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
@@ -1161,9 +1433,9 @@ void launch_scan_cm(const ScanArgs& a_in, const CmScratch& cm_in, size_t nq, int
         const bool timed = cm.ev && 2 * ph + 1 < cm.nev;
         if (timed) (void)hipEventRecord(cm.ev[2 * ph], st);
         if (variant == SCAN_VARIANT_CM)
-            hipLaunchKernelGGL(k_scan_cm<CM_OH_LDS>, dim3((unsigned)cus * 4), dim3(256), 0, st, c);
+            hipLaunchKernelGGL(k_scan_cm2, dim3((unsigned)cus * 4), dim3(256), 0, st, c);
         else
-            hipLaunchKernelGGL(k_scan_cm<CM_OH_MIXED>, dim3((unsigned)cus * 4), dim3(256), 0, st, c);
+            hipLaunchKernelGGL(k_scan_cm<CM_OH_LDS>, dim3((unsigned)cus * 4), dim3(256), 0, st, c);
         if (timed) (void)hipEventRecord(cm.ev[2 * ph + 1], st);
         if (ph + 1 < cm.phases)
             hipLaunchKernelGGL((k_scan_t<64, false, 2, false, 0, SCAN_TIGHTEN>), dim3((unsigned)nq),

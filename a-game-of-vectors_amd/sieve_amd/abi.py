@@ -59,7 +59,7 @@ class Stats(ctypes.Structure):
                 ("id_reads", ctypes.c_uint64), ("reranked", ctypes.c_uint64), ("route_ns", ctypes.c_double),
                 ("scan_ns", ctypes.c_double), ("rerank_ns", ctypes.c_double), ("rescanned", ctypes.c_uint64),
                 ("graph", ctypes.c_uint64), ("cm_pairs", ctypes.c_uint64), ("cm_ns", ctypes.c_double),
-                ("cm_launches", ctypes.c_uint64)]
+                ("cm_launches", ctypes.c_uint64), ("cm_staged", ctypes.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

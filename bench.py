@@ -396,6 +396,7 @@ def main_ip(a, W, name):
     scanned, id_reads, probed = per_step("scanned"), per_step("id_reads"), per_step("probed")
     rescanned, reranked = per_step("rescanned"), per_step("reranked")
     cm_pairs, cm_ns, cm_launches = per_step("cm_pairs"), per_step("cm_ns"), per_step("cm_launches")
+    cm_staged = per_step("cm_staged")
     s, r, n = run(p)
     torch.cuda.synchronize()
     gpu_res = (s.cpu().numpy(), r.cpu().numpy().view(np.uint32), n.cpu().numpy().view(np.uint32))
@@ -415,7 +416,7 @@ def main_ip(a, W, name):
         # once per query tile). achieved = useful int8 ops per launch / mean launch time.
         launch_s = cm_ns / 1e9 / cm_launches
         ops = CM_OPS_PER_PAIR * cm_pairs / cm_launches
-        kname = "k_scan_cm<1>" if variant == 10 else "k_scan_cm<2>"
+        kname = "k_scan_cm2" if variant == 10 else "k_scan_cm<1>"
         tr = pmc_traffic(kname, name, pick["config"])
         achieved = ops / launch_s / 1e12
         roofline = {"bound": "mfma", "achieved": round(achieved, 1), "peak": MFMA_I8_PEAK_TOPS, "unit": "TOP/s",
@@ -424,9 +425,11 @@ def main_ip(a, W, name):
                     "kernel": kname, "mfma": "v_mfma_i32_16x16x64_i8 (dense int8)",
                     "ops_per_launch": int(ops), "ops_per_pair": CM_OPS_PER_PAIR,
                     "pairs_per_launch": int(cm_pairs / cm_launches), "launches_per_step": int(cm_launches),
+                    "staged_fraction": round(cm_staged / cm_pairs, 5) if cm_pairs else None,
                     "mean_launch_ms": round(launch_s * 1e3, 4),
                     "dram_GBps": round(tr["bytes_per_launch"] / launch_s / 1e9, 1) if tr else None,
                     "traffic_profile": tr["profile"] if tr else None}
+        log(f"k_scan_cm: staged {cm_staged / max(cm_pairs, 1):.4f} of the scored pairs for the exact estimate")
         log(f"k_scan_cm: {cm_launches:.0f} launches of {launch_s * 1e3:.2f} ms, {cm_pairs / cm_launches:.3g} pairs "
             f"each = {achieved:.0f} TOP/s int8 ({achieved / MFMA_I8_PEAK_TOPS:.3f} of peak)")
     else:

@@ -153,6 +153,7 @@ typedef struct sieve_stats {
     uint64_t cm_pairs;   /* cluster-major scan: (query, slot) pairs its MFMA pass scored          */
     double cm_ns;        /* GPU time of the MFMA pass's launches (HIP events around each)         */
     uint64_t cm_launches; /* MFMA pass launches timed (one per work-list phase)                   */
+    uint64_t cm_staged;  /* pairs the MFMA pass's integer bound let through to the exact estimate  */
 } sieve_stats;
 
 typedef struct sieve_index sieve_index;
