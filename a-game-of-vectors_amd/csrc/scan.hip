@@ -1202,7 +1202,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     auto claim = [&]() {
         uint32_t v = 0;
         if (lane == 0) v = i_lo + atomicAdd(a.work + a.phase, 1u);
-        return (uint32_t)__shfl(v, 0, 64);
+        return __builtin_amdgcn_readfirstlane((uint32_t)__shfl(v, 0, 64));
     };
     uint32_t nidx = claim();
     uint4 nitem = nidx < i_hi ? a.items[nidx] : make_uint4(0u, 0u, 0u, 0u);
@@ -1212,7 +1212,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         const uint4 item = nitem;
         nidx = claim();
         if (nidx < i_hi) nitem = a.items[nidx];
-        const uint32_t s_lo = item.x, s_hi = item.x + item.y, qpos0 = item.z, nvalid = item.w;
+        // The descriptor is the same in every lane: kept in scalar registers.
+        const uint32_t s_lo = __builtin_amdgcn_readfirstlane(item.x);
+        const uint32_t s_n = __builtin_amdgcn_readfirstlane(item.y);
+        const uint32_t qpos0 = __builtin_amdgcn_readfirstlane(item.z);
+        const uint32_t nvalid = __builtin_amdgcn_readfirstlane(item.w);
+        const uint32_t s_hi = s_lo + s_n;
         const bool two = nvalid > 16;
         // B operands: lane (j, r) holds query j's table entries of groups 4 st + r, minus 128.
         uint4 B0[16], B1[16];
@@ -1261,8 +1266,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             }
         }
         if (lane == 0 && a.scanned) {
-            atomicAdd(a.scanned, (unsigned long long)item.y * nvalid);
-            atomicAdd(a.pairs, (unsigned long long)item.y * nvalid);
+            atomicAdd(a.scanned, (unsigned long long)s_n * nvalid);
+            atomicAdd(a.pairs, (unsigned long long)s_n * nvalid);
         }
         wave_lds_sync();
 
@@ -1339,31 +1344,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         };
         // 16-slot tiles of the chunk in order, the codes and multipliers of the next CM_PREFETCH
         // tiles in flight (lane (j, r): chunk r of slot j's code, mul of slots 4 r .. 4 r + 3).
+        // The loads take a wave-uniform base and a 32-bit lane offset; a tile's slots are
+        // refilled with the tile PF ahead once the tile has consumed them (loads past the last
+        // tile re-read it, in bounds, so no condition keeps later loads from staying in flight).
         constexpr int PF = CM_PREFETCH;
-        const uint32_t ntiles = (s_hi - s_lo) / 16;
+        const uint32_t ntiles = s_n / 16;
+        const uint8_t* cbase = reinterpret_cast<const uint8_t*>(a.codes) + 32 * (size_t)s_lo;
+        const uint8_t* mbase = reinterpret_cast<const uint8_t*>(a.mul) + 4 * (size_t)s_lo;
+        const uint32_t clane = 32 * (uint32_t)j + 8 * (uint32_t)r, mlane = 16 * (uint32_t)r;
+        const uint32_t tlast = ntiles > 0 ? ntiles - 1 : 0;
+        auto load_c = [&](uint32_t ti) {
+            return *reinterpret_cast<const uint2*>(cbase + 512 * min(ti, tlast) + clane);
+        };
+        auto load_m = [&](uint32_t ti) {
+            return *reinterpret_cast<const float4*>(mbase + 64 * min(ti, tlast) + mlane);
+        };
         uint2 cr[PF];
         float4 mr[PF];
-        auto load = [&](uint32_t ti, uint2& c, float4& m) {
-            const uint32_t slot0 = s_lo + 16 * ti;
-            c = reinterpret_cast<const uint2*>(a.codes + 2 * (size_t)(slot0 + j))[r];
-            m = *reinterpret_cast<const float4*>(a.mul + slot0 + 4 * r);
-        };
 #pragma unroll
-        for (int p = 0; p < PF; p++)
-            if ((uint32_t)p < ntiles) load(p, cr[p], mr[p]);
-        for (uint32_t t0 = 0; t0 < ntiles; t0 += PF) {
+        for (int p = 0; p < PF; p++) {
+            cr[p] = load_c((uint32_t)p);
+            mr[p] = load_m((uint32_t)p);
+        }
+        // Whole groups of PF tiles run with no condition around the loads (so the waits for a
+        // tile's data count the later loads as still in flight), the last < PF tiles after.
+        const uint32_t nfull = ntiles & ~(uint32_t)(PF - 1);
+        auto group = [&](auto two_c) {
+            for (uint32_t t0 = 0; t0 < nfull; t0 += PF) {
 #pragma unroll
-            for (int p = 0; p < PF; p++) {
-                const uint32_t ti = t0 + p;
-                if (ti < ntiles) {
-                    const uint2 c = cr[p];
-                    const float4 m = mr[p];
-                    if (ti + PF < ntiles) load(ti + PF, cr[p], mr[p]);
-                    if (two) tile16(c, m, s_lo + 16 * ti, std::true_type{});
-                    else tile16(c, m, s_lo + 16 * ti, std::false_type{});
+                for (int p = 0; p < PF; p++) {
+                    const uint32_t ti = t0 + p;
+                    tile16(cr[p], mr[p], s_lo + 16 * ti, two_c);
+                    cr[p] = load_c(ti + PF);
+                    mr[p] = load_m(ti + PF);
                 }
             }
-        }
+#pragma unroll
+            for (int p = 0; p < PF - 1; p++)
+                if (nfull + p < ntiles) tile16(cr[p], mr[p], s_lo + 16 * (nfull + p), two_c);
+        };
+        if (two) group(std::true_type{});
+        else group(std::false_type{});
         flush();  // before the next item replaces the column constants
     }
 }
