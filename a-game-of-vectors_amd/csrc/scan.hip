@@ -75,18 +75,19 @@ size_t scan_lds_bytes(int cand, int probe_stride) {
 
 // misc words: 0 count, 1 has threshold, 2 threshold key, 3 scratch count, 4..7 scan scratch,
 // 8..10 select scratch, 12..13 round stamps, 14 threshold ~slot, 15 last-chunk flag,
-// 16..17 adopt stamps.
+// 16..17 adopt stamps, 18 entries at the buffer's front whose ids were checked, 19 live count.
 constexpr int M_COUNT = 0, M_HAS = 1, M_THR = 2, M_TMP = 3, M_SCAN = 4, M_SEL = 8, M_STAMP = 12, M_THR_LO = 14,
-              M_LAST = 15, M_ADOPT = 16;
+              M_LAST = 15, M_ADOPT = 16, M_VALID = 18, M_LIVE = 19;
 
 __device__ __forceinline__ unsigned long long pack_pair(uint2 e) {
     return ((unsigned long long)e.x << 32) | e.y;
 }
 
-// Cuts buf[0, n) to its best k entries (under (est key, ~slot)) and raises the threshold; with
-// a per-query shared threshold, publishes it there.
+// Cuts buf[0, n) to its best k entries (under (est key, ~slot)) and, with `raise`, raises the
+// threshold to the k-th; with a per-query shared threshold, publishes it there. Dead entries
+// (0, 0) rank below every live one; k never exceeds the live entries, so none is kept.
 __device__ void scan_compact(uint2* buf, uint2* tmp, int n, int k, uint32_t* hist, uint32_t* misc,
-                             unsigned long long* gthr) {
+                             unsigned long long* gthr, bool raise_thr = true) {
     const int t = threadIdx.x;
     uint2 kth = block_kth_largest(buf, n, k, hist, misc + M_SEL);
     if (t == 0) misc[M_TMP] = 0;
@@ -102,13 +103,17 @@ __device__ void scan_compact(uint2* buf, uint2* tmp, int n, int k, uint32_t* his
     for (int i = t; i < k; i += SCAN_THREADS) buf[i] = tmp[i];
     if (t == 0) {
         misc[M_COUNT] = (uint32_t)k;
-        const bool raise = !misc[M_HAS] || kth.x > misc[M_THR] || (kth.x == misc[M_THR] && kth.y > misc[M_THR_LO]);
-        if (raise) {
-            misc[M_HAS] = 1;
-            misc[M_THR] = kth.x;
-            misc[M_THR_LO] = kth.y;
+        misc[M_VALID] = (uint32_t)k;
+        if (raise_thr) {
+            const bool raise =
+                !misc[M_HAS] || kth.x > misc[M_THR] || (kth.x == misc[M_THR] && kth.y > misc[M_THR_LO]);
+            if (raise) {
+                misc[M_HAS] = 1;
+                misc[M_THR] = kth.x;
+                misc[M_THR_LO] = kth.y;
+            }
+            if (gthr) atomicMax(gthr, pack_pair(kth));
         }
-        if (gthr) atomicMax(gthr, pack_pair(kth));
     }
     __syncthreads();
 }
@@ -213,6 +218,66 @@ void k_scan_t(ScanArgs a) {
         allow = a.view.allow + qi * a.view.allow_stride;
     const int cand = a.cand;
     uint32_t id_reads = 0;  // slots past the threshold: their id is read
+    // Without a per-query allowlist the admission's id check (src/ivf.rs:669) is deferred: an
+    // entry enters the buffer on its (estimate, slot) alone, and the ids of the entries past the
+    // checked prefix are read in one batch when the buffer is cut and at the end; padding slots
+    // and rows the view hides become dead entries (0, 0), which rank below every live entry.
+    // The kept set is the best cand live entries either way, and no id load sits between one
+    // round's code loads and the next's. (With an allowlist most passing slots are usually
+    // hidden, so they are checked at once and never fill the buffer.)
+    const bool lazy = allow == nullptr;
+
+    // Checks the buffer's unchecked entries; returns the live count.
+    auto validate = [&]() -> int {
+        const int n = (int)misc[M_COUNT], v0 = (int)misc[M_VALID];
+        if (!lazy || v0 >= n) return n;
+        uint32_t dead = 0;
+        for (int i0 = v0; i0 < n; i0 += 4 * SCAN_THREADS) {
+            uint32_t id[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int i = i0 + u * SCAN_THREADS + t;
+                id[u] = i < n ? a.ids[~buf[i].y] : PAD;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int i = i0 + u * SCAN_THREADS + t;
+                if (i >= n) continue;
+                id_reads++;
+                const bool ok = id[u] != PAD && !(a.view.deleted && bit_set(a.view.deleted, a.view.deleted_words, id[u]));
+                if (!ok) {
+                    buf[i] = make_uint2(0u, 0u);
+                    dead++;
+                }
+            }
+        }
+        dead = wave_sum(dead);
+        if (lane == 0 && dead) atomicAdd(&misc[M_LIVE], dead);
+        __syncthreads();
+        const int live = n - (int)misc[M_LIVE];
+        __syncthreads();
+        if (t == 0) {
+            misc[M_LIVE] = 0;
+            misc[M_VALID] = (uint32_t)n;
+        }
+        return live;
+    };
+    // Cuts the buffer to its best k live entries, raising the threshold (and publishing it to
+    // `gt`), when more than k are live; otherwise only drops the dead entries.
+    auto cut = [&](int k, unsigned long long* gt) {
+        const int n = (int)misc[M_COUNT];
+        const int live = validate();
+        if (live > k) {
+            scan_compact(buf, tmp, n, k, hist, misc, gt);
+        } else if (live < n) {
+            if (live > 0) {
+                scan_compact(buf, tmp, n, live, hist, misc, nullptr, false);
+            } else {
+                if (t == 0) misc[M_COUNT] = misc[M_VALID] = 0;
+                __syncthreads();
+            }
+        }
+    };
 
     // Appends the wave's passing entries; the wave whose append takes the buffer to 2 * cand
     // stamps the round so that every wave compacts after the barrier.
@@ -233,8 +298,7 @@ void k_scan_t(ScanArgs a) {
     // After the barrier of round `round`: compact if stamped, then adopt a better shared
     // threshold that thread 0 read during the round (`g`, valid in thread 0 only).
     auto settle = [&](uint32_t round, unsigned long long g) {
-        if (misc[M_STAMP + (round & 1)] == round)
-            scan_compact(buf, tmp, (int)misc[M_COUNT], cand, hist, misc, gthr);
+        if (misc[M_STAMP + (round & 1)] == round) cut(cand, gthr);
         if (misc[M_ADOPT + (round & 1)] == round) {
             if (t == 0) {
                 const uint2 e = make_uint2((uint32_t)(g >> 32), (uint32_t)g);
@@ -248,10 +312,13 @@ void k_scan_t(ScanArgs a) {
         }
     };
     // Thread 0 reads the shared threshold during the round and flags the round if it beats
-    // the local one (called before the round's barrier).
-    auto peek = [&](uint32_t round, unsigned long long& g) {
+    // the local one (before the round's barrier). The load is issued at the start of the round, before the round's code loads: loads retire
+    // in order, so a wait for it issued after them would also wait for the next round's data.
+    auto peek_load = [&](unsigned long long& g) {
+        if (gthr && t == 0) g = __hip_atomic_load(gthr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    auto peek = [&](uint32_t round, const unsigned long long g) {
         if (gthr && t == 0) {
-            g = __hip_atomic_load(gthr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const unsigned long long mine =
                 misc[M_HAS] ? (((unsigned long long)misc[M_THR] << 32) | misc[M_THR_LO]) : 0ull;
             if (g > mine) misc[M_ADOPT + (round & 1)] = round;
@@ -264,6 +331,7 @@ void k_scan_t(ScanArgs a) {
     struct Regs {
         uint4 cw[SLOTS_PER_THREAD][CW];
         float mul[SLOTS_PER_THREAD], cent[SLOTS_PER_THREAD];
+        float x2[L2 ? SLOTS_PER_THREAD : 1];  // L2: the slot's |x|^2, loaded with its code
         uint32_t gs[SLOTS_PER_THREAD];
         bool ok[SLOTS_PER_THREAD];
     };
@@ -272,17 +340,17 @@ void k_scan_t(ScanArgs a) {
         for (int u = 0; u < SLOTS_PER_THREAD; u++) {
             const uint32_t vb = vb0 + (t >> 5) + 8 * u;
             R.ok[u] = vb < vb_hi;
-            R.gs[u] = 0;
-            R.cent[u] = 0.0f;
-            if (R.ok[u]) {
-                while (cur + 1 < np && pref[cur + 1] <= vb) cur++;
-                R.gs[u] = (pstart[cur] + (vb - pref[cur])) * BLOCK + (t & 31);
-                R.cent[u] = pcent[cur];
-                const uint4* cp = a.codes + CW * (size_t)R.gs[u];
+            // Past the chunk's end a lane re-reads the chunk's last block (in bounds): no load is
+            // conditional, so the wait for one round's data leaves the next round's loads in flight.
+            const uint32_t vbc = min(vb, vb_hi - 1);
+            while (cur + 1 < np && pref[cur + 1] <= vbc) cur++;
+            R.gs[u] = (pstart[cur] + (vbc - pref[cur])) * BLOCK + (t & 31);
+            R.cent[u] = pcent[cur];
+            const uint4* cp = a.codes + CW * (size_t)R.gs[u];
 #pragma unroll
-                for (int w = 0; w < CW; w++) R.cw[u][w] = cp[w];
-                R.mul[u] = a.mul[R.gs[u]];
-            }
+            for (int w = 0; w < CW; w++) R.cw[u][w] = cp[w];
+            R.mul[u] = a.mul[R.gs[u]];
+            if constexpr (L2) R.x2[u] = a.err[R.gs[u]];
         }
     };
     auto score = [&](const Regs& R, uint32_t round) {
@@ -315,7 +383,7 @@ void k_scan_t(ScanArgs a) {
                         sc += tab[g * 16 + ((w[d] >> (4 * j)) & 15u)];
                     }
                 float est = R.cent[u] + R.mul[u] * (qab.x * (float)sc + qab.y);
-                if constexpr (L2) est = 2.0f * est - a.err[R.gs[u]];
+                if constexpr (L2) est = 2.0f * est - R.x2[u];
                 est = est + 0.0f;  // -0.0 -> +0.0 so keys order like the floats compare
                 key[u] = f2key(est);
                 if constexpr (MODE == SCAN_APPEND) {
@@ -325,7 +393,7 @@ void k_scan_t(ScanArgs a) {
                 }
             }
         }
-        if constexpr (MODE != SCAN_APPEND) {
+        if (MODE != SCAN_APPEND && !lazy) {
             uint32_t id[SLOTS_PER_THREAD];
 #pragma unroll
             for (int u = 0; u < SLOTS_PER_THREAD; u++) id[u] = pass[u] ? a.ids[R.gs[u]] : PAD;
@@ -374,13 +442,13 @@ void k_scan_t(ScanArgs a) {
     // barrier r + 1.
     unsigned long long g = 0;
     auto step = [&](Regs& now, Regs& next, uint32_t vb0, uint32_t round) {
-        if constexpr (PF) {
-            if (vb0 + ROUND_BLOCKS < vb_hi) fetch(vb0 + ROUND_BLOCKS, next);
-        } else {
-            fetch(vb0, now);
-        }
-        peek(round, g);
+        peek_load(g);
+        // The last round's prefetch re-reads the chunk's last block (unused): no condition
+        // around the loads.
+        if constexpr (PF) fetch(vb0 + ROUND_BLOCKS, next);
+        else fetch(vb0, now);
         score(now, round);
+        peek(round, g);
         __syncthreads();
         settle(round, g);
     };
@@ -394,7 +462,9 @@ void k_scan_t(ScanArgs a) {
     // Streams m listed entries through the buffer under the running threshold; with check_ids
     // (entries the cluster pass appended unchecked) an entry past the threshold is admitted only
     // if its slot holds a row the view shows (src/ivf.rs:669).
-    auto merge_list = [&](const uint2* lst, int m, bool check_ids) {
+    // `coherent`: the list was written by another workgroup of this launch, with agent-scope
+    // atomic stores; its entries are read with agent-scope atomic loads.
+    auto merge_list = [&](const uint2* lst, int m, bool check_ids, bool coherent = false) {
         for (int i0 = 0; i0 < m; i0 += SCAN_THREADS * SLOTS_PER_THREAD) {
             const bool has_thr = misc[M_HAS] != 0;
             const uint32_t thr = misc[M_THR], thr_lo = misc[M_THR_LO];
@@ -404,7 +474,14 @@ void k_scan_t(ScanArgs a) {
 #pragma unroll
             for (int u = 0; u < SLOTS_PER_THREAD; u++) {
                 const int i = i0 + u * SCAN_THREADS + t;
-                e[u] = i < m ? lst[i] : make_uint2(0u, 0u);
+                if (i < m && coherent) {
+                    const unsigned long long v =
+                        __hip_atomic_load(reinterpret_cast<unsigned long long*>(const_cast<uint2*>(lst)) + i,
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    e[u] = make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+                } else {
+                    e[u] = i < m ? lst[i] : make_uint2(0u, 0u);
+                }
                 pass[u] = i < m && (!has_thr || e[u].x > thr || (e[u].x == thr && e[u].y > thr_lo));
             }
             if (check_ids) {
@@ -426,6 +503,9 @@ void k_scan_t(ScanArgs a) {
                 scan_compact(buf, tmp, (int)misc[M_COUNT], cand, hist, misc, nullptr);
             round++;
         }
+        // Every listed entry was checked (a published list, or the ids read above).
+        if (t == 0) misc[M_VALID] = misc[M_COUNT];
+        __syncthreads();
     };
     if (PF && vb_lo < vb_hi) fetch(vb_lo, ra);
     for (uint32_t vb0 = vb_lo; vb0 < vb_hi;) {
@@ -439,37 +519,47 @@ void k_scan_t(ScanArgs a) {
     if constexpr (MODE == SCAN_BOUND) {
         // The chunk's best entries: every one is an admissible slot (id read, view checked).
         const int b = min(BOUND_PER_CHUNK, cand);
-        if ((int)misc[M_COUNT] > b) scan_compact(buf, tmp, (int)misc[M_COUNT], b, hist, misc, nullptr);
+        cut(b, nullptr);
         const int n = min((int)misc[M_COUNT], b);
         const size_t at = qi * a.chunk_max + ci;
         if (t < n) a.part[at * BOUND_PER_CHUNK + t] = buf[t];
         if (t == 0) a.part_n[at] = (uint32_t)n;
         return;
     }
+    cut(cand, gthr);
     id_reads = wave_sum(id_reads);
     if (lane == 0 && a.scanned) atomicAdd(a.scanned + 2, (unsigned long long)id_reads);
     id_reads = 0;
-    if ((int)misc[M_COUNT] > cand) scan_compact(buf, tmp, (int)misc[M_COUNT], cand, hist, misc, gthr);
 
     if (nch > 1) {
         // Publish this chunk's list; the last chunk of the query to finish merges them all.
+        // The list and its length are written with agent-scope atomic stores, which are coherent
+        // across the XCDs' caches, and every store has completed (s_waitcnt) before the chunk
+        // counts itself done; the last chunk reads them with agent-scope atomic loads. No
+        // agent-scope fence: on this GPU one writes back and invalidates the whole L2 of the
+        // XCD, stalling every workgroup there (a batch of split queries issues thousands).
         const size_t slot0 = qi * a.chunk_max;
         const int n = (int)misc[M_COUNT];
-        uint2* mine = a.part + (slot0 + ci) * (size_t)cand;
-        for (int i = t; i < n; i += SCAN_THREADS) mine[i] = buf[i];
-        if (t == 0) a.part_n[slot0 + ci] = (uint32_t)n;
-        __threadfence();
+        unsigned long long* mine = reinterpret_cast<unsigned long long*>(a.part + (slot0 + ci) * (size_t)cand);
+        for (int i = t; i < n; i += SCAN_THREADS)
+            __hip_atomic_store(mine + i, ((unsigned long long)buf[i].y << 32) | buf[i].x, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        if (t == 0) __hip_atomic_store(a.part_n + slot0 + ci, (uint32_t)n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_s_waitcnt(0);  // every store of this thread acknowledged
         __syncthreads();
-        if (t == 0) misc[M_LAST] = atomicAdd(a.qdone + qi, 1u) == nch - 1 ? 1u : 0u;
+        if (t == 0)
+            misc[M_LAST] = __hip_atomic_fetch_add(a.qdone + qi, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                                   nch - 1
+                               ? 1u
+                               : 0u;
         __syncthreads();
         if (!misc[M_LAST]) return;
-        __threadfence();
         // Stream the other chunks' entries through the buffer (own entries are in it already).
         for (uint32_t c = 0; c < nch; c++) {
             if (c == ci) continue;
             const int m = (int)min((uint32_t)cand, __hip_atomic_load(a.part_n + slot0 + c, __ATOMIC_RELAXED,
                                                                      __HIP_MEMORY_SCOPE_AGENT));
-            merge_list(a.part + (slot0 + c) * (size_t)cand, m, false);
+            merge_list(a.part + (slot0 + c) * (size_t)cand, m, false, true);
         }
         if ((int)misc[M_COUNT] > cand) scan_compact(buf, tmp, (int)misc[M_COUNT], cand, hist, misc, nullptr);
         if (t == 0) {  // ready for the next launch
@@ -870,9 +960,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // ONEHOT: where the one-hot A operands come from: CM_OH_LDS a 16-entry LDS table (2 VALU +
-// 1 LDS read per step), CM_OH_MIXED the table on even steps and shifts and selects (about 7
-// VALU) on odd ones, halving the LDS reads.
-constexpr int CM_OH_LDS = 1, CM_OH_MIXED = 2;
+// 1 LDS read per step); any other value takes the table on even steps and shifts and selects
+// (about 7 VALU) on odd ones, halving the LDS reads.
+constexpr int CM_OH_LDS = 1;
 template <int ONEHOT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_scan_cm(CmArgs a) {
     __shared__ uint4 onehot[16];
@@ -1217,7 +1307,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         const uint32_t s_n = __builtin_amdgcn_readfirstlane(item.y);
         const uint32_t qpos0 = __builtin_amdgcn_readfirstlane(item.z);
         const uint32_t nvalid = __builtin_amdgcn_readfirstlane(item.w);
-        const uint32_t s_hi = s_lo + s_n;
         const bool two = nvalid > 16;
         // B operands: lane (j, r) holds query j's table entries of groups 4 st + r, minus 128.
         uint4 B0[16], B1[16];
