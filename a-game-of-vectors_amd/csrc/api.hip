@@ -1025,7 +1025,7 @@ int sieve_index_set_cluster_scan(sieve_index* ix, uint32_t min_queries, uint32_t
         ix->cm_min_queries = min_queries ? min_queries : 64;
         ix->cm_seed_slots = seed_slots;
         ix->cm_ext_cap = ext_cap;
-        ix->cm_chunk_blocks = chunk_blocks ? chunk_blocks : 64;
+        ix->cm_chunk_blocks = chunk_blocks;
         ix->cm_phases = phases ? phases : 2;
     });
 }

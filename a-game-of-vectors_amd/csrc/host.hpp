@@ -130,13 +130,14 @@ struct sieve_index {
     uint32_t scan_chunk_blocks = 0;  // 0: chunk size chosen per batch (sieve_index_set_scan_chunk)
     // Cluster-major scan (sieve_index_set_cluster_scan): batches of at least cm_min_queries
     // unfiltered queries; seed pass of cm_seed_slots slots and appended lists of cm_ext_cap
-    // entries per query (0: 24 and 64 * cand); cm_chunk_blocks blocks per work item; the work
-    // list in cm_phases launches.
-    uint32_t cm_min_queries = 64, cm_seed_slots = 0, cm_ext_cap = 0, cm_chunk_blocks = 64, cm_phases = 2;
+    // entries per query (0: 24 and 64 * cand); cm_chunk_blocks blocks per work item (0: 128
+    // for variant 10, 64 for variant 11, the fastest of each on the 113.5M batch); the work list
+    // in cm_phases launches.
+    uint32_t cm_min_queries = 64, cm_seed_slots = 0, cm_ext_cap = 0, cm_chunk_blocks = 0, cm_phases = 2;
     // Below cm_min_queries, unfiltered batches of at most small_max_queries queries use the
     // small-batch scan (per-chunk bounds, then appended lists); larger ones the per-query scan
-    // (on the 113.5M index the small-batch scan is faster up to about 8 queries).
-    uint32_t small_max_queries = 8;
+    // (on the 113.5M index the small-batch scan is faster up to about 12 queries).
+    uint32_t small_max_queries = 12;
 
     ~sieve_index() {
         void* ps[] = {cents, cents_bf, block_off, cl_slots, codes, mul, err, ids, rotation, quant, norm256, t2_sigma,
@@ -485,7 +486,9 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
             // Work items: sum over segments of ceil(queries / 32) * ceil(blocks / chunk) <= the
             // bound below (pairs * max blocks / (32 chunk) + pairs / 32 + blocks / chunk + segments).
             const double pairs = (double)nq * pstride * (a.dblock_off ? 2 : 1);
-            const double chunk = (double)std::max<uint32_t>(1, ix->cm_chunk_blocks);
+            const uint32_t cm_chunk =
+                ix->cm_chunk_blocks ? ix->cm_chunk_blocks : (variant == SCAN_VARIANT_CM ? 128u : 64u);
+            const double chunk = (double)cm_chunk;
             const double maxnb = std::max<double>(ix->max_cluster_blocks, lv ? lv->dblocks : 0);
             const double allnb = (double)ix->nblocks + (lv ? lv->dblocks : 0);
             const double bound = pairs * maxnb / (32.0 * chunk) + pairs / 32.0 + allnb / chunk + cm.nseg + 2.0;
@@ -495,7 +498,7 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
             cm.phases = std::max<uint32_t>(1, ix->cm_phases);
             cm.work = sgrow<uint32_t>(s, B_CM_WORK, cm.phases);
             cm.seed_slots = ix->cm_seed_slots ? ix->cm_seed_slots : 24 * cand;
-            cm.chunk_blocks = std::max<uint32_t>(1, ix->cm_chunk_blocks);
+            cm.chunk_blocks = cm_chunk;
             a.ext_cap = ix->cm_ext_cap ? ix->cm_ext_cap : 64 * cand;
             a.seed = sgrow<uint2>(s, B_CM_SEED, nq * (size_t)cand);
             a.seed_n = sgrow<uint32_t>(s, B_CM_SEEDN, nq);

@@ -199,7 +199,7 @@ class Index:
         abi.check(abi.lib().sieve_index_set_cluster_scan(self._h, min_queries, seed_slots, ext_cap, chunk_blocks,
                                                          phases))
 
-    def set_small_batch(self, max_queries=8):
+    def set_small_batch(self, max_queries=12):
         """Largest unfiltered batch for the small-batch scan of the cluster-major variants
         (sieve_index_set_small_batch); 0 disables it."""
         abi.check(abi.lib().sieve_index_set_small_batch(self._h, max_queries))

@@ -243,7 +243,7 @@ int sieve_index_set_scan_chunk(sieve_index* ix, uint32_t chunk_blocks);
  * without an allow filter; smaller unfiltered batches of at most the small-batch limit run the
  * small-batch scan (sieve_index_set_small_batch), the rest scan variant 2. Results equal the
  * per-query scan's. 0 keeps a setting's default (min_queries 64, seed_slots 24 * cand, ext_cap
- * 64 * cand, chunk_blocks 64, phases 2). */
+ * 64 * cand, chunk_blocks 128 for variant 10 and 64 for variant 11, phases 2). */
 int sieve_index_set_cluster_scan(sieve_index* ix, uint32_t min_queries, uint32_t seed_slots, uint32_t ext_cap,
                                  uint32_t chunk_blocks, uint32_t phases);
 /* The small-batch scan of the cluster-major variants (no counterpart in the reference): for an
@@ -251,7 +251,7 @@ int sieve_index_set_cluster_scan(sieve_index* ix, uint32_t min_queries, uint32_t
  * keeps its best 16 admissible slots, the (cand + 1)-th best of those is the query's
  * admission threshold, every slot above it is appended to the query's list (ext_cap entries,
  * as above) and the list is merged into the query's candidates; an overflowing list is
- * rescanned. Results equal the per-query scan's. max_queries 0 disables it (default 8). */
+ * rescanned. Results equal the per-query scan's. max_queries 0 disables it (default 12). */
 int sieve_index_set_small_batch(sieve_index* ix, uint32_t max_queries);
 
 /* ------------------------------------------------------------- the path --- */
