@@ -284,7 +284,7 @@ def test_query_i8_closed_form(rng):
 
 # ----------------------------------------------------------------------- TopK ---------
 def test_topk_matches_full_sort(rng):
-    # topk.rs:257-270
+    # topk.rs:64-82
     for _ in range(100):
         n = int(rng.integers(0, 2000))
         k = int(rng.integers(1, 64))
@@ -307,7 +307,7 @@ def test_topk_f32_order_and_ids(rng):
 
 # ----------------------------------------------------------------------- tier2 --------
 def test_tier2_table_score_matches_reconstruction():
-    # tier2.rs:326-341 with its xorshift fixture.
+    # tier2.rs:81-96 with its xorshift fixture.
     x = [0x2545F4914F6CDD1D]
 
     def rnd():
@@ -346,6 +346,6 @@ def test_prune_threshold_is_mth_lower_bound():
 
 
 def test_recall_counts_overlap():
-    # eval.rs:241-244
+    # eval.rs:61-64
     from sieve_amd.evaluate import recall_at
     assert recall_at(2, [1, 2, 3], [2, 9, 1]) == 0.5
