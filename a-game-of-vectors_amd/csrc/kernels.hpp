@@ -88,6 +88,11 @@ struct RouteArgs {
     float err_coef;            // |approx - exact| <= err_coef * qnorm
 };
 void launch_route_select(const RouteArgs& a, size_t nq, hipStream_t st);
+// (primary, secondary) clusters of n unit rows from their best-first candidate lists (pc, pn,
+// row stride pstride): the first candidate and soar_secondary over the first 16 (src/ivf.rs:164-186),
+// or PAD as the secondary without SOAR.
+void launch_soar_pairs(const float* xs, size_t n, const float* cents, const uint32_t* pc, const uint32_t* pn,
+                       int pstride, int soar, float lambda, uint32_t* pairs, hipStream_t st);
 // out[0..3] = cnt[0..3], out[4] = sum of pn[0, nq), out[5] = *ovf, out[6] = cnt[4] (one workgroup).
 void launch_pack_stats(const unsigned long long* cnt, const uint32_t* pn, size_t nq, const uint32_t* ovf,
                        unsigned long long* out, hipStream_t st);

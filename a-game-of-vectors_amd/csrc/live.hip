@@ -435,6 +435,13 @@ void live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, size_t n
 
 namespace sieve {
 
+void launch_soar_pairs(const float* xs, size_t n, const float* cents, const uint32_t* pc, const uint32_t* pn,
+                       int pstride, int soar, float lambda, uint32_t* pairs, hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_live_pairs, dim3(grid(n, 64)), dim3(64), 0, st, xs, n, cents, pc, pn, pstride, soar, lambda,
+                       pairs);
+}
+
 LiveArgs live_args(const sieve_live* l) { return l->args(); }
 LiveArgs live_args_max(const sieve_live* l) {
     LiveArgs a = l->args();

@@ -214,6 +214,9 @@ def lib():
         "sieve_live_get_info": (i32, [vp, ctypes.POINTER(LiveInfo)]),
         "sieve_live_export": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "sieve_live_search_batch": (i32, [vp, vp, vp, vp, sz, P, V, vp, vp, vp, T]),
+        "sieve_build_kmeans": (i32, [vp, sz, u32, u32, ctypes.c_uint64, vp]),
+        "sieve_build_assign": (i32, [vp, sz, vp, u32, ctypes.c_float, vp]),
+        "sieve_build_tier2_fit": (i32, [vp, sz, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -14,10 +14,12 @@ is given (the MI355X for large corpora). It follows the reference's recipe:
 
 Element-wise steps reproduce the reference's f32 arithmetic exactly (sums run sequentially in
 the reference's order; no fused multiply-add), so the codes and [mul, err] factors written
-for a given (row, cluster) are bit-identical to the reference's. The GEMMs behind k-means and
-cluster assignment run on torch's f32 GEMM, whose summation order differs from the
-reference's sgemm; rows within rounding of a tie between two centroids may be assigned
-differently.
+for a given (row, cluster) are bit-identical to the reference's. On the GPU, k-means, the
+assignment (assign_top + SOAR) and the tier-2 fit run in the library (sieve_build_kmeans,
+sieve_build_assign, sieve_build_tier2_fit): every score is the reference's sequential f32 dot,
+so the whole index is the reference's. On the CPU (test fixtures) they run on torch's f32 GEMM,
+whose summation order differs; there a row within rounding of a tie between two centroids may
+be assigned differently.
 """
 import json
 import os
@@ -244,8 +246,21 @@ def assign_top(rows, cents, m, block=65536):
     return torch.cat(out).to(torch.int64)
 
 
+def _lib():
+    from . import abi
+    return abi
+
+
 def kmeans(sample, k, iters, seed=7):
-    """Spherical k-means (src/ivf.rs:188-225, balance off)."""
+    """Spherical k-means (src/ivf.rs:188-225, balance off). On the GPU through
+    sieve_build_kmeans, bit-exact with the reference; on the CPU with torch's f32 GEMM and
+    scatter sums (a row within rounding of a tie may be assigned differently)."""
+    if sample.is_cuda:
+        abi = _lib()
+        x = sample.contiguous()
+        cents = torch.empty(k, PDIM, dtype=torch.float32, device=sample.device)
+        abi.check(abi.lib().sieve_build_kmeans(x.data_ptr(), x.shape[0], k, iters, seed, cents.data_ptr()))
+        return cents
     n = sample.shape[0]
     rng = Rng(seed)
     idx = [rng.below(n) for _ in range(k)]
@@ -259,6 +274,25 @@ def kmeans(sample, k, iters, seed=7):
         for c in empty:
             cents[c] = sample[rng.below(n)]
     return cents
+
+
+def assign_pairs(x, cents, soar_lambda, soar_cand=16):
+    """The clusters of each row as ivf::build assigns them (src/ivf.rs:313-321): (n, 2) primary and
+    SOAR secondary, or (n, 1) without SOAR. On the GPU through sieve_build_assign (bit-exact with
+    the reference up to exact f32 ties); on the CPU with torch's f32 GEMM."""
+    if x.is_cuda:
+        abi = _lib()
+        xc, cc = x.contiguous(), cents.contiguous()
+        pairs = torch.empty(xc.shape[0], 2, dtype=torch.int32, device=x.device)
+        lam = float("nan") if soar_lambda is None else float(soar_lambda)
+        abi.check(abi.lib().sieve_build_assign(xc.data_ptr(), xc.shape[0], cc.data_ptr(), cc.shape[0], lam,
+                                               pairs.data_ptr()))
+        pairs = pairs.to(torch.int64)
+        return pairs[:, :1] if soar_lambda is None else pairs
+    if soar_lambda is None:
+        return assign_top(x, cents, 1)
+    top = assign_top(x, cents, soar_cand)
+    return torch.stack((top[:, 0], soar_secondary(x, cents, top, soar_lambda)), dim=1)
 
 
 def soar_secondary(x, cents, cand, lam):
@@ -278,7 +312,14 @@ def soar_secondary(x, cents, cand, lam):
 
 # ----------------------------------------------------------------------- tier2 ---------
 def tier2_fit(sample):
-    """Tier2::fit (src/tier2.rs:23-31)."""
+    """Tier2::fit (src/tier2.rs:20-27). On the GPU through sieve_build_tier2_fit (sequential f64
+    sums, as the reference's); on the CPU with torch's f64 sum."""
+    if sample.is_cuda:
+        abi = _lib()
+        x = sample.contiguous()
+        sigma = torch.empty(PDIM, dtype=torch.float32, device=sample.device)
+        abi.check(abi.lib().sieve_build_tier2_fit(x.data_ptr(), x.shape[0], sigma.data_ptr()))
+        return sigma
     n = float(np.float32(max(sample.shape[0], 1)))
     ss = (sample * sample).double().sum(0)
     return torch.clamp((ss / torch.full_like(ss, n)).sqrt().float(), min=1e-6)
@@ -376,11 +417,7 @@ def build(corpus: Corpus, k, sample_n=None, iters=12, soar_lambda=1.0, rabitq=Tr
         x = prefix_f32(corpus, rows)
         if tier2:
             t2_codes[s:s + m], t2_norm[s:s + m] = tier2_encode(x, t2_sigma)
-        if soar_lambda is None:
-            pairs = assign_top(x, cents, 1)
-        else:
-            top = assign_top(x, cents, soar_cand)
-            pairs = torch.stack((top[:, 0], soar_secondary(x, cents, top, soar_lambda)), dim=1)
+        pairs = assign_pairs(x, cents, soar_lambda, soar_cand)
         xr = rotate(x, signs) if rabitq else None
         for h in range(pairs.shape[1]):
             c = pairs[:, h]

@@ -580,6 +580,29 @@ int sieve_live_search_batch(const sieve_live* l, sieve_stream* s, const float* q
                             const sieve_params* p, const sieve_view* view, float* scores, uint32_t* ext_ids,
                             uint32_t* counts, sieve_timing* t);
 
+/* ------------------------------------------------------------ index build --- */
+/* The GEMM-shaped steps of ivf::build (src/ivf.rs:264-395), on the current device, bit-exact
+ * with the reference: every row-centroid score is the reference's sequential f32 dot (the x86
+ * gemm_abt, src/blas.rs:43-54), reached through the routing GEMM and the exact rescoring of
+ * its error band. Arrays may be host or device memory; each call waits for the device before
+ * it starts and returns with its outputs written. */
+
+/* kmeans (src/ivf.rs:188-225) without the balance split: spherical k-means of n unit rows
+ * (sample, n x 256 f32) into k centroids (cents, k x 256): initial centroids
+ * sample[rng.below(n)] drawn k times from Rng(seed), then iters Lloyd steps of assign (the first
+ * best centroid per row), per-cluster sums of the members in sample order and
+ * quant::normalize; an empty cluster takes sample[rng.below(n)] (drawn in cluster order). */
+int sieve_build_kmeans(const float* sample, size_t n, uint32_t k, uint32_t iters, uint64_t seed, float* cents);
+/* The clusters of n unit rows (rows, n x 256) against k centroids (cents, k x 256), as the build
+ * assigns them (src/ivf.rs:313-321): pairs[2i] = the best centroid; pairs[2i + 1] =
+ * soar_secondary (src/ivf.rs:164-186) over the 16 best of assign_top, or 0xFFFFFFFF when
+ * soar_lambda is NaN (no SOAR spill). Ties at equal f32 scores go to the lower centroid id. */
+int sieve_build_assign(const float* rows, size_t n, const float* cents, uint32_t k, float soar_lambda,
+                       uint32_t* pairs);
+/* Tier2::fit (src/tier2.rs:20-27): sigma (256 f32) of n rows (n x 256): per dimension the f64
+ * sum of x * x in row order, over n, its square root, at least 1e-6. */
+int sieve_build_tier2_fit(const float* rows, size_t n, float* sigma);
+
 #ifdef __cplusplus
 }
 #endif

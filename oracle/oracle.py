@@ -147,6 +147,8 @@ def lib():
             "or_search_latency": (None, [vp, vp, vp, sz, ctypes.POINTER(_Params), vp]),
             "or_assign_top": (None, [vp, sz, vp, sz, sz, vp, i32]),
             "or_soar_secondary": (u32, [vp, vp, vp, sz, f32]),
+            "or_kmeans": (None, [vp, sz, sz, sz, ctypes.c_uint64, vp, i32]),
+            "or_build_pairs": (None, [vp, sz, vp, sz, f32, vp, i32]),
             "or_l2_leaf_norms": (None, [vp, sz, vp]),
             "or_l2_create": (vp, [u32, vp, vp, vp, vp, vp, vp, vp, vp]),
             "or_l2_free": (None, [vp]),
@@ -408,6 +410,24 @@ def assign_top(rows, cents, m, threads=8):
 def soar_secondary(x, cents, cand, lam):
     x, cents, cand = _c(x, np.float32), _c(cents, np.float32), _c(cand, np.uint32)
     return lib().or_soar_secondary(_p(x), _p(cents), _p(cand), cand.shape[0], lam)
+
+
+def kmeans(sample, k, iters, seed=7, threads=8):
+    """kmeans (src/ivf.rs:188-225, balance None): k x 256 centroids."""
+    sample = _c(sample, np.float32)
+    out = np.empty((k, PDIM), np.float32)
+    lib().or_kmeans(_p(sample), sample.shape[0], k, iters, seed, _p(out), threads)
+    return out
+
+
+def build_pairs(rows, cents, lam, threads=8):
+    """(primary, secondary) cluster of each row as ivf::build assigns them (src/ivf.rs:313-321);
+    lam None: no SOAR (secondary 0xFFFFFFFF)."""
+    rows, cents = _c(rows, np.float32), _c(cents, np.float32)
+    out = np.empty((rows.shape[0], 2), np.uint32)
+    lib().or_build_pairs(_p(rows), rows.shape[0], _p(cents), cents.shape[0],
+                         float("nan") if lam is None else lam, _p(out), threads)
+    return out
 
 
 # ------------------------------------------------------------------ params / view ----

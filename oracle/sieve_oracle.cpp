@@ -1004,6 +1004,54 @@ uint32_t or_soar_secondary(const float* x, const float* cents, const uint32_t* c
     return bc;
 }
 
+// kmeans, src/ivf.rs:188-225, with balance None (no split_large): initial centroids drawn
+// from Rng(seed), then per iteration assign (the first best centroid), the members' sums in
+// sample order from +0.0, quant::normalize, and a fresh sample row for each empty cluster in
+// cluster order.
+void or_kmeans(const float* sample, size_t n, size_t k, size_t iters, uint64_t seed, float* cents, int threads) {
+    Rng rng(seed);
+    for (size_t c = 0; c < k; c++) std::memcpy(cents + c * PDIM, sample + rng.below(n) * PDIM, PDIM * sizeof(float));
+    std::vector<uint32_t> a(n);
+    std::vector<float> sums(k * PDIM);
+    std::vector<size_t> counts(k);
+    for (size_t it = 0; it < iters; it++) {
+        or_assign_top(sample, n, cents, k, 1, a.data(), threads);
+        std::fill(sums.begin(), sums.end(), 0.0f);
+        std::fill(counts.begin(), counts.end(), 0);
+        for (size_t i = 0; i < n; i++) {
+            counts[a[i]]++;
+            float* s = sums.data() + (size_t)a[i] * PDIM;
+            const float* x = sample + i * PDIM;
+            for (int d = 0; d < PDIM; d++) s[d] = s[d] + x[d];
+        }
+        for (size_t c = 0; c < k; c++) {
+            float* dst = cents + c * PDIM;
+            if (counts[c] == 0) {
+                std::memcpy(dst, sample + rng.below(n) * PDIM, PDIM * sizeof(float));
+            } else {
+                std::memcpy(dst, sums.data() + c * PDIM, PDIM * sizeof(float));
+                normalize(dst, PDIM);
+            }
+        }
+    }
+}
+
+// The (primary, secondary) clusters of each row as ivf::build assigns them (src/ivf.rs:313-321):
+// assign_top(16) and soar_secondary, or (assign, PAD) when lambda is NaN.
+void or_build_pairs(const float* rows, size_t n, const float* cents, size_t k, float lambda, uint32_t* pairs,
+                    int threads) {
+    const bool soar = lambda == lambda;
+    const size_t m = soar ? std::min<size_t>(16, k) : 1;
+    std::vector<uint32_t> top(n * 16, 0u);  // a list shorter than 16 reads cluster 0 past its end
+    std::vector<uint32_t> t(n * m);
+    or_assign_top(rows, n, cents, k, m, t.data(), threads);
+    for (size_t i = 0; i < n; i++) {
+        for (size_t j = 0; j < m; j++) top[i * 16 + j] = t[i * m + j];
+        pairs[2 * i] = top[i * 16];
+        pairs[2 * i + 1] = soar ? or_soar_secondary(rows + i * PDIM, cents, top.data() + i * 16, 16, lambda) : PAD;
+    }
+}
+
 // ---- L2 path over 100-d int8 rows (src/bigann.rs) -----------------------------------
 // Index arrays: leaves k x 100 f32, leaf norms k, block_off k+1, blocks nblocks x 512 B,
 // factors nblocks x 32 x [mul, |x|^2], ids nblocks x 32, rotation 3 x 128 signs,
