@@ -217,6 +217,8 @@ def lib():
         "sieve_build_kmeans": (i32, [vp, sz, u32, u32, ctypes.c_uint64, vp]),
         "sieve_build_assign": (i32, [vp, sz, vp, u32, ctypes.c_float, vp]),
         "sieve_build_tier2_fit": (i32, [vp, sz, vp]),
+        "sieve_build_l2_train": (i32, [vp, sz, u32, u32, sz, u32, vp, vp]),
+        "sieve_build_l2_assign": (i32, [vp, sz, vp, u32, vp, u32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -27,6 +27,7 @@ class L2Arrays:
     factors: torch.Tensor     # nblocks*32 x 2 f32: [mul, |x|^2]
     rotation: np.ndarray      # 384 f32
     base: torch.Tensor        # n x 100 int8
+    coarse: torch.Tensor = None  # k1 x 100 f32: the coarse cells of a two-level build (Model::coarse)
 
     def numpy(self):
         return dict(leaves=self.leaves.cpu().numpy(), block_off=self.block_off.cpu().numpy().astype(np.uint64),
@@ -263,60 +264,110 @@ class L2Mixture:
         return torch.clamp(q, -127, 127).to(torch.int8)
 
 
-def two_level_leaves(x, coarse, fine, iters, seed):
-    """Leaves from a two-level k-means as bigann trains them (src/bigann.rs:175-226): `coarse`
-    cells over the sample, then `fine` leaves per cell over the cell's rows (a cell with fewer
-    rows repeats its rows). Returns (coarse centroids, leaves coarse * fine x 100)."""
-    cents = kmeans_l2(x, coarse, iters, seed)
-    a = (2.0 * x @ cents.T - (cents * cents).sum(1)).argmax(1)
-    leaves = torch.empty(coarse * fine, D, dtype=torch.float32, device=x.device)
-    order = torch.argsort(a)
-    counts = torch.bincount(a, minlength=coarse).tolist()
-    o = 0
-    for c in range(coarse):
-        xc = x[order[o:o + counts[c]]]
-        o += counts[c]
-        if xc.shape[0] == 0:
-            xc = cents[c:c + 1]
-        if xc.shape[0] < fine:
-            xc = xc.repeat((fine + xc.shape[0] - 1) // xc.shape[0], 1)
-        leaves[c * fine:(c + 1) * fine] = kmeans_l2(xc, fine, iters, sub_seed(seed, 14, c) & 0x7FFFFFFF)
-    return cents, leaves
+def rng_below(seed, count, n):
+    """`count` draws of Rng(seed).below(n) (splitmix64, src/ivf.rs:120-135), vectorised: the i-th
+    state is seed + i * 0x9E3779B97F4A7C15 (mod 2^64)."""
+    i = np.arange(1, count + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + i * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z % np.uint64(n)).astype(np.int64)
 
 
-def build_l2_large(base, coarse=512, fine=256, iters=8, seed=7, sample=1 << 24, chunk=1 << 22, log=None):
-    """build_l2 for base tables too large for whole-table temporaries (1B x 100 on one GPU):
-    two-level leaves; each row assigned to the nearest of its coarse cell's leaves (rows sorted
-    by cell, one GEMM per cell); codes, mul and |x|^2 per chunk; slots sorted by (leaf, row)
-    and packed into 512-byte blocks a chunk of blocks at a time. The arrays keep the layout of
-    build_l2's. Peak device memory is about the base plus 90 bytes per row."""
+def _kmeans_l2_torch(x, k, iters, seed):
+    """kmeans_l2 (src/bigann.rs:139-161) with torch's f32 GEMM for the scores (CPU fixtures):
+    f64 sums in sample order, means as f32, empty clusters reseeded in cluster order."""
+    from .build import Rng
+    n = x.shape[0]
+    rng = Rng(seed)
+    cents = x[[rng.below(n) for _ in range(k)]].clone()
+    for _ in range(iters):
+        a = (2.0 * x @ cents.T - (cents * cents).sum(1)).argmax(1)
+        sums = torch.zeros(k, D, dtype=torch.float64).index_add_(0, a, x.double())
+        cnt = torch.bincount(a, minlength=k)
+        upd = (sums / cnt.clamp(min=1)[:, None].double()).float()
+        cents = torch.where((cnt > 0)[:, None], upd, cents)
+        for c in (cnt == 0).nonzero().flatten().tolist():
+            cents[c] = x[rng.below(n)]
+    return cents
+
+
+def train_l2(base, coarse, fine, iters=8, sample=1 << 24):
+    """bigann::train (src/bigann.rs:175-227): the Rng(42) sample (indices sorted), coarse cells
+    over rows at an even stride through it, every sample row to its nearest cell, and `fine`
+    leaves per cell. On the GPU through sieve_build_l2_train, bit-exact with the reference; on
+    the CPU (test fixtures) with torch's f32 GEMM. Returns (coarse k1 x 100, leaves k1 k2 x 100)."""
+    n = base.shape[0]
+    if not base.is_cuda:
+        sample_n = min(n, sample)
+        idx = np.sort(rng_below(42, sample_n, n))
+        xs = base[torch.as_tensor(idx)].to(torch.float32)
+        coarse_n = min(sample_n, coarse * 512)
+        crow = xs[::sample_n // coarse_n][:coarse_n]
+        cells = _kmeans_l2_torch(crow, coarse, iters, 7)
+        a = (2.0 * xs @ cells.T - (cells * cells).sum(1)).argmax(1)
+        leaves = torch.empty(coarse * fine, D, dtype=torch.float32)
+        for c in range(coarse):
+            xc = xs[a == c]
+            leaves[c * fine:(c + 1) * fine] = (_kmeans_l2_torch(xc, fine, iters, c) if xc.shape[0]
+                                               else cells[c:c + 1].repeat(fine, 1))
+        return cells, leaves
+    from . import abi
+    cells = torch.empty(coarse, D, dtype=torch.float32, device=base.device)
+    leaves = torch.empty(coarse * fine, D, dtype=torch.float32, device=base.device)
+    abi.check(abi.lib().sieve_build_l2_train(base.data_ptr(), n, coarse, fine, min(n, sample), iters,
+                                             cells.data_ptr(), leaves.data_ptr()))
+    return cells, leaves
+
+
+def assign_l2(base, cells, leaves, fine, chunk=1 << 16):
+    """Model::assign (src/bigann.rs:277-302): each row's leaf, the best child of its 4 nearest
+    coarse cells (a later cell in ascending order wins only if strictly better). On the GPU
+    through sieve_build_l2_assign, bit-exact with the reference; on the CPU with torch's GEMM."""
+    if not base.is_cuda:
+        cn = (cells * cells).sum(1)
+        ln = (leaves * leaves).sum(1)
+        out = torch.empty(base.shape[0], dtype=torch.int32)
+        for s0 in range(0, base.shape[0], chunk):
+            x = base[s0:s0 + chunk].to(torch.float32)
+            top = torch.topk(2.0 * x @ cells.T - cn, 4, dim=1).indices.sort(dim=1).values
+            best = torch.full((x.shape[0],), -float("inf"))
+            bl = torch.zeros(x.shape[0], dtype=torch.int64)
+            for t in range(4):
+                c = top[:, t]
+                lc = leaves.view(-1, fine, D)[c]                                   # m x fine x 100
+                sc = 2.0 * torch.einsum("md,mfd->mf", x, lc) - ln.view(-1, fine)[c]
+                j = sc.argmax(1)
+                s = sc.gather(1, j[:, None])[:, 0]
+                win = s > best
+                best = torch.where(win, s, best)
+                bl = torch.where(win, c * fine + j, bl)
+            out[s0:s0 + x.shape[0]] = bl.to(torch.int32)
+        return out
+    from . import abi
+    leaf = torch.empty(base.shape[0], dtype=torch.int32, device=base.device)
+    abi.check(abi.lib().sieve_build_l2_assign(base.data_ptr(), base.shape[0], cells.data_ptr(), cells.shape[0],
+                                              leaves.data_ptr(), fine, leaf.data_ptr()))
+    return leaf
+
+
+def build_l2_large(base, coarse=512, fine=256, iters=8, sample=1 << 24, chunk=1 << 22, log=None):
+    """The 1B build (src/bigann.rs:175-374) for a device-resident base: train_l2 (two-level
+    k-means on an Rng(42) sample), assign_l2 (each row to the best leaf of its 4 nearest cells),
+    then codes, mul and |x|^2 per chunk, slots sorted by (leaf, row) and packed into 512-byte
+    blocks a chunk of blocks at a time. Training and assignment run in the library and equal the
+    reference's bit for bit; the element-wise encode runs in the reference's f32 order. Peak
+    device memory is about the base plus 90 bytes per row."""
     say = log or (lambda m: None)
     dev = base.device
     n = base.shape[0]
     k = coarse * fine
-    xs = base[torch.randint(0, n, (min(n, sample),), device=dev)].to(torch.float32)
-    cents, leaves = two_level_leaves(xs, coarse, fine, iters, seed)
-    del xs
+    base = base.contiguous()
+    cents, leaves = train_l2(base, coarse, fine, iters, sample)
     say(f"l2 build: {coarse} x {fine} leaves")
-    cnorm = (cents * cents).sum(1)
-    lnorm = (leaves * leaves).sum(1)
-    cell = torch.empty(n, dtype=torch.int16, device=dev)
-    for s0 in range(0, n, chunk):
-        x = base[s0:s0 + chunk].to(torch.float32)
-        cell[s0:s0 + x.shape[0]] = (2.0 * x @ cents.T - cnorm).argmax(1).to(torch.int16)
-    order = torch.argsort(cell.to(torch.int32), stable=True)
-    counts = torch.bincount(cell.to(torch.int32), minlength=coarse).tolist()
-    del cell
-    leaf = torch.empty(n, dtype=torch.int32, device=dev)
-    o = 0
-    for c in range(coarse):
-        for s0 in range(o, o + counts[c], chunk):
-            rows = order[s0:min(s0 + chunk, o + counts[c])]
-            x = base[rows].to(torch.float32)
-            lc = leaves[c * fine:(c + 1) * fine]
-            leaf[rows] = ((2.0 * x @ lc.T - lnorm[c * fine:(c + 1) * fine]).argmax(1) + c * fine).to(torch.int32)
-        o += counts[c]
-    del order
+    leaf = assign_l2(base, cents, leaves, fine)
     say("l2 build: rows assigned")
     signs = rotation_signs(11, PD)
     leaves_rot = rotate(pad(leaves), signs)
@@ -359,7 +410,7 @@ def build_l2_large(base, coarse=512, fine=256, iters=8, seed=7, sample=1 << 24, 
     del slot_codes
     say("l2 build: packed")
     return L2Arrays(leaves=leaves, block_off=block_off, blocks=blocks, ids=ids, factors=factors, rotation=signs,
-                    base=base)
+                    base=base, coarse=cents)
 
 
 def exact_l2_topk(base, q, k=32, chunk=1 << 22):

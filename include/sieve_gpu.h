@@ -602,6 +602,20 @@ int sieve_build_assign(const float* rows, size_t n, const float* cents, uint32_t
 /* Tier2::fit (src/tier2.rs:20-27): sigma (256 f32) of n rows (n x 256): per dimension the f64
  * sum of x * x in row order, over n, its square root, at least 1e-6. */
 int sieve_build_tier2_fit(const float* rows, size_t n, float* sigma);
+/* bigann::train (src/bigann.rs:175-227) without the files, over n 100-d int8 rows (base, n x 100,
+ * host or device): the sample of sample_n rows drawn with Rng(42) (indices sorted), k1 coarse
+ * cells by kmeans_l2 (src/bigann.rs:139-161, seed 7) over min(sample_n, 512 k1) rows at an even
+ * stride through the sample, every sample row to its nearest cell, and k2 leaves per cell by
+ * kmeans_l2 over the cell's rows (seed = the cell; a cell without rows gets its centroid k2
+ * times). coarse: k1 x 100, leaves: k1 k2 x 100 (cell-major). Every score is top_l2's
+ * 2 <x, c> - |c|^2 in the reference's f32 order; the k-means sums are f64 in sample order. */
+int sieve_build_l2_train(const int8_t* base, size_t n, uint32_t k1, uint32_t k2, size_t sample_n, uint32_t iters,
+                         float* coarse, float* leaves);
+/* Model::assign (src/bigann.rs:277-302): each of n rows' leaf (leaf, n u32): the best child of
+ * its 4 nearest coarse cells (top_l2), the cells taken in ascending order and a later one
+ * winning only when strictly better. k1 >= 4. */
+int sieve_build_l2_assign(const int8_t* base, size_t n, const float* coarse, uint32_t k1, const float* leaves,
+                          uint32_t k2, uint32_t* leaf);
 
 #ifdef __cplusplus
 }

@@ -148,6 +148,11 @@ def lib():
             "or_assign_top": (None, [vp, sz, vp, sz, sz, vp, i32]),
             "or_soar_secondary": (u32, [vp, vp, vp, sz, f32]),
             "or_kmeans": (None, [vp, sz, sz, sz, ctypes.c_uint64, vp, i32]),
+            "or_l2_norms": (None, [vp, sz, vp]),
+            "or_top_l2": (None, [vp, sz, vp, vp, sz, sz, vp, vp, i32]),
+            "or_kmeans_l2": (None, [vp, sz, sz, sz, ctypes.c_uint64, vp, i32]),
+            "or_l2_train": (None, [vp, sz, sz, sz, sz, sz, vp, vp, i32]),
+            "or_l2_assign": (None, [vp, sz, vp, sz, vp, sz, vp, i32]),
             "or_build_pairs": (None, [vp, sz, vp, sz, f32, vp, i32]),
             "or_l2_leaf_norms": (None, [vp, sz, vp]),
             "or_l2_create": (vp, [u32, vp, vp, vp, vp, vp, vp, vp, vp]),
@@ -417,6 +422,49 @@ def kmeans(sample, k, iters, seed=7, threads=8):
     sample = _c(sample, np.float32)
     out = np.empty((k, PDIM), np.float32)
     lib().or_kmeans(_p(sample), sample.shape[0], k, iters, seed, _p(out), threads)
+    return out
+
+
+def l2_norms(cents):
+    """norms (src/bigann.rs:135-137) of k x 100 centroids."""
+    cents = _c(cents, np.float32)
+    out = np.empty(cents.shape[0], np.float32)
+    lib().or_l2_norms(_p(cents), cents.shape[0], _p(out))
+    return out
+
+
+def top_l2(rows, cents, m, threads=8):
+    """top_l2 (src/bigann.rs:117-133): (ids, scores) n x m, best first."""
+    rows, cents = _c(rows, np.float32), _c(cents, np.float32)
+    cn = l2_norms(cents)
+    ids = np.empty((rows.shape[0], m), np.uint32)
+    sc = np.empty((rows.shape[0], m), np.float32)
+    lib().or_top_l2(_p(rows), rows.shape[0], _p(cents), _p(cn), cents.shape[0], m, _p(ids), _p(sc), threads)
+    return ids, sc
+
+
+def kmeans_l2(sample, k, iters, seed, threads=8):
+    """kmeans_l2 (src/bigann.rs:139-161): k x 100 centroids."""
+    sample = _c(sample, np.float32)
+    out = np.empty((k, 100), np.float32)
+    lib().or_kmeans_l2(_p(sample), sample.shape[0], k, iters, seed, _p(out), threads)
+    return out
+
+
+def l2_train(base, k1, k2, sample_n, iters, threads=8):
+    """train (src/bigann.rs:175-227) without the files: (coarse k1 x 100, leaves k1 k2 x 100)."""
+    base = _c(base, np.int8)
+    coarse = np.empty((k1, 100), np.float32)
+    leaves = np.empty((k1 * k2, 100), np.float32)
+    lib().or_l2_train(_p(base), base.shape[0], k1, k2, sample_n, iters, _p(coarse), _p(leaves), threads)
+    return coarse, leaves
+
+
+def l2_assign(rows, coarse, leaves, k2, threads=8):
+    """Model::assign (src/bigann.rs:277-302): each row's leaf."""
+    rows, coarse, leaves = _c(rows, np.float32), _c(coarse, np.float32), _c(leaves, np.float32)
+    out = np.empty(rows.shape[0], np.uint32)
+    lib().or_l2_assign(_p(rows), rows.shape[0], _p(coarse), coarse.shape[0], _p(leaves), k2, _p(out), threads)
     return out
 
 

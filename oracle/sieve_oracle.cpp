@@ -1072,6 +1072,138 @@ struct L2Index {
     const uint8_t* extra = nullptr;
 };
 
+// ---------------------------------------------------------------- 1B build ------------
+constexpr int L2_D = 100;
+
+// norms, src/bigann.rs:135-137: sum of x * x per centroid, left to right from -0.0.
+void or_l2_norms(const float* cents, size_t k, float* out) {
+    for (size_t c = 0; c < k; c++) {
+        float s = rsum_seed();
+        for (int i = 0; i < L2_D; i++) s = s + cents[c * L2_D + i] * cents[c * L2_D + i];
+        out[c] = s;
+    }
+}
+
+// top_l2, src/bigann.rs:117-133: the m best centroids of each row by 2 <x, c> - |c|^2 (the
+// x86 gemm_abt dot), best first, ties by id (TopK::into_sorted).
+void or_top_l2(const float* rows, size_t n, const float* cents, const float* cnorm, size_t k, size_t m, uint32_t* out_id,
+               float* out_s, int threads) {
+    const size_t B = 4096;
+    std::vector<float> s;
+    for (size_t r0 = 0; r0 < n; r0 += B) {
+        const size_t rn = std::min(B, n - r0);
+        s.resize(rn * k);
+        gemm_abt(rows + r0 * L2_D, cents, rn, k, L2_D, s.data(), threads);
+        for (size_t i = 0; i < rn; i++) {
+            TopK<float> t(m);
+            for (size_t j = 0; j < k; j++) t.push(2.0f * s[i * k + j] - cnorm[j], (uint32_t)j);
+            auto so = t.into_sorted();
+            for (size_t j = 0; j < m; j++) {
+                out_id[(r0 + i) * m + j] = j < so.size() ? so[j].second : 0u;
+                out_s[(r0 + i) * m + j] = j < so.size() ? so[j].first : 0.0f;
+            }
+        }
+    }
+}
+
+// kmeans_l2, src/bigann.rs:139-161: Lloyd in L2; f64 sums in sample order, means (s / m) as f32;
+// an empty cluster takes a fresh sample row.
+void or_kmeans_l2(const float* sample, size_t n, size_t k, size_t iters, uint64_t seed, float* cents, int threads) {
+    Rng rng(seed);
+    for (size_t c = 0; c < k; c++) std::memcpy(cents + c * L2_D, sample + rng.below(n) * L2_D, L2_D * sizeof(float));
+    std::vector<float> cn(k), sc(n);
+    std::vector<uint32_t> a(n);
+    std::vector<double> sums(k * L2_D);
+    std::vector<size_t> counts(k);
+    for (size_t it = 0; it < iters; it++) {
+        or_l2_norms(cents, k, cn.data());
+        or_top_l2(sample, n, cents, cn.data(), k, 1, a.data(), sc.data(), threads);
+        std::fill(sums.begin(), sums.end(), 0.0);
+        std::fill(counts.begin(), counts.end(), 0);
+        for (size_t i = 0; i < n; i++) {
+            counts[a[i]]++;
+            for (int d = 0; d < L2_D; d++) sums[(size_t)a[i] * L2_D + d] += (double)sample[i * L2_D + d];
+        }
+        for (size_t c = 0; c < k; c++) {
+            float* dst = cents + c * L2_D;
+            if (counts[c] == 0) {
+                std::memcpy(dst, sample + rng.below(n) * L2_D, L2_D * sizeof(float));
+            } else {
+                for (int d = 0; d < L2_D; d++) dst[d] = (float)(sums[c * L2_D + d] / (double)counts[c]);
+            }
+        }
+    }
+}
+
+// train, src/bigann.rs:175-227, without the file writes: the Rng(42) sample of base rows
+// (indices sorted), the coarse rows at an even stride through it, k1 coarse cells by
+// kmeans_l2 (seed 7), every sample row to its nearest cell, and k2 leaves per cell by
+// kmeans_l2 over the cell's rows (seed = the cell; an empty cell's leaves are its centroid).
+void or_l2_train(const int8_t* base, size_t n, size_t k1, size_t k2, size_t sample_n, size_t iters, float* coarse,
+                 float* leaves, int threads) {
+    Rng rng(42);
+    std::vector<size_t> idx(sample_n);
+    for (auto& i : idx) i = rng.below(n);
+    std::sort(idx.begin(), idx.end());
+    std::vector<float> sample(sample_n * L2_D);
+    for (size_t r = 0; r < sample_n; r++)
+        for (int d = 0; d < L2_D; d++) sample[r * L2_D + d] = (float)base[idx[r] * L2_D + d];
+    const size_t coarse_n = std::min(sample_n, k1 * 512);
+    const size_t stride = sample_n / coarse_n;
+    std::vector<float> crow(coarse_n * L2_D);
+    for (size_t r = 0; r < coarse_n; r++)
+        std::memcpy(crow.data() + r * L2_D, sample.data() + r * stride * L2_D, L2_D * sizeof(float));
+    or_kmeans_l2(crow.data(), coarse_n, k1, iters, 7, coarse, threads);
+    std::vector<float> cn(k1), sc(sample_n);
+    std::vector<uint32_t> a(sample_n);
+    or_l2_norms(coarse, k1, cn.data());
+    or_top_l2(sample.data(), sample_n, coarse, cn.data(), k1, 1, a.data(), sc.data(), threads);
+    std::vector<std::vector<uint32_t>> members(k1);
+    for (size_t i = 0; i < sample_n; i++) members[a[i]].push_back((uint32_t)i);
+    for (size_t c = 0; c < k1; c++) {
+        std::vector<float> rows;
+        if (members[c].empty()) {
+            rows.assign(coarse + c * L2_D, coarse + (c + 1) * L2_D);
+        } else {
+            rows.resize(members[c].size() * L2_D);
+            for (size_t j = 0; j < members[c].size(); j++)
+                std::memcpy(rows.data() + j * L2_D, sample.data() + (size_t)members[c][j] * L2_D, L2_D * sizeof(float));
+        }
+        or_kmeans_l2(rows.data(), rows.size() / L2_D, k2, iters, (uint64_t)c, leaves + c * k2 * L2_D, threads);
+    }
+}
+
+// Model::assign, src/bigann.rs:277-302: each row's leaf is the best child of its 4 nearest
+// coarse cells, the cells taken in ascending order and a later one winning only if strictly
+// better.
+void or_l2_assign(const float* rows, size_t n, const float* coarse, size_t k1, const float* leaves, size_t k2,
+                  uint32_t* out, int threads) {
+    const size_t TOP = 4;
+    std::vector<float> cn(k1), ln(k1 * k2), ts(n * TOP);
+    std::vector<uint32_t> top(n * TOP);
+    or_l2_norms(coarse, k1, cn.data());
+    or_l2_norms(leaves, k1 * k2, ln.data());
+    or_top_l2(rows, n, coarse, cn.data(), k1, TOP, top.data(), ts.data(), threads);
+    for (size_t i = 0; i < n; i++) {
+        uint32_t cells[TOP];
+        std::copy(top.begin() + i * TOP, top.begin() + (i + 1) * TOP, cells);
+        std::sort(cells, cells + TOP);
+        uint32_t best = 0;
+        float bs = std::numeric_limits<float>::lowest();
+        for (size_t t = 0; t < TOP; t++) {
+            const uint32_t c = cells[t];
+            uint32_t j;
+            float s;
+            or_top_l2(rows + i * L2_D, 1, leaves + (size_t)c * k2 * L2_D, ln.data() + (size_t)c * k2, k2, 1, &j, &s, 1);
+            if (s > bs) {
+                bs = s;
+                best = (uint32_t)(c * k2 + j);
+            }
+        }
+        out[i] = best;
+    }
+}
+
 // ---------------------------------------------------------------- refinement ----------
 constexpr int RF_PD = 128, RF_CODE = RF_PD / 8, RF_MAX_BITS = 3;
 

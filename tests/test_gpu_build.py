@@ -115,3 +115,44 @@ def test_build_entry_points_from_host_arrays(built):
     sig = np.empty(256, np.float32)
     abi.check(L.sieve_build_tier2_fit(p(X), 5000, p(sig)))
     assert np.array_equal(sig.view(np.uint32), O.tier2_fit(X).view(np.uint32))
+
+
+# ----------------------------------------------------------------- the 1B build ----------
+@pytest.fixture(scope="module")
+def l2built():
+    import torch
+    from sieve_amd.l2build import L2Mixture, build_l2_large
+    n = 400_000
+    mix = L2Mixture(600, seed=21, topics=8)
+    base = torch.empty(n, 100, dtype=torch.int8, device="cuda")
+    mix.fill(base)
+    say(f"building the L2 index: {n} x 100 int8, 16 x 32 leaves")
+    arr = build_l2_large(base, coarse=16, fine=32, iters=4, sample=100_000, chunk=1 << 16)
+    torch.cuda.synchronize()
+    return dict(arr=arr, base=base.cpu().numpy(), n=n)
+
+
+def test_l2_training_is_the_oracles(l2built):
+    """bigann::train on the GPU: the coarse cells and every cell's leaves bit for bit."""
+    arr = l2built["arr"]
+    coarse, leaves = O.l2_train(l2built["base"], 16, 32, 100_000, 4, threads=16)
+    bad_c = int(np.sum(np.any(arr.coarse.cpu().numpy().view(np.uint32) != coarse.view(np.uint32), axis=1)))
+    bad_l = int(np.sum(np.any(arr.leaves.cpu().numpy().view(np.uint32) != leaves.view(np.uint32), axis=1)))
+    say(f"l2 train: {bad_c} of 16 coarse cells and {bad_l} of 512 leaves differ from the oracle's")
+    assert bad_c == 0 and bad_l == 0
+
+
+def test_l2_assignment_is_the_oracles(l2built):
+    """Model::assign on the GPU: every row's leaf bit for bit (through the built layout)."""
+    arr, n = l2built["arr"], l2built["n"]
+    ids = arr.ids.cpu().numpy().view(np.uint32)
+    bo = arr.block_off.cpu().numpy()
+    leaf_of = np.full(n, -1, np.int64)
+    for c in range(len(bo) - 1):
+        seg = ids[bo[c] * 32:bo[c + 1] * 32]
+        leaf_of[seg[seg != PAD]] = c
+    want = O.l2_assign(l2built["base"].astype(np.float32), arr.coarse.cpu().numpy(), arr.leaves.cpu().numpy(), 32,
+                       threads=16)
+    bad = int(np.sum(want != leaf_of))
+    say(f"l2 assign: {bad} of {n} rows in another leaf than the oracle's")
+    assert bad == 0

@@ -106,3 +106,101 @@ def test_build_pairs_match_assign_top_and_soar():
             assert pairs[i, 1] == O.soar_secondary(rows[i], cents, cand, 1.0)
         nosoar = O.build_pairs(rows, cents, None)
         assert np.array_equal(nosoar[:, 0], pairs[:, 0]) and np.all(nosoar[:, 1] == 0xFFFFFFFF)
+
+
+# ----------------------------------------------------------------- the 1B build ----------
+def _scores_l2(x, c):
+    acc = np.full((x.shape[0], c.shape[0]), -0.0, np.float32)
+    for d in range(x.shape[1]):
+        acc = acc + x[:, d:d + 1] * c[None, :, d]
+    cn = np.full(c.shape[0], -0.0, np.float32)
+    for d in range(c.shape[1]):
+        cn = cn + c[:, d] * c[:, d]
+    return np.float32(2.0) * acc - cn[None, :]
+
+
+def _kmeans_l2(sample, k, iters, seed):
+    n = sample.shape[0]
+    rng = _Rng(seed)
+    cents = np.stack([sample[rng.below(n)] for _ in range(k)]).astype(np.float32)
+    for _ in range(iters):
+        a = np.argmax(_scores_l2(sample, cents), axis=1)
+        sums = np.zeros((k, sample.shape[1]), np.float64)
+        counts = np.zeros(k, np.int64)
+        for i in range(n):
+            counts[a[i]] += 1
+            sums[a[i]] = sums[a[i]] + sample[i].astype(np.float64)
+        for c in range(k):
+            cents[c] = sample[rng.below(n)] if counts[c] == 0 else (sums[c] / np.float64(counts[c])).astype(np.float32)
+    return cents
+
+
+def _int8_rows(rng, n, centers=6):
+    c = rng.integers(-80, 80, (centers, 100))
+    return np.clip(c[rng.integers(0, centers, n)] + rng.integers(-9, 10, (n, 100)), -127, 127).astype(np.int8)
+
+
+def test_kmeans_l2_matches_numpy_restatement():
+    rng = np.random.default_rng(8)
+    sample = _int8_rows(rng, 500).astype(np.float32)
+    want = _kmeans_l2(sample, 9, 4, 3)
+    got = O.kmeans_l2(sample, 9, 4, 3)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    # duplicate rows: equal centroids tie (the lower id wins), their twins go empty
+    dup = _int8_rows(rng, 7).astype(np.float32)[rng.integers(0, 7, 200)]
+    assert np.array_equal(O.kmeans_l2(dup, 12, 3, 5).view(np.uint32), _kmeans_l2(dup, 12, 3, 5).view(np.uint32))
+
+
+def test_kmeans_finds_separated_clusters():
+    # bigann.rs:818-829 with its seed: rows alternate between centres +30 and -30.
+    rng = _Rng(3)
+    rows = np.array([[(30.0 if i % 2 == 0 else -30.0) + (rng.below(5) - 2.0) for _ in range(100)] for i in range(400)],
+                    np.float32)
+    cents = O.kmeans_l2(rows, 2, 5, 1)
+    a, _ = O.top_l2(rows, cents, 1)
+    assert np.all(a[0::2, 0] != a[1::2, 0])
+
+
+def test_top_l2_matches_numpy():
+    rng = np.random.default_rng(9)
+    x = _int8_rows(rng, 60).astype(np.float32)
+    c = rng.standard_normal((40, 100)).astype(np.float32) * 30
+    ids, sc = O.top_l2(x, c, 4)
+    s = _scores_l2(x, c)
+    for i in range(len(x)):
+        order = sorted(range(40), key=lambda j: (-s[i, j], j))[:4]
+        assert ids[i].tolist() == order
+        assert np.array_equal(sc[i].view(np.uint32), s[i, order].view(np.uint32))
+
+
+def test_l2_train_and_assign_restate_the_reference():
+    """train's sample, coarse cells and per-cell leaves, and Model::assign, against the numpy
+    restatement on a small base."""
+    rng = np.random.default_rng(10)
+    base = _int8_rows(rng, 3000, centers=12)
+    k1, k2, sample_n, iters = 4, 6, 900, 3
+    coarse, leaves = O.l2_train(base, k1, k2, sample_n, iters)
+    r = _Rng(42)
+    idx = sorted(r.below(len(base)) for _ in range(sample_n))
+    sample = base[idx].astype(np.float32)
+    coarse_n = min(sample_n, k1 * 512)
+    crow = sample[::sample_n // coarse_n][:coarse_n]
+    want_coarse = _kmeans_l2(crow, k1, iters, 7)
+    assert np.array_equal(coarse.view(np.uint32), want_coarse.view(np.uint32))
+    a = np.argmax(_scores_l2(sample, want_coarse), axis=1)
+    for c in range(k1):
+        rows = sample[a == c]
+        want = _kmeans_l2(rows, k2, iters, c) if len(rows) else np.repeat(want_coarse[c:c + 1], k2, 0)
+        assert np.array_equal(leaves[c * k2:(c + 1) * k2].view(np.uint32), want.view(np.uint32))
+    x = base[:400].astype(np.float32)
+    got = O.l2_assign(x, coarse, leaves, k2)
+    sc = _scores_l2(x, coarse)
+    sl = _scores_l2(x, leaves)
+    for i in range(len(x)):
+        cells = sorted(sorted(range(k1), key=lambda j: (-sc[i, j], j))[:4])
+        best, bs = 0, -np.inf
+        for c in cells:
+            j = max(range(k2), key=lambda j: (sl[i, c * k2 + j], -j))
+            if sl[i, c * k2 + j] > bs:
+                best, bs = c * k2 + j, sl[i, c * k2 + j]
+        assert got[i] == best
