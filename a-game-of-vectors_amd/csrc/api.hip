@@ -708,11 +708,16 @@ void l2_route(const sieve_l2_index* ix, sieve_stream* s, const int8_t* dq, size_
     const size_t per = std::max<size_t>(1, std::min<size_t>(nq, (size_t(1) << 30) / (4 * (size_t)ix->k)));
     float* approx = grow<float>(s->b[B_APPROX], per * ix->k);
     float* tmax = grow<float>(s->b[B_TMAX], per * ((ix->k + 15) / 16));
+    // Over tile maxima the select recomputes the scores it reads: the GEMM writes no scores.
+    const bool fused = route_select_over_tiles((int)ix->k, (int)nprobe);
     for (size_t q0 = 0; q0 < nq; q0 += per) {
         size_t m = std::min(per, nq - q0);
-        launch_gemm_bf16_route(prep.qbf + q0 * PDIM, ix->leaves_bf, ix->leaf_norm, approx, tmax, (int)m,
-                               (int)ix->k, ix->k, s->st);
+        launch_gemm_bf16_route(prep.qbf + q0 * PDIM, ix->leaves_bf, ix->leaf_norm, fused ? nullptr : approx, tmax,
+                               (int)m, (int)ix->k, ix->k, s->st);
         RouteArgs ra{};
+        ra.fused = fused;
+        ra.qbf = prep.qbf + q0 * PDIM;
+        ra.cbf = ix->leaves_bf;
         ra.approx = approx;
         ra.lds = ix->k;
         ra.tmax = tmax;

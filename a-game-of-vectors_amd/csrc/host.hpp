@@ -376,11 +376,16 @@ inline void route_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt,
     float* approx = sgrow<float>(s, B_APPROX, per * ix->k);
     float* tmax = sgrow<float>(s, B_TMAX, per * ((ix->k + 15) / 16));
     if (s->sizing) return;
+    // Over tile maxima the select recomputes the scores it reads: the GEMM writes no scores.
+    const bool fused = route_select_over_tiles((int)ix->k, (int)p->nprobe);
     for (size_t q0 = 0; q0 < nq; q0 += per) {
         size_t m = std::min(per, nq - q0);
-        launch_gemm_bf16_route(bt.prep.qbf + q0 * PDIM, ix->cents_bf, nullptr, approx, tmax, (int)m, (int)ix->k, ix->k,
-                               s->st);
+        launch_gemm_bf16_route(bt.prep.qbf + q0 * PDIM, ix->cents_bf, nullptr, fused ? nullptr : approx, tmax, (int)m,
+                               (int)ix->k, ix->k, s->st);
         RouteArgs ra{};
+        ra.fused = fused;
+        ra.qbf = bt.prep.qbf + q0 * PDIM;
+        ra.cbf = ix->cents_bf;
         ra.approx = approx;
         ra.lds = ix->k;
         ra.tmax = tmax;

@@ -86,8 +86,16 @@ struct RouteArgs {
     const float* qf;           // L2: nq x 100 f32 queries (rescoring operand)
     const float* qq;           // L2: sum q_i^2 per query
     float err_coef;            // |approx - exact| <= err_coef * qnorm
+    // fused: the GEMM wrote only the tile maxima; the select recomputes the approximate scores
+    // it reads from the GEMM's BF16 operands (qbf: nq x 256 of this launch, cbf: k x 256).
+    int fused;
+    const uint16_t* qbf;
+    const uint16_t* cbf;
 };
 void launch_route_select(const RouteArgs& a, size_t nq, hipStream_t st);
+// Whether the select over k centroids at nprobe runs over tile maxima (then the GEMM may skip
+// writing the scores: RouteArgs.fused).
+bool route_select_over_tiles(int k, int nprobe);
 // (primary, secondary) clusters of n unit rows from their best-first candidate lists (pc, pn,
 // row stride pstride): the first candidate and soar_secondary over the first 16 (src/ivf.rs:164-186),
 // or PAD as the secondary without SOAR.

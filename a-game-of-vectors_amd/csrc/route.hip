@@ -34,21 +34,29 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // ------------------------------------------------------------------ BF16 routing GEMM ---
 // S = Q C^T for every (query, centroid) with v_mfma_f32_16x16x32_bf16, K = 256. Each wave
 // owns 64 centroids, whose BF16 rows stay in registers (4 MFMA tiles x 8 k-steps, 128 VGPRs),
-// and walks the 16-query tiles of its workgroup's query range, loading the next tile's
-// operand while the current tile's 32 MFMAs run: centroid rows are read once per workgroup
-// and the kernel is bound by writing S. The centroids are the MFMA's first operand, so lane
+// and walks the 16-query tiles of its workgroup's query range (staged in LDS, below): centroid
+// rows are read once per workgroup, query tiles once per workgroup and tile. The centroids are the MFMA's first operand, so lane
 // l holds centroids 4 (l / 16) + v, v < 4, of tile j for query l % 16 (one float4 store).
 // tmax (row stride ceil(n / 16)) receives the largest approximate route score of each 16
-// consecutive centroids: S itself, or the L2 route score 2 S - |c|^2 (L2).
+// consecutive centroids: S itself, or the L2 route score 2 S - |c|^2 (L2). With S null only the
+// maxima are written (the select recomputes the scores it needs): the GEMM is then no longer
+// bound by writing 4 bytes per (query, centroid).
 template <bool L2>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_bf16_route(
     const uint16_t* __restrict__ Q, const uint16_t* __restrict__ C, const float* __restrict__ cnorm,
     float* __restrict__ S, float* __restrict__ tmax, int m, int n, size_t lds, int qtiles_per) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // The workgroup's 16-query tiles pass through LDS, two buffers taking turns: each tile is
+    // read from memory once per workgroup (each thread two 16-byte chunks, two tiles ahead)
+    // and each wave reads its operand from LDS. A row's 32 chunks are stored XOR-swizzled by
+    // the row (chunk c of row r at r * 32 + (c ^ r)): the 16 rows a fragment read touches at
+    // one chunk index fall in 16 distinct bank groups.
+    __shared__ uint4 qtile[2][16 * 32];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int cb = blockIdx.x * 256 + wave * 64;
     const int qtiles = (m + 15) / 16;
     const int qt0 = blockIdx.y * qtiles_per, qt1 = min(qt0 + qtiles_per, qtiles);
-    if (cb >= n || qt0 >= qt1) return;  // wave-uniform; the kernel has no block barrier
+    if (qt0 >= qt1) return;  // workgroup-uniform
+    const bool active = cb < n;  // wave-uniform: a wave past the last centroid only stages tiles
     const int r16 = lane & 15, ko = (lane >> 4) * 8, c4 = (lane >> 4) * 4;
     const int ntile = (n + 15) / 16;
     bf16x8 bc[4][8];
@@ -67,10 +75,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             cn[j][v] = (L2 && c < n) ? cnorm[c] : 0.0f;
         }
     const bool vec = (lds & 3) == 0;
-    auto load_q = [&](bf16x8(&dst)[8], int qt) {
-        const uint16_t* row = Q + (size_t)min(qt * 16 + r16, m - 1) * PDIM + ko;
+    // thread t stages chunk t & 31 of tile rows t >> 5 and (t >> 5) + 8 (rows past m repeat
+    // row m - 1; tiles past the range repeat the last one)
+    const int sr = t >> 5, sc = t & 31;
+    auto gload = [&](int qt, uint4& x, uint4& y) {
+        const uint4* base = reinterpret_cast<const uint4*>(Q);
+        const int q = min(qt, qt1 - 1) * 16;
+        x = base[(size_t)min(q + sr, m - 1) * 32 + sc];
+        y = base[(size_t)min(q + sr + 8, m - 1) * 32 + sc];
+    };
+    auto lstore = [&](int b, const uint4& x, const uint4& y) {
+        qtile[b][sr * 32 + (sc ^ sr)] = x;
+        qtile[b][(sr + 8) * 32 + (sc ^ (sr + 8))] = y;
+    };
+    auto lload = [&](bf16x8(&dst)[8], int b) {
 #pragma unroll
-        for (int s = 0; s < 8; s++) dst[s] = *reinterpret_cast<const bf16x8*>(row + 32 * s);
+        for (int s = 0; s < 8; s++) {
+            const uint4 v = qtile[b][r16 * 32 + (((ko >> 3) + 4 * s) ^ r16)];
+            dst[s] = __builtin_bit_cast(bf16x8, v);
+        }
     };
     // One 16-query tile: 32 MFMAs, then S and the tile maxima. FULL: every query and centroid
     // of the tile exists and rows are float4-aligned, so every store is unconditional (all
@@ -91,11 +114,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             float* dst = S + (size_t)q * lds + c0;
             float mx = -INFINITY;
             if constexpr (FULL) {
-                *reinterpret_cast<f32x4*>(dst) = acc[j];
+                if (S) *reinterpret_cast<f32x4*>(dst) = acc[j];
 #pragma unroll
                 for (int v = 0; v < 4; v++) mx = fmaxf(mx, L2 ? 2.0f * acc[j][v] - cn[j][v] : acc[j][v]);
             } else {
-                if (q < m) {
+                if (S && q < m) {
                     if (vec && c0 + 3 < n) {
                         *reinterpret_cast<f32x4*>(dst) = acc[j];
                     } else {
@@ -114,26 +137,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             if (FULL || (lane < 16 && q < m && tl < ntile)) tmax[(size_t)q * ntile + tl] = mx;
         }
     };
-    // Full tiles in pairs, the two operand buffers taking turns (each tile's loads are issued
-    // a whole tile ahead); then the rest one by one.
-    const int qfull = (vec && cb + 64 <= n) ? min(qt1, m / 16) : qt0;
-    bf16x8 aq[8], an[8];
-    load_q(aq, qt0);
-    int qt = qt0;
-    for (; qt + 1 < qfull; qt += 2) {
-        load_q(an, qt + 1);
-        __builtin_amdgcn_sched_barrier(0);
-        tile(std::true_type{}, aq, qt);
-        load_q(aq, min(qt + 2, qt1 - 1));
-        __builtin_amdgcn_sched_barrier(0);
-        tile(std::true_type{}, an, qt + 1);
-    }
-    for (; qt < qt1; qt++) {
-        load_q(an, min(qt + 1, qt1 - 1));
-        __builtin_amdgcn_sched_barrier(0);
-        tile(std::false_type{}, aq, qt);
-#pragma unroll
-        for (int s = 0; s < 8; s++) aq[s] = an[s];
+    const int qfull = (vec && cb + 64 <= n) ? min(qt1, m / 16) : qt0;  // tiles before qfull are FULL
+    uint4 xa, ya, xb, yb;
+    gload(qt0, xa, ya);
+    lstore(0, xa, ya);
+    gload(qt0 + 1, xa, ya);
+    __syncthreads();
+    int b = 0;
+    for (int qt = qt0; qt < qt1; qt++, b ^= 1) {
+        gload(qt + 2, xb, yb);
+        if (active) {
+            bf16x8 cur[8];
+            lload(cur, b);
+            if (qt < qfull)
+                tile(std::true_type{}, cur, qt);
+            else
+                tile(std::false_type{}, cur, qt);
+        }
+        // buffer b ^ 1 was last read in the previous tile, before the barrier that ended it
+        lstore(b ^ 1, xa, ya);
+        xa = xb;
+        ya = yb;
+        __syncthreads();
     }
 }
 
@@ -347,6 +372,7 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
     __shared__ uint32_t ncand, ntl;
     __shared__ uint32_t cids[MAX_RESCORE];
     __shared__ uint2 sorted[MAX_RESCORE];
+    __shared__ __align__(16) float qb[PDIM];
     const size_t qi = blockIdx.x;
     const int t = threadIdx.x;
     const float* S = a.approx + qi * a.lds;
@@ -356,6 +382,8 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
     } else {
         for (int i = t; i < PDIM; i += 256) qs[i] = a.q256[qi * PDIM + i];
     }
+    if (a.fused)
+        for (int i = t; i < PDIM; i += 256) qb[i] = __uint_as_float((uint32_t)a.qbf[qi * PDIM + i] << 16);
     if (t == 0) {
         ncand = 0;
         ntl = 0;
@@ -365,8 +393,32 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
         if (t == 0) a.out_n[qi] = 0;
         return;
     }
-    // approximate route score of centroid i
-    auto approx = [&](int i) -> float { return a.l2 ? 2.0f * S[i] - a.cnorm[i] : S[i]; };
+    // approximate route score of centroid i: the GEMM's, or (fused) recomputed from its BF16
+    // operands. Products of BF16 values are exact in f32, so the recomputed sum, in any order,
+    // is within the same error bound of the exact score as the GEMM's.
+    auto approx = [&](int i) -> float {
+        float s;
+        if (!a.fused) {
+            s = S[i];
+        } else {
+            const uint4* row = reinterpret_cast<const uint4*>(a.cbf + (size_t)i * PDIM);
+            const float4* qb4 = reinterpret_cast<const float4*>(qb);
+            s = 0.0f;
+            for (int j = 0; j < PDIM / 8; j++) {
+                const uint4 w = row[j];
+                const float4 x0 = qb4[2 * j], x1 = qb4[2 * j + 1];
+                s = s + x0.x * __uint_as_float(w.x << 16);
+                s = s + x0.y * __uint_as_float(w.x & 0xFFFF0000u);
+                s = s + x0.z * __uint_as_float(w.y << 16);
+                s = s + x0.w * __uint_as_float(w.y & 0xFFFF0000u);
+                s = s + x1.x * __uint_as_float(w.z << 16);
+                s = s + x1.y * __uint_as_float(w.z & 0xFFFF0000u);
+                s = s + x1.z * __uint_as_float(w.w << 16);
+                s = s + x1.w * __uint_as_float(w.w & 0xFFFF0000u);
+            }
+        }
+        return a.l2 ? 2.0f * s - a.cnorm[i] : s;
+    };
     auto key = [&](int i) { return f2key(approx(i) + 0.0f); };
     const float e = a.err_coef * a.qnorm[qi];
     // The band of the n-th largest approximate score kv: every centroid whose exact score can
@@ -442,7 +494,41 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
         }
         __syncthreads();
         const uint32_t nl = ntl;
-        if (nl <= (uint32_t)MAX_RESCORE) {
+        if (nl <= (uint32_t)MAX_RESCORE && a.fused) {
+            // The collected tiles' scores from the BF16 operands, 8 lanes per centroid: lane l8
+            // takes the 16-byte pieces l8, l8 + 8, l8 + 16, l8 + 24 of the 512-byte row, so each
+            // load of a lane group reads one whole 128-byte line.
+            const int g = t >> 3, l8 = t & 7;
+            const float4* qb4 = reinterpret_cast<const float4*>(qb);
+            for (uint32_t f = g; f < nl * 16; f += 32) {
+                const int c = (int)cids[f >> 4] * 16 + (int)(f & 15);
+                if (c >= a.k) continue;  // the whole lane group
+                const uint4* row = reinterpret_cast<const uint4*>(a.cbf + (size_t)c * PDIM);
+                float s = 0.0f;
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    const int j = l8 + 8 * m;
+                    const uint4 w = row[j];
+                    const float4 x0 = qb4[2 * j], x1 = qb4[2 * j + 1];
+                    s = s + x0.x * __uint_as_float(w.x << 16);
+                    s = s + x0.y * __uint_as_float(w.x & 0xFFFF0000u);
+                    s = s + x0.z * __uint_as_float(w.y << 16);
+                    s = s + x0.w * __uint_as_float(w.y & 0xFFFF0000u);
+                    s = s + x1.x * __uint_as_float(w.z << 16);
+                    s = s + x1.y * __uint_as_float(w.z & 0xFFFF0000u);
+                    s = s + x1.z * __uint_as_float(w.w << 16);
+                    s = s + x1.w * __uint_as_float(w.w & 0xFFFF0000u);
+                }
+                s += __shfl_xor(s, 1, 64);
+                s += __shfl_xor(s, 2, 64);
+                s += __shfl_xor(s, 4, 64);
+                const float v = a.l2 ? 2.0f * s - a.cnorm[c] : s;
+                if (l8 == 0 && v >= lim) {
+                    const uint32_t j = atomicAdd(&ncand, 1u);
+                    if (j < (uint32_t)MAX_RESCORE) sorted[j] = make_uint2(f2key(v + 0.0f), ~(uint32_t)c);
+                }
+            }
+        } else if (nl <= (uint32_t)MAX_RESCORE) {
             for (uint32_t f = t; f < nl * 16; f += 256) {
                 const int c = (int)cids[f >> 4] * 16 + (int)(f & 15);
                 if (c < a.k) {
@@ -534,19 +620,21 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
             Sw[i] = acc;
         }
         __syncthreads();
-        const uint32_t kth = block_kth_u32(a.k, n, key, hist, sh);
+        // keys of the exact route scores in the row
+        auto skey = [&](int i) { return f2key((a.l2 ? 2.0f * Sw[i] - a.cnorm[i] : Sw[i]) + 0.0f); };
+        const uint32_t kth = block_kth_u32(a.k, n, skey, hist, sh);
         if (t == 0) {
             ncand = 0;
             ntl = 0;
         }
         __syncthreads();
         for (int i = t; i < a.k; i += 256)
-            if (key(i) > kth) atomicAdd(&ntl, 1u);
+            if (skey(i) > kth) atomicAdd(&ntl, 1u);
         __syncthreads();
         const int need = n - (int)ntl;  // >= 1: at least n keys are >= kth
-        const uint32_t idk = block_kth_u32(a.k, need, [&](int i) { return key(i) == kth ? ~(uint32_t)i : 0u; }, hist, sh);
+        const uint32_t idk = block_kth_u32(a.k, need, [&](int i) { return skey(i) == kth ? ~(uint32_t)i : 0u; }, hist, sh);
         for (int i = t; i < a.k; i += 256) {
-            const uint32_t x = key(i);
+            const uint32_t x = skey(i);
             if (x > kth || (x == kth && ~(uint32_t)i >= idk)) cids[atomicAdd(&ncand, 1u)] = (uint32_t)i;
         }
         __syncthreads();
@@ -635,11 +723,16 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
     }
 }
 
+bool route_select_over_tiles(int k, int nprobe) {
+    const int nt = (k + 15) / 16;
+    return nt >= std::min(nprobe, k) && nt <= 256 * 64;
+}
+
 void launch_route_select(const RouteArgs& a, size_t nq, hipStream_t st) {
     if (nq == 0) return;
     const int nt = (a.k + 15) / 16;
     const dim3 g((unsigned)nq), b(256);
-    if (a.tmax && nt >= std::min(a.nprobe, a.k) && nt <= 256 * 64) {
+    if (a.tmax && route_select_over_tiles(a.k, a.nprobe)) {
         if (nt <= 256 * 4)
             hipLaunchKernelGGL((k_route_select<ROUTE_TILES, 4>), g, b, 0, st, a);
         else if (nt <= 256 * 16)
