@@ -357,6 +357,71 @@ __device__ void block_sort_desc(uint2* a, int n2) {
     }
 }
 
+// Bitonic sort of one key per lane across a wave, descending: lane 0 gets the largest.
+__device__ __forceinline__ uint32_t wave_sort_desc(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t o = __shfl_xor(v, j, 64);
+            // the lower lane of a pair keeps the larger key in the blocks that end descending
+            const bool keep_max = ((lane & j) == 0) == ((lane & k) == 0);
+            v = keep_max ? max(v, o) : min(v, o);
+        }
+    return v;
+}
+
+// block_kth_regs for k <= 64 without a radix pass over every key. Each wave sorts its lanes'
+// largest keys; a wave's k-th largest lane key L_w has k keys >= it, so the largest L_w, L,
+// is at most the k-th largest key. The keys >= L (about k to 4k of them) are gathered into
+// buf and their k-th largest is the answer: one more wave sort when they fit a wave, else
+// the radix over buf; more than `cap` of them (equal keys), the radix over key(i) (the same
+// keys, read again). k <= 64, 1 <= k <= n. ws: 6 words of LDS. Ends with a barrier.
+template <int VALS, class KeyFn>
+__device__ uint32_t block_kth_regs_small(const uint32_t (&x)[VALS], int n, int k, KeyFn key, uint2* buf, int cap,
+                                         uint32_t* ws, uint32_t* hist, uint32_t* sh) {
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    uint32_t mk = 0;
+#pragma unroll
+    for (int j = 0; j < VALS; j++)
+        if (t + 256 * j < n) mk = max(mk, x[j]);
+    mk = wave_sort_desc(mk);
+    const uint32_t lw = __shfl(mk, k - 1, 64);
+    if (lane == 0) ws[1 + wave] = lw;
+    if (t == 0) ws[0] = 0;
+    __syncthreads();
+    const uint32_t L = max(max(ws[1], ws[2]), max(ws[3], ws[4]));
+#pragma unroll
+    for (int j = 0; j < VALS; j++) {
+        const bool in = t + 256 * j < n && x[j] >= L;
+        const uint64_t b = __ballot(in);
+        if (b) {  // wave-uniform
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&ws[0], (uint32_t)__popcll(b));
+            base = __shfl(base, 0, 64) + (uint32_t)__popcll(b & ((1ull << lane) - 1));
+            if (in && base < (uint32_t)cap) buf[base] = make_uint2(x[j], ~(uint32_t)(t + 256 * j));
+        }
+    }
+    __syncthreads();
+    const uint32_t got = ws[0];  // >= k unless keys are 0 (a NaN's)
+    uint32_t res;
+    if (got > (uint32_t)cap || got < (uint32_t)k) {
+        res = block_kth_u32(n, k, key, hist, sh);
+    } else if (got <= 64) {
+        if (wave == 0) {
+            const uint32_t v = wave_sort_desc(lane < (int)got ? buf[lane].x : 0u);
+            if (lane == k - 1) ws[5] = v;
+        }
+        __syncthreads();
+        res = ws[5];
+    } else {
+        res = block_kth_largest(buf, (int)got, k, hist, sh).x;
+    }
+    __syncthreads();
+    return res;
+}
+
 // Route select modes. ROUTE_REGS: the approximate scores (k <= 256 VALS) are read once into
 // registers and the select and the band pass run over them. ROUTE_MEM: over memory (any k).
 // ROUTE_TILES: over the GEMM's maxima of 16 consecutive scores (ceil(k / 16) <= 256 VALS,
@@ -370,6 +435,7 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
     __shared__ uint32_t sh[4];
     __shared__ float red[4];
     __shared__ uint32_t ncand, ntl;
+    __shared__ uint32_t ws[6];
     __shared__ uint32_t cids[MAX_RESCORE];
     __shared__ uint2 sorted[MAX_RESCORE];
     __shared__ __align__(16) float qb[PDIM];
@@ -453,7 +519,8 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
             const int i = t + 256 * j;
             x[j] = i < a.k ? key(i) : 0u;
         }
-        const float thr = band_of(key2f(block_kth_regs<VALS>(x, a.k, n, hist, sh)));
+        const float thr = band_of(key2f(n <= 64 ? block_kth_regs_small<VALS>(x, a.k, n, key, sorted, MAX_RESCORE, ws, hist, sh)
+                                                : block_kth_regs<VALS>(x, a.k, n, hist, sh)));
 #pragma unroll
         for (int j = 0; j < VALS; j++) {
             const int i = t + 256 * j;
@@ -481,7 +548,10 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
         }
         mx = wave_max(mx);
         if ((t & 63) == 0) red[t >> 6] = mx;
-        const float t0 = key2f(block_kth_regs<VALS>(tk, nt, n, hist, sh));  // ends with a barrier
+        // ends with a barrier
+        auto tkey = [&](int i) { return f2key(T[i] + 0.0f); };
+        const float t0 = key2f(n <= 64 ? block_kth_regs_small<VALS>(tk, nt, n, tkey, sorted, MAX_RESCORE, ws, hist, sh)
+                                       : block_kth_regs<VALS>(tk, nt, n, hist, sh));
         const float top = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
         const float lim = t0 - 2.2f * e - 1e-6f * (1.0f + fmaxf(fabsf(t0), fabsf(top)));
 #pragma unroll
