@@ -609,6 +609,7 @@ static uint64_t search_graphed(const sieve_index* ix, sieve_stream* s, const flo
     put(ix->cm_chunk_blocks);
     put(ix->cm_phases);
     put(ix->small_max_queries);
+    put(ix->route_fuse);
     put(nq);
     put(*p);
     put(view.deleted);
@@ -708,8 +709,8 @@ void l2_route(const sieve_l2_index* ix, sieve_stream* s, const int8_t* dq, size_
     const size_t per = std::max<size_t>(1, std::min<size_t>(nq, (size_t(1) << 30) / (4 * (size_t)ix->k)));
     float* approx = grow<float>(s->b[B_APPROX], per * ix->k);
     float* tmax = grow<float>(s->b[B_TMAX], per * ((ix->k + 15) / 16));
-    // Over tile maxima the select recomputes the scores it reads: the GEMM writes no scores.
-    const bool fused = route_select_over_tiles((int)ix->k, (int)nprobe);
+    // Fused: the select recomputes the scores it reads and the GEMM writes none.
+    const bool fused = route_fuse(ix->route_fuse, (int)ix->k, (int)nprobe);
     for (size_t q0 = 0; q0 < nq; q0 += per) {
         size_t m = std::min(per, nq - q0);
         launch_gemm_bf16_route(prep.qbf + q0 * PDIM, ix->leaves_bf, ix->leaf_norm, fused ? nullptr : approx, tmax,
@@ -1032,6 +1033,20 @@ int sieve_index_set_cluster_scan(sieve_index* ix, uint32_t min_queries, uint32_t
         ix->cm_ext_cap = ext_cap;
         ix->cm_chunk_blocks = chunk_blocks;
         ix->cm_phases = phases ? phases : 2;
+    });
+}
+
+int sieve_index_set_route_fuse(sieve_index* ix, int mode) {
+    return guarded([&] {
+        SIEVE_REQUIRE(ix, SIEVE_EINVAL, "null index");
+        ix->route_fuse = mode < 0 ? -1 : mode > 0 ? 1 : 0;
+    });
+}
+
+int sieve_l2_index_set_route_fuse(sieve_l2_index* ix, int mode) {
+    return guarded([&] {
+        SIEVE_REQUIRE(ix, SIEVE_EINVAL, "null index");
+        ix->route_fuse = mode < 0 ? -1 : mode > 0 ? 1 : 0;
     });
 }
 

@@ -216,7 +216,7 @@ void exact_top(const Cents& C, const float* rows, size_t n, int m, uint32_t* pc,
         const float* x = rows + q0 * PDIM;
         launch_to_bf16(x, mm * PDIM, xbf, st);
         hipLaunchKernelGGL(k_row_norms, dim3((unsigned)mm), dim3(64), 0, st, x, mm, xn);
-        const bool fused = route_select_over_tiles((int)C.k, m);
+        const bool fused = route_fuse(-1, (int)C.k, m);
         launch_gemm_bf16_route(xbf, C.bf, nullptr, fused ? nullptr : approx, tmax, (int)mm, (int)C.k, C.k, st);
         RouteArgs ra{};
         ra.fused = fused;

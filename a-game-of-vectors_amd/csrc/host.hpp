@@ -101,6 +101,7 @@ using sieve::DevBuf;
 struct sieve_index {
     uint64_t uid = sieve::next_index_uid();  // never reused: keys of captured graphs
     int device = 0;
+    int route_fuse = -1;  // sieve_index_set_route_fuse
     uint32_t k = 0;
     uint64_t n = 0, nblocks = 0, nslots = 0;
     uint32_t max_cluster_blocks = 0;
@@ -151,6 +152,7 @@ struct sieve_index {
 struct sieve_l2_index {
     int device = 0;
     uint32_t small_max_queries = 8;  // batches of at most this many queries: the small-batch scan
+    int route_fuse = -1;             // sieve_l2_index_set_route_fuse
     uint32_t k = 0;
     uint64_t n = 0, nblocks = 0, nslots = 0;
     uint32_t max_cluster_blocks = 0;
@@ -376,8 +378,8 @@ inline void route_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt,
     float* approx = sgrow<float>(s, B_APPROX, per * ix->k);
     float* tmax = sgrow<float>(s, B_TMAX, per * ((ix->k + 15) / 16));
     if (s->sizing) return;
-    // Over tile maxima the select recomputes the scores it reads: the GEMM writes no scores.
-    const bool fused = route_select_over_tiles((int)ix->k, (int)p->nprobe);
+    // Fused: the select recomputes the scores it reads and the GEMM writes none.
+    const bool fused = route_fuse(ix->route_fuse, (int)ix->k, (int)p->nprobe);
     for (size_t q0 = 0; q0 < nq; q0 += per) {
         size_t m = std::min(per, nq - q0);
         launch_gemm_bf16_route(bt.prep.qbf + q0 * PDIM, ix->cents_bf, nullptr, fused ? nullptr : approx, tmax, (int)m,

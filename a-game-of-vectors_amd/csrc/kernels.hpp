@@ -93,9 +93,12 @@ struct RouteArgs {
     const uint16_t* cbf;
 };
 void launch_route_select(const RouteArgs& a, size_t nq, hipStream_t st);
-// Whether the select over k centroids at nprobe runs over tile maxima (then the GEMM may skip
-// writing the scores: RouteArgs.fused).
+// Whether the select over k centroids at nprobe runs over tile maxima.
 bool route_select_over_tiles(int k, int nprobe);
+// Whether the GEMM should skip writing the scores (RouteArgs.fused). mode < 0: when the select
+// runs over tile maxima and recomputing its tiles reads less than the score row it replaces;
+// 0: never; > 0: whenever the select runs over tile maxima.
+bool route_fuse(int mode, int k, int nprobe);
 // (primary, secondary) clusters of n unit rows from their best-first candidate lists (pc, pn,
 // row stride pstride): the first candidate and soar_secondary over the first 16 (src/ivf.rs:164-186),
 // or PAD as the secondary without SOAR.

@@ -179,6 +179,8 @@ def lib():
         "sieve_l2_index_set_scan_chunk": (i32, [vp, u32]),
         "sieve_l2_index_set_scan_variant": (i32, [vp, u32]),
         "sieve_l2_index_set_small_batch": (i32, [vp, u32]),
+        "sieve_index_set_route_fuse": (i32, [vp, i32]),
+        "sieve_l2_index_set_route_fuse": (i32, [vp, i32]),
         "sieve_server_create": (i32, [vp, P, ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)]),
         "sieve_server_submit": (i32, [vp, vp, vp, ctypes.POINTER(ctypes.c_uint64)]),
         "sieve_server_submit_filtered": (i32, [vp, vp, vp, vp, u32, ctypes.POINTER(ctypes.c_uint64)]),

@@ -204,6 +204,11 @@ class Index:
         (sieve_index_set_small_batch); 0 disables it."""
         abi.check(abi.lib().sieve_index_set_small_batch(self._h, max_queries))
 
+    def set_route_fuse(self, mode=-1):
+        """Routing GEMM fused with its select: -1 auto, 0 never, 1 whenever the select runs over
+        tile maxima (sieve_index_set_route_fuse)."""
+        abi.check(abi.lib().sieve_index_set_route_fuse(self._h, mode))
+
     def search(self, q768, q256, p: Params, view=None, timing=None, stream=None, out=None):
         """Index::search / search_view for a batch: (scores nq x k, rows, counts). `out`: the
         (scores, rows, counts) arrays to write, else new ones."""
@@ -445,6 +450,11 @@ class L2Index:
     def set_small_batch(self, max_queries=8):
         """Largest batch for the small-batch scan (sieve_l2_index_set_small_batch); 0 disables it."""
         abi.check(abi.lib().sieve_l2_index_set_small_batch(self._h, max_queries))
+
+    def set_route_fuse(self, mode=-1):
+        """Routing GEMM fused with its select: -1 auto, 0 never, 1 whenever the select runs over
+        tile maxima (sieve_l2_index_set_route_fuse)."""
+        abi.check(abi.lib().sieve_l2_index_set_route_fuse(self._h, mode))
 
     def search(self, q, nprobe, cand, k, margin=0.0, timing=None, stream=None, details=False, refine=0):
         """bigann::Index::search for a batch of int8 queries (nq x 100): (scores, rows, counts)

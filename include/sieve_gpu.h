@@ -253,6 +253,13 @@ int sieve_index_set_cluster_scan(sieve_index* ix, uint32_t min_queries, uint32_t
  * as above) and the list is merged into the query's candidates; an overflowing list is
  * rescanned. Results equal the per-query scan's. max_queries 0 disables it (default 12). */
 int sieve_index_set_small_batch(sieve_index* ix, uint32_t max_queries);
+/* Routing (no counterpart in the reference; probe lists are the same in every mode): whether
+ * the routing GEMM writes every (query, centroid) score or only the maxima of 16 consecutive
+ * scores, the select then recomputing from the BF16 operands the scores of the tiles it
+ * reads. mode < 0 (default): fused when the select runs over tile maxima and nprobe * 3072 <=
+ * k (the recomputed rows cost less than writing the scores); 0: never; > 0: whenever the select
+ * runs over tile maxima. */
+int sieve_index_set_route_fuse(sieve_index* ix, int mode);
 
 /* ------------------------------------------------------------- the path --- */
 
@@ -445,6 +452,8 @@ int sieve_l2_index_set_scan_variant(sieve_l2_index* ix, uint32_t variant);
  * of at most max_queries queries run the small-batch scan, larger ones the selected per-query
  * variant. Results equal the per-query scan's. 0 disables it (default 8). */
 int sieve_l2_index_set_small_batch(sieve_l2_index* ix, uint32_t max_queries);
+/* As sieve_index_set_route_fuse, for the L2 index. */
+int sieve_l2_index_set_route_fuse(sieve_l2_index* ix, int mode);
 
 /* bigann::Index::search (src/bigann.rs:652-657) for nq queries (nq x 100 i8): the top k
  * (score = |q|^2 - |q - x|^2 - |q|^2 = 2 <q, x> - |x|^2, row) best first. Optional outputs

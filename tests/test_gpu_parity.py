@@ -86,10 +86,13 @@ def test_lut_scan_narrow_fixture():
 
 # ---------------------------------------------------------------------- route -----------
 @pytest.mark.parametrize("cfg", [dict(nprobe=7), dict(nprobe=16), dict(nprobe=48), dict(nprobe=20, margin=0.05),
-                                 dict(nprobe=20, budget=3000), dict(nprobe=1)])
+                                 dict(nprobe=20, budget=3000), dict(nprobe=1), dict(nprobe=1, fuse=1)])
 def test_route_matches_oracle(small, oracle_index, gpu_index, cfg):
+    cfg = dict(cfg)
+    gpu_index.set_route_fuse(cfg.pop("fuse", -1))
     p = Params(cand=100, **cfg)
     s, c, n = gpu_index.route(small.q256, p)
+    gpu_index.set_route_fuse(-1)
     for i in range(small.nq):
         ws, wc = oracle_index.route(small.q256[i], p)
         gs, gc = s[i, :n[i]], c[i, :n[i]]
@@ -110,11 +113,15 @@ def wide():
     return w, O.Index(w.host), Index.create(w.host)
 
 
+@pytest.mark.parametrize("fuse", [-1, 1])
 @pytest.mark.parametrize("cfg", [dict(nprobe=64), dict(nprobe=256, margin=0.05), dict(nprobe=9)])
-def test_route_over_many_centroids_matches_oracle(wide, cfg):
+def test_route_over_many_centroids_matches_oracle(wide, cfg, fuse):
+    """fuse 1: the GEMM writes only tile maxima and the select recomputes its tiles' scores."""
     w, oix, gix = wide
     p = Params(cand=100, **cfg)
+    gix.set_route_fuse(fuse)
     s, c, n = gix.route(w.q256, p)
+    gix.set_route_fuse(-1)
     for i in range(w.nq):
         ws, wc = oix.route(w.q256[i], p)
         assert n[i] == len(wc)
@@ -138,10 +145,12 @@ def huge_k(small):
     return h, O.Index(h), Index.create(h)
 
 
+@pytest.mark.parametrize("fuse", [-1, 1, 0])
 @pytest.mark.parametrize("cfg", [dict(nprobe=32), dict(nprobe=200, margin=0.1), dict(nprobe=1)])
-def test_route_over_20k_centroids_matches_oracle(small, huge_k, cfg):
+def test_route_over_20k_centroids_matches_oracle(small, huge_k, cfg, fuse):
     h, oix, gix = huge_k
     p = Params(cand=100, **cfg)
+    gix.set_route_fuse(fuse)
     s, c, n = gix.route(small.q256, p)
     for i in range(small.nq):
         ws, wc = oix.route(small.q256[i], p)
@@ -151,6 +160,7 @@ def test_route_over_20k_centroids_matches_oracle(small, huge_k, cfg):
     p = Params(nprobe=24, cand=300, k=10, rerank=Rerank.FULL)
     gs, gr, gn = gix.search(small.q768, small.q256, p)
     ws, wr, wn, _ = oix.search_batch(small.q768, small.q256, p, threads=4)
+    gix.set_route_fuse(-1)
     for i in range(small.nq):
         assert_same_up_to_ties(gr[i, :gn[i]], gs[i, :gn[i]], wr[i, :wn[i]], ws[i, :wn[i]])
 
@@ -380,13 +390,16 @@ def test_errors_are_reported(small, gpu_index):
 
 
 # ---------------------------------------------------------------------- degenerate queries ---
-def test_zero_query_routes_exactly_and_leaves_the_batch_alone(small, oracle_index, gpu_index):
+@pytest.mark.parametrize("fuse", [-1, 1])
+def test_zero_query_routes_exactly_and_leaves_the_batch_alone(small, oracle_index, gpu_index, fuse):
     """A zero q256 ties every centroid score, so its BF16 error band holds every centroid of a
     wide index: that query alone takes the exact route (all scores in f32, then the top n by
-    score desc, id asc), and the other queries of its batch get their usual results."""
+    score desc, id asc), and the other queries of its batch get their usual results. fuse 1:
+    the same with the select recomputing scores from the BF16 operands."""
     from conftest import SmallWorkload
     w = SmallWorkload(k=6000, nq=12, tier2=False)
     gix = Index.create(w.host)
+    gix.set_route_fuse(fuse)
     oix = O.Index(w.host)
     q256 = w.q256.copy()
     q768 = w.q768.copy()

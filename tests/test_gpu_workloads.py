@@ -312,10 +312,12 @@ def c4shape():
 
 
 @pytest.mark.timeout(600)
-@pytest.mark.parametrize("nprobe,cand", [(8, 1000), (64, 2000)])
-def test_c4_routing_shape_matches_oracle(c4shape, nprobe, cand):
+@pytest.mark.parametrize("nprobe,cand,fuse", [(8, 1000, -1), (64, 2000, -1), (64, 2000, 1), (8, 1000, 0)])
+def test_c4_routing_shape_matches_oracle(c4shape, nprobe, cand, fuse):
+    """fuse -1 takes the fused route at 8 probes and the unfused one at 64; 1 and 0 the others."""
     gix, oix, qh = c4shape["gix"], c4shape["oix"], c4shape["qh"]
     assert gix.k == 131072
+    gix.set_route_fuse(fuse)
     s, r, n, probes, pn, cr, ce, cn = gix.search(c4shape["q"], nprobe, cand, 10, details=True)
     s, r, n = s.cpu().numpy(), r.cpu().numpy().view(np.uint32), n.cpu().numpy().view(np.uint32)
     probes, pn = probes.cpu().numpy().view(np.uint32), pn.cpu().numpy().view(np.uint32)
@@ -337,6 +339,7 @@ def test_c4_routing_shape_matches_oracle(c4shape, nprobe, cand):
     for i in (0, 77):
         s1, r1, n1 = gix.search(qh[i:i + 1], nprobe, cand, 10)
         assert np.array_equal(r1[0, :n1[0]], r[i, :n[i]]) and _flt_eq(s1[0, :n1[0]], s[i, :n[i]])
+    gix.set_route_fuse(-1)
 
 
 @pytest.mark.timeout(600)
