@@ -377,7 +377,9 @@ __device__ __forceinline__ uint32_t wave_sort_desc(uint32_t v) {
 // is at most the k-th largest key. The keys >= L (about k to 4k of them) are gathered into
 // buf and their k-th largest is the answer: one more wave sort when they fit a wave, else
 // the radix over buf; more than `cap` of them (equal keys), the radix over key(i) (the same
-// keys, read again). k <= 64, 1 <= k <= n. ws: 6 words of LDS. Ends with a barrier.
+// keys, read again). k <= 64, 1 <= k <= n. ws: 6 words of LDS. Ends with a barrier. The
+// callers take it for k <= 16: at larger k the bound is loose (at k = 64 it is the smallest lane
+// maximum of a wave), hundreds of keys pass it and the radix over registers is faster.
 template <int VALS, class KeyFn>
 __device__ uint32_t block_kth_regs_small(const uint32_t (&x)[VALS], int n, int k, KeyFn key, uint2* buf, int cap,
                                          uint32_t* ws, uint32_t* hist, uint32_t* sh) {
@@ -519,7 +521,7 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
             const int i = t + 256 * j;
             x[j] = i < a.k ? key(i) : 0u;
         }
-        const float thr = band_of(key2f(n <= 64 ? block_kth_regs_small<VALS>(x, a.k, n, key, sorted, MAX_RESCORE, ws, hist, sh)
+        const float thr = band_of(key2f(n <= 16 ? block_kth_regs_small<VALS>(x, a.k, n, key, sorted, MAX_RESCORE, ws, hist, sh)
                                                 : block_kth_regs<VALS>(x, a.k, n, hist, sh)));
 #pragma unroll
         for (int j = 0; j < VALS; j++) {
@@ -550,7 +552,7 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
         if ((t & 63) == 0) red[t >> 6] = mx;
         // ends with a barrier
         auto tkey = [&](int i) { return f2key(T[i] + 0.0f); };
-        const float t0 = key2f(n <= 64 ? block_kth_regs_small<VALS>(tk, nt, n, tkey, sorted, MAX_RESCORE, ws, hist, sh)
+        const float t0 = key2f(n <= 16 ? block_kth_regs_small<VALS>(tk, nt, n, tkey, sorted, MAX_RESCORE, ws, hist, sh)
                                        : block_kth_regs<VALS>(tk, nt, n, hist, sh));
         const float top = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
         const float lim = t0 - 2.2f * e - 1e-6f * (1.0f + fmaxf(fabsf(t0), fabsf(top)));
