@@ -2,8 +2,11 @@
 //
 // 1. k_gemm_bf16_route: S = Q * C^T with v_mfma_f32_16x16x32_bf16 (BF16 inputs, f32
 //    accumulation) for every (query, centroid), and the maximum of every 16 consecutive
-//    scores of a query.
-// 2. k_route_select, one workgroup per query: the nprobe-th largest approximate score (a radix
+//    scores of a query. Fused (route_fuse: few probes over many centroids) it writes only the
+//    maxima, and the select recomputes the scores of the tiles it reads from the same BF16
+//    operands (BF16 x BF16 products are exact in f32, so any summation order keeps the bound
+//    below).
+// 2. k_route_select, one workgroup per query: the nprobe-th largest approximate score (a
 //    select over the tile maxima, then over the scores of the tiles that can hold the top
 //    nprobe); every centroid whose approximate score lies within twice the BF16
 //    error bound of it is rescored in f32 exactly as the reference's gemm_abt fallback sums
