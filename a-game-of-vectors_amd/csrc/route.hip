@@ -35,6 +35,16 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------ BF16 routing GEMM ---
+// The maximum of x over lanes l, l ^ 16, l ^ 32 and l ^ 48 (the four 16-lane rows of a wave):
+// v_permlane32_swap pairs each lane with lane l ^ 32 (one of its two results holds each), then
+// v_permlane16_swap pairs an odd row with an even one. Every lane gets the maximum.
+__device__ __forceinline__ float max_over_rows(float x) {
+    const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    const float m = fmaxf(__uint_as_float(a[0]), __uint_as_float(a[1]));
+    const auto b = __builtin_amdgcn_permlane16_swap(__float_as_uint(m), __float_as_uint(m), false, false);
+    return fmaxf(__uint_as_float(b[0]), __uint_as_float(b[1]));
+}
+
 // S = Q C^T for every (query, centroid) with v_mfma_f32_16x16x32_bf16, K = 256. Each wave
 // owns 64 centroids, whose BF16 rows stay in registers (4 MFMA tiles x 8 k-steps, 128 VGPRs),
 // and walks the 16-query tiles of its workgroup's query range (staged in LDS, below): centroid
@@ -111,6 +121,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
             for (int j = 0; j < 4; j++) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bc[j][s], cur[s], acc[j], 0, 0, 0);
         const int q = qt * 16 + r16;
+        float mxs[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int c0 = cb + j * 16 + c4;
@@ -134,10 +145,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 for (int v = 0; v < 4; v++)
                     if (c0 + v < n) mx = fmaxf(mx, L2 ? 2.0f * acc[j][v] - cn[j][v] : acc[j][v]);
             }
-            mx = fmaxf(mx, __shfl_xor(mx, 16, 64));
-            mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+            mxs[j] = mx;
+        }
+        // The four lanes of a query (rows of 16 lanes) hold a quarter of each tile: their
+        // maximum in registers (permlane swaps, no LDS round trip), four tiles side by side.
+#pragma unroll
+        for (int j = 0; j < 4; j++) mxs[j] = max_over_rows(mxs[j]);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
             const int tl = (cb >> 4) + j;
-            if (FULL || (lane < 16 && q < m && tl < ntile)) tmax[(size_t)q * ntile + tl] = mx;
+            if (FULL || (lane < 16 && q < m && tl < ntile)) tmax[(size_t)q * ntile + tl] = mxs[j];
         }
     };
     const int qfull = (vec && cb + 64 <= n) ? min(qt1, m / 16) : qt0;  // tiles before qfull are FULL
