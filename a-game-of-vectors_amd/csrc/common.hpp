@@ -86,6 +86,34 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     return v;
 }
 
+// Sum over aligned groups of G lanes (G = 2, 4, 8 or 16) with DPP row operations, in registers
+// (a shuffle goes through LDS): the quad permutes pair lane l with l ^ 1 and l ^ 2, the half-row
+// mirror (lane i with 7 - i of its 8) the two quads of a half row, the row mirror (i with
+// 15 - i of its 16) the two halves of a row. Every lane of a group gets the group's sum; all
+// lanes must be active.
+template <int G>
+__device__ __forceinline__ int group_sum_dpp(int v) {
+    static_assert(G == 2 || G == 4 || G == 8 || G == 16, "groups of 2 to 16 lanes");
+    v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);  // quad_perm [1, 0, 3, 2]
+    if constexpr (G >= 4) v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);  // quad_perm [2, 3, 0, 1]
+    if constexpr (G >= 8) v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    if constexpr (G >= 16) v += __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);  // row_mirror
+    return v;
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+template <int G>
+__device__ __forceinline__ float group_sum_dpp(float v) {
+    static_assert(G == 2 || G == 4 || G == 8 || G == 16, "groups of 2 to 16 lanes");
+    v += dpp_f32<0xB1>(v);
+    if constexpr (G >= 4) v += dpp_f32<0x4E>(v);
+    if constexpr (G >= 8) v += dpp_f32<0x141>(v);
+    if constexpr (G >= 16) v += dpp_f32<0x140>(v);
+    return v;
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);

@@ -166,8 +166,7 @@ __global__ __launch_bounds__(256) void k_rerank_l2(L2RerankArgs a) {
             for (int j = 0; j < 4; j++)
                 if (sub + 8 * j < 25) acc = __builtin_amdgcn_sdot4(row[sub + 8 * j], wq[j], acc, false);
         }
-#pragma unroll
-        for (int d = 4; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
+        acc = group_sum_dpp<8>(acc);  // the 8 lanes of the row (integer: any order)
         if (sub == 0 && i < n) {
             score[i] = 2.0f * (float)acc - a.cand_nx[qi * cs + i];
             ids[i] = id;

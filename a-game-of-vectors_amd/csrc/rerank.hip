@@ -105,8 +105,7 @@ __device__ void int8_scores(const RerankArgs& a, const int8_t* __restrict__ wq, 
                     acc = __builtin_amdgcn_sdot4((int)x[r][j].w, wv[j][3], acc, false);
                 }
             }
-#pragma unroll
-            for (int d = 8; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
+            acc = group_sum_dpp<16>(acc);  // the 16 lanes of the row (integer: any order)
             if (sub == 0 && i < n) score[i] = (float)acc * inv;
         }
     }

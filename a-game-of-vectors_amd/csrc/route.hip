@@ -611,9 +611,7 @@ __global__ __launch_bounds__(256) void k_route_select(RouteArgs a) {
                     s = s + x1.z * __uint_as_float(w.w << 16);
                     s = s + x1.w * __uint_as_float(w.w & 0xFFFF0000u);
                 }
-                s += __shfl_xor(s, 1, 64);
-                s += __shfl_xor(s, 2, 64);
-                s += __shfl_xor(s, 4, 64);
+                s = group_sum_dpp<8>(s);
                 const float v = a.l2 ? 2.0f * s - a.cnorm[c] : s;
                 if (l8 == 0 && v >= lim) {
                     const uint32_t j = atomicAdd(&ncand, 1u);
