@@ -421,7 +421,7 @@ __device__ uint32_t block_kth_regs_small(const uint32_t (&x)[VALS], int n, int k
         if (b) {  // wave-uniform
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(&ws[0], (uint32_t)__popcll(b));
-            base = __shfl(base, 0, 64) + (uint32_t)__popcll(b & ((1ull << lane) - 1));
+            base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0) + (uint32_t)__popcll(b & ((1ull << lane) - 1));
             if (in && base < (uint32_t)cap) buf[base] = make_uint2(x[j], ~(uint32_t)(t + 256 * j));
         }
     }

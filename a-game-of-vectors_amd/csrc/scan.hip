@@ -291,7 +291,7 @@ void k_scan_t(ScanArgs a) {
                 base = atomicAdd(&misc[M_COUNT], nw);
                 if (base + nw >= (uint32_t)a.trigger) misc[M_STAMP + (round & 1)] = round;
             }
-            base = __shfl(base, 0, 64);
+            base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);  // every lane is active here
             if (pass) buf[base + before] = make_uint2(key, lo);
         }
     };
@@ -427,7 +427,7 @@ void k_scan_t(ScanArgs a) {
                     const uint32_t before = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
                     uint32_t base = 0;
                     if (lane == 0) base = atomicAdd(a.ext_n + qi, nw);
-                    base = __shfl(base, 0, 64);
+                    base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
                     if (pass[u] && base + before < a.ext_cap)
                         a.ext[qi * (size_t)a.ext_cap + base + before] = make_uint2(key[u], ~R.gs[u]);
                 }
@@ -1015,7 +1015,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     auto claim = [&]() {
         uint32_t v = 0;
         if (lane == 0) v = i_lo + atomicAdd(a.work + a.phase, 1u);
-        return (uint32_t)__shfl(v, 0, 64);
+        return (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
     };
     uint32_t nidx = claim();
     uint4 nitem = nidx < i_hi ? a.items[nidx] : make_uint4(0u, 0u, 0u, 0u);
@@ -1292,7 +1292,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     auto claim = [&]() {
         uint32_t v = 0;
         if (lane == 0) v = i_lo + atomicAdd(a.work + a.phase, 1u);
-        return __builtin_amdgcn_readfirstlane((uint32_t)__shfl(v, 0, 64));
+        return (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
     };
     uint32_t nidx = claim();
     uint4 nitem = nidx < i_hi ? a.items[nidx] : make_uint4(0u, 0u, 0u, 0u);
