@@ -114,6 +114,39 @@ __device__ __forceinline__ float group_sum_dpp(float v) {
     return v;
 }
 
+// The maximum of a u64 over the wave, in registers: DPP quad permutes and the half-row and row
+// mirrors give every lane its row's maximum, v_permlane16_swap pairs each row with another (of
+// its two results one holds the lane's own value and one its partner's, in both halves alike)
+// and v_permlane32_swap the two halves of the wave. All lanes must be active.
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long x) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)x, CTRL, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(x >> 32), CTRL, 0xF, 0xF, false);
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ unsigned long long max_u64(unsigned long long a, unsigned long long b) {
+    return a > b ? a : b;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long b) {
+    b = max_u64(b, dpp_u64<0xB1>(b));   // quad_perm [1, 0, 3, 2]
+    b = max_u64(b, dpp_u64<0x4E>(b));   // quad_perm [2, 3, 0, 1]
+    b = max_u64(b, dpp_u64<0x141>(b));  // row_half_mirror
+    b = max_u64(b, dpp_u64<0x140>(b));  // row_mirror
+    {
+        const uint32_t lo = (uint32_t)b, hi = (uint32_t)(b >> 32);
+        const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        b = max_u64(((unsigned long long)h[0] << 32) | l[0], ((unsigned long long)h[1] << 32) | l[1]);
+    }
+    {
+        const uint32_t lo = (uint32_t)b, hi = (uint32_t)(b >> 32);
+        const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+        const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+        b = max_u64(((unsigned long long)h[0] << 32) | l[0], ((unsigned long long)h[1] << 32) | l[1]);
+    }
+    return b;
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);

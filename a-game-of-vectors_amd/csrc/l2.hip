@@ -173,11 +173,24 @@ __global__ __launch_bounds__(256) void k_rerank_l2(L2RerankArgs a) {
         }
     }
     __syncthreads();
-    const int n2 = pow2_at_least(max(n, 1));
-    for (int i = t; i < n2; i += 256) keys[i] = i < n ? make_uint2(f2key(score[i] + 0.0f), ~ids[i]) : make_uint2(0u, 0u);
-    __syncthreads();
-    block_sort_pairs(keys, nullptr, n2, false);
+    // The top k by (score desc, row asc): k rounds of a block maximum for small k (rows are
+    // unique: one slot per row), else a full sort.
     const int kk = min(a.k, n);
+    if (a.k <= 32) {
+        __shared__ unsigned long long red[8];
+        __shared__ uint2 best[32];
+        for (int i = t; i < n; i += 256) keys[i] = make_uint2(f2key(score[i] + 0.0f), ~ids[i]);
+        __syncthreads();
+        block_topk_small(keys, n, a.k, best, red);
+        for (int i = t; i < kk; i += 256) keys[i] = best[i];
+        __syncthreads();
+    } else {
+        const int n2 = pow2_at_least(max(n, 1));
+        for (int i = t; i < n2; i += 256)
+            keys[i] = i < n ? make_uint2(f2key(score[i] + 0.0f), ~ids[i]) : make_uint2(0u, 0u);
+        __syncthreads();
+        block_sort_pairs(keys, nullptr, n2, false);
+    }
     for (int i = t; i < kk; i += 256) {
         a.out_s[qi * a.k + i] = key2f(keys[i].x);
         a.out_r[qi * a.k + i] = ~keys[i].y;

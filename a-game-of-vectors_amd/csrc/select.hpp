@@ -142,11 +142,7 @@ __device__ __forceinline__ int block_topk_small(const uint2* buf, int n, int k, 
         unsigned long long b = 0;
 #pragma unroll
         for (int j = 0; j < 8; j++) b = v[j] > b ? v[j] : b;
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            const unsigned long long o = __shfl_xor(b, d, 64);
-            b = o > b ? o : b;
-        }
+        b = wave_max_u64(b);
         unsigned long long* slot = red + 4 * (r & 1);
         if (lane == 0) slot[wave] = b;
         __syncthreads();  // the other parity's slots were read before the previous barrier
