@@ -42,6 +42,34 @@ __global__ __launch_bounds__(FT) void k_filter_counts(const uint32_t* __restrict
     if (threadIdx.x == 0) per_cluster[c] = sum;
 }
 
+// Cluster c's allowed slots rounded up to whole blocks of 32 (the gathered list's length).
+__global__ void k_padded_counts(const uint32_t* __restrict__ per_cluster, uint32_t k, uint32_t* __restrict__ padded) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c <= k) padded[c] = c < k ? (per_cluster[c] + BLOCK - 1) / BLOCK * BLOCK : 0u;
+}
+
+// Cluster c's allowed slots in slot order, then PAD to the end of its list (one workgroup per
+// cluster; an ordered compaction of each 256-slot stretch).
+__global__ __launch_bounds__(FT) void k_filter_slots(const uint32_t* __restrict__ block_off,
+                                                     const uint32_t* __restrict__ ids,
+                                                     const uint64_t* __restrict__ allow, uint64_t words,
+                                                     const uint32_t* __restrict__ goff, uint32_t* __restrict__ gslots) {
+    __shared__ uint32_t scratch[8];
+    const uint32_t c = blockIdx.x;
+    const size_t s0 = (size_t)block_off[c] * BLOCK, s1 = (size_t)block_off[c + 1] * BLOCK;
+    uint32_t o = goff[c];
+    for (size_t s = s0; s < s1; s += FT) {
+        const size_t si = s + threadIdx.x;
+        const uint32_t id = si < s1 ? ids[si] : PAD;
+        const uint32_t f = id != PAD && bit_set(allow, words, id) ? 1u : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(f, scratch, &tot);
+        if (f) gslots[o + ex] = (uint32_t)si;
+        o += tot;
+    }
+    for (uint32_t i = o + threadIdx.x; i < goff[c + 1]; i += FT) gslots[i] = PAD;
+}
+
 __global__ void k_word_popc(const uint64_t* __restrict__ allow, uint64_t words, uint32_t* __restrict__ cnt) {
     const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w < words) cnt[w] = (uint32_t)__popcll(allow[w]);
@@ -290,9 +318,15 @@ struct sieve_filter {
     uint32_t* rows = nullptr;
     uint64_t nrows = 0;
     uint32_t* per_cluster = nullptr;
+    // Filters below the Post plan's share of the rows keep each cluster's allowed slots (padded
+    // to blocks of 32): their Expand scans read only those.
+    uint32_t* gslots = nullptr;
+    uint32_t* goff = nullptr;      // k + 1
+    uint64_t gblocks = 0;
+    bool gather = true;            // sieve_filter_set_gather
     size_t device_bytes = 0;
     ~sieve_filter() {
-        void* ps[] = {allow, rows, per_cluster};
+        void* ps[] = {allow, rows, per_cluster, gslots, goff};
         for (void* p : ps)
             if (p) (void)hipFree(p);
     }
@@ -323,7 +357,8 @@ FilterRun enqueue_filter_search(const sieve_index* ix, const sieve_filter* f, ui
     SIEVE_REQUIRE(nq <= 65535, SIEVE_ELIMIT, "a filtered batch holds at most 65535 queries");
     FilterRun fr;
     fr.plan = sieve_plan_resolve(plan, f->nrows, ix->n);
-    if (plan == SIEVE_PLAN_AUTO && fr.plan == SIEVE_PLAN_EXPAND) expand_target = EXPAND_TARGET;
+    // AUTO's Expand, and an Expand given no target (the serving loop's), cover 200,000 slots.
+    if (fr.plan == SIEVE_PLAN_EXPAND && (plan == SIEVE_PLAN_AUTO || expand_target == 0)) expand_target = EXPAND_TARGET;
     uint32_t* ovf = sgrow<uint32_t>(s, B_OVF, 1);
     ViewArgs view{};
     view.allow = f->allow;
@@ -410,8 +445,14 @@ FilterRun enqueue_filter_search(const sieve_index* ix, const sieve_filter* f, ui
                        expand_target, maxp, ps, pc, pn, ovf);
     SIEVE_HIP(hipGetLastError());
     SIEVE_HIP(hipEventRecord(s->ev[1], s->st));
+    GatherArgs g;
+    if (f->gather && f->goff) {
+        g.slots = f->gslots;
+        g.off = f->goff;
+        g.blocks = f->gblocks;
+    }
     scan_batch(ix, s, bt, nq, ps, pc, pn, maxp, p, view, ce, cr, cx, cn, (int)p->cand, cnt, lv,
-               filtered_chunk_blocks(ix, f, p));
+               filtered_chunk_blocks(ix, f, p), g);
     SIEVE_HIP(hipEventRecord(s->ev[2], s->st));
     rerank_batch(ix, s, bt, nq, ce, cr, cx, cn, (int)p->cand, p, os, orr, on, cnt + 1, lv);
     SIEVE_HIP(hipEventRecord(s->ev[3], s->st));
@@ -485,11 +526,39 @@ int sieve_filter_create(const sieve_index* ix, const uint64_t* allow, uint64_t a
         (void)hipFree(cnt);
         (void)hipFree(off);
         (void)hipFree(tmp);
+        if ((double)total < POST_MIN * (double)ix->n) {
+            // The allowed-slot lists: padded counts, their offsets, then the slots.
+            const uint32_t k = ix->k;
+            uint32_t* padded = dmalloc<uint32_t>(k + 1, nullptr);
+            f->goff = dmalloc<uint32_t>(k + 1, acct);
+            hipLaunchKernelGGL(k_padded_counts, dim3(grid(k + 1, 256)), dim3(256), 0, st, f->per_cluster, k, padded);
+            size_t tb = 0;
+            SIEVE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, padded, f->goff, (int)(k + 1), st));
+            void* t2 = dmalloc<uint8_t>(tb, nullptr);
+            SIEVE_HIP(hipcub::DeviceScan::ExclusiveSum(t2, tb, padded, f->goff, (int)(k + 1), st));
+            uint32_t slots = 0;
+            SIEVE_HIP(hipMemcpy(&slots, f->goff + k, 4, hipMemcpyDeviceToHost));
+            f->gblocks = slots / BLOCK;
+            f->gslots = dmalloc<uint32_t>(std::max<uint32_t>(slots, 1), acct);
+            hipLaunchKernelGGL(k_filter_slots, dim3(k), dim3(FT), 0, st, ix->block_off, ix->ids, f->allow,
+                               allow_words, f->goff, f->gslots);
+            SIEVE_HIP(hipGetLastError());
+            SIEVE_HIP(hipDeviceSynchronize());
+            (void)hipFree(padded);
+            (void)hipFree(t2);
+        }
         *out = f.release();
     });
 }
 
 void sieve_filter_free(sieve_filter* f) { delete f; }
+
+int sieve_filter_set_gather(sieve_filter* f, int enable) {
+    return guarded([&] {
+        SIEVE_REQUIRE(f, SIEVE_EINVAL, "null filter");
+        f->gather = enable != 0;
+    });
+}
 
 int sieve_filter_get_info(const sieve_filter* f, uint64_t* rows, uint32_t* per_cluster) {
     return guarded([&] {

@@ -445,10 +445,18 @@ inline void set_scan_plan(ScanArgs& a, sieve_stream* s, size_t nq, uint32_t npro
     }
 }
 
+// The allowed-slot lists of a filter's Expand scan (ScanArgs::gather_slots / gather_off) and
+// their blocks in all (the scan's work bound).
+struct GatherArgs {
+    const uint32_t* slots = nullptr;
+    const uint32_t* off = nullptr;
+    uint64_t blocks = 0;
+};
+
 inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, size_t nq, const float* ps,
                 const uint32_t* pc, const uint32_t* pn, int pstride, const sieve_params* p, const ViewArgs& view,
                 float* ce, uint32_t* cr, float* cx, uint32_t* cn, int cstride, unsigned long long* scanned,
-                const LiveArgs* lv = nullptr, uint32_t min_chunk_blocks = 0) {
+                const LiveArgs* lv = nullptr, uint32_t min_chunk_blocks = 0, const GatherArgs& gather = GatherArgs{}) {
     if (!s->sizing) s->cm_timed = 0;
     ScanArgs a{};
     if (lv) {
@@ -550,8 +558,17 @@ inline void scan_batch(const sieve_index* ix, sieve_stream* s, const Batch& bt, 
         }
         variant = 2;
     }
-    set_scan_plan(a, s, nq, (uint32_t)pstride, (double)ix->nblocks / ix->k, ix->max_cluster_blocks,
-                  ix->scan_chunk_blocks, min_chunk_blocks);
+    if (gather.off) {
+        // Gathered lists: a query reads at most the filter's listed blocks and its delta lists.
+        a.gather_slots = gather.slots;
+        a.gather_off = gather.off;
+        const double work = (double)gather.blocks + (lv ? (double)lv->dblocks : 0.0);
+        set_scan_plan(a, s, nq, 1, std::max(1.0, work), (uint32_t)std::min(std::max(1.0, work), 4.0e9),
+                      ix->scan_chunk_blocks);
+    } else {
+        set_scan_plan(a, s, nq, (uint32_t)pstride, (double)ix->nblocks / ix->k, ix->max_cluster_blocks,
+                      ix->scan_chunk_blocks, min_chunk_blocks);
+    }
     if (s->sizing) return;
     launch_scan(a, nq, variant, s->st);
     SIEVE_HIP(hipGetLastError());

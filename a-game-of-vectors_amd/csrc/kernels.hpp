@@ -153,6 +153,11 @@ struct ScanArgs {
     // dbase + dblock_off[c + 1]) of the same slot arrays, scanned right after its base blocks.
     const uint32_t* dblock_off;
     uint32_t dbase;
+    // Gathered base segments (null: none), for a filter's Expand plan: cluster c's allowed slots,
+    // ascending and padded to whole blocks of 32 with PAD, are gather_slots[gather_off[c],
+    // gather_off[c + 1]); the scan reads those slots instead of the cluster's blocks.
+    const uint32_t* gather_slots;
+    const uint32_t* gather_off;
     int seg_stride;              // LDS segment table entries: scan_segments(), set by the launcher
     int trigger;                 // buffered entries that trigger a cut to cand (set by the launcher)
     // Cluster-major scan (launch_scan_cm): the seed pass scans query q's first probe_lim[q]

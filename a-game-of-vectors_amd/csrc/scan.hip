@@ -134,6 +134,9 @@ __device__ void scan_compact(uint2* buf, uint2* tmp, int n, int k, uint32_t* his
 // SCAN_APPEND (small batches): appends every slot above the query's threshold (seed_thr) to its
 // appended list unchecked, for the finish pass.
 constexpr int SCAN_FULL = 0, SCAN_SEED = 1, SCAN_FINISH = 2, SCAN_TIGHTEN = 3, SCAN_BOUND = 4, SCAN_APPEND = 5;
+// A segment's start in the LDS table with this bit set is an offset into gather_slots (its
+// blocks are gathered blocks of 32 listed slots), else a block index of the slot arrays.
+constexpr uint32_t GATHER_BIT = 0x80000000u;
 constexpr int BOUND_PER_CHUNK = 16;
 // seed_n flag: an appended list overflowed, so the query is rescanned by the finish pass.
 constexpr uint32_t SEED_LOST = 0x80000000u;
@@ -173,7 +176,11 @@ void k_scan_t(ScanArgs a) {
     for (int j = t; j < np; j += SCAN_THREADS) {
         const int pj = live ? j >> 1 : j;
         const uint32_t c = a.probe_c[qi * a.probe_stride + pj];
-        if (!live || (j & 1) == 0) {
+        if (!L2 && (!live || (j & 1) == 0) && a.gather_off) {  // filters serve the inner-product index
+            const uint32_t g0 = a.gather_off[c];
+            pstart[j] = GATHER_BIT | g0;
+            pref[j] = (a.gather_off[c + 1] - g0) / BLOCK;
+        } else if (!live || (j & 1) == 0) {
             const uint32_t b0 = a.block_off[c];
             pstart[j] = b0;
             pref[j] = a.block_off[c + 1] - b0;
@@ -344,7 +351,16 @@ void k_scan_t(ScanArgs a) {
             // conditional, so the wait for one round's data leaves the next round's loads in flight.
             const uint32_t vbc = min(vb, vb_hi - 1);
             while (cur + 1 < np && pref[cur + 1] <= vbc) cur++;
-            R.gs[u] = (pstart[cur] + (vbc - pref[cur])) * BLOCK + (t & 31);
+            const uint32_t ps = pstart[cur], off = (vbc - pref[cur]) * BLOCK + (t & 31);
+            R.gs[u] = ps * BLOCK + off;
+            if constexpr (!L2) {
+                if (ps & GATHER_BIT) {
+                    // a listed slot; a padding entry reads slot 0 and is not scored
+                    const uint32_t sl = a.gather_slots[(ps & ~GATHER_BIT) + off];
+                    R.ok[u] = R.ok[u] && sl != PAD;
+                    R.gs[u] = sl != PAD ? sl : 0u;
+                }
+            }
             R.cent[u] = pcent[cur];
             const uint4* cp = a.codes + CW * (size_t)R.gs[u];
 #pragma unroll

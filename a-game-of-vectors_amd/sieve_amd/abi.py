@@ -202,6 +202,7 @@ def lib():
         "sieve_filter_free": (None, [vp]),
         "sieve_filter_get_info": (i32, [vp, ctypes.POINTER(ctypes.c_uint64), vp]),
         "sieve_plan_resolve": (u32, [u32, ctypes.c_uint64, ctypes.c_uint64]),
+        "sieve_filter_set_gather": (i32, [vp, i32]),
         "sieve_filter_search_batch": (i32, [vp, vp, u32, ctypes.c_uint64, vp, vp, vp, sz, P, vp, vp, vp,
                                             ctypes.POINTER(u32), T]),
         "sieve_live_open": (i32, [vp, ctypes.POINTER(LiveOpts), ctypes.POINTER(vp)]),

@@ -262,6 +262,11 @@ class Filter:
         abi.check(abi.lib().sieve_filter_get_info(self._h, None, abi.ptr(out)))
         return out
 
+    def set_gather(self, enable=True):
+        """Expand scans read only the filter's listed allowed slots (default), or whole clusters
+        (sieve_filter_set_gather); the results are the same."""
+        abi.check(abi.lib().sieve_filter_set_gather(self._h, 1 if enable else 0))
+
     def search(self, q768, q256, p: Params, plan="auto", target=200_000, timing=None, stream=None):
         """filter::search for a batch sharing this filter: (scores, rows, counts, plan used)."""
         ix = self._ix

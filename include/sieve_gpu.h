@@ -482,12 +482,18 @@ void sieve_filter_free(sieve_filter* f);
 /* The number of allowed rows and (per_cluster, NULL to skip: k u32) the allowed slots per
  * cluster. */
 int sieve_filter_get_info(const sieve_filter* f, uint64_t* rows, uint32_t* per_cluster);
+/* Expand scans over a filter of fewer than 30% of the rows (no counterpart in the reference):
+ * the filter keeps each cluster's allowed slots, and the scan of an Expand plan reads only those
+ * (the delta lists of a live state as before) instead of every slot of the probed clusters;
+ * candidates and results are the same, and sieve_stats.scanned then counts the listed slots
+ * (padded to blocks of 32). enable 0 scans whole clusters (default 1). */
+int sieve_filter_set_gather(sieve_filter* f, int enable);
 /* Plan::resolve (src/filter.rs:60-69): AUTO is BRUTE at <= 250,000 allowed rows, POST at >= 30%
  * of the n corpus rows, EXPAND (200,000 slots) between; other plans are kept. */
 uint32_t sieve_plan_resolve(uint32_t plan, uint64_t allowed_rows, uint64_t n);
 /* filter::search (src/filter.rs:72-110) for nq (<= 65535) queries sharing one filter: the top
- * p->k allowed rows as (score, row) best first. `expand_target` is EXPAND's slot target (AUTO
- * uses 200,000); the plan used is written to *resolved (may be NULL). EXPAND sorts every
+ * p->k allowed rows as (score, row) best first. `expand_target` is EXPAND's slot target (AUTO,
+ * and 0, use 200,000); the plan used is written to *resolved (may be NULL). EXPAND sorts every
  * centroid by its exact f32 score (at most 16384 centroids) and fails with SIEVE_ELIMIT if its
  * cut needs more than 8192 clusters (or the scan's LDS at that many exceeds the device's). */
 int sieve_filter_search_batch(const sieve_index* ix, const sieve_filter* f, uint32_t plan, uint64_t expand_target,

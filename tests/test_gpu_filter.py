@@ -58,3 +58,29 @@ def test_filter_search_matches_oracle(small, oracle_index, fx, plan, frac, targe
     for i in range(small.nq):
         assert np.isin(r[i, :n[i]], rows).all()
         assert_same_up_to_ties(r[i, :n[i]], s[i, :n[i]], wr[i, :wn[i]], ws[i, :wn[i]])
+
+
+@pytest.mark.parametrize("frac,target,cfg", [(0.01, 300, "12:400:full"), (0.05, 400, "12:400:full"),
+                                             (0.2, 3000, "8:500:prefix"), (0.12, 200_000, "16:300:full")])
+def test_expand_over_listed_slots_equals_whole_clusters(small, oracle_index, fx, frac, target, cfg):
+    """A filter below the Post share keeps each cluster's allowed slots; the Expand scan reads
+    only those: the same results bit for bit as scanning the probed clusters whole, fewer slots
+    read, and the oracle's results up to ties."""
+    from sieve_amd import abi
+    rows, allow = _allow(small, frac, 29)
+    f, of = Filter(fx, allow), O.Filter(oracle_index, allow)
+    _, p = parse_config(cfg, small.k, k=10)
+    t_list, t_whole = abi.Timing(), abi.Timing()
+    got = f.search(small.q768, small.q256, p, plan="expand", target=target, timing=t_list)
+    f.set_gather(False)
+    want = f.search(small.q768, small.q256, p, plan="expand", target=target, timing=t_whole)
+    f.set_gather(True)
+    for g, w in zip(got[:3], want[:3]):
+        assert np.array_equal(np.asarray(g).view(np.uint8), np.asarray(w).view(np.uint8))
+    assert got[3] == want[3] == 1  # Expand
+    assert t_list.scanned < t_whole.scanned
+    ws, wr, wn, _ = of.search(small.q768, small.q256, p, plan="expand", target=target)
+    s, r, n = got[:3]
+    for i in range(small.nq):
+        assert np.isin(r[i, :n[i]], rows).all()
+        assert_same_up_to_ties(r[i, :n[i]], s[i, :n[i]], wr[i, :wn[i]], ws[i, :wn[i]])

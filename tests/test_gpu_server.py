@@ -138,6 +138,51 @@ def test_filters_and_live_writes_in_the_serving_loop(small, oracle_index):
     live.close()
 
 
+def test_expand_over_listed_slots_with_live_writes(small):
+    """Expand-filtered searches through the server with a live state: the base clusters' listed
+    allowed slots plus the delta lists (tombstones applied) give the same results bit for bit as
+    scanning the probed clusters whole, submitted between the same writes."""
+    import numpy as np
+    import torch
+    from sieve_amd.build import normalize_rows
+    from sieve_amd.index import Filter, Live, bitset
+
+    ix = Index.create(small.host)
+    live = Live(ix, max_rows=1000)
+    p = Params(nprobe=12, cand=400, k=10, rerank=Rerank.FULL)
+    rng = np.random.default_rng(23)
+    allow_rows = np.sort(rng.choice(small.n + 1000, (small.n + 1000) // 20, replace=False))
+    allow = bitset(allow_rows, small.n + 1000)
+    listed, whole = Filter(ix, allow), Filter(ix, allow)
+    whole.set_gather(False)
+    new, _ = normalize_rows(small.mix.rows(small.n + 700, 200))
+    new = new.numpy().astype(np.float32)
+    q768 = np.concatenate([small.q768[:10], new[:6]])
+    q256 = np.concatenate([small.q256[:10], normalize_rows(torch.from_numpy(new[:6, :256].copy()))[0].numpy()])
+    sv = Server(ix, p, max_batch=8, max_inflight=3, live=live)
+    pairs = []
+
+    def searches():
+        for i in range(len(q768)):
+            pairs.append((sv.submit_filtered(q768[i], q256[i], listed, plan="expand"),
+                          sv.submit_filtered(q768[i], q256[i], whole, plan="expand")))
+
+    searches()
+    sv.insert((90_000_000 + np.arange(150)).astype(np.uint32), new[:150])
+    searches()
+    sv.delete(np.concatenate([np.arange(90_000_000, 90_000_040), allow_rows[:30]]).astype(np.uint32))
+    searches()
+    nonempty = 0
+    for a, b in pairs:
+        ga, gb = sv.wait(a), sv.wait(b)
+        assert np.array_equal(ga[1], gb[1]) and np.array_equal(np.asarray(ga[0]).view(np.uint32),
+                                                               np.asarray(gb[0]).view(np.uint32))
+        nonempty += len(ga[1]) > 0
+    assert nonempty == len(pairs)
+    sv.close()
+    live.close()
+
+
 def test_zero_query_in_a_batch_does_not_fail_the_others(small):
     """A degenerate request (zero vectors: every centroid ties, so its routing takes the exact
     path) batched with normal requests: every request succeeds, and each normal one gets the
