@@ -251,11 +251,20 @@ __global__ __launch_bounds__(FT) void k_expand_cut(const float* __restrict__ sco
         e[i] = i < K ? make_uint2(f2key(scores[q * K + i] + 0.0f), ~(uint32_t)i) : make_uint2(0u, 0u);
     __syncthreads();
     block_sort_pairs(e, nullptr, n2, false);
-    // Thread t owns sorted entries [t * per, (t + 1) * per): their allowed slots, then a block
-    // scan gives each entry the slots covered before it.
+    // Thread t owns sorted entries [t * per, (t + 1) * per) (per <= 64): their allowed slots,
+    // loaded once into registers with every load in flight together, then a block scan gives
+    // each entry the slots covered before it.
+    static_assert(EXPAND_MAX_K <= 64 * FT, "a thread's sorted entries fit its 64 weight registers");
     const int per = (K + FT - 1) / FT;
+    uint32_t wv[64];
+#pragma unroll
+    for (int j = 0; j < 64; j++) {
+        const int i = t * per + j;
+        wv[j] = j < per && i < K ? per_cluster[~e[i].y] : 0u;
+    }
     uint64_t mine = 0;
-    for (int i = t * per; i < min(K, (t + 1) * per); i++) mine += per_cluster[~e[i].y];
+#pragma unroll
+    for (int j = 0; j < 64; j++) mine += wv[j];
     // 64-bit block prefix (the u32 scan on high and low halves would carry wrongly): a wave scan
     // of u64 via shuffles, then across the 4 waves.
     uint64_t incl = mine;
@@ -275,8 +284,9 @@ __global__ __launch_bounds__(FT) void k_expand_cut(const float* __restrict__ sco
     uint32_t kept = 0;
     {
         uint64_t cov = before;
-        for (int i = t * per; i < min(K, (t + 1) * per); i++) {
-            const uint32_t w = per_cluster[~e[i].y];
+#pragma unroll
+        for (int j = 0; j < 64; j++) {
+            const uint32_t w = wv[j];  // 0 past the thread's entries
             if (w == 0) continue;
             if (cov < target) kept++;
             cov += w;
@@ -293,13 +303,14 @@ __global__ __launch_bounds__(FT) void k_expand_cut(const float* __restrict__ sco
         return;
     }
     uint64_t cov = before;
-    for (int i = t * per; i < min(K, (t + 1) * per); i++) {
-        const uint32_t c = ~e[i].y;
-        const uint32_t w = per_cluster[c];
+#pragma unroll
+    for (int j = 0; j < 64; j++) {
+        const uint32_t w = wv[j];
         if (w == 0) continue;
         if (cov < target) {
+            const int i = t * per + j;
             ps[q * max_probe + off] = key2f(e[i].x);
-            pc[q * max_probe + off] = c;
+            pc[q * max_probe + off] = ~e[i].y;
             off++;
         }
         cov += w;
