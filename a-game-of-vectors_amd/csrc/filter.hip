@@ -236,89 +236,133 @@ __global__ __launch_bounds__(FT) void k_exact_route(const float* __restrict__ q2
 
 // Expand's probe list (src/filter.rs:80-93): all centroids by (score desc, id asc), clusters
 // with no allowed slot skipped, taken while the allowed slots covered so far are below target.
+// The taken clusters are the weighted ones up to and including the first entry, in that order,
+// whose inclusive covered count reaches the target. A weighted radix select over the 64-bit keys
+// (score key, ~id) finds that entry: each 8-bit pass sums the weights per digit among the entries
+// on the prefix found so far and keeps the digit where the running sum reaches what is left of
+// the target. An ordered compaction keeps the weighted entries at or above it, and only those
+// are sorted. Thread t holds entries i = t + 256 j (j < 64): the key in LDS, the weight in a
+// register.
 __global__ __launch_bounds__(FT) void k_expand_cut(const float* __restrict__ scores, int K,
                                                    const uint32_t* __restrict__ per_cluster, uint64_t target,
                                                    int max_probe, float* __restrict__ ps, uint32_t* __restrict__ pc,
                                                    uint32_t* __restrict__ pn, uint32_t* __restrict__ overflow) {
-    extern __shared__ __align__(16) uint8_t lds[];
-    uint2* e = reinterpret_cast<uint2*>(lds);
-    uint32_t* scratch = reinterpret_cast<uint32_t*>(lds) + 2 * EXPAND_MAX_K;  // 16 words
+    extern __shared__ __align__(16) uint8_t lds[];  // expand_cut_lds(K) bytes
+    int kp = 1;
+    while (kp < K) kp <<= 1;
+    uint2* e = reinterpret_cast<uint2*>(lds);                    // kp entries
+    uint32_t* hist = reinterpret_cast<uint32_t*>(lds) + 2 * kp;  // 256 words
+    uint32_t* scratch = hist + 256;                              // 16 words
+    __shared__ uint32_t sel[2];                                  // digit, what is left
+    static_assert(EXPAND_MAX_K <= 64 * FT, "a thread's entries fit its 64 weight registers");
     const size_t q = blockIdx.x;
     const int t = threadIdx.x;
-    int n2 = 1;
-    while (n2 < K) n2 <<= 1;
-    for (int i = t; i < n2; i += FT)
-        e[i] = i < K ? make_uint2(f2key(scores[q * K + i] + 0.0f), ~(uint32_t)i) : make_uint2(0u, 0u);
-    __syncthreads();
-    block_sort_pairs(e, nullptr, n2, false);
-    // Thread t owns sorted entries [t * per, (t + 1) * per) (per <= 64): their allowed slots,
-    // loaded once into registers with every load in flight together, then a block scan gives
-    // each entry the slots covered before it.
-    static_assert(EXPAND_MAX_K <= 64 * FT, "a thread's sorted entries fit its 64 weight registers");
-    const int per = (K + FT - 1) / FT;
     uint32_t wv[64];
+    uint32_t wsum = 0;
 #pragma unroll
     for (int j = 0; j < 64; j++) {
-        const int i = t * per + j;
-        wv[j] = j < per && i < K ? per_cluster[~e[i].y] : 0u;
+        const int i = t + FT * j;
+        wv[j] = i < K ? per_cluster[i] : 0u;
+        wsum += wv[j];  // an index holds fewer than 2^32 - 1 slots: the sums fit 32 bits
+        // a cluster with no allowed slot gets key (0, 0): below every real key, whose low word
+        // ~id is never 0
+        if (i < K) e[i] = wv[j] ? make_uint2(f2key(scores[q * K + i] + 0.0f), ~(uint32_t)i) : make_uint2(0u, 0u);
     }
-    uint64_t mine = 0;
+    uint32_t wtot;
+    (void)block_excl_scan(wsum, scratch, &wtot);
+    // every weighted entry when the total stays below the target; none for a zero target
+    const bool all = (uint64_t)wtot < target;
+    uint32_t p_hi = 0, m_hi = 0, p_lo = 0, m_lo = 0;
+    if (!all && target > 0) {
+        uint32_t rem = (uint32_t)target;  // <= wtot
+        for (int pass = 0; pass < 8; pass++) {
+            const bool hi = pass < 4;
+            const int shift = 24 - 8 * (pass & 3);
+            hist[t] = 0;
+            __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 64; j++) mine += wv[j];
-    // 64-bit block prefix (the u32 scan on high and low halves would carry wrongly): a wave scan
-    // of u64 via shuffles, then across the 4 waves.
-    uint64_t incl = mine;
-    const int lane = t & 63, wave = t >> 6;
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint64_t y = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += y;
-    }
-    uint64_t* wsum = reinterpret_cast<uint64_t*>(scratch);
-    if (lane == 63) wsum[wave] = incl;
-    __syncthreads();
-    uint64_t before = incl - mine;
-    for (int w = 0; w < wave; w++) before += wsum[w];
-    // Kept entries: allowed slots > 0 and covered-before < target; they are a prefix of the
-    // allowed clusters in sorted order, so each thread counts its kept entries and a second scan
-    // places them.
-    uint32_t kept = 0;
-    {
-        uint64_t cov = before;
-#pragma unroll
-        for (int j = 0; j < 64; j++) {
-            const uint32_t w = wv[j];  // 0 past the thread's entries
-            if (w == 0) continue;
-            if (cov < target) kept++;
-            cov += w;
+            for (int j = 0; j < 64; j++) {
+                const int i = t + FT * j;
+                if (i >= K || wv[j] == 0) continue;
+                const uint2 v = e[i];
+                if ((v.x & m_hi) == p_hi && (v.y & m_lo) == p_lo)
+                    atomicAdd(&hist[((hi ? v.x : v.y) >> shift) & 255u], wv[j]);
+            }
+            __syncthreads();
+            if (t < 64) {
+                // lane l holds digits 255 - 4 l .. 252 - 4 l (top first)
+                const uint32_t c0 = hist[255 - 4 * t], c1 = hist[254 - 4 * t], c2 = hist[253 - 4 * t],
+                               c3 = hist[252 - 4 * t];
+                const uint32_t sum = c0 + c1 + c2 + c3;
+                const uint32_t incl = wave_incl_scan(sum);
+                const uint64_t hit = __ballot(incl >= rem);
+                const int f = __ffsll((long long)hit) - 1;
+                if (t == f) {
+                    uint32_t cum = incl - sum;
+                    const uint32_t cs[4] = {c0, c1, c2, c3};
+                    for (int k = 0; k < 4; k++) {
+                        if (cum + cs[k] >= rem) {
+                            sel[0] = 255 - 4 * t - k;
+                            sel[1] = rem - cum;
+                            break;
+                        }
+                        cum += cs[k];
+                    }
+                }
+            }
+            __syncthreads();
+            if (hi) {
+                p_hi |= sel[0] << shift;
+                m_hi |= 0xFFu << shift;
+            } else {
+                p_lo |= sel[0] << shift;
+                m_lo |= 0xFFu << shift;
+            }
+            rem = sel[1];
+            __syncthreads();
         }
     }
+    // Ordered compaction, chunk j of every thread at once: each kept entry moves to the front,
+    // never past an entry not yet read (its new place is at most its old one).
+    uint32_t m = 0;
+    for (int j = 0; j < 64 && FT * j < K; j++) {
+        const int i = t + FT * j;
+        const uint2 v = i < K ? e[i] : make_uint2(0u, 0u);
+        const bool keep = v.y != 0 && target > 0 && (all || v.x > p_hi || (v.x == p_hi && v.y >= p_lo));
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(keep ? 1u : 0u, scratch, &tot);
+        if (keep) e[m + ex] = v;
+        m += tot;
+    }
     __syncthreads();
-    uint32_t tot;
-    uint32_t off = block_excl_scan(kept, scratch + 8, &tot);
-    if ((int)tot > max_probe) {
+    if ((int)m > max_probe) {
         if (t == 0) {
             atomicOr(overflow, 1u);
             pn[q] = 0;
         }
         return;
     }
-    uint64_t cov = before;
-#pragma unroll
-    for (int j = 0; j < 64; j++) {
-        const uint32_t w = wv[j];
-        if (w == 0) continue;
-        if (cov < target) {
-            const int i = t * per + j;
-            ps[q * max_probe + off] = key2f(e[i].x);
-            pc[q * max_probe + off] = ~e[i].y;
-            off++;
-        }
-        cov += w;
+    int n2 = 1;
+    while (n2 < (int)m) n2 <<= 1;  // <= kp
+    for (int i = (int)m + t; i < n2; i += FT) e[i] = make_uint2(0u, 0u);  // below every real key
+    __syncthreads();
+    block_sort_pairs(e, nullptr, n2, false);
+    for (int i = t; i < (int)m; i += FT) {
+        ps[q * max_probe + i] = key2f(e[i].x);
+        pc[q * max_probe + i] = ~e[i].y;
     }
-    if (t == 0) pn[q] = tot;
+    if (t == 0) pn[q] = m;
 }
 
 inline unsigned grid(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+// k_expand_cut's dynamic LDS: the K keys padded to a power of two, the digit histogram, the
+// block scan's words.
+inline size_t expand_cut_lds(int K) {
+    size_t kp = 1;
+    while (kp < (size_t)K) kp <<= 1;
+    return kp * 8 + (256 + 16) * 4;
+}
 
 }  // namespace
 
@@ -450,8 +494,9 @@ FilterRun enqueue_filter_search(const sieve_index* ix, const sieve_filter* f, ui
     }
     hipLaunchKernelGGL(k_exact_route, dim3(grid(ix->k, FT), (unsigned)nq), dim3(FT), 0, s->st, q2, ix->cents,
                        (int)ix->k, all);
-    const size_t lds = (size_t)EXPAND_MAX_K * 8 + 16 * 8;
-    (void)hipFuncSetAttribute((const void*)k_expand_cut, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const size_t lds = expand_cut_lds((int)ix->k);
+    if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute((const void*)k_expand_cut, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_expand_cut, dim3((unsigned)nq), dim3(FT), lds, s->st, all, (int)ix->k, f->per_cluster,
                        expand_target, maxp, ps, pc, pn, ovf);
     SIEVE_HIP(hipGetLastError());

@@ -128,6 +128,28 @@ def test_route_over_many_centroids_matches_oracle(wide, cfg, fuse):
         assert_same_up_to_ties(c[i, :n[i]], s[i, :n[i]], wc, ws)
 
 
+@pytest.mark.parametrize("frac,target", [(0.3, 400), (0.02, 200), (0.3, 1), (0.01, 10**9)])
+def test_expand_cut_over_many_centroids_matches_oracle(wide, frac, target):
+    """Expand's probe list over 6000 centroids (24 per thread of the cut): the weighted
+    selection finds the centroid, in (score desc, id asc) order, at which the allowed slots
+    covered reach the target (one centroid for a target of 1; every centroid with an allowed
+    slot for a target above the total). Results equal the oracle's up to ties."""
+    from sieve_amd import parse_config
+    from sieve_amd.index import Filter
+    w, oix, gix = wide
+    rng = np.random.default_rng(41)
+    rows = np.sort(rng.choice(w.n, max(1, int(w.n * frac)), replace=False))
+    allow = bitset(rows, w.n)
+    f, of = Filter(gix, allow), O.Filter(oix, allow)
+    _, p = parse_config("12:400:full", w.k, k=10)
+    s, r, n, used = f.search(w.q768, w.q256, p, plan="expand", target=target)
+    ws, wr, wn, wused = of.search(w.q768, w.q256, p, plan="expand", target=target)
+    assert used == 1 and (wused == 1).all()  # Expand
+    for i in range(w.nq):
+        assert np.isin(r[i, :n[i]], rows).all()
+        assert_same_up_to_ties(r[i, :n[i]], s[i, :n[i]], wr[i, :wn[i]], ws[i, :wn[i]])
+
+
 @pytest.fixture(scope="module")
 def huge_k(small):
     """The small index with 20000 more, empty clusters (random unit centroids, some of them
