@@ -693,6 +693,18 @@ inline void finish_search(sieve_stream* s, size_t nq, sieve_timing* t) {
 // lists at the largest size the live state reserves (for sizing scratch).
 LiveArgs live_args(const sieve_live* l);
 LiveArgs live_args_max(const sieve_live* l);
+// Live::insert / Live::delete (src/live.rs:276-306): the rows encoded into the slot log, the
+// write-ahead log appended, the host state updated. With publish the tombstones and the arena
+// are published and waited for; without, they wait for live_publish_async.
+void live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, size_t n, bool publish);
+void live_delete(sieve_live* l, const uint32_t* ext, size_t n, bool publish);
+// Publishes the staged writes on the live state's own stream once the events `waits` (the
+// batches in flight, which read the current arena and tombstones) have fired, without waiting
+// on the host. Returns the event later searches wait on (hipStreamWaitEvent). Until
+// live_settle the arena's block count is its upper bound ceil(slots / 32) + k.
+hipEvent_t live_publish_async(sieve_live* l, const hipEvent_t* waits, size_t nwaits);
+// Waits for a pending live_publish_async and takes the arena's exact block count.
+void live_settle(const sieve_live* l);
 
 // filter::search enqueued without host synchronisation (csrc/filter.hip).
 struct FilterRun {

@@ -293,7 +293,9 @@ void tombstone(sieve_live* l, uint32_t id, std::vector<uint64_t>& changed) {
 
 // Rebuilds the arena from the slot log: a stable sort of the log by cluster, delta blocks per
 // cluster, then each slot to its place.
-void rebuild_arena(sieve_live* l, hipStream_t st) {
+// With exact, waits for the stream and reads the arena's block count back; without, leaves the
+// count to a read into blocks_pinned and takes its upper bound meanwhile.
+void rebuild_arena(sieve_live* l, hipStream_t st, bool exact = true) {
     sieve_index* ix = l->ix;
     const size_t m = l->nlog;
     const uint32_t k = ix->k;
@@ -338,6 +340,11 @@ void rebuild_arena(sieve_live* l, hipStream_t st) {
                            l->dblock_off, l->lcode, l->lmul, l->lerr, l->lrow, ix->codes + 2 * base, ix->mul + base,
                            ix->err + base, ix->ids + base);
     SIEVE_HIP(hipGetLastError());
+    if (!exact) {
+        SIEVE_HIP(hipMemcpyAsync(l->blocks_pinned, l->dblock_off + k, 4, hipMemcpyDeviceToHost, st));
+        l->arena_blocks = bound;
+        return;
+    }
     uint32_t blocks = 0;
     SIEVE_HIP(hipMemcpyAsync(&blocks, l->dblock_off + k, 4, hipMemcpyDeviceToHost, st));
     SIEVE_HIP(hipStreamSynchronize(st));
@@ -397,7 +404,11 @@ void apply_pass(sieve_live* l, const uint32_t* eh, size_t n, size_t r0, std::vec
 
 // Live::insert (src/live.rs:275-294): quantise on the device, log the records (op, external
 // id, prefix norm, int8 row) and wait for them to be durable, then apply.
-void live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, size_t n) {
+}  // namespace
+
+namespace sieve {
+
+void live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, size_t n, bool publish) {
     sieve_index* ix = l->ix;
     SIEVE_REQUIRE(l->nrows + n <= l->cap_rows, SIEVE_ELIMIT,
                   "live capacity: " + std::to_string(l->cap_rows) + " delta rows");
@@ -428,12 +439,25 @@ void live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, size_t n
     } else {
         SIEVE_HIP(hipStreamSynchronize(st));
     }
-    live_apply_rows(l, eh, n);
+    live_apply_rows(l, eh, n, publish);
 }
 
-}  // namespace
+void live_delete(sieve_live* l, const uint32_t* ext, size_t n, bool publish) {
+    std::vector<uint32_t> he;
+    const uint32_t* eh = ext;
+    if (n && on_device(ext)) {
+        he.resize(n);
+        SIEVE_HIP(hipMemcpy(he.data(), ext, n * 4, hipMemcpyDeviceToHost));
+        eh = he.data();
+    }
+    if (l->wal_fd >= 0 && n) {
+        std::vector<uint8_t> rec(n * WAL_DELETE_BYTES);
+        for (size_t j = 0; j < n; j++) wal_delete_record(eh[j], &rec[j * WAL_DELETE_BYTES]);
+        live_log(l, rec);
+    }
+    live_apply_delete(l, eh, n, publish);
+}
 
-namespace sieve {
 
 void launch_soar_pairs(const float* xs, size_t n, const float* cents, const uint32_t* pc, const uint32_t* pn,
                        int pstride, int soar, float lambda, uint32_t* pairs, hipStream_t st) {
@@ -463,7 +487,7 @@ void live_push_words(sieve_live* l, hipStream_t st, const std::vector<uint64_t>&
     SIEVE_HIP(hipStreamSynchronize(st));
 }
 
-void live_apply_rows(sieve_live* l, const uint32_t* eh, size_t n) {
+void live_apply_rows(sieve_live* l, const uint32_t* eh, size_t n, bool publish) {
     SIEVE_REQUIRE(l->nrows + n <= l->cap_rows, SIEVE_ELIMIT,
                   "live capacity: " + std::to_string(l->cap_rows) + " delta rows");
     if (n == 0) return;
@@ -474,11 +498,71 @@ void live_apply_rows(sieve_live* l, const uint32_t* eh, size_t n) {
         const size_t m = std::min(APPLY_ROWS, n - j0);
         apply_pass(l, eh + j0, m, l->nrows, changed);
     }
-    live_push_words(l, l->work.st, changed);
-    rebuild_arena(l, l->work.st);
+    l->pend.insert(l->pend.end(), changed.begin(), changed.end());
+    l->arena_dirty = true;
+    if (publish) live_publish(l);
 }
 
-void live_apply_delete(sieve_live* l, const uint32_t* eh, size_t n) {
+void live_settle(const sieve_live* l) {
+    std::lock_guard<std::mutex> lk(l->pub_mu);
+    if (!l->pub_pending) return;
+    SIEVE_HIP(hipEventSynchronize(l->pub_ev));
+    l->pub_pending = false;
+    if (l->blocks_from_pub) {
+        l->arena_blocks = *l->blocks_pinned;
+        l->blocks_from_pub = false;
+    }
+}
+
+void live_publish(sieve_live* l) {
+    live_settle(l);
+    live_push_words(l, l->work.st, l->pend);
+    l->pend.clear();
+    if (l->arena_dirty) {
+        rebuild_arena(l, l->work.st);
+        l->arena_dirty = false;
+    }
+}
+
+hipEvent_t live_publish_async(sieve_live* l, const hipEvent_t* waits, size_t nwaits) {
+    live_settle(l);  // one publish pending at most: its staging and scratch are free again
+    std::lock_guard<std::mutex> lk(l->pub_mu);
+    if (!l->pub) {
+        SIEVE_HIP(hipStreamCreate(&l->pub));
+        SIEVE_HIP(hipEventCreateWithFlags(&l->pub_ev, hipEventDisableTiming));
+        SIEVE_HIP(hipHostMalloc(reinterpret_cast<void**>(&l->blocks_pinned), 16, hipHostMallocDefault));
+    }
+    const hipStream_t st = l->pub;
+    const size_t nc = l->pend.size();
+    uint64_t* di = nullptr;
+    uint64_t* dv = nullptr;
+    if (nc) {
+        // Staged before the waits: the copies need not wait for the searches.
+        std::vector<uint64_t> vals(nc);
+        for (size_t i = 0; i < nc; i++) vals[i] = l->deleted_host[l->pend[i]];
+        di = grow<uint64_t>(l->words_i, nc);
+        dv = grow<uint64_t>(l->words_v, nc);
+        SIEVE_HIP(hipMemcpyAsync(di, l->pend.data(), nc * 8, hipMemcpyHostToDevice, st));
+        SIEVE_HIP(hipMemcpyAsync(dv, vals.data(), nc * 8, hipMemcpyHostToDevice, st));
+        SIEVE_HIP(hipStreamSynchronize(st));
+    }
+    for (size_t i = 0; i < nwaits; i++) SIEVE_HIP(hipStreamWaitEvent(st, waits[i], 0));
+    if (nc) {
+        hipLaunchKernelGGL(k_set_words, dim3(grid(nc, 256)), dim3(256), 0, st, l->deleted, di, dv, nc);
+        SIEVE_HIP(hipGetLastError());
+    }
+    l->pend.clear();
+    if (l->arena_dirty) {
+        rebuild_arena(l, st, false);
+        l->arena_dirty = false;
+        l->blocks_from_pub = true;
+    }
+    SIEVE_HIP(hipEventRecord(l->pub_ev, st));
+    l->pub_pending = true;
+    return l->pub_ev;
+}
+
+void live_apply_delete(sieve_live* l, const uint32_t* eh, size_t n, bool publish) {
     std::vector<uint64_t> changed;
     for (size_t j = 0; j < n; j++) {  // Live::apply_delete, src/live.rs:308-315
         uint32_t id;
@@ -487,7 +571,8 @@ void live_apply_delete(sieve_live* l, const uint32_t* eh, size_t n) {
             l->by_ext.erase(eh[j]);
         }
     }
-    live_push_words(l, l->work.st, changed);
+    l->pend.insert(l->pend.end(), changed.begin(), changed.end());
+    if (publish) live_publish(l);
 }
 
 sieve_live* live_create(sieve_index* ix, const sieve_live_opts& opts) {
@@ -562,7 +647,7 @@ int sieve_live_insert(sieve_live* l, const uint32_t* ext, const float* vecs, siz
     return guarded([&] {
         SIEVE_REQUIRE(l && (n == 0 || (ext && vecs)), SIEVE_EINVAL, "null argument");
         SIEVE_HIP(hipSetDevice(l->ix->device));
-        live_insert(l, ext, vecs, n);
+        live_insert(l, ext, vecs, n, true);
     });
 }
 
@@ -571,25 +656,15 @@ int sieve_live_delete(sieve_live* l, const uint32_t* ext, size_t n) {
     return guarded([&] {
         SIEVE_REQUIRE(l && (n == 0 || ext), SIEVE_EINVAL, "null argument");
         SIEVE_HIP(hipSetDevice(l->ix->device));
-        std::vector<uint32_t> he;
-        const uint32_t* eh = ext;
-        if (n && on_device(ext)) {
-            he.resize(n);
-            SIEVE_HIP(hipMemcpy(he.data(), ext, n * 4, hipMemcpyDeviceToHost));
-            eh = he.data();
-        }
-        if (l->wal_fd >= 0 && n) {
-            std::vector<uint8_t> rec(n * WAL_DELETE_BYTES);
-            for (size_t j = 0; j < n; j++) wal_delete_record(eh[j], &rec[j * WAL_DELETE_BYTES]);
-            live_log(l, rec);
-        }
-        live_apply_delete(l, eh, n);
+        live_delete(l, ext, n, true);
     });
 }
 
 int sieve_live_get_info(const sieve_live* l, sieve_live_info* out) {
     return guarded([&] {
         SIEVE_REQUIRE(l && out, SIEVE_EINVAL, "null argument");
+        SIEVE_HIP(hipSetDevice(l->ix->device));
+        live_settle(l);
         out->rows = l->nrows;
         out->slots = l->nlog;
         out->arena_blocks = l->arena_blocks;
@@ -609,6 +684,7 @@ int sieve_live_export(const sieve_live* l, int8_t* rows, float* norm256, uint8_t
     return guarded([&] {
         SIEVE_REQUIRE(l, SIEVE_EINVAL, "null argument");
         SIEVE_HIP(hipSetDevice(l->ix->device));
+        live_settle(l);
         const size_t r = l->nrows, base = l->ix->nslots, slots = l->arena_blocks * BLOCK;
         auto get = [&](void* dst, const void* src, size_t bytes) {
             if (dst && bytes) SIEVE_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
@@ -642,6 +718,7 @@ int sieve_live_search_batch(const sieve_live* l, sieve_stream* s, const float* q
         if (nq == 0) return;
         const sieve_index* ix = l->ix;
         SIEVE_HIP(hipSetDevice(ix->device));
+        live_settle(l);
         ensure_events(s);
         const float* q2 = dev_in(s, B_Q256, q256, nq * PDIM);
         const float* q7 = dev_in(s, B_Q768, q768, nq * DIM);

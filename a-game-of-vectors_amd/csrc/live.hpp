@@ -6,6 +6,7 @@
 #include <unistd.h>
 
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -19,7 +20,8 @@ struct sieve_live {
     uint64_t n = 0;            // corpus rows
     uint64_t cap_rows = 0;     // delta rows the device holds
     uint64_t cap_slots = 0;    // arena slots after the base slots
-    uint64_t nrows = 0, nlog = 0, arena_blocks = 0, ndeleted = 0;
+    uint64_t nrows = 0, nlog = 0, ndeleted = 0;
+    mutable uint64_t arena_blocks = 0;  // exact, or an upper bound while a publish is pending
     uint64_t seg_rows = 0;     // delta rows read from a compacted index's row segment (live.*)
     size_t device_bytes = 0;
     float* cents_rot = nullptr;
@@ -44,8 +46,26 @@ struct sieve_live {
     int wal_fd = -1;
     sieve_stream work;  // scratch for the encoder and the arena rebuild
     DevBuf xs, zq, pairs, sort_keys, sort_vals, sort_tmp, counts, nb, cstart, words_i, words_v;
+    // Staged writes: tombstone words changed and rows applied since the last publish.
+    std::vector<uint64_t> pend;
+    bool arena_dirty = false;
+    // A server's writes publish on a stream of their own, ordered after the batches in flight by
+    // device-side waits instead of a drain (live_publish_async). At most one such publish is
+    // pending; host-side readers settle it first (live_settle).
+    mutable std::mutex pub_mu;
+    hipStream_t pub = nullptr;
+    hipEvent_t pub_ev = nullptr;
+    mutable bool pub_pending = false;
+    mutable bool blocks_from_pub = false;
+    uint32_t* blocks_pinned = nullptr;  // the arena's block count, read back by the publish
 
     ~sieve_live() {
+        if (pub) {
+            (void)hipStreamSynchronize(pub);
+            (void)hipStreamDestroy(pub);
+        }
+        if (pub_ev) (void)hipEventDestroy(pub_ev);
+        if (blocks_pinned) (void)hipHostFree(blocks_pinned);
         void* ps[] = {cents_rot, rows, norm256, t2, t2n, ext, deleted, lcode, lmul, lerr, lrow, lcl, dblock_off};
         for (void* p : ps)
             if (p) (void)hipFree(p);
@@ -120,10 +140,14 @@ inline void wal_delete_record(uint32_t ext, uint8_t* out) {
 // a log), attached to the index.
 sieve_live* live_create(sieve_index* ix, const sieve_live_opts& o);
 // Live::apply_insert (src/live.rs:317-374) of n rows already quantised into the delta arrays at
-// row l->nrows on: ext_host are their external ids (host memory).
-void live_apply_rows(sieve_live* l, const uint32_t* ext_host, size_t n);
+// row l->nrows on: ext_host are their external ids (host memory). Without publish the tombstones
+// and the arena wait for the next live_publish / live_publish_async.
+void live_apply_rows(sieve_live* l, const uint32_t* ext_host, size_t n, bool publish = true);
 // Live::apply_delete (src/live.rs:308-315); ext_host in host memory.
-void live_apply_delete(sieve_live* l, const uint32_t* ext_host, size_t n);
+void live_apply_delete(sieve_live* l, const uint32_t* ext_host, size_t n, bool publish = true);
+// Publishes the staged writes on the work stream (tombstone words, then the arena rebuilt from
+// the slot log if rows were applied) and waits for them.
+void live_publish(sieve_live* l);
 // Uploads the tombstone words whose indexes `changed` lists from the host copy.
 void live_push_words(sieve_live* l, hipStream_t st, const std::vector<uint64_t>& changed);
 // Live::log (src/live.rs:251-258): appends the records to the write-ahead log, then waits for

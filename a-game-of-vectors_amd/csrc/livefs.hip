@@ -699,6 +699,7 @@ int sieve_live_compact(const sieve_live* l, const char* out_dir, sieve_compact_s
     return guarded([&] {
         SIEVE_REQUIRE(l && out_dir && *out_dir, SIEVE_EINVAL, "null argument");
         SIEVE_HIP(hipSetDevice(l->ix->device));
+        live_settle(l);
         compact(l, out_dir, stats);
     });
 }

@@ -17,8 +17,10 @@
 // request is batched only with queued requests carrying the same filter and plan and runs
 // filter::search on the device; with a live state attached, searches read its delta lists and
 // tombstones, and inserts / deletes queue with the searches. A write is a barrier in
-// submission order: the batcher lets every launched batch finish, applies the write, and only
-// then launches later searches, so each search sees exactly the writes submitted before it.
+// submission order, kept on the device rather than by draining it: the batcher encodes the
+// write while earlier batches run, publishes its tombstones and delta lists on the live state's
+// stream after those batches' events, and makes every later batch's stream wait for the
+// publish, so each search sees exactly the writes submitted before it.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -87,6 +89,7 @@ struct sieve_server {
     const sieve_index* ix = nullptr;
     sieve_live* live = nullptr;
     LiveArgs la;
+    hipEvent_t pub_ev = nullptr;  // the live state's publish event (owned by it), once writes ran
     sieve_params p{};
     sieve_server_opts o{};
     std::vector<std::unique_ptr<BatchSlot>> slots;
@@ -107,12 +110,18 @@ namespace {
 
 // Applies one write request (the batcher holds no batch in flight), then refreshes the delta
 // segment later batches read (its block count grows with inserts).
+// Stages one write (encoded into the slot log, logged, host state updated); its tombstones and
+// arena wait for the publish that follows the run of writes.
 void apply_write(sieve_server* sv, Request& r) {
-    int rc = r.kind == K_INSERT ? sieve_live_insert(sv->live, r.ext.data(), r.vecs.data(), r.ext.size())
-                                : sieve_live_delete(sv->live, r.ext.data(), r.ext.size());
+    const int rc = guarded([&] {
+        SIEVE_HIP(hipSetDevice(sv->device));
+        if (r.kind == K_INSERT)
+            live_insert(sv->live, r.ext.data(), r.vecs.data(), r.ext.size(), false);
+        else
+            live_delete(sv->live, r.ext.data(), r.ext.size(), false);
+    });
     r.status = rc;
     if (rc != 0) r.error = sieve_last_error();
-    sv->la = live_args(sv->live);
 }
 
 void batcher_loop(sieve_server* sv) {
@@ -134,7 +143,7 @@ void batcher_loop(sieve_server* sv) {
             auto ready = [&] {
                 if (sv->pending.empty()) return false;
                 const Request& h = sv->reqs[sv->pending.front()];
-                if (h.kind != K_SEARCH) return sv->inflight.empty();  // a write waits for the device to drain
+                if (h.kind != K_SEARCH) return true;  // a write waits for earlier batches on the device
                 return free_slot() != nullptr;
             };
             sv->cv_batcher.wait(lk, [&] { return sv->stop || ready(); });
@@ -142,7 +151,11 @@ void batcher_loop(sieve_server* sv) {
             const uint64_t head = sv->pending.front();
             Request& hr = sv->reqs[head];
             if (hr.kind != K_SEARCH) {
-                // The writes queued back to back at the head share one drain, applied in order.
+                // The writes queued back to back at the head are staged in order beside the batches
+                // in flight (they write only rows, slot-log entries and host state those batches
+                // do not read), then published together on the live state's stream once the
+                // batches in flight have finished there: a device-side wait, not a drain. Later
+                // batches wait for the publish on the device.
                 std::vector<Request*> writes;
                 while (!sv->pending.empty() && sv->reqs[sv->pending.front()].kind != K_SEARCH) {
                     Request& w = sv->reqs[sv->pending.front()];
@@ -150,12 +163,26 @@ void batcher_loop(sieve_server* sv) {
                     w.t_launch = Clock::now();
                     writes.push_back(&w);  // node-based map: stable until waited out
                 }
+                // Only this thread records the slots' events, so these stay the batches' own.
+                std::vector<hipEvent_t> waits;
+                for (const BatchSlot* x : sv->inflight) waits.push_back(x->done);
                 sv->launching++;
                 lk.unlock();
                 for (Request* w : writes) apply_write(sv, *w);
+                std::string perr;
+                const int prc = guarded([&] {
+                    SIEVE_HIP(hipSetDevice(sv->device));
+                    sv->pub_ev = live_publish_async(sv->live, waits.data(), waits.size());
+                });
+                if (prc != 0) perr = sieve_last_error();
+                sv->la = live_args(sv->live);
                 lk.lock();
                 sv->launching--;
                 for (Request* w : writes) {
+                    if (prc != 0 && w->status == 0) {
+                        w->status = prc;
+                        w->error = perr;
+                    }
                     w->done = true;
                     w->t_done = Clock::now();
                 }
@@ -209,6 +236,7 @@ void batcher_loop(sieve_server* sv) {
         try {
             SIEVE_HIP(hipMemcpyAsync(slot->dq768, slot->q768.p, nq * DIM * 4, hipMemcpyHostToDevice, s->st));
             SIEVE_HIP(hipMemcpyAsync(slot->dq256, slot->q256.p, nq * PDIM * 4, hipMemcpyHostToDevice, s->st));
+            if (sv->pub_ev) SIEVE_HIP(hipStreamWaitEvent(s->st, sv->pub_ev, 0));  // the last writes' publish
             const LiveArgs* lv = sv->live ? &sv->la : nullptr;
             if (filter)
                 enqueue_filter_search(sv->ix, filter, plan, 0, s, slot->dq256, slot->dq768, nq, &sv->p,
@@ -340,6 +368,7 @@ int sieve_server_create(const sieve_index* ix, const sieve_params* p, const siev
         sv->live = sv->o.live;
         if (sv->live) {
             SIEVE_REQUIRE(ix->live == sv->live, SIEVE_EINVAL, "opts.live is not this index's live state");
+            live_settle(sv->live);  // a publish of an earlier server's writes
             sv->la = live_args(sv->live);
         }
         if (sv->o.max_batch == 0) sv->o.max_batch = 1024;
@@ -522,6 +551,7 @@ void sieve_server_destroy(sieve_server* sv) {
     if (sv->batcher.joinable()) sv->batcher.join();
     if (sv->completer.joinable()) sv->completer.join();
     for (auto& sl : sv->slots) (void)hipStreamSynchronize(sl->s->st);
+    if (sv->live) (void)guarded([&] { live_settle(sv->live); });
     delete sv;
 }
 

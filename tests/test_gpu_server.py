@@ -138,6 +138,63 @@ def test_filters_and_live_writes_in_the_serving_loop(small, oracle_index):
     live.close()
 
 
+def test_writes_overlap_the_batches_in_flight(small):
+    """Inserts, deletes and upserts submitted back to back with searches, nothing waited for in
+    between: each write is encoded while earlier batches run and published after them on the
+    device, and later batches wait for the publish there. Every search returns, bit for bit,
+    what the same operations done one at a time through the synchronous live API return."""
+    import numpy as np
+    import torch
+    from sieve_amd.build import normalize_rows
+    from sieve_amd.index import Live
+
+    p = Params(nprobe=12, cand=400, k=10, rerank=Rerank.FULL)
+    new, _ = normalize_rows(small.mix.rows(small.n + 900, 400))
+    new = new.numpy().astype(np.float32)
+    q768 = np.concatenate([small.q768[:16], new[:16]])
+    q256 = np.concatenate([small.q256[:16], normalize_rows(torch.from_numpy(new[:16, :256].copy()))[0].numpy()])
+    ops = []
+    for r in range(8):
+        ops.append(("insert", (70_000_000 + 40 * r + np.arange(40)).astype(np.uint32), new[40 * r:40 * r + 40]))
+        if r % 2 == 1:
+            ops.append(("delete", (70_000_000 + 40 * (r - 1) + np.arange(12)).astype(np.uint32), None))
+        if r == 5:  # upserts of live rows
+            ops.append(("insert", (70_000_000 + 40 * r + np.arange(6)).astype(np.uint32), new[:6]))
+
+    ix_a = Index.create(small.host)
+    live_a = Live(ix_a, max_rows=1000)
+    sv = Server(ix_a, p, max_batch=8, max_inflight=3, live=live_a)
+    phases, writes = [[sv.submit(q768[i], q256[i]) for i in range(len(q768))]], []
+    for kind, ext, v in ops:
+        writes.append(sv.insert(ext, v) if kind == "insert" else sv.delete(ext))
+        phases.append([sv.submit(q768[i], q256[i]) for i in range(len(q768))])
+    got = [[sv.wait(t)[:2] for t in ts] for ts in phases]
+    for w in writes:
+        sv.wait(w)
+    assert sv.stats()["writes"] == len(ops)
+    sv.close()
+    info_a = live_a.info()
+
+    ix_b = Index.create(small.host)
+    live_b = Live(ix_b, max_rows=1000)
+    want = [live_b.search(q768, q256, p)]
+    for kind, ext, v in ops:
+        if kind == "insert":
+            live_b.insert(ext, v)
+        else:
+            live_b.delete(ext)
+        want.append(live_b.search(q768, q256, p))
+    info_b = live_b.info()
+    for key in ("rows", "slots", "arena_blocks", "deleted"):  # exact block count once settled
+        assert info_a[key] == info_b[key], key
+    for phase, (g, (ws, wr, wn)) in enumerate(zip(got, want)):
+        for i, (gs, gr) in enumerate(g):
+            assert np.array_equal(gr, wr[i, :wn[i]]), (phase, i)
+            assert np.array_equal(gs.view(np.uint32), ws[i, :wn[i]].view(np.uint32)), (phase, i)
+    for x in (live_a, live_b):
+        x.close()
+
+
 def test_expand_over_listed_slots_with_live_writes(small):
     """Expand-filtered searches through the server with a live state: the base clusters' listed
     allowed slots plus the delta lists (tombstones applied) give the same results bit for bit as
