@@ -360,8 +360,12 @@ int sieve_server_submit_filtered(sieve_server* s, const float* q768, const float
                                  const struct sieve_filter* filter, uint32_t plan, uint64_t* ticket);
 /* Live writes through the server (needs opts.live): Live::insert / Live::delete
  * (src/live.rs:275-315) of n rows (ext ids, n x 768 unit vectors in host memory). A write is a
- * barrier in submission order: it is applied once every earlier search has finished, and every
- * later search sees it. sieve_server_wait on its ticket returns when it is applied (count 0). */
+ * barrier in submission order, kept on the device without draining it: the rows are encoded and
+ * logged while earlier searches run, the tombstones and delta lists are published on the live
+ * state's stream after those searches, and every later search waits for the publish on the
+ * device. sieve_server_wait on its ticket returns once it is logged and its publish enqueued
+ * (count 0); the sieve_live_* entry points wait for a pending publish before they read the
+ * live state. */
 int sieve_server_insert(sieve_server* s, const uint32_t* ext, const float* vecs, size_t n, uint64_t* ticket);
 int sieve_server_delete(sieve_server* s, const uint32_t* ext, size_t n, uint64_t* ticket);
 /* Blocks until the ticket's results are ready; scores/rows hold p->k entries (count used). */
