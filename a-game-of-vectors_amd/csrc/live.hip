@@ -71,56 +71,75 @@ __global__ void k_live_prefix(const int8_t* __restrict__ rows, size_t n, const f
     }
 }
 
-// quant::dot_f32 (src/quant.rs:86-100): 16 lane accumulators, then their sum.
-__device__ float dot16(const float* a, const float* b) {
-    float acc[16];
-#pragma unroll
-    for (int l = 0; l < 16; l++) acc[l] = 0.0f;
-    for (int i = 0; i < PDIM; i += 16)
-#pragma unroll
-        for (int l = 0; l < 16; l++) acc[l] = acc[l] + a[i + l] * b[i + l];
-    float s = -0.0f;
-#pragma unroll
-    for (int l = 0; l < 16; l++) s = s + acc[l];
-    return s + -0.0f;
-}
-
 // The (primary, secondary) clusters of each row: assign_top's best centroid and
 // soar_secondary over its SOAR_CAND best (src/ivf.rs:164-186); PAD without SOAR. With fewer
 // than SOAR_CAND clusters, assign_top leaves the candidate list's tail at cluster 0
 // (src/ivf.rs:141, :154), and so does this.
-__global__ void k_live_pairs(const float* __restrict__ xs, size_t n, const float* __restrict__ cents,
-                             const uint32_t* __restrict__ pc, const uint32_t* __restrict__ pn, int pstride, int soar,
-                             float lambda, uint32_t* __restrict__ pairs) {
-    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
+// One wave per row. The row needs 31 dot products: x . c1, then x . c_i and c1 . c_i for the
+// other SOAR_CAND - 1 candidates, each summed as quant::dot_f32 (src/quant.rs:86-100) sums: 16
+// lane accumulators, accumulator l adding products l, l + 16, ... in order, then the 16 added
+// in order from -0.0. Lane group g (16 lanes) computes dots g, g + 4, ...: lane l holds
+// accumulator l, and the group's first lane adds the 16, so each dot is bit for bit the
+// sequential one, with the centroid rows read 64 bytes per group and step. The wave's first
+// lane then takes the loss minimum in candidate order.
+__global__ __launch_bounds__(64) void k_live_pairs(const float* __restrict__ xs, size_t n,
+                                                   const float* __restrict__ cents, const uint32_t* __restrict__ pc,
+                                                   const uint32_t* __restrict__ pn, int pstride, int soar,
+                                                   float lambda, uint32_t* __restrict__ pairs) {
+    constexpr int ND = 2 * SOAR_CAND - 1;
+    __shared__ float part[4][16];
+    __shared__ float dots[ND];
+    const size_t j = blockIdx.x;  // grid: one workgroup per row
+    if (j >= n) return;           // workgroup-uniform
+    const int lane = threadIdx.x, g = lane >> 4, l = lane & 15;
     const uint32_t* cand = pc + j * pstride;
     const int m = min((int)pn[j], pstride);
     auto cand_at = [&](int i) -> uint32_t { return i < m ? cand[i] : 0u; };
-    pairs[2 * j] = cand_at(0);
     if (!soar) {
-        pairs[2 * j + 1] = PAD;
+        if (lane == 0) {
+            pairs[2 * j] = cand_at(0);
+            pairs[2 * j + 1] = PAD;
+        }
         return;
     }
     const float* x = xs + j * PDIM;
     const float* c1 = cents + (size_t)cand_at(0) * PDIM;
-    const float x_c1 = dot16(x, c1);
+    for (int k0 = 0; k0 < ND; k0 += 4) {
+        const int k = k0 + g;
+        float acc = 0.0f;
+        if (k < ND) {
+            // k = 0: x . c1; k < SOAR_CAND: x . c_k; else c1 . c_(k - SOAR_CAND + 1)
+            const float* u = k < SOAR_CAND ? x : c1;
+            const float* v = k == 0 ? c1 : cents + (size_t)cand_at(k < SOAR_CAND ? k : k - SOAR_CAND + 1) * PDIM;
+            for (int i = 0; i < PDIM; i += 16) acc = acc + u[i + l] * v[i + l];
+        }
+        part[g][l] = acc;
+        __syncthreads();
+        if (l == 0 && k < ND) {
+            float sum = -0.0f;
+#pragma unroll
+            for (int q = 0; q < 16; q++) sum = sum + part[g][q];
+            dots[k] = sum + -0.0f;
+        }
+        __syncthreads();
+    }
+    if (lane != 0) return;
+    const float x_c1 = dots[0];
     const float r2 = fmaxf(2.0f - 2.0f * x_c1, 1e-12f);
     const float r_x = 1.0f - x_c1;
     float best = FLT_MAX;
     uint32_t bc = cand_at(1);
     for (int i = 1; i < SOAR_CAND; i++) {
-        const uint32_t ci = cand_at(i);
-        const float* cv = cents + (size_t)ci * PDIM;
-        const float x_c = dot16(x, cv);
-        const float r_c = x_c - dot16(c1, cv);
+        const float x_c = dots[i];
+        const float r_c = x_c - dots[SOAR_CAND - 1 + i];
         const float along = r_x - r_c;
         const float loss = (2.0f - 2.0f * x_c) + lambda * along * along / r2;
         if (loss < best) {
             best = loss;
-            bc = ci;
+            bc = cand_at(i);
         }
     }
+    pairs[2 * j] = cand_at(0);
     pairs[2 * j + 1] = bc;
 }
 
@@ -380,7 +399,7 @@ void apply_pass(sieve_live* l, const uint32_t* eh, size_t n, size_t r0, std::vec
     uint32_t* pn = grow<uint32_t>(s->b[B_PN], n);
     route_batch(ix, s, bt, n, &rp, ps, pc, pn, m);
     uint32_t* pairs = grow<uint32_t>(l->pairs, 2 * n);
-    hipLaunchKernelGGL(k_live_pairs, dim3(grid(n, 64)), dim3(64), 0, st, xs, n, ix->cents, pc, pn, m, l->soar ? 1 : 0,
+    hipLaunchKernelGGL(k_live_pairs, dim3((unsigned)n), dim3(64), 0, st, xs, n, ix->cents, pc, pn, m, l->soar ? 1 : 0,
                        l->lambda, pairs);
     const int spr = l->soar ? 2 : 1;
     const size_t L0 = l->nlog;
@@ -461,9 +480,13 @@ void live_delete(sieve_live* l, const uint32_t* ext, size_t n, bool publish) {
 
 void launch_soar_pairs(const float* xs, size_t n, const float* cents, const uint32_t* pc, const uint32_t* pn,
                        int pstride, int soar, float lambda, uint32_t* pairs, hipStream_t st) {
-    if (n == 0) return;
-    hipLaunchKernelGGL(k_live_pairs, dim3(grid(n, 64)), dim3(64), 0, st, xs, n, cents, pc, pn, pstride, soar, lambda,
-                       pairs);
+    // 64 work-items per row, and a launch counts its work-items in 32 bits
+    constexpr size_t ROWS = size_t(1) << 24;
+    for (size_t r0 = 0; r0 < n; r0 += ROWS) {
+        const size_t c = std::min(ROWS, n - r0);
+        hipLaunchKernelGGL(k_live_pairs, dim3((unsigned)c), dim3(64), 0, st, xs + r0 * PDIM, c, cents,
+                           pc + r0 * pstride, pn + r0, pstride, soar, lambda, pairs + 2 * r0);
+    }
 }
 
 LiveArgs live_args(const sieve_live* l) { return l->args(); }
