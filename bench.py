@@ -106,8 +106,8 @@ def pmc_traffic(kernel, workload, search):
 
 def host_cpus():
     """The host's CPU facts and the worker count the CPU baseline uses: the CPUs this process may
-    run on (affinity, cgroup quota, OMP_NUM_THREADS, whichever is least: on the GPU box that is
-    the one-GPU share of its cores), with nproc and the lscpu model recorded beside it."""
+    run on (affinity, cgroup quota, OMP_NUM_THREADS, whichever is least: on a shared GPU host that
+    is one GPU's share of its cores), with nproc and the lscpu model recorded beside it."""
     nproc = os.cpu_count() or 1
     try:
         affinity = len(os.sched_getaffinity(0))

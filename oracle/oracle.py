@@ -1,7 +1,7 @@
 """CPU oracle for the sieve search path — TEST INFRASTRUCTURE, NOT PRODUCT CODE.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module, and
-only as the checker (or, in bench.py, as the timed CPU baseline). It wraps
+only to check results (or, in bench.py, as the timed CPU baseline). It wraps
 oracle/libsieve_oracle.so, a C++ restatement of the reference crate's x86 (non-NEON) path
 (sieve_oracle.cpp; every function there cites the Rust it follows).
 

@@ -1,6 +1,6 @@
 // sieve_oracle.cpp — TEST INFRASTRUCTURE, NOT PRODUCT CODE.
 //
-// A CPU restatement of the reference's x86 (non-NEON) search path, used only as the checker:
+// A CPU restatement of the reference's x86 (non-NEON) search path, used only to check results:
 // by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg. Nothing in the
 // product (a-game-of-vectors_amd/) links, loads or calls it.
 //

@@ -10,7 +10,7 @@ sys.path.insert(0, REPO)
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X")
 
 
 class SmallWorkload:
